@@ -1,0 +1,737 @@
+/*
+ * flock_oracle.c — CPU restatement of the Flock `prove_fast` data-parallel core.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker + timed CPU baseline). The shipped
+ * library never links or loads this file. Each function cites the reference
+ * file:line it restates; paths are relative to the reference checkout.
+ *
+ * Pinning (what fixes each function independently of this text):
+ *   - F128 mul/inv: tests/golden vectors from Python integer clmul, GHASH
+ *     known answers (gf2_128.rs:503-531), field axioms.
+ *   - NTT: brute-force evaluation of the novel-basis polynomial at the subspace
+ *     points (tests), forward/inverse round trip, and the reference's own
+ *     host_check_ntt.cpp built from source (oracle/_ref).
+ *   - SHA-256 / Merkle: hashlib (FIPS 180-4) over the flat tree layout.
+ *   - FsChallenger: a hashlib restatement of the framing and the reference's
+ *     own test_challenger.cpp built from source (oracle/_ref).
+ *   - Sumcheck rounds: brute-force sums and the sumcheck identities
+ *     U(0)+U(1) = claim, Σ fold·fold = U(r) (tests).
+ */
+#include "flock_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#if defined(__x86_64__)
+#include <cpuid.h>
+#include <wmmintrin.h>
+#include <emmintrin.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* F128 arithmetic                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* clmul64, crates/flock-core/src/field/gf2_128/portable.rs:4-19 */
+static void clmul64(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi) {
+    uint64_t l = 0, h = 0;
+    for (int i = 0; i < 64; i++) {
+        if ((a >> i) & 1) {
+            l ^= b << i;
+            if (i) h ^= b >> (64 - i);
+        }
+    }
+    *lo = l;
+    *hi = h;
+}
+
+/* ghash_reduce, crates/flock-core/src/field/gf2_128.rs:278-297 */
+static or_f128 ghash_reduce(uint64_t r0, uint64_t r1, uint64_t r2, uint64_t r3) {
+    uint64_t s1_lo = r2 << 1, s1_hi = (r3 << 1) | (r2 >> 63);
+    uint64_t s2_lo = r2 << 2, s2_hi = (r3 << 2) | (r2 >> 62);
+    uint64_t s7_lo = r2 << 7, s7_hi = (r3 << 7) | (r2 >> 57);
+    uint64_t t_lo = r2 ^ s1_lo ^ s2_lo ^ s7_lo;
+    uint64_t t_hi = r3 ^ s1_hi ^ s2_hi ^ s7_hi;
+    uint64_t ov = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
+    uint64_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+    or_f128 r = {r0 ^ t_lo ^ corr, r1 ^ t_hi};
+    return r;
+}
+
+/* ghash_mul_unreduced + ghash_mul, gf2_128/portable.rs:21-43 */
+or_f128 or_f128_mul_portable(or_f128 a, or_f128 b) {
+    uint64_t ll_lo, ll_hi, lh_lo, lh_hi, hl_lo, hl_hi, hh_lo, hh_hi;
+    clmul64(a.lo, b.lo, &ll_lo, &ll_hi);
+    clmul64(a.lo, b.hi, &lh_lo, &lh_hi);
+    clmul64(a.hi, b.lo, &hl_lo, &hl_hi);
+    clmul64(a.hi, b.hi, &hh_lo, &hh_hi);
+    return ghash_reduce(ll_lo, ll_hi ^ lh_lo ^ hl_lo, hh_lo ^ lh_hi ^ hl_hi, hh_hi);
+}
+
+#if defined(__x86_64__)
+/* Same schoolbook product as the portable path, with the 64x64 carry-less
+ * products from PCLMULQDQ (the reference's x86 Mul also uses PCLMULQDQ,
+ * gf2_128/x86_64.rs:130). Reduction is the shared ghash_reduce. */
+__attribute__((target("pclmul,sse2"))) static or_f128 mul_pclmul(or_f128 a, or_f128 b) {
+    __m128i A = _mm_set_epi64x((long long)a.hi, (long long)a.lo);
+    __m128i B = _mm_set_epi64x((long long)b.hi, (long long)b.lo);
+    __m128i ll = _mm_clmulepi64_si128(A, B, 0x00);
+    __m128i hh = _mm_clmulepi64_si128(A, B, 0x11);
+    __m128i lh = _mm_clmulepi64_si128(A, B, 0x10);
+    __m128i hl = _mm_clmulepi64_si128(A, B, 0x01);
+    __m128i cr = _mm_xor_si128(lh, hl);
+    uint64_t l[2], h[2], c[2];
+    _mm_storeu_si128((__m128i*)l, ll);
+    _mm_storeu_si128((__m128i*)h, hh);
+    _mm_storeu_si128((__m128i*)c, cr);
+    return ghash_reduce(l[0], l[1] ^ c[0], h[0] ^ c[1], h[1]);
+}
+#endif
+
+static int g_pclmul = -1;
+
+int or_has_pclmul(void) {
+    if (g_pclmul < 0) {
+#if defined(__x86_64__)
+        unsigned a, b, c, d;
+        g_pclmul = (__get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL)) ? 1 : 0;
+#else
+        g_pclmul = 0;
+#endif
+    }
+    return g_pclmul;
+}
+
+or_f128 or_f128_mul(or_f128 a, or_f128 b) {
+#if defined(__x86_64__)
+    if (or_has_pclmul()) return mul_pclmul(a, b);
+#endif
+    return or_f128_mul_portable(a, b);
+}
+
+void or_f128_mul_batch(const or_f128* a, const or_f128* b, or_f128* out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = or_f128_mul(a[i], b[i]);
+}
+
+static inline or_f128 add(or_f128 a, or_f128 b) {
+    or_f128 r = {a.lo ^ b.lo, a.hi ^ b.hi};
+    return r;
+}
+static const or_f128 ZERO = {0, 0};
+static const or_f128 ONE = {1, 0};
+
+/* inv (Fermat x^(2^128-2)), gf2_128.rs:67-80 */
+or_f128 or_f128_inv(or_f128 x) {
+    or_f128 r = ONE;
+    or_f128 cur = or_f128_mul(x, x);
+    for (int i = 1; i < 128; i++) {
+        r = or_f128_mul(r, cur);
+        cur = or_f128_mul(cur, cur);
+    }
+    return r;
+}
+
+/* mul_by_x, gf2_128.rs:213-220 */
+or_f128 or_f128_mul_by_x(or_f128 z) {
+    uint64_t carry = z.hi >> 63;
+    uint64_t mask = 0 - carry;
+    or_f128 r = {(z.lo << 1) ^ (0x87 & mask), (z.hi << 1) | (z.lo >> 63)};
+    return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* SplitMix64 (dump_commit_vectors.rs:42-52); F128 = {lo: next, hi: next}, the */
+/* same words pack_witness (pcs/pack.rs:40-80) assembles from rng.bits().      */
+/* ------------------------------------------------------------------------- */
+void or_splitmix_f128(uint64_t seed, or_f128* out, size_t n) {
+    uint64_t s = seed;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t w[2];
+        for (int k = 0; k < 2; k++) {
+            s += 0x9E3779B97F4A7C15ull;
+            uint64_t z = s;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            w[k] = z ^ (z >> 31);
+        }
+        out[i].lo = w[0];
+        out[i].hi = w[1];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Additive NTT                                                               */
+/* ------------------------------------------------------------------------- */
+
+/* generate_evals_from_subspace with the standard basis {1, x, .., x^(L-1)},
+ * additive_ntt_f128.rs:67-95 and :127-133. evals[i][k] = W_i(beta_{i+k}) /
+ * W_i(beta_i). */
+int or_ntt_evals(int L, or_f128* ev) {
+    if (L < 1 || L > 64) return -1;
+    memset(ev, 0, sizeof(or_f128) * (size_t)L * (size_t)L);
+    for (int k = 0; k < L; k++) {
+        ev[k].lo = 1ull << k;
+        ev[k].hi = 0;
+    }
+    for (int i = 1; i < L; i++) {
+        const or_f128* prev = ev + (size_t)(i - 1) * L;
+        or_f128* row = ev + (size_t)i * L;
+        int prev_len = L - (i - 1);
+        for (int k = 1; k < prev_len; k++) row[k - 1] = or_f128_mul(prev[k], add(prev[k], prev[0]));
+    }
+    /* normalization uses the pre-normalization row 0 of each row; rows were
+     * computed from unnormalized predecessors above, as in the reference */
+    for (int i = 0; i < L; i++) {
+        or_f128* row = ev + (size_t)i * L;
+        or_f128 inv = or_f128_inv(row[0]);
+        for (int k = 0; k < L - i; k++) row[k] = or_f128_mul(row[k], inv);
+    }
+    return 0;
+}
+
+/* twiddle(l, b) = span of evals[L-l-1][1..] at the set bits of b, :141-144 + span_get :99-108 */
+or_f128 or_ntt_twiddle(const or_f128* ev, int L, int layer, uint64_t block) {
+    const or_f128* v = ev + (size_t)(L - layer - 1) * L + 1;
+    int len = L - (L - layer - 1) - 1; /* = layer */
+    or_f128 acc = ZERO;
+    for (int j = 0; j < len; j++)
+        if ((block >> j) & 1) acc = add(acc, v[j]);
+    return acc;
+}
+
+/* interleaved_scalar_live_from_layer, additive_ntt_f128.rs:287-319 */
+int or_ntt_forward_interleaved(or_f128* data, size_t num_ntts, size_t live, int log_d, int L,
+                               int start_layer) {
+    if (log_d > L || start_layer > log_d || live > num_ntts) return -1;
+    or_f128* ev = (or_f128*)malloc(sizeof(or_f128) * (size_t)L * L);
+    if (!ev) return -2;
+    or_ntt_evals(L, ev);
+    for (int layer = start_layer; layer < log_d; layer++) {
+        size_t num_blocks = (size_t)1 << layer;
+        size_t block_size = (size_t)1 << (log_d - layer);
+        size_t half = block_size >> 1;
+        for (size_t block = 0; block < num_blocks; block++) {
+            or_f128 tw = or_ntt_twiddle(ev, L, layer, block);
+            size_t block_start = block * block_size * num_ntts;
+            for (size_t row = 0; row < half; row++) {
+                size_t off_top = block_start + row * num_ntts;
+                size_t off_bot = off_top + half * num_ntts;
+                for (size_t lane = 0; lane < live; lane++) {
+                    or_f128 v = data[off_bot + lane];
+                    or_f128 u = add(data[off_top + lane], or_f128_mul(v, tw));
+                    data[off_top + lane] = u;
+                    data[off_bot + lane] = add(v, u);
+                }
+            }
+        }
+    }
+    free(ev);
+    return 0;
+}
+
+/* inverse_transform_scalar, additive_ntt_f128.rs:651-673 */
+int or_ntt_inverse_scalar(or_f128* data, int log_d, int L) {
+    if (log_d > L) return -1;
+    or_f128* ev = (or_f128*)malloc(sizeof(or_f128) * (size_t)L * L);
+    if (!ev) return -2;
+    or_ntt_evals(L, ev);
+    for (int layer = log_d - 1; layer >= 0; layer--) {
+        size_t num_blocks = (size_t)1 << layer;
+        size_t half = (size_t)1 << (log_d - layer - 1);
+        for (size_t block = 0; block < num_blocks; block++) {
+            or_f128 tw = or_ntt_twiddle(ev, L, layer, block);
+            size_t block_start = block << (log_d - layer);
+            for (size_t i0 = block_start; i0 < block_start + half; i0++) {
+                size_t i1 = i0 | half;
+                or_f128 u = data[i0];
+                or_f128 v = add(data[i1], u);
+                data[i0] = add(u, or_f128_mul(v, tw));
+                data[i1] = v;
+            }
+        }
+    }
+    free(ev);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* SHA-256 (FIPS 180-4)                                                       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t h[8];
+    uint64_t total;
+    uint8_t buf[64];
+    size_t blen;
+} sha_ctx;
+
+static const uint32_t SHA_K[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+static inline uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void sha_block(uint32_t h[8], const uint8_t* p) {
+    uint32_t w[64];
+    for (int i = 0; i < 16; i++)
+        w[i] = ((uint32_t)p[4 * i] << 24) | ((uint32_t)p[4 * i + 1] << 16) |
+               ((uint32_t)p[4 * i + 2] << 8) | (uint32_t)p[4 * i + 3];
+    for (int i = 16; i < 64; i++) {
+        uint32_t s0 = rotr(w[i - 15], 7) ^ rotr(w[i - 15], 18) ^ (w[i - 15] >> 3);
+        uint32_t s1 = rotr(w[i - 2], 17) ^ rotr(w[i - 2], 19) ^ (w[i - 2] >> 10);
+        w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+    }
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+    for (int i = 0; i < 64; i++) {
+        uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
+        uint32_t ch = (e & f) ^ (~e & g);
+        uint32_t t1 = hh + S1 + ch + SHA_K[i] + w[i];
+        uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
+        uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+        uint32_t t2 = S0 + mj;
+        hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
+    }
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
+static void sha_init(sha_ctx* c) {
+    static const uint32_t IV[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                                   0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    memcpy(c->h, IV, sizeof IV);
+    c->total = 0;
+    c->blen = 0;
+}
+
+static void sha_update(sha_ctx* c, const uint8_t* d, size_t n) {
+    c->total += n;
+    while (n) {
+        size_t take = 64 - c->blen;
+        if (take > n) take = n;
+        memcpy(c->buf + c->blen, d, take);
+        c->blen += take;
+        d += take;
+        n -= take;
+        if (c->blen == 64) {
+            sha_block(c->h, c->buf);
+            c->blen = 0;
+        }
+    }
+}
+
+static void sha_final(const sha_ctx* in, uint8_t out[32]) {
+    sha_ctx c = *in;
+    uint64_t bits = c.total * 8;
+    uint8_t pad = 0x80;
+    sha_update(&c, &pad, 1);
+    uint8_t z = 0;
+    while (c.blen != 56) sha_update(&c, &z, 1);
+    uint8_t len[8];
+    for (int i = 0; i < 8; i++) len[i] = (uint8_t)(bits >> (56 - 8 * i));
+    sha_update(&c, len, 8);
+    for (int i = 0; i < 8; i++) {
+        out[4 * i] = (uint8_t)(c.h[i] >> 24);
+        out[4 * i + 1] = (uint8_t)(c.h[i] >> 16);
+        out[4 * i + 2] = (uint8_t)(c.h[i] >> 8);
+        out[4 * i + 3] = (uint8_t)c.h[i];
+    }
+}
+
+void or_sha256(const uint8_t* data, size_t len, uint8_t out[32]) {
+    sha_ctx c;
+    sha_init(&c);
+    sha_update(&c, data, len);
+    sha_final(&c, out);
+}
+
+/* merkle_tree_sequential, merkle.rs:593-617; hash_leaf = SHA-256(leaf) :205-216,
+ * hash_pair = SHA-256(L || R) :220-232 */
+int or_merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, uint8_t* tree) {
+    if (num_leaves == 0 || (num_leaves & (num_leaves - 1)) || data_len % num_leaves) return -1;
+    size_t leaf = data_len / num_leaves;
+    for (size_t i = 0; i < num_leaves; i++) or_sha256(data + i * leaf, leaf, tree + 32 * i);
+    size_t rs = 0, rl = num_leaves;
+    while (rl > 1) {
+        size_t nl = rl >> 1;
+        for (size_t i = 0; i < nl; i++)
+            or_sha256(tree + 32 * (rs + 2 * i), 64, tree + 32 * (rs + rl + i));
+        rs += rl;
+        rl = nl;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* PCS commit                                                                 */
+/* ------------------------------------------------------------------------- */
+
+/* PcsParams size helpers and validate(), pcs/commit.rs:187-298 */
+int or_pcs_sizes(const or_pcs_params* p, size_t* msg_len, size_t* cw_len, size_t* n_leaves,
+                 int* k_code, size_t* num_ntts) {
+    if (p->m < 7 + p->log_batch_size || p->log_inv_rate < 1) return -1;
+    int log_msg = (int)p->m - 7;
+    int log_dim = log_msg - (int)p->log_batch_size;
+    int k = log_dim + (int)p->log_inv_rate;
+    if (k > 64) return -1;
+    size_t t = p->num_lanes ? p->num_lanes : ((size_t)1 << p->log_batch_size);
+    if (t < 1 || t > ((size_t)1 << p->log_batch_size)) return -1;
+    if (msg_len) *msg_len = t << log_dim;
+    if (cw_len) *cw_len = ((size_t)1 << k) * t;
+    if (n_leaves) *n_leaves = (size_t)1 << k;
+    if (k_code) *k_code = k;
+    if (num_ntts) *num_ntts = t;
+    return 0;
+}
+
+static int finalize(or_f128* cw, size_t t, size_t live, const or_pcs_params* p, uint8_t* tree) {
+    size_t cw_len, n_leaves;
+    int k;
+    or_pcs_sizes(p, NULL, &cw_len, &n_leaves, &k, NULL);
+    int rc = or_ntt_forward_interleaved(cw, t, live, k, k, (int)p->log_inv_rate);
+    if (rc) return rc;
+    if (tree) rc = or_merkle_tree((const uint8_t*)cw, cw_len * 16, n_leaves, tree);
+    return rc;
+}
+
+int or_commit_lane_major(const or_f128* q, size_t q_len, const or_pcs_params* p, or_f128* cw,
+                         uint8_t* tree) {
+    size_t cw_len, t;
+    if (or_pcs_sizes(p, NULL, &cw_len, NULL, NULL, &t)) return -1;
+    if (q_len != ((size_t)1 << (p->m - 7))) return -1;
+    size_t d = q_len >> p->log_batch_size;
+    /* dead-lane skip, commit.rs:494-505 (value-identical either way) */
+    size_t live = t;
+    while (live > 0) {
+        int zero = 1;
+        for (size_t i = (live - 1) * d; i < live * d && zero; i++)
+            if (q[i].lo | q[i].hi) zero = 0;
+        if (!zero) break;
+        live--;
+    }
+    size_t msg_len = t * d;
+    for (size_t rep = 0; rep < cw_len / msg_len; rep++)
+        for (size_t pp = 0; pp < d; pp++)
+            for (size_t l = 0; l < t; l++) cw[rep * msg_len + pp * t + l] = q[l * d + pp];
+    if (live == 0) {
+        return tree ? or_merkle_tree((const uint8_t*)cw, cw_len * 16, cw_len / t, tree) : 0;
+    }
+    return finalize(cw, t, live, p, tree);
+}
+
+int or_commit(const or_f128* msg, size_t msg_len, const or_pcs_params* p, or_f128* cw,
+              uint8_t* tree) {
+    size_t ml, cw_len, t;
+    if (or_pcs_sizes(p, &ml, &cw_len, NULL, NULL, &t)) return -1;
+    if (msg_len != ml) return -1;
+    for (size_t i = 0; i < cw_len; i++) cw[i] = msg[i % msg_len];
+    return finalize(cw, t, t, p, tree);
+}
+
+/* ------------------------------------------------------------------------- */
+/* FsChallenger (SHA-256 discipline), challenger.rs:609-1044                  */
+/* ------------------------------------------------------------------------- */
+struct or_fs {
+    sha_ctx st;
+};
+
+enum { OP_DOMAIN = 1, OP_LABEL = 2, OP_OBSERVE = 3, OP_SQUEEZE = 4, OP_BYTES = 5 };
+enum { KIND_NONE = 0, KIND_SCALAR = 1, KIND_SLICE = 2 };
+
+/* absorb_header, challenger.rs:708-714 */
+static void fs_header(or_fs* fs, uint8_t op, uint8_t kind, uint64_t len) {
+    uint8_t h[16] = {0};
+    h[0] = op;
+    h[1] = kind;
+    for (int i = 0; i < 8; i++) h[8 + i] = (uint8_t)(len >> (8 * i));
+    sha_update(&fs->st, h, 16);
+}
+
+/* absorb_padded, challenger.rs:720-726 */
+static void fs_padded(or_fs* fs, const uint8_t* b, size_t n) {
+    static const uint8_t zeros[16] = {0};
+    sha_update(&fs->st, b, n);
+    if (n % 16) sha_update(&fs->st, zeros, 16 - n % 16);
+}
+
+static void fs_f128(or_fs* fs, or_f128 v) {
+    uint8_t b[16];
+    for (int i = 0; i < 8; i++) {
+        b[i] = (uint8_t)(v.lo >> (8 * i));
+        b[8 + i] = (uint8_t)(v.hi >> (8 * i));
+    }
+    sha_update(&fs->st, b, 16);
+}
+
+or_fs* or_fs_new(const uint8_t* domain, size_t len) {
+    or_fs* fs = (or_fs*)malloc(sizeof(or_fs));
+    if (!fs) return NULL;
+    sha_init(&fs->st);
+    fs_header(fs, OP_DOMAIN, KIND_NONE, len); /* with_hash, :648-659 */
+    fs_padded(fs, domain, len);
+    return fs;
+}
+
+void or_fs_free(or_fs* fs) { free(fs); }
+
+void or_fs_observe_label(or_fs* fs, const uint8_t* l, size_t n) { /* :806-809 */
+    fs_header(fs, OP_LABEL, KIND_NONE, n);
+    fs_padded(fs, l, n);
+}
+
+void or_fs_observe_f128(or_fs* fs, or_f128 v) { /* :811-814 */
+    fs_header(fs, OP_OBSERVE, KIND_SCALAR, 1);
+    fs_f128(fs, v);
+}
+
+void or_fs_observe_f128_slice(or_fs* fs, const or_f128* v, size_t n) { /* :816-821 */
+    fs_header(fs, OP_OBSERVE, KIND_SLICE, n);
+    for (size_t i = 0; i < n; i++) fs_f128(fs, v[i]);
+}
+
+void or_fs_observe_bytes(or_fs* fs, const uint8_t* b, size_t n) { /* :823-826 */
+    fs_header(fs, OP_BYTES, KIND_NONE, n);
+    fs_padded(fs, b, n);
+}
+
+/* squeeze_into: SHA256(state || ctr_le) blocks, state not mutated, :745-763 */
+static void fs_squeeze(or_fs* fs, uint8_t* out, size_t n) {
+    size_t off = 0;
+    uint64_t ctr = 0;
+    while (off < n) {
+        sha_ctx c = fs->st;
+        uint8_t cb[8];
+        for (int i = 0; i < 8; i++) cb[i] = (uint8_t)(ctr >> (8 * i));
+        sha_update(&c, cb, 8);
+        uint8_t blk[32];
+        sha_final(&c, blk);
+        size_t take = n - off < 32 ? n - off : 32;
+        memcpy(out + off, blk, take);
+        off += take;
+        ctr++;
+    }
+}
+
+static or_f128 le_f128(const uint8_t* b) {
+    or_f128 v = {0, 0};
+    for (int i = 0; i < 8; i++) {
+        v.lo |= (uint64_t)b[i] << (8 * i);
+        v.hi |= (uint64_t)b[8 + i] << (8 * i);
+    }
+    return v;
+}
+
+or_f128 or_fs_sample_f128(or_fs* fs) { /* :828-846 */
+    fs_header(fs, OP_SQUEEZE, KIND_SCALAR, 1);
+    uint8_t buf[16];
+    fs_squeeze(fs, buf, 16);
+    return le_f128(buf);
+}
+
+void or_fs_sample_f128_vec(or_fs* fs, or_f128* out, size_t n) { /* :862-882 */
+    fs_header(fs, OP_SQUEEZE, KIND_SLICE, n);
+    uint8_t* buf = (uint8_t*)malloc(n * 16 + 1);
+    fs_squeeze(fs, buf, n * 16);
+    for (size_t i = 0; i < n; i++) out[i] = le_f128(buf + 16 * i);
+    free(buf);
+}
+
+/* has_leading_zero_bits, challenger.rs:1096-1109 */
+static int leading_zero_bits(const uint8_t* h, uint32_t bits) {
+    uint32_t full = bits / 8, extra = bits % 8;
+    for (uint32_t i = 0; i < full; i++)
+        if (h[i]) return 0;
+    if (extra && (h[full] >> (8 - extra))) return 0;
+    return 1;
+}
+
+/* grind_pow: smallest nonce with SHA256(state_digest || nonce_le) having `bits`
+ * leading zero bits, then observe_bytes(nonce_le), challenger.rs:884-958, 1127-1135 */
+uint64_t or_fs_grind_pow(or_fs* fs, uint32_t bits) {
+    uint8_t digest[32];
+    sha_final(&fs->st, digest);
+    uint64_t nonce = 0;
+    if (bits) {
+        for (;; nonce++) {
+            uint8_t pre[40], h[32];
+            memcpy(pre, digest, 32);
+            for (int i = 0; i < 8; i++) pre[32 + i] = (uint8_t)(nonce >> (8 * i));
+            or_sha256(pre, 40, h);
+            if (leading_zero_bits(h, bits)) break;
+        }
+    }
+    uint8_t nb[8];
+    for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
+    or_fs_observe_bytes(fs, nb, 8);
+    return nonce;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Sumcheck rounds                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* round_msg_lsb, ligerito.rs:3204-3226 (sequential branch) */
+void or_round_msg_lsb(const or_f128* f, const or_f128* b, size_t n, or_f128 msg[2]) {
+    or_f128 u0 = ZERO, u2 = ZERO;
+    for (size_t j = 0; j < n / 2; j++) {
+        or_f128 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
+        u0 = add(u0, or_f128_mul(f0, b0));
+        u2 = add(u2, or_f128_mul(add(f0, f1), add(b0, b1)));
+    }
+    msg[0] = u0;
+    msg[1] = u2;
+}
+
+/* fold_and_msg_lsb, ligerito.rs:3372-3414: fold v0 + r(v0+v1), then the next
+ * round's message over the folded pairs (none when half == 1). */
+void or_fold_and_msg_lsb(const or_f128* f, const or_f128* b, size_t n, or_f128 r, or_f128* nf,
+                         or_f128* nb, or_f128 msg[2]) {
+    size_t half = n / 2;
+    for (size_t j = 0; j < half; j++) {
+        or_f128 f0 = f[2 * j], b0 = b[2 * j];
+        nf[j] = add(f0, or_f128_mul(add(f0, f[2 * j + 1]), r));
+        nb[j] = add(b0, or_f128_mul(add(b0, b[2 * j + 1]), r));
+    }
+    or_f128 u0 = ZERO, u2 = ZERO;
+    for (size_t k = 0; k + 1 < half; k += 2) {
+        u0 = add(u0, or_f128_mul(nf[k], nb[k]));
+        u2 = add(u2, or_f128_mul(add(nf[k], nf[k + 1]), add(nb[k], nb[k + 1])));
+    }
+    msg[0] = u0;
+    msg[1] = u2;
+}
+
+/* build_eq, univariate_skip.rs:31-47 */
+void or_build_eq(const or_f128* r, size_t n, or_f128* t) {
+    t[0] = ONE;
+    for (size_t i = 0; i < n; i++) {
+        or_f128 ri = r[i], omr = add(ONE, r[i]);
+        for (size_t x = ((size_t)1 << i); x-- > 0;) {
+            t[x | ((size_t)1 << i)] = or_f128_mul(t[x], ri);
+            t[x] = or_f128_mul(t[x], omr);
+        }
+    }
+}
+
+/* round_pair_naive, multilinear.rs:331-355: (r[0]·G(1), G(inf)) */
+void or_round_pair(const or_f128* a, const or_f128* b, size_t n, const or_f128* r, or_f128 out[2]) {
+    size_t half = n / 2;
+    size_t log_n = 0;
+    while (((size_t)1 << log_n) < n) log_n++;
+    or_f128* eq = (or_f128*)malloc(sizeof(or_f128) * half);
+    or_build_eq(r + 1, log_n - 1, eq);
+    or_f128 g1 = ZERO, gi = ZERO;
+    for (size_t x = 0; x < half; x++) {
+        or_f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+        g1 = add(g1, or_f128_mul(or_f128_mul(eq[x], a1), b1));
+        gi = add(gi, or_f128_mul(or_f128_mul(eq[x], add(a0, a1)), add(b0, b1)));
+    }
+    free(eq);
+    out[0] = or_f128_mul(r[0], g1);
+    out[1] = gi;
+}
+
+/* fold_in_place_pair, multilinear.rs:1148-1163 */
+void or_fold_pair(const or_f128* a, const or_f128* b, size_t n, or_f128 c, or_f128* ao,
+                  or_f128* bo) {
+    for (size_t x = 0; x < n / 2; x++) {
+        or_f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+        ao[x] = add(a0, or_f128_mul(c, add(a1, a0)));
+        bo[x] = add(b0, or_f128_mul(c, add(b1, b0)));
+    }
+}
+
+static const uint8_t LBL_PRODUCT[] = "flock-product-sumcheck";
+static const uint8_t LBL_ZC[] = "flock-zerocheck-v0";
+
+int or_sumcheck_transcript(const uint8_t* domain, size_t dlen, const or_f128* a, const or_f128* b,
+                           int log_n, or_f128* msgs, or_f128 finals[4]) {
+    if (log_n < 1 || log_n > 40) return -1;
+    size_t n = (size_t)1 << log_n;
+    or_fs* fs = or_fs_new(domain, dlen);
+    or_f128* f = (or_f128*)malloc(sizeof(or_f128) * n);
+    or_f128* g = (or_f128*)malloc(sizeof(or_f128) * n);
+    or_f128* f2 = (or_f128*)malloc(sizeof(or_f128) * (n / 2 + 1));
+    or_f128* g2 = (or_f128*)malloc(sizeof(or_f128) * (n / 2 + 1));
+    or_f128* rr = (or_f128*)malloc(sizeof(or_f128) * (size_t)(log_n + 1));
+    if (!fs || !f || !g || !f2 || !g2 || !rr) return -2;
+
+    /* product sumcheck: SumcheckProver::new then fold loop, ligerito.rs:4986-5011 */
+    or_fs_observe_label(fs, LBL_PRODUCT, sizeof(LBL_PRODUCT) - 1);
+    memcpy(f, a, sizeof(or_f128) * n);
+    memcpy(g, b, sizeof(or_f128) * n);
+    or_f128 m2[2];
+    or_round_msg_lsb(f, g, n, m2);
+    int mi = 0;
+    msgs[mi++] = m2[0];
+    msgs[mi++] = m2[1];
+    or_fs_observe_f128(fs, m2[0]);
+    or_fs_observe_f128(fs, m2[1]);
+    size_t len = n;
+    for (int i = 0; i + 1 < log_n; i++) {
+        or_f128 r = or_fs_sample_f128(fs);
+        or_fold_and_msg_lsb(f, g, len, r, f2, g2, m2);
+        len /= 2;
+        memcpy(f, f2, sizeof(or_f128) * len);
+        memcpy(g, g2, sizeof(or_f128) * len);
+        msgs[mi++] = m2[0];
+        msgs[mi++] = m2[1];
+        or_fs_observe_f128(fs, m2[0]);
+        or_fs_observe_f128(fs, m2[1]);
+    }
+    {
+        or_f128 r = or_fs_sample_f128(fs);
+        or_fold_and_msg_lsb(f, g, len, r, f2, g2, m2); /* half == 1: fold only */
+        finals[0] = f2[0];
+        finals[1] = g2[0];
+        or_fs_observe_f128(fs, f2[0]);
+        or_fs_observe_f128(fs, g2[0]);
+    }
+
+    /* eq-weighted zerocheck tail, zerocheck.rs:790-975 with Convention A r_next[0] = ONE */
+    or_fs_observe_label(fs, LBL_ZC, sizeof(LBL_ZC) - 1);
+    or_f128* r = (or_f128*)malloc(sizeof(or_f128) * (size_t)log_n);
+    or_fs_sample_f128_vec(fs, r, (size_t)log_n);
+    memcpy(f, a, sizeof(or_f128) * n);
+    memcpy(g, b, sizeof(or_f128) * n);
+    len = n;
+    rr[0] = ONE;
+    for (int k = 1; k < log_n; k++) rr[k] = r[k];
+    or_round_pair(f, g, len, rr, m2);
+    msgs[mi++] = m2[0];
+    msgs[mi++] = m2[1];
+    or_fs_observe_f128(fs, m2[0]);
+    or_fs_observe_f128(fs, m2[1]);
+    or_f128 rho = or_fs_sample_f128(fs);
+    for (int i = 0; i + 1 < log_n; i++) {
+        or_fold_pair(f, g, len, rho, f2, g2);
+        len /= 2;
+        memcpy(f, f2, sizeof(or_f128) * len);
+        memcpy(g, g2, sizeof(or_f128) * len);
+        rr[0] = ONE;
+        for (int k = i + 2; k < log_n; k++) rr[k - i - 1] = r[k];
+        or_round_pair(f, g, len, rr, m2);
+        msgs[mi++] = m2[0];
+        msgs[mi++] = m2[1];
+        or_fs_observe_f128(fs, m2[0]);
+        or_fs_observe_f128(fs, m2[1]);
+        rho = or_fs_sample_f128(fs);
+    }
+    or_fold_pair(f, g, len, rho, f2, g2);
+    finals[2] = f2[0];
+    finals[3] = g2[0];
+    or_fs_observe_f128(fs, f2[0]);
+    or_fs_observe_f128(fs, g2[0]);
+
+    free(r);
+    free(rr);
+    free(f);
+    free(g);
+    free(f2);
+    free(g2);
+    or_fs_free(fs);
+    return 0;
+}

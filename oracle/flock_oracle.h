@@ -1,0 +1,102 @@
+/*
+ * flock_oracle.h — CPU restatement of the Flock `prove_fast` data-parallel core.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the parity checker for the HIP
+ * kernels in flock_amd/csrc and the timed CPU baseline in bench.py. Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+ * The shipped library (libflock_hip.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference checkout). Field elements are F128 = {lo, hi}, lo holding the
+ * coefficients of x^0..x^63, reduction polynomial x^128 + x^7 + x^2 + x + 1
+ * (crates/flock-core/src/field/gf2_128.rs:10-27).
+ */
+#ifndef FLOCK_ORACLE_H
+#define FLOCK_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t lo, hi; } or_f128;
+
+/* ---- field (crates/flock-core/src/field/gf2_128.rs) ---- */
+or_f128 or_f128_mul(or_f128 a, or_f128 b);            /* Mul, :174-201 */
+or_f128 or_f128_mul_portable(or_f128 a, or_f128 b);   /* software::ghash_mul, gf2_128/portable.rs:40-43 */
+or_f128 or_f128_inv(or_f128 a);                       /* inv, :67-80 */
+or_f128 or_f128_mul_by_x(or_f128 a);                  /* mul_by_x, :213-220 */
+void or_f128_mul_batch(const or_f128* a, const or_f128* b, or_f128* out, size_t n);
+int or_has_pclmul(void);
+
+/* ---- SplitMix64 (crates/flock-prover/src/bin/dump_commit_vectors.rs:42-68) ---- */
+void or_splitmix_f128(uint64_t seed, or_f128* out, size_t n);
+
+/* ---- additive NTT (crates/flock-core/src/ntt/additive_ntt_f128.rs) ---- */
+/* evals table for the standard basis {1, x, .., x^(L-1)}: row i has L-i entries,
+ * stored row-major in an L x L array (unused tail zero). :67-95, :127-133 */
+int or_ntt_evals(int L, or_f128* evals_LxL);
+or_f128 or_ntt_twiddle(const or_f128* evals_LxL, int L, int layer, uint64_t block); /* :141-144 */
+/* interleaved_scalar_live_from_layer, :287-319. data has 2^log_d * num_ntts words. */
+int or_ntt_forward_interleaved(or_f128* data, size_t num_ntts, size_t live, int log_d, int L,
+                               int start_layer);
+int or_ntt_inverse_scalar(or_f128* data, int log_d, int L); /* :651-673 */
+
+/* ---- SHA-256 (FIPS 180-4) and the Merkle tree (crates/flock-core/src/merkle.rs) ---- */
+void or_sha256(const uint8_t* data, size_t len, uint8_t out[32]);
+/* merkle_tree_sequential, :593-617 — tree has 2*num_leaves-1 nodes of 32 bytes */
+int or_merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, uint8_t* tree);
+
+/* ---- PCS commit (crates/flock-core/src/pcs/commit.rs) ---- */
+typedef struct {
+    uint32_t m, log_inv_rate, log_batch_size;
+    uint32_t num_lanes; /* 0 = None (all 2^log_batch_size lanes) */
+} or_pcs_params;
+int or_pcs_sizes(const or_pcs_params* p, size_t* msg_len, size_t* cw_len, size_t* n_leaves,
+                 int* k_code, size_t* num_ntts);
+/* commit_lane_major, :474-510 (+ replicate_lane_major_fill :516-536, finalize :572-633) */
+int or_commit_lane_major(const or_f128* q, size_t q_len, const or_pcs_params* p, or_f128* codeword,
+                         uint8_t* tree);
+/* commit / commit_into, :345-397 (+ replicate_message_fill :543-568) */
+int or_commit(const or_f128* msg, size_t msg_len, const or_pcs_params* p, or_f128* codeword,
+              uint8_t* tree);
+
+/* ---- Fiat-Shamir challenger, SHA-256 discipline (crates/flock-core/src/challenger.rs:609-1044) ---- */
+typedef struct or_fs or_fs;
+or_fs* or_fs_new(const uint8_t* domain, size_t len);
+void or_fs_free(or_fs* fs);
+void or_fs_observe_label(or_fs* fs, const uint8_t* label, size_t len);
+void or_fs_observe_f128(or_fs* fs, or_f128 v);
+void or_fs_observe_f128_slice(or_fs* fs, const or_f128* v, size_t n);
+void or_fs_observe_bytes(or_fs* fs, const uint8_t* b, size_t n);
+or_f128 or_fs_sample_f128(or_fs* fs);
+void or_fs_sample_f128_vec(or_fs* fs, or_f128* out, size_t n);
+uint64_t or_fs_grind_pow(or_fs* fs, uint32_t bits);
+
+/* ---- sumcheck rounds ---- */
+/* round_msg_lsb, crates/flock-core/src/pcs/ligerito.rs:3204-3240 */
+void or_round_msg_lsb(const or_f128* f, const or_f128* b, size_t n, or_f128 msg[2]);
+/* fold_and_msg_lsb, ligerito.rs:3372-3414; nf/nb get n/2 words */
+void or_fold_and_msg_lsb(const or_f128* f, const or_f128* b, size_t n, or_f128 r, or_f128* nf,
+                         or_f128* nb, or_f128 msg[2]);
+/* build_eq, crates/flock-core/src/zerocheck/univariate_skip.rs:31-47 */
+void or_build_eq(const or_f128* r, size_t n, or_f128* out);
+/* round_pair_naive, crates/flock-core/src/zerocheck/multilinear.rs:331-355 */
+void or_round_pair(const or_f128* a, const or_f128* b, size_t n, const or_f128* r, or_f128 out[2]);
+/* fold_in_place_pair, multilinear.rs:1148-1163 (out-of-place: a_out/b_out get n/2) */
+void or_fold_pair(const or_f128* a, const or_f128* b, size_t n, or_f128 c, or_f128* a_out,
+                  or_f128* b_out);
+
+/* Full product sumcheck + eq-weighted zerocheck tail over one transcript. Mirrors the
+ * SumcheckProver loop at ligerito.rs:4986-5011 and the tail loop at
+ * crates/flock-core/src/zerocheck.rs:790-975. msgs: 2*log_n product messages then
+ * 2*log_n zerocheck messages; finals: f, b (product), a, b (zerocheck). */
+int or_sumcheck_transcript(const uint8_t* domain, size_t dlen, const or_f128* a, const or_f128* b,
+                           int log_n, or_f128* msgs, or_f128 finals[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

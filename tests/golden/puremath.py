@@ -1,0 +1,194 @@
+"""Pure-math restatements used to write and check the golden fixtures.
+
+Independent of the C oracle: F_{2^128} products come from integer carry-less
+multiplication followed by textbook polynomial long division by
+x^128 + x^7 + x^2 + x + 1; the additive NTT is checked by evaluating the
+novel-basis polynomial X_j(z) = prod_{i in bits(j)} W_i(z)/W_i(beta_i) directly
+at the subspace points (definition: crates/flock-core/src/ntt/additive_ntt_f128.rs:20-41);
+SHA-256 is hashlib (FIPS 180-4).
+"""
+from __future__ import annotations
+
+import hashlib
+
+POLY = (1 << 128) | 0x87
+MASK64 = (1 << 64) - 1
+
+
+def clmul(a: int, b: int) -> int:
+    r = 0
+    while b:
+        low = b & -b
+        r ^= a * low  # a << bit, no carries: a * 2^k
+        b ^= low
+    return r
+
+
+def reduce(x: int) -> int:
+    """Remainder of the polynomial x modulo POLY (long division)."""
+    while x.bit_length() > 128:
+        x ^= POLY << (x.bit_length() - 129)
+    return x
+
+
+def mul(a: int, b: int) -> int:
+    return reduce(clmul(a, b))
+
+
+def power(a: int, e: int) -> int:
+    r = 1
+    while e:
+        if e & 1:
+            r = mul(r, a)
+        a = mul(a, a)
+        e >>= 1
+    return r
+
+
+def inv(a: int) -> int:
+    return power(a, (1 << 128) - 2)
+
+
+def splitmix(seed: int, n: int) -> list[int]:
+    s = seed
+    out = []
+    for _ in range(n):
+        w = []
+        for _ in range(2):
+            s = (s + 0x9E3779B97F4A7C15) & MASK64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+            w.append(z ^ (z >> 31))
+        out.append(w[0] | (w[1] << 64))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Novel-basis polynomial evaluation (LCH basis over beta_i = x^i)
+# ---------------------------------------------------------------------------
+
+def subspace_W(i: int, z: int, basis: list[int]) -> int:
+    """W_0(z) = z; W_i(z) = W_{i-1}(z) * (W_{i-1}(z) + W_{i-1}(beta_{i-1}))."""
+    if i == 0:
+        return z
+    w = subspace_W(i - 1, z, basis)
+    return mul(w, w ^ subspace_W(i - 1, basis[i - 1], basis))
+
+
+def novel_basis_eval(coeffs: list[int], z: int, L: int) -> int:
+    basis = [1 << i for i in range(L)]
+    norm = [inv(subspace_W(i, basis[i], basis)) for i in range(L)]
+    what = [mul(subspace_W(i, z, basis), norm[i]) for i in range(L)]
+    acc = 0
+    for j, c in enumerate(coeffs):
+        if c == 0:
+            continue
+        x = 1
+        for i in range(L):
+            if (j >> i) & 1:
+                x = mul(x, what[i])
+        acc ^= mul(c, x)
+    return acc
+
+
+def ntt_by_definition(coeffs: list[int], L: int) -> list[int]:
+    """Evaluations at omega_k = sum_i bit_i(k) beta_i = the integer k."""
+    return [novel_basis_eval(coeffs, k, L) for k in range(1 << L)]
+
+
+# ---------------------------------------------------------------------------
+# Sumcheck brute force
+# ---------------------------------------------------------------------------
+
+def eq_point(r: list[int], x: int) -> int:
+    acc = 1
+    for i, ri in enumerate(r):
+        acc = mul(acc, ri if (x >> i) & 1 else ri ^ 1)
+    return acc
+
+
+def sha256(b: bytes) -> bytes:
+    return hashlib.sha256(b).digest()
+
+
+def f128_bytes(v: int) -> bytes:
+    return v.to_bytes(16, "little")
+
+
+def merkle_tree(data: bytes, n_leaves: int) -> list[bytes]:
+    leaf = len(data) // n_leaves
+    level = [sha256(data[i * leaf:(i + 1) * leaf]) for i in range(n_leaves)]
+    tree = list(level)
+    while len(level) > 1:
+        level = [sha256(level[2 * i] + level[2 * i + 1]) for i in range(len(level) // 2)]
+        tree += level
+    return tree
+
+
+class FsChallenger:
+    """SHA-256 Fiat-Shamir transcript from the framing rules of
+    crates/flock-core/src/challenger.rs:690-882 (16-byte headers, zero-padded
+    payloads, SHA256(state || ctr_le) squeezes)."""
+
+    def __init__(self, domain: bytes):
+        self.h = hashlib.sha256()
+        self._hdr(1, 0, len(domain))
+        self._pad(domain)
+
+    def _hdr(self, op, kind, n):
+        self.h.update(bytes([op, kind]) + bytes(6) + n.to_bytes(8, "little"))
+
+    def _pad(self, b):
+        self.h.update(b)
+        if len(b) % 16:
+            self.h.update(bytes(16 - len(b) % 16))
+
+    def observe_label(self, b):
+        self._hdr(2, 0, len(b))
+        self._pad(b)
+
+    def observe_f128(self, v):
+        self._hdr(3, 1, 1)
+        self.h.update(f128_bytes(v))
+
+    def observe_f128_slice(self, vs):
+        self._hdr(3, 2, len(vs))
+        for v in vs:
+            self.h.update(f128_bytes(v))
+
+    def observe_bytes(self, b):
+        self._hdr(5, 0, len(b))
+        self._pad(b)
+
+    def _squeeze(self, n):
+        out = b""
+        ctr = 0
+        while len(out) < n:
+            c = self.h.copy()
+            c.update(ctr.to_bytes(8, "little"))
+            out += c.digest()
+            ctr += 1
+        return out[:n]
+
+    def sample_f128(self):
+        self._hdr(4, 1, 1)
+        return int.from_bytes(self._squeeze(16), "little")
+
+    def sample_f128_vec(self, n):
+        self._hdr(4, 2, n)
+        raw = self._squeeze(16 * n)
+        return [int.from_bytes(raw[16 * i:16 * i + 16], "little") for i in range(n)]
+
+    def grind_pow(self, bits):
+        digest = self.h.copy().digest()
+        nonce = 0
+        if bits:
+            while True:
+                h = hashlib.sha256(digest + nonce.to_bytes(8, "little")).digest()
+                v = int.from_bytes(h, "big")
+                if v >> (256 - bits) == 0:
+                    break
+                nonce += 1
+        self.observe_bytes(nonce.to_bytes(8, "little"))
+        return nonce
