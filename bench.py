@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the Ligerito commit hot path (LCH additive NTT + SHA-256 Merkle) on one
+MI355X, plus the standalone sumcheck/zerocheck of 2^24 tables.
+
+One step = one `commit` of a resident message (replicate fill, NTT layers
+log_inv_rate..k_code, Merkle tree): the config-1 shape (rate 1/4, 256 interleaved lanes,
+4 KiB leaves) with 2^16 message rows (m = 31) instead of 2^10, so every NTT pass fuses
+exactly 4 layers and the run is not launch-bound. `value` is the NTT throughput in
+compulsory bytes (read message + write codeword = 16*(msg_len + cw_len) per commit) over
+the NTT time of the timed steps, measured with HIP events on the library's stream.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("prove_fast k hashes/s (SHA-256 2^16); NTT GB/s vs HBM peak; "
+          "p50/p99 proof latency")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's own)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def new(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        out = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(out), a, b) == 0
+        return float(out.value)
+
+
+def cpu_baseline(m: int, reps: int) -> dict:
+    """The oracle's scalar NTT (one core) on the same commit shape at a smaller m."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the checker doubles as the CPU baseline
+
+    rate, lb = 2, 8
+    s = O.pcs_sizes(m, rate, lb)
+    msg = O.splitmix(0xC0FFEE, s["msg_len"])
+    cw = np.ascontiguousarray(np.tile(msg, (1 << rate, 1)))
+    k = s["k_code"]
+    best = None
+    O.ntt_forward_interleaved(cw[: (1 << (k - 4)) * 256], 256, 256, k - 4, k - 4, 0)  # warm-up
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.ntt_forward_interleaved(cw, 256, 256, k, k, rate)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    byts = 16 * (s["msg_len"] + s["cw_len"])
+    return {"value": round(byts / best / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"oracle scalar interleaved NTT, commit m={m} rate 1/4 256 lanes "
+                      f"(k_code={k}), best of {reps} after 1 warm-up, "
+                      f"{best*1e3:.0f} ms per transform"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--m", type=int, default=31)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sumcheck", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md and not launched.", file=sys.stderr)
+        sys.exit(2)
+
+    from flock_amd import lib as F
+
+    ctx = F.Context(0)
+    ev = HipEvents()
+    stream = ctx.stream
+    m, rate, lb = args.m, 2, 8
+    p = F.PcsParams(m, rate, lb, 0, 0)
+    s = F.pcs_sizes(p)
+    k = s["k_code"]
+    log(f"commit m={m} k_code={k} msg={s['msg_len']} cw={s['cw_len']} leaves={s['n_leaves']}")
+    rng = np.random.default_rng(0xC0FFEE)
+    msg = rng.integers(0, 2**63, size=(s["msg_len"], 2), dtype=np.uint64)
+    dmsg = ctx.upload(msg)
+    del msg
+    cw = ctx.alloc(16 * s["cw_len"])
+    tree = ctx.alloc(32 * (2 * s["n_leaves"] - 1))
+    ntt_bytes = 16 * (s["msg_len"] + s["cw_len"])
+    n_passes = -(-(k - rate) // 4)
+
+    for _ in range(args.warmup):
+        ctx.commit(dmsg, s["msg_len"], p, cw, tree)
+    ctx.sync()
+
+    # Timed region: exactly K full commits, bracketed by stream syncs; each commit is also
+    # bracketed by HIP events on the library's stream.
+    e = [ev.new() for _ in range(4)]
+    commit_times = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev.record(e[0], stream)
+        ctx.commit(dmsg, s["msg_len"], p, cw, tree)
+        ev.record(e[1], stream)
+        commit_times.append(ev.ms(e[0], e[1]))
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    commit_ms = float(np.mean(commit_times))
+    # Stage split: the NTT alone (same passes over the codeword) and the Merkle tree alone.
+    ntt_times, mt = [], []
+    for _ in range(args.steps):
+        ev.record(e[2], stream)
+        ctx.ntt_forward_interleaved(cw, 256, 256, k, rate)
+        ev.record(e[3], stream)
+        ntt_times.append(ev.ms(e[2], e[3]))
+    for _ in range(args.steps):
+        ev.record(e[2], stream)
+        ctx.merkle_tree(cw, 16 * s["cw_len"], s["n_leaves"], tree)
+        ev.record(e[3], stream)
+        mt.append(ev.ms(e[2], e[3]))
+    ntt_ms = float(np.mean(ntt_times))
+    merkle_ms = float(np.mean(mt))
+    ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
+    pass_ms = ntt_ms / n_passes
+    pass_bytes = 32 * s["cw_len"]  # each fused pass reads and writes the codeword once
+    pass_gbs = pass_bytes / (pass_ms * 1e-3) / 1e9
+    muls = (k - rate) * s["cw_len"] // 2
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ntt_pass_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as fh:
+                tj = json.load(fh)
+            if tj.get("m") == m:
+                traffic = tj.get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": round(ntt_gbs, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(commit_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (seeded uniform F128 message)",
+        "config": {
+            "workload": (f"Ligerito commit: LCH additive NTT rate 1/4 + SHA-256 Merkle, "
+                         f"m={m}, 256 lanes, 2^{m - 15} message rows, k_code={k}"),
+            "m": m, "log_inv_rate": rate, "log_batch_size": lb,
+            "msg_words": s["msg_len"], "codeword_words": s["cw_len"], "leaves": s["n_leaves"],
+        },
+        "ntt_ms": round(ntt_ms, 4),
+        "merkle_ms": round(merkle_ms, 4),
+        "commit_ms": round(commit_ms, 4),
+        "ntt_compulsory_frac_of_hbm": round(ntt_gbs / HBM_PEAK_GBS, 5),
+        "ntt_f128_muls_per_s": round(muls / (ntt_ms * 1e-3), 1),
+        "merkle_leaf_gbs": round(16 * s["cw_len"] / (merkle_ms * 1e-3) / 1e9, 3),
+        "roofline": {
+            "bound": "hbm", "achieved": round(pass_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(pass_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "kernel": "k_ntt_pass<4>", "launch_ms": round(pass_ms, 4),
+            "bytes_per_launch": pass_bytes,
+        },
+        "timed_wall_s": round(wall, 3),
+    }
+
+    if not args.no_sumcheck:
+        log_n = 24
+        n = 1 << log_n
+        a = rng.integers(0, 2**63, size=(n, 2), dtype=np.uint64)
+        b = rng.integers(0, 2**63, size=(n, 2), dtype=np.uint64)
+        da, db = ctx.upload(a), ctx.upload(b)
+        del a, b
+        ctx.prove_sumcheck(F.FsChallenger(b"flock-bench-sumcheck"), da, db, log_n)  # warm-up
+        times = []
+        for _ in range(3):
+            ctx.sync()
+            t1 = time.perf_counter()
+            ctx.prove_sumcheck(F.FsChallenger(b"flock-bench-sumcheck"), da, db, log_n)
+            times.append(time.perf_counter() - t1)
+        result["sumcheck_2^24_ms"] = round(min(times) * 1e3, 3)
+        # product: 2*16*2^(L+1)*... ≈ 48*2^(L+1) B; eq-weighted tail ≈ 56*2^(L+1) B
+        result["sumcheck_2^24_algorithmic_gbs"] = round(
+            (48 + 56) * 2 * n / (min(times)) / 1e9, 3)
+
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(m=27, reps=3)
+
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,810 @@
+// Host runtime and C ABI of libflock_hip.so (declared in include/flock_hip.h).
+//
+// Holds the context (stream + scratch), the NTT basis tables, the host Fiat-Shamir
+// transcript, and the protocol drivers that sequence kernels between challenges.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#if defined(__x86_64__)
+#include <cpuid.h>
+#include <emmintrin.h>
+#include <wmmintrin.h>
+#endif
+
+#include "../../include/flock_hip.h"
+#include "flock_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            return fail(FLOCK_HIP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+inline int log2u(uint64_t x) {
+    int l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// Host F128 arithmetic (setup work only: NTT basis tables and the r0 factor).
+// ---------------------------------------------------------------------------
+struct F {
+    uint64_t lo, hi;
+};
+
+inline void clmul64(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi) {
+    uint64_t l = 0, h = 0;
+    for (int i = 0; i < 64; i++)
+        if ((a >> i) & 1) {
+            l ^= b << i;
+            if (i) h ^= b >> (64 - i);
+        }
+    *lo = l;
+    *hi = h;
+}
+
+// ghash reduction identity: U*x^128 = U + U<<1 + U<<2 + U<<7 (mod x^128+x^7+x^2+x+1)
+inline F reduce256(uint64_t r0, uint64_t r1, uint64_t r2, uint64_t r3) {
+    uint64_t tl = r2 ^ (r2 << 1) ^ (r2 << 2) ^ (r2 << 7);
+    uint64_t th = r3 ^ ((r3 << 1) | (r2 >> 63)) ^ ((r3 << 2) | (r2 >> 62)) ^
+                  ((r3 << 7) | (r2 >> 57));
+    uint64_t ov = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
+    uint64_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+    return F{r0 ^ tl ^ corr, r1 ^ th};
+}
+
+#if defined(__x86_64__)
+__attribute__((target("pclmul,sse2"))) F mul_pclmul(F a, F b) {
+    __m128i A = _mm_set_epi64x((long long)a.hi, (long long)a.lo);
+    __m128i B = _mm_set_epi64x((long long)b.hi, (long long)b.lo);
+    __m128i ll = _mm_clmulepi64_si128(A, B, 0x00);
+    __m128i hh = _mm_clmulepi64_si128(A, B, 0x11);
+    __m128i cr = _mm_xor_si128(_mm_clmulepi64_si128(A, B, 0x10), _mm_clmulepi64_si128(A, B, 0x01));
+    uint64_t l[2], h[2], c[2];
+    _mm_storeu_si128((__m128i*)l, ll);
+    _mm_storeu_si128((__m128i*)h, hh);
+    _mm_storeu_si128((__m128i*)c, cr);
+    return reduce256(l[0], l[1] ^ c[0], h[0] ^ c[1], h[1]);
+}
+bool have_pclmul() {
+    static int v = -1;
+    if (v < 0) {
+        unsigned a, b, c, d;
+        v = (__get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL)) ? 1 : 0;
+    }
+    return v == 1;
+}
+#endif
+
+F fmul(F a, F b) {
+#if defined(__x86_64__)
+    if (have_pclmul()) return mul_pclmul(a, b);
+#endif
+    uint64_t ll, lh, hl, hh, t0, t1, t2, t3;
+    clmul64(a.lo, b.lo, &ll, &t0);
+    clmul64(a.lo, b.hi, &lh, &t1);
+    clmul64(a.hi, b.lo, &hl, &t2);
+    clmul64(a.hi, b.hi, &hh, &t3);
+    return reduce256(ll, t0 ^ lh ^ hl, hh ^ t1 ^ t2, t3);
+}
+inline F fadd(F a, F b) { return F{a.lo ^ b.lo, a.hi ^ b.hi}; }
+F finv(F x) {  // x^(2^128 - 2)
+    F r{1, 0}, cur = fmul(x, x);
+    for (int i = 1; i < 128; i++) {
+        r = fmul(r, cur);
+        cur = fmul(cur, cur);
+    }
+    return r;
+}
+
+// Normalized subspace-polynomial table for the basis {1, x, .., x^(L-1)}:
+// row i = [W_i(b_i), .., W_i(b_{L-1})] / W_i(b_i), rows padded to L entries.
+std::vector<F> ntt_evals(int L) {
+    std::vector<F> ev((size_t)L * L, F{0, 0});
+    for (int k = 0; k < L; k++) ev[k] = F{1ull << k, 0};
+    for (int i = 1; i < L; i++) {
+        const F* prev = &ev[(size_t)(i - 1) * L];
+        F* row = &ev[(size_t)i * L];
+        for (int k = 1; k < L - i + 1; k++) row[k - 1] = fmul(prev[k], fadd(prev[k], prev[0]));
+    }
+    for (int i = 0; i < L; i++) {
+        F* row = &ev[(size_t)i * L];
+        F inv = finv(row[0]);
+        for (int k = 0; k < L - i; k++) row[k] = fmul(row[k], inv);
+    }
+    return ev;
+}
+
+// ---------------------------------------------------------------------------
+// SHA-256 with a copyable running state (the transcript hash).
+// ---------------------------------------------------------------------------
+const uint32_t K256[64] = {
+    0x428a2f98, 0x71374491, 0xb5c0fbcf, 0xe9b5dba5, 0x3956c25b, 0x59f111f1, 0x923f82a4, 0xab1c5ed5,
+    0xd807aa98, 0x12835b01, 0x243185be, 0x550c7dc3, 0x72be5d74, 0x80deb1fe, 0x9bdc06a7, 0xc19bf174,
+    0xe49b69c1, 0xefbe4786, 0x0fc19dc6, 0x240ca1cc, 0x2de92c6f, 0x4a7484aa, 0x5cb0a9dc, 0x76f988da,
+    0x983e5152, 0xa831c66d, 0xb00327c8, 0xbf597fc7, 0xc6e00bf3, 0xd5a79147, 0x06ca6351, 0x14292967,
+    0x27b70a85, 0x2e1b2138, 0x4d2c6dfc, 0x53380d13, 0x650a7354, 0x766a0abb, 0x81c2c92e, 0x92722c85,
+    0xa2bfe8a1, 0xa81a664b, 0xc24b8b70, 0xc76c51a3, 0xd192e819, 0xd6990624, 0xf40e3585, 0x106aa070,
+    0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
+    0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
+
+struct Sha256 {
+    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                     0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    uint64_t total = 0;
+    uint8_t buf[64];
+    size_t blen = 0;
+
+    static uint32_t rr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+    void block(const uint8_t* p) {
+        uint32_t w[64];
+        for (int i = 0; i < 16; i++)
+            w[i] = (uint32_t)p[4 * i] << 24 | (uint32_t)p[4 * i + 1] << 16 |
+                   (uint32_t)p[4 * i + 2] << 8 | p[4 * i + 3];
+        for (int i = 16; i < 64; i++)
+            w[i] = w[i - 16] + (rr(w[i - 15], 7) ^ rr(w[i - 15], 18) ^ (w[i - 15] >> 3)) +
+                   w[i - 7] + (rr(w[i - 2], 17) ^ rr(w[i - 2], 19) ^ (w[i - 2] >> 10));
+        uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], k = h[7];
+        for (int i = 0; i < 64; i++) {
+            uint32_t t1 = k + (rr(e, 6) ^ rr(e, 11) ^ rr(e, 25)) + ((e & f) ^ (~e & g)) + K256[i] + w[i];
+            uint32_t t2 = (rr(a, 2) ^ rr(a, 13) ^ rr(a, 22)) + ((a & b) ^ (a & c) ^ (b & c));
+            k = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
+        }
+        h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += k;
+    }
+    void update(const uint8_t* d, size_t n) {
+        total += n;
+        while (n) {
+            size_t t = 64 - blen < n ? 64 - blen : n;
+            memcpy(buf + blen, d, t);
+            blen += t; d += t; n -= t;
+            if (blen == 64) { block(buf); blen = 0; }
+        }
+    }
+    void digest(uint8_t out[32]) const {
+        Sha256 c = *this;
+        uint64_t bits = c.total * 8;
+        uint8_t pad = 0x80, z = 0;
+        c.update(&pad, 1);
+        while (c.blen != 56) c.update(&z, 1);
+        uint8_t len[8];
+        for (int i = 0; i < 8; i++) len[i] = (uint8_t)(bits >> (56 - 8 * i));
+        c.update(len, 8);
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 4; j++) out[4 * i + j] = (uint8_t)(c.h[i] >> (24 - 8 * j));
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fiat-Shamir transcript: FsChallenger with the SHA-256 discipline
+// (crates/flock-core/src/challenger.rs:609-1044). Every op absorbs a 16-byte header
+// [op][kind][0 x6][len u64 LE] (:708-714); byte payloads are zero-padded to 16 (:720-726);
+// a squeeze is SHA256(state || ctr_le) blocks without mutating the state (:745-763).
+// ---------------------------------------------------------------------------
+struct flock_fs {
+    Sha256 st;
+    void header(uint8_t op, uint8_t kind, uint64_t len) {
+        uint8_t h[16] = {0};
+        h[0] = op;
+        h[1] = kind;
+        for (int i = 0; i < 8; i++) h[8 + i] = (uint8_t)(len >> (8 * i));
+        st.update(h, 16);
+    }
+    void padded(const uint8_t* b, size_t n) {
+        static const uint8_t z[16] = {0};
+        st.update(b, n);
+        if (n % 16) st.update(z, 16 - n % 16);
+    }
+    void f128(flock_f128 v) {
+        uint8_t b[16];
+        for (int i = 0; i < 8; i++) {
+            b[i] = (uint8_t)(v.lo >> (8 * i));
+            b[8 + i] = (uint8_t)(v.hi >> (8 * i));
+        }
+        st.update(b, 16);
+    }
+    void squeeze(uint8_t* out, size_t n) const {
+        size_t off = 0;
+        uint64_t ctr = 0;
+        while (off < n) {
+            Sha256 c = st;
+            uint8_t cb[8];
+            for (int i = 0; i < 8; i++) cb[i] = (uint8_t)(ctr >> (8 * i));
+            c.update(cb, 8);
+            uint8_t blk[32];
+            c.digest(blk);
+            size_t t = n - off < 32 ? n - off : 32;
+            memcpy(out + off, blk, t);
+            off += t;
+            ctr++;
+        }
+    }
+};
+
+namespace {
+enum : uint8_t { OP_DOMAIN = 1, OP_LABEL = 2, OP_OBSERVE = 3, OP_SQUEEZE = 4, OP_BYTES = 5 };
+enum : uint8_t { KIND_NONE = 0, KIND_SCALAR = 1, KIND_SLICE = 2 };
+
+flock_f128 le128(const uint8_t* b) {
+    flock_f128 v{0, 0};
+    for (int i = 0; i < 8; i++) {
+        v.lo |= (uint64_t)b[i] << (8 * i);
+        v.hi |= (uint64_t)b[8 + i] << (8 * i);
+    }
+    return v;
+}
+}  // namespace
+
+extern "C" {
+
+flock_fs* flock_fs_new(const uint8_t* domain, size_t len) {
+    flock_fs* fs = new (std::nothrow) flock_fs();
+    if (!fs) return nullptr;
+    fs->header(OP_DOMAIN, KIND_NONE, len);
+    fs->padded(domain, len);
+    return fs;
+}
+void flock_fs_free(flock_fs* fs) { delete fs; }
+void flock_fs_observe_label(flock_fs* fs, const uint8_t* l, size_t n) {
+    fs->header(OP_LABEL, KIND_NONE, n);
+    fs->padded(l, n);
+}
+void flock_fs_observe_f128(flock_fs* fs, flock_f128 v) {
+    fs->header(OP_OBSERVE, KIND_SCALAR, 1);
+    fs->f128(v);
+}
+void flock_fs_observe_f128_slice(flock_fs* fs, const flock_f128* v, size_t n) {
+    fs->header(OP_OBSERVE, KIND_SLICE, n);
+    for (size_t i = 0; i < n; i++) fs->f128(v[i]);
+}
+void flock_fs_observe_bytes(flock_fs* fs, const uint8_t* b, size_t n) {
+    fs->header(OP_BYTES, KIND_NONE, n);
+    fs->padded(b, n);
+}
+void flock_fs_sample_f128(flock_fs* fs, flock_f128* out) {
+    fs->header(OP_SQUEEZE, KIND_SCALAR, 1);
+    uint8_t b[16];
+    fs->squeeze(b, 16);
+    *out = le128(b);
+}
+void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n) {
+    fs->header(OP_SQUEEZE, KIND_SLICE, n);
+    std::vector<uint8_t> b(n * 16 + 1);
+    fs->squeeze(b.data(), n * 16);
+    for (size_t i = 0; i < n; i++) out[i] = le128(b.data() + 16 * i);
+}
+// PoW (challenger.rs:884-958, 1096-1135): smallest nonce with SHA256(digest || nonce_le)
+// having `bits` leading zero bits (MSB-first), then observe_bytes(nonce_le).
+uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
+    uint8_t digest[32];
+    fs->st.digest(digest);
+    uint64_t nonce = 0;
+    if (bits) {
+        for (;; nonce++) {
+            uint8_t pre[40], h[32];
+            memcpy(pre, digest, 32);
+            for (int i = 0; i < 8; i++) pre[32 + i] = (uint8_t)(nonce >> (8 * i));
+            Sha256 s;
+            s.update(pre, 40);
+            s.digest(h);
+            uint32_t full = bits / 8, extra = bits % 8;
+            bool ok = true;
+            for (uint32_t i = 0; i < full && ok; i++) ok = h[i] == 0;
+            if (ok && extra) ok = (h[full] >> (8 - extra)) == 0;
+            if (ok) break;
+        }
+    }
+    uint8_t nb[8];
+    for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
+    flock_fs_observe_bytes(fs, nb, 8);
+    return nonce;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct flock_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words
+    uint4* dout = nullptr;      // 2 words
+    flock_f128* hout = nullptr; // pinned, 2 words
+    std::map<int, uint4*> evals;  // k -> k x k basis table
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    int ensure_scratch(size_t bytes) {
+        if (scratch_bytes >= bytes) return 0;
+        if (scratch) {
+            hipStreamSynchronize(stream);  // kernels may still read the old scratch
+            hipFree(scratch);
+        }
+        scratch = nullptr;
+        scratch_bytes = 0;
+        if (hipMalloc(&scratch, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FLOCK_HIP_E_NOMEM, "scratch allocation of " + std::to_string(bytes) +
+                                               " bytes failed");
+        }
+        scratch_bytes = bytes;
+        return 0;
+    }
+    int get_evals(int k, const uint4** out) {
+        auto it = evals.find(k);
+        if (it != evals.end()) {
+            *out = it->second;
+            return 0;
+        }
+        std::vector<F> ev = ntt_evals(k);
+        uint4* d = nullptr;
+        HIP_TRY(hipMalloc(&d, ev.size() * sizeof(F)));
+        HIP_TRY(hipMemcpy(d, ev.data(), ev.size() * sizeof(F), hipMemcpyHostToDevice));
+        evals[k] = d;
+        *out = d;
+        return 0;
+    }
+    // launch result pair -> host
+    int fetch_pair(flock_f128 out[2]) {
+        HIP_TRY(hipMemcpyAsync(hout, dout, 2 * sizeof(flock_f128), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        out[0] = hout[0];
+        out[1] = hout[1];
+        return 0;
+    }
+};
+
+namespace {
+inline uint4 U(flock_f128 v) {
+    return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi,
+                      (uint32_t)(v.hi >> 32));
+}
+inline const uint4* U(const flock_f128* p) { return reinterpret_cast<const uint4*>(p); }
+inline uint4* U(flock_f128* p) { return reinterpret_cast<uint4*>(p); }
+inline flock_f128 mulh(flock_f128 a, flock_f128 b) {
+    F r = fmul(F{a.lo, a.hi}, F{b.lo, b.hi});
+    return flock_f128{r.lo, r.hi};
+}
+#define CTX_CHECK(ctx) \
+    if (!(ctx)) return fail(FLOCK_HIP_E_INVALID, "null context")
+#define LAUNCH(expr) HIP_TRY(expr)
+}  // namespace
+
+extern "C" {
+
+int flock_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+const char* flock_hip_last_error(void) { return g_err.c_str(); }
+
+int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
+    if (!out) return fail(FLOCK_HIP_E_INVALID, "null out");
+    *out = nullptr;
+    int n = flock_hip_device_count();
+    if (device < 0 || device >= n) return fail(FLOCK_HIP_E_NODEVICE, "no such device");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(FLOCK_HIP_E_NODEVICE, std::string("device is ") + prop.gcnArchName +
+                                              ", this library is built for gfx950");
+    HIP_TRY(hipSetDevice(device));
+    flock_hip_ctx* c = new (std::nothrow) flock_hip_ctx();
+    if (!c) return fail(FLOCK_HIP_E_NOMEM, "ctx");
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->partials, sizeof(uint4) * flock::kMaxReduceBlocks * 2) != hipSuccess ||
+        hipMalloc(&c->dout, sizeof(uint4) * 2) != hipSuccess ||
+        hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 2, hipHostMallocDefault) !=
+            hipSuccess) {
+        hipError_t e = hipGetLastError();
+        flock_hip_ctx_destroy(c);
+        return fail(FLOCK_HIP_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+void flock_hip_ctx_destroy(flock_hip_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& kv : c->evals) hipFree(kv.second);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->partials) hipFree(c->partials);
+    if (c->dout) hipFree(c->dout);
+    if (c->hout) hipHostFree(c->hout);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void* flock_hip_ctx_stream(flock_hip_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int flock_hip_sync(flock_hip_ctx* c) {
+    CTX_CHECK(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int flock_hip_malloc(flock_hip_ctx* c, void** p, size_t bytes) {
+    CTX_CHECK(c);
+    if (!p) return fail(FLOCK_HIP_E_INVALID, "null out");
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FLOCK_HIP_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return 0;
+}
+
+int flock_hip_dfree(flock_hip_ctx* c, void* p) {
+    CTX_CHECK(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipFree(p));
+    return 0;
+}
+
+int flock_hip_memcpy_h2d(flock_hip_ctx* c, void* d, const void* h, size_t n) {
+    CTX_CHECK(c);
+    HIP_TRY(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int flock_hip_memcpy_d2h(flock_hip_ctx* c, void* h, const void* d, size_t n) {
+    CTX_CHECK(c);
+    HIP_TRY(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void flock_hip_free(void* p) { std::free(p); }
+
+// ---- field ----
+int flock_hip_f128_mul_batch(flock_hip_ctx* c, const flock_f128* a, const flock_f128* b,
+                             flock_f128* out, size_t n) {
+    CTX_CHECK(c);
+    if (n && (!a || !b || !out)) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_f128_mul_batch(U(a), U(b), U(out), n, c->stream));
+    return 0;
+}
+
+int flock_hip_f128_mul_const_batch(flock_hip_ctx* c, const flock_f128* a, flock_f128 t,
+                                   flock_f128* out, size_t n) {
+    CTX_CHECK(c);
+    if (n && (!a || !out)) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_f128_mul_const_batch(U(a), U(t), U(out), n, c->stream));
+    return 0;
+}
+
+// ---- NTT ----
+int flock_hip_ntt_forward_interleaved(flock_hip_ctx* c, flock_f128* data, size_t num_ntts,
+                                      size_t live, uint32_t log_d, uint32_t start_layer) {
+    CTX_CHECK(c);
+    if (!data || num_ntts == 0 || live > num_ntts || log_d > 62 || start_layer > log_d ||
+        num_ntts > 0xffffffffull)
+        return fail(FLOCK_HIP_E_INVALID, "bad NTT shape");
+    if (live == 0 || start_layer == log_d) return 0;
+    const uint4* ev = nullptr;
+    int rc = c->get_evals((int)log_d, &ev);
+    if (rc) return rc;
+    LAUNCH(flock::launch_ntt_forward(U(data), (uint32_t)num_ntts, (uint32_t)live, (int)log_d,
+                                     (int)start_layer, ev, c->stream));
+    return 0;
+}
+
+// ---- PCS ----
+int flock_hip_pcs_sizes(const flock_pcs_params* p, size_t* msg_len, size_t* cw_len,
+                        size_t* n_leaves, uint32_t* k_code, size_t* num_ntts) {
+    if (!p) return fail(FLOCK_HIP_E_INVALID, "null params");
+    // PcsParams::validate, pcs/commit.rs:270-297
+    if (p->m < 7 + p->log_batch_size)
+        return fail(FLOCK_HIP_E_M_TOO_SMALL, "m too small (need m >= 7 + log_batch_size)");
+    if (p->log_inv_rate < 1) return fail(FLOCK_HIP_E_INVALID, "log_inv_rate must be >= 1");
+    if (p->merkle_hash != 0) return fail(FLOCK_HIP_E_INVALID, "only SHA-256 Merkle trees");
+    const int log_dim = (int)p->m - 7 - (int)p->log_batch_size;
+    const int k = log_dim + (int)p->log_inv_rate;
+    if (k > 40 || p->log_batch_size > 20) return fail(FLOCK_HIP_E_INVALID, "size out of range");
+    const size_t full = (size_t)1 << p->log_batch_size;
+    const size_t t = p->num_lanes ? p->num_lanes : full;
+    if (t < 1 || t > full) return fail(FLOCK_HIP_E_INVALID, "num_lanes out of range");
+    if (msg_len) *msg_len = t << log_dim;
+    if (cw_len) *cw_len = ((size_t)1 << k) * t;
+    if (n_leaves) *n_leaves = (size_t)1 << k;
+    if (k_code) *k_code = (uint32_t)k;
+    if (num_ntts) *num_ntts = t;
+    return 0;
+}
+
+int flock_hip_merkle_tree(flock_hip_ctx* c, const uint8_t* data, size_t len, size_t n_leaves,
+                          uint8_t* tree) {
+    CTX_CHECK(c);
+    if (!data || !tree || !is_pow2(n_leaves) || len % n_leaves || (len / n_leaves) % 16)
+        return fail(FLOCK_HIP_E_INVALID, "bad Merkle shape");
+    LAUNCH(flock::launch_merkle_tree(data, n_leaves, len / n_leaves, tree, c->stream));
+    return 0;
+}
+
+static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len,
+                         const flock_pcs_params* p, bool lane_major, flock_f128* cw,
+                         uint8_t* tree, uint32_t cap_depth, uint8_t* h_cap) {
+    CTX_CHECK(c);
+    size_t msg_len, cw_len, n_leaves, t;
+    uint32_t k;
+    int rc = flock_hip_pcs_sizes(p, &msg_len, &cw_len, &n_leaves, &k, &t);
+    if (rc) return rc;
+    if (!src || !cw || !tree) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    if (cap_depth > k) return fail(FLOCK_HIP_E_INVALID, "cap depth exceeds tree depth");
+    const uint32_t reps = 1u << p->log_inv_rate;
+    if (lane_major) {
+        // commit_lane_major takes the full padded stack (pcs/commit.rs:477-482)
+        if (src_len != ((size_t)1 << (p->m - 7)))
+            return fail(FLOCK_HIP_E_INVALID, "lane-major message must be the full 2^(m-7) stack");
+        const uint64_t d = src_len >> p->log_batch_size;
+        LAUNCH(flock::launch_fill_lane_major(U(src), U(cw), d, (uint32_t)t, reps, c->stream));
+    } else {
+        if (src_len != msg_len) return fail(FLOCK_HIP_E_INVALID, "message length mismatch");
+        LAUNCH(flock::launch_fill_replicate(U(src), U(cw), msg_len, reps, c->stream));
+    }
+    const uint4* ev = nullptr;
+    rc = c->get_evals((int)k, &ev);
+    if (rc) return rc;
+    LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
+                                     (int)p->log_inv_rate, ev, c->stream));
+    LAUNCH(flock::launch_merkle_tree(reinterpret_cast<const uint8_t*>(cw), n_leaves, t * 16, tree,
+                                     c->stream));
+    if (h_cap) {
+        const size_t l = (size_t)1 << cap_depth;  // cap_layer, merkle.rs:649-656
+        HIP_TRY(hipMemcpyAsync(h_cap, tree + (2 * n_leaves - 2 * l) * 32, l * 32,
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int flock_hip_commit_lane_major(flock_hip_ctx* c, const flock_f128* q, size_t q_len,
+                                const flock_pcs_params* p, flock_f128* cw, uint8_t* tree,
+                                uint32_t cap_depth, uint8_t* h_cap) {
+    return commit_common(c, q, q_len, p, true, cw, tree, cap_depth, h_cap);
+}
+
+int flock_hip_commit(flock_hip_ctx* c, const flock_f128* msg, size_t msg_len,
+                     const flock_pcs_params* p, flock_f128* cw, uint8_t* tree,
+                     uint32_t cap_depth, uint8_t* h_cap) {
+    return commit_common(c, msg, msg_len, p, false, cw, tree, cap_depth, h_cap);
+}
+
+// ---- sumcheck rounds ----
+static int check_n(size_t n, size_t min) {
+    if (!is_pow2(n) || n < min)
+        return fail(FLOCK_HIP_E_INVALID, "table length must be a power of two >= " +
+                                             std::to_string(min));
+    return 0;
+}
+
+int flock_hip_sumcheck_round_msg(flock_hip_ctx* c, const flock_f128* f, const flock_f128* b,
+                                 size_t n, flock_f128 msg[2]) {
+    CTX_CHECK(c);
+    if (int rc = check_n(n, 2)) return rc;
+    if (!f || !b || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_sc_round_msg(U(f), U(b), n, c->partials, c->dout, c->stream));
+    return c->fetch_pair(msg);
+}
+
+int flock_hip_sumcheck_fold_msg(flock_hip_ctx* c, const flock_f128* f, const flock_f128* b,
+                                size_t n, flock_f128 r, flock_f128* nf, flock_f128* nb,
+                                flock_f128 msg[2]) {
+    CTX_CHECK(c);
+    if (int rc = check_n(n, 2)) return rc;
+    if (!f || !b || !nf || !nb || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_sc_fold_msg(U(f), U(b), n, U(r), U(nf), U(nb), c->partials, c->dout,
+                                     c->stream));
+    return c->fetch_pair(msg);
+}
+
+int flock_hip_build_eq(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* out) {
+    CTX_CHECK(c);
+    if (nv > 40 || !out || (nv && !h_r)) return fail(FLOCK_HIP_E_INVALID, "bad eq arguments");
+    const size_t scratch_words = nv + ((size_t)1 << (nv / 2)) + ((size_t)1 << (nv - nv / 2));
+    if (int rc = c->ensure_scratch(scratch_words * 16)) return rc;
+    uint4* dr = reinterpret_cast<uint4*>(c->scratch);
+    if (nv) HIP_TRY(hipMemcpyAsync(dr, h_r, nv * 16, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(flock::launch_build_eq(dr, (int)nv, U(out), dr + nv, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // h_r may be pageable and short-lived
+    return 0;
+}
+
+int flock_hip_zc_round_pair(flock_hip_ctx* c, const flock_f128* a, const flock_f128* b, size_t n,
+                            const flock_f128* eq, flock_f128 r0, flock_f128* eq_next,
+                            flock_f128 msg[2]) {
+    CTX_CHECK(c);
+    if (int rc = check_n(n, 2)) return rc;
+    if (!a || !b || !eq || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_zc_round(U(a), U(b), n, U(eq), n >= 4 ? U(eq_next) : nullptr,
+                                  c->partials, c->dout, c->stream));
+    if (int rc = c->fetch_pair(msg)) return rc;
+    msg[0] = mulh(r0, msg[0]);
+    return 0;
+}
+
+int flock_hip_zc_fold_round_pair(flock_hip_ctx* c, const flock_f128* a, const flock_f128* b,
+                                 size_t n, flock_f128 rho, flock_f128* ao, flock_f128* bo,
+                                 const flock_f128* eq, flock_f128 r0, flock_f128* eq_next,
+                                 flock_f128 msg[2]) {
+    CTX_CHECK(c);
+    if (int rc = check_n(n, 4)) return rc;
+    if (!a || !b || !ao || !bo || !eq || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_zc_fold_round(U(a), U(b), n, U(rho), U(ao), U(bo), U(eq),
+                                       n >= 8 ? U(eq_next) : nullptr, c->partials, c->dout,
+                                       c->stream));
+    if (int rc = c->fetch_pair(msg)) return rc;
+    msg[0] = mulh(r0, msg[0]);
+    return 0;
+}
+
+int flock_hip_fold_pair(flock_hip_ctx* c, const flock_f128* a, const flock_f128* b, size_t n,
+                        flock_f128 cc, flock_f128* ao, flock_f128* bo) {
+    CTX_CHECK(c);
+    if (int rc = check_n(n, 2)) return rc;
+    if (!a || !b || !ao || !bo) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    LAUNCH(flock::launch_fold_pair(U(a), U(b), n, U(cc), U(ao), U(bo), c->stream));
+    return 0;
+}
+
+// ---- protocol driver ----
+static const uint8_t kLabelProduct[] = "flock-product-sumcheck";
+static const uint8_t kLabelZc[] = "flock-zerocheck-v0";
+
+int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d_a,
+                             const flock_f128* d_b, uint32_t log_n, flock_f128* msgs,
+                             flock_f128 finals[4]) {
+    CTX_CHECK(c);
+    if (!fs || !d_a || !d_b || !msgs || !finals || log_n < 1 || log_n > 36)
+        return fail(FLOCK_HIP_E_INVALID, "bad sumcheck arguments");
+    const size_t n = (size_t)1 << log_n;
+    // scratch: two ping-pong table pairs (n/2 and n/4 words each), two eq tables (n/2,
+    // n/4), eq challenges and build_eq scratch.
+    const size_t h = n / 2 > 0 ? n / 2 : 1, q = n / 4 > 0 ? n / 4 : 1;
+    const size_t eq_scratch = 64 + ((size_t)1 << (log_n / 2 + 1)) + ((size_t)1 << (log_n / 2 + 1));
+    const size_t words = 2 * h + 2 * q + h + q + eq_scratch;
+    if (int rc = c->ensure_scratch(words * 16)) return rc;
+    uint4* base = reinterpret_cast<uint4*>(c->scratch);
+    uint4* P0f = base;
+    uint4* P0b = P0f + h;
+    uint4* P1f = P0b + h;
+    uint4* P1b = P1f + q;
+    uint4* E0 = P1b + q;
+    uint4* E1 = E0 + h;
+    uint4* R = E1 + q;  // up to 64 challenges
+    uint4* EQS = R + 64;
+    uint4* bufs[2][2] = {{P0f, P0b}, {P1f, P1b}};
+    uint4* eqb[2] = {E0, E1};
+    int mi = 0;
+    flock_f128 m2[2];
+
+    // ---- product sumcheck ----
+    flock_fs_observe_label(fs, kLabelProduct, sizeof(kLabelProduct) - 1);
+    LAUNCH(flock::launch_sc_round_msg(U(d_a), U(d_b), n, c->partials, c->dout, c->stream));
+    if (int rc = c->fetch_pair(m2)) return rc;
+    msgs[mi++] = m2[0];
+    msgs[mi++] = m2[1];
+    flock_fs_observe_f128(fs, m2[0]);
+    flock_fs_observe_f128(fs, m2[1]);
+    const uint4* cf = U(d_a);
+    const uint4* cb = U(d_b);
+    size_t len = n;
+    int pp = 0;
+    for (uint32_t i = 0; i + 1 < log_n; i++) {
+        flock_f128 r;
+        flock_fs_sample_f128(fs, &r);
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_sc_fold_msg(cf, cb, len, U(r), nf, nb, c->partials, c->dout,
+                                         c->stream));
+        if (int rc = c->fetch_pair(m2)) return rc;
+        msgs[mi++] = m2[0];
+        msgs[mi++] = m2[1];
+        flock_fs_observe_f128(fs, m2[0]);
+        flock_fs_observe_f128(fs, m2[1]);
+        cf = nf;
+        cb = nb;
+        len /= 2;
+        pp ^= 1;
+    }
+    {
+        flock_f128 r;
+        flock_fs_sample_f128(fs, &r);
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(r), nf, nb, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        finals[0] = c->hout[0];
+        finals[1] = c->hout[1];
+        flock_fs_observe_f128(fs, finals[0]);
+        flock_fs_observe_f128(fs, finals[1]);
+    }
+
+    // ---- eq-weighted zerocheck tail ----
+    flock_fs_observe_label(fs, kLabelZc, sizeof(kLabelZc) - 1);
+    std::vector<flock_f128> r(log_n);
+    flock_fs_sample_f128_vec(fs, r.data(), log_n);
+    if (log_n > 1) HIP_TRY(hipMemcpyAsync(R, r.data() + 1, (log_n - 1) * 16, hipMemcpyHostToDevice,
+                                          c->stream));
+    LAUNCH(flock::launch_build_eq(R, (int)log_n - 1, E0, EQS, c->stream));
+    LAUNCH(flock::launch_zc_round(U(d_a), U(d_b), n, E0, n >= 4 ? E1 : nullptr, c->partials,
+                                  c->dout, c->stream));
+    if (int rc = c->fetch_pair(m2)) return rc;  // also orders the pageable r copy above
+    msgs[mi++] = m2[0];  // r_next[0] = ONE (Convention A)
+    msgs[mi++] = m2[1];
+    flock_fs_observe_f128(fs, m2[0]);
+    flock_fs_observe_f128(fs, m2[1]);
+    flock_f128 rho;
+    flock_fs_sample_f128(fs, &rho);
+    cf = U(d_a);
+    cb = U(d_b);
+    len = n;
+    pp = 0;
+    int ep = 1;  // eq table for the next message lives in eqb[ep]
+    for (uint32_t i = 0; i + 1 < log_n; i++) {
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_zc_fold_round(cf, cb, len, U(rho), nf, nb, eqb[ep],
+                                           len >= 8 ? eqb[ep ^ 1] : nullptr, c->partials,
+                                           c->dout, c->stream));
+        if (int rc = c->fetch_pair(m2)) return rc;
+        msgs[mi++] = m2[0];
+        msgs[mi++] = m2[1];
+        flock_fs_observe_f128(fs, m2[0]);
+        flock_fs_observe_f128(fs, m2[1]);
+        flock_fs_sample_f128(fs, &rho);
+        cf = nf;
+        cb = nb;
+        len /= 2;
+        pp ^= 1;
+        ep ^= 1;
+    }
+    {
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(rho), nf, nb, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        finals[2] = c->hout[0];
+        finals[3] = c->hout[1];
+        flock_fs_observe_f128(fs, finals[2]);
+        flock_fs_observe_f128(fs, finals[3]);
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// GF(2^128) arithmetic for gfx950 (CDNA4), GHASH polynomial x^128 + x^7 + x^2 + x + 1.
+//
+// An element is a uint4 {x: bits 0..31, y: 32..63, z: 64..95, w: 96..127}, which is
+// byte-identical to the reference's little-endian {lo: u64, hi: u64} layout
+// (crates/flock-core/src/field/gf2_128.rs:22-27). CDNA4 has no carry-less multiply,
+// so products are built from 32-bit VALU shifts/funnel shifts and XORs:
+//
+//  * f128_mul        — generic a*b: 32 funnel-shifted copies of b, one per bit offset
+//                      s in [0,32), each masked by bit s of every word of a and
+//                      XOR-accumulated into a 256-bit product, then reduced.
+//  * ShoupTable       — a*t for a fixed t: 32 LDS subtables T_k[n] = (n * x^(4k)) * t
+//                      (16 entries of 16 B = one 256-byte row = all 64 LDS banks, so a
+//                      64-lane lookup never bank-conflicts); the product is the XOR of
+//                      32 lookups, one per nibble of a.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint4 f128;
+
+__device__ __forceinline__ f128 f128_zero() { return make_uint4(0u, 0u, 0u, 0u); }
+__device__ __forceinline__ f128 f128_one() { return make_uint4(1u, 0u, 0u, 0u); }
+
+__device__ __forceinline__ f128 f128_add(f128 a, f128 b) {
+    return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
+}
+__device__ __forceinline__ f128 f128_add3(f128 a, f128 b, f128 c) {
+    return make_uint4(a.x ^ b.x ^ c.x, a.y ^ b.y ^ c.y, a.z ^ b.z ^ c.z, a.w ^ b.w ^ c.w);
+}
+
+// (hi:lo) >> (32 - s) in 32 bits, i.e. the word of (x << s) that receives lo's top bits.
+__device__ __forceinline__ uint32_t fshl(uint32_t hi, uint32_t lo, int s) {
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(32 - s));
+}
+
+// 256-bit unreduced product accumulator (r[0] = bits 0..31).
+struct f256u {
+    uint32_t r[8];
+};
+
+__device__ __forceinline__ void f256u_zero(f256u& a) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) a.r[i] = 0u;
+}
+__device__ __forceinline__ void f256u_xor(f256u& a, const f256u& b) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) a.r[i] ^= b.r[i];
+}
+
+// acc ^= a * b (carry-less, unreduced). For every bit offset s in [0, 32) the 160-bit
+// value b << s is built with funnel shifts, and bit s of each word w of a selects whether
+// it is XORed into acc at word offset w. The s loop is kept rolled so the product needs
+// only ~30 registers; each step is 5 shifts, 4 bit-extracts and 20 and-xor (bitop3) ops.
+__device__ __forceinline__ void clmul128_acc(f256u& acc, f128 a, f128 b) {
+    // s = 0: b itself
+    {
+        const uint32_t m0 = 0u - (a.x & 1u), m1 = 0u - (a.y & 1u), m2 = 0u - (a.z & 1u),
+                       m3 = 0u - (a.w & 1u);
+        acc.r[0] ^= b.x & m0;
+        acc.r[1] ^= (b.y & m0) ^ (b.x & m1);
+        acc.r[2] ^= (b.z & m0) ^ (b.y & m1) ^ (b.x & m2);
+        acc.r[3] ^= (b.w & m0) ^ (b.z & m1) ^ (b.y & m2) ^ (b.x & m3);
+        acc.r[4] ^= (b.w & m1) ^ (b.z & m2) ^ (b.y & m3);
+        acc.r[5] ^= (b.w & m2) ^ (b.z & m3);
+        acc.r[6] ^= b.w & m3;
+    }
+    uint32_t r0 = acc.r[0], r1 = acc.r[1], r2 = acc.r[2], r3 = acc.r[3], r4 = acc.r[4],
+             r5 = acc.r[5], r6 = acc.r[6], r7 = acc.r[7];
+#pragma unroll 1
+    for (int s = 1; s < 32; s++) {
+        const uint32_t s0 = b.x << s;
+        const uint32_t s1 = fshl(b.y, b.x, s);
+        const uint32_t s2 = fshl(b.z, b.y, s);
+        const uint32_t s3 = fshl(b.w, b.z, s);
+        const uint32_t s4 = b.w >> (32 - s);
+        const uint32_t m0 = (uint32_t)(((int32_t)(a.x << (31 - s))) >> 31);
+        const uint32_t m1 = (uint32_t)(((int32_t)(a.y << (31 - s))) >> 31);
+        const uint32_t m2 = (uint32_t)(((int32_t)(a.z << (31 - s))) >> 31);
+        const uint32_t m3 = (uint32_t)(((int32_t)(a.w << (31 - s))) >> 31);
+        r0 ^= s0 & m0;
+        r1 ^= (s1 & m0) ^ (s0 & m1);
+        r2 ^= (s2 & m0) ^ (s1 & m1) ^ (s0 & m2);
+        r3 ^= (s3 & m0) ^ (s2 & m1) ^ (s1 & m2) ^ (s0 & m3);
+        r4 ^= (s4 & m0) ^ (s3 & m1) ^ (s2 & m2) ^ (s1 & m3);
+        r5 ^= (s4 & m1) ^ (s3 & m2) ^ (s2 & m3);
+        r6 ^= (s4 & m2) ^ (s3 & m3);
+        r7 ^= s4 & m3;
+    }
+    acc.r[0] = r0; acc.r[1] = r1; acc.r[2] = r2; acc.r[3] = r3;
+    acc.r[4] = r4; acc.r[5] = r5; acc.r[6] = r6; acc.r[7] = r7;
+}
+
+// Fold the top 128 bits into the bottom 128 mod x^128 + x^7 + x^2 + x + 1
+// (the same identity as ghash_reduce, crates/flock-core/src/field/gf2_128.rs:278-297):
+// U * x^128 = U + U<<1 + U<<2 + U<<7, and the <=7 bits pushed past x^127 fold again.
+__device__ __forceinline__ f128 f256u_reduce(const f256u& a) {
+    const uint32_t u0 = a.r[4], u1 = a.r[5], u2 = a.r[6], u3 = a.r[7];
+    uint32_t t0 = u0 ^ (u0 << 1) ^ (u0 << 2) ^ (u0 << 7);
+    uint32_t t1 = u1 ^ fshl(u1, u0, 1) ^ fshl(u1, u0, 2) ^ fshl(u1, u0, 7);
+    uint32_t t2 = u2 ^ fshl(u2, u1, 1) ^ fshl(u2, u1, 2) ^ fshl(u2, u1, 7);
+    uint32_t t3 = u3 ^ fshl(u3, u2, 1) ^ fshl(u3, u2, 2) ^ fshl(u3, u2, 7);
+    const uint32_t ov = (u3 >> 31) ^ (u3 >> 30) ^ (u3 >> 25);
+    const uint32_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+    return make_uint4(a.r[0] ^ t0 ^ corr, a.r[1] ^ t1, a.r[2] ^ t2, a.r[3] ^ t3);
+}
+
+__device__ __forceinline__ f128 f128_mul(f128 a, f128 b) {
+    f256u acc;
+    f256u_zero(acc);
+    clmul128_acc(acc, a, b);
+    return f256u_reduce(acc);
+}
+
+// a * x mod p (gf2_128.rs:213-220).
+__device__ __forceinline__ f128 f128_mul_x(f128 a) {
+    const uint32_t m = (uint32_t)((int32_t)a.w >> 31);
+    return make_uint4((a.x << 1) ^ (0x87u & m), fshl(a.y, a.x, 1), fshl(a.z, a.y, 1),
+                      fshl(a.w, a.z, 1));
+}
+
+// a * x^s mod p for s in [0, 128): shift, then reduce the spilled bits.
+__device__ __forceinline__ f128 f128_shl_mod(f128 a, int s) {
+    f256u t;
+    f256u_zero(t);
+    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+    const int q = s >> 5, b = s & 31;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        // word i of a lands in words i+q (low part) and i+q+1 (high part)
+        const uint32_t lo = b ? (w[i] << b) : w[i];
+        const uint32_t hi = b ? (w[i] >> (32 - b)) : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (j == i + q) t.r[j] ^= lo;
+            if (j == i + q + 1) t.r[j] ^= hi;
+        }
+    }
+    return f256u_reduce(t);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-multiplier tables (Shoup, 4-bit windows) in LDS.
+// Layout: table[k * 16 + n] = (n(x) * x^(4k)) * t, k in [0,32), n in [0,16). 8 KiB.
+// ---------------------------------------------------------------------------
+constexpr int SHOUP_ENTRIES = 512;
+
+// Build subtable k (16 entries) of the table for multiplier t.
+__device__ __forceinline__ void shoup_build_row(f128* table, f128 t, int k) {
+    const f128 e1 = f128_shl_mod(t, 4 * k);
+    const f128 e2 = f128_mul_x(e1);
+    const f128 e4 = f128_mul_x(e2);
+    const f128 e8 = f128_mul_x(e4);
+    f128* row = table + 16 * k;
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+        f128 v = f128_zero();
+        if (n & 1) v = f128_add(v, e1);
+        if (n & 2) v = f128_add(v, e2);
+        if (n & 4) v = f128_add(v, e4);
+        if (n & 8) v = f128_add(v, e8);
+        row[n] = v;
+    }
+}
+
+__device__ __forceinline__ f128 shoup_mul(const f128* __restrict__ table, f128 a) {
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
+    f128 acc = f128_zero();
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t n = (aw[w] >> (4 * j)) & 15u;
+            acc = f128_add(acc, table[(w * 8 + j) * 16 + n]);
+        }
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block XOR reductions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v ^= __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ f128 wave_xor_f128(f128 v) {
+    return make_uint4(wave_xor(v.x), wave_xor(v.y), wave_xor(v.z), wave_xor(v.w));
+}

@@ -1,0 +1,61 @@
+// Internal interface between the host runtime (api.hip) and the kernels
+// (field.hip, ntt.hip, merkle.hip, sumcheck.hip). Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace flock {
+
+// Two F128 per block partial (e.g. u0, u2) — 32 bytes.
+constexpr int kPartialWords = 2;
+constexpr int kMaxReduceBlocks = 2048;
+
+// ---- field.hip ----
+hipError_t launch_f128_mul_batch(const uint4* a, const uint4* b, uint4* out, size_t n,
+                                 hipStream_t s);
+hipError_t launch_f128_mul_const_batch(const uint4* a, uint4 t, uint4* out, size_t n,
+                                       hipStream_t s);
+hipError_t launch_xor_reduce(const uint4* partials, int nblocks, uint4* out, hipStream_t s);
+
+// ---- ntt.hip ----
+// Lane-major replicate fill: cw[(rep*D + p)*t + l] = q[l*D + p] for l < t.
+hipError_t launch_fill_lane_major(const uint4* q, uint4* cw, uint64_t d, uint32_t t,
+                                  uint32_t reps, hipStream_t s);
+// Interleaved replicate fill: cw[i] = msg[i % msg_len].
+hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, uint32_t reps,
+                                 hipStream_t s);
+// Forward interleaved LCH NTT over layers [start_layer, k) of 2^k positions x t lanes,
+// transforming only lanes < live. evals: k x k basis table (device).
+hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
+                              const uint4* evals, hipStream_t s);
+
+// ---- merkle.hip ----
+// SHA-256 Merkle tree, flat layout (leaves first, root last). leaf_bytes % 16 == 0.
+hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
+                              uint8_t* tree, hipStream_t s);
+
+// ---- sumcheck.hip ----
+// round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2.
+hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
+                               uint4* out, hipStream_t s);
+// fold_and_msg_lsb: nf/nb (n/2) = v0 + r(v0+v1); out = next message (zero if n/2 == 1).
+hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
+                              uint4* nb, uint4* partials, uint4* out, hipStream_t s);
+// Zerocheck first message on (a, b) of size n with eq table e (n/2): out = (G(1), G(inf)).
+// If e_next != nullptr, e_next[z] = e[2z] ^ e[2z+1] (n/4 entries).
+hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
+                           uint4* e_next, uint4* partials, uint4* out, hipStream_t s);
+// Fused: fold (a, b) of size n at rho into (ao, bo) of size n/2, then the message on the
+// folded tables with eq table e (n/4), and e_next (n/8) as above.
+hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
+                                uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
+                                uint4* partials, uint4* out, hipStream_t s);
+// Plain fold (fold_in_place_pair): out-of-place, n -> n/2.
+hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
+                            uint4* bo, hipStream_t s);
+// eq table build_eq(r[0..nv)) into out (2^nv entries). r is a device array; scratch holds
+// at least 2^(nv/2) + 2^(nv - nv/2) entries.
+hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s);
+
+}  // namespace flock

@@ -1,0 +1,224 @@
+// SHA-256 Merkle tree over codeword rows (crates/flock-core/src/merkle.rs).
+//
+// Flat layout (merkle.rs:15-22): tree[0..N) = leaf hashes, then each level above, the
+// root at tree[2N-2]. hash_leaf = SHA-256(leaf bytes) (:205-216), hash_pair =
+// SHA-256(left || right) (:220-232), no domain separation. A leaf is one codeword
+// position: t lanes x 16 B, little-endian {lo, hi} words (pcs/commit.rs:601-614).
+//
+// Kernels: k_merkle_leaves hashes one leaf per thread (SHA-256 is sequential within a
+// message) and then builds the workgroup's subtree in LDS, so a 256-thread group emits
+// its 256 leaves and the 8 levels above them in one launch. k_merkle_reduce does the same
+// for an internal level (2 children per thread). Digests are kept as the eight
+// big-endian state words, which are exactly the message words of the parent's block.
+#include "f128_device.h"
+#include "flock_internal.h"
+
+namespace flock {
+
+__constant__ uint32_t kSha256K[64] = {
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u,
+    0xab1c5ed5u, 0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu,
+    0x9bdc06a7u, 0xc19bf174u, 0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu,
+    0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau, 0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u,
+    0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u, 0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu,
+    0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u, 0xa2bfe8a1u, 0xa81a664bu,
+    0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u, 0x19a4c116u,
+    0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+    0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u,
+    0xc67178f2u};
+
+__device__ __forceinline__ uint32_t rotr(uint32_t x, int n) {
+    return __builtin_amdgcn_alignbit(x, x, (uint32_t)n);
+}
+__device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
+
+__device__ __forceinline__ void sha256_iv(uint32_t h[8]) {
+    h[0] = 0x6a09e667u; h[1] = 0xbb67ae85u; h[2] = 0x3c6ef372u; h[3] = 0xa54ff53au;
+    h[4] = 0x510e527fu; h[5] = 0x9b05688cu; h[6] = 0x1f83d9abu; h[7] = 0x5be0cd19u;
+}
+
+// FIPS 180-4 compression of one 16-word big-endian block.
+__device__ __forceinline__ void sha256_compress(uint32_t h[8], uint32_t w[16]) {
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+#pragma unroll
+    for (int i = 0; i < 64; i++) {
+        uint32_t wi;
+        if (i < 16) {
+            wi = w[i];
+        } else {
+            const uint32_t w15 = w[(i + 1) & 15], w2 = w[(i + 14) & 15];
+            const uint32_t s0 = rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3);
+            const uint32_t s1 = rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10);
+            wi = w[i & 15] + s0 + w[(i + 9) & 15] + s1;
+            w[i & 15] = wi;
+        }
+        const uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
+        const uint32_t ch = (e & f) ^ (~e & g);
+        const uint32_t t1 = hh + S1 + ch + kSha256K[i] + wi;
+        const uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
+        const uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+        const uint32_t t2 = S0 + mj;
+        hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
+    }
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
+// SHA-256 of a 64-byte message given as 16 big-endian words (a parent node).
+__device__ __forceinline__ void sha256_64(const uint32_t m[16], uint32_t out[8]) {
+    sha256_iv(out);
+    uint32_t w[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) w[i] = m[i];
+    sha256_compress(out, w);
+    uint32_t p[16];
+    p[0] = 0x80000000u;
+#pragma unroll
+    for (int i = 1; i < 15; i++) p[i] = 0u;
+    p[15] = 512u;
+    sha256_compress(out, p);
+}
+
+__device__ __forceinline__ void store_digest(uint8_t* tree, uint64_t node, const uint32_t h[8]) {
+    uint4* dst = reinterpret_cast<uint4*>(tree + node * 32);
+    dst[0] = make_uint4(bswap32(h[0]), bswap32(h[1]), bswap32(h[2]), bswap32(h[3]));
+    dst[1] = make_uint4(bswap32(h[4]), bswap32(h[5]), bswap32(h[6]), bswap32(h[7]));
+}
+
+// Level offset in the flat tree: the level holding L nodes starts at 2N - 2L.
+__device__ __forceinline__ uint64_t level_start(uint64_t n_leaves, uint64_t level_nodes) {
+    return 2 * n_leaves - 2 * level_nodes;
+}
+
+// Build the subtree above `width` digests held in lds[0..width) (this group's slice of a
+// level with `level_nodes` nodes), writing every produced node into the tree.
+__device__ void subtree_up(uint32_t (*lds)[8], int width, uint64_t group, uint64_t level_nodes,
+                           uint64_t n_leaves, uint8_t* tree) {
+    int w = width;
+    uint64_t ln = level_nodes;
+    while (w > 1) {
+        w >>= 1;
+        ln >>= 1;
+        uint32_t out[8];
+        const int i = threadIdx.x;
+        if (i < w) {
+            uint32_t m[16];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                m[j] = lds[2 * i][j];
+                m[8 + j] = lds[2 * i + 1][j];
+            }
+            sha256_64(m, out);
+        }
+        __syncthreads();
+        if (i < w) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) lds[i][j] = out[j];
+            store_digest(tree, level_start(n_leaves, ln) + group * w + i, out);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_merkle_leaves(const uint8_t* __restrict__ data,
+                                                        uint64_t n_leaves, uint64_t leaf_bytes,
+                                                        uint8_t* __restrict__ tree) {
+    __shared__ uint32_t lds[256][8];
+    const int width = blockDim.x;
+    const uint64_t leaf = (uint64_t)blockIdx.x * width + threadIdx.x;
+    const uint4* src = reinterpret_cast<const uint4*>(data + leaf * leaf_bytes);
+    uint32_t h[8];
+    sha256_iv(h);
+    const uint64_t full = leaf_bytes / 64;
+    for (uint64_t blk = 0; blk < full; blk++) {
+        uint32_t w[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = src[blk * 4 + q];
+            w[4 * q + 0] = bswap32(v.x);
+            w[4 * q + 1] = bswap32(v.y);
+            w[4 * q + 2] = bswap32(v.z);
+            w[4 * q + 3] = bswap32(v.w);
+        }
+        sha256_compress(h, w);
+    }
+    // final block: the remaining 0/16/32/48 bytes, 0x80, zeros, 64-bit bit length
+    {
+        const int rem_q = (int)((leaf_bytes % 64) / 16);
+        uint32_t w[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) w[i] = 0u;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (q < rem_q) {
+                const uint4 v = src[full * 4 + q];
+                w[4 * q + 0] = bswap32(v.x);
+                w[4 * q + 1] = bswap32(v.y);
+                w[4 * q + 2] = bswap32(v.z);
+                w[4 * q + 3] = bswap32(v.w);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (q == rem_q) w[4 * q] = 0x80000000u;
+        const uint64_t bits = leaf_bytes * 8;
+        w[14] = (uint32_t)(bits >> 32);
+        w[15] = (uint32_t)bits;
+        sha256_compress(h, w);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) lds[threadIdx.x][j] = h[j];
+    store_digest(tree, leaf, h);
+    __syncthreads();
+    subtree_up(lds, width, blockIdx.x, n_leaves, n_leaves, tree);
+}
+
+// Internal level with `level_nodes` nodes starting at tree node `in_start`; each group
+// takes 2*blockDim.x children.
+__global__ __launch_bounds__(256) void k_merkle_reduce(uint8_t* __restrict__ tree,
+                                                        uint64_t n_leaves, uint64_t level_nodes) {
+    __shared__ uint32_t lds[256][8];
+    const int width = blockDim.x;  // parents produced by this group at the first step
+    const uint64_t in_start = level_start(n_leaves, level_nodes);
+    const uint64_t parent = (uint64_t)blockIdx.x * width + threadIdx.x;
+    const uint4* src = reinterpret_cast<const uint4*>(tree + (in_start + 2 * parent) * 32);
+    uint32_t m[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = src[q];
+        m[4 * q + 0] = bswap32(v.x);
+        m[4 * q + 1] = bswap32(v.y);
+        m[4 * q + 2] = bswap32(v.z);
+        m[4 * q + 3] = bswap32(v.w);
+    }
+    uint32_t h[8];
+    sha256_64(m, h);
+    const uint64_t parents = level_nodes >> 1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) lds[threadIdx.x][j] = h[j];
+    store_digest(tree, level_start(n_leaves, parents) + parent, h);
+    __syncthreads();
+    subtree_up(lds, width, blockIdx.x, parents, n_leaves, tree);
+}
+
+hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
+                              uint8_t* tree, hipStream_t s) {
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1)) || leaf_bytes % 16)
+        return hipErrorInvalidValue;
+    const uint64_t width = n_leaves < 256 ? n_leaves : 256;
+    k_merkle_leaves<<<(unsigned)(n_leaves / width), (unsigned)width, 0, s>>>(data, n_leaves,
+                                                                            leaf_bytes, tree);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    uint64_t level = n_leaves / width;  // nodes at the current top
+    while (level > 1) {
+        const uint64_t parents = level / 2;
+        const uint64_t w = parents < 256 ? parents : 256;
+        k_merkle_reduce<<<(unsigned)(parents / w), (unsigned)w, 0, s>>>(tree, n_leaves, level);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        level = parents / w;
+    }
+    return hipSuccess;
+}
+
+}  // namespace flock

@@ -1,0 +1,182 @@
+// Interleaved LCH additive NTT (neighbors-last) and the commit's replicate fills.
+//
+// Layout (crates/flock-core/src/ntt/additive_ntt_f128.rs:168-186): position-major SoA,
+// data[pos * t + lane], 2^k positions x t lanes, F128 = 16 B. Butterfly at layer l,
+// block b (additive_ntt_f128.rs:303-316):  u += v * tw(l, b);  v += u,  pairs
+// (pos, pos + 2^(k-l-1)). tw(l, b) = XOR of evals[k-l-1][1 + j] over the set bits j of
+// b (:141-144).
+//
+// One pass fuses K consecutive layers [l0, l0+K): a "group" is the 2^K positions
+// hb*2^(k-l0) + i*S + lowp (S = 2^(k-l0-K), i < 2^K). All groups with the same high
+// block hb share the same 2^K - 1 twiddles, so a workgroup serves one hb: it derives
+// those twiddles, builds one 8 KiB Shoup table per twiddle in LDS, and then each thread
+// owns one column c = lowp*t + lane, keeps its 2^K elements in registers and runs the K
+// butterfly layers there. Columns are contiguous in memory, so every load/store of a
+// wave is one coalesced 1 KiB segment.
+#include "f128_device.h"
+#include "flock_internal.h"
+
+namespace flock {
+
+// ------------------------------------------------------------------ fills --
+
+// cw[(rep*D + p)*t + l] = q[l*D + p]  (replicate_lane_major_fill, pcs/commit.rs:516-536)
+__global__ __launch_bounds__(256) void k_fill_lane_major(const f128* __restrict__ q,
+                                                          f128* __restrict__ cw, uint64_t d,
+                                                          uint32_t t, uint32_t reps) {
+    __shared__ f128 tile[32][33];
+    const uint64_t p0 = (uint64_t)blockIdx.x * 32;
+    const uint32_t l0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int ly = 0; ly < 32; ly += 8) {
+        const uint32_t l = l0 + ly + ty;
+        const uint64_t p = p0 + tx;
+        if (l < t && p < d) tile[ly + ty][tx] = q[(uint64_t)l * d + p];
+    }
+    __syncthreads();
+    const uint64_t msg_len = d * t;
+#pragma unroll
+    for (int py = 0; py < 32; py += 8) {
+        const uint64_t p = p0 + py + ty;
+        const uint32_t l = l0 + tx;
+        if (l < t && p < d) {
+            const f128 v = tile[tx][py + ty];
+            for (uint32_t rep = 0; rep < reps; rep++) cw[rep * msg_len + p * t + l] = v;
+        }
+    }
+}
+
+// cw[i] = msg[i % msg_len]  (replicate_message_fill, pcs/commit.rs:543-568)
+__global__ __launch_bounds__(256) void k_fill_replicate(const f128* __restrict__ msg,
+                                                         f128* __restrict__ cw, uint64_t msg_len,
+                                                         uint32_t reps) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < msg_len;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 v = msg[i];
+        for (uint32_t rep = 0; rep < reps; rep++) cw[rep * msg_len + i] = v;
+    }
+}
+
+hipError_t launch_fill_lane_major(const uint4* q, uint4* cw, uint64_t d, uint32_t t,
+                                  uint32_t reps, hipStream_t s) {
+    dim3 grid((unsigned)((d + 31) / 32), (t + 31) / 32);
+    k_fill_lane_major<<<grid, 256, 0, s>>>(q, cw, d, t, reps);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, uint32_t reps,
+                                 hipStream_t s) {
+    uint64_t g = (msg_len + 255) / 256;
+    if (g > 8192) g = 8192;
+    k_fill_replicate<<<(unsigned)g, 256, 0, s>>>(msg, cw, msg_len, reps);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- NTT --
+
+template <int K>
+__global__ __launch_bounds__(256) void k_ntt_pass(f128* __restrict__ data, uint32_t t,
+                                                   uint32_t live, int k, int l0,
+                                                   const f128* __restrict__ evals,
+                                                   uint64_t cols_per_hb, uint32_t wgs_per_hb) {
+    constexpr int NT = (1 << K) - 1;  // twiddles (and Shoup tables) per group
+    extern __shared__ f128 tables[];   // NT * SHOUP_ENTRIES
+    __shared__ f128 tw[NT];
+
+    const uint64_t hb = blockIdx.x / wgs_per_hb;
+    const uint32_t chunk = blockIdx.x - (uint32_t)(hb * wgs_per_hb);
+
+    // Twiddle index tix = 2^j - 1 + u serves layer l0 + j, block (hb << j) | u.
+    if (threadIdx.x < NT) {
+        const int tix = threadIdx.x;
+        const int j = 31 - __clz(tix + 1);
+        const uint64_t u = (uint64_t)(tix + 1 - (1 << j));
+        const int layer = l0 + j;
+        const uint64_t block = (hb << j) | u;
+        const f128* row = evals + (size_t)(k - layer - 1) * k + 1;
+        f128 acc = f128_zero();
+        for (int bit = 0; bit < layer; bit++)
+            if ((block >> bit) & 1) acc = f128_add(acc, row[bit]);
+        tw[tix] = acc;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < NT * 32; r += blockDim.x)
+        shoup_build_row(tables + (r >> 5) * SHOUP_ENTRIES, tw[r >> 5], r & 31);
+    __syncthreads();
+
+    const uint64_t S = 1ull << (k - l0 - K);
+    const uint64_t stride = S * t;                      // elements between group members
+    const uint64_t base = (hb << (k - l0)) * (uint64_t)t;  // first element of the group
+    for (uint64_t c = (uint64_t)chunk * blockDim.x + threadIdx.x; c < cols_per_hb;
+         c += (uint64_t)wgs_per_hb * blockDim.x) {
+        const uint32_t lane = (uint32_t)(c % t);
+        if (lane >= live) continue;  // dead lanes are zero and stay zero
+        f128 e[1 << K];
+#pragma unroll
+        for (int i = 0; i < (1 << K); i++) e[i] = data[base + i * stride + c];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int h = 1 << (K - 1 - j);
+#pragma unroll
+            for (int i = 0; i < (1 << K); i++) {
+                if (i & h) continue;
+                const int tix = (1 << j) - 1 + (i >> (K - j));
+                e[i] = f128_add(e[i], shoup_mul(tables + tix * SHOUP_ENTRIES, e[i | h]));
+                e[i | h] = f128_add(e[i | h], e[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < (1 << K); i++) data[base + i * stride + c] = e[i];
+    }
+}
+
+template <int K>
+static hipError_t launch_pass(f128* data, uint32_t t, uint32_t live, int k, int l0,
+                              const f128* evals, hipStream_t s) {
+    constexpr int NT = (1 << K) - 1;
+    const size_t lds = (size_t)NT * SHOUP_ENTRIES * sizeof(f128);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_ntt_pass<K>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const uint64_t n_hb = 1ull << l0;
+    const uint64_t cols = (1ull << (k - l0 - K)) * t;
+    // up to 4 columns per thread so the twiddle-table build is amortised
+    uint64_t wgs = (cols + 1023) / 1024;
+    if (wgs < 1) wgs = 1;
+    const uint64_t grid = n_hb * wgs;
+    if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    k_ntt_pass<K><<<(unsigned)grid, 256, lds, s>>>(data, t, live, k, l0, evals, cols,
+                                                   (uint32_t)wgs);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
+                              const uint4* evals, hipStream_t s) {
+    int l = start_layer;
+    while (l < k) {
+        const int rem = k - l;
+        hipError_t e;
+        if (rem >= 4) {
+            e = launch_pass<4>(data, t, live, k, l, evals, s);
+            l += 4;
+        } else if (rem == 3) {
+            e = launch_pass<3>(data, t, live, k, l, evals, s);
+            l += 3;
+        } else if (rem == 2) {
+            e = launch_pass<2>(data, t, live, k, l, evals, s);
+            l += 2;
+        } else {
+            e = launch_pass<1>(data, t, live, k, l, evals, s);
+            l += 1;
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace flock

@@ -1,0 +1,258 @@
+// Sumcheck round kernels: the product sumcheck of the Ligerito opening and the
+// eq-weighted zerocheck tail.
+//
+// Product sumcheck (crates/flock-core/src/pcs/ligerito.rs):
+//   round_msg_lsb  (:3204-3240)  u0 = sum f0*b0, u2 = sum (f0+f1)(b0+b1)
+//   fold_and_msg_lsb (:3372-3414) v' = v0 + r(v0+v1), then the next message on v'
+// Zerocheck tail (crates/flock-core/src/zerocheck/multilinear.rs):
+//   round_pair_naive (:331-355)  (G(1), G(inf)) = (sum eq*a1*b1, sum eq*(a0+a1)(b0+b1))
+//   fold_in_place_pair (:1148-1163)
+//
+// Sums are exact in GF(2^128) and order-free, so every kernel accumulates
+// unreduced 256-bit products per thread (one reduction per thread), XOR-reduces across
+// the wave with shuffles and across the block in LDS, and writes one partial pair per
+// block; k_xor_reduce folds the partials. Fixed multipliers (the fold challenge) go
+// through an LDS Shoup table; products of two varying values use f128_mul.
+//
+// The eq table of round i+1 is the pairwise XOR of round i's: eq(r', (0,x)) +
+// eq(r', (1,x)) = ((1+r0) + r0) eq(r'', x) = eq(r'', x), so each message kernel also
+// emits the next round's table (n/4 -> n/8 entries) instead of rebuilding it.
+#include "f128_device.h"
+#include "flock_internal.h"
+
+namespace flock {
+
+__device__ __forceinline__ void block_xor_pair(f128 a0, f128 a1, f128* partials) {
+    __shared__ f128 red[2][16];
+    a0 = wave_xor_f128(a0);
+    a1 = wave_xor_f128(a1);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wid] = a0;
+        red[1][wid] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        f128 r0 = f128_zero(), r1 = f128_zero();
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; w++) {
+            r0 = f128_add(r0, red[0][w]);
+            r1 = f128_add(r1, red[1][w]);
+        }
+        partials[2 * blockIdx.x] = r0;
+        partials[2 * blockIdx.x + 1] = r1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f,
+                                                       const f128* __restrict__ b, uint64_t n,
+                                                       f128* __restrict__ partials) {
+    f256u u0, u2;
+    f256u_zero(u0);
+    f256u_zero(u2);
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n / 2;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
+        clmul128_acc(u0, f0, b0);
+        clmul128_acc(u2, f128_add(f0, f1), f128_add(b0, b1));
+    }
+    block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
+}
+
+__global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
+                                                      const f128* __restrict__ b, uint64_t n,
+                                                      f128 r, f128* __restrict__ nf,
+                                                      f128* __restrict__ nb,
+                                                      f128* __restrict__ partials) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    if (threadIdx.x < 32) shoup_build_row(table, r, threadIdx.x);
+    __syncthreads();
+    const uint64_t half = n / 2;
+    f256u u0, u2;
+    f256u_zero(u0);
+    f256u_zero(u2);
+    if (half == 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            nf[0] = f128_add(f[0], shoup_mul(table, f128_add(f[0], f[1])));
+            nb[0] = f128_add(b[0], shoup_mul(table, f128_add(b[0], b[1])));
+        }
+    } else {
+        for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < half / 2;
+             k += (uint64_t)gridDim.x * blockDim.x) {
+            const f128 f0 = f[4 * k], f1 = f[4 * k + 1], f2 = f[4 * k + 2], f3 = f[4 * k + 3];
+            const f128 b0 = b[4 * k], b1 = b[4 * k + 1], b2 = b[4 * k + 2], b3 = b[4 * k + 3];
+            const f128 g0 = f128_add(f0, shoup_mul(table, f128_add(f0, f1)));
+            const f128 g1 = f128_add(f2, shoup_mul(table, f128_add(f2, f3)));
+            const f128 c0 = f128_add(b0, shoup_mul(table, f128_add(b0, b1)));
+            const f128 c1 = f128_add(b2, shoup_mul(table, f128_add(b2, b3)));
+            nf[2 * k] = g0;
+            nf[2 * k + 1] = g1;
+            nb[2 * k] = c0;
+            nb[2 * k + 1] = c1;
+            clmul128_acc(u0, g0, c0);
+            clmul128_acc(u2, f128_add(g0, g1), f128_add(c0, c1));
+        }
+    }
+    block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
+}
+
+__global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
+                                                   const f128* __restrict__ b, uint64_t n,
+                                                   const f128* __restrict__ e,
+                                                   f128* __restrict__ e_next,
+                                                   f128* __restrict__ partials) {
+    f256u g1, gi;
+    f256u_zero(g1);
+    f256u_zero(gi);
+    const uint64_t half = n / 2;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < half;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+        const f128 ex = e[x];
+        clmul128_acc(g1, ex, f128_mul(a1, b1));
+        clmul128_acc(gi, ex, f128_mul(f128_add(a0, a1), f128_add(b0, b1)));
+        if (e_next && !(x & 1) && x + 1 < half) e_next[x >> 1] = f128_add(ex, e[x + 1]);
+    }
+    block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
+}
+
+__global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ a,
+                                                        const f128* __restrict__ b, uint64_t n,
+                                                        f128 rho, f128* __restrict__ ao,
+                                                        f128* __restrict__ bo,
+                                                        const f128* __restrict__ e,
+                                                        f128* __restrict__ e_next,
+                                                        f128* __restrict__ partials) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    if (threadIdx.x < 32) shoup_build_row(table, rho, threadIdx.x);
+    __syncthreads();
+    f256u g1, gi;
+    f256u_zero(g1);
+    f256u_zero(gi);
+    const uint64_t quarter = n / 4;
+    for (uint64_t y = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; y < quarter;
+         y += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 a0 = a[4 * y], a1 = a[4 * y + 1], a2 = a[4 * y + 2], a3 = a[4 * y + 3];
+        const f128 b0 = b[4 * y], b1 = b[4 * y + 1], b2 = b[4 * y + 2], b3 = b[4 * y + 3];
+        const f128 p0 = f128_add(a0, shoup_mul(table, f128_add(a0, a1)));
+        const f128 p1 = f128_add(a2, shoup_mul(table, f128_add(a2, a3)));
+        const f128 q0 = f128_add(b0, shoup_mul(table, f128_add(b0, b1)));
+        const f128 q1 = f128_add(b2, shoup_mul(table, f128_add(b2, b3)));
+        ao[2 * y] = p0;
+        ao[2 * y + 1] = p1;
+        bo[2 * y] = q0;
+        bo[2 * y + 1] = q1;
+        const f128 ey = e[y];
+        clmul128_acc(g1, ey, f128_mul(p1, q1));
+        clmul128_acc(gi, ey, f128_mul(f128_add(p0, p1), f128_add(q0, q1)));
+        if (e_next && !(y & 1) && y + 1 < quarter) e_next[y >> 1] = f128_add(ey, e[y + 1]);
+    }
+    block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
+}
+
+__global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
+                                                    const f128* __restrict__ b, uint64_t n, f128 c,
+                                                    f128* __restrict__ ao, f128* __restrict__ bo) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    if (threadIdx.x < 32) shoup_build_row(table, c, threadIdx.x);
+    __syncthreads();
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n / 2;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+        ao[x] = f128_add(a0, shoup_mul(table, f128_add(a0, a1)));
+        bo[x] = f128_add(b0, shoup_mul(table, f128_add(b0, b1)));
+    }
+}
+
+// eq(r[0..nv), x) for every x, directly as a product of nv factors.
+__global__ __launch_bounds__(256) void k_eq_direct(const f128* __restrict__ r, int nv,
+                                                    f128* __restrict__ out) {
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= (1ull << nv)) return;
+    f128 acc = f128_one();
+    for (int i = 0; i < nv; i++) {
+        f128 fac = r[i];
+        if (!((x >> i) & 1)) fac.x ^= 1u;  // 1 + r_i
+        acc = (i == 0) ? fac : f128_mul(acc, fac);
+    }
+    out[x] = acc;
+}
+
+// out[x_lo + 2^nlo * x_hi] = lo[x_lo] * hi[x_hi]; one group per x_hi (Shoup table of hi[x_hi]).
+__global__ __launch_bounds__(256) void k_eq_outer(const f128* __restrict__ lo, int nlo,
+                                                   const f128* __restrict__ hi,
+                                                   f128* __restrict__ out) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    const uint64_t xh = blockIdx.x;
+    if (threadIdx.x < 32) shoup_build_row(table, hi[xh], threadIdx.x);
+    __syncthreads();
+    const uint64_t nl = 1ull << nlo;
+    for (uint64_t xl = threadIdx.x; xl < nl; xl += blockDim.x)
+        out[(xh << nlo) + xl] = shoup_mul(table, lo[xl]);
+}
+
+static inline unsigned red_grid(uint64_t work) {
+    uint64_t g = (work + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > (uint64_t)kMaxReduceBlocks) g = kMaxReduceBlocks;
+    return (unsigned)g;
+}
+
+hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
+                               uint4* out, hipStream_t s) {
+    const unsigned g = red_grid(n / 2);
+    k_sc_round_msg<<<g, 256, 0, s>>>(f, b, n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_xor_reduce(partials, (int)g, out, s);
+}
+
+hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
+                              uint4* nb, uint4* partials, uint4* out, hipStream_t s) {
+    const unsigned g = red_grid(n / 4);
+    k_sc_fold_msg<<<g, 256, 0, s>>>(f, b, n, r, nf, nb, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_xor_reduce(partials, (int)g, out, s);
+}
+
+hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
+                           uint4* e_next, uint4* partials, uint4* out, hipStream_t s) {
+    const unsigned g = red_grid(n / 2);
+    k_zc_round<<<g, 256, 0, s>>>(a, b, n, e, e_next, partials);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return launch_xor_reduce(partials, (int)g, out, s);
+}
+
+hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
+                                uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
+                                uint4* partials, uint4* out, hipStream_t s) {
+    const unsigned g = red_grid(n / 4);
+    k_zc_fold_round<<<g, 256, 0, s>>>(a, b, n, rho, ao, bo, e, e_next, partials);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return launch_xor_reduce(partials, (int)g, out, s);
+}
+
+hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
+                            uint4* bo, hipStream_t s) {
+    uint64_t g = (n / 2 + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    k_fold_pair<<<(unsigned)g, 256, 0, s>>>(a, b, n, c, ao, bo);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s) {
+    // k_eq_direct with zero variables writes the single entry 1.
+    const int nlo = nv / 2, nhi = nv - nlo;
+    uint4* lo = scratch;
+    uint4* hi = scratch + (1ull << nlo);
+    k_eq_direct<<<(unsigned)(((1ull << nlo) + 255) / 256), 256, 0, s>>>(r, nlo, lo);
+    k_eq_direct<<<(unsigned)(((1ull << nhi) + 255) / 256), 256, 0, s>>>(r + nlo, nhi, hi);
+    k_eq_outer<<<(unsigned)(1ull << nhi), 256, 0, s>>>(lo, nlo, hi, out);
+    return hipGetLastError();
+}
+
+}  // namespace flock

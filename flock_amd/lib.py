@@ -1,0 +1,351 @@
+"""Python binding of libflock_hip.so (the C ABI in include/flock_hip.h).
+
+Mirrors the reference's Rust seams for the `prove_fast` data-parallel core:
+`pcs::commit_lane_major` / `pcs::commit` (crates/flock-core/src/pcs/commit.rs:345, 474),
+`merkle::merkle_tree` (merkle.rs:543), `round_msg_lsb` / `fold_and_msg_lsb`
+(pcs/ligerito.rs:3204, 3372), `round_pair_naive` / `fold_and_compute_round_pair_into` /
+`fold_in_place_pair` (zerocheck/multilinear.rs:331, 1197, 1148), `build_eq`
+(zerocheck/univariate_skip.rs:31) and `FsChallenger` (challenger.rs:609).
+
+There is no CPU fallback: importing this module without the built library raises.
+Field elements cross the boundary as numpy uint64 arrays of shape (n, 2) = (lo, hi),
+the same bytes as the Rust F128.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflock_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "flock_hip.h")
+
+OK = 0
+E_INVALID = 1
+E_NOMEM = 2
+E_NODEVICE = 3
+E_HIP = 100
+E_M_TOO_SMALL = 101
+
+
+class FlockHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"flock_hip error {code}: {msg}")
+        self.code = code
+
+
+class F128(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
+
+
+class PcsParams(ctypes.Structure):
+    """PcsParams (pcs/commit.rs:31-65); num_lanes = 0 means None."""
+
+    _fields_ = [
+        ("m", ctypes.c_uint32),
+        ("log_inv_rate", ctypes.c_uint32),
+        ("log_batch_size", ctypes.c_uint32),
+        ("num_lanes", ctypes.c_uint32),
+        ("merkle_hash", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Every function the public header declares."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(flock_(?:hip|fs)_[a-z0-9_]+)\s*\(", text)))
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m flock_amd.build` "
+                          "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, S, I = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    U32, U64 = ctypes.c_uint32, ctypes.c_uint64
+    PP = ctypes.POINTER(PcsParams)
+    sig = {
+        "flock_hip_device_count": (I, []),
+        "flock_hip_last_error": (ctypes.c_char_p, []),
+        "flock_hip_ctx_create": (I, [I, ctypes.POINTER(P)]),
+        "flock_hip_ctx_destroy": (None, [P]),
+        "flock_hip_ctx_stream": (P, [P]),
+        "flock_hip_sync": (I, [P]),
+        "flock_hip_malloc": (I, [P, ctypes.POINTER(P), S]),
+        "flock_hip_dfree": (I, [P, P]),
+        "flock_hip_memcpy_h2d": (I, [P, P, P, S]),
+        "flock_hip_memcpy_d2h": (I, [P, P, P, S]),
+        "flock_hip_free": (None, [P]),
+        "flock_hip_f128_mul_batch": (I, [P, P, P, P, S]),
+        "flock_hip_f128_mul_const_batch": (I, [P, P, F128, P, S]),
+        "flock_hip_ntt_forward_interleaved": (I, [P, P, S, S, U32, U32]),
+        "flock_hip_pcs_sizes": (I, [PP, P, P, P, P, P]),
+        "flock_hip_commit_lane_major": (I, [P, P, S, PP, P, P, U32, P]),
+        "flock_hip_commit": (I, [P, P, S, PP, P, P, U32, P]),
+        "flock_hip_merkle_tree": (I, [P, P, S, S, P]),
+        "flock_hip_sumcheck_round_msg": (I, [P, P, P, S, P]),
+        "flock_hip_sumcheck_fold_msg": (I, [P, P, P, S, F128, P, P, P]),
+        "flock_hip_build_eq": (I, [P, P, S, P]),
+        "flock_hip_zc_round_pair": (I, [P, P, P, S, P, F128, P, P]),
+        "flock_hip_zc_fold_round_pair": (I, [P, P, P, S, F128, P, P, P, F128, P, P]),
+        "flock_hip_fold_pair": (I, [P, P, P, S, F128, P, P]),
+        "flock_hip_prove_sumcheck": (I, [P, P, P, P, U32, P, P]),
+        "flock_fs_new": (P, [ctypes.c_char_p, S]),
+        "flock_fs_free": (None, [P]),
+        "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
+        "flock_fs_observe_f128": (None, [P, F128]),
+        "flock_fs_observe_f128_slice": (None, [P, P, S]),
+        "flock_fs_observe_bytes": (None, [P, ctypes.c_char_p, S]),
+        "flock_fs_sample_f128": (None, [P, ctypes.POINTER(F128)]),
+        "flock_fs_sample_f128_vec": (None, [P, P, S]),
+        "flock_fs_grind_pow": (U64, [P, U32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != OK:
+        raise FlockHipError(rc, load().flock_hip_last_error().decode())
+
+
+def f128(v: int) -> F128:
+    return F128(v & ((1 << 64) - 1), v >> 64)
+
+
+def to_int(x: F128) -> int:
+    return int(x.lo) | (int(x.hi) << 64)
+
+
+def _np_ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ints(a: np.ndarray) -> list[int]:
+    return [int(lo) | (int(hi) << 64) for lo, hi in a.reshape(-1, 2)]
+
+
+def device_count() -> int:
+    return int(load().flock_hip_device_count())
+
+
+# ---------------------------------------------------------------------------
+# Context and device buffers
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(load().flock_hip_malloc(ctx.h, ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @property
+    def words(self) -> int:
+        return self.nbytes // 16
+
+    def upload(self, arr: np.ndarray, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes
+        _check(load().flock_hip_memcpy_h2d(self.ctx.h, self.ptr + offset, _np_ptr(arr), arr.nbytes))
+
+    def download(self, nbytes: int | None = None, offset: int = 0) -> bytes:
+        n = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n, dtype=np.uint8)
+        _check(load().flock_hip_memcpy_d2h(self.ctx.h, _np_ptr(out), self.ptr + offset, n))
+        return out.tobytes()
+
+    def download_f128(self, n: int | None = None, offset_words: int = 0) -> np.ndarray:
+        n = self.words - offset_words if n is None else n
+        out = np.empty((n, 2), dtype=np.uint64)
+        _check(load().flock_hip_memcpy_d2h(self.ctx.h, _np_ptr(out), self.ptr + 16 * offset_words,
+                                           16 * n))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(load().flock_hip_dfree(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                load().flock_hip_dfree(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP stream + scratch on one device (flock_hip_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(load().flock_hip_ctx_create(device, ctypes.byref(h)))
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            load().flock_hip_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        buf = DeviceBuffer(self, max(arr.nbytes, 16))
+        if arr.nbytes:
+            buf.upload(arr)
+        return buf
+
+    def sync(self):
+        _check(load().flock_hip_sync(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(load().flock_hip_ctx_stream(self.h) or 0)
+
+    # ---- field ----
+    def f128_mul_batch(self, a: DeviceBuffer, b: DeviceBuffer, out: DeviceBuffer, n: int):
+        _check(load().flock_hip_f128_mul_batch(self.h, a.ptr, b.ptr, out.ptr, n))
+
+    def f128_mul_const_batch(self, a: DeviceBuffer, t: int, out: DeviceBuffer, n: int):
+        _check(load().flock_hip_f128_mul_const_batch(self.h, a.ptr, f128(t), out.ptr, n))
+
+    # ---- NTT / commit ----
+    def ntt_forward_interleaved(self, data: DeviceBuffer, num_ntts: int, live: int, log_d: int,
+                                start_layer: int):
+        _check(load().flock_hip_ntt_forward_interleaved(self.h, data.ptr, num_ntts, live, log_d,
+                                                        start_layer))
+
+    def merkle_tree(self, data: DeviceBuffer, data_len: int, n_leaves: int, tree: DeviceBuffer):
+        _check(load().flock_hip_merkle_tree(self.h, data.ptr, data_len, n_leaves, tree.ptr))
+
+    def commit_lane_major(self, q: DeviceBuffer, q_len: int, params: PcsParams,
+                          codeword: DeviceBuffer, tree: DeviceBuffer, cap_depth: int = 0) -> list[bytes]:
+        cap = ctypes.create_string_buffer(32 << cap_depth)
+        _check(load().flock_hip_commit_lane_major(self.h, q.ptr, q_len, ctypes.byref(params),
+                                                  codeword.ptr, tree.ptr, cap_depth, cap))
+        return [cap.raw[32 * i:32 * i + 32] for i in range(1 << cap_depth)]
+
+    def commit(self, msg: DeviceBuffer, msg_len: int, params: PcsParams, codeword: DeviceBuffer,
+               tree: DeviceBuffer, cap_depth: int = 0) -> list[bytes]:
+        cap = ctypes.create_string_buffer(32 << cap_depth)
+        _check(load().flock_hip_commit(self.h, msg.ptr, msg_len, ctypes.byref(params),
+                                       codeword.ptr, tree.ptr, cap_depth, cap))
+        return [cap.raw[32 * i:32 * i + 32] for i in range(1 << cap_depth)]
+
+    # ---- sumcheck rounds ----
+    def round_msg_lsb(self, f: DeviceBuffer, b: DeviceBuffer, n: int) -> tuple[int, int]:
+        out = (F128 * 2)()
+        _check(load().flock_hip_sumcheck_round_msg(self.h, f.ptr, b.ptr, n, out))
+        return to_int(out[0]), to_int(out[1])
+
+    def fold_and_msg_lsb(self, f, b, n, r, nf, nb) -> tuple[int, int]:
+        out = (F128 * 2)()
+        _check(load().flock_hip_sumcheck_fold_msg(self.h, f.ptr, b.ptr, n, f128(r), nf.ptr, nb.ptr,
+                                                  out))
+        return to_int(out[0]), to_int(out[1])
+
+    def build_eq(self, r: list[int], out: DeviceBuffer):
+        arr = np.zeros((max(len(r), 1), 2), dtype=np.uint64)
+        for i, v in enumerate(r):
+            arr[i] = (v & ((1 << 64) - 1), v >> 64)
+        _check(load().flock_hip_build_eq(self.h, _np_ptr(arr), len(r), out.ptr))
+
+    def round_pair(self, a, b, n, eq, r0, eq_next=None) -> tuple[int, int]:
+        out = (F128 * 2)()
+        _check(load().flock_hip_zc_round_pair(self.h, a.ptr, b.ptr, n, eq.ptr, f128(r0),
+                                              eq_next.ptr if eq_next else None, out))
+        return to_int(out[0]), to_int(out[1])
+
+    def fold_and_compute_round_pair(self, a, b, n, rho, ao, bo, eq, r0, eq_next=None):
+        out = (F128 * 2)()
+        _check(load().flock_hip_zc_fold_round_pair(self.h, a.ptr, b.ptr, n, f128(rho), ao.ptr,
+                                                   bo.ptr, eq.ptr, f128(r0),
+                                                   eq_next.ptr if eq_next else None, out))
+        return to_int(out[0]), to_int(out[1])
+
+    def fold_pair(self, a, b, n, c, ao, bo):
+        _check(load().flock_hip_fold_pair(self.h, a.ptr, b.ptr, n, f128(c), ao.ptr, bo.ptr))
+
+    def prove_sumcheck(self, fs: "FsChallenger", a: DeviceBuffer, b: DeviceBuffer, log_n: int):
+        msgs = np.zeros((4 * log_n, 2), dtype=np.uint64)
+        finals = np.zeros((4, 2), dtype=np.uint64)
+        _check(load().flock_hip_prove_sumcheck(self.h, fs.h, a.ptr, b.ptr, log_n, _np_ptr(msgs),
+                                               _np_ptr(finals)))
+        return msgs, finals
+
+
+def pcs_sizes(params: PcsParams) -> dict:
+    ml, cl, nl, nn = (ctypes.c_size_t() for _ in range(4))
+    k = ctypes.c_uint32()
+    _check(load().flock_hip_pcs_sizes(ctypes.byref(params), ctypes.byref(ml), ctypes.byref(cl),
+                                      ctypes.byref(nl), ctypes.byref(k), ctypes.byref(nn)))
+    return dict(msg_len=ml.value, cw_len=cl.value, n_leaves=nl.value, k_code=k.value,
+                num_ntts=nn.value)
+
+
+class FsChallenger:
+    """FsChallenger with the SHA-256 discipline (challenger.rs:609-1044)."""
+
+    def __init__(self, domain: bytes):
+        self.h = load().flock_fs_new(domain, len(domain))
+        if not self.h:
+            raise MemoryError("flock_fs_new")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().flock_fs_free(self.h)
+            self.h = None
+
+    def observe_label(self, b: bytes):
+        load().flock_fs_observe_label(self.h, b, len(b))
+
+    def observe_f128(self, v: int):
+        load().flock_fs_observe_f128(self.h, f128(v))
+
+    def observe_f128_slice(self, vs):
+        arr = np.zeros((max(len(vs), 1), 2), dtype=np.uint64)
+        for i, v in enumerate(vs):
+            arr[i] = (v & ((1 << 64) - 1), v >> 64)
+        load().flock_fs_observe_f128_slice(self.h, _np_ptr(arr), len(vs))
+
+    def observe_bytes(self, b: bytes):
+        load().flock_fs_observe_bytes(self.h, b, len(b))
+
+    def sample_f128(self) -> int:
+        out = F128()
+        load().flock_fs_sample_f128(self.h, ctypes.byref(out))
+        return to_int(out)
+
+    def sample_f128_vec(self, n: int) -> list[int]:
+        arr = np.zeros((max(n, 1), 2), dtype=np.uint64)
+        load().flock_fs_sample_f128_vec(self.h, _np_ptr(arr), n)
+        return _ints(arr[:n])
+
+    def grind_pow(self, bits: int) -> int:
+        return int(load().flock_fs_grind_pow(self.h, bits))

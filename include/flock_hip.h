@@ -1,0 +1,181 @@
+/*
+ * flock_hip.h — C ABI of libflock_hip.so, the MI355X (gfx950) implementation of the
+ * data-parallel core of Flock's `prove_fast`: the F_{2^128} multiply, the LCH additive
+ * NTT + SHA-256 Merkle commit, and the product-sumcheck / eq-weighted zerocheck rounds.
+ *
+ * It replaces the reference's CUDA bridge (cuda-ghash/prove_ffi.cu:363-1010, bound from
+ * Rust by crates/flock-cuda-ffi) with phase-level entry points that map one-to-one onto
+ * the Rust seams the prover calls (citations are reference file:line).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a FLOCK_HIP_E_* code; it never aborts on bad
+ *    input. flock_hip_last_error() describes the last failure on the calling thread.
+ *  - Pointers named d_* are device pointers (HBM, e.g. from flock_hip_malloc); h_* are
+ *    host pointers. Field elements are flock_f128 {lo, hi}: lo = coefficients of
+ *    x^0..x^63, reduction polynomial x^128 + x^7 + x^2 + x + 1
+ *    (crates/flock-core/src/field/gf2_128.rs:10-27). Same bytes as the Rust F128.
+ *  - A context owns one HIP stream and its scratch memory. Distinct contexts may be used
+ *    concurrently from different threads; one context must be externally synchronized.
+ *    Calls are stream-ordered; functions that return values to the host synchronize.
+ *  - Fixed-size outputs are caller-allocated.
+ */
+#ifndef FLOCK_HIP_H
+#define FLOCK_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLOCK_HIP_OK 0
+#define FLOCK_HIP_E_INVALID 1       /* bad argument or shape */
+#define FLOCK_HIP_E_NOMEM 2         /* device or host allocation failed */
+#define FLOCK_HIP_E_NODEVICE 3      /* no usable gfx950 device */
+#define FLOCK_HIP_E_HIP 100         /* HIP runtime error (prove_ffi.cu:44-46 uses 100 for CUDA) */
+#define FLOCK_HIP_E_M_TOO_SMALL 101 /* m below the commit minimum (prove_ffi.cu:429) */
+
+typedef struct {
+    uint64_t lo, hi;
+} flock_f128;
+
+typedef struct flock_hip_ctx flock_hip_ctx;
+typedef struct flock_fs flock_fs;
+
+/* ---- device / context -------------------------------------------------------- */
+
+/* Number of visible devices. Replaces flock_cuda_device_count (prove_ffi.cu:410-415). */
+int flock_hip_device_count(void);
+/* Message for the last error returned on this thread ("" if none). */
+const char* flock_hip_last_error(void);
+/* One HIP stream + scratch per context; replaces the reference's process-global mutex
+ * and arena (prove_ffi.cu:52-131, 421-422). */
+int flock_hip_ctx_create(int device, flock_hip_ctx** out);
+void flock_hip_ctx_destroy(flock_hip_ctx* ctx);
+/* The context's hipStream_t (as void*), for callers that order their own work on it. */
+void* flock_hip_ctx_stream(flock_hip_ctx* ctx);
+int flock_hip_sync(flock_hip_ctx* ctx);
+int flock_hip_malloc(flock_hip_ctx* ctx, void** d_ptr, size_t bytes);
+int flock_hip_dfree(flock_hip_ctx* ctx, void* d_ptr);
+int flock_hip_memcpy_h2d(flock_hip_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int flock_hip_memcpy_d2h(flock_hip_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+/* Free a host buffer the library allocated. Replaces flock_cuda_free (prove_ffi.cu:417). */
+void flock_hip_free(void* p);
+
+/* ---- field ------------------------------------------------------------------- */
+
+/* d_out[i] = d_a[i] * d_b[i]. Test surface for `impl Mul for F128` (gf2_128.rs:174-201). */
+int flock_hip_f128_mul_batch(flock_hip_ctx* ctx, const flock_f128* d_a, const flock_f128* d_b,
+                             flock_f128* d_out, size_t n);
+/* d_out[i] = d_a[i] * t through the fixed-multiplier (table) path used by the NTT/folds. */
+int flock_hip_f128_mul_const_batch(flock_hip_ctx* ctx, const flock_f128* d_a, flock_f128 t,
+                                   flock_f128* d_out, size_t n);
+
+/* ---- additive NTT ------------------------------------------------------------ */
+
+/* In-place forward interleaved NTT of AdditiveNttF128::standard(log_d) over layers
+ * [start_layer, log_d), transforming only lanes < live (the rest must be zero):
+ * forward_transform_interleaved_live_from_layer (ntt/additive_ntt_f128.rs:220-266).
+ * d_data holds 2^log_d positions x num_ntts lanes, data[pos * num_ntts + lane]. */
+int flock_hip_ntt_forward_interleaved(flock_hip_ctx* ctx, flock_f128* d_data, size_t num_ntts,
+                                      size_t live, uint32_t log_d, uint32_t start_layer);
+
+/* ---- PCS commit -------------------------------------------------------------- */
+
+/* PcsParams (pcs/commit.rs:31-65). num_lanes = 0 means None (all 2^log_batch_size lanes).
+ * merkle_hash: 0 = SHA-256 (the only hash implemented here; HashKind, hash.rs:23-58). */
+typedef struct {
+    uint32_t m, log_inv_rate, log_batch_size;
+    uint32_t num_lanes;
+    uint32_t merkle_hash;
+} flock_pcs_params;
+
+/* Derived sizes (PcsParams::{msg_len_f128, codeword_len_f128, n_leaves, k_code, num_ntts},
+ * pcs/commit.rs:187-228). Any output pointer may be NULL. */
+int flock_hip_pcs_sizes(const flock_pcs_params* p, size_t* msg_len, size_t* cw_len,
+                        size_t* n_leaves, uint32_t* k_code, size_t* num_ntts);
+
+/* pcs::commit_lane_major (pcs/commit.rs:474-510): d_q is the full 2^(m-7)-word lane-major
+ * stack (lane l = q[l*D .. (l+1)*D), lanes >= num_ntts zero). Writes the codeword
+ * (cw_len words, codeword[pos * num_ntts + lane]) and the flat Merkle tree
+ * (2*n_leaves - 1 nodes x 32 B, merkle.rs:15-22) to caller-allocated device buffers, and,
+ * if h_cap != NULL, the 2^cap_depth cap nodes (cap_layer, merkle.rs:649-656) to h_cap. */
+int flock_hip_commit_lane_major(flock_hip_ctx* ctx, const flock_f128* d_q, size_t q_len,
+                                const flock_pcs_params* p, flock_f128* d_codeword,
+                                uint8_t* d_tree, uint32_t cap_depth, uint8_t* h_cap);
+
+/* pcs::commit / commit_into (pcs/commit.rs:345-397) on the interleaved message
+ * d_msg[p * num_ntts + lane] of msg_len = num_ntts * 2^log_dim words. */
+int flock_hip_commit(flock_hip_ctx* ctx, const flock_f128* d_msg, size_t msg_len,
+                     const flock_pcs_params* p, flock_f128* d_codeword, uint8_t* d_tree,
+                     uint32_t cap_depth, uint8_t* h_cap);
+
+/* merkle::merkle_tree with HashKind::Sha256 (merkle.rs:543-589): n_leaves (a power of two)
+ * equal leaves of data_len / n_leaves bytes (a multiple of 16). */
+int flock_hip_merkle_tree(flock_hip_ctx* ctx, const uint8_t* d_data, size_t data_len,
+                          size_t n_leaves, uint8_t* d_tree);
+
+/* ---- sumcheck rounds ----------------------------------------------------------- */
+
+/* round_msg_lsb (pcs/ligerito.rs:3204-3240): h_msg = {u_0, u_2}. n >= 2, power of two. */
+int flock_hip_sumcheck_round_msg(flock_hip_ctx* ctx, const flock_f128* d_f,
+                                 const flock_f128* d_b, size_t n, flock_f128 h_msg[2]);
+/* fold_and_msg_lsb (pcs/ligerito.rs:3372-3414): d_nf/d_nb (n/2 words) = v0 + r(v0 + v1);
+ * h_msg = next round's {u_0, u_2} (zero when n == 2). */
+int flock_hip_sumcheck_fold_msg(flock_hip_ctx* ctx, const flock_f128* d_f,
+                                const flock_f128* d_b, size_t n, flock_f128 r, flock_f128* d_nf,
+                                flock_f128* d_nb, flock_f128 h_msg[2]);
+/* build_eq (zerocheck/univariate_skip.rs:31-47): d_out[x] = prod_i (bit_i(x) ? r_i : 1+r_i)
+ * for x < 2^nv. */
+int flock_hip_build_eq(flock_hip_ctx* ctx, const flock_f128* h_r, size_t nv, flock_f128* d_out);
+/* round_pair_naive (zerocheck/multilinear.rs:331-355) with d_eq = build_eq(r[1..]) (n/2
+ * words) and r0 = r[0]: h_msg = {r0 * G(1), G(inf)}. If d_eq_next != NULL and n >= 4 it
+ * receives the next round's table d_eq_next[z] = d_eq[2z] + d_eq[2z+1] (n/4 words). */
+int flock_hip_zc_round_pair(flock_hip_ctx* ctx, const flock_f128* d_a, const flock_f128* d_b,
+                            size_t n, const flock_f128* d_eq, flock_f128 r0,
+                            flock_f128* d_eq_next, flock_f128 h_msg[2]);
+/* fold_and_compute_round_pair_into (zerocheck/multilinear.rs:1197-1420): fold (a, b) of size
+ * n >= 4 at rho into d_ao/d_bo (n/2), then round_pair on the folded tables with d_eq
+ * (n/4 words) and r0. d_eq_next (n/8 words, may be NULL) as above. */
+int flock_hip_zc_fold_round_pair(flock_hip_ctx* ctx, const flock_f128* d_a,
+                                 const flock_f128* d_b, size_t n, flock_f128 rho,
+                                 flock_f128* d_ao, flock_f128* d_bo, const flock_f128* d_eq,
+                                 flock_f128 r0, flock_f128* d_eq_next, flock_f128 h_msg[2]);
+/* fold_in_place_pair (zerocheck/multilinear.rs:1148-1163), out of place: n -> n/2. */
+int flock_hip_fold_pair(flock_hip_ctx* ctx, const flock_f128* d_a, const flock_f128* d_b,
+                        size_t n, flock_f128 c, flock_f128* d_ao, flock_f128* d_bo);
+
+/* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
+
+/* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */
+flock_fs* flock_fs_new(const uint8_t* domain, size_t len);
+void flock_fs_free(flock_fs* fs);
+void flock_fs_observe_label(flock_fs* fs, const uint8_t* label, size_t len);
+void flock_fs_observe_f128(flock_fs* fs, flock_f128 v);
+void flock_fs_observe_f128_slice(flock_fs* fs, const flock_f128* v, size_t n);
+void flock_fs_observe_bytes(flock_fs* fs, const uint8_t* b, size_t n);
+void flock_fs_sample_f128(flock_fs* fs, flock_f128* out);
+void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n);
+uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits);
+
+/* ---- protocol drivers ------------------------------------------------------------ */
+
+/* Product sumcheck followed by an eq-weighted zerocheck tail on the same two tables of
+ * 2^log_n words, over transcript fs. Product phase: observe label
+ * "flock-product-sumcheck", then SumcheckProver::new + the sample/fold/observe loop of
+ * pcs/ligerito.rs:4986-5011 for log_n rounds, the last fold without message, then
+ * observe the two folded values. Zerocheck phase: observe label "flock-zerocheck-v0",
+ * sample log_n eq challenges r, then the message/sample/fold loop of
+ * zerocheck.rs:790-975 with Convention A (r_next[0] = ONE), a final fold, and observe
+ * the two folded values. h_msgs receives 4*log_n words ({u0,u2} per product round, then
+ * {G(1), G(inf)} per zerocheck round); h_finals = {f, b, a, b} final values.
+ * d_a and d_b are not modified. */
+int flock_hip_prove_sumcheck(flock_hip_ctx* ctx, flock_fs* fs, const flock_f128* d_a,
+                             const flock_f128* d_b, uint32_t log_n, flock_f128* h_msgs,
+                             flock_f128 h_finals[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
