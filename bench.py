@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--m", type=int, default=31)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sumcheck", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (the reference kernel for "
+                         "scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -201,6 +204,17 @@ def main():
         "timed_wall_s": round(wall, 3),
     }
 
+    if args.copy_calibration:
+        nbytes = 1 << 30
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        ev.h.hipMemcpyDtoD.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        ctx.sync()
+        assert ev.h.hipMemcpyDtoD(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes) == 0
+        ctx.sync()
+        result["copy_calibration_bytes"] = nbytes
+        src.free()
+        dst.free()
+
     if not args.no_sumcheck:
         log_n = 24
         n = 1 << log_n
@@ -221,7 +235,7 @@ def main():
             (48 + 56) * 2 * n / (min(times)) / 1e9, 3)
 
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(m=27, reps=3)
+        result["cpu_baseline"] = cpu_baseline(m=29, reps=3)
 
     ctx.close()
     print(json.dumps(result), flush=True)

@@ -111,6 +111,95 @@ __device__ __forceinline__ f128 f128_mul(f128 a, f128 b) {
     return f256u_reduce(acc);
 }
 
+// ---------------------------------------------------------------------------
+// Generic product with a per-lane 4-bit comb table in LDS (Lopez-Dahab comb).
+//
+// b = b_lo + b_hi * x^125 with b_lo < 2^125 and b_hi < 8. The lane writes
+// T[n] = n(x) * b_lo (n < 16, all < 2^128) into its LDS column, then for nibble
+// position j = 7..0 XORs T[nibble_j(a_w)] into acc at word offset w (w = 0..3) and shifts
+// acc left by 4 between positions; a * b_hi * x^125 is added with three masked shifts.
+// Workspace layout: entry n of lane l at ws[n * 64 + l] (entry-major, lane-minor), so the
+// 16 lanes served per LDS cycle hit 16 distinct 16-byte bank groups: no bank conflicts.
+// T[0] must be zero (written once by comb_ws_init).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void comb_ws_init(f128* ws_lane) { ws_lane[0] = f128_zero(); }
+
+// Build the table of b (b_lo part) in the lane's column; returns b_hi.
+__device__ __forceinline__ uint32_t comb_build(f128* ws_lane, f128 b) {
+    const uint32_t bh = b.w >> 29;
+    const f128 t1 = make_uint4(b.x, b.y, b.z, b.w & 0x1fffffffu);
+    const f128 t2 = make_uint4(t1.x << 1, fshl(t1.y, t1.x, 1), fshl(t1.z, t1.y, 1),
+                               fshl(t1.w, t1.z, 1));
+    const f128 t4 = make_uint4(t1.x << 2, fshl(t1.y, t1.x, 2), fshl(t1.z, t1.y, 2),
+                               fshl(t1.w, t1.z, 2));
+    const f128 t8 = make_uint4(t1.x << 3, fshl(t1.y, t1.x, 3), fshl(t1.z, t1.y, 3),
+                               fshl(t1.w, t1.z, 3));
+    const f128 t3 = f128_add(t1, t2), t5 = f128_add(t1, t4), t6 = f128_add(t2, t4);
+    const f128 t7 = f128_add(t3, t4), t9 = f128_add(t1, t8), t10 = f128_add(t2, t8);
+    const f128 t12 = f128_add(t4, t8);
+    ws_lane[1 * 64] = t1;
+    ws_lane[2 * 64] = t2;
+    ws_lane[3 * 64] = t3;
+    ws_lane[4 * 64] = t4;
+    ws_lane[5 * 64] = t5;
+    ws_lane[6 * 64] = t6;
+    ws_lane[7 * 64] = t7;
+    ws_lane[8 * 64] = t8;
+    ws_lane[9 * 64] = t9;
+    ws_lane[10 * 64] = t10;
+    ws_lane[11 * 64] = f128_add(t3, t8);
+    ws_lane[12 * 64] = t12;
+    ws_lane[13 * 64] = f128_add(t5, t8);
+    ws_lane[14 * 64] = f128_add(t6, t8);
+    ws_lane[15 * 64] = f128_add(t7, t8);
+    return bh;
+}
+
+// acc ^= a * b where the lane's table holds b_lo and bh = b_hi (unreduced 256-bit).
+__device__ __forceinline__ void comb_apply(f256u& out, const f128* ws_lane, uint32_t bh, f128 a) {
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
+    uint32_t r[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int j = 7; j >= 0; j--) {
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const f128 t = ws_lane[((aw[w] >> (4 * j)) & 15u) * 64];
+            r[w] ^= t.x;
+            r[w + 1] ^= t.y;
+            r[w + 2] ^= t.z;
+            r[w + 3] ^= t.w;
+        }
+        if (j) {
+#pragma unroll
+            for (int i = 7; i > 0; i--) r[i] = fshl(r[i], r[i - 1], 4);
+            r[0] <<= 4;
+        }
+    }
+    // a * b_hi (b_hi < 8) placed at x^125 = word 3, bit 29
+    const uint32_t m0 = 0u - (bh & 1u), m1 = 0u - ((bh >> 1) & 1u), m2 = 0u - ((bh >> 2) & 1u);
+    uint32_t p[5];
+    p[0] = (a.x & m0) ^ ((a.x << 1) & m1) ^ ((a.x << 2) & m2);
+    p[1] = (a.y & m0) ^ (fshl(a.y, a.x, 1) & m1) ^ (fshl(a.y, a.x, 2) & m2);
+    p[2] = (a.z & m0) ^ (fshl(a.z, a.y, 1) & m1) ^ (fshl(a.z, a.y, 2) & m2);
+    p[3] = (a.w & m0) ^ (fshl(a.w, a.z, 1) & m1) ^ (fshl(a.w, a.z, 2) & m2);
+    p[4] = ((a.w >> 31) & m1) ^ ((a.w >> 30) & m2);
+    r[3] ^= p[0] << 29;
+    r[4] ^= fshl(p[1], p[0], 29);
+    r[5] ^= fshl(p[2], p[1], 29);
+    r[6] ^= fshl(p[3], p[2], 29);
+    r[7] ^= fshl(p[4], p[3], 29);
+#pragma unroll
+    for (int i = 0; i < 8; i++) out.r[i] ^= r[i];
+}
+
+__device__ __forceinline__ f128 comb_mul(f128* ws_lane, f128 a, f128 b) {
+    const uint32_t bh = comb_build(ws_lane, b);
+    f256u acc;
+    f256u_zero(acc);
+    comb_apply(acc, ws_lane, bh, a);
+    return f256u_reduce(acc);
+}
+
 // a * x mod p (gf2_128.rs:213-220).
 __device__ __forceinline__ f128 f128_mul_x(f128 a) {
     const uint32_t m = (uint32_t)((int32_t)a.w >> 31);
