@@ -516,7 +516,22 @@ int flock_hip_ntt_forward_interleaved(flock_hip_ctx* c, flock_f128* data, size_t
     int rc = c->get_evals((int)log_d, &ev);
     if (rc) return rc;
     LAUNCH(flock::launch_ntt_forward(U(data), (uint32_t)num_ntts, (uint32_t)live, (int)log_d,
-                                     (int)start_layer, ev, c->stream));
+                                     (int)start_layer, ev, nullptr, c->stream));
+    return 0;
+}
+
+int flock_hip_ntt_pass(flock_hip_ctx* c, flock_f128* data, size_t num_ntts, size_t live,
+                       uint32_t log_d, uint32_t first_layer, uint32_t layers) {
+    CTX_CHECK(c);
+    if (!data || num_ntts == 0 || live > num_ntts || log_d > 62 || layers < 1 || layers > 3 ||
+        first_layer + layers > log_d || num_ntts > 0xffffffffull)
+        return fail(FLOCK_HIP_E_INVALID, "bad NTT pass shape");
+    if (live == 0) return 0;
+    const uint4* ev = nullptr;
+    int rc = c->get_evals((int)log_d, &ev);
+    if (rc) return rc;
+    LAUNCH(flock::launch_ntt_pass(U(data), (uint32_t)num_ntts, (uint32_t)live, (int)log_d,
+                                  (int)first_layer, (int)layers, ev, c->stream));
     return 0;
 }
 
@@ -552,32 +567,51 @@ int flock_hip_merkle_tree(flock_hip_ctx* c, const uint8_t* data, size_t len, siz
     return 0;
 }
 
-static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len,
-                         const flock_pcs_params* p, bool lane_major, flock_f128* cw,
-                         uint8_t* tree, uint32_t cap_depth, uint8_t* h_cap) {
+static int encode_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len,
+                         const flock_pcs_params* p, bool lane_major, flock_f128* cw) {
     CTX_CHECK(c);
-    size_t msg_len, cw_len, n_leaves, t;
+    size_t msg_len, t;
     uint32_t k;
-    int rc = flock_hip_pcs_sizes(p, &msg_len, &cw_len, &n_leaves, &k, &t);
+    int rc = flock_hip_pcs_sizes(p, &msg_len, nullptr, nullptr, &k, &t);
     if (rc) return rc;
-    if (!src || !cw || !tree) return fail(FLOCK_HIP_E_INVALID, "null buffer");
-    if (cap_depth > k) return fail(FLOCK_HIP_E_INVALID, "cap depth exceeds tree depth");
+    if (!src || !cw) return fail(FLOCK_HIP_E_INVALID, "null buffer");
     const uint32_t reps = 1u << p->log_inv_rate;
+    const uint4* ev = nullptr;
+    rc = c->get_evals((int)k, &ev);
+    if (rc) return rc;
     if (lane_major) {
         // commit_lane_major takes the full padded stack (pcs/commit.rs:477-482)
         if (src_len != ((size_t)1 << (p->m - 7)))
             return fail(FLOCK_HIP_E_INVALID, "lane-major message must be the full 2^(m-7) stack");
         const uint64_t d = src_len >> p->log_batch_size;
         LAUNCH(flock::launch_fill_lane_major(U(src), U(cw), d, (uint32_t)t, reps, c->stream));
+        LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
+                                         (int)p->log_inv_rate, ev, nullptr, c->stream));
     } else {
         if (src_len != msg_len) return fail(FLOCK_HIP_E_INVALID, "message length mismatch");
-        LAUNCH(flock::launch_fill_replicate(U(src), U(cw), msg_len, reps, c->stream));
+        if (k > p->log_inv_rate) {
+            // replicate_message_fill fused into the first NTT pass
+            LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
+                                             (int)p->log_inv_rate, ev, U(src), c->stream));
+        } else {
+            LAUNCH(flock::launch_fill_replicate(U(src), U(cw), msg_len, reps, c->stream));
+        }
     }
-    const uint4* ev = nullptr;
-    rc = c->get_evals((int)k, &ev);
+    return 0;
+}
+
+static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len,
+                         const flock_pcs_params* p, bool lane_major, flock_f128* cw,
+                         uint8_t* tree, uint32_t cap_depth, uint8_t* h_cap) {
+    CTX_CHECK(c);
+    size_t n_leaves, t;
+    uint32_t k;
+    int rc = flock_hip_pcs_sizes(p, nullptr, nullptr, &n_leaves, &k, &t);
     if (rc) return rc;
-    LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
-                                     (int)p->log_inv_rate, ev, c->stream));
+    if (!tree) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    if (cap_depth > k) return fail(FLOCK_HIP_E_INVALID, "cap depth exceeds tree depth");
+    rc = encode_common(c, src, src_len, p, lane_major, cw);
+    if (rc) return rc;
     LAUNCH(flock::launch_merkle_tree(reinterpret_cast<const uint8_t*>(cw), n_leaves, t * 16, tree,
                                      c->stream));
     if (h_cap) {
@@ -587,6 +621,16 @@ static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return 0;
+}
+
+int flock_hip_encode(flock_hip_ctx* c, const flock_f128* msg, size_t msg_len,
+                     const flock_pcs_params* p, flock_f128* cw) {
+    return encode_common(c, msg, msg_len, p, false, cw);
+}
+
+int flock_hip_encode_lane_major(flock_hip_ctx* c, const flock_f128* q, size_t q_len,
+                                const flock_pcs_params* p, flock_f128* cw) {
+    return encode_common(c, q, q_len, p, true, cw);
 }
 
 int flock_hip_commit_lane_major(flock_hip_ctx* c, const flock_f128* q, size_t q_len,

@@ -159,21 +159,26 @@ __device__ __forceinline__ uint32_t comb_build(f128* ws_lane, f128 b) {
 __device__ __forceinline__ void comb_apply(f256u& out, const f128* ws_lane, uint32_t bh, f128 a) {
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
     uint32_t r[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-#pragma unroll
+    // nibble position j = 7: no shift before it; kept rolled so only 4 lookups are live
+#pragma unroll 1
     for (int j = 7; j >= 0; j--) {
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            const f128 t = ws_lane[((aw[w] >> (4 * j)) & 15u) * 64];
-            r[w] ^= t.x;
-            r[w + 1] ^= t.y;
-            r[w + 2] ^= t.z;
-            r[w + 3] ^= t.w;
-        }
-        if (j) {
+        if (j != 7) {
 #pragma unroll
             for (int i = 7; i > 0; i--) r[i] = fshl(r[i], r[i - 1], 4);
             r[0] <<= 4;
         }
+        const f128 t0 = ws_lane[((aw[0] >> (4 * j)) & 15u) * 64];
+        const f128 t1 = ws_lane[((aw[1] >> (4 * j)) & 15u) * 64];
+        const f128 t2 = ws_lane[((aw[2] >> (4 * j)) & 15u) * 64];
+        const f128 t3 = ws_lane[((aw[3] >> (4 * j)) & 15u) * 64];
+        r[0] ^= t0.x;
+        r[1] = __builtin_amdgcn_bitop3_b32(r[1], t0.y, t1.x, 0x96);
+        r[2] = __builtin_amdgcn_bitop3_b32(r[2], t0.z, t1.y, 0x96) ^ t2.x;
+        r[3] = __builtin_amdgcn_bitop3_b32(r[3], t0.w, t1.z, 0x96) ^
+               __builtin_amdgcn_bitop3_b32(t2.y, t3.x, 0u, 0x96);
+        r[4] = __builtin_amdgcn_bitop3_b32(r[4], t1.w, t2.z, 0x96) ^ t3.y;
+        r[5] = __builtin_amdgcn_bitop3_b32(r[5], t2.w, t3.z, 0x96);
+        r[6] ^= t3.w;
     }
     // a * b_hi (b_hi < 8) placed at x^125 = word 3, bit 29
     const uint32_t m0 = 0u - (bh & 1u), m1 = 0u - ((bh >> 1) & 1u), m2 = 0u - ((bh >> 2) & 1u);
@@ -251,15 +256,20 @@ __device__ __forceinline__ void shoup_build_row(f128* table, f128 t, int k) {
     }
 }
 
+// XOR of 32 lookups, two per three-input XOR (v_bitop3 0x96).
 __device__ __forceinline__ f128 shoup_mul(const f128* __restrict__ table, f128 a) {
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
     f128 acc = f128_zero();
 #pragma unroll
     for (int w = 0; w < 4; w++) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t n = (aw[w] >> (4 * j)) & 15u;
-            acc = f128_add(acc, table[(w * 8 + j) * 16 + n]);
+        for (int j = 0; j < 8; j += 2) {
+            const f128 t0 = table[(w * 8 + j) * 16 + ((aw[w] >> (4 * j)) & 15u)];
+            const f128 t1 = table[(w * 8 + j + 1) * 16 + ((aw[w] >> (4 * j + 4)) & 15u)];
+            acc.x = __builtin_amdgcn_bitop3_b32(acc.x, t0.x, t1.x, 0x96);
+            acc.y = __builtin_amdgcn_bitop3_b32(acc.y, t0.y, t1.y, 0x96);
+            acc.z = __builtin_amdgcn_bitop3_b32(acc.z, t0.z, t1.z, 0x96);
+            acc.w = __builtin_amdgcn_bitop3_b32(acc.w, t0.w, t1.w, 0x96);
         }
     }
     return acc;

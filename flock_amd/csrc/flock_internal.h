@@ -26,9 +26,16 @@ hipError_t launch_fill_lane_major(const uint4* q, uint4* cw, uint64_t d, uint32_
 hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, uint32_t reps,
                                  hipStream_t s);
 // Forward interleaved LCH NTT over layers [start_layer, k) of 2^k positions x t lanes,
-// transforming only lanes < live. evals: k x k basis table (device).
+// transforming only lanes < live. evals: k x k basis table (device). If msg != nullptr the
+// first pass reads the 2^(k-start_layer) x t message instead of data (the state after
+// start_layer copy layers is the message replicated 2^start_layer times), so data needs
+// no prior fill.
 hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
-                              const uint4* evals, hipStream_t s);
+                              const uint4* evals, const uint4* msg, hipStream_t s);
+
+// One fused pass of `layers` in {1,2,3} layers starting at l0, in place.
+hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0, int layers,
+                           const uint4* evals, hipStream_t s);
 
 // ---- merkle.hip ----
 // SHA-256 Merkle tree, flat layout (leaves first, root last). leaf_bytes % 16 == 0.

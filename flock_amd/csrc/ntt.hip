@@ -75,11 +75,19 @@ hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, 
 
 // ------------------------------------------------------------------- NTT --
 
+constexpr int kNttThreads = 512;
+
+// One fused pass over layers [l0, l0 + K). src == data for an in-place pass; for the
+// first pass of a commit src is the message itself: the state after the log_inv_rate
+// copy layers is the message replicated, so position P reads message position
+// P mod 2^(k-l0) and the group's source offset drops the high block (base) term.
 template <int K>
-__global__ __launch_bounds__(256) void k_ntt_pass(f128* __restrict__ data, uint32_t t,
-                                                   uint32_t live, int k, int l0,
-                                                   const f128* __restrict__ evals,
-                                                   uint64_t cols_per_hb, uint32_t wgs_per_hb) {
+__global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict__ src,
+                                                           f128* __restrict__ data, uint32_t t,
+                                                           uint32_t live, int k, int l0,
+                                                           const f128* __restrict__ evals,
+                                                           uint64_t cols_per_hb,
+                                                           uint32_t wgs_per_hb, int from_msg) {
     constexpr int NT = (1 << K) - 1;  // twiddles (and Shoup tables) per group
     extern __shared__ f128 tables[];   // NT * SHOUP_ENTRIES
     __shared__ f128 tw[NT];
@@ -106,15 +114,16 @@ __global__ __launch_bounds__(256) void k_ntt_pass(f128* __restrict__ data, uint3
     __syncthreads();
 
     const uint64_t S = 1ull << (k - l0 - K);
-    const uint64_t stride = S * t;                      // elements between group members
+    const uint64_t stride = S * t;                         // elements between group members
     const uint64_t base = (hb << (k - l0)) * (uint64_t)t;  // first element of the group
+    const uint64_t sbase = from_msg ? 0 : base;
     for (uint64_t c = (uint64_t)chunk * blockDim.x + threadIdx.x; c < cols_per_hb;
          c += (uint64_t)wgs_per_hb * blockDim.x) {
         const uint32_t lane = (uint32_t)(c % t);
         if (lane >= live) continue;  // dead lanes are zero and stay zero
         f128 e[1 << K];
 #pragma unroll
-        for (int i = 0; i < (1 << K); i++) e[i] = data[base + i * stride + c];
+        for (int i = 0; i < (1 << K); i++) e[i] = src[sbase + i * stride + c];
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int h = 1 << (K - 1 - j);
@@ -132,8 +141,8 @@ __global__ __launch_bounds__(256) void k_ntt_pass(f128* __restrict__ data, uint3
 }
 
 template <int K>
-static hipError_t launch_pass(f128* data, uint32_t t, uint32_t live, int k, int l0,
-                              const f128* evals, hipStream_t s) {
+static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t live, int k,
+                              int l0, const f128* evals, bool from_msg, hipStream_t s) {
     constexpr int NT = (1 << K) - 1;
     const size_t lds = (size_t)NT * SHOUP_ENTRIES * sizeof(f128);
     static bool attr_set = false;
@@ -145,36 +154,65 @@ static hipError_t launch_pass(f128* data, uint32_t t, uint32_t live, int k, int 
     }
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
-    // up to 4 columns per thread so the twiddle-table build is amortised
-    uint64_t wgs = (cols + 1023) / 1024;
+    // up to 2 columns per thread so the twiddle-table build is amortised
+    uint64_t wgs = (cols + 2 * kNttThreads - 1) / (2 * kNttThreads);
     if (wgs < 1) wgs = 1;
     const uint64_t grid = n_hb * wgs;
     if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    k_ntt_pass<K><<<(unsigned)grid, 256, lds, s>>>(data, t, live, k, l0, evals, cols,
-                                                   (uint32_t)wgs);
+    k_ntt_pass<K><<<(unsigned)grid, kNttThreads, lds, s>>>(src, data, t, live, k, l0, evals, cols,
+                                                           (uint32_t)wgs, from_msg ? 1 : 0);
     return hipGetLastError();
 }
 
+// Fused depth of each pass: 3 layers (7 Shoup tables = 56 KiB of LDS, two groups per CU);
+// a remainder of 1 or 2 layers is covered by one or two 2-layer passes.
+static int plan_passes(int layers, int* ks) {
+    int n = 0;
+    if (layers == 1) {
+        ks[n++] = 1;
+        return n;
+    }
+    int threes = layers / 3, rem = layers % 3;
+    if (rem == 1) {
+        threes -= 1;
+        rem = 4;
+    }
+    for (int i = 0; i < threes; i++) ks[n++] = 3;
+    while (rem > 0) {
+        ks[n++] = 2;
+        rem -= 2;
+    }
+    return n;
+}
+
+hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0, int layers,
+                           const uint4* evals, hipStream_t s) {
+    switch (layers) {
+        case 3: return launch_pass<3>(data, data, t, live, k, l0, evals, false, s);
+        case 2: return launch_pass<2>(data, data, t, live, k, l0, evals, false, s);
+        case 1: return launch_pass<1>(data, data, t, live, k, l0, evals, false, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
-                              const uint4* evals, hipStream_t s) {
+                              const uint4* evals, const uint4* msg, hipStream_t s) {
+    int ks[64];
+    const int n = plan_passes(k - start_layer, ks);
     int l = start_layer;
-    while (l < k) {
-        const int rem = k - l;
+    for (int p = 0; p < n; p++) {
+        const bool from_msg = (p == 0) && msg != nullptr;
+        const f128* src = from_msg ? msg : data;
+        // a message-sourced pass writes every lane (dead lanes copy their zeros)
+        const uint32_t lv = from_msg ? t : live;
         hipError_t e;
-        if (rem >= 4) {
-            e = launch_pass<4>(data, t, live, k, l, evals, s);
-            l += 4;
-        } else if (rem == 3) {
-            e = launch_pass<3>(data, t, live, k, l, evals, s);
-            l += 3;
-        } else if (rem == 2) {
-            e = launch_pass<2>(data, t, live, k, l, evals, s);
-            l += 2;
-        } else {
-            e = launch_pass<1>(data, t, live, k, l, evals, s);
-            l += 1;
+        switch (ks[p]) {
+            case 3: e = launch_pass<3>(src, data, t, lv, k, l, evals, from_msg, s); break;
+            case 2: e = launch_pass<2>(src, data, t, lv, k, l, evals, from_msg, s); break;
+            default: e = launch_pass<1>(src, data, t, lv, k, l, evals, from_msg, s); break;
         }
         if (e != hipSuccess) return e;
+        l += ks[p];
     }
     return hipSuccess;
 }

@@ -44,17 +44,27 @@ __device__ __forceinline__ void block_xor_pair(f128 a0, f128 a1, f128* partials)
     }
 }
 
+// Per-wave comb workspace: 16 entries x 64 lanes x 16 B (see comb_build).
+constexpr int kCombWsBytes = 4 * 16 * 64 * 16;  // four waves per 256-thread block
+__device__ __forceinline__ f128* comb_lane(f128* ws) {
+    f128* w = ws + (threadIdx.x >> 6) * 1024 + (threadIdx.x & 63);
+    comb_ws_init(w);
+    return w;
+}
+
 __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f,
                                                        const f128* __restrict__ b, uint64_t n,
                                                        f128* __restrict__ partials) {
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
     f256u u0, u2;
     f256u_zero(u0);
     f256u_zero(u2);
     for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n / 2;
          j += (uint64_t)gridDim.x * blockDim.x) {
         const f128 f0 = f[2 * j], f1 = f[2 * j + 1], b0 = b[2 * j], b1 = b[2 * j + 1];
-        clmul128_acc(u0, f0, b0);
-        clmul128_acc(u2, f128_add(f0, f1), f128_add(b0, b1));
+        comb_apply(u0, wl, comb_build(wl, b0), f0);
+        comb_apply(u2, wl, comb_build(wl, f128_add(b0, b1)), f128_add(f0, f1));
     }
     block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
 }
@@ -65,6 +75,8 @@ __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
                                                       f128* __restrict__ nb,
                                                       f128* __restrict__ partials) {
     __shared__ f128 table[SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
     if (threadIdx.x < 32) shoup_build_row(table, r, threadIdx.x);
     __syncthreads();
     const uint64_t half = n / 2;
@@ -89,8 +101,8 @@ __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
             nf[2 * k + 1] = g1;
             nb[2 * k] = c0;
             nb[2 * k + 1] = c1;
-            clmul128_acc(u0, g0, c0);
-            clmul128_acc(u2, f128_add(g0, g1), f128_add(c0, c1));
+            comb_apply(u0, wl, comb_build(wl, c0), g0);
+            comb_apply(u2, wl, comb_build(wl, f128_add(c0, c1)), f128_add(g0, g1));
         }
     }
     block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
@@ -101,6 +113,8 @@ __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
                                                    const f128* __restrict__ e,
                                                    f128* __restrict__ e_next,
                                                    f128* __restrict__ partials) {
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
     f256u g1, gi;
     f256u_zero(g1);
     f256u_zero(gi);
@@ -109,8 +123,11 @@ __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
          x += (uint64_t)gridDim.x * blockDim.x) {
         const f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
         const f128 ex = e[x];
-        clmul128_acc(g1, ex, f128_mul(a1, b1));
-        clmul128_acc(gi, ex, f128_mul(f128_add(a0, a1), f128_add(b0, b1)));
+        const f128 p1 = comb_mul(wl, a1, b1);
+        const f128 pi = comb_mul(wl, f128_add(a0, a1), f128_add(b0, b1));
+        const uint32_t eh = comb_build(wl, ex);
+        comb_apply(g1, wl, eh, p1);
+        comb_apply(gi, wl, eh, pi);
         if (e_next && !(x & 1) && x + 1 < half) e_next[x >> 1] = f128_add(ex, e[x + 1]);
     }
     block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
@@ -124,6 +141,8 @@ __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ 
                                                         f128* __restrict__ e_next,
                                                         f128* __restrict__ partials) {
     __shared__ f128 table[SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
     if (threadIdx.x < 32) shoup_build_row(table, rho, threadIdx.x);
     __syncthreads();
     f256u g1, gi;
@@ -143,8 +162,11 @@ __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ 
         bo[2 * y] = q0;
         bo[2 * y + 1] = q1;
         const f128 ey = e[y];
-        clmul128_acc(g1, ey, f128_mul(p1, q1));
-        clmul128_acc(gi, ey, f128_mul(f128_add(p0, p1), f128_add(q0, q1)));
+        const f128 m1 = comb_mul(wl, p1, q1);
+        const f128 mi = comb_mul(wl, f128_add(p0, p1), f128_add(q0, q1));
+        const uint32_t eh = comb_build(wl, ey);
+        comb_apply(g1, wl, eh, m1);
+        comb_apply(gi, wl, eh, mi);
         if (e_next && !(y & 1) && y + 1 < quarter) e_next[y >> 1] = f128_add(ey, e[y + 1]);
     }
     block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
@@ -191,6 +213,21 @@ __global__ __launch_bounds__(256) void k_eq_outer(const f128* __restrict__ lo, i
         out[(xh << nlo) + xl] = shoup_mul(table, lo[xl]);
 }
 
+static hipError_t set_ws_attr(const void* fn) {
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kCombWsBytes);
+}
+
+static hipError_t init_attrs() {
+    static hipError_t st = [] {
+        hipError_t e = set_ws_attr((const void*)k_sc_round_msg);
+        if (e == hipSuccess) e = set_ws_attr((const void*)k_sc_fold_msg);
+        if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_round);
+        if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_fold_round);
+        return e;
+    }();
+    return st;
+}
+
 static inline unsigned red_grid(uint64_t work) {
     uint64_t g = (work + 255) / 256;
     if (g < 1) g = 1;
@@ -200,8 +237,9 @@ static inline unsigned red_grid(uint64_t work) {
 
 hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
                                uint4* out, hipStream_t s) {
+    if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_sc_round_msg<<<g, 256, 0, s>>>(f, b, n, partials);
+    k_sc_round_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_xor_reduce(partials, (int)g, out, s);
@@ -209,8 +247,9 @@ hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4
 
 hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
                               uint4* nb, uint4* partials, uint4* out, hipStream_t s) {
+    if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_sc_fold_msg<<<g, 256, 0, s>>>(f, b, n, r, nf, nb, partials);
+    k_sc_fold_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, r, nf, nb, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_xor_reduce(partials, (int)g, out, s);
@@ -218,8 +257,9 @@ hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 
 
 hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
                            uint4* e_next, uint4* partials, uint4* out, hipStream_t s) {
+    if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_zc_round<<<g, 256, 0, s>>>(a, b, n, e, e_next, partials);
+    k_zc_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, e, e_next, partials);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     return launch_xor_reduce(partials, (int)g, out, s);
@@ -228,8 +268,9 @@ hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uin
 hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
                                 uint4* partials, uint4* out, hipStream_t s) {
+    if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_zc_fold_round<<<g, 256, 0, s>>>(a, b, n, rho, ao, bo, e, e_next, partials);
+    k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, ao, bo, e, e_next, partials);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return err;
     return launch_xor_reduce(partials, (int)g, out, s);

@@ -89,10 +89,13 @@ def load():
         "flock_hip_f128_mul_batch": (I, [P, P, P, P, S]),
         "flock_hip_f128_mul_const_batch": (I, [P, P, F128, P, S]),
         "flock_hip_ntt_forward_interleaved": (I, [P, P, S, S, U32, U32]),
+        "flock_hip_ntt_pass": (I, [P, P, S, S, U32, U32, U32]),
         "flock_hip_pcs_sizes": (I, [PP, P, P, P, P, P]),
         "flock_hip_commit_lane_major": (I, [P, P, S, PP, P, P, U32, P]),
         "flock_hip_commit": (I, [P, P, S, PP, P, P, U32, P]),
         "flock_hip_merkle_tree": (I, [P, P, S, S, P]),
+        "flock_hip_encode": (I, [P, P, S, PP, P]),
+        "flock_hip_encode_lane_major": (I, [P, P, S, PP, P]),
         "flock_hip_sumcheck_round_msg": (I, [P, P, P, S, P]),
         "flock_hip_sumcheck_fold_msg": (I, [P, P, P, S, F128, P, P, P]),
         "flock_hip_build_eq": (I, [P, P, S, P]),
@@ -241,6 +244,11 @@ class Context:
         _check(load().flock_hip_ntt_forward_interleaved(self.h, data.ptr, num_ntts, live, log_d,
                                                         start_layer))
 
+    def ntt_pass(self, data: DeviceBuffer, num_ntts: int, live: int, log_d: int,
+                 first_layer: int, layers: int):
+        _check(load().flock_hip_ntt_pass(self.h, data.ptr, num_ntts, live, log_d, first_layer,
+                                         layers))
+
     def merkle_tree(self, data: DeviceBuffer, data_len: int, n_leaves: int, tree: DeviceBuffer):
         _check(load().flock_hip_merkle_tree(self.h, data.ptr, data_len, n_leaves, tree.ptr))
 
@@ -250,6 +258,14 @@ class Context:
         _check(load().flock_hip_commit_lane_major(self.h, q.ptr, q_len, ctypes.byref(params),
                                                   codeword.ptr, tree.ptr, cap_depth, cap))
         return [cap.raw[32 * i:32 * i + 32] for i in range(1 << cap_depth)]
+
+    def encode(self, msg: DeviceBuffer, msg_len: int, params: PcsParams, codeword: DeviceBuffer):
+        _check(load().flock_hip_encode(self.h, msg.ptr, msg_len, ctypes.byref(params), codeword.ptr))
+
+    def encode_lane_major(self, q: DeviceBuffer, q_len: int, params: PcsParams,
+                          codeword: DeviceBuffer):
+        _check(load().flock_hip_encode_lane_major(self.h, q.ptr, q_len, ctypes.byref(params),
+                                                  codeword.ptr))
 
     def commit(self, msg: DeviceBuffer, msg_len: int, params: PcsParams, codeword: DeviceBuffer,
                tree: DeviceBuffer, cap_depth: int = 0) -> list[bytes]:

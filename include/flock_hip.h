@@ -81,6 +81,13 @@ int flock_hip_f128_mul_const_batch(flock_hip_ctx* ctx, const flock_f128* d_a, fl
 int flock_hip_ntt_forward_interleaved(flock_hip_ctx* ctx, flock_f128* d_data, size_t num_ntts,
                                       size_t live, uint32_t log_d, uint32_t start_layer);
 
+/* One fused pass of the same transform: layers [first_layer, first_layer + layers) with
+ * layers in {1, 2, 3}, in place. flock_hip_ntt_forward_interleaved is a sequence of these
+ * passes (3-layer passes, then 2-layer passes for a remainder); exposed so callers can
+ * schedule or time passes individually. */
+int flock_hip_ntt_pass(flock_hip_ctx* ctx, flock_f128* d_data, size_t num_ntts, size_t live,
+                       uint32_t log_d, uint32_t first_layer, uint32_t layers);
+
 /* ---- PCS commit -------------------------------------------------------------- */
 
 /* PcsParams (pcs/commit.rs:31-65). num_lanes = 0 means None (all 2^log_batch_size lanes).
@@ -110,6 +117,15 @@ int flock_hip_commit_lane_major(flock_hip_ctx* ctx, const flock_f128* d_q, size_
 int flock_hip_commit(flock_hip_ctx* ctx, const flock_f128* d_msg, size_t msg_len,
                      const flock_pcs_params* p, flock_f128* d_codeword, uint8_t* d_tree,
                      uint32_t cap_depth, uint8_t* h_cap);
+
+/* The RS-encoding half of commit / commit_lane_major, without the Merkle tree:
+ * replicate_message_fill + forward_transform_interleaved_live_from_layer from layer
+ * log_inv_rate (pcs/commit.rs:374-397, 583-588; lane-major fill :516-536). d_codeword
+ * receives cw_len words. */
+int flock_hip_encode(flock_hip_ctx* ctx, const flock_f128* d_msg, size_t msg_len,
+                     const flock_pcs_params* p, flock_f128* d_codeword);
+int flock_hip_encode_lane_major(flock_hip_ctx* ctx, const flock_f128* d_q, size_t q_len,
+                                const flock_pcs_params* p, flock_f128* d_codeword);
 
 /* merkle::merkle_tree with HashKind::Sha256 (merkle.rs:543-589): n_leaves (a power of two)
  * equal leaves of data_len / n_leaves bytes (a multiple of 16). */

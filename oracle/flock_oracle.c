@@ -735,3 +735,263 @@ int or_sumcheck_transcript(const uint8_t* domain, size_t dlen, const or_f128* a,
     or_fs_free(fs);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* GF(2^8), phi_8                                                             */
+/* ------------------------------------------------------------------------- */
+
+/* clmul8_software + gf8_reduce, gf2_8.rs:102-137 */
+uint8_t or_f8_mul(uint8_t a, uint8_t b) {
+    uint16_t acc = 0;
+    for (int i = 0; i < 8; i++)
+        if ((a >> i) & 1) acc ^= (uint16_t)b << i;
+    uint16_t h = acc >> 8;
+    uint16_t t = (acc & 0xff) ^ h ^ (h << 1) ^ (h << 3) ^ (h << 4);
+    uint16_t h2 = t >> 8;
+    return (uint8_t)((t & 0xff) ^ h2 ^ (h2 << 1) ^ (h2 << 3) ^ (h2 << 4));
+}
+
+uint8_t or_f8_inv(uint8_t a) { /* a^254 */
+    uint8_t r = 1, sq = a;
+    for (int i = 0; i < 8; i++) {
+        if ((0xFE >> i) & 1) r = or_f8_mul(r, sq);
+        sq = or_f8_mul(sq, sq);
+    }
+    return r;
+}
+
+/* phi_8 is the ring homomorphism F8 -> F128 fixed by alpha = phi_8(0x02)
+ * (phi8.rs:28-31); phi_8(v) = sum over set bits b of v of alpha^b. */
+or_f128 or_phi8(uint8_t v) {
+    const or_f128 alpha = {0x6b8330483c2e9849ull, 0x0dcb364640a222feull};
+    or_f128 acc = ZERO, p = ONE;
+    for (int b = 0; b < 8; b++) {
+        if ((v >> b) & 1) acc = add(acc, p);
+        p = or_f128_mul(p, alpha);
+    }
+    return acc;
+}
+
+/* compute_twiddles, ntt.rs:28-63 */
+static void f8_twiddles(int k, uint8_t beta, uint8_t* tw) {
+    if (k == 0) return;
+    int len = 1 << (k - 1);
+    uint8_t layer[128];
+    for (int i = 0; i < len; i++) layer[i] = beta ^ (uint8_t)(2 * i);
+    uint8_t s_root = 1;
+    int write_at = len;
+    for (int i = 0; i < len; i++) tw[write_at - 1 + i] = layer[i];
+    for (int it = 1; it < k; it++) {
+        write_at >>= 1;
+        uint8_t s = layer[1] ^ layer[0];
+        uint8_t next_root = or_f8_mul(s, s) ^ or_f8_mul(s_root, s); /* next_s, ntt.rs:22-25 */
+        for (int i = 0; i < write_at; i++)
+            layer[i] = or_f8_mul(layer[2 * i], layer[2 * i]) ^ or_f8_mul(s_root, layer[2 * i]);
+        len = write_at;
+        s_root = next_root;
+        uint8_t s_inv = or_f8_inv(s_root);
+        for (int j = 0; j < len; j++) tw[write_at - 1 + j] = or_f8_mul(s_inv, layer[j]);
+    }
+}
+
+static void f8_fft_rec(uint8_t* v, int n, const uint8_t* tw, int idx) {
+    if (n == 1) return;
+    int half = n >> 1;
+    uint8_t lam = tw[idx - 1];
+    for (int i = 0; i < half; i++) { /* fft_butterfly, ntt.rs:65-74 */
+        uint8_t w = v[half + i];
+        v[i] ^= or_f8_mul(lam, w);
+        v[half + i] = w ^ v[i];
+    }
+    f8_fft_rec(v, half, tw, 2 * idx);
+    f8_fft_rec(v + half, half, tw, 2 * idx + 1);
+}
+
+static void f8_ifft_rec(uint8_t* v, int n, const uint8_t* tw, int idx) {
+    if (n == 1) return;
+    int half = n >> 1;
+    f8_ifft_rec(v, half, tw, 2 * idx);
+    f8_ifft_rec(v + half, half, tw, 2 * idx + 1);
+    uint8_t lam = tw[idx - 1];
+    for (int i = 0; i < half; i++) { /* ifft_butterfly, ntt.rs:88-95 */
+        v[half + i] ^= v[i];
+        v[i] ^= or_f8_mul(lam, v[half + i]);
+    }
+}
+
+void or_f8_ntt_forward(uint8_t* v, int k, uint8_t beta) {
+    uint8_t tw[256];
+    f8_twiddles(k, beta, tw);
+    if (k > 0) f8_fft_rec(v, 1 << k, tw, 1);
+}
+
+void or_f8_ntt_inverse(uint8_t* v, int k, uint8_t beta) {
+    uint8_t tw[256];
+    f8_twiddles(k, beta, tw);
+    if (k > 0) f8_ifft_rec(v, 1 << k, tw, 1);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Zerocheck front                                                           */
+/* ------------------------------------------------------------------------- */
+static int bit_at(const uint8_t* p, size_t i) { return (p[i >> 3] >> (i & 7)) & 1; }
+
+/* round1_naive, univariate_skip.rs:68-124 */
+void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
+                  or_f128 p_ab[64], or_f128 p_c[64]) {
+    const int ks = 6, ell = 64;
+    size_t nx = (size_t)1 << (m - ks);
+    or_f128* eq = (or_f128*)malloc(sizeof(or_f128) * nx);
+    or_build_eq(r + ks, (size_t)(m - ks), eq);
+    or_f128 phi[256];
+    for (int v = 0; v < 256; v++) phi[v] = or_phi8((uint8_t)v);
+    for (int i = 0; i < ell; i++) p_ab[i] = p_c[i] = ZERO;
+    uint8_t ac[64], bc[64], cc[64];
+    for (size_t x = 0; x < nx; x++) {
+        for (int s = 0; s < ell; s++) {
+            ac[s] = (uint8_t)bit_at(a, x * ell + s);
+            bc[s] = (uint8_t)bit_at(b, x * ell + s);
+            cc[s] = (uint8_t)bit_at(c, x * ell + s);
+        }
+        or_f8_ntt_inverse(ac, ks, 0);
+        or_f8_ntt_forward(ac, ks, (uint8_t)ell);
+        or_f8_ntt_inverse(bc, ks, 0);
+        or_f8_ntt_forward(bc, ks, (uint8_t)ell);
+        or_f8_ntt_inverse(cc, ks, 0);
+        or_f8_ntt_forward(cc, ks, (uint8_t)ell);
+        for (int i = 0; i < ell; i++) {
+            p_ab[i] = add(p_ab[i], or_f128_mul(eq[x], phi[or_f8_mul(ac[i], bc[i])]));
+            p_c[i] = add(p_c[i], or_f128_mul(eq[x], phi[cc[i]]));
+        }
+    }
+    free(eq);
+}
+
+/* Lagrange basis on nodes phi_8(offset + i), i < 64, at z (product form). */
+void or_lagrange_phi8(int offset, or_f128 z, or_f128 w[64]) {
+    or_f128 nodes[64];
+    for (int i = 0; i < 64; i++) nodes[i] = or_phi8((uint8_t)(offset + i));
+    for (int i = 0; i < 64; i++) {
+        or_f128 num = ONE, den = ONE;
+        for (int j = 0; j < 64; j++) {
+            if (j == i) continue;
+            num = or_f128_mul(num, add(z, nodes[j]));
+            den = or_f128_mul(den, add(nodes[i], nodes[j]));
+        }
+        w[i] = or_f128_mul(num, or_f128_inv(den));
+    }
+}
+
+/* fold_at_z_naive, multilinear.rs:290-310 */
+void or_fold_at_z(const uint8_t* bits, int m, or_f128 z, or_f128* out) {
+    or_f128 w[64];
+    or_lagrange_phi8(0, z, w);
+    size_t nx = (size_t)1 << (m - 6);
+    for (size_t x = 0; x < nx; x++) {
+        or_f128 acc = ZERO;
+        for (int s = 0; s < 64; s++)
+            if (bit_at(bits, x * 64 + s)) acc = add(acc, w[s]);
+        out[x] = acc;
+    }
+}
+
+/* small_challenges_ghash / medium_challenges_ghash, univariate_skip_optimized.rs:91-139 */
+void or_zc_inner_constants(or_f128 out[7]) {
+    const uint8_t small[3] = {0xF7, 0x53, 0xB5};
+    for (int i = 0; i < 3; i++) out[i] = or_phi8(small[i]);
+    for (int i = 0; i < 4; i++) {
+        or_f128 g = {1ull << (1 << i), 0}; /* gamma^(2^i), i.e. x^1, x^2, x^4, x^8 */
+        out[3 + i] = or_f128_mul(g, or_f128_inv(add(ONE, g)));
+    }
+}
+
+/* bits_for_numerator, zerocheck.rs:91-94 */
+static uint32_t bits_for(size_t n) {
+    uint32_t b = 0;
+    while (n) {
+        b++;
+        n >>= 1;
+    }
+    return b;
+}
+
+int or_zerocheck_prove(or_fs* fs, const uint8_t* a, const uint8_t* b, const uint8_t* c, int m,
+                       int grinding, or_f128* round1, or_f128* ml, or_f128 finals[3],
+                       uint64_t* nonces, or_f128* claim) {
+    const int ks = 6, n_inner = 7;
+    if (m < ks + n_inner || m > 40) return -1;
+    const int n_mlv = m - ks;
+    int ni = 0;
+    static const uint8_t LBL[] = "flock-zerocheck-v0";
+    or_fs_observe_label(fs, LBL, sizeof(LBL) - 1);
+    or_f128* r = (or_f128*)malloc(sizeof(or_f128) * (size_t)m);
+    /* r layout: r_skip (6) | 7 inner constants | r_outer (zerocheck.rs:644-669) */
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, bits_for((size_t)m));
+    or_fs_sample_f128_vec(fs, r, ks);
+    or_fs_sample_f128_vec(fs, r + ks + n_inner, (size_t)(m - ks - n_inner));
+    or_zc_inner_constants(r + ks);
+    /* round 1 (naive convention) */
+    or_zc_round1(a, b, c, m, r, round1, round1 + 64);
+    or_fs_observe_f128_slice(fs, round1, 64);
+    or_fs_observe_f128_slice(fs, round1 + 64, 64);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, bits_for((1u << (ks + 1)) - 1));
+    or_f128 z = or_fs_sample_f128(fs);
+    /* c-claim: interpolate round1_c on Lambda at z (multilinear.rs:217-228) */
+    or_f128 wl[64];
+    or_lagrange_phi8(64, z, wl);
+    or_f128 c_eval = ZERO;
+    for (int i = 0; i < 64; i++) c_eval = add(c_eval, or_f128_mul(wl[i], round1[64 + i]));
+    /* round 2: fold at z, first multilinear message with mlv_arg = [ONE, r[ks+1..]] */
+    size_t n = (size_t)1 << n_mlv;
+    or_f128* A = (or_f128*)malloc(sizeof(or_f128) * n);
+    or_f128* B = (or_f128*)malloc(sizeof(or_f128) * n);
+    or_f128* A2 = (or_f128*)malloc(sizeof(or_f128) * (n / 2 + 1));
+    or_f128* B2 = (or_f128*)malloc(sizeof(or_f128) * (n / 2 + 1));
+    or_f128* rr = (or_f128*)malloc(sizeof(or_f128) * (size_t)(n_mlv + 1));
+    or_fold_at_z(a, m, z, A);
+    or_fold_at_z(b, m, z, B);
+    rr[0] = ONE;
+    for (int k = 1; k < n_mlv; k++) rr[k] = r[ks + k];
+    or_f128 m2[2];
+    or_round_pair(A, B, n, rr, m2);
+    int mi = 0;
+    ml[mi++] = m2[0];
+    ml[mi++] = m2[1];
+    or_fs_observe_f128(fs, m2[0]);
+    or_fs_observe_f128(fs, m2[1]);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, bits_for(2));
+    or_f128 rho = or_fs_sample_f128(fs);
+    claim[1] = rho;
+    size_t len = n;
+    for (int i = 0; i + 1 < n_mlv; i++) {
+        or_fold_pair(A, B, len, rho, A2, B2);
+        len /= 2;
+        memcpy(A, A2, sizeof(or_f128) * len);
+        memcpy(B, B2, sizeof(or_f128) * len);
+        rr[0] = ONE;
+        for (int k = ks + i + 2; k < m; k++) rr[k - (ks + i + 2) + 1] = r[k];
+        or_round_pair(A, B, len, rr, m2);
+        ml[mi++] = m2[0];
+        ml[mi++] = m2[1];
+        or_fs_observe_f128(fs, m2[0]);
+        or_fs_observe_f128(fs, m2[1]);
+        if (grinding) nonces[ni++] = or_fs_grind_pow(fs, bits_for(2));
+        rho = or_fs_sample_f128(fs);
+        claim[2 + i] = rho;
+    }
+    or_fold_pair(A, B, len, rho, A2, B2);
+    finals[0] = A2[0];
+    finals[1] = B2[0];
+    finals[2] = c_eval;
+    or_fs_observe_f128(fs, A2[0]);
+    or_fs_observe_f128(fs, B2[0]);
+    claim[0] = z;
+    for (int k = 0; k < n_mlv; k++) claim[1 + n_mlv + k] = r[ks + k];
+    free(r);
+    free(A);
+    free(B);
+    free(A2);
+    free(B2);
+    free(rr);
+    return 0;
+}

@@ -96,6 +96,32 @@ void or_fold_pair(const or_f128* a, const or_f128* b, size_t n, or_f128 c, or_f1
 int or_sumcheck_transcript(const uint8_t* domain, size_t dlen, const or_f128* a, const or_f128* b,
                            int log_n, or_f128* msgs, or_f128 finals[4]);
 
+/* ---- GF(2^8) and the embedding phi_8 (crates/flock-core/src/field/{gf2_8,phi8}.rs) ---- */
+uint8_t or_f8_mul(uint8_t a, uint8_t b);   /* AES polynomial 0x11B, gf2_8.rs:60-137 */
+uint8_t or_f8_inv(uint8_t a);              /* gf2_8.rs:26-37 */
+or_f128 or_phi8(uint8_t v);                /* PHI_8_TABLE[v], phi8.rs:19-1050 */
+/* GF(2^8) additive NTT (crates/flock-core/src/ntt.rs:20-165): domain beta + span{1..2^(k-1)} */
+void or_f8_ntt_forward(uint8_t* v, int k, uint8_t beta);
+void or_f8_ntt_inverse(uint8_t* v, int k, uint8_t beta);
+
+/* ---- zerocheck front (crates/flock-core/src/zerocheck/*.rs) ---- */
+/* round1_naive (univariate_skip.rs:68-124) on LSB-first packed bits of length 2^m;
+ * r has m entries (only r[6..] used). p_ab, p_c: 64 values on Lambda. */
+void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
+                  or_f128 p_ab[64], or_f128 p_c[64]);
+/* Lagrange weights on phi_8(offset .. offset+63) at z (multilinear.rs:190-210) */
+void or_lagrange_phi8(int offset, or_f128 z, or_f128 w[64]);
+/* fold_at_z_naive (multilinear.rs:290-310) on packed bits: out[x] = sum_s bit(x*64+s) L_s(z) */
+void or_fold_at_z(const uint8_t* bits, int m, or_f128 z, or_f128* out);
+/* The fixed inner eq constants r[6..13) (univariate_skip_optimized.rs:91-139). */
+void or_zc_inner_constants(or_f128 out[7]);
+/* prove_packed_padded_inner (zerocheck.rs:618-1011), dense padding, naive round 1.
+ * Outputs: round1[128] = (round1_ab, round1_c); ml[2*(m-6)]; finals[3] = (a, b, c);
+ * nonces[2+m-6] (if grinding); claim[1 + 2*(m-6)] = (z, mlv_challenges, r_rest[1..]). */
+int or_zerocheck_prove(or_fs* fs, const uint8_t* a, const uint8_t* b, const uint8_t* c, int m,
+                       int grinding, or_f128* round1, or_f128* ml, or_f128 finals[3],
+                       uint64_t* nonces, or_f128* claim);
+
 #ifdef __cplusplus
 }
 #endif

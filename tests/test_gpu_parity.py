@@ -80,6 +80,21 @@ def test_ntt_forward_interleaved(ctx, log_d, t, live, start):
     assert np.array_equal(d.download_f128(n), exp)
 
 
+def test_ntt_single_passes_compose(ctx):
+    say("case: ntt single passes 3+3+1+2")
+    log_d, t, start = 11, 24, 2
+    n = (1 << log_d) * t
+    data = O.splitmix(0x5A55, n)
+    d = ctx.upload(data)
+    l = start
+    for k in (3, 3, 1, 2):
+        ctx.ntt_pass(d, t, t, log_d, l, k)
+        l += k
+    assert np.array_equal(d.download_f128(n), O.ntt_forward_interleaved(data, t, t, log_d, log_d, start))
+    with pytest.raises(F.FlockHipError):
+        ctx.ntt_pass(d, t, t, log_d, 9, 3)  # runs past the last layer
+
+
 # ----------------------------------------------------------------- Merkle ----
 
 @pytest.mark.parametrize("n_leaves,leaf", [(1, 16), (2, 32), (4, 48), (256, 64), (512, 800),
@@ -148,6 +163,21 @@ def test_commit_edge_shapes(ctx):
     for m, rate, lb, nl in [(7, 1, 0, 0), (9, 3, 2, 1), (14, 2, 5, 17), (16, 1, 3, 0)]:
         say(f"case: commit edge m={m} rate={rate} lb={lb} nl={nl}")
         _commit_vs_oracle(ctx, m, rate, lb, nl, m * 31 + nl, lane_major=True, cap_depth=1)
+
+
+def test_encode_matches_commit_codeword(ctx):
+    say("case: encode / encode_lane_major")
+    m, rate, lb, nl = 18, 2, 4, 11
+    p = F.PcsParams(m, rate, lb, nl, 0)
+    s = F.pcs_sizes(p)
+    q = O.splitmix(0xE1, 1 << (m - 7))
+    q[nl * ((1 << (m - 7)) >> lb):] = 0
+    cw = ctx.alloc(16 * s["cw_len"])
+    ctx.encode_lane_major(ctx.upload(q), q.shape[0], p, cw)
+    assert np.array_equal(cw.download_f128(s["cw_len"]), O.commit_lane_major(q, m, rate, lb, nl)[0])
+    msg = O.splitmix(0xE2, s["msg_len"])
+    ctx.encode(ctx.upload(msg), s["msg_len"], p, cw)
+    assert np.array_equal(cw.download_f128(s["cw_len"]), O.commit(msg, m, rate, lb, nl)[0])
 
 
 def test_commit_rejects_bad_input(ctx):
