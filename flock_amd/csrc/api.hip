@@ -136,6 +136,78 @@ std::vector<F> ntt_evals(int L) {
 }
 
 // ---------------------------------------------------------------------------
+// Host GF(2^8) (AES polynomial x^8+x^4+x^3+x+1), phi_8 and Lagrange weights (setup work).
+// ---------------------------------------------------------------------------
+uint8_t f8_mul(uint8_t a, uint8_t b) {
+    uint16_t r = 0;
+    for (int i = 0; i < 8; i++)
+        if ((a >> i) & 1) r ^= (uint16_t)b << i;
+    for (int bit = 14; bit >= 8; bit--)
+        if ((r >> bit) & 1) r ^= (uint16_t)(0x11B << (bit - 8));
+    return (uint8_t)r;
+}
+uint8_t f8_inv(uint8_t a) {
+    uint8_t r = 1, sq = a;
+    for (int i = 0; i < 8; i++) {
+        if ((0xFE >> i) & 1) r = f8_mul(r, sq);
+        sq = f8_mul(sq, sq);
+    }
+    return r;
+}
+
+// phi_8(v) = sum over set bits b of v of alpha^b, alpha = phi_8(0x02) (the F8 -> F128
+// embedding of crates/flock-core/src/field/phi8.rs, fixed by its image of x).
+const F* phi8_table() {
+    static F table[256];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const F alpha{0x6b8330483c2e9849ull, 0x0dcb364640a222feull};
+        F pw[8];
+        pw[0] = F{1, 0};
+        for (int b = 1; b < 8; b++) pw[b] = fmul(pw[b - 1], alpha);
+        for (int v = 0; v < 256; v++) {
+            F acc{0, 0};
+            for (int b = 0; b < 8; b++)
+                if ((v >> b) & 1) acc = fadd(acc, pw[b]);
+            table[v] = acc;
+        }
+    });
+    return table;
+}
+
+// Lagrange basis L_i(z) on the 64 nodes phi8(offset + i) (offset 0: S, 64: Lambda).
+// Closed form L_i(z) = Z(z) / ((z + node_i) * D), Z(z) = prod (z + node_j), D = prod of
+// phi8(v), v = 1..63 (the same for S and its coset Lambda); indicator when z is a node.
+void lagrange_phi8(int offset, F z, F w[64]) {
+    const F* phi = phi8_table();
+    F diff[64];
+    F Z{1, 0};
+    int hit = -1;
+    for (int i = 0; i < 64; i++) {
+        diff[i] = fadd(z, phi[offset + i]);
+        if (diff[i].lo == 0 && diff[i].hi == 0) hit = i;
+        Z = fmul(Z, diff[i]);
+    }
+    if (hit >= 0) {
+        for (int i = 0; i < 64; i++) w[i] = F{(uint64_t)(i == hit), 0};
+        return;
+    }
+    // batch inversion of diff[i] * D
+    F D{1, 0};
+    for (int v = 1; v < 64; v++) D = fmul(D, phi[v]);
+    F pre[65];
+    pre[0] = F{1, 0};
+    for (int i = 0; i < 64; i++) pre[i + 1] = fmul(pre[i], diff[i]);
+    F inv_all = finv(fmul(pre[64], D));  // 1 / (D * prod diff)
+    F suf{1, 0};
+    for (int i = 63; i >= 0; i--) {
+        const F inv_i = fmul(fmul(pre[i], suf), inv_all);  // 1 / (diff_i * D)
+        w[i] = fmul(Z, inv_i);
+        suf = fmul(suf, diff[i]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // SHA-256 with a copyable running state (the transcript hash).
 // ---------------------------------------------------------------------------
 const uint32_t K256[64] = {
@@ -336,6 +408,63 @@ struct flock_hip_ctx {
     std::map<int, uint4*> evals;  // k -> k x k basis table
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // zerocheck round-1 constants (device): extension table, log/exp, phi8(2^b)
+    uint4* zc_ext = nullptr;
+    uint8_t* zc_logexp = nullptr;
+    uint4* zc_phipow = nullptr;
+    uint4* zc_partials = nullptr;  // 512 groups x 1024 accumulators
+    uint4* zc_out = nullptr;       // 128 words
+
+    int ensure_zc() {
+        if (zc_ext) return 0;
+        // Extension matrix: M[s][i] = L_s(64 + i) over F8, nodes S = {0..63}: the S -> Lambda
+        // extension of the unit vector e_s (inverse NTT on S, forward on Lambda).
+        static uint8_t M[64][64];
+        for (int s0 = 0; s0 < 64; s0++) {
+            uint8_t den = 1;
+            for (int j = 0; j < 64; j++)
+                if (j != s0) den = f8_mul(den, (uint8_t)(s0 ^ j));
+            const uint8_t dinv = f8_inv(den);
+            for (int i = 0; i < 64; i++) {
+                uint8_t num = 1;
+                const int x = 64 + i;
+                for (int j = 0; j < 64; j++)
+                    if (j != s0) num = f8_mul(num, (uint8_t)(x ^ j));
+                M[s0][i] = f8_mul(num, dinv);
+            }
+        }
+        // nibble tables: entry (g, v) = XOR of M[4g + b] over set bits b of v (64 bytes)
+        std::vector<uint8_t> ext(16 * 16 * 64, 0);
+        for (int g = 0; g < 16; g++)
+            for (int v = 0; v < 16; v++)
+                for (int bb = 0; bb < 4; bb++)
+                    if ((v >> bb) & 1)
+                        for (int i = 0; i < 64; i++) ext[(g * 16 + v) * 64 + i] ^= M[4 * g + bb][i];
+        // log/exp with generator 0x03; exp doubled so log[u] + log[v] needs no reduction
+        std::vector<uint8_t> le(256 + 512, 0);
+        uint8_t x = 1;
+        for (int i = 0; i < 255; i++) {
+            le[256 + i] = x;
+            le[256 + 255 + i] = x;
+            le[x] = (uint8_t)i;
+            x = f8_mul(x, 3);
+        }
+        const F* phi = phi8_table();
+        F pp[8];
+        for (int bb = 0; bb < 8; bb++) pp[bb] = phi[1 << bb];
+        if (hipMalloc(&zc_ext, ext.size()) != hipSuccess ||
+            hipMalloc(&zc_logexp, le.size()) != hipSuccess ||
+            hipMalloc(&zc_phipow, sizeof(pp)) != hipSuccess ||
+            hipMalloc(&zc_partials, sizeof(uint4) * 512 * 1024) != hipSuccess ||
+            hipMalloc(&zc_out, sizeof(uint4) * 128) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FLOCK_HIP_E_NOMEM, "zerocheck tables");
+        }
+        HIP_TRY(hipMemcpy(zc_ext, ext.data(), ext.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(zc_logexp, le.data(), le.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(zc_phipow, pp, sizeof(pp), hipMemcpyHostToDevice));
+        return 0;
+    }
 
     int ensure_scratch(size_t bytes) {
         if (scratch_bytes >= bytes) return 0;
@@ -438,6 +567,11 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& kv : c->evals) hipFree(kv.second);
+    if (c->zc_ext) hipFree(c->zc_ext);
+    if (c->zc_logexp) hipFree(c->zc_logexp);
+    if (c->zc_phipow) hipFree(c->zc_phipow);
+    if (c->zc_partials) hipFree(c->zc_partials);
+    if (c->zc_out) hipFree(c->zc_out);
     if (c->scratch) hipFree(c->scratch);
     if (c->partials) hipFree(c->partials);
     if (c->dout) hipFree(c->dout);
@@ -719,6 +853,202 @@ int flock_hip_fold_pair(flock_hip_ctx* c, const flock_f128* a, const flock_f128*
     if (int rc = check_n(n, 2)) return rc;
     if (!a || !b || !ao || !bo) return fail(FLOCK_HIP_E_INVALID, "null buffer");
     LAUNCH(flock::launch_fold_pair(U(a), U(b), n, U(cc), U(ao), U(bo), c->stream));
+    return 0;
+}
+
+// ---- zerocheck front ----
+static int zc_round1_impl(flock_hip_ctx* c, const uint8_t* a, const uint8_t* b, const uint8_t* cc,
+                          uint32_t m, const uint4* eq, flock_f128 out[128]) {
+    if (int rc = c->ensure_zc()) return rc;
+    const uint64_t rows = 1ull << (m - 6);
+    LAUNCH(flock::launch_zc_round1(a, b, cc, eq, rows, c->zc_ext, c->zc_logexp, c->zc_phipow,
+                                   c->zc_partials, c->zc_out, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, c->zc_out, 128 * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int flock_hip_zc_round1(flock_hip_ctx* c, const uint8_t* a, const uint8_t* b, const uint8_t* cc,
+                        uint32_t m, const flock_f128* eq, flock_f128 out[128]) {
+    CTX_CHECK(c);
+    if (!a || !b || !cc || !eq || !out || m < 6 || m > 40)
+        return fail(FLOCK_HIP_E_INVALID, "bad zerocheck round-1 arguments (need 6 <= m <= 40)");
+    return zc_round1_impl(c, a, b, cc, m, U(eq), out);
+}
+
+// 8 x 256 fold table at z: T[j][v] = sum of L_{8j+b}(z) over set bits b of v.
+static void fold_table_at(F z, std::vector<F>& t) {
+    F w[64];
+    lagrange_phi8(0, z, w);
+    t.assign(8 * 256, F{0, 0});
+    for (int j = 0; j < 8; j++)
+        for (int v = 1; v < 256; v++) {
+            const int low = v & -v;
+            const int bit = __builtin_ctz(v);
+            t[j * 256 + v] = fadd(t[j * 256 + (v ^ low)], w[8 * j + bit]);
+        }
+}
+
+int flock_hip_zc_fold_at_z(flock_hip_ctx* c, const uint8_t* a, const uint8_t* b, uint32_t m,
+                           flock_f128 z, flock_f128* az, flock_f128* bz) {
+    CTX_CHECK(c);
+    if (!a || !b || !az || !bz || m < 6 || m > 40)
+        return fail(FLOCK_HIP_E_INVALID, "bad fold-at-z arguments (need 6 <= m <= 40)");
+    std::vector<F> t;
+    fold_table_at(F{z.lo, z.hi}, t);
+    if (int rc = c->ensure_scratch(t.size() * 16)) return rc;
+    HIP_TRY(hipMemcpyAsync(c->scratch, t.data(), t.size() * 16, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(flock::launch_zc_fold_at_z(a, b, 1ull << (m - 6), reinterpret_cast<uint4*>(c->scratch),
+                                      U(az), U(bz), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static uint32_t bits_for(uint64_t n) {  // floor(log2 n) + 1, zerocheck.rs:91-94
+    uint32_t b = 0;
+    while (n) {
+        b++;
+        n >>= 1;
+    }
+    return b;
+}
+
+static const uint8_t kLabelZcDriver[] = "flock-zerocheck-v0";
+
+int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a,
+                              const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
+                              flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
+                              uint64_t* h_nonces, flock_f128* h_claim) {
+    CTX_CHECK(c);
+    const int ks = 6, n_inner = 7;
+    if (!fs || !d_a || !d_b || !d_c || !h_round1 || !h_ml || !h_finals || !h_claim ||
+        (grinding && !h_nonces) || m < (uint32_t)(ks + n_inner) || m > 40)
+        return fail(FLOCK_HIP_E_INVALID, "bad zerocheck arguments (need 13 <= m <= 40)");
+    if (int rc = c->ensure_zc()) return rc;
+    const uint32_t n_mlv = m - ks;
+    const uint64_t N = 1ull << n_mlv;  // rows = round-2 table length
+    // scratch: Az, Bz (N each), ping-pong P0 (2 x N/2), P1 (2 x N/4), eq EA (N), EB (N/2),
+    // EC (N/4), fold table (2048), r (64), build_eq scratch
+    const uint64_t h = N / 2, q = N / 4 > 0 ? N / 4 : 1;
+    const uint64_t eqs = 2 * (1ull << (n_mlv / 2 + 1));
+    const uint64_t words = 2 * N + 2 * h + 2 * q + N + h + q + 2048 + 64 + eqs;
+    if (int rc = c->ensure_scratch(words * 16)) return rc;
+    uint4* base = reinterpret_cast<uint4*>(c->scratch);
+    uint4* Az = base;
+    uint4* Bz = Az + N;
+    uint4* P0f = Bz + N;
+    uint4* P0b = P0f + h;
+    uint4* P1f = P0b + h;
+    uint4* P1b = P1f + q;
+    uint4* EA = P1b + q;
+    uint4* EB = EA + N;
+    uint4* EC = EB + h;
+    uint4* TBL = EC + q;
+    uint4* R = TBL + 2048;
+    uint4* EQS = R + 64;
+    uint4* bufs[2][2] = {{P0f, P0b}, {P1f, P1b}};
+    uint4* eqb[2] = {EB, EC};
+
+    // ---- 1. r: r_skip | 7 fixed inner constants | r_outer (zerocheck.rs:644-669) ----
+    int ni = 0;
+    flock_fs_observe_label(fs, kLabelZcDriver, sizeof(kLabelZcDriver) - 1);
+    std::vector<flock_f128> r(m);
+    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(m));
+    flock_fs_sample_f128_vec(fs, r.data(), ks);
+    flock_fs_sample_f128_vec(fs, r.data() + ks + n_inner, m - ks - n_inner);
+    {
+        const F* phi = phi8_table();
+        const uint8_t small[3] = {0xF7, 0x53, 0xB5};  // univariate_skip_optimized.rs:81
+        for (int i = 0; i < 3; i++) r[ks + i] = flock_f128{phi[small[i]].lo, phi[small[i]].hi};
+        for (int i = 0; i < 4; i++) {  // gamma^(2^i) / (1 + gamma^(2^i)), :109-139
+            const F g{1ull << (1 << i), 0};
+            const F v = fmul(g, finv(fadd(F{1, 0}, g)));
+            r[ks + 3 + i] = flock_f128{v.lo, v.hi};
+        }
+    }
+    // eq over r[6..m) (2^(m-6) words)
+    HIP_TRY(hipMemcpyAsync(R, r.data() + ks, n_mlv * 16, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(flock::launch_build_eq(R, (int)n_mlv, EA, EQS, c->stream));
+
+    // ---- 2. round 1 ----
+    if (int rc = zc_round1_impl(c, d_a, d_b, d_c, m, EA, h_round1)) return rc;
+    flock_fs_observe_f128_slice(fs, h_round1, 64);
+    flock_fs_observe_f128_slice(fs, h_round1 + 64, 64);
+    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for((1u << (ks + 1)) - 1));
+    flock_f128 z;
+    flock_fs_sample_f128(fs, &z);
+    // c-claim: round1_c interpolated on Lambda at z (multilinear.rs:217-228)
+    {
+        F w[64];
+        lagrange_phi8(64, F{z.lo, z.hi}, w);
+        F acc{0, 0};
+        for (int i = 0; i < 64; i++) acc = fadd(acc, fmul(w[i], F{h_round1[64 + i].lo, h_round1[64 + i].hi}));
+        h_finals[2] = flock_f128{acc.lo, acc.hi};
+    }
+
+    // ---- 3. round 2: fold at z, first multilinear message with eq over r[7..m) ----
+    {
+        std::vector<F> t;
+        fold_table_at(F{z.lo, z.hi}, t);
+        HIP_TRY(hipMemcpyAsync(TBL, t.data(), t.size() * 16, hipMemcpyHostToDevice, c->stream));
+        LAUNCH(flock::launch_zc_fold_at_z(d_a, d_b, N, TBL, Az, Bz, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));  // t is host memory
+    }
+    LAUNCH(flock::launch_eq_fold(EA, h, EB, c->stream));
+    flock_f128 m2[2];
+    LAUNCH(flock::launch_zc_round(Az, Bz, N, EB, N >= 4 ? EC : nullptr, c->partials, c->dout,
+                                  c->stream));
+    if (int rc = c->fetch_pair(m2)) return rc;
+    int mi = 0;
+    h_ml[mi++] = m2[0];  // mlv_arg[0] = ONE
+    h_ml[mi++] = m2[1];
+    flock_fs_observe_f128(fs, m2[0]);
+    flock_fs_observe_f128(fs, m2[1]);
+    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(2));
+    flock_f128 rho;
+    flock_fs_sample_f128(fs, &rho);
+    h_claim[1] = rho;
+
+    // ---- 4. tail: fold at rho, next message (zerocheck.rs:790-947) ----
+    const uint4* cf = Az;
+    const uint4* cb = Bz;
+    uint64_t len = N;
+    int pp = 0, ep = 1;
+    for (uint32_t i = 0; i + 1 < n_mlv; i++) {
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_zc_fold_round(cf, cb, len, U(rho), nf, nb, eqb[ep],
+                                           len >= 8 ? eqb[ep ^ 1] : nullptr, c->partials,
+                                           c->dout, c->stream));
+        if (int rc = c->fetch_pair(m2)) return rc;
+        h_ml[mi++] = m2[0];
+        h_ml[mi++] = m2[1];
+        flock_fs_observe_f128(fs, m2[0]);
+        flock_fs_observe_f128(fs, m2[1]);
+        if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(2));
+        flock_fs_sample_f128(fs, &rho);
+        h_claim[2 + i] = rho;
+        cf = nf;
+        cb = nb;
+        len /= 2;
+        pp ^= 1;
+        ep ^= 1;
+    }
+    // ---- 5. final binding and observe (zerocheck.rs:955-975) ----
+    {
+        uint4* nf = bufs[pp][0];
+        uint4* nb = bufs[pp][1];
+        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(rho), nf, nb, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        h_finals[0] = c->hout[0];
+        h_finals[1] = c->hout[1];
+        flock_fs_observe_f128(fs, h_finals[0]);
+        flock_fs_observe_f128(fs, h_finals[1]);
+    }
+    h_claim[0] = z;
+    for (uint32_t k = 0; k < n_mlv; k++) h_claim[1 + n_mlv + k] = r[ks + k];
     return 0;
 }
 

@@ -65,4 +65,18 @@ hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c,
 // at least 2^(nv/2) + 2^(nv - nv/2) entries.
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s);
 
+// ---- zerocheck.hip ----
+// Round 1 on packed bit rows (2^(m-6) rows of 64 bits): out[0..64) = p_ab, out[64..128) = p_c.
+// eq: 2^(m-6) words; ext: 16x16x64-byte extension table; logexp: log[256] + exp[512];
+// phi_pow: phi8(2^b), b < 8; partials >= 512 * 1024 words.
+hipError_t launch_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, const uint4* eq,
+                            uint64_t rows, const uint4* ext, const uint8_t* logexp,
+                            const uint4* phi_pow, uint4* partials, uint4* out, hipStream_t s);
+uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group);
+// Fold at z of two packed bit tables with the 8 x 256 fold table.
+hipError_t launch_zc_fold_at_z(const uint8_t* a, const uint8_t* b, uint64_t rows,
+                               const uint4* table, uint4* az, uint4* bz, hipStream_t s);
+// out[z] = in[2z] ^ in[2z+1].
+hipError_t launch_eq_fold(const uint4* in, uint64_t n_out, uint4* out, hipStream_t s);
+
 }  // namespace flock

@@ -103,6 +103,9 @@ def load():
         "flock_hip_zc_fold_round_pair": (I, [P, P, P, S, F128, P, P, P, F128, P, P]),
         "flock_hip_fold_pair": (I, [P, P, P, S, F128, P, P]),
         "flock_hip_prove_sumcheck": (I, [P, P, P, P, U32, P, P]),
+        "flock_hip_zc_round1": (I, [P, P, P, P, U32, P, P]),
+        "flock_hip_zc_fold_at_z": (I, [P, P, P, U32, F128, P, P]),
+        "flock_hip_zerocheck_prove": (I, [P, P, P, P, P, U32, I, P, P, P, P, P]),
         "flock_fs_new": (P, [ctypes.c_char_p, S]),
         "flock_fs_free": (None, [P]),
         "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
@@ -307,6 +310,32 @@ class Context:
 
     def fold_pair(self, a, b, n, c, ao, bo):
         _check(load().flock_hip_fold_pair(self.h, a.ptr, b.ptr, n, f128(c), ao.ptr, bo.ptr))
+
+    # ---- zerocheck front ----
+    def zc_round1(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, m: int,
+                  eq: DeviceBuffer) -> np.ndarray:
+        out = np.zeros((128, 2), dtype=np.uint64)
+        _check(load().flock_hip_zc_round1(self.h, a.ptr, b.ptr, c.ptr, m, eq.ptr, _np_ptr(out)))
+        return out
+
+    def zc_fold_at_z(self, a: DeviceBuffer, b: DeviceBuffer, m: int, z: int, az: DeviceBuffer,
+                     bz: DeviceBuffer):
+        _check(load().flock_hip_zc_fold_at_z(self.h, a.ptr, b.ptr, m, f128(z), az.ptr, bz.ptr))
+
+    def zerocheck_prove(self, fs: "FsChallenger", a: DeviceBuffer, b: DeviceBuffer,
+                        c: DeviceBuffer, m: int, grinding: bool):
+        """prove_packed_padded_inner (zerocheck.rs:618-1011). Returns (round1[128],
+        multilinear[2(m-6)], finals[3], nonces, claim[1 + 2(m-6)])."""
+        n_mlv = m - 6
+        r1 = np.zeros((128, 2), dtype=np.uint64)
+        ml = np.zeros((2 * n_mlv, 2), dtype=np.uint64)
+        fin = np.zeros((3, 2), dtype=np.uint64)
+        nonces = np.zeros(2 + n_mlv, dtype=np.uint64)
+        claim = np.zeros((1 + 2 * n_mlv, 2), dtype=np.uint64)
+        _check(load().flock_hip_zerocheck_prove(self.h, fs.h, a.ptr, b.ptr, c.ptr, m,
+                                                1 if grinding else 0, _np_ptr(r1), _np_ptr(ml),
+                                                _np_ptr(fin), _np_ptr(nonces), _np_ptr(claim)))
+        return r1, ml, fin, nonces, claim
 
     def prove_sumcheck(self, fs: "FsChallenger", a: DeviceBuffer, b: DeviceBuffer, log_n: int):
         msgs = np.zeros((4 * log_n, 2), dtype=np.uint64)

@@ -162,6 +162,34 @@ int flock_hip_zc_fold_round_pair(flock_hip_ctx* ctx, const flock_f128* d_a,
 int flock_hip_fold_pair(flock_hip_ctx* ctx, const flock_f128* d_a, const flock_f128* d_b,
                         size_t n, flock_f128 c, flock_f128* d_ao, flock_f128* d_bo);
 
+/* ---- zerocheck front --------------------------------------------------------------- */
+
+/* Univariate-skip round 1 (k_skip = 6) with the naive-convention output of
+ * round1_naive (zerocheck/univariate_skip.rs:68-124), which is what
+ * prove_packed_padded_inner sends after its C_s rescale (zerocheck.rs:700-704).
+ * d_a/d_b/d_c: 2^m LSB-first packed bits (2^(m-3) bytes); d_eq = build_eq(r[6..m])
+ * (2^(m-6) words). h_out[0..64) = P^AB on Lambda, h_out[64..128) = P^C on Lambda. */
+int flock_hip_zc_round1(flock_hip_ctx* ctx, const uint8_t* d_a, const uint8_t* d_b,
+                        const uint8_t* d_c, uint32_t m, const flock_f128* d_eq,
+                        flock_f128 h_out[128]);
+/* Round-2 fold at the skip challenge z (fold_at_z_naive, zerocheck/multilinear.rs:290-310;
+ * UniSkipFoldTable, :414-457) of two packed tables into 2^(m-6) words each. */
+int flock_hip_zc_fold_at_z(flock_hip_ctx* ctx, const uint8_t* d_a, const uint8_t* d_b,
+                           uint32_t m, flock_f128 z, flock_f128* d_az, flock_f128* d_bz);
+/* Zerocheck prover for a.b + c = 0 on packed bits, dense padding:
+ * prove_packed_padded_inner (zerocheck.rs:618-1011) with the transcript ops in its order:
+ * label, [PoW] r_skip, r_outer, round 1, [PoW] z, round 2 fused with the first
+ * multilinear message, the tail, final folds. grinding != 0 selects
+ * ZerocheckGrinding::per_challenge_128 (zerocheck.rs:67-158), 0 disabled. Outputs
+ * (caller-allocated): h_round1[128] = (round1_ab, round1_c); h_ml[2*(m-6)] = multilinear
+ * round messages; h_finals[3] = (final_a_eval, final_b_eval, final_c_eval);
+ * h_nonces[2 + m-6] when grinding (may be NULL otherwise); h_claim[1 + 2*(m-6)] =
+ * (z, mlv_challenges, r_rest = r[6..m)) — ZerocheckClaim (zerocheck.rs:405-425). m >= 13. */
+int flock_hip_zerocheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const uint8_t* d_a,
+                              const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
+                              flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
+                              uint64_t* h_nonces, flock_f128* h_claim);
+
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 
 /* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */

@@ -192,3 +192,56 @@ class FsChallenger:
                 nonce += 1
         self.observe_bytes(nonce.to_bytes(8, "little"))
         return nonce
+
+
+# ---------------------------------------------------------------------------
+# GF(2^8) (AES polynomial x^8 + x^4 + x^3 + x + 1) and Lagrange helpers
+# ---------------------------------------------------------------------------
+
+def f8_mul(a: int, b: int) -> int:
+    r = 0
+    for i in range(8):
+        if (a >> i) & 1:
+            r ^= b << i
+    for bit in range(14, 7, -1):
+        if (r >> bit) & 1:
+            r ^= 0x11B << (bit - 8)
+    return r
+
+
+def f8_inv(a: int) -> int:
+    for x in range(1, 256):
+        if f8_mul(a, x) == 1:
+            return x
+    raise ZeroDivisionError
+
+
+def f8_lagrange_extend(vals: list[int], src: list[int], dst: list[int]) -> list[int]:
+    """Evaluate the degree < len(src) polynomial through (src[i], vals[i]) at dst, over F8."""
+    out = []
+    for x in dst:
+        acc = 0
+        for i, xi in enumerate(src):
+            if vals[i] == 0:
+                continue
+            num, den = 1, 1
+            for j, xj in enumerate(src):
+                if j != i:
+                    num = f8_mul(num, x ^ xj)
+                    den = f8_mul(den, xi ^ xj)
+            acc ^= f8_mul(vals[i], f8_mul(num, f8_inv(den)))
+        out.append(acc)
+    return out
+
+
+def lagrange_f128(nodes: list[int], z: int) -> list[int]:
+    """Lagrange basis values L_i(z) on F128 nodes (product form)."""
+    w = []
+    for i, xi in enumerate(nodes):
+        num, den = 1, 1
+        for j, xj in enumerate(nodes):
+            if j != i:
+                num = mul(num, z ^ xj)
+                den = mul(den, xi ^ xj)
+        w.append(mul(num, inv(den)))
+    return w

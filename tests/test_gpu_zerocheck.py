@@ -1,0 +1,71 @@
+"""GPU parity of the zerocheck front (round 1, fold at z, the full prover) against the
+oracle, bit-exact, with and without grinding."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from flock_amd import lib as F
+from test_oracle_zerocheck import L, _witness, zerocheck_prove
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = F.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("m", [6, 7, 13, 15, 18])
+def test_round1(ctx, m):
+    say(f"case: zc round1 m={m}")
+    a, b, c = _witness(m, 100 + m, satisfying=False)
+    r = O.arr_to_ints(O.splitmix(0x200 + m, m))
+    exp = np.zeros((128, 2), dtype=np.uint64)
+    ra = O.ints_to_arr(r)
+    L.or_zc_round1(a.ctypes.data, b.ctypes.data, c.ctypes.data, m, O._p(ra), O._p(exp),
+                   O._p(exp[64:]))
+    eq = ctx.upload(O.build_eq(r[6:]))
+    got = ctx.zc_round1(ctx.upload(a), ctx.upload(b), ctx.upload(c), m, eq)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("m", [6, 9, 14, 17])
+def test_fold_at_z(ctx, m):
+    say(f"case: zc fold at z m={m}")
+    a, b, _ = _witness(m, 200 + m, satisfying=False)
+    z = O.arr_to_ints(O.splitmix(0x300 + m, 1))[0]
+    n = 1 << (m - 6)
+    az, bz = ctx.alloc(16 * n), ctx.alloc(16 * n)
+    ctx.zc_fold_at_z(ctx.upload(a), ctx.upload(b), m, z, az, bz)
+    for bits, dev in ((a, az), (b, bz)):
+        exp = np.zeros((n, 2), dtype=np.uint64)
+        L.or_fold_at_z(bits.ctypes.data, m, O.f(z), O._p(exp))
+        assert np.array_equal(dev.download_f128(n), exp)
+
+
+@pytest.mark.parametrize("m,grinding,satisfying", [(13, True, True), (14, False, False),
+                                                   (17, True, True), (20, True, False),
+                                                   (25, True, True)])
+def test_zerocheck_prove(ctx, m, grinding, satisfying):
+    say(f"case: zerocheck prove m={m} grinding={grinding} satisfying={satisfying}")
+    a, b, c = _witness(m, 300 + m, satisfying=satisfying)
+    dom = b"flock-zc-gpu-test"
+    exp = zerocheck_prove(dom, a, b, c, m, 1 if grinding else 0)
+    got = ctx.zerocheck_prove(F.FsChallenger(dom), ctx.upload(a), ctx.upload(b), ctx.upload(c), m,
+                              grinding)
+    names = ["round1", "multilinear", "finals", "nonces", "claim"]
+    for name, g, e in zip(names, got, exp):
+        if name == "nonces" and not grinding:
+            continue
+        assert np.array_equal(g, e), name
