@@ -904,6 +904,19 @@ int flock_hip_zc_fold_at_z(flock_hip_ctx* c, const uint8_t* a, const uint8_t* b,
     return 0;
 }
 
+int flock_hip_fold_1b_rows(flock_hip_ctx* c, const flock_f128* rows, size_t n_rows,
+                           const flock_f128* t, flock_f128 out[128]) {
+    CTX_CHECK(c);
+    if (!rows || !t || !out || n_rows == 0)
+        return fail(FLOCK_HIP_E_INVALID, "bad fold_1b_rows arguments");
+    if (int rc = c->ensure_zc()) return rc;
+    LAUNCH(flock::launch_fold_1b_rows(U(rows), U(t), n_rows, c->zc_partials, c->zc_out,
+                                      c->stream));
+    HIP_TRY(hipMemcpyAsync(out, c->zc_out, 128 * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 static uint32_t bits_for(uint64_t n) {  // floor(log2 n) + 1, zerocheck.rs:91-94
     uint32_t b = 0;
     while (n) {
@@ -918,7 +931,8 @@ static const uint8_t kLabelZcDriver[] = "flock-zerocheck-v0";
 int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a,
                               const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
                               flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
-                              uint64_t* h_nonces, flock_f128* h_claim) {
+                              uint64_t* h_nonces, flock_f128* h_claim,
+                              flock_f128* h_s_hat_v_c) {
     CTX_CHECK(c);
     const int ks = 6, n_inner = 7;
     if (!fs || !d_a || !d_b || !d_c || !h_round1 || !h_ml || !h_finals || !h_claim ||
@@ -972,6 +986,17 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
 
     // ---- 2. round 1 ----
     if (int rc = zc_round1_impl(c, d_a, d_b, d_c, m, EA, h_round1)) return rc;
+    // eq over r[7..m) = pairwise XOR of eq over r[6..m); it weights round 2 and s_hat_v_c
+    LAUNCH(flock::launch_eq_fold(EA, h, EB, c->stream));
+    if (h_s_hat_v_c) {
+        // s_hat_v_c[lane | b7 << 6] = sum_y eq(r[7..m), y) c_bit(y*128 + b7*64 + lane)
+        // (univariate_skip.rs:354-383): fold_1b_rows of c viewed as 128-bit rows.
+        LAUNCH(flock::launch_fold_1b_rows(reinterpret_cast<const uint4*>(d_c), EB, h,
+                                          c->zc_partials, c->zc_out, c->stream));
+        HIP_TRY(hipMemcpyAsync(h_s_hat_v_c, c->zc_out, 128 * 16, hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     flock_fs_observe_f128_slice(fs, h_round1, 64);
     flock_fs_observe_f128_slice(fs, h_round1 + 64, 64);
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for((1u << (ks + 1)) - 1));
@@ -994,7 +1019,6 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
         LAUNCH(flock::launch_zc_fold_at_z(d_a, d_b, N, TBL, Az, Bz, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));  // t is host memory
     }
-    LAUNCH(flock::launch_eq_fold(EA, h, EB, c->stream));
     flock_f128 m2[2];
     LAUNCH(flock::launch_zc_round(Az, Bz, N, EB, N >= 4 ? EC : nullptr, c->partials, c->dout,
                                   c->stream));

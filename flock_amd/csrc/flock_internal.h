@@ -78,5 +78,8 @@ hipError_t launch_zc_fold_at_z(const uint8_t* a, const uint8_t* b, uint64_t rows
                                const uint4* table, uint4* az, uint4* bz, hipStream_t s);
 // out[z] = in[2z] ^ in[2z+1].
 hipError_t launch_eq_fold(const uint4* in, uint64_t n_out, uint4* out, hipStream_t s);
+// out[j] = sum_y t[y] * bit j of 128-bit row y (128 words; partials: 512 x 256 words).
+hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
+                              uint4* partials, uint4* out, hipStream_t s);
 
 }  // namespace flock

@@ -182,6 +182,60 @@ __global__ __launch_bounds__(256) void k_eq_fold(const f128* __restrict__ in, ui
         out[z] = f128_add(in[2 * z], in[2 * z + 1]);
 }
 
+// fold_1b_rows (crates/flock-core/src/pcs/ring_switch.rs:182-230, scalar form :410-470):
+// out[j] = sum over rows y of t[y] * bit j of the 128-bit row y, j < 128.  Thread t of 256
+// owns column j = t & 127 for the rows of parity t >> 7 in each 256-row batch staged in LDS;
+// a 64-lane wave reads one row word per step (LDS broadcast).
+__global__ __launch_bounds__(256) void k_fold_1b_rows(const uint4* __restrict__ rows_g,
+                                                       const f128* __restrict__ t_g,
+                                                       uint64_t rows, uint32_t rows_per_group,
+                                                       f128* __restrict__ partials) {
+    __shared__ uint4 rw[256];
+    __shared__ f128 tw[256];
+    const int t = threadIdx.x;
+    const int j = t & 127, par = t >> 7, word = j >> 5, bit = j & 31;
+    f128 acc = f128_zero();
+    const uint64_t row0 = (uint64_t)blockIdx.x * rows_per_group;
+    uint64_t row_end = row0 + rows_per_group;
+    if (row_end > rows) row_end = rows;
+    for (uint64_t base = row0; base < row_end; base += 256) {
+        const uint64_t y = base + t;
+        if (y < row_end) {
+            rw[t] = rows_g[y];
+            tw[t] = t_g[y];
+        } else {
+            rw[t] = make_uint4(0u, 0u, 0u, 0u);
+            tw[t] = f128_zero();
+        }
+        __syncthreads();
+        const int n = (int)((row_end - base) < 256u ? (row_end - base) : 256u);
+        for (int rr = par; rr < n; rr += 2) {
+            const uint4 r = rw[rr];
+            const uint32_t w = word == 0 ? r.x : word == 1 ? r.y : word == 2 ? r.z : r.w;
+            const uint32_t msk = 0u - ((w >> bit) & 1u);
+            const f128 e = tw[rr];
+            acc.x ^= e.x & msk;
+            acc.y ^= e.y & msk;
+            acc.z ^= e.z & msk;
+            acc.w ^= e.w & msk;
+        }
+        __syncthreads();
+    }
+    partials[(size_t)blockIdx.x * 256 + t] = acc;
+}
+
+__global__ __launch_bounds__(128) void k_fold_1b_rows_finish(const f128* __restrict__ partials,
+                                                              int groups,
+                                                              f128* __restrict__ out) {
+    const int j = threadIdx.x;
+    f128 acc = f128_zero();
+    for (int g = 0; g < groups; g++) {
+        acc = f128_add(acc, partials[(size_t)g * 256 + j]);
+        acc = f128_add(acc, partials[(size_t)g * 256 + 128 + j]);
+    }
+    out[j] = acc;
+}
+
 uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group) {
     // at most 512 groups; each group walks whole 256-row batches
     uint64_t per = (rows + 511) / 512;
@@ -200,6 +254,17 @@ hipError_t launch_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_zc_round1_finish<<<1, kR1Acc, 0, s>>>(partials, (int)groups, phi_pow, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
+                              uint4* partials, uint4* out, hipStream_t s) {
+    uint32_t per = 0;
+    const uint32_t groups = zc_round1_groups(rows, &per);  // <= 512 groups of 256-row batches
+    k_fold_1b_rows<<<groups, 256, 0, s>>>(rows_g, t, rows, per, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_fold_1b_rows_finish<<<1, 128, 0, s>>>(partials, (int)groups, out);
     return hipGetLastError();
 }
 

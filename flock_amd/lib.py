@@ -105,7 +105,8 @@ def load():
         "flock_hip_prove_sumcheck": (I, [P, P, P, P, U32, P, P]),
         "flock_hip_zc_round1": (I, [P, P, P, P, U32, P, P]),
         "flock_hip_zc_fold_at_z": (I, [P, P, P, U32, F128, P, P]),
-        "flock_hip_zerocheck_prove": (I, [P, P, P, P, P, U32, I, P, P, P, P, P]),
+        "flock_hip_zerocheck_prove": (I, [P, P, P, P, P, U32, I, P, P, P, P, P, P]),
+        "flock_hip_fold_1b_rows": (I, [P, P, S, P, P]),
         "flock_fs_new": (P, [ctypes.c_char_p, S]),
         "flock_fs_free": (None, [P]),
         "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
@@ -323,19 +324,30 @@ class Context:
         _check(load().flock_hip_zc_fold_at_z(self.h, a.ptr, b.ptr, m, f128(z), az.ptr, bz.ptr))
 
     def zerocheck_prove(self, fs: "FsChallenger", a: DeviceBuffer, b: DeviceBuffer,
-                        c: DeviceBuffer, m: int, grinding: bool):
+                        c: DeviceBuffer, m: int, grinding: bool, capture: bool = False):
         """prove_packed_padded_inner (zerocheck.rs:618-1011). Returns (round1[128],
-        multilinear[2(m-6)], finals[3], nonces, claim[1 + 2(m-6)])."""
+        multilinear[2(m-6)], finals[3], nonces, claim[1 + 2(m-6)]), plus s_hat_v_c[128]
+        when capture is set (zerocheck.rs:598-615)."""
         n_mlv = m - 6
         r1 = np.zeros((128, 2), dtype=np.uint64)
         ml = np.zeros((2 * n_mlv, 2), dtype=np.uint64)
         fin = np.zeros((3, 2), dtype=np.uint64)
         nonces = np.zeros(2 + n_mlv, dtype=np.uint64)
         claim = np.zeros((1 + 2 * n_mlv, 2), dtype=np.uint64)
+        shv = np.zeros((128, 2), dtype=np.uint64)
         _check(load().flock_hip_zerocheck_prove(self.h, fs.h, a.ptr, b.ptr, c.ptr, m,
                                                 1 if grinding else 0, _np_ptr(r1), _np_ptr(ml),
-                                                _np_ptr(fin), _np_ptr(nonces), _np_ptr(claim)))
+                                                _np_ptr(fin), _np_ptr(nonces), _np_ptr(claim),
+                                                _np_ptr(shv) if capture else None))
+        if capture:
+            return r1, ml, fin, nonces, claim, shv
         return r1, ml, fin, nonces, claim
+
+    def fold_1b_rows(self, rows: DeviceBuffer, n_rows: int, t: DeviceBuffer) -> np.ndarray:
+        """fold_1b_rows (pcs/ring_switch.rs:182-230): out[j] = sum_y t[y] bit_j(rows[y])."""
+        out = np.zeros((128, 2), dtype=np.uint64)
+        _check(load().flock_hip_fold_1b_rows(self.h, rows.ptr, n_rows, t.ptr, _np_ptr(out)))
+        return out
 
     def prove_sumcheck(self, fs: "FsChallenger", a: DeviceBuffer, b: DeviceBuffer, log_n: int):
         msgs = np.zeros((4 * log_n, 2), dtype=np.uint64)

@@ -184,11 +184,19 @@ int flock_hip_zc_fold_at_z(flock_hip_ctx* ctx, const uint8_t* d_a, const uint8_t
  * (caller-allocated): h_round1[128] = (round1_ab, round1_c); h_ml[2*(m-6)] = multilinear
  * round messages; h_finals[3] = (final_a_eval, final_b_eval, final_c_eval);
  * h_nonces[2 + m-6] when grinding (may be NULL otherwise); h_claim[1 + 2*(m-6)] =
- * (z, mlv_challenges, r_rest = r[6..m)) — ZerocheckClaim (zerocheck.rs:405-425). m >= 13. */
+ * (z, mlv_challenges, r_rest = r[6..m)) — ZerocheckClaim (zerocheck.rs:405-425). m >= 13.
+ * h_s_hat_v_c[128], when not NULL, receives s_hat_v_c
+ * (prove_packed_padded_capture_s_hat_v_c_with_grinding, zerocheck.rs:598-615;
+ * round1_extract_c_packed_with_s_hat_v, zerocheck/univariate_skip.rs:354-489):
+ * s_hat_v_c[lane | b7 << 6] = sum_y eq(r[7..m), y) * bit(c, y*128 + b7*64 + lane). */
 int flock_hip_zerocheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const uint8_t* d_a,
                               const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
                               flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
-                              uint64_t* h_nonces, flock_f128* h_claim);
+                              uint64_t* h_nonces, flock_f128* h_claim, flock_f128* h_s_hat_v_c);
+/* fold_1b_rows (crates/flock-core/src/pcs/ring_switch.rs:182-230, :410-470): out[j] =
+ * sum_y d_t[y] * bit j of the 128-bit row d_rows[y], j < 128 (d_rows and d_t n_rows words). */
+int flock_hip_fold_1b_rows(flock_hip_ctx* ctx, const flock_f128* d_rows, size_t n_rows,
+                           const flock_f128* d_t, flock_f128 h_out[128]);
 
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 

@@ -867,6 +867,38 @@ void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, c
     free(eq);
 }
 
+/* round1_extract_c_packed_with_s_hat_v's c side (univariate_skip.rs:384-489): two banks
+ * keyed by bit 0 of the row index x, each weighted by eq(r[6..m), x), then each divided by
+ * eq(r[6], bank). out[lane | bank << 6]. */
+void or_zc_s_hat_v_c(const uint8_t* c, int m, const or_f128* r, or_f128 out[128]) {
+    const int ks = 6, ell = 64;
+    size_t nx = (size_t)1 << (m - ks);
+    or_f128* eq = (or_f128*)malloc(sizeof(or_f128) * nx);
+    or_build_eq(r + ks, (size_t)(m - ks), eq);
+    or_f128 bank[2][64];
+    for (int i = 0; i < ell; i++) bank[0][i] = bank[1][i] = ZERO;
+    for (size_t x = 0; x < nx; x++)
+        for (int s = 0; s < ell; s++)
+            if (bit_at(c, x * ell + s)) bank[x & 1][s] = add(bank[x & 1][s], eq[x]);
+    const or_f128 inv0 = or_f128_inv(add(ONE, r[ks])), inv1 = or_f128_inv(r[ks]);
+    for (int i = 0; i < ell; i++) {
+        out[i] = or_f128_mul(bank[0][i], inv0);
+        out[ell + i] = or_f128_mul(bank[1][i], inv1);
+    }
+    free(eq);
+}
+
+/* fold_1b_rows scalar form (pcs/ring_switch.rs:182-230, 410-470): out[j] = sum_y t[y] bit j
+ * of the 128-bit row y. */
+void or_fold_1b_rows(const or_f128* rows, const or_f128* t, size_t n, or_f128 out[128]) {
+    for (int j = 0; j < 128; j++) out[j] = ZERO;
+    for (size_t y = 0; y < n; y++)
+        for (int j = 0; j < 128; j++) {
+            const uint64_t w = j < 64 ? rows[y].lo : rows[y].hi;
+            if ((w >> (j & 63)) & 1) out[j] = add(out[j], t[y]);
+        }
+}
+
 /* Lagrange basis on nodes phi_8(offset + i), i < 64, at z (product form). */
 void or_lagrange_phi8(int offset, or_f128 z, or_f128 w[64]) {
     or_f128 nodes[64];
@@ -993,5 +1025,167 @@ int or_zerocheck_prove(or_fs* fs, const uint8_t* a, const uint8_t* b, const uint
     free(A2);
     free(B2);
     free(rr);
+    return 0;
+}
+
+/* ======================================================================== */
+/* Lincheck (crates/flock-core/src/lincheck.rs, lincheck/union.rs)          */
+/* ======================================================================== */
+
+/* build_quirky_eq_table_from_weights (lincheck.rs:1355-1371) with phi_8 skip weights
+ * (build_quirky_eq_table, :1376-1378): out[i_skip + 64 i_rest] = L_i_skip(z) eq(x_rest, i_rest). */
+void or_quirky_eq(or_f128 z_skip, const or_f128* x_rest, size_t n_rest, or_f128* out) {
+    or_f128 w[64];
+    or_lagrange_phi8(0, z_skip, w);
+    size_t nr = (size_t)1 << n_rest;
+    or_f128* er = (or_f128*)malloc(sizeof(or_f128) * nr);
+    or_build_eq(x_rest, n_rest, er);
+    for (size_t j = 0; j < nr; j++)
+        for (int i = 0; i < 64; i++) out[j * 64 + i] = or_f128_mul(w[i], er[j]);
+    free(er);
+}
+
+/* CscCircuit::fold_split (lincheck.rs:463-497): sa[c] = sum of eq over the rows of column c
+ * of A, sb likewise for B. */
+void or_csc_fold_split(size_t n_cols, const uint32_t* a_ptr, const uint32_t* a_rows,
+                       const uint32_t* b_ptr, const uint32_t* b_rows, const or_f128* eq,
+                       or_f128* sa, or_f128* sb) {
+    for (size_t c = 0; c < n_cols; c++) {
+        or_f128 x = ZERO, y = ZERO;
+        for (uint32_t k = a_ptr[c]; k < a_ptr[c + 1]; k++) x = add(x, eq[a_rows[k]]);
+        for (uint32_t k = b_ptr[c]; k < b_ptr[c + 1]; k++) y = add(y, eq[b_rows[k]]);
+        sa[c] = x;
+        sb[c] = y;
+    }
+}
+
+/* partial_fold_packed_z (lincheck.rs:804-835) in the support-proportional form of
+ * partial_fold_packed_z_rows_padded / _fast_padded (:865-990): stripes s < ceil(n_rows/8)
+ * (whole bytes), columns i < useful_bits; out[i] = sum of eq_outer[8s + r] over set bits r of
+ * z[s*k + i]. Columns >= useful_bits are zero. */
+void or_partial_fold(const uint8_t* z, int k_log, int nu, size_t useful_bits, size_t n_rows,
+                     const or_f128* eq_outer, or_f128* out) {
+    size_t k = (size_t)1 << k_log;
+    size_t n_stripes = (n_rows + 7) / 8;
+    (void)nu;
+    for (size_t i = 0; i < k; i++) out[i] = ZERO;
+    for (size_t s = 0; s < n_stripes; s++)
+        for (size_t i = 0; i < useful_bits; i++) {
+            uint8_t byte = z[s * k + i];
+            for (int r = 0; r < 8; r++)
+                if ((byte >> r) & 1) out[i] = add(out[i], eq_outer[8 * s + r]);
+        }
+}
+
+/* sumcheck_round_eval_par (lincheck.rs:1479-1502): top-bit split. */
+void or_lc_round_eval(const or_f128* c, const or_f128* z, size_t len, or_f128 out[2]) {
+    size_t h = len / 2;
+    or_f128 e1 = ZERO, ei = ZERO;
+    for (size_t i = 0; i < h; i++) {
+        e1 = add(e1, or_f128_mul(c[h + i], z[h + i]));
+        ei = add(ei, or_f128_mul(add(c[h + i], c[i]), add(z[h + i], z[i])));
+    }
+    out[0] = e1;
+    out[1] = ei;
+}
+
+/* sumcheck_bind_top_in_place_par (lincheck.rs:1506-1523): v[i] += r (v[i+h] + v[i]). */
+void or_lc_bind_top(or_f128* v, size_t len, or_f128 r) {
+    size_t h = len / 2;
+    for (size_t i = 0; i < h; i++) v[i] = add(v[i], or_f128_mul(r, add(v[h + i], v[i])));
+}
+
+/* Single-slot union lincheck prover: prove_union_capture_z_vec_with_grinding
+ * (lincheck/union.rs:203-430) for a registry with one boolean type filling the boolean
+ * region (the T = 1 path, union.rs:270-300), followed by column_sumcheck_prove
+ * (lincheck.rs:1907-2028; each round evaluated then bound, as in the unfused form).
+ * Grinding (LincheckGrinding::per_challenge_128, lincheck.rs:565-603): alpha 1 bit, beta 1
+ * bit, rounds 2 bits, skip k_skip bits. const_pin < 0 = none.
+ * Outputs: rounds[2L] (L = k_log - 6), z_partial[64], matrix_evals[2], nonces,
+ * claim[2 + L] = (r_inner_skip, w, r_inner_rest[0..L)), z_vec[2^k_log] (captured). */
+int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_t* a_rows,
+                      const uint32_t* b_ptr, const uint32_t* b_rows, long const_pin,
+                      const uint8_t* z, int nu, size_t useful_bits, size_t n_rows,
+                      or_f128 z_skip, const or_f128* x_inner_rest, const or_f128* x_outer,
+                      int grinding, or_f128* rounds, or_f128 z_partial[64],
+                      or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
+                      or_f128* z_vec_out) {
+    const int ks = 6;
+    if (k_log < ks || nu < 3) return 1;
+    const size_t K = (size_t)1 << k_log;
+    const int L = k_log - ks;
+    int ni = 0;
+    static const uint8_t label[] = "flock-lincheck-v0";
+    or_fs_observe_label(fs, label, sizeof(label) - 1);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 1);
+    const or_f128 alpha = or_fs_sample_f128(fs);
+    or_f128* eq_inner = (or_f128*)malloc(sizeof(or_f128) * K);
+    or_f128* sa = (or_f128*)malloc(sizeof(or_f128) * K);
+    or_f128* sb = (or_f128*)malloc(sizeof(or_f128) * K);
+    or_f128* comb = (or_f128*)malloc(sizeof(or_f128) * K);
+    or_f128* zv = (or_f128*)malloc(sizeof(or_f128) * K);
+    or_quirky_eq(z_skip, x_inner_rest, (size_t)L, eq_inner);
+    or_csc_fold_split(K, a_ptr, a_rows, b_ptr, b_rows, eq_inner, sa, sb);
+    for (size_t c = 0; c < K; c++) comb[c] = add(or_f128_mul(alpha, sa[c]), sb[c]);
+    if (const_pin >= 0) {
+        if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 1);
+        const or_f128 beta = or_fs_sample_f128(fs);
+        comb[const_pin] = add(comb[const_pin], beta);
+    }
+    size_t n_outer = (size_t)1 << nu;
+    or_f128* eq_outer = (or_f128*)malloc(sizeof(or_f128) * n_outer);
+    or_build_eq(x_outer, (size_t)nu, eq_outer);
+    or_partial_fold(z, k_log, nu, useful_bits, n_rows, eq_outer, zv);
+    if (z_vec_out) memcpy(z_vec_out, zv, sizeof(or_f128) * K);
+    /* column_sumcheck_prove */
+    or_f128* rr = (or_f128*)malloc(sizeof(or_f128) * (L > 0 ? L : 1));
+    size_t len = K;
+    for (int t = 0; t < L; t++) {
+        or_f128 msg[2];
+        or_lc_round_eval(comb, zv, len, msg);
+        or_fs_observe_f128(fs, msg[0]);
+        or_fs_observe_f128(fs, msg[1]);
+        if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 2);
+        const or_f128 r = or_fs_sample_f128(fs);
+        rounds[2 * t] = msg[0];
+        rounds[2 * t + 1] = msg[1];
+        rr[t] = r;
+        or_lc_bind_top(comb, len, r);
+        or_lc_bind_top(zv, len, r);
+        len /= 2;
+    }
+    for (int i = 0; i < 64; i++) z_partial[i] = zv[i];
+    or_fs_observe_f128_slice(fs, z_partial, 64);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, (uint32_t)ks);
+    const or_f128 r_skip = or_fs_sample_f128(fs);
+    or_f128 lam[64];
+    or_lagrange_phi8(0, r_skip, lam);
+    or_f128 w = ZERO;
+    for (int i = 0; i < 64; i++) w = add(w, or_f128_mul(lam[i], z_partial[i]));
+    claim[0] = r_skip;
+    claim[1] = w;
+    for (int j = 0; j < L; j++) claim[2 + j] = rr[L - 1 - j]; /* LSB-first, lincheck.rs:2007-2013 */
+    /* matrix_evals (union.rs:404-416): W_col = z_partial (x) eq(r_inner_rest) */
+    {
+        or_f128* e = (or_f128*)malloc(sizeof(or_f128) * ((size_t)1 << L));
+        or_build_eq(claim + 2, (size_t)L, e);
+        or_f128 ma = ZERO, mb = ZERO;
+        for (size_t j = 0; j < ((size_t)1 << L); j++)
+            for (int i = 0; i < 64; i++) {
+                const or_f128 wc = or_f128_mul(e[j], z_partial[i]);
+                ma = add(ma, or_f128_mul(wc, sa[j * 64 + i]));
+                mb = add(mb, or_f128_mul(wc, sb[j * 64 + i]));
+            }
+        matrix_evals[0] = ma;
+        matrix_evals[1] = mb;
+        free(e);
+    }
+    free(rr);
+    free(eq_outer);
+    free(zv);
+    free(comb);
+    free(sb);
+    free(sa);
+    free(eq_inner);
     return 0;
 }

@@ -109,6 +109,10 @@ void or_f8_ntt_inverse(uint8_t* v, int k, uint8_t beta);
  * r has m entries (only r[6..] used). p_ab, p_c: 64 values on Lambda. */
 void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
                   or_f128 p_ab[64], or_f128 p_c[64]);
+/* s_hat_v_c of round1_extract_c_packed_with_s_hat_v (univariate_skip.rs:384-489), 128 values */
+void or_zc_s_hat_v_c(const uint8_t* c, int m, const or_f128* r, or_f128 out[128]);
+/* fold_1b_rows (pcs/ring_switch.rs:182-230, 410-470) on n 128-bit rows */
+void or_fold_1b_rows(const or_f128* rows, const or_f128* t, size_t n, or_f128 out[128]);
 /* Lagrange weights on phi_8(offset .. offset+63) at z (multilinear.rs:190-210) */
 void or_lagrange_phi8(int offset, or_f128 z, or_f128 w[64]);
 /* fold_at_z_naive (multilinear.rs:290-310) on packed bits: out[x] = sum_s bit(x*64+s) L_s(z) */
@@ -121,6 +125,29 @@ void or_zc_inner_constants(or_f128 out[7]);
 int or_zerocheck_prove(or_fs* fs, const uint8_t* a, const uint8_t* b, const uint8_t* c, int m,
                        int grinding, or_f128* round1, or_f128* ml, or_f128 finals[3],
                        uint64_t* nonces, or_f128* claim);
+
+/* ---- lincheck (crates/flock-core/src/lincheck.rs, lincheck/union.rs) ---- */
+/* build_quirky_eq_table (lincheck.rs:1355-1378): out has 64 * 2^n_rest words */
+void or_quirky_eq(or_f128 z_skip, const or_f128* x_rest, size_t n_rest, or_f128* out);
+/* CscCircuit::fold_split (lincheck.rs:463-497) */
+void or_csc_fold_split(size_t n_cols, const uint32_t* a_ptr, const uint32_t* a_rows,
+                       const uint32_t* b_ptr, const uint32_t* b_rows, const or_f128* eq,
+                       or_f128* sa, or_f128* sb);
+/* partial_fold_packed_z{,_rows_padded} (lincheck.rs:804-835, 865-990) */
+void or_partial_fold(const uint8_t* z, int k_log, int nu, size_t useful_bits, size_t n_rows,
+                     const or_f128* eq_outer, or_f128* out);
+/* sumcheck_round_eval_par / sumcheck_bind_top_in_place_par (lincheck.rs:1479-1523) */
+void or_lc_round_eval(const or_f128* c, const or_f128* z, size_t len, or_f128 out[2]);
+void or_lc_bind_top(or_f128* v, size_t len, or_f128 r);
+/* prove_union_capture_z_vec_with_grinding, single boolean slot (lincheck/union.rs:203-430)
+ * + column_sumcheck_prove (lincheck.rs:1907-2028). claim = (r_inner_skip, w, r_inner_rest). */
+int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_t* a_rows,
+                      const uint32_t* b_ptr, const uint32_t* b_rows, long const_pin,
+                      const uint8_t* z, int nu, size_t useful_bits, size_t n_rows,
+                      or_f128 z_skip, const or_f128* x_inner_rest, const or_f128* x_outer,
+                      int grinding, or_f128* rounds, or_f128 z_partial[64],
+                      or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
+                      or_f128* z_vec_out);
 
 #ifdef __cplusplus
 }

@@ -69,3 +69,34 @@ def test_zerocheck_prove(ctx, m, grinding, satisfying):
         if name == "nonces" and not grinding:
             continue
         assert np.array_equal(g, e), name
+
+
+@pytest.mark.parametrize("m,grinding", [(13, False), (18, True), (24, False)])
+def test_zerocheck_prove_capture_s_hat_v_c(ctx, m, grinding):
+    # prove_packed_padded_capture_s_hat_v_c_with_grinding: the same wire output plus
+    # s_hat_v_c at r[6..m) (the claim's r_rest)
+    say(f"case: zerocheck prove capture m={m} grinding={grinding}")
+    a, b, c = _witness(m, 400 + m, satisfying=True)
+    dom = b"flock-zc-capture-test"
+    exp = zerocheck_prove(dom, a, b, c, m, 1 if grinding else 0)
+    got = ctx.zerocheck_prove(F.FsChallenger(dom), ctx.upload(a), ctx.upload(b), ctx.upload(c), m,
+                              grinding, capture=True)
+    for i in (0, 1, 2, 4):
+        assert np.array_equal(got[i], exp[i])
+    n_mlv = m - 6
+    r_rest = O.arr_to_ints(exp[4])[1 + n_mlv:]
+    shv = np.zeros((128, 2), dtype=np.uint64)
+    ra = O.ints_to_arr([0] * 6 + r_rest)
+    L.or_zc_s_hat_v_c(c.ctypes.data, m, O._p(ra), O._p(shv))
+    assert np.array_equal(got[5], shv)
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 1000, 131075])
+def test_fold_1b_rows(ctx, n):
+    say(f"case: fold_1b_rows n={n}")
+    rows = O.splitmix(0x500 + n, n)
+    t = O.splitmix(0x600 + n, n)
+    exp = np.zeros((128, 2), dtype=np.uint64)
+    L.or_fold_1b_rows(O._p(rows), O._p(t), n, O._p(exp))
+    got = ctx.fold_1b_rows(ctx.upload(rows), n, ctx.upload(t))
+    assert np.array_equal(got, exp)

@@ -206,3 +206,76 @@ def test_zerocheck_prove_final_claims_are_mle_evaluations():
     assert r_rest[:7] == O.arr_to_ints(inner)
     # grinding: nonce bits per zerocheck.rs:96-139 (initial m -> 4 bits, skip 7, rounds 2)
     assert len([n for n in nonces]) == 2 + n_mlv
+
+
+L.or_zc_s_hat_v_c.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+L.or_fold_1b_rows.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+
+
+def s_hat_v_c(c, m, r):
+    out = np.zeros((128, 2), dtype=np.uint64)
+    ra = O.ints_to_arr(r)
+    L.or_zc_s_hat_v_c(c.ctypes.data, m, O._p(ra), O._p(out))
+    return O.arr_to_ints(out)
+
+
+def test_s_hat_v_c_by_definition():
+    # canonical form: weight eq(r[7..m), y) on the 128-bit row y, no r[6] factor
+    # (univariate_skip.rs:375-383); the oracle divides banks instead
+    m = 9
+    _, _, c = _witness(m, 21, satisfying=False)
+    r = M.splitmix(0x7A, m)
+    got = s_hat_v_c(c, m, r)
+    bits = lambda i: (int(c[i >> 3]) >> (i & 7)) & 1
+    exp = [0] * 128
+    for y in range(1 << (m - 7)):
+        e = M.eq_point(r[7:], y)
+        for j in range(128):
+            if bits(y * 128 + j):
+                exp[j] ^= e
+    assert got == exp
+
+
+def test_s_hat_v_c_recombines_to_round1_c():
+    # sum_b7 eq(r[6], b7) s_hat_v_c[lane | b7 << 6] is P^C on S; extended to Lambda by
+    # Lagrange interpolation on phi8(S) it is round 1's p_c
+    m = 8
+    a, b, c = _witness(m, 22, satisfying=False)
+    r = M.splitmix(0x7B, m)
+    shv = s_hat_v_c(c, m, r)
+    _, pc = _round1(a, b, c, m, r)
+    on_s = [M.mul(1 ^ r[6], shv[s]) ^ M.mul(r[6], shv[64 + s]) for s in range(64)]
+    nodes = [phi8(v) for v in range(64)]
+    dinv = []
+    for i in range(64):
+        d = 1
+        for j in range(64):
+            if j != i:
+                d = M.mul(d, nodes[i] ^ nodes[j])
+        dinv.append(M.inv(d))
+    for i in (0, 37, 63):
+        zz = phi8(64 + i)
+        acc = 0
+        for s in range(64):
+            num = 1
+            for j in range(64):
+                if j != s:
+                    num = M.mul(num, zz ^ nodes[j])
+            acc ^= M.mul(on_s[s], M.mul(num, dinv[s]))
+        assert acc == pc[i]
+
+
+def test_fold_1b_rows_by_definition():
+    rng = random.Random(23)
+    n = 37
+    rows = [rng.getrandbits(128) for _ in range(n)]
+    t = [rng.getrandbits(128) for _ in range(n)]
+    out = np.zeros((128, 2), dtype=np.uint64)
+    ra, ta = O.ints_to_arr(rows), O.ints_to_arr(t)
+    L.or_fold_1b_rows(O._p(ra), O._p(ta), n, O._p(out))
+    exp = [0] * 128
+    for y in range(n):
+        for j in range(128):
+            if (rows[y] >> j) & 1:
+                exp[j] ^= t[y]
+    assert O.arr_to_ints(out) == exp
