@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libflock_hip.so")
-SOURCES = ["api.hip", "field.hip", "ntt.hip", "merkle.hip", "sumcheck.hip", "zerocheck.hip"]
+SOURCES = ["api.hip", "field.hip", "ntt.hip", "merkle.hip", "sumcheck.hip", "zerocheck.hip",
+           "lincheck.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -1205,4 +1205,190 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Lincheck (crates/flock-core/src/lincheck.rs, lincheck/union.rs)
+// ---------------------------------------------------------------------------
+static int csc_check(const flock_csc* k) {
+    if (!k || !k->a_col_ptr || !k->b_col_ptr || (k->nnz_a && !k->a_rows) ||
+        (k->nnz_b && !k->b_rows) || k->k_log < 6 || k->k_log > 24 || k->nnz_a > 0xffffffffull ||
+        k->nnz_b > 0xffffffffull || k->const_pin >= ((int64_t)1 << k->k_log))
+        return fail(FLOCK_HIP_E_INVALID, "bad CSC circuit (need 6 <= k_log <= 24, nnz < 2^32, "
+                                         "const_pin < 2^k_log)");
+    return 0;
+}
+
+static int csc_fold_impl(flock_hip_ctx* c, const flock_csc* k, const uint4* eq, uint4 alpha,
+                         uint4* sa, uint4* sb, uint4* comb, int* d_err) {
+    const uint64_t K = 1ull << k->k_log;
+    LAUNCH(flock::launch_lc_csc_fold(K, k->a_col_ptr, k->a_rows, k->nnz_a, k->b_col_ptr,
+                                     k->b_rows, k->nnz_b, eq, alpha, sa, sb, comb, d_err,
+                                     c->stream));
+    int herr = 0;
+    HIP_TRY(hipMemcpyAsync(&herr, d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (herr) return fail(FLOCK_HIP_E_INVALID, "CSC circuit has a row index >= n_cols or a "
+                                               "col_ptr entry > nnz");
+    return 0;
+}
+
+int flock_hip_lc_csc_fold(flock_hip_ctx* c, const flock_csc* k, const flock_f128* eq,
+                          flock_f128 alpha, flock_f128* sa, flock_f128* sb, flock_f128* comb) {
+    CTX_CHECK(c);
+    if (int rc = csc_check(k)) return rc;
+    if (!eq || (!sa && !sb && !comb)) return fail(FLOCK_HIP_E_INVALID, "bad CSC fold arguments");
+    if (int rc = c->ensure_scratch(16)) return rc;
+    return csc_fold_impl(c, k, U(eq), U(alpha), U(sa), U(sb), U(comb),
+                         reinterpret_cast<int*>(c->scratch));
+}
+
+static int lc_fold_args(const uint8_t* z, uint32_t k_log, uint32_t nu, size_t useful_bits,
+                        size_t n_rows) {
+    if (!z || (reinterpret_cast<uintptr_t>(z) & 3) || k_log < 6 || k_log > 24 || nu < 3 ||
+        nu > 24 || useful_bits > (1ull << k_log) || n_rows > (1ull << nu))
+        return fail(FLOCK_HIP_E_INVALID, "bad partial-fold arguments (need 4-byte aligned z, "
+                                         "6 <= k_log <= 24, 3 <= nu <= 24, useful_bits <= 2^k_log, "
+                                         "n_rows <= 2^nu)");
+    return 0;
+}
+
+static uint64_t lc_fold_partial_words(uint64_t K) {
+    return (uint64_t)flock::lc_partial_fold_rows(K) * K;
+}
+
+int flock_hip_lc_partial_fold(flock_hip_ctx* c, const uint8_t* z, uint32_t k_log, uint32_t nu,
+                              size_t useful_bits, size_t n_rows, const flock_f128* eq_outer,
+                              flock_f128* out) {
+    CTX_CHECK(c);
+    if (int rc = lc_fold_args(z, k_log, nu, useful_bits, n_rows)) return rc;
+    if (!eq_outer || !out) return fail(FLOCK_HIP_E_INVALID, "bad partial-fold arguments");
+    const uint64_t K = 1ull << k_log;
+    if (int rc = c->ensure_scratch(16 * lc_fold_partial_words(K))) return rc;
+    LAUNCH(flock::launch_lc_partial_fold(z, K, useful_bits, (n_rows + 7) / 8, U(eq_outer),
+                                         reinterpret_cast<uint4*>(c->scratch), U(out), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static const uint8_t kLabelLincheck[] = "flock-lincheck-v0";
+
+int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
+                             const uint8_t* d_z, uint32_t nu, size_t useful_bits, size_t n_rows,
+                             flock_f128 z_skip, const flock_f128* h_x_inner_rest,
+                             const flock_f128* h_x_outer, int grinding, flock_f128* h_rounds,
+                             flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
+                             uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec) {
+    CTX_CHECK(c);
+    const int ks = 6;
+    if (int rc = csc_check(k)) return rc;
+    if (int rc = lc_fold_args(d_z, k->k_log, nu, useful_bits, n_rows)) return rc;
+    const uint32_t L = k->k_log - ks;
+    if (!fs || (L && (!h_x_inner_rest || !h_rounds)) || !h_x_outer || !h_z_partial ||
+        !h_matrix_evals || !h_claim || (grinding && !h_nonces))
+        return fail(FLOCK_HIP_E_INVALID, "bad lincheck arguments");
+    const uint64_t K = 1ull << k->k_log;
+    const uint64_t n_outer = 1ull << nu;
+    // scratch (words): EQI | SA | SB | COMB | ZV (K each) | EO (2^nu) | ER (2^L) |
+    // EQS (build_eq) | PF (partial-fold rows) | RV (64 + L + nu) | ERR (1)
+    const uint32_t vmax = nu > L ? nu : L;
+    const uint64_t eqs_words = 2ull << ((vmax + 1) / 2);
+    const uint64_t pf_words = lc_fold_partial_words(K);
+    const uint64_t words = 5 * K + n_outer + (1ull << L) + eqs_words + pf_words + 64 + L + nu + 1;
+    if (int rc = c->ensure_scratch(words * 16)) return rc;
+    uint4* EQI = reinterpret_cast<uint4*>(c->scratch);
+    uint4* SA = EQI + K;
+    uint4* SB = SA + K;
+    uint4* COMB = SB + K;
+    uint4* ZV = COMB + K;
+    uint4* EO = ZV + K;
+    uint4* ER = EO + n_outer;
+    uint4* EQS = ER + (1ull << L);
+    uint4* PF = EQS + eqs_words;
+    uint4* RV = PF + pf_words;
+    int* ERR = reinterpret_cast<int*>(RV + 64 + L + nu);
+
+    int ni = 0;
+    flock_fs_observe_label(fs, kLabelLincheck, sizeof(kLabelLincheck) - 1);
+    // 1. alpha (lincheck/union.rs:252-260)
+    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
+    flock_f128 alpha;
+    flock_fs_sample_f128(fs, &alpha);
+    // 2. quirky eq over the column index, then the CSC comb (union.rs:270-300)
+    std::vector<flock_f128> rv(64 + L + nu);
+    {
+        F w[64];
+        lagrange_phi8(0, F{z_skip.lo, z_skip.hi}, w);
+        for (int i = 0; i < 64; i++) rv[i] = flock_f128{w[i].lo, w[i].hi};
+        for (uint32_t j = 0; j < L; j++) rv[64 + j] = h_x_inner_rest[j];
+        for (uint32_t j = 0; j < nu; j++) rv[64 + L + j] = h_x_outer[j];
+    }
+    HIP_TRY(hipMemcpyAsync(RV, rv.data(), rv.size() * 16, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
+    LAUNCH(flock::launch_outer_product(RV, 6, ER, (int)L, EQI, c->stream));
+    if (int rc = csc_fold_impl(c, k, EQI, U(alpha), SA, SB, COMB, ERR)) return rc;
+    // 2b. constant-wire pin (union.rs:318-335)
+    if (k->const_pin >= 0) {
+        if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
+        flock_f128 beta;
+        flock_fs_sample_f128(fs, &beta);
+        LAUNCH(flock::launch_lc_add_at(COMB, (uint64_t)k->const_pin, U(beta), c->stream));
+    }
+    // 3. partial fold of the stripe at x_outer (union.rs:354-384); 3b. capture z_vec
+    LAUNCH(flock::launch_build_eq(RV + 64 + L, (int)nu, EO, EQS, c->stream));
+    LAUNCH(flock::launch_lc_partial_fold(d_z, K, useful_bits, (n_rows + 7) / 8, EO, PF, ZV,
+                                         c->stream));
+    if (d_z_vec)
+        HIP_TRY(hipMemcpyAsync(d_z_vec, ZV, K * 16, hipMemcpyDeviceToDevice, c->stream));
+    // 4-5. product sumcheck over the top L bits (column_sumcheck_prove, lincheck.rs:1907-1980)
+    std::vector<flock_f128> rr(L ? L : 1);
+    if (L) {
+        flock_f128 msg[2];
+        LAUNCH(flock::launch_lc_eval(COMB, ZV, K, c->partials, c->dout, c->stream));
+        if (int rc = c->fetch_pair(msg)) return rc;
+        uint64_t len = K;
+        for (uint32_t t = 0; t < L; t++) {
+            flock_fs_observe_f128(fs, msg[0]);
+            flock_fs_observe_f128(fs, msg[1]);
+            if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 2);
+            flock_f128 r;
+            flock_fs_sample_f128(fs, &r);
+            h_rounds[2 * t] = msg[0];
+            h_rounds[2 * t + 1] = msg[1];
+            rr[t] = r;
+            if (t + 1 < L) {
+                LAUNCH(flock::launch_lc_bind_eval(COMB, ZV, len, U(r), c->partials, c->dout,
+                                                  c->stream));
+                if (int rc = c->fetch_pair(msg)) return rc;
+            } else {
+                LAUNCH(flock::launch_lc_bind(COMB, ZV, len, U(r), c->stream));
+            }
+            len /= 2;
+        }
+    }
+    // 6-8. z_partial, fresh skip point, w (lincheck.rs:1983-2005)
+    HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    flock_fs_observe_f128_slice(fs, h_z_partial, 64);
+    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, (uint32_t)ks);
+    flock_f128 r_skip;
+    flock_fs_sample_f128(fs, &r_skip);
+    {
+        F lam[64];
+        lagrange_phi8(0, F{r_skip.lo, r_skip.hi}, lam);
+        F w{0, 0};
+        for (int i = 0; i < 64; i++)
+            w = fadd(w, fmul(lam[i], F{h_z_partial[i].lo, h_z_partial[i].hi}));
+        h_claim[0] = r_skip;
+        h_claim[1] = flock_f128{w.lo, w.hi};
+    }
+    // 9. r_inner_rest = reversed round challenges (LSB-first)
+    for (uint32_t j = 0; j < L; j++) h_claim[2 + j] = rr[L - 1 - j];
+    // 10. matrix evals with W_col = z_partial (x) eq(r_inner_rest) (union.rs:404-430)
+    HIP_TRY(hipMemcpyAsync(RV + 64, h_claim + 2, L * 16, hipMemcpyHostToDevice, c->stream));
+    LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
+    LAUNCH(flock::launch_outer_product(ZV, 6, ER, (int)L, EQI, c->stream));
+    LAUNCH(flock::launch_dot2(EQI, SA, SB, K, c->partials, c->dout, c->stream));
+    if (int rc = c->fetch_pair(h_matrix_evals)) return rc;
+    return 0;
+}
+
 }  // extern "C"

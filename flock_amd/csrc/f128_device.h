@@ -287,3 +287,34 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t v) {
 __device__ __forceinline__ f128 wave_xor_f128(f128 v) {
     return make_uint4(wave_xor(v.x), wave_xor(v.y), wave_xor(v.z), wave_xor(v.w));
 }
+
+// Per-wave comb workspace: 16 entries x 64 lanes x 16 B (see comb_build).
+constexpr int kCombWsBytes = 4 * 16 * 64 * 16;  // four waves per 256-thread block
+__device__ __forceinline__ f128* comb_lane(f128* ws) {
+    f128* w = ws + (threadIdx.x >> 6) * 1024 + (threadIdx.x & 63);
+    comb_ws_init(w);
+    return w;
+}
+
+// XOR-reduce two values over the block; thread 0 writes partials[2*blockIdx.x + {0,1}].
+__device__ __forceinline__ void block_xor_pair(f128 a0, f128 a1, f128* partials) {
+    __shared__ f128 red[2][16];
+    a0 = wave_xor_f128(a0);
+    a1 = wave_xor_f128(a1);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wid] = a0;
+        red[1][wid] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        f128 r0 = f128_zero(), r1 = f128_zero();
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; w++) {
+            r0 = f128_add(r0, red[0][w]);
+            r1 = f128_add(r1, red[1][w]);
+        }
+        partials[2 * blockIdx.x] = r0;
+        partials[2 * blockIdx.x + 1] = r1;
+    }
+}

@@ -64,6 +64,9 @@ hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c,
 // eq table build_eq(r[0..nv)) into out (2^nv entries). r is a device array; scratch holds
 // at least 2^(nv/2) + 2^(nv - nv/2) entries.
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s);
+// out[x_lo + 2^nlo x_hi] = lo[x_lo] * hi[x_hi] (2^(nlo+nhi) words).
+hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int nhi, uint4* out,
+                               hipStream_t s);
 
 // ---- zerocheck.hip ----
 // Round 1 on packed bit rows (2^(m-6) rows of 64 bits): out[0..64) = p_ab, out[64..128) = p_c.
@@ -81,5 +84,27 @@ hipError_t launch_eq_fold(const uint4* in, uint64_t n_out, uint4* out, hipStream
 // out[j] = sum_y t[y] * bit j of 128-bit row y (128 words; partials: 512 x 256 words).
 hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
                               uint4* partials, uint4* out, hipStream_t s);
+
+// ---- lincheck ----
+hipError_t launch_lc_csc_fold(uint64_t n_cols, const uint32_t* a_ptr, const uint32_t* a_rows,
+                              uint64_t nnz_a, const uint32_t* b_ptr, const uint32_t* b_rows,
+                              uint64_t nnz_b, const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,
+                              uint4* comb, int* err, hipStream_t s);
+hipError_t launch_lc_add_at(uint4* v, uint64_t idx, uint4 beta, hipStream_t s);
+// upper bound of the partial rows (gy) for any stripe count
+uint32_t lc_partial_fold_rows(uint64_t K);
+void lc_partial_fold_plan(uint64_t K, uint64_t n_stripes, uint32_t* spg, uint64_t* n_groups,
+                          uint32_t* gy);
+// partials: gy * K words (lc_partial_fold_plan)
+hipError_t launch_lc_partial_fold(const uint8_t* z, uint64_t K, uint64_t useful_bits,
+                                  uint64_t n_stripes, const uint4* eq_outer, uint4* partials,
+                                  uint4* out, hipStream_t s);
+hipError_t launch_lc_eval(const uint4* c, const uint4* z, uint64_t len, uint4* partials,
+                          uint4* out, hipStream_t s);
+hipError_t launch_lc_bind_eval(uint4* c, uint4* z, uint64_t len, uint4 r, uint4* partials,
+                               uint4* out, hipStream_t s);
+hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s);
+hipError_t launch_dot2(const uint4* w, const uint4* a, const uint4* b, uint64_t n,
+                       uint4* partials, uint4* out, hipStream_t s);
 
 }  // namespace flock

@@ -22,36 +22,6 @@
 
 namespace flock {
 
-__device__ __forceinline__ void block_xor_pair(f128 a0, f128 a1, f128* partials) {
-    __shared__ f128 red[2][16];
-    a0 = wave_xor_f128(a0);
-    a1 = wave_xor_f128(a1);
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        red[0][wid] = a0;
-        red[1][wid] = a1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        f128 r0 = f128_zero(), r1 = f128_zero();
-        const int nw = (blockDim.x + 63) >> 6;
-        for (int w = 0; w < nw; w++) {
-            r0 = f128_add(r0, red[0][w]);
-            r1 = f128_add(r1, red[1][w]);
-        }
-        partials[2 * blockIdx.x] = r0;
-        partials[2 * blockIdx.x + 1] = r1;
-    }
-}
-
-// Per-wave comb workspace: 16 entries x 64 lanes x 16 B (see comb_build).
-constexpr int kCombWsBytes = 4 * 16 * 64 * 16;  // four waves per 256-thread block
-__device__ __forceinline__ f128* comb_lane(f128* ws) {
-    f128* w = ws + (threadIdx.x >> 6) * 1024 + (threadIdx.x & 63);
-    comb_ws_init(w);
-    return w;
-}
-
 __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f,
                                                        const f128* __restrict__ b, uint64_t n,
                                                        f128* __restrict__ partials) {
@@ -282,6 +252,12 @@ hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c,
     if (g < 1) g = 1;
     if (g > 8192) g = 8192;
     k_fold_pair<<<(unsigned)g, 256, 0, s>>>(a, b, n, c, ao, bo);
+    return hipGetLastError();
+}
+
+hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int nhi, uint4* out,
+                               hipStream_t s) {
+    k_eq_outer<<<(unsigned)(1ull << nhi), 256, 0, s>>>(lo, nlo, hi, out);
     return hipGetLastError();
 }
 

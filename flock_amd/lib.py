@@ -41,6 +41,22 @@ class F128(ctypes.Structure):
     _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
 
 
+class Csc(ctypes.Structure):
+    """flock_csc: the CSC form of (A_0, B_0) on the device (CscCircuit, lincheck.rs:353-431)."""
+
+    _fields_ = [
+        ("k_log", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("a_col_ptr", ctypes.c_void_p),
+        ("a_rows", ctypes.c_void_p),
+        ("b_col_ptr", ctypes.c_void_p),
+        ("b_rows", ctypes.c_void_p),
+        ("nnz_a", ctypes.c_uint64),
+        ("nnz_b", ctypes.c_uint64),
+        ("const_pin", ctypes.c_int64),
+    ]
+
+
 class PcsParams(ctypes.Structure):
     """PcsParams (pcs/commit.rs:31-65); num_lanes = 0 means None."""
 
@@ -107,6 +123,10 @@ def load():
         "flock_hip_zc_fold_at_z": (I, [P, P, P, U32, F128, P, P]),
         "flock_hip_zerocheck_prove": (I, [P, P, P, P, P, U32, I, P, P, P, P, P, P]),
         "flock_hip_fold_1b_rows": (I, [P, P, S, P, P]),
+        "flock_hip_lc_csc_fold": (I, [P, ctypes.POINTER(Csc), P, F128, P, P, P]),
+        "flock_hip_lc_partial_fold": (I, [P, P, U32, U32, S, S, P, P]),
+        "flock_hip_lincheck_prove": (I, [P, P, ctypes.POINTER(Csc), P, U32, S, S, F128, P, P, I,
+                                         P, P, P, P, P, P]),
         "flock_fs_new": (P, [ctypes.c_char_p, S]),
         "flock_fs_free": (None, [P]),
         "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
@@ -141,6 +161,15 @@ def to_int(x: F128) -> int:
 def _np_ptr(a: np.ndarray):
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ints_arr(v) -> np.ndarray:
+    """F128 ints -> (n, 2) uint64 array (one dummy word for an empty list)."""
+    out = np.zeros((max(len(v), 1), 2), dtype=np.uint64)
+    for i, x in enumerate(v):
+        out[i, 0] = x & ((1 << 64) - 1)
+        out[i, 1] = x >> 64
+    return out
 
 
 def _ints(a: np.ndarray) -> list[int]:
@@ -342,6 +371,47 @@ class Context:
         if capture:
             return r1, ml, fin, nonces, claim, shv
         return r1, ml, fin, nonces, claim
+
+    # ---- lincheck ----
+    def csc(self, k_log: int, a_ptr: np.ndarray, a_rows: np.ndarray, b_ptr: np.ndarray,
+            b_rows: np.ndarray, const_pin: int = -1):
+        """Upload a CSC pair; returns (Csc, buffers) -- keep the buffers alive while used."""
+        bufs = [self.upload(np.ascontiguousarray(x, dtype=np.uint32))
+                if len(x) else self.alloc(4) for x in (a_ptr, a_rows, b_ptr, b_rows)]
+        c = Csc(k_log, 0, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, len(a_rows),
+                len(b_rows), const_pin)
+        return c, bufs
+
+    def lc_csc_fold(self, circuit: "Csc", eq: DeviceBuffer, alpha: int, sa=None, sb=None,
+                    comb=None):
+        _check(load().flock_hip_lc_csc_fold(self.h, ctypes.byref(circuit), eq.ptr, f128(alpha),
+                                            sa.ptr if sa else None, sb.ptr if sb else None,
+                                            comb.ptr if comb else None))
+
+    def lc_partial_fold(self, z: DeviceBuffer, k_log: int, nu: int, useful_bits: int,
+                        n_rows: int, eq_outer: DeviceBuffer, out: DeviceBuffer):
+        _check(load().flock_hip_lc_partial_fold(self.h, z.ptr, k_log, nu, useful_bits, n_rows,
+                                                eq_outer.ptr, out.ptr))
+
+    def lincheck_prove(self, fs: "FsChallenger", circuit: "Csc", z: DeviceBuffer, nu: int,
+                       useful_bits: int, n_rows: int, z_skip: int, x_inner_rest, x_outer,
+                       grinding: bool, z_vec: DeviceBuffer | None = None) -> dict:
+        """prove_union_capture_z_vec_with_grinding (lincheck/union.rs:203-430), one slot."""
+        L = circuit.k_log - 6
+        out = dict(rounds=np.zeros((max(2 * L, 1), 2), dtype=np.uint64),
+                   z_partial=np.zeros((64, 2), dtype=np.uint64),
+                   matrix_evals=np.zeros((2, 2), dtype=np.uint64),
+                   nonces=np.zeros(L + 3, dtype=np.uint64),
+                   claim=np.zeros((2 + L, 2), dtype=np.uint64))
+        xr = _ints_arr(x_inner_rest)
+        xo = _ints_arr(x_outer)
+        _check(load().flock_hip_lincheck_prove(
+            self.h, fs.h, ctypes.byref(circuit), z.ptr, nu, useful_bits, n_rows, f128(z_skip),
+            _np_ptr(xr), _np_ptr(xo), 1 if grinding else 0, _np_ptr(out["rounds"]),
+            _np_ptr(out["z_partial"]), _np_ptr(out["matrix_evals"]), _np_ptr(out["nonces"]),
+            _np_ptr(out["claim"]), z_vec.ptr if z_vec else None))
+        out["rounds"] = out["rounds"][:2 * L]
+        return out
 
     def fold_1b_rows(self, rows: DeviceBuffer, n_rows: int, t: DeviceBuffer) -> np.ndarray:
         """fold_1b_rows (pcs/ring_switch.rs:182-230): out[j] = sum_y t[y] bit_j(rows[y])."""

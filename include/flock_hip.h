@@ -198,6 +198,53 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const uint8_t* d
 int flock_hip_fold_1b_rows(flock_hip_ctx* ctx, const flock_f128* d_rows, size_t n_rows,
                            const flock_f128* d_t, flock_f128 h_out[128]);
 
+/* ---- lincheck (crates/flock-core/src/lincheck.rs, lincheck/union.rs) ---------------- */
+
+/* A boolean R1CS base pair (A_0, B_0), 2^k_log x 2^k_log, in CSC form on the device: the rows
+ * holding a 1 in column c are rows[col_ptr[c] .. col_ptr[c+1]) (CscCircuit and csc_from_rows,
+ * lincheck.rs:353-431). const_pin = the constant-wire pin column, -1 for none
+ * (LincheckCircuit::const_pin_col). */
+typedef struct {
+    uint32_t k_log;
+    uint32_t reserved;
+    const uint32_t* a_col_ptr; /* 2^k_log + 1 entries */
+    const uint32_t* a_rows;    /* nnz_a entries, each < 2^k_log */
+    const uint32_t* b_col_ptr;
+    const uint32_t* b_rows;
+    uint64_t nnz_a, nnz_b;
+    int64_t const_pin;
+} flock_csc;
+
+/* CscCircuit::fold_split / fold_alpha_batched (lincheck.rs:432-497) on a device eq table of
+ * 2^k_log words: d_sa = A^T eq, d_sb = B^T eq, d_comb = alpha * d_sa + d_sb. Any output may be
+ * NULL (not all three). Out-of-range CSC entries give FLOCK_HIP_E_INVALID. */
+int flock_hip_lc_csc_fold(flock_hip_ctx* ctx, const flock_csc* circuit, const flock_f128* d_eq,
+                          flock_f128 alpha, flock_f128* d_sa, flock_f128* d_sb,
+                          flock_f128* d_comb);
+/* Partial fold of the lincheck stripe (byte d_z[s*2^k_log + i] = outer rows 8s..8s+7 of column
+ * i, 2^(nu+k_log-3) bytes, 4-byte aligned) at the outer eq table d_eq_outer (2^nu words):
+ * partial_fold_packed_z (lincheck.rs:804-835) with the support-proportional restriction of
+ * partial_fold_packed_z_rows_best (:865-1100): columns < useful_bits, stripes <
+ * ceil(n_rows / 8). d_out gets 2^k_log words. */
+int flock_hip_lc_partial_fold(flock_hip_ctx* ctx, const uint8_t* d_z, uint32_t k_log, uint32_t nu,
+                              size_t useful_bits, size_t n_rows, const flock_f128* d_eq_outer,
+                              flock_f128* d_out);
+/* Boolean lincheck prover for a registry with one boolean type filling the boolean region:
+ * prove_union_capture_z_vec_with_grinding (lincheck/union.rs:203-430) with column_sumcheck_prove
+ * (lincheck.rs:1907-2028). x_ab = (z_skip, h_x_inner_rest[k_log-6], h_x_outer[nu]); d_z the
+ * stripe as in flock_hip_lc_partial_fold. grinding != 0 selects
+ * LincheckGrinding::per_challenge_128 (lincheck.rs:590-603). Outputs (caller-allocated, host):
+ * h_rounds[2 (k_log-6)] = (q(1), q(inf)) per round; h_z_partial[64]; h_matrix_evals[2] =
+ * (<W, A_0>, <W, B_0>); h_nonces[k_log-6 + 3] when grinding; h_claim[2 + k_log-6] =
+ * (r_inner_skip, w, r_inner_rest) (LincheckClaim, lincheck.rs:646-657); d_z_vec (device,
+ * 2^k_log words, may be NULL) = the captured pre-sumcheck fold. */
+int flock_hip_lincheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const flock_csc* circuit,
+                             const uint8_t* d_z, uint32_t nu, size_t useful_bits, size_t n_rows,
+                             flock_f128 z_skip, const flock_f128* h_x_inner_rest,
+                             const flock_f128* h_x_outer, int grinding, flock_f128* h_rounds,
+                             flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
+                             uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
+
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 
 /* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */
