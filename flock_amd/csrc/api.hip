@@ -1391,4 +1391,164 @@ int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Union instance host glue (crates/flock-core/src/{schedule,union}.rs)
+// ---------------------------------------------------------------------------
+static unsigned pow2_log(uint64_t n) {
+    unsigned l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+int flock_registry_new(const flock_table_type* types, size_t n, uint32_t nu,
+                       flock_registry* out) {
+    if (!types || !out || n == 0 || n > FLOCK_MAX_TYPES || nu > 40)
+        return fail(FLOCK_HIP_E_INVALID, "registry needs 1..16 types and nu <= 40");
+    for (size_t i = 0; i < n; i++) {
+        const flock_table_type& t = types[i];
+        if (t.k_log < 7 || t.k_log > 40 || t.useful_bits > (1ull << t.k_log) ||
+            (t.is_element && t.useful_bits % 128) || nu + t.k_log > 62)
+            return fail(FLOCK_HIP_E_INVALID, "bad table type (7 <= k_log <= 40, useful_bits <= "
+                                             "2^k_log, element useful_bits = 128 k)");
+    }
+    std::memset(out, 0, sizeof(*out));
+    // class-major (boolean first), then k_log descending, stable
+    size_t order[FLOCK_MAX_TYPES];
+    size_t m = 0;
+    for (int cls = 0; cls < 2; cls++)
+        for (int k = 40; k >= 7; k--)
+            for (size_t i = 0; i < n; i++)
+                if ((types[i].is_element != 0) == (cls == 1) && (int)types[i].k_log == k)
+                    order[m++] = i;
+    uint64_t bool_extent = 0, elem_extent = 0;
+    uint32_t nb = 0;
+    for (size_t s = 0; s < n; s++) {
+        const flock_table_type& t = types[order[s]];
+        if (t.is_element) {
+            elem_extent += 1ull << (nu + t.k_log);
+        } else {
+            bool_extent += 1ull << (nu + t.k_log);
+            nb++;
+        }
+    }
+    out->m_bool = bool_extent ? pow2_log(bool_extent) : 0;
+    out->m_elem = elem_extent ? pow2_log(elem_extent) : 0;
+    out->element_base = (elem_extent == 0 || bool_extent == 0)
+                            ? 0
+                            : 1ull << (out->m_bool > out->m_elem ? out->m_bool : out->m_elem);
+    uint64_t offset = 0;
+    for (size_t s = 0; s < n; s++) {
+        if (s == nb) offset = out->element_base;
+        const uint32_t ms = nu + types[order[s]].k_log;
+        if (offset & ((1ull << ms) - 1)) return fail(FLOCK_HIP_E_INVALID, "slot misaligned");
+        out->slots[s].type_index = (uint32_t)order[s];
+        out->slots[s].m_slot = ms;
+        out->slots[s].offset = offset;
+        out->types[s] = types[order[s]];
+        offset += 1ull << ms;
+    }
+    out->m_total = pow2_log(offset);
+    for (size_t s = 0; s < n; s++) {
+        out->slots[s].prefix = out->slots[s].offset >> out->slots[s].m_slot;
+        out->slots[s].prefix_bits = out->m_total - out->slots[s].m_slot;
+    }
+    out->n_types = (uint32_t)n;
+    out->num_boolean = nb;
+    out->nu = nu;
+    return 0;
+}
+
+static uint64_t union_used_cols(const flock_table_type& t) {
+    const uint64_t u = (t.useful_bits + 127) / 128, cap = 1ull << (t.k_log - 7);
+    return u < cap ? u : cap;
+}
+
+int flock_union_sizes_compute(const flock_registry* reg, const uint64_t* counts,
+                              uint32_t dense_floor_m, uint32_t log_batch_size,
+                              flock_union_sizes* out) {
+    if (!reg || !counts || !out || reg->n_types == 0 || reg->n_types > FLOCK_MAX_TYPES)
+        return fail(FLOCK_HIP_E_INVALID, "bad union size arguments");
+    const unsigned kMinDenseM = 22;  // union.rs:45
+    uint64_t dense = 0;
+    for (uint32_t s = 0; s < reg->n_types; s++) {
+        if (counts[s] > (1ull << reg->nu))
+            return fail(FLOCK_HIP_E_INVALID, "count exceeds the row capacity 2^nu");
+        dense += union_used_cols(reg->types[s]) * counts[s];
+    }
+    const uint64_t packed = 1ull << (reg->m_total - 7);
+    if (dense_floor_m && (dense_floor_m < kMinDenseM || (1ull << (dense_floor_m - 7)) > packed))
+        return fail(FLOCK_HIP_E_INVALID, "dense floor below 22 or above the padded domain");
+    uint64_t floor = 1ull << (kMinDenseM - 7);
+    if (dense_floor_m && (1ull << (dense_floor_m - 7)) > floor) floor = 1ull << (dense_floor_m - 7);
+    uint64_t committed = 1;
+    while (committed < dense) committed <<= 1;
+    if (committed < floor) committed = floor;
+    if (committed > packed) committed = packed;
+    out->dense_words = dense;
+    out->committed_words = committed;
+    out->packed_len = packed;
+    out->dense_m = pow2_log(committed) + 7;
+    if (out->dense_m < 7 + log_batch_size)
+        return fail(FLOCK_HIP_E_INVALID, "log_batch_size exceeds dense_m - 7");
+    const unsigned log_dim = out->dense_m - 7 - log_batch_size;
+    uint64_t t = (dense + (1ull << log_dim) - 1) >> log_dim;  // dense_lanes, pcs/commit.rs:428
+    if (t < 1) t = 1;
+    if (t > (1ull << log_batch_size)) t = 1ull << log_batch_size;
+    out->commit_lanes = t < (1ull << log_batch_size) ? t : 0;
+    bool ident = committed == packed;
+    uint64_t cursor = 0;
+    for (uint32_t s = 0; s < reg->n_types && ident; s++) {
+        if (cursor != (reg->slots[s].offset >> 7) || counts[s] != (1ull << reg->nu))
+            ident = false;
+        cursor += union_used_cols(reg->types[s]) << reg->nu;
+    }
+    out->compaction_is_identity = ident ? 1u : 0u;
+    return 0;
+}
+
+int flock_union_x_ab_from_mlv(const flock_registry* reg, const flock_f128* mlv, size_t n_mlv,
+                              flock_f128* x_inner_rest, flock_f128* x_outer) {
+    if (!reg || !mlv || !x_inner_rest || !x_outer || reg->num_boolean == 0 ||
+        n_mlv != (size_t)reg->m_bool - 6 || n_mlv < 1 + (size_t)reg->nu)
+        return fail(FLOCK_HIP_E_INVALID, "mlv must have m_bool - 6 entries");
+    x_inner_rest[0] = mlv[0];
+    for (size_t j = 1 + reg->nu; j < n_mlv; j++) x_inner_rest[j - reg->nu] = mlv[j];
+    for (uint32_t j = 0; j < reg->nu; j++) x_outer[j] = mlv[1 + j];
+    return 0;
+}
+
+static const uint8_t kLabelMixed[] = "flock-mixed-v1";
+
+int flock_union_bind_statement(flock_fs* fs, const uint8_t digest[32], const uint64_t* counts,
+                               size_t n, const uint8_t* cap, size_t cap_len) {
+    if (!fs || !digest || (n && !counts) || (cap_len && !cap))
+        return fail(FLOCK_HIP_E_INVALID, "bad bind_statement arguments");
+    flock_fs_observe_label(fs, kLabelMixed, sizeof(kLabelMixed) - 1);
+    flock_fs_observe_bytes(fs, digest, 32);
+    std::vector<uint8_t> le(8 * n);
+    for (size_t i = 0; i < n; i++)
+        for (int b = 0; b < 8; b++) le[8 * i + b] = (uint8_t)(counts[i] >> (8 * b));
+    flock_fs_observe_bytes(fs, le.data(), le.size());
+    flock_fs_observe_bytes(fs, cap, cap_len);
+    return 0;
+}
+
+int flock_hip_union_compact(flock_hip_ctx* c, const flock_registry* reg, const uint64_t* counts,
+                            uint32_t dense_floor_m, const flock_f128* z, flock_f128* q) {
+    CTX_CHECK(c);
+    if (!z || !q) return fail(FLOCK_HIP_E_INVALID, "bad compact arguments");
+    flock_union_sizes sz;
+    if (int rc = flock_union_sizes_compute(reg, counts, dense_floor_m, 0, &sz)) return rc;
+    uint64_t cursor = 0;
+    for (uint32_t s = 0; s < reg->n_types; s++) {
+        const uint64_t cols = union_used_cols(reg->types[s]);
+        LAUNCH(flock::launch_compact_slot(U(z), reg->slots[s].offset >> 7, reg->nu, cols,
+                                          counts[s], U(q) + cursor, c->stream));
+        cursor += cols * counts[s];
+    }
+    LAUNCH(flock::launch_zero_words(U(q) + cursor, sz.committed_words - cursor, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 }  // extern "C"

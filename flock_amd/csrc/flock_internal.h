@@ -107,4 +107,9 @@ hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t
 hipError_t launch_dot2(const uint4* w, const uint4* a, const uint4* b, uint64_t n,
                        uint4* partials, uint4* out, hipStream_t s);
 
+// ---- witness data movement ----
+hipError_t launch_compact_slot(const uint4* z, uint64_t start, uint32_t nu, uint64_t cols,
+                               uint64_t n_t, uint4* q, hipStream_t s);
+hipError_t launch_zero_words(uint4* q, uint64_t n, hipStream_t s);
+
 }  // namespace flock

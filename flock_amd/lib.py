@@ -57,6 +57,35 @@ class Csc(ctypes.Structure):
     ]
 
 
+class TableType(ctypes.Structure):
+    """flock_table_type (TableType, schedule.rs:150-215)."""
+
+    _fields_ = [("k_log", ctypes.c_uint32), ("is_element", ctypes.c_uint32),
+                ("useful_bits", ctypes.c_uint64)]
+
+
+class Slot(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_uint64), ("prefix", ctypes.c_uint64),
+                ("m_slot", ctypes.c_uint32), ("prefix_bits", ctypes.c_uint32),
+                ("type_index", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Registry(ctypes.Structure):
+    """flock_registry (Registry, schedule.rs:275-495), types in slot order."""
+
+    _fields_ = [("n_types", ctypes.c_uint32), ("num_boolean", ctypes.c_uint32),
+                ("nu", ctypes.c_uint32), ("m_total", ctypes.c_uint32),
+                ("m_bool", ctypes.c_uint32), ("m_elem", ctypes.c_uint32),
+                ("element_base", ctypes.c_uint64), ("slots", Slot * 16),
+                ("types", TableType * 16)]
+
+
+class UnionSizes(ctypes.Structure):
+    _fields_ = [("dense_words", ctypes.c_uint64), ("committed_words", ctypes.c_uint64),
+                ("packed_len", ctypes.c_uint64), ("commit_lanes", ctypes.c_uint64),
+                ("dense_m", ctypes.c_uint32), ("compaction_is_identity", ctypes.c_uint32)]
+
+
 class PcsParams(ctypes.Structure):
     """PcsParams (pcs/commit.rs:31-65); num_lanes = 0 means None."""
 
@@ -123,6 +152,12 @@ def load():
         "flock_hip_zc_fold_at_z": (I, [P, P, P, U32, F128, P, P]),
         "flock_hip_zerocheck_prove": (I, [P, P, P, P, P, U32, I, P, P, P, P, P, P]),
         "flock_hip_fold_1b_rows": (I, [P, P, S, P, P]),
+        "flock_registry_new": (I, [P, S, U32, ctypes.POINTER(Registry)]),
+        "flock_union_sizes_compute": (I, [ctypes.POINTER(Registry), P, U32, U32,
+                                          ctypes.POINTER(UnionSizes)]),
+        "flock_union_x_ab_from_mlv": (I, [ctypes.POINTER(Registry), P, S, P, P]),
+        "flock_union_bind_statement": (I, [P, ctypes.c_char_p, P, S, ctypes.c_char_p, S]),
+        "flock_hip_union_compact": (I, [P, ctypes.POINTER(Registry), P, U32, P, P]),
         "flock_hip_lc_csc_fold": (I, [P, ctypes.POINTER(Csc), P, F128, P, P, P]),
         "flock_hip_lc_partial_fold": (I, [P, P, U32, U32, S, S, P, P]),
         "flock_hip_lincheck_prove": (I, [P, P, ctypes.POINTER(Csc), P, U32, S, S, F128, P, P, I,
@@ -372,6 +407,14 @@ class Context:
             return r1, ml, fin, nonces, claim, shv
         return r1, ml, fin, nonces, claim
 
+    # ---- union ----
+    def union_compact(self, reg: Registry, counts, z_padded: DeviceBuffer, q: DeviceBuffer,
+                      dense_floor_m: int = 0):
+        """compact_witness_unchecked (union.rs:449-490)."""
+        c = np.ascontiguousarray(counts, dtype=np.uint64)
+        _check(load().flock_hip_union_compact(self.h, ctypes.byref(reg), c.ctypes.data,
+                                              dense_floor_m, z_padded.ptr, q.ptr))
+
     # ---- lincheck ----
     def csc(self, k_log: int, a_ptr: np.ndarray, a_rows: np.ndarray, b_ptr: np.ndarray,
             b_rows: np.ndarray, const_pin: int = -1):
@@ -436,6 +479,37 @@ def pcs_sizes(params: PcsParams) -> dict:
                 num_ntts=nn.value)
 
 
+def registry_new(types, nu: int) -> Registry:
+    """Registry::new (schedule.rs:337-495); types = [(k_log, useful_bits, is_element)]."""
+    arr = (TableType * max(len(types), 1))(*[TableType(k, int(e), u) for k, u, e in types])
+    reg = Registry()
+    _check(load().flock_registry_new(arr, len(types), nu, ctypes.byref(reg)))
+    return reg
+
+
+def union_sizes(reg: Registry, counts, dense_floor_m: int = 0,
+                log_batch_size: int = 0) -> dict:
+    """UnionInstance sizes (union.rs:325-416); counts in slot order."""
+    c = np.ascontiguousarray(counts, dtype=np.uint64)
+    out = UnionSizes()
+    _check(load().flock_union_sizes_compute(ctypes.byref(reg), c.ctypes.data, dense_floor_m,
+                                            log_batch_size, ctypes.byref(out)))
+    return dict(dense_words=out.dense_words, committed_words=out.committed_words,
+                packed_len=out.packed_len, dense_m=out.dense_m,
+                commit_lanes=out.commit_lanes or None,
+                compaction_is_identity=bool(out.compaction_is_identity))
+
+
+def x_ab_from_mlv(reg: Registry, mlv):
+    """x_ab_from_mlv (union.rs:541-552) -> (x_inner_rest, x_outer)."""
+    m = _ints_arr(mlv)
+    xr = np.zeros((max(len(mlv) - reg.nu, 1), 2), dtype=np.uint64)
+    xo = np.zeros((max(reg.nu, 1), 2), dtype=np.uint64)
+    _check(load().flock_union_x_ab_from_mlv(ctypes.byref(reg), _np_ptr(m), len(mlv), _np_ptr(xr),
+                                            _np_ptr(xo)))
+    return _ints(xr[:len(mlv) - reg.nu]), _ints(xo[:reg.nu])
+
+
 class FsChallenger:
     """FsChallenger with the SHA-256 discipline (challenger.rs:609-1044)."""
 
@@ -476,3 +550,9 @@ class FsChallenger:
 
     def grind_pow(self, bits: int) -> int:
         return int(load().flock_fs_grind_pow(self.h, bits))
+
+    def bind_statement(self, registry_digest: bytes, counts, cap: bytes):
+        """UnionInstance::bind_statement (union.rs:619-628)."""
+        c = np.ascontiguousarray(counts, dtype=np.uint64)
+        _check(load().flock_union_bind_statement(self.h, registry_digest, c.ctypes.data, len(c),
+                                                 cap, len(cap)))

@@ -245,6 +245,61 @@ int flock_hip_lincheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const flock_csc* 
                              flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
                              uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
 
+/* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ------------ */
+
+#define FLOCK_MAX_TYPES 16
+/* TableType (schedule.rs:150-215): k_log in [7, 40], useful_bits <= 2^k_log; is_element
+ * selects TableClass::LargeField (useful_bits a multiple of 128). */
+typedef struct {
+    uint32_t k_log;
+    uint32_t is_element;
+    uint64_t useful_bits;
+} flock_table_type;
+/* Slot (schedule.rs:247-268); type_index = the caller's index of the slot's type. */
+typedef struct {
+    uint64_t offset, prefix;
+    uint32_t m_slot, prefix_bits;
+    uint32_t type_index, reserved;
+} flock_slot;
+/* Registry (schedule.rs:275-300) with the types copied in slot order. */
+typedef struct {
+    uint32_t n_types, num_boolean, nu, m_total, m_bool, m_elem;
+    uint64_t element_base;
+    flock_slot slots[FLOCK_MAX_TYPES];
+    flock_table_type types[FLOCK_MAX_TYPES];
+} flock_registry;
+/* UnionInstance sizes (union.rs:325-416). commit_lanes 0 = None. */
+typedef struct {
+    uint64_t dense_words, committed_words, packed_len, commit_lanes;
+    uint32_t dense_m, compaction_is_identity;
+} flock_union_sizes;
+
+/* Registry::new (schedule.rs:337-495): class-major, k_log-descending stable slot order, aligned
+ * offsets, M, M_bool, M_elem, element_base. Host only. */
+int flock_registry_new(const flock_table_type* types, size_t n_types, uint32_t nu,
+                       flock_registry* out);
+/* dense_words / committed_words / dense_m / commit_lanes(log_batch_size) /
+ * compaction_is_identity (union.rs:325-416, MIN_DENSE_M = 22) for the declared counts (slot
+ * order, each <= 2^nu). dense_floor_m = 0: none (set_dense_floor, :91-105). Host only. */
+int flock_union_sizes_compute(const flock_registry* reg, const uint64_t* counts,
+                              uint32_t dense_floor_m, uint32_t log_batch_size,
+                              flock_union_sizes* out);
+/* x_ab_from_mlv (union.rs:541-552): h_mlv has m_bool - 6 entries; h_x_inner_rest gets
+ * m_bool - 6 - nu, h_x_outer nu. Host only. */
+int flock_union_x_ab_from_mlv(const flock_registry* reg, const flock_f128* h_mlv, size_t n_mlv,
+                              flock_f128* h_x_inner_rest, flock_f128* h_x_outer);
+/* bind_statement (union.rs:619-628): label "flock-mixed-v1", the 32-byte Registry::digest
+ * (BLAKE3, schedule.rs:496-580, computed by the caller), the counts as u64 LE, the cap bytes. */
+int flock_union_bind_statement(flock_fs* fs, const uint8_t registry_digest[32],
+                               const uint64_t* counts, size_t n_counts, const uint8_t* cap,
+                               size_t cap_len);
+/* compact_witness_unchecked (union.rs:449-490): gather the committed dense stack
+ * d_q[committed_words] (committed_words for the given dense_floor_m, 0 = none) from the padded
+ * batch-major union buffer d_z_padded[packed_len]. */
+int flock_hip_union_compact(flock_hip_ctx* ctx, const flock_registry* reg, const uint64_t* counts,
+                            uint32_t dense_floor_m, const flock_f128* d_z_padded,
+                            flock_f128* d_q);
+
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 
 /* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */

@@ -1189,3 +1189,152 @@ int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_
     free(eq_inner);
     return 0;
 }
+
+/* ======================================================================== */
+/* Union instance host glue (crates/flock-core/src/{schedule,union}.rs)     */
+/* ======================================================================== */
+
+static unsigned pow2_log(uint64_t n) { /* log2(next_power_of_two(n)) */
+    unsigned l = 0;
+    while ((1ull << l) < n) l++;
+    return l;
+}
+
+/* Registry::new (schedule.rs:337-495): class-major stable sort (boolean first, k_log
+ * descending), slots packed from 0 (boolean) and element_base (element). order[i] = caller
+ * index of slot i. */
+int or_registry_layout(const or_table_type* types, size_t n, unsigned nu, or_registry* reg) {
+    if (n == 0 || n > OR_MAX_TYPES) return 1;
+    for (size_t i = 0; i < n; i++)
+        if (types[i].k_log < 7 || types[i].k_log > 40 ||
+            types[i].useful_bits > (1ull << types[i].k_log))
+            return 1;
+    size_t order[OR_MAX_TYPES];
+    size_t m = 0;
+    for (int cls = 0; cls < 2; cls++)              /* boolean, then element */
+        for (unsigned k = 41; k-- > 7;)            /* k_log descending */
+            for (size_t i = 0; i < n; i++)         /* stable */
+                if ((types[i].is_element != 0) == (cls == 1) && types[i].k_log == k)
+                    order[m++] = i;
+    uint64_t bool_extent = 0, elem_extent = 0;
+    size_t nb = 0;
+    for (size_t s = 0; s < n; s++) {
+        const or_table_type* t = &types[order[s]];
+        if (t->is_element) elem_extent += 1ull << (nu + t->k_log);
+        else {
+            bool_extent += 1ull << (nu + t->k_log);
+            nb++;
+        }
+    }
+    reg->m_bool = bool_extent ? pow2_log(bool_extent) : 0;
+    reg->m_elem = elem_extent ? pow2_log(elem_extent) : 0;
+    reg->element_base = (elem_extent == 0 || bool_extent == 0)
+                            ? 0
+                            : 1ull << (reg->m_bool > reg->m_elem ? reg->m_bool : reg->m_elem);
+    uint64_t offset = 0;
+    for (size_t s = 0; s < n; s++) {
+        if (s == nb) offset = reg->element_base;
+        const unsigned ms = nu + types[order[s]].k_log;
+        if (offset & ((1ull << ms) - 1)) return 1;
+        reg->slots[s].type_index = order[s];
+        reg->slots[s].m_slot = ms;
+        reg->slots[s].offset = offset;
+        offset += 1ull << ms;
+    }
+    reg->m_total = pow2_log(offset);
+    for (size_t s = 0; s < n; s++) {
+        reg->slots[s].prefix = reg->slots[s].offset >> reg->slots[s].m_slot;
+        reg->slots[s].prefix_bits = reg->m_total - reg->slots[s].m_slot;
+    }
+    reg->n = n;
+    reg->nu = nu;
+    reg->num_boolean = nb;
+    return 0;
+}
+
+/* used_cols (union.rs:316-319) */
+static uint64_t used_cols(const or_table_type* t) {
+    uint64_t u = (t->useful_bits + 127) / 128, cap = 1ull << (t->k_log - 7);
+    return u < cap ? u : cap;
+}
+
+/* dense_words / committed_words / dense_m / commit_lanes / compaction_is_identity
+ * (union.rs:325-416; MIN_DENSE_M = 22, :45; dense_lanes, pcs/commit.rs:428-432). counts in
+ * slot order; dense_floor_m = 0 for none; commit_lanes 0 = None. */
+int or_union_sizes(const or_table_type* types, const or_registry* reg, const uint64_t* counts,
+                   unsigned dense_floor_m, unsigned log_batch_size, or_union_size_info* out) {
+    uint64_t dense = 0;
+    for (size_t s = 0; s < reg->n; s++) {
+        if (counts[s] > (1ull << reg->nu)) return 1;
+        dense += used_cols(&types[reg->slots[s].type_index]) * counts[s];
+    }
+    const uint64_t packed = 1ull << (reg->m_total - 7);
+    uint64_t floor = 1ull << (22 - 7);
+    if (dense_floor_m && (1ull << (dense_floor_m - 7)) > floor) floor = 1ull << (dense_floor_m - 7);
+    uint64_t committed = 1;
+    while (committed < dense) committed <<= 1;
+    if (committed < floor) committed = floor;
+    if (committed > packed) committed = packed;
+    unsigned dm = 0;
+    while ((1ull << dm) < committed) dm++;
+    dm += 7;
+    out->dense_words = dense;
+    out->committed_words = committed;
+    out->packed_len = packed;
+    out->dense_m = dm;
+    if (dm < 7 + log_batch_size) return 1;
+    const unsigned log_dim = dm - 7 - log_batch_size;
+    uint64_t t = (dense + (1ull << log_dim) - 1) >> log_dim;
+    if (t < 1) t = 1;
+    if (t > (1ull << log_batch_size)) t = 1ull << log_batch_size;
+    out->commit_lanes = t < (1ull << log_batch_size) ? t : 0;
+    int ident = 1;
+    uint64_t cursor = 0;
+    for (size_t s = 0; s < reg->n; s++) {
+        if (cursor != (reg->slots[s].offset >> 7) || counts[s] != (1ull << reg->nu)) {
+            ident = 0;
+            break;
+        }
+        cursor += used_cols(&types[reg->slots[s].type_index]) << reg->nu;
+    }
+    out->compaction_is_identity = ident && committed == packed;
+    return 0;
+}
+
+/* compact_witness_unchecked (union.rs:449-490): q has committed_words words. */
+void or_compact_witness(const or_table_type* types, const or_registry* reg,
+                        const uint64_t* counts, const or_f128* z_padded, size_t committed,
+                        or_f128* q) {
+    size_t cursor = 0;
+    for (size_t s = 0; s < reg->n; s++) {
+        const uint64_t uc = used_cols(&types[reg->slots[s].type_index]);
+        const uint64_t start = reg->slots[s].offset >> 7;
+        for (uint64_t c = 0; c < uc; c++)
+            for (uint64_t i = 0; i < counts[s]; i++)
+                q[cursor++] = z_padded[start + (c << reg->nu) + i];
+    }
+    for (; cursor < committed; cursor++) q[cursor] = ZERO;
+}
+
+/* x_ab_from_mlv (union.rs:541-552): x_inner_rest = [mlv[0], mlv[1+nu..]], x_outer =
+ * mlv[1..1+nu]; mlv has m_bool - 6 entries. */
+void or_x_ab_from_mlv(const or_f128* mlv, size_t n_mlv, unsigned nu, or_f128* x_inner_rest,
+                      or_f128* x_outer) {
+    x_inner_rest[0] = mlv[0];
+    for (size_t j = 1 + nu; j < n_mlv; j++) x_inner_rest[j - nu] = mlv[j];
+    for (unsigned j = 0; j < nu; j++) x_outer[j] = mlv[1 + j];
+}
+
+/* bind_statement (union.rs:619-628) given the registry digest bytes. */
+void or_bind_statement(or_fs* fs, const uint8_t digest[32], const uint64_t* counts, size_t n,
+                       const uint8_t* cap, size_t cap_len) {
+    static const uint8_t label[] = "flock-mixed-v1";
+    or_fs_observe_label(fs, label, sizeof(label) - 1);
+    or_fs_observe_bytes(fs, digest, 32);
+    uint8_t* le = (uint8_t*)malloc(8 * (n ? n : 1));
+    for (size_t i = 0; i < n; i++)
+        for (int b = 0; b < 8; b++) le[8 * i + b] = (uint8_t)(counts[i] >> (8 * b));
+    or_fs_observe_bytes(fs, le, 8 * n);
+    free(le);
+    or_fs_observe_bytes(fs, cap, cap_len);
+}

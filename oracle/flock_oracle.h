@@ -149,6 +149,38 @@ int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_
                       or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
                       or_f128* z_vec_out);
 
+/* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ---- */
+#define OR_MAX_TYPES 16
+typedef struct {
+    uint32_t k_log, is_element;
+    uint64_t useful_bits;
+} or_table_type;
+typedef struct {
+    size_t type_index;
+    uint32_t m_slot, prefix_bits;
+    uint64_t offset, prefix;
+} or_slot;
+typedef struct {
+    size_t n, num_boolean;
+    uint32_t nu, m_total, m_bool, m_elem;
+    uint64_t element_base;
+    or_slot slots[OR_MAX_TYPES];
+} or_registry;
+typedef struct {
+    uint64_t dense_words, committed_words, packed_len, commit_lanes;
+    uint32_t dense_m, compaction_is_identity;
+} or_union_size_info;
+int or_registry_layout(const or_table_type* types, size_t n, unsigned nu, or_registry* reg);
+int or_union_sizes(const or_table_type* types, const or_registry* reg, const uint64_t* counts,
+                   unsigned dense_floor_m, unsigned log_batch_size, or_union_size_info* out);
+void or_compact_witness(const or_table_type* types, const or_registry* reg,
+                        const uint64_t* counts, const or_f128* z_padded, size_t committed,
+                        or_f128* q);
+void or_x_ab_from_mlv(const or_f128* mlv, size_t n_mlv, unsigned nu, or_f128* x_inner_rest,
+                      or_f128* x_outer);
+void or_bind_statement(or_fs* fs, const uint8_t digest[32], const uint64_t* counts, size_t n,
+                       const uint8_t* cap, size_t cap_len);
+
 #ifdef __cplusplus
 }
 #endif
