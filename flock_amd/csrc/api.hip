@@ -1551,4 +1551,21 @@ int flock_hip_union_compact(flock_hip_ctx* c, const flock_registry* reg, const u
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Witness generation (crates/flock-prover/src/r1cs_hashes)
+// ---------------------------------------------------------------------------
+int flock_hip_witness_sha256(flock_hip_ctx* c, const uint32_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe) {
+    CTX_CHECK(c);
+    if (n_log < 3 || n_log > 26 || n_decl > (1ull << n_log) || (n_decl && !d_inputs) || !d_z ||
+        !d_a || !d_b || (reinterpret_cast<uintptr_t>(d_stripe) & 7))
+        return fail(FLOCK_HIP_E_INVALID, "bad SHA-256 witness arguments (3 <= n_log <= 26, "
+                                         "n_decl <= 2^n_log, 8-byte aligned stripe)");
+    LAUNCH(flock::launch_sha2_witness(d_inputs, n_decl, n_log, U(d_z), U(d_a), U(d_b), d_stripe,
+                                      c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 }  // extern "C"

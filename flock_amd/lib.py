@@ -158,6 +158,7 @@ def load():
         "flock_union_x_ab_from_mlv": (I, [ctypes.POINTER(Registry), P, S, P, P]),
         "flock_union_bind_statement": (I, [P, ctypes.c_char_p, P, S, ctypes.c_char_p, S]),
         "flock_hip_union_compact": (I, [P, ctypes.POINTER(Registry), P, U32, P, P]),
+        "flock_hip_witness_sha256": (I, [P, P, S, U32, P, P, P, P]),
         "flock_hip_lc_csc_fold": (I, [P, ctypes.POINTER(Csc), P, F128, P, P, P]),
         "flock_hip_lc_partial_fold": (I, [P, P, U32, U32, S, S, P, P]),
         "flock_hip_lincheck_prove": (I, [P, P, ctypes.POINTER(Csc), P, U32, S, S, F128, P, P, I,
@@ -406,6 +407,15 @@ class Context:
         if capture:
             return r1, ml, fin, nonces, claim, shv
         return r1, ml, fin, nonces, claim
+
+    # ---- witness ----
+    def witness_sha256(self, inputs: DeviceBuffer | None, n_decl: int, n_log: int,
+                       z: DeviceBuffer, a: DeviceBuffer, b: DeviceBuffer,
+                       stripe: DeviceBuffer | None = None):
+        """generate_witness_batch_major_partial (r1cs_hashes/sha2.rs:1876-1898)."""
+        _check(load().flock_hip_witness_sha256(self.h, inputs.ptr if inputs else None, n_decl,
+                                               n_log, z.ptr, a.ptr, b.ptr,
+                                               stripe.ptr if stripe else None))
 
     # ---- union ----
     def union_compact(self, reg: Registry, counts, z_padded: DeviceBuffer, q: DeviceBuffer,

@@ -300,6 +300,18 @@ int flock_hip_union_compact(flock_hip_ctx* ctx, const flock_registry* reg, const
                             uint32_t dense_floor_m, const flock_f128* d_z_padded,
                             flock_f128* d_q);
 
+/* ---- witness generation (crates/flock-prover/src/r1cs_hashes) -------------------------- */
+
+/* SHA-256 compression witness, batch-major: generate_witness_batch_major_partial
+ * (r1cs_hashes/sha2.rs:1876-1898 over drive_witness_batch_major_partial, common.rs:805-960).
+ * d_inputs holds n_decl compressions of 24 u32 (h_in[8], m[16]); rows n_decl..2^n_log are
+ * dummy rows, zero in every output. d_z, d_a, d_b: 2^(n_log+8) F128 each, word (chunk << n_log)
+ * + row = bits [128 chunk, 128 chunk + 128) of that row's 2^15-bit (z, A_0 z, B_0 z);
+ * d_stripe (may be NULL): the lincheck stripe, 2^(n_log+12) bytes. 3 <= n_log <= 26. */
+int flock_hip_witness_sha256(flock_hip_ctx* ctx, const uint32_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe);
+
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 
 /* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */

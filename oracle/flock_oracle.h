@@ -181,6 +181,23 @@ void or_x_ab_from_mlv(const or_f128* mlv, size_t n_mlv, unsigned nu, or_f128* x_
 void or_bind_statement(or_fs* fs, const uint8_t digest[32], const uint64_t* counts, size_t n,
                        const uint8_t* cap, size_t cap_len);
 
+/* ---- SHA-256 compression R1CS (crates/flock-prover/src/r1cs_hashes/sha2.rs), sha2_r1cs.c ---- */
+/* layout constants: {SCHED_CARRY_BASE, E_NEW_BASE, A_NEW_BASE, OUT_CARRY_BASE, Z_CONST_POS,
+ * USEFUL_BITS, ROUND_BASE[1], 0} (sha2.rs:95-194) */
+void or_sha2_layout(uint32_t out[8]);
+/* build_block_ab_packed_into (sha2.rs:944-1128): 512 u64 each */
+void or_sha2_instance(const uint32_t h_in[8], const uint32_t m[16], uint64_t* z, uint64_t* a,
+                      uint64_t* b);
+/* drive_witness_batch_major_partial (common.rs:805-960); inputs 24 u32 per declared row */
+int or_sha2_witness_batch_major(const uint32_t* inputs, size_t n_decl, int n_log, or_f128* z,
+                                or_f128* a, or_f128* b, uint8_t* stripe);
+/* build_matrices (sha2.rs:548-701) as CSR; a_ptr/b_ptr have 32769 entries */
+int or_sha2_matrices(uint64_t* a_ptr, uint32_t* a_cols, uint64_t* b_ptr, uint32_t* b_cols);
+void or_csr_to_csc(size_t k, const uint64_t* ptr, const uint32_t* cols, uint32_t* col_ptr,
+                   uint32_t* rows);
+void or_csr_apply_bits(size_t k, const uint64_t* ptr, const uint32_t* cols, const uint64_t* z,
+                       uint64_t* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,3 +102,31 @@ def test_lincheck_prove(ctx, k_log, nu, pin, grinding, ub, nr):
     if grinding:
         assert np.array_equal(got["nonces"], exp["nonces"])
     assert np.array_equal(zv.download_f128(k), exp["z_vec"])
+
+
+def test_lincheck_sha256_circuit_config2(ctx):
+    # the SHA-256 compression circuit (47.4M nonzeros) at 1024 instances (k_log 15, nu 10) with
+    # the GPU-built witness stripe, the constant-wire pin at Z_CONST_POS and grinding on
+    from test_oracle_sha2_r1cs import matrices, random_inputs
+    say("case: lincheck sha256 circuit k_log=15 nu=10")
+    _, _, csc_a, csc_b = matrices()
+    k_log, nu, n_decl = 15, 10, 1000
+    inputs = random_inputs(n_decl, 4242)
+    n = 1 << nu
+    z, a, b = (ctx.alloc(16 * 256 * n) for _ in range(3))
+    stripe = ctx.alloc(4096 * n)
+    ctx.witness_sha256(ctx.upload(np.ascontiguousarray(inputs, dtype=np.uint32)), n_decl, nu, z,
+                       a, b, stripe)
+    zs = np.frombuffer(stripe.download(4096 * n), dtype=np.uint8).copy()
+    L = k_log - 6
+    pts = O.arr_to_ints(O.splitmix(0x4243, 1 + L + nu))
+    z_skip, x_rest, x_outer = pts[0], pts[1:1 + L], pts[1 + L:]
+    dom = b"flock-readme-bench-v0"
+    pin, ub = 25_469, 25_470
+    exp = lincheck_prove(dom, k_log, csc_a, csc_b, pin, zs, nu, ub, n_decl, z_skip, x_rest,
+                         x_outer, 1)
+    circuit, keep = ctx.csc(k_log, *csc_a, *csc_b, const_pin=pin)
+    got = ctx.lincheck_prove(F.FsChallenger(dom), circuit, stripe, nu, ub, n_decl, z_skip,
+                             x_rest, x_outer, True)
+    for name in ("rounds", "z_partial", "matrix_evals", "claim", "nonces"):
+        assert np.array_equal(got[name], exp[name]), name

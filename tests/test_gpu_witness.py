@@ -59,3 +59,41 @@ def test_union_compact_sha256_full_size(ctx):
     q = ctx.alloc(16 * sz["committed_words"])
     ctx.union_compact(reg, counts, ctx.upload(z), q)
     assert np.array_equal(q.download_f128(sz["committed_words"]), exp)
+
+
+@pytest.mark.parametrize("n_log,n_decl", [(3, 8), (3, 5), (6, 64), (8, 200), (10, 1024),
+                                          (10, 1000), (12, 3000)])
+def test_sha256_witness(ctx, n_log, n_decl):
+    from test_oracle_sha2_r1cs import batch_major, random_inputs
+    say(f"case: sha256 witness n_log={n_log} n_decl={n_decl}")
+    inputs = random_inputs(n_decl, 700 + n_log)
+    ez, ea, eb, es = batch_major(inputs, n_log)
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 256 * n) for _ in range(3))
+    stripe = ctx.alloc(4096 * n)
+    dev_in = ctx.upload(np.ascontiguousarray(inputs, dtype=np.uint32))
+    ctx.witness_sha256(dev_in, n_decl, n_log, z, a, b, stripe)
+    assert np.array_equal(z.download_f128(256 * n), ez)
+    assert np.array_equal(a.download_f128(256 * n), ea)
+    assert np.array_equal(b.download_f128(256 * n), eb)
+    assert np.array_equal(np.frombuffer(stripe.download(4096 * n), dtype=np.uint8), es)
+
+
+def test_zerocheck_on_sha256_witness(ctx):
+    # the zerocheck reads the batch-major buffers as packed bits (m = n_log + 15), with c = z
+    from test_oracle_sha2_r1cs import random_inputs
+    from test_oracle_zerocheck import zerocheck_prove
+    n_log, n_decl = 7, 100
+    m = n_log + 15
+    say(f"case: zerocheck on sha256 witness m={m}")
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 256 * n) for _ in range(3))
+    ctx.witness_sha256(ctx.upload(random_inputs(n_decl, 77)), n_decl, n_log, z, a, b)
+    hz, ha, hb = (np.frombuffer(x.download(16 * 256 * n), dtype=np.uint8).copy()
+                  for x in (z, a, b))
+    assert np.array_equal(ha & hb, hz)  # honest witness: a AND b = z = c
+    dom = b"flock-zc-sha256-witness"
+    exp = zerocheck_prove(dom, ha, hb, hz, m, 1)
+    got = ctx.zerocheck_prove(F.FsChallenger(dom), a, b, z, m, True)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
