@@ -31,12 +31,18 @@ __global__ __launch_bounds__(kR1Threads) void k_zc_round1(
     const f128* __restrict__ eq, uint64_t rows, uint32_t rows_per_group,
     const uint4* __restrict__ ext_g, const uint8_t* __restrict__ logexp_g,
     f128* __restrict__ partials) {
+    // ext in LDS is k-major: ext[(g*4 + k)*16 + n] = 16-byte slice k of entry (g, n), so the
+    // 16 entries of one slice are 256 B (all banks once) and random nibbles never conflict.
     __shared__ uint4 ext[16 * 16 * 4];       // 16 KiB
     __shared__ uint8_t lg[256];
     __shared__ uint8_t ex[512];
     __shared__ uint32_t ys[kR1Threads][32];  // Y of one batch, transposed words (32 KiB)
     __shared__ f128 eqs[kR1Threads];
-    for (int i = threadIdx.x; i < 16 * 16 * 4; i += blockDim.x) ext[i] = ext_g[i];
+    __shared__ f128 tabs[kR1Threads / 8 * 32];  // nibble sum tables per 8-row group (16 KiB)
+    for (int i = threadIdx.x; i < 16 * 16 * 4; i += blockDim.x) {
+        const int g = i >> 6, n = (i >> 2) & 15, k = i & 3;  // source index (g*16 + n)*4 + k
+        ext[(g * 4 + k) * 16 + n] = ext_g[i];
+    }
     for (int i = threadIdx.x; i < 256; i += blockDim.x) lg[i] = logexp_g[i];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) ex[i] = logexp_g[256 + i];
     __syncthreads();
@@ -60,17 +66,17 @@ __global__ __launch_bounds__(kR1Threads) void k_zc_round1(
             const uint2 ra = reinterpret_cast<const uint2*>(a)[x];
             const uint2 rb = reinterpret_cast<const uint2*>(b)[x];
             const uint2 rc = reinterpret_cast<const uint2*>(c)[x];
-#pragma unroll
+#pragma unroll 2  // fully unrolled, the 16 groups of loads spill
             for (int g = 0; g < 16; g++) {
                 const uint32_t wa = g < 8 ? ra.x : ra.y, wb = g < 8 ? rb.x : rb.y,
                                wc = g < 8 ? rc.x : rc.y;
                 const int sh = 4 * (g & 7);
-                const uint4* ea = ext + (g * 16 + ((wa >> sh) & 15u)) * 4;
-                const uint4* eb = ext + (g * 16 + ((wb >> sh) & 15u)) * 4;
-                const uint4* ec = ext + (g * 16 + ((wc >> sh) & 15u)) * 4;
+                const uint4* ea = ext + g * 64 + ((wa >> sh) & 15u);
+                const uint4* eb = ext + g * 64 + ((wb >> sh) & 15u);
+                const uint4* ec = ext + g * 64 + ((wc >> sh) & 15u);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    const uint4 va = ea[k], vb = eb[k], vc = ec[k];
+                    const uint4 va = ea[16 * k], vb = eb[16 * k], vc = ec[16 * k];
                     ya[4 * k] ^= va.x; ya[4 * k + 1] ^= va.y; ya[4 * k + 2] ^= va.z; ya[4 * k + 3] ^= va.w;
                     yb[4 * k] ^= vb.x; yb[4 * k + 1] ^= vb.y; yb[4 * k + 2] ^= vb.z; yb[4 * k + 3] ^= vb.w;
                     yc[4 * k] ^= vc.x; yc[4 * k + 1] ^= vc.y; yc[4 * k + 2] ^= vc.z; yc[4 * k + 3] ^= vc.w;
@@ -103,20 +109,33 @@ __global__ __launch_bounds__(kR1Threads) void k_zc_round1(
             ys[t][d] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
         }
         __syncthreads();
-        // ---- accumulate: thread t owns bits j = t + 256 q, byte (t>>3) + 32 q, bit t&7 ----
+        // ---- accumulate (method of four Russians over 8-row groups) ----
+        // tabs[32 g + 16 h + n] = XOR of eqs[8g + 4h + i] over the bits i of n. Rows past the
+        // batch's end have zero Y and zero eq, so they add nothing.
+        for (int e = t; e < kR1Threads / 8 * 32; e += kR1Threads) {
+            const f128* q = eqs + 8 * (e >> 5) + 4 * ((e >> 4) & 1);
+            const int n = e & 15;
+            f128 te = f128_zero();
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if ((n >> i) & 1) te = f128_add(te, q[i]);
+            tabs[e] = te;
+        }
+        __syncthreads();
+        // thread t owns bits j = t + 256 q: byte (t>>3) + 32 q, bit t&7. For a group, byte q of
+        // v collects that bit of the group's 8 rows (row i -> bit i); its nibbles index tabs.
         const int nrows = (int)((row_end - base) < (uint64_t)kR1Threads ? (row_end - base)
                                                                         : (uint64_t)kR1Threads);
         const int d = t >> 3, bt = t & 7;
-        for (int rr = 0; rr < nrows; rr++) {
-            const uint32_t w = ys[rr][d];
-            const f128 e = eqs[rr];
+        for (int g = 0; g < (nrows + 7) >> 3; g++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) v |= ((ys[8 * g + i][d] >> bt) & 0x01010101u) << i;
+            const f128* T = tabs + 32 * g;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const uint32_t msk = (uint32_t)(((int32_t)(w << (31 - (8 * q + bt)))) >> 31);
-                acc[q].x ^= e.x & msk;
-                acc[q].y ^= e.y & msk;
-                acc[q].z ^= e.z & msk;
-                acc[q].w ^= e.w & msk;
+                const uint32_t pat = (v >> (8 * q)) & 0xffu;
+                acc[q] = f128_add(acc[q], f128_add(T[pat & 15u], T[16 + (pat >> 4)]));
             }
         }
         __syncthreads();
