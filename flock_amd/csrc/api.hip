@@ -642,7 +642,7 @@ int flock_hip_f128_mul_const_batch(flock_hip_ctx* c, const flock_f128* a, flock_
 int flock_hip_ntt_forward_interleaved(flock_hip_ctx* c, flock_f128* data, size_t num_ntts,
                                       size_t live, uint32_t log_d, uint32_t start_layer) {
     CTX_CHECK(c);
-    if (!data || num_ntts == 0 || live > num_ntts || log_d > 62 || start_layer > log_d ||
+    if (!data || num_ntts == 0 || live > num_ntts || log_d > 40 || start_layer > log_d ||
         num_ntts > 0xffffffffull)
         return fail(FLOCK_HIP_E_INVALID, "bad NTT shape");
     if (live == 0 || start_layer == log_d) return 0;
@@ -657,7 +657,7 @@ int flock_hip_ntt_forward_interleaved(flock_hip_ctx* c, flock_f128* data, size_t
 int flock_hip_ntt_pass(flock_hip_ctx* c, flock_f128* data, size_t num_ntts, size_t live,
                        uint32_t log_d, uint32_t first_layer, uint32_t layers) {
     CTX_CHECK(c);
-    if (!data || num_ntts == 0 || live > num_ntts || log_d > 62 || layers < 1 || layers > 3 ||
+    if (!data || num_ntts == 0 || live > num_ntts || log_d > 40 || layers < 1 || layers > 3 ||
         first_layer + layers > log_d || num_ntts > 0xffffffffull)
         return fail(FLOCK_HIP_E_INVALID, "bad NTT pass shape");
     if (live == 0) return 0;

@@ -124,7 +124,10 @@ __device__ __forceinline__ f128 f128_mul(f128 a, f128 b) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void comb_ws_init(f128* ws_lane) { ws_lane[0] = f128_zero(); }
 
-// Build the table of b (b_lo part) in the lane's column; returns b_hi.
+// Build the table of b (b_lo part) in the lane's column; returns b_hi. STRIDE = distance in
+// words between entries: 64 for the per-lane workspace, 1 for a table shared by many lanes
+// (16 contiguous entries = 256 B, every bank once).
+template <int STRIDE = 64>
 __device__ __forceinline__ uint32_t comb_build(f128* ws_lane, f128 b) {
     const uint32_t bh = b.w >> 29;
     const f128 t1 = make_uint4(b.x, b.y, b.z, b.w & 0x1fffffffu);
@@ -137,25 +140,26 @@ __device__ __forceinline__ uint32_t comb_build(f128* ws_lane, f128 b) {
     const f128 t3 = f128_add(t1, t2), t5 = f128_add(t1, t4), t6 = f128_add(t2, t4);
     const f128 t7 = f128_add(t3, t4), t9 = f128_add(t1, t8), t10 = f128_add(t2, t8);
     const f128 t12 = f128_add(t4, t8);
-    ws_lane[1 * 64] = t1;
-    ws_lane[2 * 64] = t2;
-    ws_lane[3 * 64] = t3;
-    ws_lane[4 * 64] = t4;
-    ws_lane[5 * 64] = t5;
-    ws_lane[6 * 64] = t6;
-    ws_lane[7 * 64] = t7;
-    ws_lane[8 * 64] = t8;
-    ws_lane[9 * 64] = t9;
-    ws_lane[10 * 64] = t10;
-    ws_lane[11 * 64] = f128_add(t3, t8);
-    ws_lane[12 * 64] = t12;
-    ws_lane[13 * 64] = f128_add(t5, t8);
-    ws_lane[14 * 64] = f128_add(t6, t8);
-    ws_lane[15 * 64] = f128_add(t7, t8);
+    ws_lane[1 * STRIDE] = t1;
+    ws_lane[2 * STRIDE] = t2;
+    ws_lane[3 * STRIDE] = t3;
+    ws_lane[4 * STRIDE] = t4;
+    ws_lane[5 * STRIDE] = t5;
+    ws_lane[6 * STRIDE] = t6;
+    ws_lane[7 * STRIDE] = t7;
+    ws_lane[8 * STRIDE] = t8;
+    ws_lane[9 * STRIDE] = t9;
+    ws_lane[10 * STRIDE] = t10;
+    ws_lane[11 * STRIDE] = f128_add(t3, t8);
+    ws_lane[12 * STRIDE] = t12;
+    ws_lane[13 * STRIDE] = f128_add(t5, t8);
+    ws_lane[14 * STRIDE] = f128_add(t6, t8);
+    ws_lane[15 * STRIDE] = f128_add(t7, t8);
     return bh;
 }
 
 // acc ^= a * b where the lane's table holds b_lo and bh = b_hi (unreduced 256-bit).
+template <int STRIDE = 64>
 __device__ __forceinline__ void comb_apply(f256u& out, const f128* ws_lane, uint32_t bh, f128 a) {
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
     uint32_t r[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
@@ -167,10 +171,10 @@ __device__ __forceinline__ void comb_apply(f256u& out, const f128* ws_lane, uint
             for (int i = 7; i > 0; i--) r[i] = fshl(r[i], r[i - 1], 4);
             r[0] <<= 4;
         }
-        const f128 t0 = ws_lane[((aw[0] >> (4 * j)) & 15u) * 64];
-        const f128 t1 = ws_lane[((aw[1] >> (4 * j)) & 15u) * 64];
-        const f128 t2 = ws_lane[((aw[2] >> (4 * j)) & 15u) * 64];
-        const f128 t3 = ws_lane[((aw[3] >> (4 * j)) & 15u) * 64];
+        const f128 t0 = ws_lane[((aw[0] >> (4 * j)) & 15u) * STRIDE];
+        const f128 t1 = ws_lane[((aw[1] >> (4 * j)) & 15u) * STRIDE];
+        const f128 t2 = ws_lane[((aw[2] >> (4 * j)) & 15u) * STRIDE];
+        const f128 t3 = ws_lane[((aw[3] >> (4 * j)) & 15u) * STRIDE];
         r[0] ^= t0.x;
         r[1] = __builtin_amdgcn_bitop3_b32(r[1], t0.y, t1.x, 0x96);
         r[2] = __builtin_amdgcn_bitop3_b32(r[2], t0.z, t1.y, 0x96) ^ t2.x;

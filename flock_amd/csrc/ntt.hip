@@ -76,6 +76,8 @@ hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, 
 // ------------------------------------------------------------------- NTT --
 
 constexpr int kNttThreads = 512;
+constexpr int kMaxNttK = 40;       // largest transform log-size the library accepts
+constexpr uint64_t kDeepCols = 256; // up to this many columns per high block: k_ntt_pass_deep
 
 // One fused pass over layers [l0, l0 + K). src == data for an in-place pass; for the
 // first pass of a commit src is the message itself: the state after the log_inv_rate
@@ -140,6 +142,78 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict
     }
 }
 
+// Deep passes (at most 256 columns per high block): a 512-entry Shoup table per twiddle would
+// be used only a few hundred times and a 512-thread group would mostly idle. Here a 256-thread
+// group takes H = 256 / cols consecutive high blocks, builds one shared 16-entry comb table
+// per (block, twiddle) (256 B, every LDS bank once, so random nibble lookups never conflict),
+// and each thread runs the K layers of one (block, column) with those tables.
+constexpr int kDeepThreads = 256;
+constexpr int kDeepMaxHb = 32;  // high blocks per group (tables: 32 x 7 x 256 B = 56 KiB)
+
+template <int K>
+__global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
+    const f128* __restrict__ src, f128* __restrict__ data, uint32_t t, uint32_t live, int k,
+    int l0, const f128* __restrict__ evals, uint32_t cols_per_hb, uint32_t hbs_per_wg,
+    uint64_t n_hb, int from_msg) {
+    constexpr int NT = (1 << K) - 1;
+    __shared__ f128 rows[K][kMaxNttK];  // rows[j][bit] = evals[k - (l0+j) - 1][1 + bit]
+    extern __shared__ f128 tabs[];      // hbs_per_wg * NT tables of 16 entries, then b_hi words
+    uint32_t* his = reinterpret_cast<uint32_t*>(tabs + 16 * NT * hbs_per_wg);
+    for (int i = threadIdx.x; i < K * kMaxNttK; i += blockDim.x) {
+        const int j = i / kMaxNttK, bit = i - j * kMaxNttK;
+        const int layer = l0 + j;
+        rows[j][bit] = bit < layer ? evals[(size_t)(k - layer - 1) * k + 1 + bit] : f128_zero();
+    }
+    __syncthreads();
+    const uint64_t hb0 = (uint64_t)blockIdx.x * hbs_per_wg;
+    // table (h, tix): twiddle index tix = 2^j - 1 + u serves layer l0 + j, block (hb << j) | u
+    for (uint32_t i = threadIdx.x; i < hbs_per_wg * NT; i += blockDim.x) {
+        const uint32_t h = i / NT, tix = i - h * NT;
+        const uint64_t hb = hb0 + h;
+        if (hb >= n_hb) continue;
+        const int j = 31 - __clz(tix + 1);
+        uint64_t blk = (hb << j) | (uint64_t)(tix + 1 - (1u << j));
+        f128 tw = f128_zero();
+        while (blk) {
+            tw = f128_add(tw, rows[j][__builtin_ctzll(blk)]);
+            blk &= blk - 1;
+        }
+        f128* tab = tabs + 16 * i;
+        tab[0] = f128_zero();
+        his[i] = comb_build<1>(tab, tw);
+    }
+    __syncthreads();
+    const uint32_t h = threadIdx.x / cols_per_hb;
+    const uint32_t c = threadIdx.x - h * cols_per_hb;
+    const uint64_t hb = hb0 + h;
+    if (h >= hbs_per_wg || hb >= n_hb || (c % t) >= live) return;  // no barriers below
+    const uint64_t S = 1ull << (k - l0 - K);
+    const uint64_t stride = S * t;
+    const uint64_t base = (hb << (k - l0)) * (uint64_t)t;
+    const uint64_t sbase = from_msg ? 0 : base;
+    f128 e[1 << K];
+#pragma unroll
+    for (int i = 0; i < (1 << K); i++) e[i] = src[sbase + i * stride + c];
+    const f128* mytabs = tabs + 16 * NT * h;
+    const uint32_t* myhis = his + NT * h;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int hh = 1 << (K - 1 - j);
+#pragma unroll
+        for (int i = 0; i < (1 << K); i++) {
+            if (i & hh) continue;
+            const int tix = (1 << j) - 1 + (i >> (K - j));
+            f256u acc;
+            f256u_zero(acc);
+            comb_apply<1>(acc, mytabs + 16 * tix, myhis[tix], e[i | hh]);
+            e[i] = f128_add(e[i], f256u_reduce(acc));
+            e[i | hh] = f128_add(e[i | hh], e[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << K); i++) data[base + i * stride + c] = e[i];
+}
+
 template <int K>
 static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t live, int k,
                               int l0, const f128* evals, bool from_msg, hipStream_t s) {
@@ -154,6 +228,25 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
     }
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
+    if (cols <= kDeepCols) {
+        uint32_t per = (uint32_t)(kDeepThreads / cols);  // high blocks per group
+        if (per > (uint32_t)kDeepMaxHb) per = kDeepMaxHb;
+        const uint64_t grid = (n_hb + per - 1) / per;
+        if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
+        const size_t dyn = (size_t)per * NT * (16 * sizeof(f128) + sizeof(uint32_t));
+        static bool deep_attr = false;
+        if (!deep_attr) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_ntt_pass_deep<K>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)((size_t)kDeepMaxHb * NT *
+                                                     (16 * sizeof(f128) + sizeof(uint32_t))));
+            if (e != hipSuccess) return e;
+            deep_attr = true;
+        }
+        k_ntt_pass_deep<K><<<(unsigned)grid, kDeepThreads, dyn, s>>>(
+            src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
+        return hipGetLastError();
+    }
     // up to 2 columns per thread so the twiddle-table build is amortised
     uint64_t wgs = (cols + 2 * kNttThreads - 1) / (2 * kNttThreads);
     if (wgs < 1) wgs = 1;
