@@ -1568,4 +1568,32 @@ int flock_hip_witness_sha256(flock_hip_ctx* c, const uint32_t* d_inputs, size_t 
     return 0;
 }
 
+int flock_hip_witness_keccak(flock_hip_ctx* c, const uint64_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe) {
+    CTX_CHECK(c);
+    if (n_log < 3 || n_log > 24 || n_decl > (1ull << n_log) || (n_decl && !d_inputs) || !d_z ||
+        !d_a || !d_b || (reinterpret_cast<uintptr_t>(d_stripe) & 7))
+        return fail(FLOCK_HIP_E_INVALID, "bad Keccak witness arguments (3 <= n_log <= 24, "
+                                         "n_decl <= 2^n_log, 8-byte aligned stripe)");
+    LAUNCH(flock::launch_keccak_witness(d_inputs, n_decl, n_log, U(d_z), U(d_a), U(d_b),
+                                        d_stripe, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int flock_hip_witness_blake3(flock_hip_ctx* c, const uint32_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe) {
+    CTX_CHECK(c);
+    if (n_log < 3 || n_log > 26 || n_decl > (1ull << n_log) || (n_decl && !d_inputs) || !d_z ||
+        !d_a || !d_b || (reinterpret_cast<uintptr_t>(d_stripe) & 7))
+        return fail(FLOCK_HIP_E_INVALID, "bad BLAKE3 witness arguments (3 <= n_log <= 26, "
+                                         "n_decl <= 2^n_log, 8-byte aligned stripe)");
+    LAUNCH(flock::launch_blake3_witness(d_inputs, n_decl, n_log, U(d_z), U(d_a), U(d_b),
+                                        d_stripe, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 }  // extern "C"

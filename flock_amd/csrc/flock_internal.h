@@ -113,6 +113,14 @@ hipError_t launch_compact_slot(const uint4* z, uint64_t start, uint32_t nu, uint
 hipError_t launch_zero_words(uint4* q, uint64_t n, hipStream_t s);
 // SHA-256 compression witness, batch-major: z/a/b have 256 << n_log words; stripe (may be null)
 // 2^n_log * 4096 bytes. n_log >= 3, rows >= n_decl are zero.
+// Keccak-f[1600] witness, batch-major: z/a/b have 512 << n_log words; stripe (may be null)
+// 2^n_log * 8192 bytes. Rows >= n_decl run keccak_f on the zero state.
+hipError_t launch_keccak_witness(const uint64_t* inputs, uint64_t n_decl, uint32_t n_log,
+                                 uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s);
+// BLAKE3 compression witness, batch-major: z/a/b have 128 << n_log words; stripe (may be null)
+// 2^n_log * 2048 bytes. Rows >= n_decl are zero.
+hipError_t launch_blake3_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t n_log,
+                                 uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s);
 hipError_t launch_sha2_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t n_log,
                                uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s);
 

@@ -83,6 +83,10 @@ __device__ __forceinline__ uint32_t ss1(uint32_t x) { return ror(x, 17) ^ ror(x,
 
 constexpr int kWitThreads = 128;
 
+// Output of one witness row: a finished 64-bit word goes to its batch-major address, except
+// the words listed by the policy (SHARED: words that straddle two regions written by
+// different streams), which are OR-collected in LDS and stored at the end of the row.
+template <class SHARED>
 struct WitOut {
     uint64_t* z;
     uint64_t* a;
@@ -90,18 +94,20 @@ struct WitOut {
     uint64_t row;       // this thread's row o
     uint32_t n_log;
     bool live;          // a declared row (else zeros are written)
-    uint64_t* lds;      // 4 shared words x 3 buffers, lane-minor: lds[(k*3+j)*kWitThreads + tid]
+    uint64_t* lds;      // SHARED::kN words x 3 buffers, lane-minor: lds[(k*3+j)*blockDim + tid]
     __device__ __forceinline__ void emit(uint32_t w, uint64_t vz, uint64_t va, uint64_t vb) const {
         if (!live) vz = va = vb = 0;
-        int k = -1;
+        if constexpr (SHARED::kN > 0) {
+            int k = -1;
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (w == sha2::kShared[i]) k = i;
-        if (k >= 0) {
-            lds[(k * 3 + 0) * kWitThreads + threadIdx.x] |= vz;
-            lds[(k * 3 + 1) * kWitThreads + threadIdx.x] |= va;
-            lds[(k * 3 + 2) * kWitThreads + threadIdx.x] |= vb;
-            return;
+            for (int i = 0; i < SHARED::kN; i++)
+                if (w == SHARED::word(i)) k = i;
+            if (k >= 0) {
+                lds[(k * 3 + 0) * blockDim.x + threadIdx.x] |= vz;
+                lds[(k * 3 + 1) * blockDim.x + threadIdx.x] |= va;
+                lds[(k * 3 + 2) * blockDim.x + threadIdx.x] |= vb;
+                return;
+            }
         }
         const uint64_t at = 2 * ((((uint64_t)w >> 1) << n_log) + row) + (w & 1u);
         z[at] = vz;
@@ -110,13 +116,23 @@ struct WitOut {
     }
 };
 
+struct Sha2Shared {
+    static constexpr int kN = 4;
+    __device__ static constexpr uint32_t word(int i) { return sha2::kShared[i]; }
+};
+struct NoShared {
+    static constexpr int kN = 0;
+    __device__ static constexpr uint32_t word(int) { return 0; }
+};
+
 // A region written in bit order: partial 64-bit words of z, a, b.
 struct BitStream {
     uint32_t pos;
     uint64_t z, a, b;
     __device__ __forceinline__ explicit BitStream(uint32_t base) : pos(base), z(0), a(0), b(0) {}
     // append n <= 32 bits (values already masked to n bits)
-    __device__ __forceinline__ void push(const WitOut& o, uint32_t vz, uint32_t va, uint32_t vb,
+    template <class O>
+    __device__ __forceinline__ void push(const O& o, uint32_t vz, uint32_t va, uint32_t vb,
                                          uint32_t n) {
         const uint32_t sh = pos & 63u;
         z |= (uint64_t)vz << sh;
@@ -131,10 +147,12 @@ struct BitStream {
         }
         pos += n;
     }
-    __device__ __forceinline__ void push3(const WitOut& o, const uint32_t t[3], uint32_t n) {
+    template <class O>
+    __device__ __forceinline__ void push3(const O& o, const uint32_t t[3], uint32_t n) {
         push(o, t[2], t[0], t[1], n);  // (z, a, b) = (product, left, right)
     }
-    __device__ __forceinline__ void finish(const WitOut& o) {
+    template <class O>
+    __device__ __forceinline__ void finish(const O& o) {
         if (pos & 63u) o.emit(pos >> 6, z, a, b);
     }
 };
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(kWitThreads) void k_sha2_witness(const uint32_t* __
     const int tid = threadIdx.x;
     const uint64_t row = (uint64_t)blockIdx.x * kWitThreads + tid;
     if (row >= (1ull << n_log)) return;  // the block is wider than 2^n_log (no barriers below)
-    WitOut o{z, a, b, row, n_log, row < n_decl, shared_words};
+    WitOut<Sha2Shared> o{z, a, b, row, n_log, row < n_decl, shared_words};
 #pragma unroll
     for (int i = 0; i < 12; i++) shared_words[i * kWitThreads + tid] = 0;
     uint32_t h_in[8];
@@ -371,6 +389,265 @@ hipError_t launch_sha2_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t
     if (g > 16384) g = 16384;
     k_stripe_from_z<<<(unsigned)g, 256, 0, s>>>(reinterpret_cast<const uint64_t*>(z), n_log, 512,
                                                 sha2::kUsefulWords,
+                                                reinterpret_cast<uint64_t*>(stripe));
+    return hipGetLastError();
+}
+
+// ---- BLAKE3 compression (blake3.rs:122-340, 1283-1452) ----
+// Row layout (2^14 bits): cv [0,256), out_lo [256,512), m [512,1024), counter_lo | counter_hi |
+// block_len | flags [1024,1152), out_hi [1152,1408), then the 56 G blocks (183/183/182/182 bits
+// for round 1's column G's, 184 otherwise) and the constant at 11,706. Every region starts on
+// a 64-bit boundary, so each is its own stream and no word is shared.
+namespace b3 {
+constexpr uint32_t kIV[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                             0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+constexpr int kPerm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+constexpr int kLanes[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14}, {3, 7, 11, 15},
+                              {0, 5, 10, 15}, {1, 6, 11, 12}, {2, 7, 8, 13}, {3, 4, 9, 14}};
+struct MsgTable {
+    uint8_t v[7][8][2];  // (mx, my) message word of round r, G g (per_round_msg_idx)
+};
+constexpr MsgTable make_msg_table() {
+    MsgTable t{};
+    int perm[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+    for (int r = 0; r < 7; r++) {
+        for (int g = 0; g < 8; g++) {
+            t.v[r][g][0] = (uint8_t)perm[2 * g];
+            t.v[r][g][1] = (uint8_t)perm[2 * g + 1];
+        }
+        int next[16] = {};
+        for (int i = 0; i < 16; i++) next[i] = perm[kPerm[i]];
+        for (int i = 0; i < 16; i++) perm[i] = next[i];
+    }
+    return t;
+}
+__constant__ MsgTable kMsg = make_msg_table();
+constexpr uint32_t kGsBase = 1408, kZConst = 11706, kUseful = 11707;
+constexpr uint32_t kUsefulWords = (kUseful + 63) / 64;  // 183
+}  // namespace b3
+
+constexpr int kB3Threads = 128;
+
+// One thread per row o < 2^n_log; inputs[28 o ..] = (cv[8], m[16], counter_lo, counter_hi,
+// block_len, flags) for o < n_decl; rows past n_decl are zero (the partial drivers' dummies).
+__global__ __launch_bounds__(kB3Threads) void k_blake3_witness(const uint32_t* __restrict__ inputs,
+                                                                uint64_t n_decl, uint32_t n_log,
+                                                                uint64_t* __restrict__ z,
+                                                                uint64_t* __restrict__ a,
+                                                                uint64_t* __restrict__ b) {
+    __shared__ uint32_t msg[16 * kB3Threads];
+    const int tid = threadIdx.x;
+    const uint64_t row = (uint64_t)blockIdx.x * kB3Threads + tid;
+    if (row >= (1ull << n_log)) return;  // no barriers below
+    WitOut<NoShared> o{z, a, b, row, n_log, row < n_decl, nullptr};
+    const uint32_t* in = inputs + 28 * (o.live ? row : 0);
+    uint32_t s[16];
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] = o.live ? in[i] : 0u;
+    for (int i = 0; i < 16; i++) msg[i * kB3Threads + tid] = o.live ? in[8 + i] : 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        s[8 + i] = b3::kIV[i];
+        s[12 + i] = o.live ? in[24 + i] : 0u;
+    }
+    {  // cv, m, (counter_lo, counter_hi, block_len, flags): (z, a, b) = (v, v, 1)
+        BitStream cv(0);
+#pragma unroll
+        for (int w = 0; w < 8; w++) cv.push(o, s[w], s[w], 0xFFFFFFFFu, 32);
+        cv.finish(o);
+        BitStream mp(512);
+        for (int i = 0; i < 16; i++) {
+            const uint32_t v = msg[i * kB3Threads + tid];
+            mp.push(o, v, v, 0xFFFFFFFFu, 32);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) mp.push(o, s[12 + i], s[12 + i], 0xFFFFFFFFu, 32);
+        mp.finish(o);
+    }
+    BitStream gs(b3::kGsBase);
+#pragma unroll 1
+    for (int r = 0; r < 7; r++) {
+#pragma unroll
+        for (int gi = 0; gi < 8; gi++) {
+            const int la = b3::kLanes[gi][0], lb = b3::kLanes[gi][1], lc = b3::kLanes[gi][2],
+                      ld = b3::kLanes[gi][3];
+            const uint32_t mx = msg[b3::kMsg.v[r][gi][0] * kB3Threads + tid];
+            const uint32_t my = msg[b3::kMsg.v[r][gi][1] * kB3Threads + tid];
+            uint32_t maj1[3], rip1[3], t1[3], maj2[3], rip2[3], t2[3];
+            const uint32_t a1 = add3_parts(s[la], s[lb], mx, maj1, rip1);
+            gs.push3(o, maj1, 31);
+            gs.push3(o, rip1, 30);
+            const uint32_t d1 = sha2::ror(s[ld] ^ a1, 16);
+            uint32_t c1;
+            if (r == 0 && gi < 4) {
+                // c = IV[gi], a constant: rows for bits t..30 only (g_c1_rows, blake3.rs:154-160)
+                c1 = add_parts(b3::kIV[gi], d1, t1);
+                const uint32_t n = gi < 2 ? 30u : 29u, t = 31u - n, mask = (1u << n) - 1u;
+                const uint32_t tt[3] = {(t1[0] >> t) & mask, (t1[1] >> t) & mask,
+                                        (t1[2] >> t) & mask};
+                gs.push3(o, tt, n);
+            } else {
+                c1 = add_parts(s[lc], d1, t1);
+                gs.push3(o, t1, 31);
+            }
+            const uint32_t b1 = sha2::ror(s[lb] ^ c1, 12);
+            const uint32_t a2 = add3_parts(a1, b1, my, maj2, rip2);
+            const uint32_t d2 = sha2::ror(d1 ^ a2, 8);
+            const uint32_t c2 = add_parts(c1, d2, t2);
+            gs.push3(o, maj2, 31);
+            gs.push3(o, rip2, 30);
+            gs.push3(o, t2, 31);
+            s[la] = a2;
+            s[lb] = sha2::ror(b1 ^ c2, 7);
+            s[lc] = c2;
+            s[ld] = d2;
+        }
+    }
+    gs.push(o, 1u, 1u, 1u, 1);  // the constant wire, (1, 1, 1)
+    gs.finish(o);
+    o.emit(b3::kUsefulWords, 0, 0, 0);  // the upper half of the last useful chunk
+    // finalization rows: out_lo = s[w] ^ s[w+8], out_hi = s[w+8] ^ cv[w]
+    BitStream lo(256), hi(1152);
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        const uint32_t v = s[w] ^ s[w + 8];
+        lo.push(o, v, v, 0xFFFFFFFFu, 32);
+        const uint32_t cvw = o.live ? in[w] : 0u;
+        const uint32_t h = s[w + 8] ^ cvw;
+        hi.push(o, h, h, 0xFFFFFFFFu, 32);
+    }
+    lo.finish(o);
+    hi.finish(o);
+}
+
+hipError_t launch_blake3_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t n_log,
+                                 uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s) {
+    const uint64_t n = 1ull << n_log;
+    const uint32_t useful_chunks = (b3::kUseful + 127) / 128;  // 92
+    const uint64_t words = 128ull << n_log;                    // F128 per buffer
+    for (uint4* buf : {z, a, b}) {
+        hipError_t e = launch_zero_words(buf + ((uint64_t)useful_chunks << n_log),
+                                         words - ((uint64_t)useful_chunks << n_log), s);
+        if (e != hipSuccess) return e;
+    }
+    const unsigned grid = (unsigned)((n + kB3Threads - 1) / kB3Threads);
+    k_blake3_witness<<<grid, kB3Threads, 0, s>>>(inputs, n_decl, n_log,
+                                                 reinterpret_cast<uint64_t*>(z),
+                                                 reinterpret_cast<uint64_t*>(a),
+                                                 reinterpret_cast<uint64_t*>(b));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !stripe) return e;
+    const uint64_t total = (n / 8) * 256;
+    uint64_t g = (total + 255) / 256;
+    if (g > 16384) g = 16384;
+    k_stripe_from_z<<<(unsigned)g, 256, 0, s>>>(reinterpret_cast<const uint64_t*>(z), n_log, 256,
+                                                b3::kUsefulWords,
+                                                reinterpret_cast<uint64_t*>(stripe));
+    return hipGetLastError();
+}
+
+// ---- Keccak-f[1600] (keccak.rs:98-330, 436-643) ----
+// Row layout (u64 words of a 2^16-bit row): state_0 lanes 0..24, state_24 lanes 32..56, the
+// constant at word 64 (bit 0), t_r lanes at 65 + 25 r + lane; 42,560 useful bits. Rows are
+// whole lanes, so every store is one aligned u64 at its batch-major address.
+namespace keccak {
+__constant__ uint64_t kRC[24] = {
+    0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
+    0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
+    0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
+    0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
+    0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
+    0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+// rho offset of lane (x, y), indexed [x][y]
+constexpr int kRho[5][5] = {{0, 36, 3, 41, 18}, {1, 44, 10, 45, 2}, {62, 6, 43, 15, 61},
+                            {28, 55, 25, 21, 56}, {27, 20, 39, 8, 14}};
+constexpr uint32_t kUseful = 42560;
+constexpr uint32_t kUsefulWords = kUseful / 64;  // 665
+__device__ __forceinline__ uint64_t rotl(uint64_t v, int r) {
+    return r ? (v << r) | (v >> (64 - r)) : v;
+}
+}  // namespace keccak
+
+constexpr int kKecThreads = 128;
+
+// One thread per row o < 2^n_log; inputs[25 o ..] = the initial lanes for o < n_decl, the zero
+// state otherwise (the padding invocations of generate_witness_batch_major, keccak.rs:1228-1285).
+__global__ __launch_bounds__(kKecThreads) void k_keccak_witness(const uint64_t* __restrict__ inputs,
+                                                                 uint64_t n_decl, uint32_t n_log,
+                                                                 uint64_t* __restrict__ z,
+                                                                 uint64_t* __restrict__ a,
+                                                                 uint64_t* __restrict__ b) {
+    const uint64_t row = (uint64_t)blockIdx.x * kKecThreads + threadIdx.x;
+    if (row >= (1ull << n_log)) return;
+    auto at = [&](uint32_t w) { return 2 * ((((uint64_t)w >> 1) << n_log) + row) + (w & 1u); };
+    auto put = [&](uint32_t w, uint64_t vz, uint64_t va, uint64_t vb) {
+        const uint64_t i = at(w);
+        z[i] = vz;
+        a[i] = va;
+        b[i] = vb;
+    };
+    uint64_t s[25];
+    const bool live = row < n_decl;
+#pragma unroll
+    for (int l = 0; l < 25; l++) s[l] = live ? inputs[25 * row + l] : 0ull;
+#pragma unroll
+    for (int l = 0; l < 25; l++) put(l, s[l], s[l], ~0ull);
+    for (uint32_t w = 25; w < 32; w++) put(w, 0, 0, 0);
+    for (uint32_t w = 57; w < 64; w++) put(w, 0, 0, 0);
+    put(64, 1, 1, 1);
+    put(keccak::kUsefulWords, 0, 0, 0);  // the upper half of the last useful chunk
+#pragma unroll 1
+    for (int r = 0; r < 24; r++) {
+        uint64_t c[5], d[5], B[25];
+#pragma unroll
+        for (int x = 0; x < 5; x++) c[x] = s[x] ^ s[x + 5] ^ s[x + 10] ^ s[x + 15] ^ s[x + 20];
+#pragma unroll
+        for (int x = 0; x < 5; x++) d[x] = c[(x + 4) % 5] ^ keccak::rotl(c[(x + 1) % 5], 1);
+#pragma unroll
+        for (int y = 0; y < 5; y++)
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                const int sx = (x + 3 * y) % 5, sy = x;
+                B[x + 5 * y] = keccak::rotl(s[sx + 5 * sy] ^ d[sx], keccak::kRho[sx][sy]);
+            }
+        const uint32_t tb = 65 + 25 * r;
+#pragma unroll
+        for (int y = 0; y < 5; y++)
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                const uint64_t b1 = B[(x + 1) % 5 + 5 * y], b2 = B[(x + 2) % 5 + 5 * y];
+                const uint64_t t = ~b1 & b2;
+                put(tb + x + 5 * y, t, ~b1, b2);
+                s[x + 5 * y] = B[x + 5 * y] ^ t;
+            }
+        s[0] ^= keccak::kRC[r];
+    }
+#pragma unroll
+    for (int l = 0; l < 25; l++) put(32 + l, s[l], s[l], ~0ull);
+}
+
+hipError_t launch_keccak_witness(const uint64_t* inputs, uint64_t n_decl, uint32_t n_log,
+                                 uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s) {
+    const uint64_t n = 1ull << n_log;
+    const uint32_t useful_chunks = (keccak::kUseful + 127) / 128;  // 333
+    const uint64_t words = 512ull << n_log;                        // F128 per buffer
+    for (uint4* buf : {z, a, b}) {
+        hipError_t e = launch_zero_words(buf + ((uint64_t)useful_chunks << n_log),
+                                         words - ((uint64_t)useful_chunks << n_log), s);
+        if (e != hipSuccess) return e;
+    }
+    const unsigned grid = (unsigned)((n + kKecThreads - 1) / kKecThreads);
+    k_keccak_witness<<<grid, kKecThreads, 0, s>>>(inputs, n_decl, n_log,
+                                                  reinterpret_cast<uint64_t*>(z),
+                                                  reinterpret_cast<uint64_t*>(a),
+                                                  reinterpret_cast<uint64_t*>(b));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !stripe) return e;
+    const uint64_t total = (n / 8) * 1024;
+    uint64_t g = (total + 255) / 256;
+    if (g > 16384) g = 16384;
+    k_stripe_from_z<<<(unsigned)g, 256, 0, s>>>(reinterpret_cast<const uint64_t*>(z), n_log, 1024,
+                                                keccak::kUsefulWords,
                                                 reinterpret_cast<uint64_t*>(stripe));
     return hipGetLastError();
 }

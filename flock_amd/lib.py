@@ -159,6 +159,8 @@ def load():
         "flock_union_bind_statement": (I, [P, ctypes.c_char_p, P, S, ctypes.c_char_p, S]),
         "flock_hip_union_compact": (I, [P, ctypes.POINTER(Registry), P, U32, P, P]),
         "flock_hip_witness_sha256": (I, [P, P, S, U32, P, P, P, P]),
+        "flock_hip_witness_keccak": (I, [P, P, S, U32, P, P, P, P]),
+        "flock_hip_witness_blake3": (I, [P, P, S, U32, P, P, P, P]),
         "flock_hip_lc_csc_fold": (I, [P, ctypes.POINTER(Csc), P, F128, P, P, P]),
         "flock_hip_lc_partial_fold": (I, [P, P, U32, U32, S, S, P, P]),
         "flock_hip_lincheck_prove": (I, [P, P, ctypes.POINTER(Csc), P, U32, S, S, F128, P, P, I,
@@ -414,6 +416,22 @@ class Context:
                        stripe: DeviceBuffer | None = None):
         """generate_witness_batch_major_partial (r1cs_hashes/sha2.rs:1876-1898)."""
         _check(load().flock_hip_witness_sha256(self.h, inputs.ptr if inputs else None, n_decl,
+                                               n_log, z.ptr, a.ptr, b.ptr,
+                                               stripe.ptr if stripe else None))
+
+    def witness_blake3(self, inputs: DeviceBuffer | None, n_decl: int, n_log: int,
+                       z: DeviceBuffer, a: DeviceBuffer, b: DeviceBuffer,
+                       stripe: DeviceBuffer | None = None):
+        """generate_witness_batch_major_partial (r1cs_hashes/blake3.rs:2072-2088)."""
+        _check(load().flock_hip_witness_blake3(self.h, inputs.ptr if inputs else None, n_decl,
+                                               n_log, z.ptr, a.ptr, b.ptr,
+                                               stripe.ptr if stripe else None))
+
+    def witness_keccak(self, inputs: DeviceBuffer | None, n_decl: int, n_log: int,
+                       z: DeviceBuffer, a: DeviceBuffer, b: DeviceBuffer,
+                       stripe: DeviceBuffer | None = None):
+        """generate_witness_batch_major (r1cs_hashes/keccak.rs:1228-1285)."""
+        _check(load().flock_hip_witness_keccak(self.h, inputs.ptr if inputs else None, n_decl,
                                                n_log, z.ptr, a.ptr, b.ptr,
                                                stripe.ptr if stripe else None))
 

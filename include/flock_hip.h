@@ -312,6 +312,25 @@ int flock_hip_witness_sha256(flock_hip_ctx* ctx, const uint32_t* d_inputs, size_
                              uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
                              uint8_t* d_stripe);
 
+/* BLAKE3 compression witness, batch-major: generate_witness_batch_major_partial
+ * (r1cs_hashes/blake3.rs:2072-2088, rows from build_block_witness_ab_packed_into :1299-1452).
+ * d_inputs holds n_decl compressions of 28 u32 (cv[8], m[16], counter_lo, counter_hi,
+ * block_len, flags); rows n_decl..2^n_log are zero. d_z, d_a, d_b: 2^(n_log+7) F128 each at
+ * (chunk << n_log) + row of the 2^14-bit rows; d_stripe (may be NULL): 2^(n_log+11) bytes.
+ * 3 <= n_log <= 26. */
+int flock_hip_witness_blake3(flock_hip_ctx* ctx, const uint32_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe);
+/* Keccak-f[1600] permutation witness, batch-major: generate_witness_batch_major
+ * (r1cs_hashes/keccak.rs:1228-1285, rows from build_chain_witness_ab_packed_into :565-643).
+ * d_inputs holds n_decl initial states of 25 u64 lanes (lane x + 5y, bit z); rows n_decl..2^n_log
+ * run keccak_f on the zero state (the reference's padding invocations). d_z, d_a, d_b:
+ * 2^(n_log+9) F128 each at (chunk << n_log) + row of the 2^16-bit rows; d_stripe (may be NULL):
+ * 2^(n_log+13) bytes. 3 <= n_log <= 24. */
+int flock_hip_witness_keccak(flock_hip_ctx* ctx, const uint64_t* d_inputs, size_t n_decl,
+                             uint32_t n_log, flock_f128* d_z, flock_f128* d_a, flock_f128* d_b,
+                             uint8_t* d_stripe);
+
 /* ---- Fiat-Shamir transcript (host; FsChallenger, SHA-256 discipline) ---------------- */
 
 /* FsChallenger::new (challenger.rs:648-659) and the Challenger methods (:806-958). */

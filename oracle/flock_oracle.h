@@ -198,6 +198,29 @@ void or_csr_to_csc(size_t k, const uint64_t* ptr, const uint32_t* cols, uint32_t
 void or_csr_apply_bits(size_t k, const uint64_t* ptr, const uint32_t* cols, const uint64_t* z,
                        uint64_t* y);
 
+/* ---- Keccak-f[1600] R1CS (crates/flock-prover/src/r1cs_hashes/keccak.rs), keccak_r1cs.c ---- */
+void or_keccak_f(uint64_t s[25]); /* lanes s[x + 5y], keccak.rs:184-330 */
+/* build_chain_witness_ab_packed_into (keccak.rs:565-643): 1024 u64 each */
+void or_keccak_instance(const uint64_t initial[25], uint64_t* z, uint64_t* a, uint64_t* b);
+/* generate_witness_batch_major (keccak.rs:1228-1285): padding rows run keccak_f(0) */
+int or_keccak_witness_batch_major(const uint64_t* inputs, size_t n_decl, int n_log, or_f128* z,
+                                  or_f128* a, or_f128* b, uint8_t* stripe);
+
+/* ---- BLAKE3 compression R1CS (crates/flock-prover/src/r1cs_hashes/blake3.rs), blake3_r1cs.c ---- */
+/* {Z_CONST_POS, USEFUL_BITS, G_BASE[4], G_BASE[1]} (blake3.rs:233-247) */
+void or_blake3_layout(uint32_t out[4]);
+/* blake3_compress (crates/flock-core/src/hash.rs:101-142): the 16-word output state */
+void or_blake3_compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter,
+                        uint32_t blen, uint32_t flags, uint32_t out[16]);
+/* build_block_witness_ab_packed_into (blake3.rs:1299-1452): 256 u64 each */
+void or_blake3_instance(const uint32_t cv[8], const uint32_t m[16], uint64_t counter,
+                        uint32_t blen, uint32_t flags, uint64_t* z, uint64_t* a, uint64_t* b);
+/* generate_witness_batch_major_partial (blake3.rs:2072-2088); 28 u32 per declared row */
+int or_blake3_witness_batch_major(const uint32_t* inputs, size_t n_decl, int n_log, or_f128* z,
+                                  or_f128* a, or_f128* b, uint8_t* stripe);
+/* build_matrices (blake3.rs:659-783) as CSR; a_ptr/b_ptr have 16385 entries */
+int or_blake3_matrices(uint64_t* a_ptr, uint32_t* a_cols, uint64_t* b_ptr, uint32_t* b_cols);
+
 #ifdef __cplusplus
 }
 #endif

@@ -284,82 +284,8 @@ int or_sha2_witness_batch_major(const uint32_t* inputs, size_t n_decl, int n_log
 /* Symbolic rows (sha2.rs:250-705): supports are sorted, deduplicated       */
 /* column lists; XOR of two supports is their symmetric difference.         */
 /* ======================================================================== */
-typedef struct {
-    uint32_t* v;
-    uint32_t n;
-} sup;
+#include "r1cs_sup.h"
 
-static sup sup_new(uint32_t cap) {
-    sup s;
-    s.v = (uint32_t*)malloc(sizeof(uint32_t) * (cap ? cap : 1));
-    s.n = 0;
-    return s;
-}
-static sup sup_one(uint32_t x) {
-    sup s = sup_new(1);
-    s.v[0] = x;
-    s.n = 1;
-    return s;
-}
-static sup sup_copy(sup a) {
-    sup s = sup_new(a.n);
-    memcpy(s.v, a.v, sizeof(uint32_t) * a.n);
-    s.n = a.n;
-    return s;
-}
-static void sup_free(sup* s) {
-    free(s->v);
-    s->v = NULL;
-    s->n = 0;
-}
-static sup sup_xor(sup a, sup b) { /* xor_sup, :260-278 */
-    sup o = sup_new(a.n + b.n);
-    uint32_t i = 0, j = 0;
-    while (i < a.n && j < b.n) {
-        if (a.v[i] < b.v[j]) o.v[o.n++] = a.v[i++];
-        else if (a.v[i] > b.v[j]) o.v[o.n++] = b.v[j++];
-        else {
-            i++;
-            j++;
-        }
-    }
-    while (i < a.n) o.v[o.n++] = a.v[i++];
-    while (j < b.n) o.v[o.n++] = b.v[j++];
-    return o;
-}
-static sup sup_xor3(sup a, sup b, sup c) {
-    sup t = sup_xor(a, b), o = sup_xor(t, c);
-    sup_free(&t);
-    return o;
-}
-
-typedef struct {
-    sup b[32];
-} word;
-
-static void word_free(word* w) {
-    for (int i = 0; i < 32; i++) sup_free(&w->b[i]);
-}
-static word word_zero(void) {
-    word w;
-    for (int i = 0; i < 32; i++) w.b[i] = sup_new(0);
-    return w;
-}
-static word word_copy(const word* x) {
-    word w;
-    for (int i = 0; i < 32; i++) w.b[i] = sup_copy(x->b[i]);
-    return w;
-}
-static word word_wire(unsigned base) {
-    word w;
-    for (int i = 0; i < 32; i++) w.b[i] = sup_one(base + i);
-    return w;
-}
-static word word_xor(const word* x, const word* y) {
-    word w;
-    for (int i = 0; i < 32; i++) w.b[i] = sup_xor(x->b[i], y->b[i]);
-    return w;
-}
 /* rot_xor3 / sigma_xor (:288-320): bit i = w[(i+r1)%32] ^ w[(i+r2)%32] ^ (w[i+s] or rot) */
 static word word_sigma(const word* x, int r1, int r2, int r3, int is_shift) {
     word w;

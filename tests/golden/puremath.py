@@ -245,3 +245,125 @@ def lagrange_f128(nodes: list[int], z: int) -> list[int]:
                 den = mul(den, xi ^ xj)
         w.append(mul(num, inv(den)))
     return w
+
+
+# ---------------------------------------------------------------------------
+# Keccak-f[1600] from FIPS 202 (lane A[x + 5y], bit z of the 64-bit lane)
+# ---------------------------------------------------------------------------
+
+_KECCAK_RHO = [[0, 36, 3, 41, 18], [1, 44, 10, 45, 2], [62, 6, 43, 15, 61],
+               [28, 55, 25, 21, 56], [27, 20, 39, 8, 14]]  # r[x][y], FIPS 202 Table 2
+M64 = (1 << 64) - 1
+
+
+def _rc(t: int) -> int:
+    """rc(t) of FIPS 202 Algorithm 5 (LFSR x^8 + x^6 + x^5 + x^4 + 1)."""
+    if t % 255 == 0:
+        return 1
+    r = 1
+    for _ in range(t % 255):
+        r <<= 1
+        if r & 0x100:
+            r ^= 0x171
+    return r & 1
+
+
+def keccak_round_constant(ir: int) -> int:
+    rc = 0
+    for j in range(7):
+        if _rc(j + 7 * ir):
+            rc |= 1 << ((1 << j) - 1)
+    return rc
+
+
+def _rotl64(v, r):
+    r %= 64
+    return ((v << r) | (v >> (64 - r))) & M64 if r else v
+
+
+def keccak_round_parts(A):
+    """One round's (B = pi(rho(theta(A))), t = chi AND term, A') on 25 lanes."""
+    C = [A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20] for x in range(5)]
+    D = [C[(x - 1) % 5] ^ _rotl64(C[(x + 1) % 5], 1) for x in range(5)]
+    T = [A[i] ^ D[i % 5] for i in range(25)]
+    B = [0] * 25
+    for x in range(5):
+        for y in range(5):
+            # pi: A'[x, y] = A[(x + 3y) mod 5, x]; rho offset of the source lane
+            sx, sy = (x + 3 * y) % 5, x
+            B[x + 5 * y] = _rotl64(T[sx + 5 * sy], _KECCAK_RHO[sx][sy])
+    t = [(~B[(i % 5 + 1) % 5 + 5 * (i // 5)] & M64) & B[(i % 5 + 2) % 5 + 5 * (i // 5)]
+         for i in range(25)]
+    return B, t
+
+
+def keccak_f(A):
+    A = list(A)
+    for ir in range(24):
+        B, t = keccak_round_parts(A)
+        A = [B[i] ^ t[i] for i in range(25)]
+        A[0] ^= keccak_round_constant(ir)
+    return A
+
+
+def sha3_256(msg: bytes) -> bytes:
+    rate = 136
+    m = bytearray(msg) + b"\x06"
+    m += bytes(-len(m) % rate)
+    m[-1] |= 0x80
+    A = [0] * 25
+    for off in range(0, len(m), rate):
+        for i in range(rate // 8):
+            A[i] ^= int.from_bytes(m[off + 8 * i:off + 8 * i + 8], "little")
+        A = keccak_f(A)
+    return b"".join(A[i].to_bytes(8, "little") for i in range(4))
+
+
+# ---------------------------------------------------------------------------
+# BLAKE3 compression from the BLAKE3 specification (section 2.2)
+# ---------------------------------------------------------------------------
+
+BLAKE3_IV = [0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B05688C,
+             0x1F83D9AB, 0x5BE0CD19]
+_B3_PERM = [2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8]
+M32 = (1 << 32) - 1
+
+
+def _ror32(x, n):
+    return ((x >> n) | (x << (32 - n))) & M32
+
+
+def blake3_compress(cv, m, counter, block_len, flags):
+    v = list(cv) + BLAKE3_IV[:4] + [counter & M32, counter >> 32, block_len, flags]
+    m = list(m)
+
+    def g(a, b, c, d, x, y):
+        v[a] = (v[a] + v[b] + x) & M32
+        v[d] = _ror32(v[d] ^ v[a], 16)
+        v[c] = (v[c] + v[d]) & M32
+        v[b] = _ror32(v[b] ^ v[c], 12)
+        v[a] = (v[a] + v[b] + y) & M32
+        v[d] = _ror32(v[d] ^ v[a], 8)
+        v[c] = (v[c] + v[d]) & M32
+        v[b] = _ror32(v[b] ^ v[c], 7)
+
+    for r in range(7):
+        g(0, 4, 8, 12, m[0], m[1])
+        g(1, 5, 9, 13, m[2], m[3])
+        g(2, 6, 10, 14, m[4], m[5])
+        g(3, 7, 11, 15, m[6], m[7])
+        g(0, 5, 10, 15, m[8], m[9])
+        g(1, 6, 11, 12, m[10], m[11])
+        g(2, 7, 8, 13, m[12], m[13])
+        g(3, 4, 9, 14, m[14], m[15])
+        m = [m[_B3_PERM[i]] for i in range(16)]
+    return [v[i] ^ v[i + 8] for i in range(8)] + [v[i + 8] ^ cv[i] for i in range(8)]
+
+
+def blake3_hash_short(msg: bytes) -> bytes:
+    """BLAKE3 of a message of at most 64 bytes: one chunk, one block, root output."""
+    assert len(msg) <= 64
+    blk = msg + bytes(64 - len(msg))
+    words = [int.from_bytes(blk[4 * i:4 * i + 4], "little") for i in range(16)]
+    out = blake3_compress(BLAKE3_IV, words, 0, len(msg), 1 | 2 | 8)  # CHUNK_START|END|ROOT
+    return b"".join(w.to_bytes(4, "little") for w in out[:8])

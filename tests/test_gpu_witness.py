@@ -97,3 +97,73 @@ def test_zerocheck_on_sha256_witness(ctx):
     got = ctx.zerocheck_prove(F.FsChallenger(dom), a, b, z, m, True)
     for g, e in zip(got, exp):
         assert np.array_equal(g, e)
+
+
+@pytest.mark.parametrize("n_log,n_decl", [(3, 8), (3, 3), (7, 100), (10, 1024), (11, 2000)])
+def test_keccak_witness(ctx, n_log, n_decl):
+    from test_oracle_keccak_r1cs import batch_major, random_states
+    say(f"case: keccak witness n_log={n_log} n_decl={n_decl}")
+    inputs = random_states(n_decl, 900 + n_log)
+    ez, ea, eb, es = batch_major(inputs, n_log)
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 512 * n) for _ in range(3))
+    stripe = ctx.alloc(8192 * n)
+    ctx.witness_keccak(ctx.upload(np.ascontiguousarray(inputs, dtype=np.uint64)), n_decl, n_log,
+                       z, a, b, stripe)
+    assert np.array_equal(z.download_f128(512 * n), ez)
+    assert np.array_equal(a.download_f128(512 * n), ea)
+    assert np.array_equal(b.download_f128(512 * n), eb)
+    assert np.array_equal(np.frombuffer(stripe.download(8192 * n), dtype=np.uint8), es)
+
+
+@pytest.mark.parametrize("n_log,n_decl", [(3, 8), (3, 1), (7, 100), (10, 1024), (12, 4000)])
+def test_blake3_witness(ctx, n_log, n_decl):
+    from test_oracle_blake3_r1cs import batch_major, random_inputs
+    say(f"case: blake3 witness n_log={n_log} n_decl={n_decl}")
+    inputs = random_inputs(n_decl, 1100 + n_log)
+    ez, ea, eb, es = batch_major(inputs, n_log)
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 128 * n) for _ in range(3))
+    stripe = ctx.alloc(2048 * n)
+    ctx.witness_blake3(ctx.upload(np.ascontiguousarray(inputs, dtype=np.uint32)), n_decl, n_log,
+                       z, a, b, stripe)
+    assert np.array_equal(z.download_f128(128 * n), ez)
+    assert np.array_equal(a.download_f128(128 * n), ea)
+    assert np.array_equal(b.download_f128(128 * n), eb)
+    assert np.array_equal(np.frombuffer(stripe.download(2048 * n), dtype=np.uint8), es)
+
+
+def test_lincheck_and_zerocheck_on_blake3_witness(ctx):
+    # the BLAKE3 circuit (restated A_0, B_0) with the GPU witness: lincheck at K = 2^14 and the
+    # zerocheck over (a, b, c = z), both against the oracle
+    from test_oracle_blake3_r1cs import matrices, random_inputs
+    from test_oracle_lincheck import lincheck_prove
+    from test_oracle_zerocheck import zerocheck_prove
+    n_log, n_decl = 7, 120
+    say(f"case: lincheck+zerocheck on blake3 witness n_log={n_log}")
+    _, _, csc_a, csc_b = matrices()
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 128 * n) for _ in range(3))
+    stripe = ctx.alloc(2048 * n)
+    ctx.witness_blake3(ctx.upload(random_inputs(n_decl, 55)), n_decl, n_log, z, a, b, stripe)
+    hz, ha, hb = (np.frombuffer(x.download(16 * 128 * n), dtype=np.uint8).copy()
+                  for x in (z, a, b))
+    assert np.array_equal(ha & hb, hz)
+    m = n_log + 14
+    dom = b"flock-blake3-witness"
+    exp = zerocheck_prove(dom, ha, hb, hz, m, 1)
+    got = ctx.zerocheck_prove(F.FsChallenger(dom), a, b, z, m, True)
+    for g, e in zip(got, exp):
+        assert np.array_equal(g, e)
+    k_log = 14
+    L = k_log - 6
+    pts = O.arr_to_ints(O.splitmix(0xB3, 1 + L + n_log))
+    zs = np.frombuffer(stripe.download(2048 * n), dtype=np.uint8).copy()
+    pin, ub = 11_706, 11_707
+    exp = lincheck_prove(dom, k_log, csc_a, csc_b, pin, zs, n_log, ub, n_decl, pts[0],
+                         pts[1:1 + L], pts[1 + L:], 1)
+    circuit, keep = ctx.csc(k_log, *csc_a, *csc_b, const_pin=pin)
+    got = ctx.lincheck_prove(F.FsChallenger(dom), circuit, stripe, n_log, ub, n_decl, pts[0],
+                             pts[1:1 + L], pts[1 + L:], True)
+    for name in ("rounds", "z_partial", "matrix_evals", "claim", "nonces"):
+        assert np.array_equal(got[name], exp[name]), name
