@@ -176,7 +176,9 @@ def main():
         fs.bind_statement(bytes(32), [n], cap[0])
         ctx.sync()
         t0 = time.perf_counter()
-        r1, ml, fin, nonces, claim = ctx.zerocheck_prove(fs, a, b, z, m_zc, True)
+        # the prover captures s_hat_v_c for the PCS opening (prover.rs:1095-1103)
+        r1, ml, fin, nonces, claim, _ = ctx.zerocheck_prove(fs, a, b, z, m_zc, True,
+                                                            capture=True)
         times.setdefault("zerocheck", []).append((time.perf_counter() - t0) * 1e3)
         mlv = [int(v[0]) | (int(v[1]) << 64) for v in claim[1:1 + (m_zc - 6)]]
         x_rest, x_outer = F.x_ab_from_mlv(reg, mlv)

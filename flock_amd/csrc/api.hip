@@ -412,8 +412,13 @@ struct flock_hip_ctx {
     uint4* zc_ext = nullptr;
     uint8_t* zc_logexp = nullptr;
     uint4* zc_phipow = nullptr;
+    uint4* zc_ext_bs = nullptr;    // bit-sliced extension table (same shape as zc_ext)
+    uint4* zc_phim = nullptr;      // phi8(L_s(64 + i)) at s * 64 + i
     uint4* zc_partials = nullptr;  // 512 groups x 1024 accumulators
-    uint4* zc_out = nullptr;       // 128 words
+    uint4* zc_out = nullptr;       // 2048 words: message [0,128), shat [128,256), red [1024,2048)
+    flock_f128* zc_host = nullptr;  // pinned, 256 words: message and shat
+    uint4* zc_shat() { return zc_out + 128; }
+    uint4* zc_red() { return zc_out + 1024; }
 
     int ensure_zc() {
         if (zc_ext) return 0;
@@ -440,6 +445,17 @@ struct flock_hip_ctx {
                 for (int bb = 0; bb < 4; bb++)
                     if ((v >> bb) & 1)
                         for (int i = 0; i < 64; i++) ext[(g * 16 + v) * 64 + i] ^= M[4 * g + bb][i];
+        // the same entries bit-sliced: word 2b + h, bit q = bit b of byte 32h + q
+        std::vector<uint32_t> ext_bs(16 * 16 * 16, 0);
+        for (int e = 0; e < 16 * 16; e++)
+            for (int i = 0; i < 64; i++)
+                for (int bb = 0; bb < 8; bb++)
+                    if ((ext[e * 64 + i] >> bb) & 1)
+                        ext_bs[e * 16 + 2 * bb + (i >> 5)] |= 1u << (i & 31);
+        const F* phi0 = phi8_table();
+        std::vector<F> phim(64 * 64);
+        for (int s0 = 0; s0 < 64; s0++)
+            for (int i = 0; i < 64; i++) phim[s0 * 64 + i] = phi0[M[s0][i]];
         // log/exp with generator 0x03; exp doubled so log[u] + log[v] needs no reduction
         std::vector<uint8_t> le(256 + 512, 0);
         uint8_t x = 1;
@@ -453,14 +469,19 @@ struct flock_hip_ctx {
         F pp[8];
         for (int bb = 0; bb < 8; bb++) pp[bb] = phi[1 << bb];
         if (hipMalloc(&zc_ext, ext.size()) != hipSuccess ||
+            hipMalloc(&zc_ext_bs, ext_bs.size() * 4) != hipSuccess ||
+            hipMalloc(&zc_phim, phim.size() * 16) != hipSuccess ||
             hipMalloc(&zc_logexp, le.size()) != hipSuccess ||
             hipMalloc(&zc_phipow, sizeof(pp)) != hipSuccess ||
             hipMalloc(&zc_partials, sizeof(uint4) * 512 * 1024) != hipSuccess ||
-            hipMalloc(&zc_out, sizeof(uint4) * 128) != hipSuccess) {
+            hipMalloc(&zc_out, sizeof(uint4) * 2048) != hipSuccess ||
+            hipHostMalloc(&zc_host, sizeof(flock_f128) * 256) != hipSuccess) {
             (void)hipGetLastError();
             return fail(FLOCK_HIP_E_NOMEM, "zerocheck tables");
         }
         HIP_TRY(hipMemcpy(zc_ext, ext.data(), ext.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(zc_ext_bs, ext_bs.data(), ext_bs.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(zc_phim, phim.data(), phim.size() * 16, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(zc_logexp, le.data(), le.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(zc_phipow, pp, sizeof(pp), hipMemcpyHostToDevice));
         return 0;
@@ -568,10 +589,13 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& kv : c->evals) hipFree(kv.second);
     if (c->zc_ext) hipFree(c->zc_ext);
+    if (c->zc_ext_bs) hipFree(c->zc_ext_bs);
+    if (c->zc_phim) hipFree(c->zc_phim);
     if (c->zc_logexp) hipFree(c->zc_logexp);
     if (c->zc_phipow) hipFree(c->zc_phipow);
     if (c->zc_partials) hipFree(c->zc_partials);
     if (c->zc_out) hipFree(c->zc_out);
+    if (c->zc_host) hipHostFree(c->zc_host);
     if (c->scratch) hipFree(c->scratch);
     if (c->partials) hipFree(c->partials);
     if (c->dout) hipFree(c->dout);
@@ -862,7 +886,7 @@ static int zc_round1_impl(flock_hip_ctx* c, const uint8_t* a, const uint8_t* b, 
     if (int rc = c->ensure_zc()) return rc;
     const uint64_t rows = 1ull << (m - 6);
     LAUNCH(flock::launch_zc_round1(a, b, cc, eq, rows, c->zc_ext, c->zc_logexp, c->zc_phipow,
-                                   c->zc_partials, c->zc_out, c->stream));
+                                   c->zc_partials, c->zc_red(), c->zc_out, c->stream));
     HIP_TRY(hipMemcpyAsync(out, c->zc_out, 128 * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -910,8 +934,8 @@ int flock_hip_fold_1b_rows(flock_hip_ctx* c, const flock_f128* rows, size_t n_ro
     if (!rows || !t || !out || n_rows == 0)
         return fail(FLOCK_HIP_E_INVALID, "bad fold_1b_rows arguments");
     if (int rc = c->ensure_zc()) return rc;
-    LAUNCH(flock::launch_fold_1b_rows(U(rows), U(t), n_rows, c->zc_partials, c->zc_out,
-                                      c->stream));
+    LAUNCH(flock::launch_fold_1b_rows(U(rows), U(t), n_rows, c->zc_partials, c->zc_red(),
+                                      c->zc_out, c->stream));
     HIP_TRY(hipMemcpyAsync(out, c->zc_out, 128 * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -941,11 +965,11 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     if (int rc = c->ensure_zc()) return rc;
     const uint32_t n_mlv = m - ks;
     const uint64_t N = 1ull << n_mlv;  // rows = round-2 table length
-    // scratch: Az, Bz (N each), ping-pong P0 (2 x N/2), P1 (2 x N/4), eq EA (N), EB (N/2),
-    // EC (N/4), fold table (2048), r (64), build_eq scratch
-    const uint64_t h = N / 2, q = N / 4 > 0 ? N / 4 : 1;
+    // scratch: Az, Bz (N each), ping-pong P0 (2 x N/2), P1 (2 x N/4), eq EB (N/2), EC (N/4),
+    // E9 (N/8), fold table (2048), r (64), build_eq scratch
+    const uint64_t h = N / 2, q = N / 4 > 0 ? N / 4 : 1, e9n = N / 8;
     const uint64_t eqs = 2 * (1ull << (n_mlv / 2 + 1));
-    const uint64_t words = 2 * N + 2 * h + 2 * q + N + h + q + 2048 + 64 + eqs;
+    const uint64_t words = 2 * N + 2 * h + 2 * q + h + q + e9n + 2048 + 64 + eqs;
     if (int rc = c->ensure_scratch(words * 16)) return rc;
     uint4* base = reinterpret_cast<uint4*>(c->scratch);
     uint4* Az = base;
@@ -954,10 +978,10 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     uint4* P0b = P0f + h;
     uint4* P1f = P0b + h;
     uint4* P1b = P1f + q;
-    uint4* EA = P1b + q;
-    uint4* EB = EA + N;
+    uint4* EB = P1b + q;
     uint4* EC = EB + h;
-    uint4* TBL = EC + q;
+    uint4* E9 = EC + q;
+    uint4* TBL = E9 + e9n;
     uint4* R = TBL + 2048;
     uint4* EQS = R + 64;
     uint4* bufs[2][2] = {{P0f, P0b}, {P1f, P1b}};
@@ -980,23 +1004,25 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
             r[ks + 3 + i] = flock_f128{v.lo, v.hi};
         }
     }
-    // eq over r[6..m) (2^(m-6) words)
+    // eq over r[7..m) (2^(m-7) words: round 2 and shat) and over r[9..m) (round 1's groups)
     HIP_TRY(hipMemcpyAsync(R, r.data() + ks, n_mlv * 16, hipMemcpyHostToDevice, c->stream));
-    LAUNCH(flock::launch_build_eq(R, (int)n_mlv, EA, EQS, c->stream));
+    LAUNCH(flock::launch_build_eq(R + 1, (int)n_mlv - 1, EB, EQS, c->stream));
+    LAUNCH(flock::launch_build_eq(R + 3, (int)n_mlv - 3, E9, EQS, c->stream));
 
     // ---- 2. round 1 ----
-    if (int rc = zc_round1_impl(c, d_a, d_b, d_c, m, EA, h_round1)) return rc;
-    // eq over r[7..m) = pairwise XOR of eq over r[6..m); it weights round 2 and s_hat_v_c
-    LAUNCH(flock::launch_eq_fold(EA, h, EB, c->stream));
-    if (h_s_hat_v_c) {
-        // s_hat_v_c[lane | b7 << 6] = sum_y eq(r[7..m), y) c_bit(y*128 + b7*64 + lane)
-        // (univariate_skip.rs:354-383): fold_1b_rows of c viewed as 128-bit rows.
-        LAUNCH(flock::launch_fold_1b_rows(reinterpret_cast<const uint4*>(d_c), EB, h,
-                                          c->zc_partials, c->zc_out, c->stream));
-        HIP_TRY(hipMemcpyAsync(h_s_hat_v_c, c->zc_out, 128 * 16, hipMemcpyDeviceToHost,
-                               c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    // p_ab over groups of 8 rows (r[6..9) are phi8 constants); shat[lane | b7 << 6] =
+    // sum_y eq(r[7..m), y) c_bit(y*128 + b7*64 + lane) (univariate_skip.rs:354-383), from which
+    // p_c follows.
+    LAUNCH(flock::launch_zc_round1_groups(d_a, d_b, E9, N / 8, c->zc_ext_bs, c->zc_phipow,
+                                          c->zc_partials, c->zc_red(), c->zc_out, c->stream));
+    LAUNCH(flock::launch_fold_1b_rows(reinterpret_cast<const uint4*>(d_c), EB, h, c->zc_partials,
+                                      c->zc_red(), c->zc_shat(), c->stream));
+    LAUNCH(flock::launch_zc_round1_c(c->zc_shat(), c->zc_phim, U(r[ks]), c->zc_out + 64,
+                                     c->stream));
+    HIP_TRY(hipMemcpyAsync(c->zc_host, c->zc_out, 256 * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(h_round1, c->zc_host, 128 * 16);
+    if (h_s_hat_v_c) std::memcpy(h_s_hat_v_c, c->zc_host + 128, 128 * 16);
     flock_fs_observe_f128_slice(fs, h_round1, 64);
     flock_fs_observe_f128_slice(fs, h_round1 + 64, 64);
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for((1u << (ks + 1)) - 1));

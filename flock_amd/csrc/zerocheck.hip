@@ -144,16 +144,15 @@ __global__ __launch_bounds__(kR1Threads) void k_zc_round1(
     for (int q = 0; q < 4; q++) partials[(size_t)blockIdx.x * kR1Acc + t + kR1Threads * q] = acc[q];
 }
 
-// S[j] = XOR over groups of partials; then p[k] = sum_b phi8(2^b) S[8k + b] for k < 128
-// (k < 64: p_ab, k >= 64: p_c). One thread per accumulator column, then 128 combiners.
-__global__ __launch_bounds__(kR1Acc) void k_zc_round1_finish(const f128* __restrict__ partials,
-                                                              int groups,
+// red[j] = XOR over blocks of the partials (k_col_xor_reduce); then
+// p[k] = sum_b phi8(2^b) S[8k + b] for k < 128 (k < 64: p_ab, k >= 64: p_c). One thread per
+// accumulator column, then 128 combiners.
+__global__ __launch_bounds__(kR1Acc) void k_zc_round1_finish(const f128* __restrict__ red,
                                                               const f128* __restrict__ phi_pow,
                                                               f128* __restrict__ out) {
     __shared__ f128 s[kR1Acc];
     const int j = threadIdx.x;
-    f128 acc = f128_zero();
-    for (int g = 0; g < groups; g++) acc = f128_add(acc, partials[(size_t)g * kR1Acc + j]);
+    const f128 acc = red[j];
     // accumulator column j = t + 256 q holds byte (t>>3) + 32 q, bit t & 7
     const int t = j & 255, q = j >> 8;
     const int byte = (t >> 3) + 32 * q, bit = t & 7;
@@ -201,57 +200,351 @@ __global__ __launch_bounds__(256) void k_eq_fold(const f128* __restrict__ in, ui
         out[z] = f128_add(in[2 * z], in[2 * z + 1]);
 }
 
+// ---------------------------------------------------------------------------
+// Method of four Russians over 8-item groups, shared by the round-1 group kernel and
+// fold_1b_rows. A batch of up to 256 items is staged in LDS: item n's W words at
+// ys[n * STRIDE], its weight at eqs[n]. For group g (items 8g..8g+7),
+// tabs[32 g + 16 h + v] = XOR of eqs[8g + 4h + i] over the set bits i of v, so that
+// sum_i eqs[8g + i] * bit_i(p) = tabs[32g + (p & 15)] ^ tabs[32g + 16 + (p >> 4)].
+// Thread t owns slot t / (8W) and bits 32w + 8k + bt (k < 4) with w = (t % 8W) >> 3,
+// bt = t & 7: one LDS word per item gives the pattern of all four. Slots take the groups
+// g = slot (mod 256 / 8W). The partial of bit j in slot s is at s * 32W + j.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void m4rm_tabs(const f128* eqs, f128* tabs) {
+    for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+        const f128* q = eqs + 8 * (e >> 5) + 4 * ((e >> 4) & 1);
+        const int n = e & 15;
+        f128 te = f128_zero();
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            if ((n >> i) & 1) te = f128_add(te, q[i]);
+        tabs[e] = te;
+    }
+}
+
+template <int W, int STRIDE>
+__device__ __forceinline__ void m4rm_accumulate(const uint32_t* ys, const f128* tabs, int ngroups,
+                                                f128 acc[4]) {
+    constexpr int kSlots = 256 / (8 * W);
+    const int t = threadIdx.x;
+    const int slot = t / (8 * W), w = (t % (8 * W)) >> 3, bt = t & 7;
+    for (int g = slot; g < ngroups; g += kSlots) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) v |= ((ys[(8 * g + i) * STRIDE + w] >> bt) & 0x01010101u) << i;
+        const f128* T = tabs + 32 * g;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t pat = (v >> (8 * k)) & 0xffu;
+            acc[k] = f128_add3(acc[k], T[pat & 15u], T[16 + (pat >> 4)]);
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void m4rm_store(const f128 acc[4], f128* partials_block) {
+    const int t = threadIdx.x;
+    const int slot = t / (8 * W), w = (t % (8 * W)) >> 3, bt = t & 7;
+#pragma unroll
+    for (int k = 0; k < 4; k++) partials_block[slot * 32 * W + 32 * w + 8 * k + bt] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Round 1 with the protocol's inner constants (the zerocheck driver's path).
+//
+// The driver fixes r[6..9) = phi8(0xF7), phi8(0x53), phi8(0xB5)
+// (zerocheck/univariate_skip_optimized.rs:81, zerocheck.rs:644-669). phi8 is a field
+// embedding GF(2^8) -> GF(2^128), so eq(r[6..m), x) = phi8(f(x mod 8)) * eq(r[9..m), x / 8)
+// with f(l) = prod_t (bit t of l ? s_t : 1 + s_t) in GF(2^8), and with G = x / 8
+//   p_ab[i] = sum_G eq(r[9..m), G) phi8(Q_i(G)),
+//   Q_i(G)  = sum_{l < 8} f(l) A_{8G+l}(lambda_i) B_{8G+l}(lambda_i)     (in GF(2^8)).
+// The bit accumulation then runs over 2^(m-9) group vectors instead of 2^(m-6) rows.
+// Q is bit-sliced: word w = 2b + h holds bit b of Q_i for i = 32h .. 32h+31. The extension
+// table holds bit-sliced entries, the byte products are a bit-sliced GF(2^8) multiply
+// modulo x^8 + x^4 + x^3 + x + 1, and f(l) is a fixed GF(2)-linear map of the 8 planes.
+// Bit j of the accumulation is (b, i) with j = 64 b + i.
+// ---------------------------------------------------------------------------
+constexpr int kQW = 16;        // words of a group vector (512 bits)
+constexpr int kQStride = 20;   // LDS words per staged group vector (b128 stores hit distinct banks)
+
+constexpr uint8_t f8_mul_c(uint8_t a, uint8_t b) {
+    uint32_t r = 0;
+    for (int i = 0; i < 8; i++)
+        if ((b >> i) & 1) r ^= (uint32_t)a << i;
+    for (int bit = 14; bit >= 8; bit--)
+        if ((r >> bit) & 1) r ^= 0x11Bu << (bit - 8);
+    return (uint8_t)r;
+}
+constexpr uint8_t f8_inv_c(uint8_t a) {  // a^254
+    uint8_t r = 1, sq = a;
+    for (int i = 1; i < 8; i++) {
+        sq = f8_mul_c(sq, sq);
+        r = f8_mul_c(r, sq);
+    }
+    return r;
+}
+// f(l) = prod_t (l_t ? s_t : 1 + s_t) = C * prod_t w_t^(l_t) with w_t = s_t / (1 + s_t) and
+// C = prod_t (1 + s_t); s = 0xF7, 0x53, 0xB5. Maps 0..2 multiply by w_0..w_2, map 3 by C.
+// m[b] = the set of input planes a such that bit b of v * x^a is set.
+struct F8Maps {
+    uint8_t m[4][8];
+};
+constexpr F8Maps make_skip3_maps() {
+    F8Maps s{};
+    const uint8_t small[3] = {0xF7, 0x53, 0xB5};
+    uint8_t v[4] = {0, 0, 0, 1};
+    for (int t = 0; t < 3; t++) {
+        const uint8_t one_s = (uint8_t)(small[t] ^ 1);
+        v[t] = f8_mul_c(small[t], f8_inv_c(one_s));
+        v[3] = f8_mul_c(v[3], one_s);
+    }
+    for (int k = 0; k < 4; k++)
+        for (int a = 0; a < 8; a++) {
+            const uint8_t p = f8_mul_c(v[k], (uint8_t)(1u << a));
+            for (int b = 0; b < 8; b++)
+                if ((p >> b) & 1) s.m[k][b] |= (uint8_t)(1u << a);
+        }
+    return s;
+}
+constexpr F8Maps kSkip3 = make_skip3_maps();
+
+// out = v_K * in + add, bit-sliced (16 words = 8 planes x 2 halves); out may alias in or add.
+template <int K>
+__device__ __forceinline__ void f8_scale_add_bs(const uint32_t in[16], const uint32_t add[16],
+                                                uint32_t out[16]) {
+    uint32_t o[16];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint32_t v = add[2 * b + h];
+#pragma unroll
+            for (int a = 0; a < 8; a++)
+                if ((kSkip3.m[K][b] >> a) & 1) v ^= in[2 * a + h];
+            o[2 * b + h] = v;
+        }
+#pragma unroll
+    for (int w = 0; w < 16; w++) out[w] = o[w];
+}
+
+// y = bit-sliced extension of the 64-bit row (w0 = bits 0..31, w1 = bits 32..63).
+__device__ __forceinline__ void ext_bitsliced(const uint4* ext, uint32_t w0, uint32_t w1,
+                                              uint32_t y[16]) {
+#pragma unroll
+    for (int w = 0; w < 16; w++) y[w] = 0u;
+#pragma unroll 2
+    for (int g = 0; g < 16; g++) {
+        const uint32_t n = ((g < 8 ? w0 : w1) >> (4 * (g & 7))) & 15u;
+        const uint4* e = ext + g * 64 + n;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint4 v = e[16 * k];
+            y[4 * k] ^= v.x;
+            y[4 * k + 1] ^= v.y;
+            y[4 * k + 2] ^= v.z;
+            y[4 * k + 3] ^= v.w;
+        }
+    }
+}
+
+// p = A(lambda) * B(lambda) for the 64 points, bit-sliced: schoolbook product of the planes,
+// then x^8 = x^4 + x^3 + x + 1.
+__device__ __forceinline__ void row_product_bs(const uint4* ext, uint2 ra, uint2 rb,
+                                               uint32_t p[16]) {
+    uint32_t ya[16], yb[16];
+    ext_bitsliced(ext, ra.x, ra.y, ya);
+    ext_bitsliced(ext, rb.x, rb.y, yb);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint32_t t[15];
+#pragma unroll
+        for (int k = 0; k < 15; k++) t[k] = 0u;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) t[i + j] ^= ya[2 * i + h] & yb[2 * j + h];
+#pragma unroll
+        for (int k = 14; k >= 8; k--) {
+            t[k - 4] ^= t[k];
+            t[k - 5] ^= t[k];
+            t[k - 7] ^= t[k];
+            t[k - 8] ^= t[k];
+        }
+#pragma unroll
+        for (int b = 0; b < 8; b++) p[2 * b + h] = t[b];
+    }
+}
+
+// Q(G) = C * sum_l (w_0^l0 w_1^l1 w_2^l2) P_l by Horner over l = 7..0: acc0 closes each row
+// pair with w_0, acc1 each quad with w_1, acc2 the group with w_2.
+__device__ __forceinline__ void group_vector(const uint4* ext, const uint2* a, const uint2* b,
+                                             uint32_t q[16]) {
+    uint32_t acc0[16], acc1[16], acc2[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) acc1[w] = acc2[w] = 0u;
+#pragma unroll 1
+    for (int l = 7; l >= 0; l--) {
+        uint32_t p[16];
+        row_product_bs(ext, a[l], b[l], p);
+        if (l & 1) {
+#pragma unroll
+            for (int w = 0; w < 16; w++) acc0[w] = p[w];
+            continue;
+        }
+        f8_scale_add_bs<0>(acc0, p, acc0);
+        if ((l >> 1) & 1) {
+#pragma unroll
+            for (int w = 0; w < 16; w++) acc1[w] = acc0[w];
+            continue;
+        }
+        f8_scale_add_bs<1>(acc1, acc0, acc1);
+        if ((l >> 2) & 1) {
+#pragma unroll
+            for (int w = 0; w < 16; w++) acc2[w] = acc1[w];
+            continue;
+        }
+        f8_scale_add_bs<2>(acc2, acc1, acc2);
+    }
+    uint32_t zero[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) zero[w] = 0u;
+    f8_scale_add_bs<3>(acc2, zero, q);
+}
+
+// Group G covers rows 8G..8G+7 = 64 contiguous bytes of a and of b.
+__global__ __launch_bounds__(256) void k_zc_round1_groups(
+    const uint2* __restrict__ a, const uint2* __restrict__ b, const f128* __restrict__ eq9,
+    uint64_t n_groups, uint32_t groups_per_block, const uint4* __restrict__ ext_g,
+    f128* __restrict__ partials) {
+    __shared__ uint4 ext[16 * 16 * 4];                       // 16 KiB, k-major as in k_zc_round1
+    __shared__ __align__(16) uint32_t ys[256 * kQStride];   // 20 KiB
+    __shared__ f128 eqs[256];
+    __shared__ f128 tabs[32 * 32];                          // 16 KiB
+    for (int i = threadIdx.x; i < 16 * 16 * 4; i += blockDim.x) {
+        const int g = i >> 6, n = (i >> 2) & 15, k = i & 3;
+        ext[(g * 4 + k) * 16 + n] = ext_g[i];
+    }
+    __syncthreads();
+    f128 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = f128_zero();
+    const int t = threadIdx.x;
+    const uint64_t g0 = (uint64_t)blockIdx.x * groups_per_block;
+    uint64_t g_end = g0 + groups_per_block;
+    if (g_end > n_groups) g_end = n_groups;
+    for (uint64_t base = g0; base < g_end; base += 256) {
+        const uint64_t G = base + t;
+        uint32_t q[16];
+        if (G < g_end) {
+            group_vector(ext, a + 8 * G, b + 8 * G, q);
+            eqs[t] = eq9[G];
+        } else {
+#pragma unroll
+            for (int w = 0; w < 16; w++) q[w] = 0u;
+            eqs[t] = f128_zero();
+        }
+        uint4* yrow = reinterpret_cast<uint4*>(ys + t * kQStride);
+#pragma unroll
+        for (int k = 0; k < 4; k++) yrow[k] = make_uint4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]);
+        __syncthreads();
+        m4rm_tabs(eqs, tabs);
+        __syncthreads();
+        const uint64_t live = g_end - base;
+        const int ngroups = (int)(((live < 256u ? live : 256u) + 7) >> 3);
+        m4rm_accumulate<kQW, kQStride>(ys, tabs, ngroups, acc);
+        __syncthreads();
+    }
+    m4rm_store<kQW>(acc, partials + (size_t)blockIdx.x * 1024);
+}
+
+// red[j] for the 1024 columns of the block partials (column reduce, see k_col_xor_reduce).
+// p_ab[i] = sum_b phi8(2^b) (red[64 b + i] + red[512 + 64 b + i]).
+__global__ __launch_bounds__(64) void k_zc_round1_groups_finish(const f128* __restrict__ red,
+                                                                 const f128* __restrict__ phi_pow,
+                                                                 f128* __restrict__ out) {
+    const int i = threadIdx.x;
+    f128 r = f128_zero();
+#pragma unroll 1
+    for (int bb = 0; bb < 8; bb++)
+        r = f128_add(r, f128_mul(phi_pow[bb], f128_add(red[64 * bb + i], red[512 + 64 * bb + i])));
+    out[i] = r;
+}
+
+// p_c[i] = sum_s phi8(M[s][i]) S_c[s] with S_c[s] = sum_x eq(r[6..m), x) c_s(x)
+//        = (1 + r6) shat[s] + r6 shat[64 + s], shat = fold_1b_rows(c, eq(r[7..m))):
+// row y of c as 128 bits holds rows x = 2y (bits 0..63) and 2y + 1 (bits 64..127), and
+// eq(r[6..m), 2y + b) = (b ? r6 : 1 + r6) eq(r[7..m), y). One block per i, one lane per s.
+__global__ __launch_bounds__(64) void k_zc_round1_c(const f128* __restrict__ shat,
+                                                     const f128* __restrict__ phim, f128 r6,
+                                                     f128* __restrict__ out) {
+    __shared__ f128 part[64];
+    const int i = blockIdx.x, s = threadIdx.x;
+    f128 one_r6 = r6;
+    one_r6.x ^= 1u;
+    const f128 sc = f128_add(f128_mul(one_r6, shat[s]), f128_mul(r6, shat[64 + s]));
+    part[s] = f128_mul(phim[s * 64 + i], sc);
+    __syncthreads();
+    if (s == 0) {
+        f128 r = f128_zero();
+        for (int k = 0; k < 64; k++) r = f128_add(r, part[k]);
+        out[i] = r;
+    }
+}
+
+// out[c] = XOR over rows g < n of in[g * ncols + c]; block b takes columns 64b .. 64b+63.
+__global__ __launch_bounds__(256) void k_col_xor_reduce(const f128* __restrict__ in, int n,
+                                                         int ncols, f128* __restrict__ out) {
+    __shared__ f128 part[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+    f128 acc = f128_zero();
+    if (c < ncols)
+        for (int g = r0; g < n; g += 4) acc = f128_add(acc, in[(size_t)g * ncols + c]);
+    part[r0][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (r0 == 0 && c < ncols)
+        out[c] = f128_add(f128_add(part[0][threadIdx.x], part[1][threadIdx.x]),
+                          f128_add(part[2][threadIdx.x], part[3][threadIdx.x]));
+}
+
 // fold_1b_rows (crates/flock-core/src/pcs/ring_switch.rs:182-230, scalar form :410-470):
-// out[j] = sum over rows y of t[y] * bit j of the 128-bit row y, j < 128.  Thread t of 256
-// owns column j = t & 127 for the rows of parity t >> 7 in each 256-row batch staged in LDS;
-// a 64-lane wave reads one row word per step (LDS broadcast).
+// out[j] = sum over rows y of t[y] * bit j of the 128-bit row y, j < 128, by the method of
+// four Russians over 8-row groups (8 slots of 32 threads per 256-row batch).
 __global__ __launch_bounds__(256) void k_fold_1b_rows(const uint4* __restrict__ rows_g,
                                                        const f128* __restrict__ t_g,
                                                        uint64_t rows, uint32_t rows_per_group,
                                                        f128* __restrict__ partials) {
-    __shared__ uint4 rw[256];
-    __shared__ f128 tw[256];
+    __shared__ __align__(16) uint32_t ys[256 * 4];
+    __shared__ f128 eqs[256];
+    __shared__ f128 tabs[32 * 32];
     const int t = threadIdx.x;
-    const int j = t & 127, par = t >> 7, word = j >> 5, bit = j & 31;
-    f128 acc = f128_zero();
+    f128 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = f128_zero();
     const uint64_t row0 = (uint64_t)blockIdx.x * rows_per_group;
     uint64_t row_end = row0 + rows_per_group;
     if (row_end > rows) row_end = rows;
     for (uint64_t base = row0; base < row_end; base += 256) {
         const uint64_t y = base + t;
-        if (y < row_end) {
-            rw[t] = rows_g[y];
-            tw[t] = t_g[y];
-        } else {
-            rw[t] = make_uint4(0u, 0u, 0u, 0u);
-            tw[t] = f128_zero();
-        }
+        const bool live = y < row_end;
+        reinterpret_cast<uint4*>(ys)[t] = live ? rows_g[y] : make_uint4(0u, 0u, 0u, 0u);
+        eqs[t] = live ? t_g[y] : f128_zero();
         __syncthreads();
-        const int n = (int)((row_end - base) < 256u ? (row_end - base) : 256u);
-        for (int rr = par; rr < n; rr += 2) {
-            const uint4 r = rw[rr];
-            const uint32_t w = word == 0 ? r.x : word == 1 ? r.y : word == 2 ? r.z : r.w;
-            const uint32_t msk = 0u - ((w >> bit) & 1u);
-            const f128 e = tw[rr];
-            acc.x ^= e.x & msk;
-            acc.y ^= e.y & msk;
-            acc.z ^= e.z & msk;
-            acc.w ^= e.w & msk;
-        }
+        m4rm_tabs(eqs, tabs);
+        __syncthreads();
+        const uint64_t n = row_end - base;
+        m4rm_accumulate<4, 4>(ys, tabs, (int)(((n < 256u ? n : 256u) + 7) >> 3), acc);
         __syncthreads();
     }
-    partials[(size_t)blockIdx.x * 256 + t] = acc;
+    m4rm_store<4>(acc, partials + (size_t)blockIdx.x * 1024);
 }
 
-__global__ __launch_bounds__(128) void k_fold_1b_rows_finish(const f128* __restrict__ partials,
-                                                              int groups,
+// out[j] = sum over the 8 slots of red[128 slot + j].
+__global__ __launch_bounds__(128) void k_fold_1b_rows_finish(const f128* __restrict__ red,
                                                               f128* __restrict__ out) {
     const int j = threadIdx.x;
     f128 acc = f128_zero();
-    for (int g = 0; g < groups; g++) {
-        acc = f128_add(acc, partials[(size_t)g * 256 + j]);
-        acc = f128_add(acc, partials[(size_t)g * 256 + 128 + j]);
-    }
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc = f128_add(acc, red[128 * s + j]);
     out[j] = acc;
 }
 
@@ -264,26 +557,55 @@ uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group) {
     return (uint32_t)((rows + per - 1) / per);
 }
 
+static hipError_t launch_col_reduce(const uint4* partials, uint32_t n, uint4* red, hipStream_t s) {
+    k_col_xor_reduce<<<1024 / 64, 256, 0, s>>>(partials, (int)n, 1024, red);
+    return hipGetLastError();
+}
+
 hipError_t launch_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, const uint4* eq,
                             uint64_t rows, const uint4* ext, const uint8_t* logexp,
-                            const uint4* phi_pow, uint4* partials, uint4* out, hipStream_t s) {
+                            const uint4* phi_pow, uint4* partials, uint4* red, uint4* out,
+                            hipStream_t s) {
     uint32_t per = 0;
     const uint32_t groups = zc_round1_groups(rows, &per);
     k_zc_round1<<<groups, kR1Threads, 0, s>>>(a, b, c, eq, rows, per, ext, logexp, partials);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_col_reduce(partials, groups, red, s);
     if (e != hipSuccess) return e;
-    k_zc_round1_finish<<<1, kR1Acc, 0, s>>>(partials, (int)groups, phi_pow, out);
+    k_zc_round1_finish<<<1, kR1Acc, 0, s>>>(red, phi_pow, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_round1_groups(const uint8_t* a, const uint8_t* b, const uint4* eq9,
+                                   uint64_t n_groups, const uint4* ext_bs, const uint4* phi_pow,
+                                   uint4* partials, uint4* red, uint4* out, hipStream_t s) {
+    uint32_t per = 0;
+    const uint32_t blocks = zc_round1_groups(n_groups, &per);
+    k_zc_round1_groups<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint2*>(a),
+                                              reinterpret_cast<const uint2*>(b), eq9, n_groups,
+                                              per, ext_bs, partials);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_col_reduce(partials, blocks, red, s);
+    if (e != hipSuccess) return e;
+    k_zc_round1_groups_finish<<<1, 64, 0, s>>>(red, phi_pow, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_round1_c(const uint4* shat, const uint4* phim, uint4 r6, uint4* out,
+                              hipStream_t s) {
+    k_zc_round1_c<<<64, 64, 0, s>>>(shat, phim, r6, out);
     return hipGetLastError();
 }
 
 hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
-                              uint4* partials, uint4* out, hipStream_t s) {
+                              uint4* partials, uint4* red, uint4* out, hipStream_t s) {
     uint32_t per = 0;
     const uint32_t groups = zc_round1_groups(rows, &per);  // <= 512 groups of 256-row batches
     k_fold_1b_rows<<<groups, 256, 0, s>>>(rows_g, t, rows, per, partials);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_col_reduce(partials, groups, red, s);
     if (e != hipSuccess) return e;
-    k_fold_1b_rows_finish<<<1, 128, 0, s>>>(partials, (int)groups, out);
+    k_fold_1b_rows_finish<<<1, 128, 0, s>>>(red, out);
     return hipGetLastError();
 }
 
