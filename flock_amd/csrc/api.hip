@@ -414,7 +414,7 @@ struct flock_hip_ctx {
     uint4* zc_phipow = nullptr;
     uint4* zc_ext_bs = nullptr;    // bit-sliced extension table (same shape as zc_ext)
     uint4* zc_phim = nullptr;      // phi8(L_s(64 + i)) at s * 64 + i
-    uint4* zc_partials = nullptr;  // 512 groups x 1024 accumulators
+    uint4* zc_partials = nullptr;  // kZcMaxBlocks groups x 1024 accumulators
     uint4* zc_out = nullptr;       // 2048 words: message [0,128), shat [128,256), red [1024,2048)
     flock_f128* zc_host = nullptr;  // pinned, 256 words: message and shat
     uint4* zc_shat() { return zc_out + 128; }
@@ -473,7 +473,7 @@ struct flock_hip_ctx {
             hipMalloc(&zc_phim, phim.size() * 16) != hipSuccess ||
             hipMalloc(&zc_logexp, le.size()) != hipSuccess ||
             hipMalloc(&zc_phipow, sizeof(pp)) != hipSuccess ||
-            hipMalloc(&zc_partials, sizeof(uint4) * 512 * 1024) != hipSuccess ||
+            hipMalloc(&zc_partials, sizeof(uint4) * flock::kZcMaxBlocks * 1024) != hipSuccess ||
             hipMalloc(&zc_out, sizeof(uint4) * 2048) != hipSuccess ||
             hipHostMalloc(&zc_host, sizeof(flock_f128) * 256) != hipSuccess) {
             (void)hipGetLastError();

@@ -71,7 +71,7 @@ hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int n
 // ---- zerocheck.hip ----
 // Round 1 on packed bit rows (2^(m-6) rows of 64 bits): out[0..64) = p_ab, out[64..128) = p_c.
 // eq: 2^(m-6) words; ext: 16x16x64-byte extension table; logexp: log[256] + exp[512];
-// phi_pow: phi8(2^b), b < 8; partials >= 512 * 1024 words; red: 1024 words.
+// phi_pow: phi8(2^b), b < 8; partials >= kZcMaxBlocks * 1024 words; red: 1024 words.
 hipError_t launch_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, const uint4* eq,
                             uint64_t rows, const uint4* ext, const uint8_t* logexp,
                             const uint4* phi_pow, uint4* partials, uint4* red, uint4* out,
@@ -86,13 +86,14 @@ hipError_t launch_zc_round1_groups(const uint8_t* a, const uint8_t* b, const uin
 // phi8(L_s(64+i)): out[i] for i < 64.
 hipError_t launch_zc_round1_c(const uint4* shat, const uint4* phim, uint4 r6, uint4* out,
                               hipStream_t s);
-uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group);
+uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group, uint32_t max_groups);
+constexpr uint32_t kZcMaxBlocks = 2048;  // zerocheck partials hold kZcMaxBlocks x 1024 words
 // Fold at z of two packed bit tables with the 8 x 256 fold table.
 hipError_t launch_zc_fold_at_z(const uint8_t* a, const uint8_t* b, uint64_t rows,
                                const uint4* table, uint4* az, uint4* bz, hipStream_t s);
 // out[z] = in[2z] ^ in[2z+1].
 hipError_t launch_eq_fold(const uint4* in, uint64_t n_out, uint4* out, hipStream_t s);
-// out[j] = sum_y t[y] * bit j of 128-bit row y (128 words; partials: 512 x 1024 words; red:
+// out[j] = sum_y t[y] * bit j of 128-bit row y (128 words; partials: kZcMaxBlocks x 1024; red:
 // 1024 words).
 hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
                               uint4* partials, uint4* red, uint4* out, hipStream_t s);

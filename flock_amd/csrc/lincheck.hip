@@ -29,8 +29,12 @@ __global__ __launch_bounds__(256) void k_lc_csc_fold(
     __shared__ f128 table[SHOUP_ENTRIES];
     if (threadIdx.x < 32) shoup_build_row(table, alpha, threadIdx.x);
     __syncthreads();
-    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_cols;
-         c += (uint64_t)gridDim.x * blockDim.x) {
+    // one wave per column: the lanes stride over its nonzeros (coalesced row indices, eq
+    // gathers from a table that stays in L2), then XOR-reduce across the wave
+    const int lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n_cols;
+         c += nwaves) {
         f128 sa = f128_zero(), sb = f128_zero();
         bool bad = false;
         uint64_t k0 = a_ptr[c], k1 = a_ptr[c + 1];
@@ -38,7 +42,7 @@ __global__ __launch_bounds__(256) void k_lc_csc_fold(
             k1 = nnz_a;
             bad = true;
         }
-        for (uint64_t k = k0; k < k1; k++) {
+        for (uint64_t k = k0 + lane; k < k1; k += 64) {
             const uint32_t r = a_rows[k];
             if (r < n_cols) sa = f128_add(sa, eq[r]);
             else bad = true;
@@ -49,15 +53,20 @@ __global__ __launch_bounds__(256) void k_lc_csc_fold(
             k1 = nnz_b;
             bad = true;
         }
-        for (uint64_t k = k0; k < k1; k++) {
+        for (uint64_t k = k0 + lane; k < k1; k += 64) {
             const uint32_t r = b_rows[k];
             if (r < n_cols) sb = f128_add(sb, eq[r]);
             else bad = true;
         }
-        if (bad) atomicOr(err, 1);
-        if (sa_out) sa_out[c] = sa;
-        if (sb_out) sb_out[c] = sb;
-        if (comb_out) comb_out[c] = f128_add(shoup_mul(table, sa), sb);
+        sa = wave_xor_f128(sa);
+        sb = wave_xor_f128(sb);
+        const bool any_bad = __any(bad);
+        if (lane == 0) {
+            if (any_bad) atomicOr(err, 1);
+            if (sa_out) sa_out[c] = sa;
+            if (sb_out) sb_out[c] = sb;
+            if (comb_out) comb_out[c] = f128_add(shoup_mul(table, sa), sb);
+        }
     }
 }
 
@@ -252,7 +261,8 @@ hipError_t launch_lc_csc_fold(uint64_t n_cols, const uint32_t* a_ptr, const uint
                               uint64_t nnz_b, const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,
                               uint4* comb, int* err, hipStream_t s) {
     k_fill_i32_zero<<<1, 64, 0, s>>>(err);
-    k_lc_csc_fold<<<grid_for(n_cols, 65535), 256, 0, s>>>(n_cols, a_ptr, a_rows, nnz_a, b_ptr,
+    // four columns (waves) per workgroup
+    k_lc_csc_fold<<<grid_for(n_cols * 64, 65535), 256, 0, s>>>(n_cols, a_ptr, a_rows, nnz_a, b_ptr,
                                                           b_rows, nnz_b, eq, alpha, sa, sb, comb,
                                                           err);
     return hipGetLastError();

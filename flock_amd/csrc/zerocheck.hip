@@ -491,19 +491,24 @@ __global__ __launch_bounds__(64) void k_zc_round1_c(const f128* __restrict__ sha
     }
 }
 
-// out[c] = XOR over rows g < n of in[g * ncols + c]; block b takes columns 64b .. 64b+63.
-__global__ __launch_bounds__(256) void k_col_xor_reduce(const f128* __restrict__ in, int n,
-                                                         int ncols, f128* __restrict__ out) {
-    __shared__ f128 part[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), r0 = threadIdx.x >> 6;
+// out[c] = XOR over rows g < n of in[g * ncols + c]; block b takes columns 64b .. 64b+63 with
+// 16 row lanes.
+__global__ __launch_bounds__(1024) void k_col_xor_reduce(const f128* __restrict__ in, int n,
+                                                          int ncols, f128* __restrict__ out) {
+    __shared__ f128 part[16][64];
+    const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + col;
     f128 acc = f128_zero();
     if (c < ncols)
-        for (int g = r0; g < n; g += 4) acc = f128_add(acc, in[(size_t)g * ncols + c]);
-    part[r0][threadIdx.x & 63] = acc;
+        for (int g = r0; g < n; g += 16) acc = f128_add(acc, in[(size_t)g * ncols + c]);
+    part[r0][col] = acc;
     __syncthreads();
-    if (r0 == 0 && c < ncols)
-        out[c] = f128_add(f128_add(part[0][threadIdx.x], part[1][threadIdx.x]),
-                          f128_add(part[2][threadIdx.x], part[3][threadIdx.x]));
+    if (r0 == 0 && c < ncols) {
+        f128 r = part[0][col];
+#pragma unroll
+        for (int k = 1; k < 16; k++) r = f128_add(r, part[k][col]);
+        out[c] = r;
+    }
 }
 
 // fold_1b_rows (crates/flock-core/src/pcs/ring_switch.rs:182-230, scalar form :410-470):
@@ -548,9 +553,9 @@ __global__ __launch_bounds__(128) void k_fold_1b_rows_finish(const f128* __restr
     out[j] = acc;
 }
 
-uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group) {
-    // at most 512 groups; each group walks whole 256-row batches
-    uint64_t per = (rows + 511) / 512;
+uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group, uint32_t max_groups) {
+    // at most max_groups groups; each group walks whole 256-row batches
+    uint64_t per = (rows + max_groups - 1) / max_groups;
     per = ((per + kR1Threads - 1) / kR1Threads) * kR1Threads;
     if (per < kR1Threads) per = kR1Threads;
     *rows_per_group = (uint32_t)per;
@@ -558,7 +563,7 @@ uint32_t zc_round1_groups(uint64_t rows, uint32_t* rows_per_group) {
 }
 
 static hipError_t launch_col_reduce(const uint4* partials, uint32_t n, uint4* red, hipStream_t s) {
-    k_col_xor_reduce<<<1024 / 64, 256, 0, s>>>(partials, (int)n, 1024, red);
+    k_col_xor_reduce<<<1024 / 64, 1024, 0, s>>>(partials, (int)n, 1024, red);
     return hipGetLastError();
 }
 
@@ -567,7 +572,7 @@ hipError_t launch_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c
                             const uint4* phi_pow, uint4* partials, uint4* red, uint4* out,
                             hipStream_t s) {
     uint32_t per = 0;
-    const uint32_t groups = zc_round1_groups(rows, &per);
+    const uint32_t groups = zc_round1_groups(rows, &per, 512);
     k_zc_round1<<<groups, kR1Threads, 0, s>>>(a, b, c, eq, rows, per, ext, logexp, partials);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = launch_col_reduce(partials, groups, red, s);
@@ -580,7 +585,7 @@ hipError_t launch_zc_round1_groups(const uint8_t* a, const uint8_t* b, const uin
                                    uint64_t n_groups, const uint4* ext_bs, const uint4* phi_pow,
                                    uint4* partials, uint4* red, uint4* out, hipStream_t s) {
     uint32_t per = 0;
-    const uint32_t blocks = zc_round1_groups(n_groups, &per);
+    const uint32_t blocks = zc_round1_groups(n_groups, &per, 512);  // 2 workgroups per CU (LDS)
     k_zc_round1_groups<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint2*>(a),
                                               reinterpret_cast<const uint2*>(b), eq9, n_groups,
                                               per, ext_bs, partials);
@@ -600,7 +605,8 @@ hipError_t launch_zc_round1_c(const uint4* shat, const uint4* phim, uint4 r6, ui
 hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t rows,
                               uint4* partials, uint4* red, uint4* out, hipStream_t s) {
     uint32_t per = 0;
-    const uint32_t groups = zc_round1_groups(rows, &per);  // <= 512 groups of 256-row batches
+    // up to 2048 workgroups of 256-row batches (six fit a CU)
+    const uint32_t groups = zc_round1_groups(rows, &per, kZcMaxBlocks);
     k_fold_1b_rows<<<groups, 256, 0, s>>>(rows_g, t, rows, per, partials);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = launch_col_reduce(partials, groups, red, s);
