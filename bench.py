@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--n-log", type=int, default=16, help="log2 of the SHA-256 batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sumcheck", action="store_true")
+    ap.add_argument("--latency-requests", type=int, default=50,
+                    help="sequential prove-core requests timed one by one for p50/p99")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the commit (for profiler runs of the NTT)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -290,6 +292,22 @@ def main():
         result["prove_core_khps"] = round(n / core, 2)  # k hashes/s = hashes per ms
         result["prove_core_note"] = ("witness+compact+commit+zerocheck+lincheck; excludes the "
                                      "PCS opening and the registry digest (not a full proof)")
+
+    if not args.ntt_only and args.latency_requests > 0:
+        # one request = one prove-core step on inputs already resident in HBM, timed from
+        # submission to the last host-visible result (the lincheck's final message)
+        lat = []
+        for _ in range(args.latency_requests):
+            ctx.sync()
+            t1 = time.perf_counter()
+            step({})
+            ctx.sync()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        result["prove_core_latency_ms"] = {
+            "p50": round(float(np.percentile(lat, 50)), 4),
+            "p99": round(float(np.percentile(lat, 99)), 4),
+            "requests": len(lat),
+        }
 
     if args.copy_calibration:
         nbytes = 1 << 30

@@ -31,34 +31,45 @@ __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) {
     return __builtin_amdgcn_alignbit(x, x, (uint32_t)n);
 }
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 
 __device__ __forceinline__ void sha256_iv(uint32_t h[8]) {
     h[0] = 0x6a09e667u; h[1] = 0xbb67ae85u; h[2] = 0x3c6ef372u; h[3] = 0xa54ff53au;
     h[4] = 0x510e527fu; h[5] = 0x9b05688cu; h[6] = 0x1f83d9abu; h[7] = 0x5be0cd19u;
 }
 
-// FIPS 180-4 compression of one 16-word big-endian block.
+// FIPS 180-4 compression of one 16-word big-endian block. Rounds 16..63 run as a rolled
+// loop of three 16-round bodies, which keeps the message-schedule window (and the register
+// count) to one block.
+__device__ __forceinline__ void sha256_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
+                                             uint32_t& e, uint32_t& f, uint32_t& g, uint32_t& hh,
+                                             uint32_t k, uint32_t wi) {
+    const uint32_t S1 = xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25));
+    const uint32_t ch = (e & f) ^ (~e & g);
+    const uint32_t t1 = hh + S1 + ch + k + wi;
+    const uint32_t S0 = xor3(rotr(a, 2), rotr(a, 13), rotr(a, 22));
+    const uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
+    const uint32_t t2 = S0 + mj;
+    hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
+}
+
 __device__ __forceinline__ void sha256_compress(uint32_t h[8], uint32_t w[16]) {
     uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
 #pragma unroll
-    for (int i = 0; i < 64; i++) {
-        uint32_t wi;
-        if (i < 16) {
-            wi = w[i];
-        } else {
+    for (int i = 0; i < 16; i++) sha256_round(a, b, c, d, e, f, g, hh, kSha256K[i], w[i]);
+#pragma unroll 1
+    for (int r = 16; r < 64; r += 16) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
             const uint32_t w15 = w[(i + 1) & 15], w2 = w[(i + 14) & 15];
-            const uint32_t s0 = rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3);
-            const uint32_t s1 = rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10);
-            wi = w[i & 15] + s0 + w[(i + 9) & 15] + s1;
-            w[i & 15] = wi;
+            const uint32_t s0 = xor3(rotr(w15, 7), rotr(w15, 18), w15 >> 3);
+            const uint32_t s1 = xor3(rotr(w2, 17), rotr(w2, 19), w2 >> 10);
+            const uint32_t wi = w[i] + s0 + w[(i + 9) & 15] + s1;
+            w[i] = wi;
+            sha256_round(a, b, c, d, e, f, g, hh, kSha256K[r + i], wi);
         }
-        const uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
-        const uint32_t ch = (e & f) ^ (~e & g);
-        const uint32_t t1 = hh + S1 + ch + kSha256K[i] + wi;
-        const uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
-        const uint32_t mj = (a & b) ^ (a & c) ^ (b & c);
-        const uint32_t t2 = S0 + mj;
-        hh = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;
     }
     h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
 }
@@ -91,7 +102,7 @@ __device__ __forceinline__ uint64_t level_start(uint64_t n_leaves, uint64_t leve
 
 // Build the subtree above `width` digests held in lds[0..width) (this group's slice of a
 // level with `level_nodes` nodes), writing every produced node into the tree.
-__device__ void subtree_up(uint32_t (*lds)[8], int width, uint64_t group, uint64_t level_nodes,
+__device__ __forceinline__ void subtree_up(uint32_t (*lds)[8], int width, uint64_t group, uint64_t level_nodes,
                            uint64_t n_leaves, uint8_t* tree) {
     int w = width;
     uint64_t ln = level_nodes;
