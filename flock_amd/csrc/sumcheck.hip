@@ -156,16 +156,30 @@ __global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
     }
 }
 
-// eq(r[0..nv), x) for every x, directly as a product of nv factors.
+// eq(r[0..nv), x) for every x as a product of nv factors (nv <= kEqDirectMax). Every factor
+// multiply is by a fixed r_i: acc * (1 + r_i) = acc + acc * r_i, with acc * r_i through a
+// 16-entry comb table of r_i shared by the workgroup, so no thread runs a chain of generic
+// products.
+constexpr int kEqDirectMax = 20;
+
 __global__ __launch_bounds__(256) void k_eq_direct(const f128* __restrict__ r, int nv,
                                                     f128* __restrict__ out) {
+    __shared__ f128 tabs[kEqDirectMax][16];
+    __shared__ uint32_t his[kEqDirectMax];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        tabs[i][0] = f128_zero();
+        his[i] = comb_build<1>(tabs[i], r[i]);
+    }
+    __syncthreads();
     const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= (1ull << nv)) return;
     f128 acc = f128_one();
     for (int i = 0; i < nv; i++) {
-        f128 fac = r[i];
-        if (!((x >> i) & 1)) fac.x ^= 1u;  // 1 + r_i
-        acc = (i == 0) ? fac : f128_mul(acc, fac);
+        f256u p;
+        f256u_zero(p);
+        comb_apply<1>(p, tabs[i], his[i], acc);
+        const f128 ar = f256u_reduce(p);  // acc * r_i
+        acc = ((x >> i) & 1) ? ar : f128_add(acc, ar);
     }
     out[x] = acc;
 }
@@ -264,6 +278,7 @@ hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int n
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s) {
     // k_eq_direct with zero variables writes the single entry 1.
     const int nlo = nv / 2, nhi = nv - nlo;
+    if (nhi > kEqDirectMax) return hipErrorInvalidValue;
     uint4* lo = scratch;
     uint4* hi = scratch + (1ull << nlo);
     k_eq_direct<<<(unsigned)(((1ull << nlo) + 255) / 256), 256, 0, s>>>(r, nlo, lo);

@@ -205,6 +205,9 @@ void or_keccak_instance(const uint64_t initial[25], uint64_t* z, uint64_t* a, ui
 /* generate_witness_batch_major (keccak.rs:1228-1285): padding rows run keccak_f(0) */
 int or_keccak_witness_batch_major(const uint64_t* inputs, size_t n_decl, int n_log, or_f128* z,
                                   or_f128* a, or_f128* b, uint8_t* stripe);
+/* KeccakLincheckCircuit column marginals (keccak.rs:692-835) as fold_split (lincheck.rs:258):
+ * sa = A_0^T eq, sb = B_0^T eq over 2^16 columns; fold_alpha_batched = alpha * sa + sb */
+void or_keccak_lc_fold_split(const or_f128* eq, or_f128* sa, or_f128* sb);
 
 /* ---- BLAKE3 compression R1CS (crates/flock-prover/src/r1cs_hashes/blake3.rs), blake3_r1cs.c ---- */
 /* {Z_CONST_POS, USEFUL_BITS, G_BASE[4], G_BASE[1]} (blake3.rs:233-247) */

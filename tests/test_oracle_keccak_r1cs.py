@@ -97,3 +97,45 @@ def test_batch_major_layout_with_padding_rows():
     for o in range(n):
         exp[o // 8] |= (zb[o] << (o % 8)).astype(np.uint8)
     assert np.array_equal(stripe.reshape(-1, K), exp)
+
+
+L.or_keccak_lc_fold_split.argtypes = [P, P, P]
+
+
+def lc_fold_split(eq):
+    sa = np.zeros((K, 2), dtype=np.uint64)
+    sb = np.zeros((K, 2), dtype=np.uint64)
+    L.or_keccak_lc_fold_split(O._p(eq), O._p(sa), O._p(sb))
+    return sa, sb
+
+
+def _xor_where(vals, bits_u64):
+    """XOR of vals[c] over the set bits c of a packed LSB-first bit vector."""
+    mask = np.unpackbits(bits_u64.view(np.uint8), bitorder="little").astype(bool)
+    sel = vals[:len(mask)][mask]
+    return tuple(np.bitwise_xor.reduce(sel, axis=0)) if len(sel) else (0, 0)
+
+
+def test_lincheck_marginals_are_the_transposed_matrices():
+    # A_0 z = a and B_0 z = b for every witness, so for any row weights eq:
+    #   <A_0^T eq, z> = <eq, a>  and  <B_0^T eq, z> = <eq, b>.
+    # Random states make every t and state_0 column take both values; each column's
+    # coefficient is then tested by several independent witnesses.
+    rng = np.random.default_rng(0xCC)
+    eq = rng.integers(0, 1 << 63, (K, 2), dtype=np.uint64) * 2 + 1
+    sa, sb = lc_fold_split(eq)
+    for st in [np.zeros(25, dtype=np.uint64)] + list(random_states(7, 4)):
+        z, a, b = instance(st)
+        assert _xor_where(sa, z) == _xor_where(eq, a)
+        assert _xor_where(sb, z) == _xor_where(eq, b)
+    # columns that no witness bit occupies carry no coefficient
+    for lo, hi in ((1600, 2048), (3648, 4096), (4097, 4160), (USEFUL_BITS, K)):
+        assert not sa[lo:hi].any() and not sb[lo:hi].any()
+
+
+def test_lincheck_marginals_are_linear_in_eq():
+    rng = np.random.default_rng(0xCD)
+    e1 = rng.integers(0, 1 << 63, (K, 2), dtype=np.uint64)
+    e2 = rng.integers(0, 1 << 63, (K, 2), dtype=np.uint64)
+    s1, s2, s12 = lc_fold_split(e1), lc_fold_split(e2), lc_fold_split(e1 ^ e2)
+    assert np.array_equal(s12[0], s1[0] ^ s2[0]) and np.array_equal(s12[1], s1[1] ^ s2[1])
