@@ -4,9 +4,11 @@
 // transcript, and the protocol drivers that sequence kernels between challenges.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <new>
@@ -417,6 +419,11 @@ struct flock_hip_ctx {
     uint4* zc_partials = nullptr;  // kZcMaxBlocks groups x 1024 accumulators
     uint4* zc_out = nullptr;       // 2048 words: message [0,128), shat [128,256), red [1024,2048)
     flock_f128* zc_host = nullptr;  // pinned, 256 words: message and shat
+    // Keccak-f[1600] lincheck walker tables (device): phi^T gathers, chi row offsets, RC_r
+    uint16_t* kc_inv = nullptr;
+    uint16_t* kc_trow = nullptr;
+    uint32_t* kc_rcm = nullptr;
+    int ensure_kc();
     uint4* zc_shat() { return zc_out + 128; }
     uint4* zc_red() { return zc_out + 1024; }
 
@@ -596,6 +603,9 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     if (c->zc_partials) hipFree(c->zc_partials);
     if (c->zc_out) hipFree(c->zc_out);
     if (c->zc_host) hipHostFree(c->zc_host);
+    if (c->kc_inv) hipFree(c->kc_inv);
+    if (c->kc_trow) hipFree(c->kc_trow);
+    if (c->kc_rcm) hipFree(c->kc_rcm);
     if (c->scratch) hipFree(c->scratch);
     if (c->partials) hipFree(c->partials);
     if (c->dout) hipFree(c->dout);
@@ -1295,7 +1305,113 @@ int flock_hip_lc_partial_fold(flock_hip_ctx* c, const uint8_t* z, uint32_t k_log
     return 0;
 }
 
+// ---- Keccak-f[1600] lincheck walker (r1cs_hashes/keccak.rs:136-176, 436-510, 692-835) ----
+// Host tables for k_kc_walk. State bit s = x + 5y + 25z; phi = pi o rho o theta and
+// B[x, y, z] is the XOR of the 11 state bits of its preimage:
+//   pi:    B[x, y, z] = R[(x + 3y) mod 5, x, z]
+//   rho:   R[a, b, c] = T[a, b, (c - rho[a][b]) mod 64]
+//   theta: T[a, b, c] = S[a, b, c] + sum_y' S[a - 1, y', c] + sum_y' S[a + 1, y', c - 1].
+static const unsigned kKcRho[5][5] = {{0, 36, 3, 41, 18}, {1, 44, 10, 45, 2},
+                                      {62, 6, 43, 15, 61}, {28, 55, 25, 21, 56},
+                                      {27, 20, 39, 8, 14}};  // FIPS 202 Table 2, [x][y]
+static const uint64_t kKcRc[24] = {
+    0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
+    0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
+    0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
+    0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
+    0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
+    0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+
+static int kc_state(int x, int y, int z) { return x + 5 * y + 25 * z; }
+static int kc_lane_pos(int s) { return 64 * (s % 25) + s / 25; }
+static void kc_preimage(int s_out, int out[11]) {
+    const int x = s_out % 5, y = (s_out % 25) / 5, z = s_out / 25;
+    const int a = (x + 3 * y) % 5, b = x;
+    const int cz = (z + 64 - (int)(kKcRho[a][b] % 64)) % 64, cp = (cz + 63) % 64;
+    out[0] = kc_state(a, b, cz);
+    for (int yp = 0; yp < 5; yp++) {
+        out[1 + yp] = kc_state((a + 4) % 5, yp, cz);
+        out[6 + yp] = kc_state((a + 1) % 5, yp, cp);
+    }
+}
+
+int flock_hip_ctx::ensure_kc() {
+    if (kc_inv) return 0;
+    constexpr int S = 1600;
+    std::vector<uint16_t> inv(S * 11), trow(2 * S * 11);
+    std::vector<int> fill(S, 0);
+    std::vector<std::array<int, 11>> pre(S);
+    for (int so = 0; so < S; so++) {
+        kc_preimage(so, pre[so].data());
+        for (int i = 0; i < 11; i++) {
+            const int si = pre[so][i];
+            if (fill[si] >= 11) return fail(FLOCK_HIP_E_INVALID, "Keccak phi table");
+            inv[si * 11 + fill[si]++] = (uint16_t)so;
+        }
+    }
+    for (int si = 0; si < S; si++) {
+        if (fill[si] != 11) return fail(FLOCK_HIP_E_INVALID, "Keccak phi table");
+        for (int i = 0; i < 11; i++) {
+            const int u = inv[si * 11 + i];
+            const int ux = u % 5, uy = (u % 25) / 5, uz = u / 25;
+            // chi A operand of row x reads B[x + 1]; the B operand reads B[x + 2]
+            trow[si * 11 + i] = (uint16_t)kc_lane_pos(kc_state((ux + 4) % 5, uy, uz));
+            trow[(S + si) * 11 + i] = (uint16_t)kc_lane_pos(kc_state((ux + 3) % 5, uy, uz));
+        }
+    }
+    // RC_0 = 0, RC_{r+1} = phi(RC_r) + iota's constant on lane (0, 0)
+    std::vector<uint32_t> rcm(25 * 50, 0);
+    std::vector<uint8_t> rc(S, 0), nrc(S);
+    for (int r = 0;; r++) {
+        for (int s0 = 0; s0 < S; s0++)
+            if (rc[s0]) rcm[r * 50 + (s0 >> 5)] |= 1u << (s0 & 31);
+        if (r == 24) break;
+        for (int so = 0; so < S; so++) {
+            uint8_t bit = 0;
+            for (int i = 0; i < 11; i++) bit ^= rc[pre[so][i]];
+            nrc[so] = bit;
+        }
+        rc = nrc;
+        for (int z = 0; z < 64; z++)
+            if ((kKcRc[r] >> z) & 1) rc[kc_state(0, 0, z)] ^= 1;
+    }
+    if (hipMalloc(&kc_inv, inv.size() * 2) != hipSuccess ||
+        hipMalloc(&kc_trow, trow.size() * 2) != hipSuccess ||
+        hipMalloc(&kc_rcm, rcm.size() * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FLOCK_HIP_E_NOMEM, "Keccak lincheck tables");
+    }
+    HIP_TRY(hipMemcpy(kc_inv, inv.data(), inv.size() * 2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(kc_trow, trow.data(), trow.size() * 2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(kc_rcm, rcm.data(), rcm.size() * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int flock_hip_lc_keccak_fold(flock_hip_ctx* c, const flock_f128* eq, flock_f128 alpha,
+                             flock_f128* sa, flock_f128* sb, flock_f128* comb) {
+    CTX_CHECK(c);
+    if (!eq || !sa || !sb) return fail(FLOCK_HIP_E_INVALID, "bad Keccak fold arguments");
+    if (int rc = c->ensure_kc()) return rc;
+    LAUNCH(flock::launch_kc_fold(U(eq), c->kc_inv, c->kc_trow, c->kc_rcm, U(alpha), U(sa), U(sb),
+                                 comb ? U(comb) : nullptr, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 static const uint8_t kLabelLincheck[] = "flock-lincheck-v0";
+
+// The circuit enters only through its column marginals (LincheckCircuit, lincheck.rs:238-278):
+// fold(eq, alpha) -> sa = A_0^T eq, sb = B_0^T eq, comb = alpha sa + sb.
+using LcFold = std::function<int(const uint4* eq, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
+                                 int* d_err)>;
+
+static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fold,
+                               uint32_t k_log, int64_t const_pin, const uint8_t* d_z, uint32_t nu,
+                               size_t useful_bits, size_t n_rows, flock_f128 z_skip,
+                               const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                               int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
+                               flock_f128 h_matrix_evals[2], uint64_t* h_nonces,
+                               flock_f128* h_claim, flock_f128* d_z_vec);
 
 int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
                              const uint8_t* d_z, uint32_t nu, size_t useful_bits, size_t n_rows,
@@ -1304,14 +1420,50 @@ int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
                              flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
                              uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec) {
     CTX_CHECK(c);
-    const int ks = 6;
     if (int rc = csc_check(k)) return rc;
-    if (int rc = lc_fold_args(d_z, k->k_log, nu, useful_bits, n_rows)) return rc;
-    const uint32_t L = k->k_log - ks;
+    const LcFold fold = [c, k](const uint4* eq, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
+                               int* d_err) {
+        return csc_fold_impl(c, k, eq, alpha, sa, sb, comb, d_err);
+    };
+    return lincheck_prove_impl(c, fs, fold, k->k_log, k->const_pin, d_z, nu, useful_bits, n_rows,
+                               z_skip, h_x_inner_rest, h_x_outer, grinding, h_rounds, h_z_partial,
+                               h_matrix_evals, h_nonces, h_claim, d_z_vec);
+}
+
+int flock_hip_lincheck_prove_keccak(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_z,
+                                    uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                    const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                    int grinding, flock_f128* h_rounds,
+                                    flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
+                                    uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec) {
+    CTX_CHECK(c);
+    if (int rc = c->ensure_kc()) return rc;
+    const LcFold fold = [c](const uint4* eq, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
+                            int*) {
+        LAUNCH(flock::launch_kc_fold(eq, c->kc_inv, c->kc_trow, c->kc_rcm, alpha, sa, sb, comb,
+                                     c->stream));
+        return 0;
+    };
+    // K_LOG 16, USEFUL_BITS 42,560, const pin Z_CONST = 4096 (keccak.rs:436-451, 688-690)
+    return lincheck_prove_impl(c, fs, fold, 16, 4096, d_z, nu, 42560, n_rows, z_skip,
+                               h_x_inner_rest, h_x_outer, grinding, h_rounds, h_z_partial,
+                               h_matrix_evals, h_nonces, h_claim, d_z_vec);
+}
+
+static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fold,
+                               uint32_t k_log, int64_t const_pin, const uint8_t* d_z, uint32_t nu,
+                               size_t useful_bits, size_t n_rows, flock_f128 z_skip,
+                               const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                               int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
+                               flock_f128 h_matrix_evals[2], uint64_t* h_nonces,
+                               flock_f128* h_claim, flock_f128* d_z_vec) {
+    const int ks = 6;
+    if (int rc = lc_fold_args(d_z, k_log, nu, useful_bits, n_rows)) return rc;
+    const uint32_t L = k_log - ks;
     if (!fs || (L && (!h_x_inner_rest || !h_rounds)) || !h_x_outer || !h_z_partial ||
         !h_matrix_evals || !h_claim || (grinding && !h_nonces))
         return fail(FLOCK_HIP_E_INVALID, "bad lincheck arguments");
-    const uint64_t K = 1ull << k->k_log;
+    const uint64_t K = 1ull << k_log;
     const uint64_t n_outer = 1ull << nu;
     // scratch (words): EQI | SA | SB | COMB | ZV (K each) | EO (2^nu) | ER (2^L) |
     // EQS (build_eq) | PF (partial-fold rows) | RV (64 + L + nu) | ERR (1)
@@ -1350,13 +1502,13 @@ int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
     HIP_TRY(hipMemcpyAsync(RV, rv.data(), rv.size() * 16, hipMemcpyHostToDevice, c->stream));
     LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
     LAUNCH(flock::launch_outer_product(RV, 6, ER, (int)L, EQI, c->stream));
-    if (int rc = csc_fold_impl(c, k, EQI, U(alpha), SA, SB, COMB, ERR)) return rc;
+    if (int rc = fold(EQI, U(alpha), SA, SB, COMB, ERR)) return rc;
     // 2b. constant-wire pin (union.rs:318-335)
-    if (k->const_pin >= 0) {
+    if (const_pin >= 0) {
         if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
         flock_f128 beta;
         flock_fs_sample_f128(fs, &beta);
-        LAUNCH(flock::launch_lc_add_at(COMB, (uint64_t)k->const_pin, U(beta), c->stream));
+        LAUNCH(flock::launch_lc_add_at(COMB, (uint64_t)const_pin, U(beta), c->stream));
     }
     // 3. partial fold of the stripe at x_outer (union.rs:354-384); 3b. capture z_vec
     LAUNCH(flock::launch_build_eq(RV + 64 + L, (int)nu, EO, EQS, c->stream));

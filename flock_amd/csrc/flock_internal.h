@@ -99,6 +99,12 @@ hipError_t launch_fold_1b_rows(const uint4* rows_g, const uint4* t, uint64_t row
                               uint4* partials, uint4* red, uint4* out, hipStream_t s);
 
 // ---- lincheck ----
+// Keccak-f[1600] circuit marginals over 2^16 columns: sa = A_0^T eq, sb = B_0^T eq and, when
+// comb is non-null, comb = alpha * sa + sb. inv: 1600 x 11 phi^T gather indices; trow: 2 x
+// 1600 x 11 chi row offsets (A then B); rcm: 25 x 50 words of round-constant vectors.
+hipError_t launch_kc_fold(const uint4* eq, const uint16_t* inv, const uint16_t* trow,
+                          const uint32_t* rcm, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
+                          hipStream_t s);
 hipError_t launch_lc_csc_fold(uint64_t n_cols, const uint32_t* a_ptr, const uint32_t* a_rows,
                               uint64_t nnz_a, const uint32_t* b_ptr, const uint32_t* b_rows,
                               uint64_t nnz_b, const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,

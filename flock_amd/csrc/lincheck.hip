@@ -19,7 +19,7 @@
 namespace flock {
 
 // ---- CSC fold (CscCircuit::fold_split / fold_alpha_batched, lincheck.rs:432-497) ----
-// One thread per column. Row indexes must be < n_cols and col_ptr entries <= nnz; a
+// One wave per column. Row indexes must be < n_cols and col_ptr entries <= nnz; a
 // violation sets *err and the offending entries are skipped.
 __global__ __launch_bounds__(256) void k_lc_csc_fold(
     uint64_t n_cols, const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_rows,
@@ -233,6 +233,98 @@ __global__ __launch_bounds__(256) void k_dot2(const f128* __restrict__ w, const 
     block_xor_pair(f256u_reduce(sa), f256u_reduce(sb), partials);
 }
 
+// ---- Keccak-f[1600] circuit marginals (KeccakLincheckCircuit, r1cs_hashes/keccak.rs:692-835) ----
+// sa = A_0^T eq and sb = B_0^T eq over the 2^16 columns without forming A_0, B_0. state_r
+// (r >= 1) is the linear form phi(state_{r-1}) + t_{r-1} + RC_{r-1} with phi = pi o rho o
+// theta, so the coefficient vector on state_r is carried from r = 24 down to 0,
+//   K_{r-1} = phi^T(K_r) + chi[r-1],
+// and K_r lands on the t_{r-1} columns (K_0 on state_0). chi[r][s] sums eq over the t rows of
+// round r whose chi operand (A: B[x+1], B: B[x+2]) reads state_r bit s; the constant column
+// collects the NOT of the A operand, the propagated round constants and the B = [1] sides.
+// Workgroup 0 walks the A side (K_24 = the pin rows' eq), workgroup 1 the B side (K_24 = 0);
+// 1024 threads hold the 1600 state coefficients in LDS.
+//   inv[s][i]: the 11 outputs s_out of phi whose preimage holds s (phi^T gathers);
+//   trow[side][s][i]: column offset within a round's t block of the row of chi[.][s], term i;
+//   rcm[r][w]: bit s of RC_r, the constant vector of state_r, r <= 24 (50 words per r).
+constexpr int kKcStateBits = 1600;
+constexpr uint32_t kKcZc = 4096, kKcS24 = 2048, kKcT = 4160;
+
+__device__ __forceinline__ uint32_t kc_lane_pos(int s) { return 64u * (uint32_t)(s % 25) + (uint32_t)(s / 25); }
+
+__global__ __launch_bounds__(1024) void k_kc_walk(const f128* __restrict__ eq,
+                                                   const uint16_t* __restrict__ inv,
+                                                   const uint16_t* __restrict__ trow,
+                                                   const uint32_t* __restrict__ rcm,
+                                                   f128* __restrict__ sa, f128* __restrict__ sb) {
+    __shared__ f128 kv[2][kKcStateBits];  // 50 KiB ping-pong
+    __shared__ f128 red[16];
+    const int side = blockIdx.x;  // 0: A, 1: B
+    f128* out = side ? sb : sa;
+    const uint16_t* tr = trow + (size_t)side * kKcStateBits * 11;
+    const int t = threadIdx.x;
+    f128 zc = f128_zero();  // this thread's share of the constant column
+    for (int s = t; s < kKcStateBits; s += blockDim.x) {
+        const f128 pin = eq[kKcS24 + kc_lane_pos(s)];
+        if (side == 0) {
+            kv[0][s] = pin;                              // K_24
+            out[kKcT + 23u * 1600u + kc_lane_pos(s)] = pin;  // t_23 takes K_24
+            if ((rcm[24 * 50 + (s >> 5)] >> (s & 31)) & 1u) zc = f128_add(zc, pin);  // RC_24 . pin
+            for (int r = 0; r < 24; r++)  // NOT of every chi A operand
+                zc = f128_add(zc, eq[kKcT + 1600u * r + kc_lane_pos(s)]);
+        } else {
+            kv[0][s] = f128_zero();
+            zc = f128_add3(zc, pin, eq[kc_lane_pos(s)]);  // pin and state_0 rows: B = [1]
+        }
+    }
+    if (t == 0) zc = f128_add(zc, eq[kKcZc]);  // the constant row
+    __syncthreads();
+    int cur = 0;
+    for (int r = 24; r >= 1; r--) {
+        for (int s = t; s < kKcStateBits; s += blockDim.x) {
+            const uint16_t* iv = inv + s * 11;
+            const uint16_t* tv = tr + s * 11;
+            const f128* erow = eq + kKcT + 1600u * (r - 1);
+            f128 v = f128_zero(), c = f128_zero();
+#pragma unroll
+            for (int i = 0; i < 11; i++) {
+                v = f128_add(v, kv[cur][iv[i]]);
+                c = f128_add(c, erow[tv[i]]);
+            }
+            if ((rcm[(r - 1) * 50 + (s >> 5)] >> (s & 31)) & 1u) zc = f128_add(zc, c);
+            const f128 k = f128_add(v, c);  // K_{r-1}
+            kv[cur ^ 1][s] = k;
+            if (r >= 2) {
+                out[kKcT + 1600u * (r - 2) + kc_lane_pos(s)] = k;
+            } else {
+                const uint32_t p = kc_lane_pos(s);
+                out[p] = side == 0 ? f128_add(k, eq[p]) : k;  // state_0 self-loop on A
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    zc = wave_xor_f128(zc);
+    if ((t & 63) == 0) red[t >> 6] = zc;
+    __syncthreads();
+    if (t == 0) {
+        f128 v = f128_zero();
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) v = f128_add(v, red[w]);
+        out[kKcZc] = v;
+    }
+}
+
+// comb = alpha * sa + sb
+__global__ __launch_bounds__(256) void k_lc_combine(const f128* __restrict__ sa,
+                                                     const f128* __restrict__ sb, f128 alpha,
+                                                     uint64_t n, f128* __restrict__ comb) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    if (threadIdx.x < 32) shoup_build_row(table, alpha, threadIdx.x);
+    __syncthreads();
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (uint64_t)gridDim.x * blockDim.x)
+        comb[c] = f128_add(shoup_mul(table, sa[c]), sb[c]);
+}
+
 static hipError_t lc_attrs() {
     static hipError_t st = [] {
         hipError_t e = hipFuncSetAttribute((const void*)k_lc_eval,
@@ -265,6 +357,19 @@ hipError_t launch_lc_csc_fold(uint64_t n_cols, const uint32_t* a_ptr, const uint
     k_lc_csc_fold<<<grid_for(n_cols * 64, 65535), 256, 0, s>>>(n_cols, a_ptr, a_rows, nnz_a, b_ptr,
                                                           b_rows, nnz_b, eq, alpha, sa, sb, comb,
                                                           err);
+    return hipGetLastError();
+}
+
+hipError_t launch_kc_fold(const uint4* eq, const uint16_t* inv, const uint16_t* trow,
+                          const uint32_t* rcm, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
+                          hipStream_t s) {
+    constexpr uint64_t K = 1ull << 16;
+    k_fill_f128_zero<<<grid_for(K, 4096), 256, 0, s>>>(sa, K);
+    k_fill_f128_zero<<<grid_for(K, 4096), 256, 0, s>>>(sb, K);
+    k_kc_walk<<<2, 1024, 0, s>>>(eq, inv, trow, rcm, sa, sb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !comb) return e;
+    k_lc_combine<<<grid_for(K, 1024), 256, 0, s>>>(sa, sb, alpha, K, comb);
     return hipGetLastError();
 }
 

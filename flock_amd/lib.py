@@ -165,6 +165,9 @@ def load():
         "flock_hip_lc_partial_fold": (I, [P, P, U32, U32, S, S, P, P]),
         "flock_hip_lincheck_prove": (I, [P, P, ctypes.POINTER(Csc), P, U32, S, S, F128, P, P, I,
                                          P, P, P, P, P, P]),
+        "flock_hip_lc_keccak_fold": (I, [P, P, F128, P, P, P]),
+        "flock_hip_lincheck_prove_keccak": (I, [P, P, P, U32, S, F128, P, P, I, P, P, P, P, P,
+                                                P]),
         "flock_fs_new": (P, [ctypes.c_char_p, S]),
         "flock_fs_free": (None, [P]),
         "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
@@ -464,16 +467,21 @@ class Context:
         _check(load().flock_hip_lc_partial_fold(self.h, z.ptr, k_log, nu, useful_bits, n_rows,
                                                 eq_outer.ptr, out.ptr))
 
+    @staticmethod
+    def _lincheck_out(k_log: int) -> dict:
+        L = k_log - 6
+        return dict(rounds=np.zeros((max(2 * L, 1), 2), dtype=np.uint64),
+                    z_partial=np.zeros((64, 2), dtype=np.uint64),
+                    matrix_evals=np.zeros((2, 2), dtype=np.uint64),
+                    nonces=np.zeros(L + 3, dtype=np.uint64),
+                    claim=np.zeros((2 + L, 2), dtype=np.uint64))
+
     def lincheck_prove(self, fs: "FsChallenger", circuit: "Csc", z: DeviceBuffer, nu: int,
                        useful_bits: int, n_rows: int, z_skip: int, x_inner_rest, x_outer,
                        grinding: bool, z_vec: DeviceBuffer | None = None) -> dict:
         """prove_union_capture_z_vec_with_grinding (lincheck/union.rs:203-430), one slot."""
         L = circuit.k_log - 6
-        out = dict(rounds=np.zeros((max(2 * L, 1), 2), dtype=np.uint64),
-                   z_partial=np.zeros((64, 2), dtype=np.uint64),
-                   matrix_evals=np.zeros((2, 2), dtype=np.uint64),
-                   nonces=np.zeros(L + 3, dtype=np.uint64),
-                   claim=np.zeros((2 + L, 2), dtype=np.uint64))
+        out = self._lincheck_out(circuit.k_log)
         xr = _ints_arr(x_inner_rest)
         xo = _ints_arr(x_outer)
         _check(load().flock_hip_lincheck_prove(
@@ -482,6 +490,27 @@ class Context:
             _np_ptr(out["z_partial"]), _np_ptr(out["matrix_evals"]), _np_ptr(out["nonces"]),
             _np_ptr(out["claim"]), z_vec.ptr if z_vec else None))
         out["rounds"] = out["rounds"][:2 * L]
+        return out
+
+    def lc_keccak_fold(self, eq: DeviceBuffer, alpha: int, sa: DeviceBuffer, sb: DeviceBuffer,
+                       comb: DeviceBuffer | None = None):
+        """KeccakLincheckCircuit marginals (r1cs_hashes/keccak.rs:692-835) over 2^16 columns."""
+        _check(load().flock_hip_lc_keccak_fold(self.h, eq.ptr, f128(alpha), sa.ptr, sb.ptr,
+                                               comb.ptr if comb else None))
+
+    def lincheck_prove_keccak(self, fs: "FsChallenger", z: DeviceBuffer, nu: int, n_rows: int,
+                              z_skip: int, x_inner_rest, x_outer, grinding: bool,
+                              z_vec: DeviceBuffer | None = None) -> dict:
+        """lincheck_prove with KeccakLincheckCircuit (k_log 16, keccak.rs:678-835)."""
+        out = self._lincheck_out(16)
+        xr = _ints_arr(x_inner_rest)
+        xo = _ints_arr(x_outer)
+        _check(load().flock_hip_lincheck_prove_keccak(
+            self.h, fs.h, z.ptr, nu, n_rows, f128(z_skip), _np_ptr(xr), _np_ptr(xo),
+            1 if grinding else 0, _np_ptr(out["rounds"]), _np_ptr(out["z_partial"]),
+            _np_ptr(out["matrix_evals"]), _np_ptr(out["nonces"]), _np_ptr(out["claim"]),
+            z_vec.ptr if z_vec else None))
+        out["rounds"] = out["rounds"][:2 * 10]
         return out
 
     def fold_1b_rows(self, rows: DeviceBuffer, n_rows: int, t: DeviceBuffer) -> np.ndarray:

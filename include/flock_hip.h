@@ -244,6 +244,21 @@ int flock_hip_lincheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const flock_csc* 
                              const flock_f128* h_x_outer, int grinding, flock_f128* h_rounds,
                              flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
                              uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
+/* Keccak-f[1600] circuit marginals without forming its matrices: KeccakLincheckCircuit::
+ * fold_alpha_batched (r1cs_hashes/keccak.rs:692-835) as fold_split (lincheck.rs:258-266).
+ * Over the 2^16 columns, d_sa = A_0^T eq and d_sb = B_0^T eq for the row weights d_eq
+ * (2^16 words); d_comb = alpha d_sa + d_sb when non-NULL. */
+int flock_hip_lc_keccak_fold(flock_hip_ctx* ctx, const flock_f128* d_eq, flock_f128 alpha,
+                             flock_f128* d_sa, flock_f128* d_sb, flock_f128* d_comb);
+/* flock_hip_lincheck_prove with KeccakLincheckCircuit (keccak.rs:678-835) in place of a CSC
+ * circuit: k_log 16, useful_bits 42,560, constant pin at column 4096 (keccak.rs:436-451,
+ * 688-690); the lincheck call site for Keccak batches (prover.rs:1154-1160). */
+int flock_hip_lincheck_prove_keccak(flock_hip_ctx* ctx, flock_fs* fs, const uint8_t* d_z,
+                                    uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                    const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                    int grinding, flock_f128* h_rounds,
+                                    flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
+                                    uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
 
 /* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ------------ */
 

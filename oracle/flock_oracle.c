@@ -1103,6 +1103,34 @@ void or_lc_bind_top(or_f128* v, size_t len, or_f128 r) {
  * bit, rounds 2 bits, skip k_skip bits. const_pin < 0 = none.
  * Outputs: rounds[2L] (L = k_log - 6), z_partial[64], matrix_evals[2], nonces,
  * claim[2 + L] = (r_inner_skip, w, r_inner_rest[0..L)), z_vec[2^k_log] (captured). */
+/* The circuit enters the lincheck only through its column marginals (LincheckCircuit,
+ * lincheck.rs:238-278): a CSC matrix pair or a matrix-free walker. */
+typedef struct {
+    const uint32_t *a_ptr, *a_rows, *b_ptr, *b_rows;
+} csc_ref;
+typedef void (*fold_split_fn)(const void* circuit, size_t n_cols, const or_f128* eq,
+                              or_f128* sa, or_f128* sb);
+
+static void csc_fold_cb(const void* circuit, size_t n_cols, const or_f128* eq, or_f128* sa,
+                        or_f128* sb) {
+    const csc_ref* k = (const csc_ref*)circuit;
+    or_csc_fold_split(n_cols, k->a_ptr, k->a_rows, k->b_ptr, k->b_rows, eq, sa, sb);
+}
+
+static void keccak_fold_cb(const void* circuit, size_t n_cols, const or_f128* eq, or_f128* sa,
+                           or_f128* sb) {
+    (void)circuit;
+    (void)n_cols;
+    or_keccak_lc_fold_split(eq, sa, sb);
+}
+
+static int lincheck_prove_with(fold_split_fn fold, const void* circuit, or_fs* fs, int k_log,
+                               long const_pin, const uint8_t* z, int nu, size_t useful_bits,
+                               size_t n_rows, or_f128 z_skip, const or_f128* x_inner_rest,
+                               const or_f128* x_outer, int grinding, or_f128* rounds,
+                               or_f128 z_partial[64], or_f128 matrix_evals[2],
+                               uint64_t* nonces, or_f128* claim, or_f128* z_vec_out);
+
 int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_t* a_rows,
                       const uint32_t* b_ptr, const uint32_t* b_rows, long const_pin,
                       const uint8_t* z, int nu, size_t useful_bits, size_t n_rows,
@@ -1110,6 +1138,29 @@ int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_
                       int grinding, or_f128* rounds, or_f128 z_partial[64],
                       or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
                       or_f128* z_vec_out) {
+    const csc_ref k = {a_ptr, a_rows, b_ptr, b_rows};
+    return lincheck_prove_with(csc_fold_cb, &k, fs, k_log, const_pin, z, nu, useful_bits, n_rows,
+                               z_skip, x_inner_rest, x_outer, grinding, rounds, z_partial,
+                               matrix_evals, nonces, claim, z_vec_out);
+}
+
+int or_lincheck_prove_keccak(or_fs* fs, const uint8_t* z, int nu, size_t n_rows,
+                             or_f128 z_skip, const or_f128* x_inner_rest, const or_f128* x_outer,
+                             int grinding, or_f128* rounds, or_f128 z_partial[64],
+                             or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
+                             or_f128* z_vec_out) {
+    /* K_LOG 16, USEFUL_BITS 42,560, const pin Z_CONST = 4096 (keccak.rs:436-451, 688-690) */
+    return lincheck_prove_with(keccak_fold_cb, NULL, fs, 16, 4096, z, nu, 42560, n_rows, z_skip,
+                               x_inner_rest, x_outer, grinding, rounds, z_partial, matrix_evals,
+                               nonces, claim, z_vec_out);
+}
+
+static int lincheck_prove_with(fold_split_fn fold, const void* circuit, or_fs* fs, int k_log,
+                               long const_pin, const uint8_t* z, int nu, size_t useful_bits,
+                               size_t n_rows, or_f128 z_skip, const or_f128* x_inner_rest,
+                               const or_f128* x_outer, int grinding, or_f128* rounds,
+                               or_f128 z_partial[64], or_f128 matrix_evals[2],
+                               uint64_t* nonces, or_f128* claim, or_f128* z_vec_out) {
     const int ks = 6;
     if (k_log < ks || nu < 3) return 1;
     const size_t K = (size_t)1 << k_log;
@@ -1125,7 +1176,7 @@ int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_
     or_f128* comb = (or_f128*)malloc(sizeof(or_f128) * K);
     or_f128* zv = (or_f128*)malloc(sizeof(or_f128) * K);
     or_quirky_eq(z_skip, x_inner_rest, (size_t)L, eq_inner);
-    or_csc_fold_split(K, a_ptr, a_rows, b_ptr, b_rows, eq_inner, sa, sb);
+    fold(circuit, K, eq_inner, sa, sb);
     for (size_t c = 0; c < K; c++) comb[c] = add(or_f128_mul(alpha, sa[c]), sb[c]);
     if (const_pin >= 0) {
         if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 1);

@@ -148,6 +148,12 @@ int or_lincheck_prove(or_fs* fs, int k_log, const uint32_t* a_ptr, const uint32_
                       int grinding, or_f128* rounds, or_f128 z_partial[64],
                       or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
                       or_f128* z_vec_out);
+/* the same with KeccakLincheckCircuit (keccak.rs:678-835): k_log 16, const pin 4096 */
+int or_lincheck_prove_keccak(or_fs* fs, const uint8_t* z, int nu, size_t n_rows,
+                             or_f128 z_skip, const or_f128* x_inner_rest, const or_f128* x_outer,
+                             int grinding, or_f128* rounds, or_f128 z_partial[64],
+                             or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
+                             or_f128* z_vec_out);
 
 /* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ---- */
 #define OR_MAX_TYPES 16

@@ -167,3 +167,57 @@ def test_lincheck_and_zerocheck_on_blake3_witness(ctx):
                              pts[1:1 + L], pts[1 + L:], True)
     for name in ("rounds", "z_partial", "matrix_evals", "claim", "nonces"):
         assert np.array_equal(got[name], exp[name]), name
+
+
+def test_keccak_lincheck_marginals(ctx):
+    # the GPU walker against the oracle's restated walker (itself pinned by
+    # <A_0^T eq, z> = <eq, a> on FIPS 202 witnesses), on random row weights
+    from test_oracle_keccak_r1cs import K, lc_fold_split
+    say("case: keccak lincheck marginals")
+    rng = np.random.default_rng(0x4B)
+    eq = rng.integers(0, 1 << 63, (K, 2), dtype=np.uint64) * 2 + 1
+    alpha = int(rng.integers(1, 1 << 62)) << 64 | int(rng.integers(1, 1 << 62))
+    esa, esb = lc_fold_split(eq)
+    d_eq = ctx.upload(eq)
+    sa, sb, comb = (ctx.alloc(16 * K) for _ in range(3))
+    ctx.lc_keccak_fold(d_eq, alpha, sa, sb, comb)
+    assert np.array_equal(sa.download_f128(K), esa)
+    assert np.array_equal(sb.download_f128(K), esb)
+    ecomb = O.mul_batch(np.tile(O.ints_to_arr([alpha]), (K, 1)), esa) ^ esb
+    assert np.array_equal(comb.download_f128(K), ecomb)
+
+
+@pytest.mark.parametrize("n_log,n_decl", [(3, 5), (7, 100)])
+def test_lincheck_and_zerocheck_on_keccak_witness(ctx, n_log, n_decl):
+    # the Keccak-f[1600] witness from the GPU, the lincheck with the walker circuit (k_log 16)
+    # and the zerocheck over (a, b, c = z), both against the oracle
+    from test_oracle_keccak_r1cs import random_states
+    from test_oracle_lincheck import lincheck_prove_keccak
+    from test_oracle_zerocheck import zerocheck_prove
+    say(f"case: lincheck+zerocheck on keccak witness n_log={n_log} n_decl={n_decl}")
+    n = 1 << n_log
+    z, a, b = (ctx.alloc(16 * 512 * n) for _ in range(3))
+    stripe = ctx.alloc(8192 * n)
+    inputs = random_states(n_decl, 77 + n_log)
+    ctx.witness_keccak(ctx.upload(np.ascontiguousarray(inputs, dtype=np.uint64)), n_decl, n_log,
+                       z, a, b, stripe)
+    dom = b"flock-keccak-witness"
+    L = 10
+    pts = O.arr_to_ints(O.splitmix(0x4C + n_log, 1 + L + n_log))
+    zs = np.frombuffer(stripe.download(8192 * n), dtype=np.uint8).copy()
+    # padding rows run keccak_f(0), so every row of the batch is a constraint row
+    exp = lincheck_prove_keccak(dom, zs, n_log, n, pts[0], pts[1:1 + L], pts[1 + L:], 1)
+    zv = ctx.alloc(16 << 16)
+    got = ctx.lincheck_prove_keccak(F.FsChallenger(dom), stripe, n_log, n, pts[0],
+                                    pts[1:1 + L], pts[1 + L:], True, z_vec=zv)
+    for name in ("rounds", "z_partial", "matrix_evals", "claim", "nonces"):
+        assert np.array_equal(got[name], exp[name]), name
+    assert np.array_equal(zv.download_f128(1 << 16), exp["z_vec"])
+    if n_log <= 3:
+        hz, ha, hb = (np.frombuffer(x.download(16 * 512 * n), dtype=np.uint8).copy()
+                      for x in (z, a, b))
+        m = n_log + 16
+        exp = zerocheck_prove(dom, ha, hb, hz, m, 1)
+        got = ctx.zerocheck_prove(F.FsChallenger(dom), a, b, z, m, True)
+        for g, e in zip(got, exp):
+            assert np.array_equal(g, e)

@@ -281,3 +281,28 @@ def test_lincheck_prover_no_rounds():
     # k_log = k_skip: no multilinear rounds, z_partial = z_vec
     out = _check_prover(6, 3, 0, 1, seed=3)
     assert np.array_equal(out["z_partial"], out["z_vec"])
+
+
+L.or_lincheck_prove_keccak.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_size_t, O.F128, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int] + [ctypes.c_void_p] * 6
+
+
+def lincheck_prove_keccak(dom, z, nu, n_rows, z_skip, x_inner_rest, x_outer, grinding):
+    """The oracle prover with KeccakLincheckCircuit (k_log 16). Returns dict of numpy arrays."""
+    Lr = 10
+    fs = O.Fs(dom)
+    out = dict(rounds=np.zeros((2 * Lr, 2), dtype=np.uint64),
+               z_partial=np.zeros((64, 2), dtype=np.uint64),
+               matrix_evals=np.zeros((2, 2), dtype=np.uint64),
+               nonces=np.zeros(Lr + 3, dtype=np.uint64),
+               claim=np.zeros((2 + Lr, 2), dtype=np.uint64),
+               z_vec=np.zeros((1 << 16, 2), dtype=np.uint64))
+    xr, xo = O.ints_to_arr(x_inner_rest), O.ints_to_arr(x_outer)
+    rc = L.or_lincheck_prove_keccak(fs.h, z.ctypes.data, nu, n_rows, O.f(z_skip), O._p(xr),
+                                    O._p(xo), grinding, O._p(out["rounds"]),
+                                    O._p(out["z_partial"]), O._p(out["matrix_evals"]),
+                                    out["nonces"].ctypes.data, O._p(out["claim"]),
+                                    O._p(out["z_vec"]))
+    assert rc == 0
+    return out
