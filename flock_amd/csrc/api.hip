@@ -89,12 +89,11 @@ __attribute__((target("pclmul,sse2"))) F mul_pclmul(F a, F b) {
     return reduce256(l[0], l[1] ^ c[0], h[0] ^ c[1], h[1]);
 }
 bool have_pclmul() {
-    static int v = -1;
-    if (v < 0) {
+    static const bool v = [] {
         unsigned a, b, c, d;
-        v = (__get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL)) ? 1 : 0;
-    }
-    return v == 1;
+        return __get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL);
+    }();
+    return v;
 }
 #endif
 
@@ -369,6 +368,7 @@ void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n) {
     fs->squeeze(b.data(), n * 16);
     for (size_t i = 0; i < n; i++) out[i] = le128(b.data() + 16 * i);
 }
+void flock_fs_digest(flock_fs* fs, uint8_t out[32]) { fs->st.digest(out); }
 // PoW (challenger.rs:884-958, 1096-1135): smallest nonce with SHA256(digest || nonce_le)
 // having `bits` leading zero bits (MSB-first), then observe_bytes(nonce_le).
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
@@ -431,7 +431,7 @@ struct flock_hip_ctx {
         if (zc_ext) return 0;
         // Extension matrix: M[s][i] = L_s(64 + i) over F8, nodes S = {0..63}: the S -> Lambda
         // extension of the unit vector e_s (inverse NTT on S, forward on Lambda).
-        static uint8_t M[64][64];
+        uint8_t M[64][64];
         for (int s0 = 0; s0 < 64; s0++) {
             uint8_t den = 1;
             for (int j = 0; j < 64; j++)

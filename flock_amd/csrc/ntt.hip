@@ -219,13 +219,10 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
                               int l0, const f128* evals, bool from_msg, hipStream_t s) {
     constexpr int NT = (1 << K) - 1;
     const size_t lds = (size_t)NT * SHOUP_ENTRIES * sizeof(f128);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_ntt_pass<K>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
+    // set once per process (thread-safe static initialisation)
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_ntt_pass<K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (attr != hipSuccess) return attr;
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
     if (cols <= kDeepCols) {
@@ -234,15 +231,10 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
         const uint64_t grid = (n_hb + per - 1) / per;
         if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
         const size_t dyn = (size_t)per * NT * (16 * sizeof(f128) + sizeof(uint32_t));
-        static bool deep_attr = false;
-        if (!deep_attr) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_ntt_pass_deep<K>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)((size_t)kDeepMaxHb * NT *
-                                                     (16 * sizeof(f128) + sizeof(uint32_t))));
-            if (e != hipSuccess) return e;
-            deep_attr = true;
-        }
+        static const hipError_t deep_attr = hipFuncSetAttribute(
+            (const void*)k_ntt_pass_deep<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            (int)((size_t)kDeepMaxHb * NT * (16 * sizeof(f128) + sizeof(uint32_t))));
+        if (deep_attr != hipSuccess) return deep_attr;
         k_ntt_pass_deep<K><<<(unsigned)grid, kDeepThreads, dyn, s>>>(
             src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
         return hipGetLastError();

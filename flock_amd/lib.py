@@ -98,6 +98,29 @@ class PcsParams(ctypes.Structure):
     ]
 
 
+HASH_SHA256, HASH_BLAKE3, HASH_KECCAK = 0, 1, 2
+
+
+class ProveRequest(ctypes.Structure):
+    """flock_prove_request: one prove-core request (include/flock_hip.h)."""
+
+    _fields_ = [("kind", ctypes.c_uint32), ("n_log", ctypes.c_uint32),
+                ("n_decl", ctypes.c_uint64), ("h_inputs", ctypes.c_void_p),
+                ("circuit", ctypes.c_void_p), ("digest", ctypes.c_char_p),
+                ("domain", ctypes.c_char_p), ("domain_len", ctypes.c_size_t),
+                ("arrival_s", ctypes.c_double)]
+
+
+class ProveResult(ctypes.Structure):
+    """flock_prove_result."""
+
+    _fields_ = [("root", ctypes.c_uint8 * 32), ("transcript", ctypes.c_uint8 * 32),
+                ("zc_z", F128), ("lc_w", F128), ("hashes", ctypes.c_uint64),
+                ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("status", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -177,6 +200,14 @@ def load():
         "flock_fs_sample_f128": (None, [P, ctypes.POINTER(F128)]),
         "flock_fs_sample_f128_vec": (None, [P, P, S]),
         "flock_fs_grind_pow": (U64, [P, U32]),
+        "flock_fs_digest": (None, [P, ctypes.c_char_p]),
+        "flock_hip_prover_create": (I, [I, ctypes.POINTER(P)]),
+        "flock_hip_prover_destroy": (None, [P]),
+        "flock_hip_prover_ctx": (P, [P]),
+        "flock_hip_prove_footprint": (I, [ctypes.POINTER(ProveRequest), ctypes.POINTER(U64)]),
+        "flock_hip_prove_core": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult)]),
+        "flock_hip_serve": (I, [ctypes.POINTER(P), U32, ctypes.POINTER(ProveRequest), S, U64,
+                                ctypes.POINTER(ProveResult), ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -608,8 +639,74 @@ class FsChallenger:
     def grind_pow(self, bits: int) -> int:
         return int(load().flock_fs_grind_pow(self.h, bits))
 
+    def digest(self) -> bytes:
+        """SHA-256 of everything the transcript has absorbed."""
+        out = ctypes.create_string_buffer(32)
+        load().flock_fs_digest(self.h, out)
+        return out.raw
+
     def bind_statement(self, registry_digest: bytes, counts, cap: bytes):
         """UnionInstance::bind_statement (union.rs:619-628)."""
         c = np.ascontiguousarray(counts, dtype=np.uint64)
         _check(load().flock_union_bind_statement(self.h, registry_digest, c.ctypes.data, len(c),
                                                  cap, len(cap)))
+
+
+class Prover:
+    """flock_hip_prover: a context plus grow-only device buffers for prove-core requests."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(load().flock_hip_prover_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().flock_hip_prover_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_requests(specs, digest: bytes = bytes(32), domain: bytes = b"flock-readme-bench-v0"):
+    """specs: (kind, n_log, inputs ndarray, circuit Csc or None, arrival_s). Returns the ctypes
+    request array and the objects it points into (keep them alive while it is used)."""
+    arr = (ProveRequest * len(specs))()
+    keep = [digest, domain]
+    for i, (kind, n_log, inputs, circuit, arrival) in enumerate(specs):
+        inp = np.ascontiguousarray(inputs)
+        keep.append(inp)
+        arr[i].kind, arr[i].n_log, arr[i].n_decl = kind, n_log, inp.shape[0]
+        arr[i].h_inputs = inp.ctypes.data
+        arr[i].circuit = ctypes.cast(ctypes.pointer(circuit), ctypes.c_void_p) if circuit else None
+        if circuit is not None:
+            keep.append(circuit)
+        arr[i].digest, arr[i].domain, arr[i].domain_len = digest, domain, len(domain)
+        arr[i].arrival_s = arrival
+    return arr, keep
+
+
+def prove_core(prover: Prover, req: ProveRequest) -> ProveResult:
+    res = ProveResult()
+    _check(load().flock_hip_prove_core(prover.h, ctypes.byref(req), ctypes.byref(res)))
+    return res
+
+
+def prove_footprint(req: ProveRequest) -> int:
+    b = ctypes.c_uint64()
+    _check(load().flock_hip_prove_footprint(ctypes.byref(req), ctypes.byref(b)))
+    return int(b.value)
+
+
+def serve(provers, reqs, hbm_budget: int):
+    """flock_hip_serve over the given provers (one worker each). Returns (results, wall_ms)."""
+    hs = (ctypes.c_void_p * len(provers))(*[p.h.value for p in provers])
+    res = (ProveResult * len(reqs))()
+    wall = ctypes.c_double()
+    _check(load().flock_hip_serve(hs, len(provers), reqs, len(reqs), hbm_budget, res,
+                                  ctypes.byref(wall)))
+    return res, wall.value

@@ -359,6 +359,11 @@ void flock_fs_sample_f128(flock_fs* fs, flock_f128* out);
 void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n);
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits);
 
+/* The transcript's running state digest: SHA-256 of everything absorbed so far (the value
+ * grind_pow hashes with its nonce, challenger.rs:884-958). Two transcripts that absorbed the
+ * same bytes give the same digest. */
+void flock_fs_digest(flock_fs* fs, uint8_t out[32]);
+
 /* ---- protocol drivers ------------------------------------------------------------ */
 
 /* Product sumcheck followed by an eq-weighted zerocheck tail on the same two tables of
@@ -374,6 +379,68 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits);
 int flock_hip_prove_sumcheck(flock_hip_ctx* ctx, flock_fs* fs, const flock_f128* d_a,
                              const flock_f128* d_b, uint32_t log_n, flock_f128* h_msgs,
                              flock_f128 h_finals[4]);
+
+/* ---- prove core and serving (host orchestration over the entry points above) ----------- */
+
+/* The three hash circuits of the batched and served workloads (r1cs_hashes/{sha2,blake3,
+ * keccak}.rs). */
+#define FLOCK_HASH_SHA256 0u
+#define FLOCK_HASH_BLAKE3 1u
+#define FLOCK_HASH_KECCAK 2u
+
+/* One proving request. h_inputs: n_decl instances of 24 u32 (SHA-256: h_in[8], m[16]), 28 u32
+ * (BLAKE3: cv[8], m[16], counter_lo, counter_hi, block_len, flags) or 25 u64 (Keccak state
+ * lanes). circuit: the device CSC circuit (SHA-256, BLAKE3); NULL for Keccak (walker).
+ * digest: the 32-byte Registry::digest bound into the transcript. arrival_s: arrival offset
+ * for flock_hip_serve (non-decreasing across a request array). 3 <= n_log <= 24. */
+typedef struct {
+    uint32_t kind;
+    uint32_t n_log;
+    uint64_t n_decl;
+    const void* h_inputs;
+    const flock_csc* circuit;
+    const uint8_t* digest;
+    const uint8_t* domain;
+    size_t domain_len;
+    double arrival_s;
+} flock_prove_request;
+
+/* root: the commitment root (cap depth 0); transcript: flock_fs_digest after the lincheck,
+ * which has absorbed the statement, every zerocheck and lincheck message and nonce;
+ * zc_z / lc_w: the zerocheck skip point and the lincheck claim value; hashes = n_decl.
+ * queued_ms, run_ms, done_s: set by flock_hip_serve (wait after arrival, run time, completion
+ * time from the start of the call). */
+typedef struct {
+    uint8_t root[32];
+    uint8_t transcript[32];
+    flock_f128 zc_z, lc_w;
+    uint64_t hashes;
+    double queued_ms, run_ms, done_s;
+    int status;
+    int reserved;
+} flock_prove_result;
+
+/* A prover: one context (HIP stream) plus grow-only device buffers reused across requests. */
+typedef struct flock_hip_prover flock_hip_prover;
+int flock_hip_prover_create(int device, flock_hip_prover** out);
+void flock_hip_prover_destroy(flock_hip_prover* p);
+flock_hip_ctx* flock_hip_prover_ctx(flock_hip_prover* p);
+/* Device bytes one request needs (the admission size of flock_hip_serve). */
+int flock_hip_prove_footprint(const flock_prove_request* req, uint64_t* bytes);
+/* The GPU prove core of prove_fast for a registry with one boolean hash type
+ * (prover.rs:926-1160): witness, dense stack, lane-major commit (rate 1/2, log_batch_size 6),
+ * bind_statement, zerocheck with s_hat_v_c, x_ab split and the union lincheck, with grinding.
+ * Stops before the PCS opening (pcs.rs:1433), so it is not a complete proof. */
+int flock_hip_prove_core(flock_hip_prover* p, const flock_prove_request* req,
+                         flock_prove_result* res);
+/* Serving loop: `workers` host threads, one prover each, take the requests in arrival order;
+ * a request starts once its arrival time has passed and its footprint fits hbm_budget beside
+ * the requests in flight (always, when none is). Results and timings go to res[i]; wall_ms
+ * gets the time from the call's start to the last completion. No counterpart in the reference
+ * Rust (its CUDA FFI serialises calls behind one mutex, prove_ffi.cu:421-422). */
+int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
+                    const flock_prove_request* reqs, size_t n, uint64_t hbm_budget,
+                    flock_prove_result* res, double* wall_ms);
 
 #ifdef __cplusplus
 }

@@ -550,6 +550,9 @@ static int leading_zero_bits(const uint8_t* h, uint32_t bits) {
 
 /* grind_pow: smallest nonce with SHA256(state_digest || nonce_le) having `bits`
  * leading zero bits, then observe_bytes(nonce_le), challenger.rs:884-958, 1127-1135 */
+/* SHA-256 of everything absorbed so far (the digest grind_pow hashes, challenger.rs:884-958) */
+void or_fs_digest(or_fs* fs, uint8_t out[32]) { sha_final(&fs->st, out); }
+
 uint64_t or_fs_grind_pow(or_fs* fs, uint32_t bits) {
     uint8_t digest[32];
     sha_final(&fs->st, digest);

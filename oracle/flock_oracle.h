@@ -74,6 +74,7 @@ void or_fs_observe_bytes(or_fs* fs, const uint8_t* b, size_t n);
 or_f128 or_fs_sample_f128(or_fs* fs);
 void or_fs_sample_f128_vec(or_fs* fs, or_f128* out, size_t n);
 uint64_t or_fs_grind_pow(or_fs* fs, uint32_t bits);
+void or_fs_digest(or_fs* fs, uint8_t out[32]);
 
 /* ---- sumcheck rounds ---- */
 /* round_msg_lsb, crates/flock-core/src/pcs/ligerito.rs:3204-3240 */
