@@ -40,6 +40,8 @@ SHA256_IV = [0x6A09E667, 0xBB67AE85, 0x3C6EF372, 0xA54FF53A, 0x510E527F, 0x9B056
              0x5BE0CD19]
 SHA256_USEFUL_BITS, SHA256_K_LOG, SHA256_Z_CONST = 25_470, 15, 25_469
 SHA256_NNZ = (28_839_569, 18_527_069)  # nonzeros of (A_0, B_0) of the SHA-256 circuit
+BLAKE3_K_LOG, BLAKE3_Z_CONST = 14, 11_706
+BLAKE3_NNZ = (27_251_212, 17_191_286)  # nonzeros of (A_0, B_0) of the BLAKE3 circuit
 
 
 def log(*a):
@@ -82,6 +84,88 @@ def synthetic_csc(k: int, nnz: int, seed: int):
     return ptr, rows
 
 
+def request_inputs(F, kind: int, n: int, rng) -> np.ndarray:
+    """Seeded inputs of one request: SHA-256 (IV, random block), BLAKE3 (random cv and block,
+    block_len 64, flags 11), Keccak-f[1600] (random 1600-bit states)."""
+    if kind == F.HASH_SHA256:
+        x = np.zeros((n, 24), dtype=np.uint32)
+        x[:, :8] = SHA256_IV
+        x[:, 8:] = rng.integers(0, 1 << 32, (n, 16), dtype=np.uint64).astype(np.uint32)
+    elif kind == F.HASH_BLAKE3:
+        x = rng.integers(0, 1 << 32, (n, 28), dtype=np.uint64).astype(np.uint32)
+        x[:, 26], x[:, 27] = 64, 11
+    else:
+        x = rng.integers(0, 1 << 63, (n, 25), dtype=np.uint64) * 2 + \
+            rng.integers(0, 2, (n, 25), dtype=np.uint64)
+    return x
+
+
+def mixed_workloads(F, ctx, args) -> dict:
+    """The batched form (64 uneven requests at once, kinds uniform over SHA-256 / BLAKE3 /
+    Keccak-f[1600], batch log2 uniform in [10, 16]) and the served form (Poisson arrivals,
+    batch log2 in [10, 18], a Keccak 2^18 request included), through flock_hip_serve."""
+    circ = {}
+    keep = []
+    for kind, k_log, nnz, pin in ((F.HASH_SHA256, SHA256_K_LOG, SHA256_NNZ, SHA256_Z_CONST),
+                                  (F.HASH_BLAKE3, BLAKE3_K_LOG, BLAKE3_NNZ, BLAKE3_Z_CONST)):
+        csc = [synthetic_csc(1 << k_log, z, 31 + kind * 2 + i) for i, z in enumerate(nnz)]
+        c, kp = ctx.csc(k_log, *csc[0], *csc[1], const_pin=pin)
+        circ[kind] = c
+        keep.append(kp)
+    rng = np.random.default_rng(0xC4C5)
+
+    def specs(n_req, lo, hi, rate, force_big):
+        out, t = [], 0.0
+        for i in range(n_req):
+            kind = int(rng.integers(0, 3))
+            nl = int(rng.integers(lo, hi + 1))
+            if force_big and i == n_req // 2:
+                kind, nl = F.HASH_KECCAK, hi
+            if rate:
+                t += float(rng.exponential(1.0 / rate))
+            out.append((kind, nl, request_inputs(F, kind, 1 << nl, rng),
+                        circ.get(kind), t))
+        return out
+
+    budget = int(288e9 * 0.9)
+    provers = [F.Prover(0) for _ in range(args.workers)]
+    res = {}
+    # batched: every request at t = 0; one worker (one proof at a time) vs all workers
+    b_specs = specs(64, 10, 16, 0.0, False)
+    reqs, kb = F.make_requests(b_specs)
+    hashes = sum(1 << s[1] for s in b_specs)
+    F.serve(provers, reqs, budget)  # warm-up: buffers grow to their largest shapes
+    _, wall1 = F.serve(provers[:1], reqs, budget)
+    out, walln = F.serve(provers, reqs, budget)
+    lat = [r.done_s * 1e3 for r in out]
+    res["batched"] = {
+        "requests": len(b_specs), "hashes": hashes, "workers": args.workers,
+        "one_at_a_time_khps": round(hashes / wall1, 2),
+        "batched_khps": round(hashes / walln, 2),
+        "speedup": round(wall1 / walln, 3),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+    }
+    # served: Poisson arrivals at args.serve_rate requests/s
+    s_specs = specs(32, 10, 18, args.serve_rate, True)
+    reqs, ks = F.make_requests(s_specs)
+    hashes = sum(1 << s[1] for s in s_specs)
+    F.serve(provers, reqs, budget)  # warm-up
+    out, wall = F.serve(provers, reqs, budget)
+    lat = [r.queued_ms + r.run_ms for r in out]
+    res["served"] = {
+        "requests": len(s_specs), "rate_per_s": args.serve_rate, "workers": args.workers,
+        "hashes": hashes, "khps": round(hashes / wall, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "largest": "keccak-f[1600] 2^18",
+        "hbm_budget_bytes": budget,
+    }
+    for p in provers:
+        p.close()
+    return res
+
+
 def cpu_baseline(t: int, log_dim: int, reps: int) -> dict:
     """The oracle's scalar NTT (one core) on the commit shape of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -112,6 +196,11 @@ def main():
     ap.add_argument("--n-log", type=int, default=16, help="log2 of the SHA-256 batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sumcheck", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the batched and served mixed-hash workloads")
+    ap.add_argument("--workers", type=int, default=8, help="serving workers (HIP streams)")
+    ap.add_argument("--serve-rate", type=float, default=60.0,
+                    help="Poisson arrival rate of the served workload, requests/s")
     ap.add_argument("--latency-requests", type=int, default=50,
                     help="sequential prove-core requests timed one by one for p50/p99")
     ap.add_argument("--ntt-only", action="store_true",
@@ -338,6 +427,9 @@ def main():
             st.append(time.perf_counter() - t1)
         result["sumcheck_2^24_ms"] = round(min(st) * 1e3, 3)
         result["sumcheck_2^24_algorithmic_gbs"] = round((48 + 56) * 2 * nn / min(st) / 1e9, 3)
+
+    if not args.no_mixed and not args.ntt_only:
+        result.update(mixed_workloads(F, ctx, args))
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(t, k - 1, reps=3)
