@@ -404,7 +404,7 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
 struct flock_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words
+    uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words, then the grid-reduce ticket counter
     uint4* dout = nullptr;      // 2 words
     flock_f128* hout = nullptr; // pinned, 2 words
     std::map<int, uint4*> evals;  // k -> k x k basis table
@@ -578,13 +578,19 @@ int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
     if (!c) return fail(FLOCK_HIP_E_NOMEM, "ctx");
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->partials, sizeof(uint4) * flock::kMaxReduceBlocks * 2) != hipSuccess ||
+        hipMalloc(&c->partials, sizeof(uint4) * (flock::kMaxReduceBlocks * 2 + 1)) != hipSuccess ||
         hipMalloc(&c->dout, sizeof(uint4) * 2) != hipSuccess ||
         hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 2, hipHostMallocDefault) !=
             hipSuccess) {
         hipError_t e = hipGetLastError();
         flock_hip_ctx_destroy(c);
         return fail(FLOCK_HIP_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    // the grid-reduce ticket counter after the partials starts at zero (each reduction that
+    // uses it leaves it at zero)
+    if (hipMemset(c->partials + 2 * flock::kMaxReduceBlocks, 0, sizeof(uint4)) != hipSuccess) {
+        flock_hip_ctx_destroy(c);
+        return fail(FLOCK_HIP_E_HIP, "context setup: counter");
     }
     *out = c;
     return 0;

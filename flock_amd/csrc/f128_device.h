@@ -300,25 +300,69 @@ __device__ __forceinline__ f128* comb_lane(f128* ws) {
     return w;
 }
 
-// XOR-reduce two values over the block; thread 0 writes partials[2*blockIdx.x + {0,1}].
-__device__ __forceinline__ void block_xor_pair(f128 a0, f128 a1, f128* partials) {
+// XOR-reduce two values over the grid into out[0], out[1] in one launch. With one workgroup,
+// thread 0 writes out directly. Otherwise each workgroup writes its pair to
+// partials[2*blockIdx.x + {0,1}] and takes a ticket from a device-scope counter kept at
+// partials[2*kGridReduceMax] (zero between launches); the workgroup that draws the last
+// ticket folds every pair (volatile loads, past any stale L1 line), writes out and resets
+// the counter. gridDim.x <= kGridReduceMax.
+constexpr int kGridReduceMax = 2048;
+__device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials, f128* out) {
     __shared__ f128 red[2][16];
+    __shared__ int last;
     a0 = wave_xor_f128(a0);
     a1 = wave_xor_f128(a1);
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = (blockDim.x + 63) >> 6;
     if (lane == 0) {
         red[0][wid] = a0;
         red[1][wid] = a1;
     }
     __syncthreads();
+    unsigned* ctr = reinterpret_cast<unsigned*>(partials + 2 * kGridReduceMax);
     if (threadIdx.x == 0) {
         f128 r0 = f128_zero(), r1 = f128_zero();
-        const int nw = (blockDim.x + 63) >> 6;
         for (int w = 0; w < nw; w++) {
             r0 = f128_add(r0, red[0][w]);
             r1 = f128_add(r1, red[1][w]);
         }
-        partials[2 * blockIdx.x] = r0;
-        partials[2 * blockIdx.x + 1] = r1;
+        if (gridDim.x == 1) {
+            out[0] = r0;
+            out[1] = r1;
+        } else {
+            partials[2 * blockIdx.x] = r0;
+            partials[2 * blockIdx.x + 1] = r1;
+            __threadfence();
+            last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+        }
+    }
+    if (gridDim.x == 1) return;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const volatile uint32_t* vp = reinterpret_cast<const volatile uint32_t*>(partials);
+    f128 c0 = f128_zero(), c1 = f128_zero();
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+        const volatile uint32_t* q = vp + 8 * i;
+        c0 = f128_add(c0, make_uint4(q[0], q[1], q[2], q[3]));
+        c1 = f128_add(c1, make_uint4(q[4], q[5], q[6], q[7]));
+    }
+    c0 = wave_xor_f128(c0);
+    c1 = wave_xor_f128(c1);
+    __syncthreads();  // red is reused
+    if (lane == 0) {
+        red[0][wid] = c0;
+        red[1][wid] = c1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        f128 r0 = f128_zero(), r1 = f128_zero();
+        for (int w = 0; w < nw; w++) {
+            r0 = f128_add(r0, red[0][w]);
+            r1 = f128_add(r1, red[1][w]);
+        }
+        out[0] = r0;
+        out[1] = r1;
+        atomicExch(ctr, 0u);
     }
 }

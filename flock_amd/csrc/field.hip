@@ -24,34 +24,6 @@ __global__ __launch_bounds__(256) void k_f128_mul_const_batch(const f128* __rest
         out[i] = shoup_mul(table, a[i]);
 }
 
-// XOR of nblocks partial pairs -> out[0..2). One block of 256 threads.
-__global__ __launch_bounds__(256) void k_xor_reduce(const f128* __restrict__ partials, int nblocks,
-                                                     f128* __restrict__ out) {
-    f128 acc0 = f128_zero(), acc1 = f128_zero();
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-        acc0 = f128_add(acc0, partials[2 * i]);
-        acc1 = f128_add(acc1, partials[2 * i + 1]);
-    }
-    acc0 = wave_xor_f128(acc0);
-    acc1 = wave_xor_f128(acc1);
-    __shared__ f128 red[2][4];
-    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        red[0][wid] = acc0;
-        red[1][wid] = acc1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        f128 r0 = f128_zero(), r1 = f128_zero();
-        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
-            r0 = f128_add(r0, red[0][w]);
-            r1 = f128_add(r1, red[1][w]);
-        }
-        out[0] = r0;
-        out[1] = r1;
-    }
-}
-
 static inline unsigned grid_for(size_t n, unsigned cap) {
     size_t g = (n + 255) / 256;
     if (g < 1) g = 1;
@@ -69,11 +41,6 @@ hipError_t launch_f128_mul_const_batch(const uint4* a, uint4 t, uint4* out, size
                                        hipStream_t s) {
     if (n == 0) return hipSuccess;
     k_f128_mul_const_batch<<<grid_for(n, 4096), 256, 0, s>>>(a, t, out, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_xor_reduce(const uint4* partials, int nblocks, uint4* out, hipStream_t s) {
-    k_xor_reduce<<<1, 256, 0, s>>>(partials, nblocks, out);
     return hipGetLastError();
 }
 

@@ -16,7 +16,6 @@ hipError_t launch_f128_mul_batch(const uint4* a, const uint4* b, uint4* out, siz
                                  hipStream_t s);
 hipError_t launch_f128_mul_const_batch(const uint4* a, uint4 t, uint4* out, size_t n,
                                        hipStream_t s);
-hipError_t launch_xor_reduce(const uint4* partials, int nblocks, uint4* out, hipStream_t s);
 
 // ---- ntt.hip ----
 // Lane-major replicate fill: cw[(rep*D + p)*t + l] = q[l*D + p] for l < t.
@@ -43,7 +42,7 @@ hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t l
                               uint8_t* tree, hipStream_t s);
 
 // ---- sumcheck.hip ----
-// round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2.
+// round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2 + 1 words (the last holds a zeroed ticket counter).
 hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
                                uint4* out, hipStream_t s);
 // fold_and_msg_lsb: nf/nb (n/2) = v0 + r(v0+v1); out = next message (zero if n/2 == 1).

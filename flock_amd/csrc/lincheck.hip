@@ -21,6 +21,32 @@ namespace flock {
 // ---- CSC fold (CscCircuit::fold_split / fold_alpha_batched, lincheck.rs:432-497) ----
 // One wave per column. Row indexes must be < n_cols and col_ptr entries <= nnz; a
 // violation sets *err and the offending entries are skipped.
+// XOR of eq over the rows rows[k0..k1) of one column, lanes striding by 64. Four independent
+// index loads (each a coalesced 256-byte segment) and four gathers are in flight per lane per
+// step, so a column's walk is not a chain of dependent round trips.
+__device__ __forceinline__ f128 csc_column_sum(const uint32_t* __restrict__ rows, uint64_t k0,
+                                               uint64_t k1, int lane, uint64_t n_cols,
+                                               const f128* __restrict__ eq, bool& bad) {
+    f128 s0 = f128_zero(), s1 = f128_zero();
+    uint64_t k = k0 + lane;
+    for (; k + 192 < k1; k += 256) {
+        const uint32_t r0 = rows[k], r1 = rows[k + 64], r2 = rows[k + 128], r3 = rows[k + 192];
+        const bool ok = r0 < n_cols && r1 < n_cols && r2 < n_cols && r3 < n_cols;
+        if (ok) {
+            s0 = f128_add3(s0, eq[r0], eq[r1]);
+            s1 = f128_add3(s1, eq[r2], eq[r3]);
+        } else {
+            bad = true;
+        }
+    }
+    for (; k < k1; k += 64) {
+        const uint32_t r = rows[k];
+        if (r < n_cols) s0 = f128_add(s0, eq[r]);
+        else bad = true;
+    }
+    return f128_add(s0, s1);
+}
+
 __global__ __launch_bounds__(256) void k_lc_csc_fold(
     uint64_t n_cols, const uint32_t* __restrict__ a_ptr, const uint32_t* __restrict__ a_rows,
     uint64_t nnz_a, const uint32_t* __restrict__ b_ptr, const uint32_t* __restrict__ b_rows,
@@ -42,22 +68,14 @@ __global__ __launch_bounds__(256) void k_lc_csc_fold(
             k1 = nnz_a;
             bad = true;
         }
-        for (uint64_t k = k0 + lane; k < k1; k += 64) {
-            const uint32_t r = a_rows[k];
-            if (r < n_cols) sa = f128_add(sa, eq[r]);
-            else bad = true;
-        }
+        sa = csc_column_sum(a_rows, k0, k1, lane, n_cols, eq, bad);
         k0 = b_ptr[c];
         k1 = b_ptr[c + 1];
         if (k1 > nnz_b) {
             k1 = nnz_b;
             bad = true;
         }
-        for (uint64_t k = k0 + lane; k < k1; k += 64) {
-            const uint32_t r = b_rows[k];
-            if (r < n_cols) sb = f128_add(sb, eq[r]);
-            else bad = true;
-        }
+        sb = csc_column_sum(b_rows, k0, k1, lane, n_cols, eq, bad);
         sa = wave_xor_f128(sa);
         sb = wave_xor_f128(sb);
         const bool any_bad = __any(bad);
@@ -155,7 +173,8 @@ __global__ __launch_bounds__(256) void k_fill_f128_zero(f128* __restrict__ out, 
 // (e1, einf) = (sum c_hi z_hi, sum (c_hi + c_lo)(z_hi + z_lo)) over i < half.
 __global__ __launch_bounds__(256) void k_lc_eval(const f128* __restrict__ c,
                                                   const f128* __restrict__ z, uint64_t half,
-                                                  f128* __restrict__ partials) {
+                                                  f128* __restrict__ partials,
+        f128* __restrict__ out) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u e1, ei;
@@ -167,14 +186,15 @@ __global__ __launch_bounds__(256) void k_lc_eval(const f128* __restrict__ c,
         comb_apply(e1, wl, comb_build(wl, zh), ch);
         comb_apply(ei, wl, comb_build(wl, f128_add(zh, zl)), f128_add(ch, cl));
     }
-    block_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials);
+    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, out);
 }
 
 // Bind the top bit of c and z at r in place (len -> len/2) and evaluate the next round on the
 // bound tables. Thread i reads quarters i, q+i, 2q+i, 3q+i and writes i and q+i only.
 __global__ __launch_bounds__(256) void k_lc_bind_eval(f128* __restrict__ c, f128* __restrict__ z,
                                                        uint64_t q, f128 r,
-                                                       f128* __restrict__ partials) {
+                                                       f128* __restrict__ partials,
+        f128* __restrict__ out) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
@@ -198,7 +218,7 @@ __global__ __launch_bounds__(256) void k_lc_bind_eval(f128* __restrict__ c, f128
         comb_apply(e1, wl, comb_build(wl, zhi), hi);
         comb_apply(ei, wl, comb_build(wl, f128_add(zhi, zlo)), f128_add(hi, lo));
     }
-    block_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials);
+    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, out);
 }
 
 // Final binding: v[i] += r (v[half + i] + v[i]) for c and z, in place.
@@ -218,7 +238,8 @@ __global__ __launch_bounds__(256) void k_lc_bind(f128* __restrict__ c, f128* __r
 // (sum w a, sum w b)
 __global__ __launch_bounds__(256) void k_dot2(const f128* __restrict__ w, const f128* __restrict__ a,
                                                const f128* __restrict__ b, uint64_t n,
-                                               f128* __restrict__ partials) {
+                                               f128* __restrict__ partials,
+        f128* __restrict__ out) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u sa, sb;
@@ -230,7 +251,7 @@ __global__ __launch_bounds__(256) void k_dot2(const f128* __restrict__ w, const 
         comb_apply(sa, wl, wh, a[i]);
         comb_apply(sb, wl, wh, b[i]);
     }
-    block_xor_pair(f256u_reduce(sa), f256u_reduce(sb), partials);
+    grid_xor_pair(f256u_reduce(sa), f256u_reduce(sb), partials, out);
 }
 
 // ---- Keccak-f[1600] circuit marginals (KeccakLincheckCircuit, r1cs_hashes/keccak.rs:692-835) ----
@@ -421,20 +442,16 @@ hipError_t launch_lc_eval(const uint4* c, const uint4* z, uint64_t len, uint4* p
                           uint4* out, hipStream_t s) {
     if (hipError_t ae = lc_attrs(); ae != hipSuccess) return ae;
     const unsigned g = grid_for(len / 2, kMaxReduceBlocks);
-    k_lc_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 2, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_lc_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 2, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_lc_bind_eval(uint4* c, uint4* z, uint64_t len, uint4 r, uint4* partials,
                                uint4* out, hipStream_t s) {
     if (hipError_t ae = lc_attrs(); ae != hipSuccess) return ae;
     const unsigned g = grid_for(len / 4, kMaxReduceBlocks);
-    k_lc_bind_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 4, r, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_lc_bind_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 4, r, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s) {
@@ -446,10 +463,8 @@ hipError_t launch_dot2(const uint4* w, const uint4* a, const uint4* b, uint64_t 
                        uint4* partials, uint4* out, hipStream_t s) {
     if (hipError_t ae = lc_attrs(); ae != hipSuccess) return ae;
     const unsigned g = grid_for(n, kMaxReduceBlocks);
-    k_dot2<<<g, 256, kCombWsBytes, s>>>(w, a, b, n, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_dot2<<<g, 256, kCombWsBytes, s>>>(w, a, b, n, partials, out);
+    return hipGetLastError();
 }
 
 }  // namespace flock

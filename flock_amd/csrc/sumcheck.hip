@@ -22,9 +22,12 @@
 
 namespace flock {
 
+static_assert(kMaxReduceBlocks == kGridReduceMax, "grid_xor_pair's counter sits after the partials");
+
 __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f,
                                                        const f128* __restrict__ b, uint64_t n,
-                                                       f128* __restrict__ partials) {
+                                                       f128* __restrict__ partials,
+        f128* __restrict__ out) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u u0, u2;
@@ -36,14 +39,15 @@ __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f
         comb_apply(u0, wl, comb_build(wl, b0), f0);
         comb_apply(u2, wl, comb_build(wl, f128_add(b0, b1)), f128_add(f0, f1));
     }
-    block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
+    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, out);
 }
 
 __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
                                                       const f128* __restrict__ b, uint64_t n,
                                                       f128 r, f128* __restrict__ nf,
                                                       f128* __restrict__ nb,
-                                                      f128* __restrict__ partials) {
+                                                      f128* __restrict__ partials,
+        f128* __restrict__ out) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
@@ -75,14 +79,15 @@ __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
             comb_apply(u2, wl, comb_build(wl, f128_add(c0, c1)), f128_add(g0, g1));
         }
     }
-    block_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials);
+    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, out);
 }
 
 __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
                                                    const f128* __restrict__ b, uint64_t n,
                                                    const f128* __restrict__ e,
                                                    f128* __restrict__ e_next,
-                                                   f128* __restrict__ partials) {
+                                                   f128* __restrict__ partials,
+        f128* __restrict__ out) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u g1, gi;
@@ -100,7 +105,7 @@ __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
         comb_apply(gi, wl, eh, pi);
         if (e_next && !(x & 1) && x + 1 < half) e_next[x >> 1] = f128_add(ex, e[x + 1]);
     }
-    block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
+    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, out);
 }
 
 __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ a,
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ 
                                                         f128* __restrict__ bo,
                                                         const f128* __restrict__ e,
                                                         f128* __restrict__ e_next,
-                                                        f128* __restrict__ partials) {
+                                                        f128* __restrict__ partials,
+        f128* __restrict__ out) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
@@ -139,7 +145,7 @@ __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ 
         comb_apply(gi, wl, eh, mi);
         if (e_next && !(y & 1) && y + 1 < quarter) e_next[y >> 1] = f128_add(ey, e[y + 1]);
     }
-    block_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials);
+    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, out);
 }
 
 __global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
@@ -162,16 +168,23 @@ __global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
 // products.
 constexpr int kEqDirectMax = 20;
 
-__global__ __launch_bounds__(256) void k_eq_direct(const f128* __restrict__ r, int nv,
-                                                    f128* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_eq_direct(const f128* __restrict__ r, int nlo, int nhi,
+                                                    uint32_t lo_blocks, f128* __restrict__ lo,
+                                                    f128* __restrict__ hi) {
+    // workgroups [0, lo_blocks) build eq(r[0..nlo)) into lo, the rest eq(r[nlo..nlo+nhi)) into hi
+    const bool is_lo = blockIdx.x < lo_blocks;
+    const int nv = is_lo ? nlo : nhi;
+    const f128* rv = is_lo ? r : r + nlo;
+    f128* out = is_lo ? lo : hi;
     __shared__ f128 tabs[kEqDirectMax][16];
     __shared__ uint32_t his[kEqDirectMax];
     for (int i = threadIdx.x; i < nv; i += blockDim.x) {
         tabs[i][0] = f128_zero();
-        his[i] = comb_build<1>(tabs[i], r[i]);
+        his[i] = comb_build<1>(tabs[i], rv[i]);
     }
     __syncthreads();
-    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t x = (uint64_t)(is_lo ? blockIdx.x : blockIdx.x - lo_blocks) * blockDim.x +
+                       threadIdx.x;
     if (x >= (1ull << nv)) return;
     f128 acc = f128_one();
     for (int i = 0; i < nv; i++) {
@@ -223,30 +236,24 @@ hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4
                                uint4* out, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_sc_round_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_sc_round_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
                               uint4* nb, uint4* partials, uint4* out, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_sc_fold_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, r, nf, nb, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_sc_fold_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, r, nf, nb, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
                            uint4* e_next, uint4* partials, uint4* out, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_zc_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, e, e_next, partials);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_zc_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, e, e_next, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
@@ -254,10 +261,8 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
                                 uint4* partials, uint4* out, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, ao, bo, e, e_next, partials);
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    return launch_xor_reduce(partials, (int)g, out, s);
+    k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, ao, bo, e, e_next, partials, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
@@ -281,8 +286,9 @@ hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, h
     if (nhi > kEqDirectMax) return hipErrorInvalidValue;
     uint4* lo = scratch;
     uint4* hi = scratch + (1ull << nlo);
-    k_eq_direct<<<(unsigned)(((1ull << nlo) + 255) / 256), 256, 0, s>>>(r, nlo, lo);
-    k_eq_direct<<<(unsigned)(((1ull << nhi) + 255) / 256), 256, 0, s>>>(r + nlo, nhi, hi);
+    const uint32_t lo_blocks = (uint32_t)(((1ull << nlo) + 255) / 256);
+    const uint32_t hi_blocks = (uint32_t)(((1ull << nhi) + 255) / 256);
+    k_eq_direct<<<lo_blocks + hi_blocks, 256, 0, s>>>(r, nlo, nhi, lo_blocks, lo, hi);
     k_eq_outer<<<(unsigned)(1ull << nhi), 256, 0, s>>>(lo, nlo, hi, out);
     return hipGetLastError();
 }

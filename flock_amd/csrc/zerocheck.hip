@@ -458,33 +458,41 @@ __global__ __launch_bounds__(256) void k_zc_round1_groups(
 }
 
 // red[j] for the 1024 columns of the block partials (column reduce, see k_col_xor_reduce).
-// p_ab[i] = sum_b phi8(2^b) (red[64 b + i] + red[512 + 64 b + i]).
-__global__ __launch_bounds__(64) void k_zc_round1_groups_finish(const f128* __restrict__ red,
+// p_ab[i] = sum_b phi8(2^b) (red[64 b + i] + red[512 + 64 b + i]): thread (b, i) forms one
+// product, then 64 threads sum the eight.
+__global__ __launch_bounds__(512) void k_zc_round1_groups_finish(const f128* __restrict__ red,
                                                                  const f128* __restrict__ phi_pow,
                                                                  f128* __restrict__ out) {
-    const int i = threadIdx.x;
-    f128 r = f128_zero();
-#pragma unroll 1
-    for (int bb = 0; bb < 8; bb++)
-        r = f128_add(r, f128_mul(phi_pow[bb], f128_add(red[64 * bb + i], red[512 + 64 * bb + i])));
-    out[i] = r;
+    __shared__ f128 part[8][64];
+    const int bb = threadIdx.x >> 6, i = threadIdx.x & 63;
+    part[bb][i] = f128_mul(phi_pow[bb], f128_add(red[64 * bb + i], red[512 + 64 * bb + i]));
+    __syncthreads();
+    if (bb == 0) {
+        f128 r = part[0][i];
+#pragma unroll
+        for (int k = 1; k < 8; k++) r = f128_add(r, part[k][i]);
+        out[i] = r;
+    }
 }
 
 // p_c[i] = sum_s phi8(M[s][i]) S_c[s] with S_c[s] = sum_x eq(r[6..m), x) c_s(x)
 //        = (1 + r6) shat[s] + r6 shat[64 + s], shat = fold_1b_rows(c, eq(r[7..m))):
 // row y of c as 128 bits holds rows x = 2y (bits 0..63) and 2y + 1 (bits 64..127), and
-// eq(r[6..m), 2y + b) = (b ? r6 : 1 + r6) eq(r[7..m), y). One block per i, one lane per s.
-__global__ __launch_bounds__(64) void k_zc_round1_c(const f128* __restrict__ shat,
-                                                     const f128* __restrict__ phim, f128 r6,
-                                                     f128* __restrict__ out) {
+// eq(r[6..m), 2y + b) = (b ? r6 : 1 + r6) eq(r[7..m), y). One block per i; 128 threads form the
+// two halves of S_c, then 64 threads one product each.
+__global__ __launch_bounds__(128) void k_zc_round1_c(const f128* __restrict__ shat,
+                                                      const f128* __restrict__ phim, f128 r6,
+                                                      f128* __restrict__ out) {
+    __shared__ f128 half[128];
     __shared__ f128 part[64];
-    const int i = blockIdx.x, s = threadIdx.x;
-    f128 one_r6 = r6;
-    one_r6.x ^= 1u;
-    const f128 sc = f128_add(f128_mul(one_r6, shat[s]), f128_mul(r6, shat[64 + s]));
-    part[s] = f128_mul(phim[s * 64 + i], sc);
+    const int i = blockIdx.x, t = threadIdx.x;
+    f128 w = r6;
+    if (t < 64) w.x ^= 1u;  // 1 + r6 for shat[s], r6 for shat[64 + s]
+    half[t] = f128_mul(w, shat[t]);
     __syncthreads();
-    if (s == 0) {
+    if (t < 64) part[t] = f128_mul(phim[t * 64 + i], f128_add(half[t], half[64 + t]));
+    __syncthreads();
+    if (t == 0) {
         f128 r = f128_zero();
         for (int k = 0; k < 64; k++) r = f128_add(r, part[k]);
         out[i] = r;
@@ -592,13 +600,13 @@ hipError_t launch_zc_round1_groups(const uint8_t* a, const uint8_t* b, const uin
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = launch_col_reduce(partials, blocks, red, s);
     if (e != hipSuccess) return e;
-    k_zc_round1_groups_finish<<<1, 64, 0, s>>>(red, phi_pow, out);
+    k_zc_round1_groups_finish<<<1, 512, 0, s>>>(red, phi_pow, out);
     return hipGetLastError();
 }
 
 hipError_t launch_zc_round1_c(const uint4* shat, const uint4* phim, uint4 r6, uint4* out,
                               hipStream_t s) {
-    k_zc_round1_c<<<64, 64, 0, s>>>(shat, phim, r6, out);
+    k_zc_round1_c<<<64, 128, 0, s>>>(shat, phim, r6, out);
     return hipGetLastError();
 }
 
