@@ -404,8 +404,23 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
 struct flock_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream for phases independent of the main chain, joined through events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // aux waits for everything enqueued on stream so far
+    int fork() {
+        HIP_TRY(hipEventRecord(ev_fork, stream));
+        HIP_TRY(hipStreamWaitEvent(aux, ev_fork, 0));
+        return 0;
+    }
+    // stream waits for everything enqueued on aux so far
+    int join() {
+        HIP_TRY(hipEventRecord(ev_join, aux));
+        HIP_TRY(hipStreamWaitEvent(stream, ev_join, 0));
+        return 0;
+    }
     uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words, then the grid-reduce ticket counter
-    uint4* dout = nullptr;      // 2 words
+    uint4* dout = nullptr;      // device view of hout: round kernels write their message there
     flock_f128* hout = nullptr; // pinned, 2 words
     std::map<int, uint4*> evals;  // k -> k x k basis table
     void* scratch = nullptr;
@@ -417,7 +432,9 @@ struct flock_hip_ctx {
     uint4* zc_ext_bs = nullptr;    // bit-sliced extension table (same shape as zc_ext)
     uint4* zc_phim = nullptr;      // phi8(L_s(64 + i)) at s * 64 + i
     uint4* zc_partials = nullptr;  // kZcMaxBlocks groups x 1024 accumulators
-    uint4* zc_out = nullptr;       // 2048 words: message [0,128), shat [128,256), red [1024,2048)
+    uint4* zc_partials2 = nullptr; // the same for a kernel on the aux stream
+    uint4* zc_out = nullptr;       // 3072 words: message [0,128), shat [128,256), red [1024,2048),
+                                   // red2 [2048,3072)
     flock_f128* zc_host = nullptr;  // pinned, 256 words: message and shat
     // Keccak-f[1600] lincheck walker tables (device): phi^T gathers, chi row offsets, RC_r
     uint16_t* kc_inv = nullptr;
@@ -426,6 +443,7 @@ struct flock_hip_ctx {
     int ensure_kc();
     uint4* zc_shat() { return zc_out + 128; }
     uint4* zc_red() { return zc_out + 1024; }
+    uint4* zc_red2() { return zc_out + 2048; }
 
     int ensure_zc() {
         if (zc_ext) return 0;
@@ -481,7 +499,8 @@ struct flock_hip_ctx {
             hipMalloc(&zc_logexp, le.size()) != hipSuccess ||
             hipMalloc(&zc_phipow, sizeof(pp)) != hipSuccess ||
             hipMalloc(&zc_partials, sizeof(uint4) * flock::kZcMaxBlocks * 1024) != hipSuccess ||
-            hipMalloc(&zc_out, sizeof(uint4) * 2048) != hipSuccess ||
+            hipMalloc(&zc_partials2, sizeof(uint4) * flock::kZcMaxBlocks * 1024) != hipSuccess ||
+            hipMalloc(&zc_out, sizeof(uint4) * 3072) != hipSuccess ||
             hipHostMalloc(&zc_host, sizeof(flock_f128) * 256) != hipSuccess) {
             (void)hipGetLastError();
             return fail(FLOCK_HIP_E_NOMEM, "zerocheck tables");
@@ -525,8 +544,9 @@ struct flock_hip_ctx {
         return 0;
     }
     // launch result pair -> host
+    // The message kernel wrote its pair through dout straight into the coherent pinned hout,
+    // so a round costs one launch and one wait, with no copy.
     int fetch_pair(flock_f128 out[2]) {
-        HIP_TRY(hipMemcpyAsync(hout, dout, 2 * sizeof(flock_f128), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         out[0] = hout[0];
         out[1] = hout[1];
@@ -578,10 +598,13 @@ int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
     if (!c) return fail(FLOCK_HIP_E_NOMEM, "ctx");
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->partials, sizeof(uint4) * (flock::kMaxReduceBlocks * 2 + 1)) != hipSuccess ||
-        hipMalloc(&c->dout, sizeof(uint4) * 2) != hipSuccess ||
-        hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 2, hipHostMallocDefault) !=
-            hipSuccess) {
+        hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 2,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->dout, c->hout, 0) != hipSuccess) {
         hipError_t e = hipGetLastError();
         flock_hip_ctx_destroy(c);
         return fail(FLOCK_HIP_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
@@ -607,6 +630,7 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     if (c->zc_logexp) hipFree(c->zc_logexp);
     if (c->zc_phipow) hipFree(c->zc_phipow);
     if (c->zc_partials) hipFree(c->zc_partials);
+    if (c->zc_partials2) hipFree(c->zc_partials2);
     if (c->zc_out) hipFree(c->zc_out);
     if (c->zc_host) hipHostFree(c->zc_host);
     if (c->kc_inv) hipFree(c->kc_inv);
@@ -614,8 +638,11 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     if (c->kc_rcm) hipFree(c->kc_rcm);
     if (c->scratch) hipFree(c->scratch);
     if (c->partials) hipFree(c->partials);
-    if (c->dout) hipFree(c->dout);
     if (c->hout) hipHostFree(c->hout);
+    if (c->aux) hipStreamSynchronize(c->aux);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->aux) hipStreamDestroy(c->aux);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1029,12 +1056,14 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     // p_ab over groups of 8 rows (r[6..9) are phi8 constants); shat[lane | b7 << 6] =
     // sum_y eq(r[7..m), y) c_bit(y*128 + b7*64 + lane) (univariate_skip.rs:354-383), from which
     // p_c follows.
+    // shat and p_c on the aux stream, beside p_ab on the main stream
+    if (int rc = c->fork()) return rc;
+    LAUNCH(flock::launch_fold_1b_rows(reinterpret_cast<const uint4*>(d_c), EB, h,
+                                      c->zc_partials2, c->zc_red2(), c->zc_shat(), c->aux));
+    LAUNCH(flock::launch_zc_round1_c(c->zc_shat(), c->zc_phim, U(r[ks]), c->zc_out + 64, c->aux));
     LAUNCH(flock::launch_zc_round1_groups(d_a, d_b, E9, N / 8, c->zc_ext_bs, c->zc_phipow,
                                           c->zc_partials, c->zc_red(), c->zc_out, c->stream));
-    LAUNCH(flock::launch_fold_1b_rows(reinterpret_cast<const uint4*>(d_c), EB, h, c->zc_partials,
-                                      c->zc_red(), c->zc_shat(), c->stream));
-    LAUNCH(flock::launch_zc_round1_c(c->zc_shat(), c->zc_phim, U(r[ks]), c->zc_out + 64,
-                                     c->stream));
+    if (int rc = c->join()) return rc;
     HIP_TRY(hipMemcpyAsync(c->zc_host, c->zc_out, 256 * 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::memcpy(h_round1, c->zc_host, 128 * 16);
@@ -1476,7 +1505,8 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     const uint32_t vmax = nu > L ? nu : L;
     const uint64_t eqs_words = 2ull << ((vmax + 1) / 2);
     const uint64_t pf_words = lc_fold_partial_words(K);
-    const uint64_t words = 5 * K + n_outer + (1ull << L) + eqs_words + pf_words + 64 + L + nu + 1;
+    const uint64_t words =
+        5 * K + n_outer + (1ull << L) + 2 * eqs_words + pf_words + 64 + L + nu + 1;
     if (int rc = c->ensure_scratch(words * 16)) return rc;
     uint4* EQI = reinterpret_cast<uint4*>(c->scratch);
     uint4* SA = EQI + K;
@@ -1486,7 +1516,8 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     uint4* EO = ZV + K;
     uint4* ER = EO + n_outer;
     uint4* EQS = ER + (1ull << L);
-    uint4* PF = EQS + eqs_words;
+    uint4* EQS2 = EQS + eqs_words;  // build_eq scratch of the aux stream
+    uint4* PF = EQS2 + eqs_words;
     uint4* RV = PF + pf_words;
     int* ERR = reinterpret_cast<int*>(RV + 64 + L + nu);
 
@@ -1506,9 +1537,19 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
         for (uint32_t j = 0; j < nu; j++) rv[64 + L + j] = h_x_outer[j];
     }
     HIP_TRY(hipMemcpyAsync(RV, rv.data(), rv.size() * 16, hipMemcpyHostToDevice, c->stream));
+    // 3. (on the aux stream, beside the comb) partial fold of the stripe at x_outer
+    // (union.rs:354-384); 3b. capture z_vec
+    if (int rc = c->fork()) return rc;
+    LAUNCH(flock::launch_build_eq(RV + 64 + L, (int)nu, EO, EQS2, c->aux));
+    LAUNCH(flock::launch_lc_partial_fold(d_z, K, useful_bits, (n_rows + 7) / 8, EO, PF, ZV,
+                                         c->aux));
+    if (d_z_vec) HIP_TRY(hipMemcpyAsync(d_z_vec, ZV, K * 16, hipMemcpyDeviceToDevice, c->aux));
     LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
     LAUNCH(flock::launch_outer_product(RV, 6, ER, (int)L, EQI, c->stream));
-    if (int rc = fold(EQI, U(alpha), SA, SB, COMB, ERR)) return rc;
+    if (int rc = fold(EQI, U(alpha), SA, SB, COMB, ERR)) {
+        hipStreamSynchronize(c->aux);
+        return rc;
+    }
     // 2b. constant-wire pin (union.rs:318-335)
     if (const_pin >= 0) {
         if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
@@ -1516,12 +1557,7 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
         flock_fs_sample_f128(fs, &beta);
         LAUNCH(flock::launch_lc_add_at(COMB, (uint64_t)const_pin, U(beta), c->stream));
     }
-    // 3. partial fold of the stripe at x_outer (union.rs:354-384); 3b. capture z_vec
-    LAUNCH(flock::launch_build_eq(RV + 64 + L, (int)nu, EO, EQS, c->stream));
-    LAUNCH(flock::launch_lc_partial_fold(d_z, K, useful_bits, (n_rows + 7) / 8, EO, PF, ZV,
-                                         c->stream));
-    if (d_z_vec)
-        HIP_TRY(hipMemcpyAsync(d_z_vec, ZV, K * 16, hipMemcpyDeviceToDevice, c->stream));
+    if (int rc = c->join()) return rc;
     // 4-5. product sumcheck over the top L bits (column_sumcheck_prove, lincheck.rs:1907-1980)
     std::vector<flock_f128> rr(L ? L : 1);
     if (L) {
