@@ -306,7 +306,7 @@ __device__ __forceinline__ f128* comb_lane(f128* ws) {
 // partials[2*kGridReduceMax] (zero between launches); the workgroup that draws the last
 // ticket folds every pair (volatile loads, past any stale L1 line), writes out and resets
 // the counter. gridDim.x <= kGridReduceMax.
-constexpr int kGridReduceMax = 2048;
+constexpr int kGridReduceMax = 512;
 __device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials, f128* out) {
     __shared__ f128 red[2][16];
     __shared__ int last;

@@ -9,7 +9,7 @@ namespace flock {
 
 // Two F128 per block partial (e.g. u0, u2) — 32 bytes.
 constexpr int kPartialWords = 2;
-constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxReduceBlocks = 512;  // two 256-thread groups per CU: the whole grid is resident
 
 // ---- field.hip ----
 hipError_t launch_f128_mul_batch(const uint4* a, const uint4* b, uint4* out, size_t n,
