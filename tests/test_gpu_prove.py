@@ -201,3 +201,23 @@ def test_footprint_grows_with_the_batch(circuits):
     small, k1 = F.make_requests([spec(F.HASH_SHA256, 4, 16, 1, circuits)])
     big, k2 = F.make_requests([spec(F.HASH_SHA256, 8, 256, 2, circuits)])
     assert F.prove_footprint(big[0]) > F.prove_footprint(small[0]) > 0
+
+
+def test_prove_core_at_config2_size_matches_golden_vectors(circuits):
+    # 1024 instances of each hash (config 2's batch); the oracle's roots and transcript
+    # digests were written by tests/golden/make_prove_core_vectors.py from the same seeds
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "prove_core_vectors.json")
+    with open(path) as fh:
+        gold = json.load(fh)
+    assert gold["domain"].encode() == DOMAIN and bytes.fromhex(gold["digest"]) == DIGEST
+    pr = F.Prover(0)
+    for case in gold["cases"]:
+        say(f"case: golden prove core kind={case['kind']} n_log={case['n_log']}")
+        sp = spec(case["kind"], case["n_log"], case["n_decl"], case["seed"], circuits)
+        reqs, keep = F.make_requests([sp], digest=DIGEST, domain=DOMAIN)
+        res = F.prove_core(pr, reqs[0])
+        assert bytes(res.root).hex() == case["root"]
+        assert bytes(res.transcript).hex() == case["transcript"]
+    pr.close()

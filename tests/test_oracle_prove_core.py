@@ -1,0 +1,22 @@
+"""The committed prove-core golden vectors regenerate from their seeds: the BLAKE3 case (the
+cheapest) is recomputed with the oracle and compared with tests/golden/prove_core_vectors.json,
+so the GPU test's fixture cannot drift from the oracle or from the seeded inputs."""
+from __future__ import annotations
+
+import json
+import os
+
+from flock_amd import lib as F
+
+
+def test_blake3_golden_case_regenerates():
+    import test_gpu_prove as T
+    import test_oracle_blake3_r1cs as B3
+    with open(os.path.join(os.path.dirname(__file__), "golden", "prove_core_vectors.json")) as fh:
+        gold = json.load(fh)
+    case = next(c for c in gold["cases"] if c["kind"] == F.HASH_BLAKE3)
+    _, _, a, b = B3.matrices()
+    root, dig = T.oracle_prove_core(case["kind"], case["n_log"],
+                                    T.inputs_for(case["kind"], case["n_decl"], case["seed"]),
+                                    {F.HASH_BLAKE3: (None, a, b)})
+    assert root.hex() == case["root"] and dig.hex() == case["transcript"]
