@@ -188,6 +188,38 @@ def cpu_baseline(t: int, log_dim: int, reps: int) -> dict:
                       f"1 warm-up, {best * 1e3:.0f} ms per transform"}
 
 
+def config1_commit(F, ctx, ev, stream, with_cpu: bool) -> dict:
+    """The reference's CPU-runnable case: a 2^10 x 2^8 message of SplitMix64 words (seed
+    0xC0FFEE, interleaved msg[p * 256 + lane]), rate 1/4, committed with the SHA-256 Merkle
+    tree (m = 25, log_batch_size 8). GPU time of the whole commit with the input resident,
+    beside the oracle's commit of the same input on one host core."""
+    p = F.PcsParams(25, 2, 8, 0, 0)
+    s = F.pcs_sizes(p)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the checker doubles as the CPU baseline
+    msg = O.splitmix(0xC0FFEE, 1 << 18)
+    d_msg, cw = ctx.upload(msg), ctx.alloc(16 * s["cw_len"])
+    tree = ctx.alloc(32 * (2 * s["n_leaves"] - 1))
+    e = [ev.new(), ev.new()]
+    ctx.commit(d_msg, 1 << 18, p, cw, tree)  # warm-up
+    ts = []
+    for _ in range(10):
+        ev.record(e[0], stream)
+        root = ctx.commit(d_msg, 1 << 18, p, cw, tree)
+        ev.record(e[1], stream)
+        ts.append(ev.ms(e[0], e[1]))
+    out = {"gpu_ms": round(float(np.median(ts)), 4), "root": root[0].hex()}
+    if with_cpu:
+        t0 = time.perf_counter()
+        _, etree = O.commit(msg, 25, 2, 8, 0)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        out["cpu_cores"] = 1
+        assert etree[-32:] == root[0], "config 1 root differs from the oracle"
+    for b in (d_msg, cw, tree):
+        b.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,6 +459,9 @@ def main():
             st.append(time.perf_counter() - t1)
         result["sumcheck_2^24_ms"] = round(min(st) * 1e3, 3)
         result["sumcheck_2^24_algorithmic_gbs"] = round((48 + 56) * 2 * nn / min(st) / 1e9, 3)
+
+    if not args.ntt_only:
+        result["config1_commit"] = config1_commit(F, ctx, ev, stream, not args.no_cpu_baseline)
 
     if not args.no_mixed and not args.ntt_only:
         result.update(mixed_workloads(F, ctx, args))
