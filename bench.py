@@ -339,7 +339,7 @@ def main():
     step_ms = wall * 1e3 / args.steps
 
     # Stage split of the commit, and the dominant kernel alone.
-    enc, mt, pt = [], [], []
+    enc, mt, mb, pt = [], [], [], []
     for _ in range(args.steps):
         ev.record(e[0], stream)
         ctx.encode_lane_major(q, sz["committed_words"], p, cw)
@@ -349,11 +349,15 @@ def main():
         ctx.merkle_tree(cw, 16 * s["cw_len"], s["n_leaves"], tree)
         ev.record(e[1], stream)
         mt.append(ev.ms(e[0], e[1]))
+        ev.record(e[0], stream)  # the same leaves under HashKind::Blake3
+        ctx.merkle_tree(cw, 16 * s["cw_len"], s["n_leaves"], tree, kind=F.MERKLE_BLAKE3)
+        ev.record(e[1], stream)
+        mb.append(ev.ms(e[0], e[1]))
         ev.record(e[0], stream)
         ctx.ntt_pass(cw, t, t, k, 4, 3)  # layers 4..6, an interior 3-layer pass
         ev.record(e[1], stream)
         pt.append(ev.ms(e[0], e[1]))
-    ntt_ms, merkle_ms, pass_ms = (float(np.mean(x)) for x in (enc, mt, pt))
+    ntt_ms, merkle_ms, merkle_b3_ms, pass_ms = (float(np.mean(x)) for x in (enc, mt, mb, pt))
     ntt_bytes = 16 * ((t << (k - 1)) + s["cw_len"])
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     pass_bytes = 32 * s["cw_len"]  # an in-place pass reads and writes the codeword once
@@ -399,6 +403,8 @@ def main():
         "ntt_compulsory_frac_of_hbm": round(ntt_gbs / HBM_PEAK_GBS, 5),
         "ntt_f128_muls_per_s": round(muls / (ntt_ms * 1e-3), 1),
         "merkle_leaf_gbs": round(16 * s["cw_len"] / (merkle_ms * 1e-3) / 1e9, 3),
+        "merkle_blake3_ms": round(merkle_b3_ms, 4),
+        "merkle_blake3_leaf_gbs": round(16 * s["cw_len"] / (merkle_b3_ms * 1e-3) / 1e9, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(pass_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(pass_gbs / HBM_PEAK_GBS, 5), "traffic": traffic,

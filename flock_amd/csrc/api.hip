@@ -4,6 +4,7 @@
 // transcript, and the protocol drivers that sequence kernels between challenges.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdio>
 #include <cstdlib>
@@ -22,6 +23,7 @@
 #endif
 
 #include "../../include/flock_hip.h"
+#include "blake3_common.h"
 #include "flock_internal.h"
 
 namespace {
@@ -744,7 +746,8 @@ int flock_hip_pcs_sizes(const flock_pcs_params* p, size_t* msg_len, size_t* cw_l
     if (p->m < 7 + p->log_batch_size)
         return fail(FLOCK_HIP_E_M_TOO_SMALL, "m too small (need m >= 7 + log_batch_size)");
     if (p->log_inv_rate < 1) return fail(FLOCK_HIP_E_INVALID, "log_inv_rate must be >= 1");
-    if (p->merkle_hash != 0) return fail(FLOCK_HIP_E_INVALID, "only SHA-256 Merkle trees");
+    if (p->merkle_hash != FLOCK_MERKLE_SHA256 && p->merkle_hash != FLOCK_MERKLE_BLAKE3)
+        return fail(FLOCK_HIP_E_INVALID, "merkle_hash must be SHA-256 (0) or BLAKE3 (1)");
     const int log_dim = (int)p->m - 7 - (int)p->log_batch_size;
     const int k = log_dim + (int)p->log_inv_rate;
     if (k > 40 || p->log_batch_size > 20) return fail(FLOCK_HIP_E_INVALID, "size out of range");
@@ -759,13 +762,25 @@ int flock_hip_pcs_sizes(const flock_pcs_params* p, size_t* msg_len, size_t* cw_l
     return 0;
 }
 
+int flock_hip_merkle_tree_kind(flock_hip_ctx* c, const uint8_t* data, size_t len,
+                               size_t n_leaves, uint32_t kind, uint8_t* tree) {
+    CTX_CHECK(c);
+    if (!data || !tree || !is_pow2(n_leaves) || len % n_leaves || len == 0 ||
+        (len / n_leaves) % 16)
+        return fail(FLOCK_HIP_E_INVALID, "bad Merkle shape");
+    if (kind != FLOCK_MERKLE_SHA256 && kind != FLOCK_MERKLE_BLAKE3)
+        return fail(FLOCK_HIP_E_INVALID, "unknown Merkle hash");
+    if (kind == FLOCK_MERKLE_BLAKE3 &&
+        (len / n_leaves + flock::kB3ChunkLen - 1) / flock::kB3ChunkLen >
+            (1ull << flock::kMerkleB3MaxDepth))
+        return fail(FLOCK_HIP_E_INVALID, "BLAKE3 leaf too long");
+    LAUNCH(flock::launch_merkle_tree(data, n_leaves, len / n_leaves, kind, tree, c->stream));
+    return 0;
+}
+
 int flock_hip_merkle_tree(flock_hip_ctx* c, const uint8_t* data, size_t len, size_t n_leaves,
                           uint8_t* tree) {
-    CTX_CHECK(c);
-    if (!data || !tree || !is_pow2(n_leaves) || len % n_leaves || (len / n_leaves) % 16)
-        return fail(FLOCK_HIP_E_INVALID, "bad Merkle shape");
-    LAUNCH(flock::launch_merkle_tree(data, n_leaves, len / n_leaves, tree, c->stream));
-    return 0;
+    return flock_hip_merkle_tree_kind(c, data, len, n_leaves, FLOCK_MERKLE_SHA256, tree);
 }
 
 static int encode_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len,
@@ -813,8 +828,11 @@ static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len
     if (cap_depth > k) return fail(FLOCK_HIP_E_INVALID, "cap depth exceeds tree depth");
     rc = encode_common(c, src, src_len, p, lane_major, cw);
     if (rc) return rc;
-    LAUNCH(flock::launch_merkle_tree(reinterpret_cast<const uint8_t*>(cw), n_leaves, t * 16, tree,
-                                     c->stream));
+    if (p->merkle_hash == FLOCK_MERKLE_BLAKE3 &&
+        (t * 16 + flock::kB3ChunkLen - 1) / flock::kB3ChunkLen > (1ull << flock::kMerkleB3MaxDepth))
+        return fail(FLOCK_HIP_E_INVALID, "BLAKE3 leaf too long");
+    LAUNCH(flock::launch_merkle_tree(reinterpret_cast<const uint8_t*>(cw), n_leaves, t * 16,
+                                     p->merkle_hash, tree, c->stream));
     if (h_cap) {
         const size_t l = (size_t)1 << cap_depth;  // cap_layer, merkle.rs:649-656
         HIP_TRY(hipMemcpyAsync(h_cap, tree + (2 * n_leaves - 2 * l) * 32, l * 32,
@@ -1734,6 +1752,79 @@ int flock_union_x_ab_from_mlv(const flock_registry* reg, const flock_f128* mlv, 
     x_inner_rest[0] = mlv[0];
     for (size_t j = 1 + reg->nu; j < n_mlv; j++) x_inner_rest[j - reg->nu] = mlv[j];
     for (uint32_t j = 0; j < reg->nu; j++) x_outer[j] = mlv[1 + j];
+    return 0;
+}
+
+void flock_blake3_hash(const void* data, size_t len, uint8_t out[32]) {
+    flock::Blake3Host h;
+    if (len) h.update(data, len);
+    h.finish(out);
+}
+
+// absorb_matrix, r1cs.rs:419-428: (num_rows, num_cols, then per row its length and columns),
+// every integer u64 little-endian
+static int absorb_matrix(flock::Blake3Host& h, const flock_sparse_matrix& m) {
+    if (m.identity) {
+        if (m.num_rows != m.num_cols) return fail(FLOCK_HIP_E_INVALID, "identity must be square");
+        h.u64(m.num_rows);
+        h.u64(m.num_cols);
+        for (uint64_t r = 0; r < m.num_rows; r++) {
+            h.u64(1);
+            h.u64(r);
+        }
+        return 0;
+    }
+    if (m.num_rows && (!m.row_ptr || (m.row_ptr[m.num_rows] > m.row_ptr[0] && !m.cols)))
+        return fail(FLOCK_HIP_E_INVALID, "null CSR arrays");
+    for (uint64_t r = 0; r < m.num_rows; r++)
+        if (m.row_ptr[r + 1] < m.row_ptr[r]) return fail(FLOCK_HIP_E_INVALID, "row_ptr decreases");
+    h.u64(m.num_rows);
+    h.u64(m.num_cols);
+    std::vector<uint8_t> row;
+    for (uint64_t r = 0; r < m.num_rows; r++) {
+        const uint64_t a = m.row_ptr[r], b = m.row_ptr[r + 1];
+        row.resize(8 * (b - a + 1));
+        const uint64_t len = b - a;
+        for (int k = 0; k < 8; k++) row[k] = (uint8_t)(len >> (8 * k));
+        for (uint64_t i = a; i < b; i++)
+            for (int k = 0; k < 8; k++) row[8 * (i - a + 1) + k] = (uint8_t)((uint64_t)m.cols[i] >> (8 * k));
+        h.update(row.data(), row.size());
+    }
+    return 0;
+}
+
+int flock_registry_digest(const flock_type_statement* types, size_t n, uint32_t nu,
+                          uint8_t out[32]) {
+    if (!types || !out || n == 0 || n > FLOCK_MAX_TYPES)
+        return fail(FLOCK_HIP_E_INVALID, "bad registry digest arguments");
+    for (size_t i = 0; i < n; i++) {
+        const flock_type_statement& t = types[i];
+        if (t.k_log < 7 || t.k_log > 40 || t.useful_bits > (1ull << t.k_log))
+            return fail(FLOCK_HIP_E_INVALID, "table type out of range (schedule.rs:337-355)");
+    }
+    // Registry::new's order: stable sort by k_log, descending (schedule.rs:386-389)
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return types[a].k_log > types[b].k_log; });
+    static const uint8_t kLabel[] = "flock-registry-v1";
+    flock::Blake3Host h;
+    h.update(kLabel, sizeof(kLabel) - 1);
+    const uint8_t version = 1;
+    h.update(&version, 1);
+    h.u32(nu);
+    h.u32((uint32_t)n);
+    for (size_t i : order) {
+        const flock_type_statement& t = types[i];
+        h.u32(t.k_log);
+        h.u64(t.useful_bits);
+        const uint8_t present = t.has_const_pin ? 1 : 0;
+        h.update(&present, 1);
+        h.u64(t.has_const_pin ? t.const_pin : 0);
+        for (const flock_sparse_matrix* m : {&t.a0, &t.b0, &t.c0})
+            if (int rc = absorb_matrix(h, *m)) return rc;
+    }
+    h.finish(out);
     return 0;
 }
 

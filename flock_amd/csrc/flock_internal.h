@@ -38,8 +38,12 @@ hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0
 
 // ---- merkle.hip ----
 // SHA-256 Merkle tree, flat layout (leaves first, root last). leaf_bytes % 16 == 0.
+// HashKind (hash.rs:23-27) as numbered in flock_pcs_params.merkle_hash
+constexpr uint32_t kMerkleSha256 = 0, kMerkleBlake3 = 1;
+// BLAKE3 leaves of up to 2^16 chunks (64 MiB): the per-thread subtree stack's depth
+constexpr int kMerkleB3MaxDepth = 16;
 hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
-                              uint8_t* tree, hipStream_t s);
+                              uint32_t kind, uint8_t* tree, hipStream_t s);
 
 // ---- sumcheck.hip ----
 // round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2 + 1 words (the last holds a zeroed ticket counter).

@@ -99,6 +99,23 @@ class PcsParams(ctypes.Structure):
 
 
 HASH_SHA256, HASH_BLAKE3, HASH_KECCAK = 0, 1, 2
+MERKLE_SHA256, MERKLE_BLAKE3 = 0, 1
+
+
+class SparseMatrix(ctypes.Structure):
+    """flock_sparse_matrix: a SparseBinaryMatrix (r1cs.rs:14-18) as CSR, or the identity."""
+
+    _fields_ = [("num_rows", ctypes.c_uint64), ("num_cols", ctypes.c_uint64),
+                ("row_ptr", ctypes.c_void_p), ("cols", ctypes.c_void_p),
+                ("identity", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class TypeStatement(ctypes.Structure):
+    """flock_type_statement: the digested fields of a boolean TableType."""
+
+    _fields_ = [("k_log", ctypes.c_uint32), ("has_const_pin", ctypes.c_uint32),
+                ("useful_bits", ctypes.c_uint64), ("const_pin", ctypes.c_uint64),
+                ("a0", SparseMatrix), ("b0", SparseMatrix), ("c0", SparseMatrix)]
 
 
 class ProveRequest(ctypes.Structure):
@@ -128,7 +145,9 @@ def header_functions() -> list[str]:
     """Every function the public header declares."""
     with open(HEADER) as fh:
         text = fh.read()
-    return sorted(set(re.findall(r"\b(flock_(?:hip|fs)_[a-z0-9_]+)\s*\(", text)))
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", " ", text)
+    return sorted(set(re.findall(r"\b(flock_[a-z0-9_]+)\s*\(", text)))
 
 
 def load():
@@ -162,6 +181,9 @@ def load():
         "flock_hip_commit_lane_major": (I, [P, P, S, PP, P, P, U32, P]),
         "flock_hip_commit": (I, [P, P, S, PP, P, P, U32, P]),
         "flock_hip_merkle_tree": (I, [P, P, S, S, P]),
+        "flock_hip_merkle_tree_kind": (I, [P, P, S, S, U32, P]),
+        "flock_registry_digest": (I, [P, S, U32, P]),
+        "flock_blake3_hash": (None, [P, S, P]),
         "flock_hip_encode": (I, [P, P, S, PP, P]),
         "flock_hip_encode_lane_major": (I, [P, P, S, PP, P]),
         "flock_hip_sumcheck_round_msg": (I, [P, P, P, S, P]),
@@ -354,8 +376,10 @@ class Context:
         _check(load().flock_hip_ntt_pass(self.h, data.ptr, num_ntts, live, log_d, first_layer,
                                          layers))
 
-    def merkle_tree(self, data: DeviceBuffer, data_len: int, n_leaves: int, tree: DeviceBuffer):
-        _check(load().flock_hip_merkle_tree(self.h, data.ptr, data_len, n_leaves, tree.ptr))
+    def merkle_tree(self, data: DeviceBuffer, data_len: int, n_leaves: int, tree: DeviceBuffer,
+                    kind: int = MERKLE_SHA256):
+        _check(load().flock_hip_merkle_tree_kind(self.h, data.ptr, data_len, n_leaves, kind,
+                                                 tree.ptr))
 
     def commit_lane_major(self, q: DeviceBuffer, q_len: int, params: PcsParams,
                           codeword: DeviceBuffer, tree: DeviceBuffer, cap_depth: int = 0) -> list[bytes]:
@@ -565,6 +589,35 @@ def pcs_sizes(params: PcsParams) -> dict:
                                       ctypes.byref(nl), ctypes.byref(k), ctypes.byref(nn)))
     return dict(msg_len=ml.value, cw_len=cl.value, n_leaves=nl.value, k_code=k.value,
                 num_ntts=nn.value)
+
+
+def blake3(data: bytes) -> bytes:
+    """blake3::hash of `data` (host)."""
+    out = ctypes.create_string_buffer(32)
+    load().flock_blake3_hash(data, len(data), out)
+    return out.raw
+
+
+def sparse_matrix(row_ptr=None, cols=None, num_cols: int = 0, num_rows=None,
+                  identity: bool = False, keep=None) -> SparseMatrix:
+    """A flock_sparse_matrix over numpy CSR arrays (kept alive in `keep`), or the identity."""
+    if identity:
+        return SparseMatrix(num_rows, num_rows, None, None, 1, 0)
+    if row_ptr is None:
+        return SparseMatrix(0, num_cols, None, None, 0, 0)
+    ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    c = np.ascontiguousarray(cols if len(cols) else np.zeros(1), dtype=np.uint32)
+    keep.extend([ptr, c])
+    return SparseMatrix(len(ptr) - 1 if num_rows is None else num_rows, num_cols,
+                        ptr.ctypes.data, c.ctypes.data, 0, 0)
+
+
+def registry_digest(types, nu: int) -> bytes:
+    """Registry::digest (schedule.rs:542-580) of TypeStatement structs in any order."""
+    arr = (TypeStatement * len(types))(*types)
+    out = ctypes.create_string_buffer(32)
+    _check(load().flock_registry_digest(arr, len(types), nu, out))
+    return out.raw
 
 
 def registry_new(types, nu: int) -> Registry:

@@ -90,8 +90,12 @@ int flock_hip_ntt_pass(flock_hip_ctx* ctx, flock_f128* d_data, size_t num_ntts, 
 
 /* ---- PCS commit -------------------------------------------------------------- */
 
+/* HashKind (hash.rs:23-27): the Merkle tree's node hash. */
+#define FLOCK_MERKLE_SHA256 0u
+#define FLOCK_MERKLE_BLAKE3 1u
+
 /* PcsParams (pcs/commit.rs:31-65). num_lanes = 0 means None (all 2^log_batch_size lanes).
- * merkle_hash: 0 = SHA-256 (the only hash implemented here; HashKind, hash.rs:23-58). */
+ * merkle_hash: FLOCK_MERKLE_SHA256 (the default) or FLOCK_MERKLE_BLAKE3. */
 typedef struct {
     uint32_t m, log_inv_rate, log_batch_size;
     uint32_t num_lanes;
@@ -131,6 +135,11 @@ int flock_hip_encode_lane_major(flock_hip_ctx* ctx, const flock_f128* d_q, size_
  * equal leaves of data_len / n_leaves bytes (a multiple of 16). */
 int flock_hip_merkle_tree(flock_hip_ctx* ctx, const uint8_t* d_data, size_t data_len,
                           size_t n_leaves, uint8_t* d_tree);
+/* merkle::merkle_tree(data, num_leaves, kind) (merkle.rs:543-589) for either HashKind. Under
+ * BLAKE3 a leaf is its bytes' non-root chaining value and a node the PARENT compression of
+ * its children (merkle.rs:195-232); leaves may span up to 2^16 chunks of 1024 bytes. */
+int flock_hip_merkle_tree_kind(flock_hip_ctx* ctx, const uint8_t* d_data, size_t data_len,
+                               size_t n_leaves, uint32_t kind, uint8_t* d_tree);
 
 /* ---- sumcheck rounds ----------------------------------------------------------- */
 
@@ -303,8 +312,33 @@ int flock_union_sizes_compute(const flock_registry* reg, const uint64_t* counts,
  * m_bool - 6 - nu, h_x_outer nu. Host only. */
 int flock_union_x_ab_from_mlv(const flock_registry* reg, const flock_f128* h_mlv, size_t n_mlv,
                               flock_f128* h_x_inner_rest, flock_f128* h_x_outer);
+/* SparseBinaryMatrix (r1cs.rs:14-18) as CSR: row r's columns are cols[row_ptr[r] ..
+ * row_ptr[r+1]) in the reference's row order (ascending for the hash circuits). identity = 1
+ * stands for the num_rows x num_rows identity (common.rs:231-237), row_ptr and cols unused. */
+typedef struct {
+    uint64_t num_rows, num_cols;
+    const uint64_t* row_ptr;
+    const uint32_t* cols;
+    uint32_t identity, reserved;
+} flock_sparse_matrix;
+/* The digested fields of a boolean TableType (schedule.rs:150-186, from_block_r1cs :175-186). */
+typedef struct {
+    uint32_t k_log, has_const_pin;
+    uint64_t useful_bits, const_pin;
+    flock_sparse_matrix a0, b0, c0;
+} flock_type_statement;
+/* Registry::digest (schedule.rs:542-580): BLAKE3 of "flock-registry-v1", 1u8, nu and the type
+ * count (u32 LE), then per type in Registry::new's order (stable, k_log descending): k_log
+ * u32, useful_bits u64, const pin (u8 present, u64 value), absorb_matrix of A_0, B_0, C_0
+ * (r1cs.rs:419-428). Boolean types without an IO schema: the element and schema payloads
+ * append nothing then. Host only. */
+int flock_registry_digest(const flock_type_statement* types, size_t n_types, uint32_t nu,
+                          uint8_t out[32]);
+/* blake3::hash (the 32-byte root hash, any length), the hash behind the registry digest.
+ * Host only. */
+void flock_blake3_hash(const void* data, size_t len, uint8_t out[32]);
 /* bind_statement (union.rs:619-628): label "flock-mixed-v1", the 32-byte Registry::digest
- * (BLAKE3, schedule.rs:496-580, computed by the caller), the counts as u64 LE, the cap bytes. */
+ * (flock_registry_digest), the counts as u64 LE, the cap bytes. */
 int flock_union_bind_statement(flock_fs* fs, const uint8_t registry_digest[32],
                                const uint64_t* counts, size_t n_counts, const uint8_t* cap,
                                size_t cap_len);

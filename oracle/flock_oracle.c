@@ -393,7 +393,8 @@ static int finalize(or_f128* cw, size_t t, size_t live, const or_pcs_params* p, 
     or_pcs_sizes(p, NULL, &cw_len, &n_leaves, &k, NULL);
     int rc = or_ntt_forward_interleaved(cw, t, live, k, k, (int)p->log_inv_rate);
     if (rc) return rc;
-    if (tree) rc = or_merkle_tree((const uint8_t*)cw, cw_len * 16, n_leaves, tree);
+    if (tree)
+        rc = or_merkle_tree_kind((const uint8_t*)cw, cw_len * 16, n_leaves, p->merkle_hash, tree);
     return rc;
 }
 
@@ -417,7 +418,9 @@ int or_commit_lane_major(const or_f128* q, size_t q_len, const or_pcs_params* p,
         for (size_t pp = 0; pp < d; pp++)
             for (size_t l = 0; l < t; l++) cw[rep * msg_len + pp * t + l] = q[l * d + pp];
     if (live == 0) {
-        return tree ? or_merkle_tree((const uint8_t*)cw, cw_len * 16, cw_len / t, tree) : 0;
+        return tree ? or_merkle_tree_kind((const uint8_t*)cw, cw_len * 16, cw_len / t,
+                                          p->merkle_hash, tree)
+                    : 0;
     }
     return finalize(cw, t, live, p, tree);
 }

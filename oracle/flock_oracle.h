@@ -48,11 +48,40 @@ int or_ntt_inverse_scalar(or_f128* data, int log_d, int L); /* :651-673 */
 void or_sha256(const uint8_t* data, size_t len, uint8_t out[32]);
 /* merkle_tree_sequential, :593-617 — tree has 2*num_leaves-1 nodes of 32 bytes */
 int or_merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, uint8_t* tree);
+/* the same tree under HashKind (hash.rs:23-27): 0 = SHA-256, 1 = BLAKE3 (merkle.rs:195-232) */
+int or_merkle_tree_kind(const uint8_t* data, size_t data_len, size_t num_leaves, uint32_t kind,
+                        uint8_t* tree);
+
+/* ---- BLAKE3 modes and Registry::digest (blake3_hash.c) ---- */
+void or_blake3_hash(const uint8_t* data, size_t len, uint8_t out[32]);
+void or_blake3_hash_pieces(const uint8_t* data, const size_t* piece_lens, size_t n_pieces,
+                           uint8_t out[32]);
+void or_blake3_leaf_cv(const uint8_t* data, size_t len, uint8_t out[32]);     /* merkle.rs:195-199 */
+void or_blake3_parent_cv(const uint8_t l[32], const uint8_t r[32], uint8_t out[32]); /* :201-205 */
+/* SparseBinaryMatrix (r1cs.rs:14-18) as CSR; identity = 1 stands for rows [[0], [1], ...]
+ * (common.rs:231-237) without materialising them. */
+typedef struct {
+    uint64_t num_rows, num_cols;
+    const uint64_t* row_ptr;
+    const uint32_t* cols;
+    uint32_t identity;
+} or_sparse;
+/* a boolean TableType's digested fields (schedule.rs:150-186) */
+typedef struct {
+    uint32_t k_log;
+    uint32_t has_const_pin;
+    uint64_t useful_bits;
+    uint64_t const_pin;
+    or_sparse a, b, c;
+} or_type_stmt;
+/* Registry::digest (schedule.rs:542-580) for boolean types without IO schemas */
+int or_registry_digest(const or_type_stmt* types, size_t n_types, uint32_t nu, uint8_t out[32]);
 
 /* ---- PCS commit (crates/flock-core/src/pcs/commit.rs) ---- */
 typedef struct {
     uint32_t m, log_inv_rate, log_batch_size;
-    uint32_t num_lanes; /* 0 = None (all 2^log_batch_size lanes) */
+    uint32_t num_lanes;   /* 0 = None (all 2^log_batch_size lanes) */
+    uint32_t merkle_hash; /* HashKind: 0 = SHA-256, 1 = BLAKE3 */
 } or_pcs_params;
 int or_pcs_sizes(const or_pcs_params* p, size_t* msg_len, size_t* cw_len, size_t* n_leaves,
                  int* k_code, size_t* num_ntts);

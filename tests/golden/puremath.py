@@ -367,3 +367,50 @@ def blake3_hash_short(msg: bytes) -> bytes:
     words = [int.from_bytes(blk[4 * i:4 * i + 4], "little") for i in range(16)]
     out = blake3_compress(BLAKE3_IV, words, 0, len(msg), 1 | 2 | 8)  # CHUNK_START|END|ROOT
     return b"".join(w.to_bytes(4, "little") for w in out[:8])
+
+
+def _b3_words(blk: bytes) -> list[int]:
+    blk = blk + bytes(64 - len(blk))
+    return [int.from_bytes(blk[4 * i:4 * i + 4], "little") for i in range(16)]
+
+
+def _b3_chunk(data: bytes, counter: int, root: bool) -> list[int]:
+    cv = BLAKE3_IV
+    nb = max(1, (len(data) + 63) // 64)
+    for i in range(nb):
+        blk = data[64 * i:64 * i + 64]
+        flags = (1 if i == 0 else 0) | (2 if i == nb - 1 else 0) | (8 if root and i == nb - 1 else 0)
+        cv = blake3_compress(cv, _b3_words(blk), counter, len(blk), flags)[:8]
+    return cv
+
+
+def _b3_subtree(data: bytes, first_chunk: int, root: bool) -> list[int]:
+    """Chaining value of the subtree over `data` (specification section 2.1: the left
+    subtree holds the largest power of two of chunks that leaves the right one non-empty)."""
+    n = max(1, (len(data) + 1023) // 1024)
+    if n == 1:
+        return _b3_chunk(data, first_chunk, root)
+    left = 1 << ((n - 1).bit_length() - 1)
+    lcv = _b3_subtree(data[:1024 * left], first_chunk, False)
+    rcv = _b3_subtree(data[1024 * left:], first_chunk + left, False)
+    return blake3_compress(BLAKE3_IV, lcv + rcv, 0, 64, 4 | (8 if root else 0))[:8]
+
+
+def _cv_bytes(cv: list[int]) -> bytes:
+    return b"".join(w.to_bytes(4, "little") for w in cv)
+
+
+def blake3_hash(msg: bytes) -> bytes:
+    """BLAKE3 root hash (32 bytes) of any message."""
+    return _cv_bytes(_b3_subtree(msg, 0, True))
+
+
+def blake3_leaf_cv(msg: bytes) -> bytes:
+    """The non-root chaining value of a message (the BLAKE3 Merkle leaf)."""
+    return _cv_bytes(_b3_subtree(msg, 0, False))
+
+
+def blake3_parent_cv(left: bytes, right: bytes) -> bytes:
+    """PARENT compression of two chaining values, no ROOT flag (the BLAKE3 Merkle node)."""
+    m = _b3_words(left[:32] + right[:32])
+    return _cv_bytes(blake3_compress(BLAKE3_IV, m, 0, 64, 4)[:8])

@@ -29,7 +29,22 @@ class PcsParams(ctypes.Structure):
         ("log_inv_rate", ctypes.c_uint32),
         ("log_batch_size", ctypes.c_uint32),
         ("num_lanes", ctypes.c_uint32),
+        ("merkle_hash", ctypes.c_uint32),
     ]
+
+
+class Sparse(ctypes.Structure):
+    """or_sparse: a SparseBinaryMatrix as CSR, or the identity."""
+    _fields_ = [("num_rows", ctypes.c_uint64), ("num_cols", ctypes.c_uint64),
+                ("row_ptr", ctypes.c_void_p), ("cols", ctypes.c_void_p),
+                ("identity", ctypes.c_uint32)]
+
+
+class TypeStmt(ctypes.Structure):
+    """or_type_stmt: the digested fields of a boolean TableType."""
+    _fields_ = [("k_log", ctypes.c_uint32), ("has_const_pin", ctypes.c_uint32),
+                ("useful_bits", ctypes.c_uint64), ("const_pin", ctypes.c_uint64),
+                ("a", Sparse), ("b", Sparse), ("c", Sparse)]
 
 
 def build() -> None:
@@ -61,6 +76,12 @@ def lib():
         L.or_ntt_inverse_scalar.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.or_sha256.argtypes = [P, S, P]
         L.or_merkle_tree.argtypes = [P, S, S, P]
+        L.or_merkle_tree_kind.argtypes = [P, S, S, ctypes.c_uint32, P]
+        L.or_blake3_hash.argtypes = [P, S, P]
+        L.or_blake3_hash_pieces.argtypes = [P, P, S, P]
+        L.or_blake3_leaf_cv.argtypes = [P, S, P]
+        L.or_blake3_parent_cv.argtypes = [P, P, P]
+        L.or_registry_digest.argtypes = [P, S, ctypes.c_uint32, P]
         L.or_pcs_sizes.argtypes = [ctypes.POINTER(PcsParams), P, P, P, P, P]
         L.or_commit_lane_major.argtypes = [P, S, ctypes.POINTER(PcsParams), P, P]
         L.or_commit.argtypes = [P, S, ctypes.POINTER(PcsParams), P, P]
@@ -147,11 +168,63 @@ def twiddle(L: int, layer: int, block: int) -> int:
     return to_int(lib().or_ntt_twiddle(_p(ev), L, layer, block))
 
 
-def merkle_tree(data: bytes, num_leaves: int) -> bytes:
+def merkle_tree(data: bytes, num_leaves: int, kind: int = 0) -> bytes:
     tree = ctypes.create_string_buffer(32 * (2 * num_leaves - 1))
     buf = ctypes.create_string_buffer(data, len(data))
-    assert lib().or_merkle_tree(buf, len(data), num_leaves, tree) == 0
+    assert lib().or_merkle_tree_kind(buf, len(data), num_leaves, kind, tree) == 0
     return tree.raw
+
+
+def blake3(data: bytes, pieces=None) -> bytes:
+    out = ctypes.create_string_buffer(32)
+    if pieces is None:
+        lib().or_blake3_hash(data, len(data), out)
+    else:
+        assert sum(pieces) == len(data)
+        pl = (ctypes.c_size_t * len(pieces))(*pieces)
+        lib().or_blake3_hash_pieces(data, pl, len(pieces), out)
+    return out.raw
+
+
+def blake3_leaf_cv(data: bytes) -> bytes:
+    out = ctypes.create_string_buffer(32)
+    lib().or_blake3_leaf_cv(data, len(data), out)
+    return out.raw
+
+
+def blake3_parent_cv(left: bytes, right: bytes) -> bytes:
+    out = ctypes.create_string_buffer(32)
+    lib().or_blake3_parent_cv(left, right, out)
+    return out.raw
+
+
+def sparse(rows=None, num_rows=None, num_cols=0, identity=False, keep=None):
+    """An or_sparse from a list of column lists (rows), or the num_rows x num_rows identity;
+    the CSR arrays are appended to `keep` so they outlive the struct."""
+    if identity:
+        return Sparse(num_rows, num_rows, None, None, 1)
+    rows = rows or []
+    ptr = np.zeros(len(rows) + 1, dtype=np.uint64)
+    ptr[1:] = np.cumsum([len(r) for r in rows]) if rows else []
+    cols = np.array([c for r in rows for c in r] or [0], dtype=np.uint32)
+    keep.extend([ptr, cols])
+    return Sparse(len(rows) if num_rows is None else num_rows, num_cols, ptr.ctypes.data,
+                  cols.ctypes.data, 0)
+
+
+def sparse_csr(row_ptr: np.ndarray, cols: np.ndarray, num_cols: int, keep) -> Sparse:
+    ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+    c = np.ascontiguousarray(cols, dtype=np.uint32)
+    keep.extend([ptr, c])
+    return Sparse(len(ptr) - 1, num_cols, ptr.ctypes.data, c.ctypes.data, 0)
+
+
+def registry_digest(types, nu: int) -> bytes:
+    """Registry::digest of boolean types given as TypeStmt structs (caller order)."""
+    arr = (TypeStmt * len(types))(*types)
+    out = ctypes.create_string_buffer(32)
+    assert lib().or_registry_digest(arr, len(types), nu, out) == 0
+    return out.raw
 
 
 def pcs_sizes(m, log_inv_rate, log_batch_size, num_lanes=0):
@@ -165,9 +238,10 @@ def pcs_sizes(m, log_inv_rate, log_batch_size, num_lanes=0):
                 num_ntts=nn.value)
 
 
-def commit_lane_major(q: np.ndarray, m, log_inv_rate, log_batch_size, num_lanes=0):
+def commit_lane_major(q: np.ndarray, m, log_inv_rate, log_batch_size, num_lanes=0,
+           merkle_hash=0):
     s = pcs_sizes(m, log_inv_rate, log_batch_size, num_lanes)
-    p = PcsParams(m, log_inv_rate, log_batch_size, num_lanes)
+    p = PcsParams(m, log_inv_rate, log_batch_size, num_lanes, merkle_hash)
     cw = np.empty((s["cw_len"], 2), dtype=np.uint64)
     tree = np.empty((2 * s["n_leaves"] - 1) * 32, dtype=np.uint8)
     rc = lib().or_commit_lane_major(_p(q), q.shape[0], ctypes.byref(p), _p(cw), _p(tree))
@@ -175,9 +249,10 @@ def commit_lane_major(q: np.ndarray, m, log_inv_rate, log_batch_size, num_lanes=
     return cw, tree.tobytes()
 
 
-def commit(msg: np.ndarray, m, log_inv_rate, log_batch_size, num_lanes=0):
+def commit(msg: np.ndarray, m, log_inv_rate, log_batch_size, num_lanes=0,
+           merkle_hash=0):
     s = pcs_sizes(m, log_inv_rate, log_batch_size, num_lanes)
-    p = PcsParams(m, log_inv_rate, log_batch_size, num_lanes)
+    p = PcsParams(m, log_inv_rate, log_batch_size, num_lanes, merkle_hash)
     cw = np.empty((s["cw_len"], 2), dtype=np.uint64)
     tree = np.empty((2 * s["n_leaves"] - 1) * 32, dtype=np.uint8)
     rc = lib().or_commit(_p(msg), msg.shape[0], ctypes.byref(p), _p(cw), _p(tree))

@@ -85,7 +85,7 @@ def test_pcs_sizes_errors():
     except F.FlockHipError as e:
         assert e.code == F.E_M_TOO_SMALL
     for bad in (F.PcsParams(20, 0, 5, 0, 0), F.PcsParams(20, 1, 5, 33, 0),
-                F.PcsParams(20, 1, 5, 0, 1)):
+                F.PcsParams(20, 1, 5, 0, 2)):
         try:
             F.pcs_sizes(bad)
             raise AssertionError("expected an error")

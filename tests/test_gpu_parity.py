@@ -109,6 +109,28 @@ def test_merkle_tree(ctx, n_leaves, leaf):
     assert tree.download() == O.merkle_tree(data.tobytes(), n_leaves)
 
 
+# BLAKE3 nodes (HashKind::Blake3, merkle.rs:195-232): one-chunk leaves, and leaves of 2, 3, 4
+# and 5 chunks of 1024 bytes (the left-balanced chunk tree; 3184 B ends in a partial chunk)
+@pytest.mark.parametrize("n_leaves,leaf", [(1, 16), (2, 32), (4, 48), (256, 64), (512, 800),
+                                           (256, 1024), (128, 1040), (64, 3072), (32, 3184),
+                                           (1024, 4096), (16, 5120), (4096, 16), (8192, 800)])
+def test_merkle_tree_blake3(ctx, n_leaves, leaf):
+    say(f"case: merkle blake3 n_leaves={n_leaves} leaf={leaf}")
+    rng = np.random.default_rng(n_leaves * 11 + leaf)
+    data = rng.integers(0, 256, size=n_leaves * leaf, dtype=np.uint8)
+    d = ctx.upload(data)
+    tree = ctx.alloc(32 * (2 * n_leaves - 1))
+    ctx.merkle_tree(d, n_leaves * leaf, n_leaves, tree, kind=F.MERKLE_BLAKE3)
+    assert tree.download() == O.merkle_tree(data.tobytes(), n_leaves, kind=1)
+
+
+def test_merkle_tree_rejects_unknown_hash(ctx):
+    d, tree = ctx.alloc(64), ctx.alloc(32)
+    with pytest.raises(F.FlockHipError) as e:
+        ctx.merkle_tree(d, 64, 1, tree, kind=2)
+    assert e.value.code == F.E_INVALID
+
+
 # ----------------------------------------------------------------- commit ----
 
 def test_commit_lane_major_golden(ctx):
@@ -124,17 +146,17 @@ def test_commit_lane_major_golden(ctx):
         assert cap[0].hex() == c["root"]
 
 
-def _commit_vs_oracle(ctx, m, rate, lb, nl, seed, lane_major=True, cap_depth=0):
-    p = F.PcsParams(m, rate, lb, nl, 0)
+def _commit_vs_oracle(ctx, m, rate, lb, nl, seed, lane_major=True, cap_depth=0, kind=0):
+    p = F.PcsParams(m, rate, lb, nl, kind)
     s = F.pcs_sizes(p)
     if lane_major:
         q = O.splitmix(seed, 1 << (m - 7))
         d = (1 << (m - 7)) >> lb
         q[s["num_ntts"] * d:] = 0
-        exp_cw, exp_tree = O.commit_lane_major(q, m, rate, lb, nl)
+        exp_cw, exp_tree = O.commit_lane_major(q, m, rate, lb, nl, merkle_hash=kind)
     else:
         q = O.splitmix(seed, s["msg_len"])
-        exp_cw, exp_tree = O.commit(q, m, rate, lb, nl)
+        exp_cw, exp_tree = O.commit(q, m, rate, lb, nl, merkle_hash=kind)
     dq, cw, tree = ctx.upload(q), ctx.alloc(16 * s["cw_len"]), ctx.alloc(32 * (2 * s["n_leaves"] - 1))
     fn = ctx.commit_lane_major if lane_major else ctx.commit
     cap = fn(dq, q.shape[0], p, cw, tree, cap_depth)
@@ -163,6 +185,17 @@ def test_commit_edge_shapes(ctx):
     for m, rate, lb, nl in [(7, 1, 0, 0), (9, 3, 2, 1), (14, 2, 5, 17), (16, 1, 3, 0)]:
         say(f"case: commit edge m={m} rate={rate} lb={lb} nl={nl}")
         _commit_vs_oracle(ctx, m, rate, lb, nl, m * 31 + nl, lane_major=True, cap_depth=1)
+
+
+def test_commit_blake3_merkle(ctx):
+    # PcsParams { merkle_hash: Blake3 }: config 1's 4096-byte leaves (4 chunks), the SHA-256
+    # union shape's 800-byte leaves (t = 50), and a small odd shape
+    say("case: commit blake3 config1 m=25 rate=1/4 lanes=256")
+    _commit_vs_oracle(ctx, 25, 2, 8, 0, 0xC0FFEE, lane_major=False, cap_depth=3, kind=1)
+    say("case: commit blake3 m=25 lb=6 t=50")
+    _commit_vs_oracle(ctx, 25, 1, 6, 50, 0x5A256, lane_major=True, cap_depth=8, kind=1)
+    say("case: commit blake3 m=14 lb=5 t=17")
+    _commit_vs_oracle(ctx, 14, 2, 5, 17, 0xB3, lane_major=True, cap_depth=1, kind=1)
 
 
 def test_encode_matches_commit_codeword(ctx):
