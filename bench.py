@@ -129,6 +129,8 @@ def mixed_workloads(F, ctx, args) -> dict:
 
     budget = int(288e9 * 0.9)
     provers = [F.Prover(0) for _ in range(args.workers)]
+    for pr in provers:
+        pr.set_round_mode({"host": F.ROUNDS_HOST, "device": F.ROUNDS_DEVICE}[args.round_mode])
     res = {}
     # batched: every request at t = 0; one worker (one proof at a time) vs all workers
     b_specs = specs(64, 10, 16, 0.0, False)
@@ -235,6 +237,8 @@ def main():
                     help="Poisson arrival rate of the served workload, requests/s")
     ap.add_argument("--latency-requests", type=int, default=50,
                     help="sequential prove-core requests timed one by one for p50/p99")
+    ap.add_argument("--round-mode", choices=["host", "device"], default="host",
+                    help="how the sumcheck round loops meet the transcript between rounds")
     ap.add_argument("--ntt-only", action="store_true",
                     help="only the commit (for profiler runs of the NTT)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -249,6 +253,8 @@ def main():
     from flock_amd import lib as F
 
     ctx = F.Context(0)
+    round_mode = {"host": F.ROUNDS_HOST, "device": F.ROUNDS_DEVICE}[args.round_mode]
+    ctx.set_round_mode(round_mode)
     ev = HipEvents()
     stream = ctx.stream
     nl = args.n_log

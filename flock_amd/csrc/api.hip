@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +26,7 @@
 #include "../../include/flock_hip.h"
 #include "blake3_common.h"
 #include "flock_internal.h"
+#include "transcript_device.h"
 
 namespace {
 
@@ -423,7 +425,7 @@ struct flock_hip_ctx {
     }
     uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words, then the grid-reduce ticket counter
     uint4* dout = nullptr;      // device view of hout: round kernels write their message there
-    flock_f128* hout = nullptr; // pinned, 2 words
+    flock_f128* hout = nullptr; // pinned, coherent: the pair, then a flag word
     std::map<int, uint4*> evals;  // k -> k x k basis table
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -545,6 +547,28 @@ struct flock_hip_ctx {
         *out = d;
         return 0;
     }
+    // Device-resident transcript for the sumcheck rounds: the state, then a log of
+    // kFsLogRounds rounds (device), mirrored in pinned host memory for upload and read-back.
+    static constexpr uint32_t kFsLogRounds = 96;
+    flock::DevTranscript* dfs = nullptr;
+    uint8_t* hfs = nullptr;
+    int round_mode = FLOCK_ROUNDS_HOST;
+    static constexpr size_t fs_bytes() {
+        return sizeof(flock::DevTranscript) + kFsLogRounds * sizeof(flock::DevRound);
+    }
+    const flock::DevRound* fs_log() const {
+        return reinterpret_cast<const flock::DevRound*>(hfs + sizeof(flock::DevTranscript));
+    }
+    // device address of round k's challenge
+    const uint4* fs_rho(uint32_t k) const {
+        return &reinterpret_cast<const flock::DevRound*>(dfs + 1)[k].rho;
+    }
+    // Whether `rounds` rounds with `bits` of PoW each can run on the device transcript: the
+    // host state must sit on a 16-byte boundary (every framed op keeps it there).
+    bool fs_usable(const flock_fs* fs, uint32_t bits, uint32_t rounds) const;
+    int fs_begin(const flock_fs* fs, uint32_t bits);
+    int fs_end(uint32_t rounds);
+
     // launch result pair -> host
     // The message kernel wrote its pair through dout straight into the coherent pinned hout,
     // so a round costs one launch and one wait, with no copy.
@@ -554,6 +578,15 @@ struct flock_hip_ctx {
         out[1] = hout[1];
         return 0;
     }
+    // A round kernel's pair, announced through the flag word after it (hout[2]): the next
+    // sequence number goes into the kernel's RoundOut, and the host polls the word instead of
+    // synchronising the stream.
+    uint32_t pair_seq = 0;
+    flock::RoundOut polled_out() {
+        if (++pair_seq == 0) ++pair_seq;
+        return flock::RoundOut{dout, reinterpret_cast<uint32_t*>(dout + 2), pair_seq, nullptr};
+    }
+    int await_pair(flock_f128 out[2]);
 };
 
 namespace {
@@ -572,6 +605,150 @@ inline flock_f128 mulh(flock_f128 a, flock_f128 b) {
 #define LAUNCH(expr) HIP_TRY(expr)
 }  // namespace
 
+bool flock_hip_ctx::fs_usable(const flock_fs* fs, uint32_t bits, uint32_t rounds) const {
+    return fs && fs->st.blen % 16 == 0 && bits <= flock::kFsMaxGrindBits &&
+           rounds <= kFsLogRounds;
+}
+
+int flock_hip_ctx::fs_begin(const flock_fs* fs, uint32_t bits) {
+    if (!dfs) {
+        if (hipMalloc(&dfs, fs_bytes()) != hipSuccess ||
+            hipHostMalloc((void**)&hfs, fs_bytes()) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(FLOCK_HIP_E_NOMEM, "device transcript");
+        }
+    }
+    // the previous read-back has completed (fs_end synchronises), so the staging is free
+    flock::DevTranscript t{};
+    for (int i = 0; i < 8; i++) t.h[i] = fs->st.h[i];
+    for (size_t i = 0; i < fs->st.blen / 4; i++) {
+        const uint8_t* b = fs->st.buf + 4 * i;
+        t.buf[i] = (uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | b[3];
+    }
+    t.total = fs->st.total;
+    t.blen = (uint32_t)fs->st.blen;
+    t.round = 0;
+    t.grind_bits = bits;
+    t.max_rounds = kFsLogRounds;
+    std::memcpy(hfs, &t, sizeof(t));
+    HIP_TRY(hipMemcpyAsync(dfs, hfs, sizeof(t), hipMemcpyHostToDevice, stream));
+    return 0;
+}
+
+int flock_hip_ctx::fs_end(uint32_t rounds) {
+    HIP_TRY(hipMemcpyAsync(hfs + sizeof(flock::DevTranscript), dfs + 1,
+                           rounds * sizeof(flock::DevRound), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+}
+
+// Poll the flag word after hout for the pair announced with pair_seq. Bounded: past 10 s, or
+// once the stream reports an error, the wait ends with an error.
+int flock_hip_ctx::await_pair(flock_f128 out[2]) {
+    const uint32_t* flag = reinterpret_cast<const uint32_t*>(hout + 2);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t it = 1;; it++) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == pair_seq) break;
+        if ((it & 4095) == 0) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(FLOCK_HIP_E_HIP, std::string("round kernel: ") + hipGetErrorString(q));
+            if (q == hipSuccess && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != pair_seq)
+                return fail(FLOCK_HIP_E_INTERNAL, "round kernel finished without its message");
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10))
+                return fail(FLOCK_HIP_E_INTERNAL, "round kernel: no message within 10 s");
+        }
+        _mm_pause();
+    }
+    out[0] = hout[0];
+    out[1] = hout[1];
+    return 0;
+}
+
+namespace {
+// Advance the host transcript through one round the device transcript ran, checking that
+// both drew the same nonce and challenge.
+int fs_replay(flock_fs* fs, const flock::DevRound& r, uint32_t bits, flock_f128 msg[2],
+              uint64_t* nonce, flock_f128* rho) {
+    msg[0] = flock_f128{(uint64_t)r.m0.y << 32 | r.m0.x, (uint64_t)r.m0.w << 32 | r.m0.z};
+    msg[1] = flock_f128{(uint64_t)r.m1.y << 32 | r.m1.x, (uint64_t)r.m1.w << 32 | r.m1.z};
+    flock_fs_observe_f128(fs, msg[0]);
+    flock_fs_observe_f128(fs, msg[1]);
+    if (bits) {
+        const uint64_t n = flock_fs_grind_pow(fs, bits);
+        if (r.err || n != r.nonce)
+            return fail(FLOCK_HIP_E_INTERNAL, "device transcript: proof-of-work nonce differs");
+        if (nonce) *nonce = n;
+    }
+    flock_fs_sample_f128(fs, rho);
+    const flock_f128 d{(uint64_t)r.rho.y << 32 | r.rho.x, (uint64_t)r.rho.w << 32 | r.rho.z};
+    if (rho->lo != d.lo || rho->hi != d.hi)
+        return fail(FLOCK_HIP_E_INTERNAL, "device transcript: challenge differs");
+    return 0;
+}
+}  // namespace
+
+namespace {
+
+inline flock_f128 from_u4(uint4 v) {
+    return flock_f128{(uint64_t)v.y << 32 | v.x, (uint64_t)v.w << 32 | v.z};
+}
+
+// A loop of n sumcheck rounds on the context's stream. launch(j, out, rin, rho) enqueues
+// round j's kernel: for j < n it emits message j to `out`; for j >= 1 it consumes challenge
+// j-1, from the device pointer rin when that is set, else the value rho; launch(n, ...) is
+// the closing fold. Each message is observed on `fs`, ground (bits > 0) and answered with a
+// challenge, in the context's round mode (flock_hip_ctx_set_round_mode).
+struct RoundLoop {
+    using Launch = std::function<int(uint32_t, flock::RoundOut, const uint4*, uint4)>;
+    flock_hip_ctx* c;
+    flock_fs* fs;
+    uint32_t n, bits;
+    Launch launch;
+    std::vector<flock_f128> msgs;  // 2n
+    std::vector<flock_f128> rhos;  // n
+    std::vector<uint64_t> nonces;  // n, when bits > 0
+    int mode = FLOCK_ROUNDS_HOST;
+
+    void host_round(uint32_t k) {
+        flock_fs_observe_f128(fs, msgs[2 * k]);
+        flock_fs_observe_f128(fs, msgs[2 * k + 1]);
+        if (bits) nonces[k] = flock_fs_grind_pow(fs, bits);
+        flock_fs_sample_f128(fs, &rhos[k]);
+    }
+
+    int run() {
+        msgs.assign(2 * n, flock_f128{0, 0});
+        rhos.assign(n, flock_f128{0, 0});
+        nonces.assign(n, 0);
+        const uint4 zero = make_uint4(0, 0, 0, 0);
+        mode = c->round_mode;
+        if (mode != FLOCK_ROUNDS_HOST && !c->fs_usable(fs, bits, n)) mode = FLOCK_ROUNDS_HOST;
+        if (mode == FLOCK_ROUNDS_DEVICE) {
+            if (int rc = c->fs_begin(fs, bits)) return rc;
+            const flock::RoundOut ro{nullptr, nullptr, 0u, c->dfs};
+            for (uint32_t j = 0; j <= n; j++)
+                if (int rc = launch(j, ro, j ? c->fs_rho(j - 1) : nullptr, zero)) return rc;
+            if (int rc = c->fs_end(n)) return rc;
+            for (uint32_t k = 0; k < n; k++)
+                if (int rc = fs_replay(fs, c->fs_log()[k], bits, &msgs[2 * k], &nonces[k],
+                                       &rhos[k]))
+                    return rc;
+            return 0;
+        }
+        if (int rc = launch(0, c->polled_out(), nullptr, zero)) return rc;
+        for (uint32_t k = 0; k < n; k++) {
+            if (int rc = c->await_pair(&msgs[2 * k])) return rc;
+            host_round(k);
+            if (int rc = launch(k + 1, k + 1 < n ? c->polled_out() : flock::RoundOut{}, nullptr,
+                                U(rhos[k])))
+                return rc;
+        }
+        return 0;
+    }
+};
+}  // namespace
+
 extern "C" {
 
 int flock_hip_device_count(void) {
@@ -584,6 +761,14 @@ int flock_hip_device_count(void) {
 }
 
 const char* flock_hip_last_error(void) { return g_err.c_str(); }
+
+int flock_hip_ctx_set_round_mode(flock_hip_ctx* c, int mode) {
+    CTX_CHECK(c);
+    if (mode != FLOCK_ROUNDS_HOST && mode != FLOCK_ROUNDS_DEVICE)
+        return fail(FLOCK_HIP_E_INVALID, "unknown round mode");
+    c->round_mode = mode;
+    return 0;
+}
 
 int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
     if (!out) return fail(FLOCK_HIP_E_INVALID, "null out");
@@ -604,13 +789,15 @@ int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&c->partials, sizeof(uint4) * (flock::kMaxReduceBlocks * 2 + 1)) != hipSuccess ||
-        hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 2,
+        hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 3,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&c->dout, c->hout, 0) != hipSuccess) {
         hipError_t e = hipGetLastError();
         flock_hip_ctx_destroy(c);
         return fail(FLOCK_HIP_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
     }
+    // the polled flag word after the pair starts at zero, below every sequence number
+    std::memset(c->hout, 0, sizeof(flock_f128) * 3);
     // the grid-reduce ticket counter after the partials starts at zero (each reduction that
     // uses it leaves it at zero)
     if (hipMemset(c->partials + 2 * flock::kMaxReduceBlocks, 0, sizeof(uint4)) != hipSuccess) {
@@ -626,6 +813,8 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& kv : c->evals) hipFree(kv.second);
+    if (c->dfs) hipFree(c->dfs);
+    if (c->hfs) hipHostFree(c->hfs);
     if (c->zc_ext) hipFree(c->zc_ext);
     if (c->zc_ext_bs) hipFree(c->zc_ext_bs);
     if (c->zc_phim) hipFree(c->zc_phim);
@@ -877,7 +1066,8 @@ int flock_hip_sumcheck_round_msg(flock_hip_ctx* c, const flock_f128* f, const fl
     CTX_CHECK(c);
     if (int rc = check_n(n, 2)) return rc;
     if (!f || !b || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
-    LAUNCH(flock::launch_sc_round_msg(U(f), U(b), n, c->partials, c->dout, c->stream));
+    LAUNCH(flock::launch_sc_round_msg(U(f), U(b), n, c->partials, flock::round_out(c->dout),
+                                      c->stream));
     return c->fetch_pair(msg);
 }
 
@@ -887,8 +1077,8 @@ int flock_hip_sumcheck_fold_msg(flock_hip_ctx* c, const flock_f128* f, const flo
     CTX_CHECK(c);
     if (int rc = check_n(n, 2)) return rc;
     if (!f || !b || !nf || !nb || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
-    LAUNCH(flock::launch_sc_fold_msg(U(f), U(b), n, U(r), U(nf), U(nb), c->partials, c->dout,
-                                     c->stream));
+    LAUNCH(flock::launch_sc_fold_msg(U(f), U(b), n, U(r), U(nf), U(nb), c->partials,
+                                     flock::round_out(c->dout), c->stream));
     return c->fetch_pair(msg);
 }
 
@@ -911,7 +1101,7 @@ int flock_hip_zc_round_pair(flock_hip_ctx* c, const flock_f128* a, const flock_f
     if (int rc = check_n(n, 2)) return rc;
     if (!a || !b || !eq || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
     LAUNCH(flock::launch_zc_round(U(a), U(b), n, U(eq), n >= 4 ? U(eq_next) : nullptr,
-                                  c->partials, c->dout, c->stream));
+                                  c->partials, flock::round_out(c->dout), c->stream));
     if (int rc = c->fetch_pair(msg)) return rc;
     msg[0] = mulh(r0, msg[0]);
     return 0;
@@ -925,8 +1115,8 @@ int flock_hip_zc_fold_round_pair(flock_hip_ctx* c, const flock_f128* a, const fl
     if (int rc = check_n(n, 4)) return rc;
     if (!a || !b || !ao || !bo || !eq || !msg) return fail(FLOCK_HIP_E_INVALID, "null buffer");
     LAUNCH(flock::launch_zc_fold_round(U(a), U(b), n, U(rho), U(ao), U(bo), U(eq),
-                                       n >= 8 ? U(eq_next) : nullptr, c->partials, c->dout,
-                                       c->stream));
+                                       n >= 8 ? U(eq_next) : nullptr, c->partials,
+                                       flock::round_out(c->dout), c->stream));
     if (int rc = c->fetch_pair(msg)) return rc;
     msg[0] = mulh(r0, msg[0]);
     return 0;
@@ -1108,58 +1298,50 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
         LAUNCH(flock::launch_zc_fold_at_z(d_a, d_b, N, TBL, Az, Bz, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));  // t is host memory
     }
-    flock_f128 m2[2];
-    LAUNCH(flock::launch_zc_round(Az, Bz, N, EB, N >= 4 ? EC : nullptr, c->partials, c->dout,
-                                  c->stream));
-    if (int rc = c->fetch_pair(m2)) return rc;
-    int mi = 0;
-    h_ml[mi++] = m2[0];  // mlv_arg[0] = ONE
-    h_ml[mi++] = m2[1];
-    flock_fs_observe_f128(fs, m2[0]);
-    flock_fs_observe_f128(fs, m2[1]);
-    if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(2));
-    flock_f128 rho;
-    flock_fs_sample_f128(fs, &rho);
-    h_claim[1] = rho;
-
-    // ---- 4. tail: fold at rho, next message (zerocheck.rs:790-947) ----
-    const uint4* cf = Az;
-    const uint4* cb = Bz;
-    uint64_t len = N;
-    int pp = 0, ep = 1;
-    for (uint32_t i = 0; i + 1 < n_mlv; i++) {
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_zc_fold_round(cf, cb, len, U(rho), nf, nb, eqb[ep],
-                                           len >= 8 ? eqb[ep ^ 1] : nullptr, c->partials,
-                                           c->dout, c->stream));
-        if (int rc = c->fetch_pair(m2)) return rc;
-        h_ml[mi++] = m2[0];
-        h_ml[mi++] = m2[1];
-        flock_fs_observe_f128(fs, m2[0]);
-        flock_fs_observe_f128(fs, m2[1]);
-        if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(2));
-        flock_fs_sample_f128(fs, &rho);
-        h_claim[2 + i] = rho;
-        cf = nf;
-        cb = nb;
-        len /= 2;
-        pp ^= 1;
-        ep ^= 1;
+    // ---- 3-5. round 2's message, the tail (zerocheck.rs:790-947) and the final binding
+    // (:955-975): round j < n_mlv folds the previous tables at challenge j-1 (j > 0) and emits
+    // message j; launch n_mlv is the closing fold ----
+    // the closing fold writes bufs[(n_mlv - 1) & 1] (launches may come from a cached graph, so
+    // nothing is recorded inside the launch callback)
+    uint4* last_f = bufs[(n_mlv - 1) & 1][0];
+    uint4* last_b = bufs[(n_mlv - 1) & 1][1];
+    RoundLoop loop{c, fs, n_mlv, grinding ? bits_for(2) : 0u,
+                   [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 rho) -> int {
+                       if (j == 0) {
+                           LAUNCH(flock::launch_zc_round(Az, Bz, N, EB, N >= 4 ? EC : nullptr,
+                                                         c->partials, ro, c->stream));
+                           return 0;
+                       }
+                       const uint32_t i = j - 1;  // the fold's index
+                       const uint4* cf = i == 0 ? Az : bufs[(i - 1) & 1][0];
+                       const uint4* cb = i == 0 ? Bz : bufs[(i - 1) & 1][1];
+                       const uint64_t len = N >> i;
+                       uint4* nf = bufs[i & 1][0];
+                       uint4* nb = bufs[i & 1][1];
+                       if (j < n_mlv) {
+                           const int ep = 1 ^ (int)(i & 1);
+                           LAUNCH(flock::launch_zc_fold_round(
+                               cf, cb, len, rho, nf, nb, eqb[ep], len >= 8 ? eqb[ep ^ 1] : nullptr,
+                               c->partials, ro, c->stream, rin));
+                       } else {
+                           LAUNCH(flock::launch_fold_pair(cf, cb, len, rho, nf, nb, c->stream, rin));
+                       }
+                       return 0;
+                   }};
+    if (int rc = loop.run()) return rc;
+    HIP_TRY(hipMemcpyAsync(c->hout, last_f, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->hout + 1, last_b, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < n_mlv; k++) {
+        h_ml[2 * k] = loop.msgs[2 * k];  // mlv_arg[0] = ONE
+        h_ml[2 * k + 1] = loop.msgs[2 * k + 1];
+        if (grinding) h_nonces[ni++] = loop.nonces[k];
+        h_claim[1 + k] = loop.rhos[k];
     }
-    // ---- 5. final binding and observe (zerocheck.rs:955-975) ----
-    {
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(rho), nf, nb, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        h_finals[0] = c->hout[0];
-        h_finals[1] = c->hout[1];
-        flock_fs_observe_f128(fs, h_finals[0]);
-        flock_fs_observe_f128(fs, h_finals[1]);
-    }
+    h_finals[0] = c->hout[0];
+    h_finals[1] = c->hout[1];
+    flock_fs_observe_f128(fs, h_finals[0]);
+    flock_fs_observe_f128(fs, h_finals[1]);
     h_claim[0] = z;
     for (uint32_t k = 0; k < n_mlv; k++) h_claim[1 + n_mlv + k] = r[ks + k];
     return 0;
@@ -1196,49 +1378,42 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     int mi = 0;
     flock_f128 m2[2];
 
-    // ---- product sumcheck ----
+    // ---- product sumcheck: round j < log_n folds at challenge j-1 (j > 0) and emits message j;
+    // launch log_n is the closing fold ----
     flock_fs_observe_label(fs, kLabelProduct, sizeof(kLabelProduct) - 1);
-    LAUNCH(flock::launch_sc_round_msg(U(d_a), U(d_b), n, c->partials, c->dout, c->stream));
-    if (int rc = c->fetch_pair(m2)) return rc;
-    msgs[mi++] = m2[0];
-    msgs[mi++] = m2[1];
-    flock_fs_observe_f128(fs, m2[0]);
-    flock_fs_observe_f128(fs, m2[1]);
-    const uint4* cf = U(d_a);
-    const uint4* cb = U(d_b);
-    size_t len = n;
-    int pp = 0;
-    for (uint32_t i = 0; i + 1 < log_n; i++) {
-        flock_f128 r;
-        flock_fs_sample_f128(fs, &r);
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_sc_fold_msg(cf, cb, len, U(r), nf, nb, c->partials, c->dout,
-                                         c->stream));
-        if (int rc = c->fetch_pair(m2)) return rc;
-        msgs[mi++] = m2[0];
-        msgs[mi++] = m2[1];
-        flock_fs_observe_f128(fs, m2[0]);
-        flock_fs_observe_f128(fs, m2[1]);
-        cf = nf;
-        cb = nb;
-        len /= 2;
-        pp ^= 1;
-    }
-    {
-        flock_f128 r;
-        flock_fs_sample_f128(fs, &r);
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(r), nf, nb, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        finals[0] = c->hout[0];
-        finals[1] = c->hout[1];
-        flock_fs_observe_f128(fs, finals[0]);
-        flock_fs_observe_f128(fs, finals[1]);
-    }
+    // both loops close with a fold into bufs[(log_n - 1) & 1]
+    uint4* last_f = bufs[(log_n - 1) & 1][0];
+    uint4* last_b = bufs[(log_n - 1) & 1][1];
+    RoundLoop prod{c, fs, log_n, 0u,
+                   [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 r) -> int {
+                       if (j == 0) {
+                           LAUNCH(flock::launch_sc_round_msg(U(d_a), U(d_b), n, c->partials, ro,
+                                                             c->stream));
+                           return 0;
+                       }
+                       const uint32_t i = j - 1;
+                       const uint4* cf = i == 0 ? U(d_a) : bufs[(i - 1) & 1][0];
+                       const uint4* cb = i == 0 ? U(d_b) : bufs[(i - 1) & 1][1];
+                       const size_t len = n >> i;
+                       uint4* nf = bufs[i & 1][0];
+                       uint4* nb = bufs[i & 1][1];
+                       if (j < log_n) {
+                           LAUNCH(flock::launch_sc_fold_msg(cf, cb, len, r, nf, nb, c->partials,
+                                                            ro, c->stream, rin));
+                       } else {
+                           LAUNCH(flock::launch_fold_pair(cf, cb, len, r, nf, nb, c->stream, rin));
+                       }
+                       return 0;
+                   }};
+    if (int rc = prod.run()) return rc;
+    HIP_TRY(hipMemcpyAsync(c->hout, last_f, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->hout + 1, last_b, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < 2 * log_n; k++) msgs[mi++] = prod.msgs[k];
+    finals[0] = c->hout[0];
+    finals[1] = c->hout[1];
+    flock_fs_observe_f128(fs, finals[0]);
+    flock_fs_observe_f128(fs, finals[1]);
 
     // ---- eq-weighted zerocheck tail ----
     flock_fs_observe_label(fs, kLabelZc, sizeof(kLabelZc) - 1);
@@ -1247,50 +1422,39 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     if (log_n > 1) HIP_TRY(hipMemcpyAsync(R, r.data() + 1, (log_n - 1) * 16, hipMemcpyHostToDevice,
                                           c->stream));
     LAUNCH(flock::launch_build_eq(R, (int)log_n - 1, E0, EQS, c->stream));
-    LAUNCH(flock::launch_zc_round(U(d_a), U(d_b), n, E0, n >= 4 ? E1 : nullptr, c->partials,
-                                  c->dout, c->stream));
-    if (int rc = c->fetch_pair(m2)) return rc;  // also orders the pageable r copy above
-    msgs[mi++] = m2[0];  // r_next[0] = ONE (Convention A)
-    msgs[mi++] = m2[1];
-    flock_fs_observe_f128(fs, m2[0]);
-    flock_fs_observe_f128(fs, m2[1]);
-    flock_f128 rho;
-    flock_fs_sample_f128(fs, &rho);
-    cf = U(d_a);
-    cb = U(d_b);
-    len = n;
-    pp = 0;
-    int ep = 1;  // eq table for the next message lives in eqb[ep]
-    for (uint32_t i = 0; i + 1 < log_n; i++) {
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_zc_fold_round(cf, cb, len, U(rho), nf, nb, eqb[ep],
-                                           len >= 8 ? eqb[ep ^ 1] : nullptr, c->partials,
-                                           c->dout, c->stream));
-        if (int rc = c->fetch_pair(m2)) return rc;
-        msgs[mi++] = m2[0];
-        msgs[mi++] = m2[1];
-        flock_fs_observe_f128(fs, m2[0]);
-        flock_fs_observe_f128(fs, m2[1]);
-        flock_fs_sample_f128(fs, &rho);
-        cf = nf;
-        cb = nb;
-        len /= 2;
-        pp ^= 1;
-        ep ^= 1;
-    }
-    {
-        uint4* nf = bufs[pp][0];
-        uint4* nb = bufs[pp][1];
-        LAUNCH(flock::launch_fold_pair(cf, cb, len, U(rho), nf, nb, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout, nf, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->hout + 1, nb, 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        finals[2] = c->hout[0];
-        finals[3] = c->hout[1];
-        flock_fs_observe_f128(fs, finals[2]);
-        flock_fs_observe_f128(fs, finals[3]);
-    }
+    RoundLoop zc{c, fs, log_n, 0u,
+                 [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 rho) -> int {
+                     if (j == 0) {
+                         LAUNCH(flock::launch_zc_round(U(d_a), U(d_b), n, E0,
+                                                       n >= 4 ? E1 : nullptr, c->partials, ro,
+                                                       c->stream));
+                         return 0;
+                     }
+                     const uint32_t i = j - 1;
+                     const uint4* cf = i == 0 ? U(d_a) : bufs[(i - 1) & 1][0];
+                     const uint4* cb = i == 0 ? U(d_b) : bufs[(i - 1) & 1][1];
+                     const size_t len = n >> i;
+                     uint4* nf = bufs[i & 1][0];
+                     uint4* nb = bufs[i & 1][1];
+                     if (j < log_n) {
+                         const int ep = 1 ^ (int)(i & 1);  // eq table of the next message
+                         LAUNCH(flock::launch_zc_fold_round(
+                             cf, cb, len, rho, nf, nb, eqb[ep], len >= 8 ? eqb[ep ^ 1] : nullptr,
+                             c->partials, ro, c->stream, rin));
+                     } else {
+                         LAUNCH(flock::launch_fold_pair(cf, cb, len, rho, nf, nb, c->stream, rin));
+                     }
+                     return 0;
+                 }};
+    if (int rc = zc.run()) return rc;  // the pageable r copy completed when its call returned
+    HIP_TRY(hipMemcpyAsync(c->hout, last_f, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->hout + 1, last_b, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < 2 * log_n; k++) msgs[mi++] = zc.msgs[k];  // r_next[0] = ONE
+    finals[2] = c->hout[0];
+    finals[3] = c->hout[1];
+    flock_fs_observe_f128(fs, finals[2]);
+    flock_fs_observe_f128(fs, finals[3]);
     return 0;
 }
 
@@ -1577,34 +1741,40 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     }
     if (int rc = c->join()) return rc;
     // 4-5. product sumcheck over the top L bits (column_sumcheck_prove, lincheck.rs:1907-1980)
+    // round t < L binds the top bit at challenge t-1 (t > 0) and emits message t; launch L
+    // is the last binding
     std::vector<flock_f128> rr(L ? L : 1);
     if (L) {
-        flock_f128 msg[2];
-        LAUNCH(flock::launch_lc_eval(COMB, ZV, K, c->partials, c->dout, c->stream));
-        if (int rc = c->fetch_pair(msg)) return rc;
-        uint64_t len = K;
+        RoundLoop loop{c, fs, L, grinding ? 2u : 0u,
+                       [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 r) -> int {
+                           if (j == 0) {
+                               LAUNCH(flock::launch_lc_eval(COMB, ZV, K, c->partials, ro,
+                                                            c->stream));
+                           } else if (j < L) {
+                               LAUNCH(flock::launch_lc_bind_eval(COMB, ZV, K >> (j - 1), r,
+                                                                 c->partials, ro, c->stream, rin));
+                           } else {
+                               LAUNCH(flock::launch_lc_bind(COMB, ZV, K >> (j - 1), r, c->stream,
+                                                            rin));
+                           }
+                           return 0;
+                       }};
+        if (int rc = loop.run()) return rc;
         for (uint32_t t = 0; t < L; t++) {
-            flock_fs_observe_f128(fs, msg[0]);
-            flock_fs_observe_f128(fs, msg[1]);
-            if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 2);
-            flock_f128 r;
-            flock_fs_sample_f128(fs, &r);
-            h_rounds[2 * t] = msg[0];
-            h_rounds[2 * t + 1] = msg[1];
-            rr[t] = r;
-            if (t + 1 < L) {
-                LAUNCH(flock::launch_lc_bind_eval(COMB, ZV, len, U(r), c->partials, c->dout,
-                                                  c->stream));
-                if (int rc = c->fetch_pair(msg)) return rc;
-            } else {
-                LAUNCH(flock::launch_lc_bind(COMB, ZV, len, U(r), c->stream));
-            }
-            len /= 2;
+            h_rounds[2 * t] = loop.msgs[2 * t];
+            h_rounds[2 * t + 1] = loop.msgs[2 * t + 1];
+            rr[t] = loop.rhos[t];
+            if (grinding) h_nonces[ni++] = loop.nonces[t];
         }
+        // z_partial = the first 64 words of the bound z (lincheck.rs:1983-1990)
+        HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    // 6-8. z_partial, fresh skip point, w (lincheck.rs:1983-2005)
-    HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    // 6-8. z_partial (read back above when L > 0), fresh skip point, w (lincheck.rs:1983-2005)
+    if (!L) {
+        HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     flock_fs_observe_f128_slice(fs, h_z_partial, 64);
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, (uint32_t)ks);
     flock_f128 r_skip;

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "transcript_device.h"
+
 typedef uint4 f128;
 
 __device__ __forceinline__ f128 f128_zero() { return make_uint4(0u, 0u, 0u, 0u); }
@@ -307,8 +309,14 @@ __device__ __forceinline__ f128* comb_lane(f128* ws) {
 // ticket folds every pair (volatile loads, past any stale L1 line), writes out and resets
 // the counter. gridDim.x <= kGridReduceMax.
 constexpr int kGridReduceMax = 512;
-__device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials, f128* out) {
+// The pair goes where `ro` says (flock::RoundOut): into ro.out when set, then ro.seq into
+// ro.flag (system scope) when set; with ro.fs the workgroup that holds the final pair also runs
+// the device transcript's round step on it.
+__device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials,
+                                              flock::RoundOut ro) {
+    f128* out = ro.out;
     __shared__ f128 red[2][16];
+    __shared__ f128 fin[2];
     __shared__ int last;
     a0 = wave_xor_f128(a0);
     a1 = wave_xor_f128(a1);
@@ -327,8 +335,16 @@ __device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials, 
             r1 = f128_add(r1, red[1][w]);
         }
         if (gridDim.x == 1) {
-            out[0] = r0;
-            out[1] = r1;
+            fin[0] = r0;
+            fin[1] = r1;
+            if (out) {
+                out[0] = r0;
+                out[1] = r1;
+            }
+            if (ro.flag) {
+                __threadfence_system();
+                __hip_atomic_store(ro.flag, ro.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
         } else {
             partials[2 * blockIdx.x] = r0;
             partials[2 * blockIdx.x + 1] = r1;
@@ -336,33 +352,47 @@ __device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials, 
             last = atomicAdd(ctr, 1u) == gridDim.x - 1;
         }
     }
-    if (gridDim.x == 1) return;
     __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const volatile uint32_t* vp = reinterpret_cast<const volatile uint32_t*>(partials);
-    f128 c0 = f128_zero(), c1 = f128_zero();
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-        const volatile uint32_t* q = vp + 8 * i;
-        c0 = f128_add(c0, make_uint4(q[0], q[1], q[2], q[3]));
-        c1 = f128_add(c1, make_uint4(q[4], q[5], q[6], q[7]));
-    }
-    c0 = wave_xor_f128(c0);
-    c1 = wave_xor_f128(c1);
-    __syncthreads();  // red is reused
-    if (lane == 0) {
-        red[0][wid] = c0;
-        red[1][wid] = c1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        f128 r0 = f128_zero(), r1 = f128_zero();
-        for (int w = 0; w < nw; w++) {
-            r0 = f128_add(r0, red[0][w]);
-            r1 = f128_add(r1, red[1][w]);
+    if (gridDim.x > 1) {
+        if (!last) return;
+        __threadfence();
+        const volatile uint32_t* vp = reinterpret_cast<const volatile uint32_t*>(partials);
+        f128 c0 = f128_zero(), c1 = f128_zero();
+        for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+            const volatile uint32_t* q = vp + 8 * i;
+            c0 = f128_add(c0, make_uint4(q[0], q[1], q[2], q[3]));
+            c1 = f128_add(c1, make_uint4(q[4], q[5], q[6], q[7]));
         }
-        out[0] = r0;
-        out[1] = r1;
-        atomicExch(ctr, 0u);
+        c0 = wave_xor_f128(c0);
+        c1 = wave_xor_f128(c1);
+        __syncthreads();  // red is reused
+        if (lane == 0) {
+            red[0][wid] = c0;
+            red[1][wid] = c1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            f128 r0 = f128_zero(), r1 = f128_zero();
+            for (int w = 0; w < nw; w++) {
+                r0 = f128_add(r0, red[0][w]);
+                r1 = f128_add(r1, red[1][w]);
+            }
+            fin[0] = r0;
+            fin[1] = r1;
+            if (out) {
+                out[0] = r0;
+                out[1] = r1;
+            }
+            if (ro.flag) {
+                __threadfence_system();
+                __hip_atomic_store(ro.flag, ro.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            atomicExch(ctr, 0u);
+        }
+    }
+    if (ro.fs) {
+        __syncthreads();
+        flock::fs_round_step(ro.fs, fin[0], fin[1]);
     }
 }
+

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "transcript_device.h"
+
 namespace flock {
 
 // Two F128 per block partial (e.g. u0, u2) — 32 bytes.
@@ -47,23 +49,28 @@ hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t l
 
 // ---- sumcheck.hip ----
 // round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2 + 1 words (the last holds a zeroed ticket counter).
+// Round kernels that emit a message pair send it where a RoundOut says (transcript_device.h:
+// a plain buffer, or the device transcript's round step); fold kernels take the challenge from
+// the device pointer rin instead of the value argument when rin is set.
 hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
-                               uint4* out, hipStream_t s);
+                               RoundOut ro, hipStream_t s);
 // fold_and_msg_lsb: nf/nb (n/2) = v0 + r(v0+v1); out = next message (zero if n/2 == 1).
 hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
-                              uint4* nb, uint4* partials, uint4* out, hipStream_t s);
+                              uint4* nb, uint4* partials, RoundOut ro, hipStream_t s,
+                              const uint4* rin = nullptr);
 // Zerocheck first message on (a, b) of size n with eq table e (n/2): out = (G(1), G(inf)).
 // If e_next != nullptr, e_next[z] = e[2z] ^ e[2z+1] (n/4 entries).
 hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
-                           uint4* e_next, uint4* partials, uint4* out, hipStream_t s);
+                           uint4* e_next, uint4* partials, RoundOut ro, hipStream_t s);
 // Fused: fold (a, b) of size n at rho into (ao, bo) of size n/2, then the message on the
 // folded tables with eq table e (n/4), and e_next (n/8) as above.
 hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
-                                uint4* partials, uint4* out, hipStream_t s);
+                                uint4* partials, RoundOut ro, hipStream_t s,
+                                const uint4* rin = nullptr);
 // Plain fold (fold_in_place_pair): out-of-place, n -> n/2.
 hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
-                            uint4* bo, hipStream_t s);
+                            uint4* bo, hipStream_t s, const uint4* cin = nullptr);
 // eq table build_eq(r[0..nv)) into out (2^nv entries). r is a device array; scratch holds
 // at least 2^(nv/2) + 2^(nv - nv/2) entries.
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s);
@@ -122,10 +129,11 @@ hipError_t launch_lc_partial_fold(const uint8_t* z, uint64_t K, uint64_t useful_
                                   uint64_t n_stripes, const uint4* eq_outer, uint4* partials,
                                   uint4* out, hipStream_t s);
 hipError_t launch_lc_eval(const uint4* c, const uint4* z, uint64_t len, uint4* partials,
-                          uint4* out, hipStream_t s);
+                          RoundOut ro, hipStream_t s);
 hipError_t launch_lc_bind_eval(uint4* c, uint4* z, uint64_t len, uint4 r, uint4* partials,
-                               uint4* out, hipStream_t s);
-hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s);
+                               RoundOut ro, hipStream_t s, const uint4* rin = nullptr);
+hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s,
+                          const uint4* rin = nullptr);
 hipError_t launch_dot2(const uint4* w, const uint4* a, const uint4* b, uint64_t n,
                        uint4* partials, uint4* out, hipStream_t s);
 

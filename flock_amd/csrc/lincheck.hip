@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_fill_f128_zero(f128* __restrict__ out, 
 __global__ __launch_bounds__(256) void k_lc_eval(const f128* __restrict__ c,
                                                   const f128* __restrict__ z, uint64_t half,
                                                   f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                  RoundOut ro) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u e1, ei;
@@ -186,18 +186,20 @@ __global__ __launch_bounds__(256) void k_lc_eval(const f128* __restrict__ c,
         comb_apply(e1, wl, comb_build(wl, zh), ch);
         comb_apply(ei, wl, comb_build(wl, f128_add(zh, zl)), f128_add(ch, cl));
     }
-    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, out);
+    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, ro);
 }
 
 // Bind the top bit of c and z at r in place (len -> len/2) and evaluate the next round on the
 // bound tables. Thread i reads quarters i, q+i, 2q+i, 3q+i and writes i and q+i only.
 __global__ __launch_bounds__(256) void k_lc_bind_eval(f128* __restrict__ c, f128* __restrict__ z,
                                                        uint64_t q, f128 r,
+                                                       const f128* __restrict__ rin,
                                                        f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                       RoundOut ro) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
+    if (rin) r = *rin;
     if (threadIdx.x < 32) shoup_build_row(table, r, threadIdx.x);
     __syncthreads();
     f256u e1, ei;
@@ -218,13 +220,15 @@ __global__ __launch_bounds__(256) void k_lc_bind_eval(f128* __restrict__ c, f128
         comb_apply(e1, wl, comb_build(wl, zhi), hi);
         comb_apply(ei, wl, comb_build(wl, f128_add(zhi, zlo)), f128_add(hi, lo));
     }
-    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, out);
+    grid_xor_pair(f256u_reduce(e1), f256u_reduce(ei), partials, ro);
 }
 
 // Final binding: v[i] += r (v[half + i] + v[i]) for c and z, in place.
 __global__ __launch_bounds__(256) void k_lc_bind(f128* __restrict__ c, f128* __restrict__ z,
-                                                  uint64_t half, f128 r) {
+                                                  uint64_t half, f128 r,
+                                                  const f128* __restrict__ rin) {
     __shared__ f128 table[SHOUP_ENTRIES];
+    if (rin) r = *rin;
     if (threadIdx.x < 32) shoup_build_row(table, r, threadIdx.x);
     __syncthreads();
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half;
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(256) void k_dot2(const f128* __restrict__ w, const 
         comb_apply(sa, wl, wh, a[i]);
         comb_apply(sb, wl, wh, b[i]);
     }
-    grid_xor_pair(f256u_reduce(sa), f256u_reduce(sb), partials, out);
+    grid_xor_pair(f256u_reduce(sa), f256u_reduce(sb), partials, round_out(out));
 }
 
 // ---- Keccak-f[1600] circuit marginals (KeccakLincheckCircuit, r1cs_hashes/keccak.rs:692-835) ----
@@ -439,23 +443,24 @@ hipError_t launch_lc_partial_fold(const uint8_t* z, uint64_t K, uint64_t useful_
 }
 
 hipError_t launch_lc_eval(const uint4* c, const uint4* z, uint64_t len, uint4* partials,
-                          uint4* out, hipStream_t s) {
+                          RoundOut ro, hipStream_t s) {
     if (hipError_t ae = lc_attrs(); ae != hipSuccess) return ae;
     const unsigned g = grid_for(len / 2, kMaxReduceBlocks);
-    k_lc_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 2, partials, out);
+    k_lc_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 2, partials, ro);
     return hipGetLastError();
 }
 
 hipError_t launch_lc_bind_eval(uint4* c, uint4* z, uint64_t len, uint4 r, uint4* partials,
-                               uint4* out, hipStream_t s) {
+                               RoundOut ro, hipStream_t s, const uint4* rin) {
     if (hipError_t ae = lc_attrs(); ae != hipSuccess) return ae;
     const unsigned g = grid_for(len / 4, kMaxReduceBlocks);
-    k_lc_bind_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 4, r, partials, out);
+    k_lc_bind_eval<<<g, 256, kCombWsBytes, s>>>(c, z, len / 4, r, rin, partials, ro);
     return hipGetLastError();
 }
 
-hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s) {
-    k_lc_bind<<<grid_for(len / 2, 8192), 256, 0, s>>>(c, z, len / 2, r);
+hipError_t launch_lc_bind(uint4* c, uint4* z, uint64_t len, uint4 r, hipStream_t s,
+                          const uint4* rin) {
+    k_lc_bind<<<grid_for(len / 2, 8192), 256, 0, s>>>(c, z, len / 2, r, rin);
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ static_assert(kMaxReduceBlocks == kGridReduceMax, "grid_xor_pair's counter sits 
 __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f,
                                                        const f128* __restrict__ b, uint64_t n,
                                                        f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                       RoundOut ro) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u u0, u2;
@@ -39,18 +39,20 @@ __global__ __launch_bounds__(256) void k_sc_round_msg(const f128* __restrict__ f
         comb_apply(u0, wl, comb_build(wl, b0), f0);
         comb_apply(u2, wl, comb_build(wl, f128_add(b0, b1)), f128_add(f0, f1));
     }
-    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, out);
+    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, ro);
 }
 
 __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
                                                       const f128* __restrict__ b, uint64_t n,
-                                                      f128 r, f128* __restrict__ nf,
+                                                      f128 r, const f128* __restrict__ rin,
+                                                      f128* __restrict__ nf,
                                                       f128* __restrict__ nb,
                                                       f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                      RoundOut ro) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
+    if (rin) r = *rin;
     if (threadIdx.x < 32) shoup_build_row(table, r, threadIdx.x);
     __syncthreads();
     const uint64_t half = n / 2;
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void k_sc_fold_msg(const f128* __restrict__ f,
             comb_apply(u2, wl, comb_build(wl, f128_add(c0, c1)), f128_add(g0, g1));
         }
     }
-    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, out);
+    grid_xor_pair(f256u_reduce(u0), f256u_reduce(u2), partials, ro);
 }
 
 __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
                                                    const f128* __restrict__ e,
                                                    f128* __restrict__ e_next,
                                                    f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                   RoundOut ro) {
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
     f256u g1, gi;
@@ -105,20 +107,22 @@ __global__ __launch_bounds__(256) void k_zc_round(const f128* __restrict__ a,
         comb_apply(gi, wl, eh, pi);
         if (e_next && !(x & 1) && x + 1 < half) e_next[x >> 1] = f128_add(ex, e[x + 1]);
     }
-    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, out);
+    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, ro);
 }
 
 __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ a,
                                                         const f128* __restrict__ b, uint64_t n,
-                                                        f128 rho, f128* __restrict__ ao,
+                                                        f128 rho, const f128* __restrict__ rin,
+                                                        f128* __restrict__ ao,
                                                         f128* __restrict__ bo,
                                                         const f128* __restrict__ e,
                                                         f128* __restrict__ e_next,
                                                         f128* __restrict__ partials,
-        f128* __restrict__ out) {
+                                                        RoundOut ro) {
     __shared__ f128 table[SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
     f128* wl = comb_lane(ws);
+    if (rin) rho = *rin;
     if (threadIdx.x < 32) shoup_build_row(table, rho, threadIdx.x);
     __syncthreads();
     f256u g1, gi;
@@ -145,13 +149,15 @@ __global__ __launch_bounds__(256) void k_zc_fold_round(const f128* __restrict__ 
         comb_apply(gi, wl, eh, mi);
         if (e_next && !(y & 1) && y + 1 < quarter) e_next[y >> 1] = f128_add(ey, e[y + 1]);
     }
-    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, out);
+    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, ro);
 }
 
 __global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
                                                     const f128* __restrict__ b, uint64_t n, f128 c,
+                                                    const f128* __restrict__ cin,
                                                     f128* __restrict__ ao, f128* __restrict__ bo) {
     __shared__ f128 table[SHOUP_ENTRIES];
+    if (cin) c = *cin;
     if (threadIdx.x < 32) shoup_build_row(table, c, threadIdx.x);
     __syncthreads();
     for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n / 2;
@@ -233,44 +239,47 @@ static inline unsigned red_grid(uint64_t work) {
 }
 
 hipError_t launch_sc_round_msg(const uint4* f, const uint4* b, uint64_t n, uint4* partials,
-                               uint4* out, hipStream_t s) {
+                               RoundOut ro, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_sc_round_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, partials, out);
+    k_sc_round_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, partials, ro);
     return hipGetLastError();
 }
 
 hipError_t launch_sc_fold_msg(const uint4* f, const uint4* b, uint64_t n, uint4 r, uint4* nf,
-                              uint4* nb, uint4* partials, uint4* out, hipStream_t s) {
+                              uint4* nb, uint4* partials, RoundOut ro, hipStream_t s,
+                              const uint4* rin) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_sc_fold_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, r, nf, nb, partials, out);
+    k_sc_fold_msg<<<g, 256, kCombWsBytes, s>>>(f, b, n, r, rin, nf, nb, partials, ro);
     return hipGetLastError();
 }
 
 hipError_t launch_zc_round(const uint4* a, const uint4* b, uint64_t n, const uint4* e,
-                           uint4* e_next, uint4* partials, uint4* out, hipStream_t s) {
+                           uint4* e_next, uint4* partials, RoundOut ro, hipStream_t s) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 2);
-    k_zc_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, e, e_next, partials, out);
+    k_zc_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, e, e_next, partials, ro);
     return hipGetLastError();
 }
 
 hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
-                                uint4* partials, uint4* out, hipStream_t s) {
+                                uint4* partials, RoundOut ro, hipStream_t s,
+                                const uint4* rin) {
     if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
     const unsigned g = red_grid(n / 4);
-    k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, ao, bo, e, e_next, partials, out);
+    k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, rin, ao, bo, e, e_next, partials,
+                                                 ro);
     return hipGetLastError();
 }
 
 hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
-                            uint4* bo, hipStream_t s) {
+                            uint4* bo, hipStream_t s, const uint4* cin) {
     uint64_t g = (n / 2 + 255) / 256;
     if (g < 1) g = 1;
     if (g > 8192) g = 8192;
-    k_fold_pair<<<(unsigned)g, 256, 0, s>>>(a, b, n, c, ao, bo);
+    k_fold_pair<<<(unsigned)g, 256, 0, s>>>(a, b, n, c, cin, ao, bo);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@ E_NOMEM = 2
 E_NODEVICE = 3
 E_HIP = 100
 E_M_TOO_SMALL = 101
+E_INTERNAL = 102
 
 
 class FlockHipError(RuntimeError):
@@ -100,6 +101,7 @@ class PcsParams(ctypes.Structure):
 
 HASH_SHA256, HASH_BLAKE3, HASH_KECCAK = 0, 1, 2
 MERKLE_SHA256, MERKLE_BLAKE3 = 0, 1
+ROUNDS_HOST, ROUNDS_DEVICE = 0, 2
 
 
 class SparseMatrix(ctypes.Structure):
@@ -168,6 +170,7 @@ def load():
         "flock_hip_ctx_destroy": (None, [P]),
         "flock_hip_ctx_stream": (P, [P]),
         "flock_hip_sync": (I, [P]),
+        "flock_hip_ctx_set_round_mode": (I, [P, I]),
         "flock_hip_malloc": (I, [P, ctypes.POINTER(P), S]),
         "flock_hip_dfree": (I, [P, P]),
         "flock_hip_memcpy_h2d": (I, [P, P, P, S]),
@@ -353,6 +356,10 @@ class Context:
 
     def sync(self):
         _check(load().flock_hip_sync(self.h))
+
+    def set_round_mode(self, mode: int):
+        """flock_hip_ctx_set_round_mode: ROUNDS_HOST (default) or ROUNDS_DEVICE."""
+        _check(load().flock_hip_ctx_set_round_mode(self.h, mode))
 
     @property
     def stream(self) -> int:
@@ -712,6 +719,11 @@ class Prover:
         h = ctypes.c_void_p()
         _check(load().flock_hip_prover_create(device, ctypes.byref(h)))
         self.h = h
+
+    def set_round_mode(self, mode: int):
+        """flock_hip_ctx_set_round_mode on the prover's context."""
+        ctx = load().flock_hip_prover_ctx(self.h)
+        _check(load().flock_hip_ctx_set_round_mode(ctx, mode))
 
     def close(self):
         if self.h:

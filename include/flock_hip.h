@@ -35,6 +35,7 @@ extern "C" {
 #define FLOCK_HIP_E_NODEVICE 3      /* no usable gfx950 device */
 #define FLOCK_HIP_E_HIP 100         /* HIP runtime error (prove_ffi.cu:44-46 uses 100 for CUDA) */
 #define FLOCK_HIP_E_M_TOO_SMALL 101 /* m below the commit minimum (prove_ffi.cu:429) */
+#define FLOCK_HIP_E_INTERNAL 102    /* a device result failed its host-side check */
 
 typedef struct {
     uint64_t lo, hi;
@@ -56,6 +57,20 @@ void flock_hip_ctx_destroy(flock_hip_ctx* ctx);
 /* The context's hipStream_t (as void*), for callers that order their own work on it. */
 void* flock_hip_ctx_stream(flock_hip_ctx* ctx);
 int flock_hip_sync(flock_hip_ctx* ctx);
+/* How the round loops of flock_hip_zerocheck_prove, flock_hip_lincheck_prove{,_keccak} and
+ * flock_hip_prove_sumcheck meet the transcript between rounds. The proofs and the final
+ * transcript are the same in both modes.
+ *  FLOCK_ROUNDS_HOST:   (default) per round, one stream synchronisation, the transcript step
+ *                       on the host, and one launch.
+ *  FLOCK_ROUNDS_DEVICE: a device-resident copy of the transcript observes each message, grinds
+ *                       and samples the next challenge inside the round kernel, so the whole
+ *                       loop is enqueued at once and runs without the host; the host then
+ *                       replays the logged rounds through its transcript and checks every nonce
+ *                       and challenge (FLOCK_HIP_E_INTERNAL on a mismatch). The analogue of the
+ *                       reference's device challenger (cuda-ghash/zc_challenger_device.cuh). */
+#define FLOCK_ROUNDS_HOST 0
+#define FLOCK_ROUNDS_DEVICE 2
+int flock_hip_ctx_set_round_mode(flock_hip_ctx* ctx, int mode);
 int flock_hip_malloc(flock_hip_ctx* ctx, void** d_ptr, size_t bytes);
 int flock_hip_dfree(flock_hip_ctx* ctx, void* d_ptr);
 int flock_hip_memcpy_h2d(flock_hip_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
