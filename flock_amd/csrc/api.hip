@@ -1192,6 +1192,9 @@ int flock_hip_fold_1b_rows(flock_hip_ctx* c, const flock_f128* rows, size_t n_ro
     return 0;
 }
 
+// Tail rounds whose eq table has at least this many variables use the split-eq kernel.
+constexpr uint32_t kZcSplitMinVars = 16;
+
 static uint32_t bits_for(uint64_t n) {  // floor(log2 n) + 1, zerocheck.rs:91-94
     uint32_t b = 0;
     while (n) {
@@ -1220,7 +1223,13 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     // E9 (N/8), fold table (2048), r (64), build_eq scratch
     const uint64_t h = N / 2, q = N / 4 > 0 ? N / 4 : 1, e9n = N / 8;
     const uint64_t eqs = 2 * (1ull << (n_mlv / 2 + 1));
-    const uint64_t words = 2 * N + 2 * h + 2 * q + h + q + e9n + 2048 + 64 + eqs;
+    // Tail rounds j < j0, whose eq table (over r[7+j..m), nv = n_mlv-1-j variables) has
+    // nv >= 16, run split: eq = e_lo (8 variables, EL) x e_hi (nv-8 variables, level j of EH0).
+    const uint32_t nv0 = n_mlv - 1;
+    const uint32_t j0 = nv0 >= kZcSplitMinVars ? nv0 - kZcSplitMinVars + 1 : 0;
+    const uint64_t eh0 = j0 ? 1ull << (nv0 - 8) : 0;
+    const uint64_t words = 2 * N + 2 * h + 2 * q + h + q + e9n + 2048 + 64 + eqs + 2 * eh0 +
+                           256ull * j0;
     if (int rc = c->ensure_scratch(words * 16)) return rc;
     uint4* base = reinterpret_cast<uint4*>(c->scratch);
     uint4* Az = base;
@@ -1235,8 +1244,17 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     uint4* TBL = E9 + e9n;
     uint4* R = TBL + 2048;
     uint4* EQS = R + 64;
+    uint4* EH0 = EQS + eqs;   // e_hi of round 0: eq over r[15..m)
+    uint4* EHL = EH0 + eh0;   // its levels 1..j0-1 (e_hi of rounds 1..j0-1)
+    uint4* EL = EHL + eh0;    // e_lo of rounds 0..j0-1: eq over r[7+j .. 15+j)
     uint4* bufs[2][2] = {{P0f, P0b}, {P1f, P1b}};
     uint4* eqb[2] = {EB, EC};
+    auto e_hi = [&](uint32_t j) -> const uint4* {
+        if (j == 0) return EH0;
+        uint64_t off = 0;
+        for (uint32_t i = 1; i < j; i++) off += eh0 >> i;
+        return EHL + off;
+    };
 
     // ---- 1. r: r_skip | 7 fixed inner constants | r_outer (zerocheck.rs:644-669) ----
     int ni = 0;
@@ -1259,6 +1277,12 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     HIP_TRY(hipMemcpyAsync(R, r.data() + ks, n_mlv * 16, hipMemcpyHostToDevice, c->stream));
     LAUNCH(flock::launch_build_eq(R + 1, (int)n_mlv - 1, EB, EQS, c->stream));
     LAUNCH(flock::launch_build_eq(R + 3, (int)n_mlv - 3, E9, EQS, c->stream));
+    if (j0) {
+        // the split tables of rounds 0..j0-1 and the full table of round j0
+        LAUNCH(flock::launch_build_eq(R + 9, (int)(nv0 - 8), EH0, EQS, c->stream));
+        LAUNCH(flock::launch_eq_levels(EH0, nv0 - 8, j0, EHL, c->stream));
+        LAUNCH(flock::launch_eq_windows(R + 1, j0, EL, c->stream));
+    }
 
     // ---- 2. round 1 ----
     // p_ab over groups of 8 rows (r[6..9) are phi8 constants); shat[lane | b7 << 6] =
@@ -1301,15 +1325,22 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     // ---- 3-5. round 2's message, the tail (zerocheck.rs:790-947) and the final binding
     // (:955-975): round j < n_mlv folds the previous tables at challenge j-1 (j > 0) and emits
     // message j; launch n_mlv is the closing fold ----
-    // the closing fold writes bufs[(n_mlv - 1) & 1] (launches may come from a cached graph, so
-    // nothing is recorded inside the launch callback)
+    // round j0's eq table (the first unsplit round): eq over r[7+j0..m), into eqb[j0 & 1]
+    if (j0)
+        LAUNCH(flock::launch_build_eq(R + 1 + j0, (int)(nv0 - j0), eqb[j0 & 1], EQS, c->stream));
+    // the closing fold writes bufs[(n_mlv - 1) & 1]
     uint4* last_f = bufs[(n_mlv - 1) & 1][0];
     uint4* last_b = bufs[(n_mlv - 1) & 1][1];
     RoundLoop loop{c, fs, n_mlv, grinding ? bits_for(2) : 0u,
                    [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 rho) -> int {
                        if (j == 0) {
-                           LAUNCH(flock::launch_zc_round(Az, Bz, N, EB, N >= 4 ? EC : nullptr,
-                                                         c->partials, ro, c->stream));
+                           if (j0)
+                               LAUNCH(flock::launch_zc_round_split(Az, Bz, N, EL, e_hi(0),
+                                                                   c->partials, ro, c->stream));
+                           else
+                               LAUNCH(flock::launch_zc_round(Az, Bz, N, EB,
+                                                             N >= 4 ? EC : nullptr, c->partials,
+                                                             ro, c->stream));
                            return 0;
                        }
                        const uint32_t i = j - 1;  // the fold's index
@@ -1318,7 +1349,11 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
                        const uint64_t len = N >> i;
                        uint4* nf = bufs[i & 1][0];
                        uint4* nb = bufs[i & 1][1];
-                       if (j < n_mlv) {
+                       if (j < j0) {
+                           LAUNCH(flock::launch_zc_fold_round_split(
+                               cf, cb, len, rho, nf, nb, EL + 256ull * j, e_hi(j), c->partials,
+                               ro, c->stream, rin));
+                       } else if (j < n_mlv) {
                            const int ep = 1 ^ (int)(i & 1);
                            LAUNCH(flock::launch_zc_fold_round(
                                cf, cb, len, rho, nf, nb, eqb[ep], len >= 8 ? eqb[ep ^ 1] : nullptr,

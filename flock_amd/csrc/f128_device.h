@@ -309,15 +309,26 @@ __device__ __forceinline__ f128* comb_lane(f128* ws) {
 // ticket folds every pair (volatile loads, past any stale L1 line), writes out and resets
 // the counter. gridDim.x <= kGridReduceMax.
 constexpr int kGridReduceMax = 512;
+
+// LDS a grid_xor_pair works in: per-wave pairs, the final pair, the last-block flag, and the
+// device transcript step's state.
+struct PairShared {
+    f128 red[2][4];  // workgroups of at most 256 threads
+    f128 fin[2];
+    int last;
+    flock::FsShared fs;
+};
+
 // The pair goes where `ro` says (flock::RoundOut): into ro.out when set, then ro.seq into
 // ro.flag (system scope) when set; with ro.fs the workgroup that holds the final pair also runs
-// the device transcript's round step on it.
-__device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials,
-                                              flock::RoundOut ro) {
+// the device transcript's round step on it. `sh` is LDS of the workgroup (a kernel whose
+// dynamic LDS is free by then passes that, so the reduction adds no static LDS).
+__device__ __forceinline__ void grid_xor_pair_in(f128 a0, f128 a1, f128* partials,
+                                                 flock::RoundOut ro, PairShared* sh) {
     f128* out = ro.out;
-    __shared__ f128 red[2][16];
-    __shared__ f128 fin[2];
-    __shared__ int last;
+    auto& red = sh->red;
+    auto& fin = sh->fin;
+    int& last = sh->last;
     a0 = wave_xor_f128(a0);
     a1 = wave_xor_f128(a1);
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -392,7 +403,12 @@ __device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials,
     }
     if (ro.fs) {
         __syncthreads();
-        flock::fs_round_step(ro.fs, fin[0], fin[1]);
+        flock::fs_round_step(ro.fs, fin[0], fin[1], &sh->fs);
     }
 }
 
+__device__ __forceinline__ void grid_xor_pair(f128 a0, f128 a1, f128* partials,
+                                              flock::RoundOut ro) {
+    __shared__ PairShared sh;
+    grid_xor_pair_in(a0, a1, partials, ro, &sh);
+}

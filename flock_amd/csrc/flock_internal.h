@@ -68,6 +68,22 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
                                 uint4* partials, RoundOut ro, hipStream_t s,
                                 const uint4* rin = nullptr);
+// Split-eq message for rounds whose eq table has >= 16 variables: eq(x) = e_lo[x & 255] *
+// e_hi[x >> 8]; n (the table length) is a multiple of 512.
+hipError_t launch_zc_round_split(const uint4* a, const uint4* b, uint64_t n, const uint4* e_lo,
+                                 const uint4* e_hi, uint4* partials, RoundOut ro, hipStream_t s);
+// Fused fold at rho (n -> n/2) and the split-eq message on the folded tables; n is a multiple
+// of 1024.
+hipError_t launch_zc_fold_round_split(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
+                                      uint4* ao, uint4* bo, const uint4* e_lo, const uint4* e_hi,
+                                      uint4* partials, RoundOut ro, hipStream_t s,
+                                      const uint4* rin = nullptr);
+// out[j * 256 + t] = eq(r[j .. j+8), t) for j < count (r: count + 7 device words).
+hipError_t launch_eq_windows(const uint4* r, uint32_t count, uint4* out, hipStream_t s);
+// Levels 1..levels-1 of the eq table e0 over nv variables (level j drops the j lowest
+// variables), packed one after another into out.
+hipError_t launch_eq_levels(const uint4* e0, uint32_t nv, uint32_t levels, uint4* out,
+                            hipStream_t s);
 // Plain fold (fold_in_place_pair): out-of-place, n -> n/2.
 hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c, uint4* ao,
                             uint4* bo, hipStream_t s, const uint4* cin = nullptr);

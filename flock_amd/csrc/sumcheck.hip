@@ -168,6 +168,171 @@ __global__ __launch_bounds__(256) void k_fold_pair(const f128* __restrict__ a,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-eq zerocheck rounds. For a round whose eq table has nv >= 16 variables,
+// eq(x) = e_lo[x & 255] * e_hi[x >> 8]. Thread t of a 256-thread workgroup keeps x_lo = t and
+// the workgroup walks x_hi together, so e_hi[x_hi] is one multiplier per step: its Shoup table
+// is built once per step by the whole workgroup, and each thread accumulates
+// sum_{x_hi} e_hi[x_hi] * P(x) with table lookups, then multiplies by e_lo[t] once at the end.
+// Per pair this replaces a per-lane comb table of eq(x) and two comb products with two Shoup
+// products. The split tables come from k_eq_windows / k_eq_levels.
+// ---------------------------------------------------------------------------
+
+// The Shoup table of t (SHOUP_ENTRIES words) by all 256 threads of the workgroup: row = t / 8,
+// two entries per thread.
+__device__ __forceinline__ void shoup_build_block(f128* table, f128 t) {
+    const int row = threadIdx.x >> 3, e = (threadIdx.x & 7) * 2;
+    const f128 e1 = f128_shl_mod(t, 4 * row);
+    const f128 e2 = f128_mul_x(e1);
+    const f128 e4 = f128_mul_x(e2);
+    const f128 e8 = f128_mul_x(e4);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int n = e + k;
+        f128 v = f128_zero();
+        if (n & 1) v = f128_add(v, e1);
+        if (n & 2) v = f128_add(v, e2);
+        if (n & 4) v = f128_add(v, e4);
+        if (n & 8) v = f128_add(v, e8);
+        table[16 * row + n] = v;
+    }
+}
+
+// Message of (a, b) (n = 2 * half words each): (sum eq*a1*b1, sum eq*(a0+a1)(b0+b1)),
+// half = 256 * n_hi.
+__global__ __launch_bounds__(256) void k_zc_round_split(const f128* __restrict__ a,
+                                                         const f128* __restrict__ b,
+                                                         uint64_t half,
+                                                         const f128* __restrict__ e_lo,
+                                                         const f128* __restrict__ e_hi,
+                                                         f128* __restrict__ partials,
+                                                         RoundOut ro) {
+    // one 8 KiB table, so that the workgroup (with the comb workspace) stays under 80 KiB of
+    // LDS, two per CU; two barriers per step
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    f128 s1 = f128_zero(), si = f128_zero();
+    const uint64_t n_hi = half >> 8;
+    for (uint64_t xh = blockIdx.x; xh < n_hi; xh += gridDim.x) {
+        __syncthreads();  // the previous step's lookups into tab are done
+        shoup_build_block(tab, e_hi[xh]);
+        __syncthreads();
+        const uint64_t x = (xh << 8) + threadIdx.x;
+        const f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+        const f128 p1 = comb_mul(wl, a1, b1);
+        const f128 pi = comb_mul(wl, f128_add(a0, a1), f128_add(b0, b1));
+        s1 = f128_add(s1, shoup_mul(tab, p1));
+        si = f128_add(si, shoup_mul(tab, pi));
+    }
+    const uint32_t eh = comb_build(wl, e_lo[threadIdx.x]);
+    f256u g1, gi;
+    f256u_zero(g1);
+    f256u_zero(gi);
+    comb_apply(g1, wl, eh, s1);
+    comb_apply(gi, wl, eh, si);
+    grid_xor_pair(f256u_reduce(g1), f256u_reduce(gi), partials, ro);
+}
+
+// Fused fold at rho (n -> n/2) and the split-eq message on the folded tables; quarter = n / 4
+// = 256 * n_hi output pairs. Static LDS is the rho table and the e_hi table only (16 KiB), and
+// the final reduction runs in the comb workspace once it is free: with the 64 KiB workspace the
+// workgroup takes exactly 80 KiB, two per CU.
+__global__ __launch_bounds__(256) void k_zc_fold_round_split(const f128* __restrict__ a,
+                                                              const f128* __restrict__ b,
+                                                              uint64_t quarter, f128 rho,
+                                                              const f128* __restrict__ rin,
+                                                              f128* __restrict__ ao,
+                                                              f128* __restrict__ bo,
+                                                              const f128* __restrict__ e_lo,
+                                                              const f128* __restrict__ e_hi,
+                                                              f128* __restrict__ partials,
+                                                              RoundOut ro) {
+    __shared__ f128 rt[SHOUP_ENTRIES];
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    if (rin) rho = *rin;
+    if (threadIdx.x < 32) shoup_build_row(rt, rho, threadIdx.x);
+    f128 s1 = f128_zero(), si = f128_zero();
+    const uint64_t n_hi = quarter >> 8;
+    for (uint64_t yh = blockIdx.x; yh < n_hi; yh += gridDim.x) {
+        __syncthreads();  // the previous step's lookups into tab are done (and rt is built)
+        shoup_build_block(tab, e_hi[yh]);
+        __syncthreads();
+        const uint64_t y = (yh << 8) + threadIdx.x;
+        const f128 a0 = a[4 * y], a1 = a[4 * y + 1], a2 = a[4 * y + 2], a3 = a[4 * y + 3];
+        const f128 b0 = b[4 * y], b1 = b[4 * y + 1], b2 = b[4 * y + 2], b3 = b[4 * y + 3];
+        const f128 p0 = f128_add(a0, shoup_mul(rt, f128_add(a0, a1)));
+        const f128 p1 = f128_add(a2, shoup_mul(rt, f128_add(a2, a3)));
+        const f128 q0 = f128_add(b0, shoup_mul(rt, f128_add(b0, b1)));
+        const f128 q1 = f128_add(b2, shoup_mul(rt, f128_add(b2, b3)));
+        ao[2 * y] = p0;
+        ao[2 * y + 1] = p1;
+        bo[2 * y] = q0;
+        bo[2 * y + 1] = q1;
+        const f128 m1 = comb_mul(wl, p1, q1);
+        const f128 mi = comb_mul(wl, f128_add(p0, p1), f128_add(q0, q1));
+        s1 = f128_add(s1, shoup_mul(tab, m1));
+        si = f128_add(si, shoup_mul(tab, mi));
+    }
+    const uint32_t eh = comb_build(wl, e_lo[threadIdx.x]);
+    f256u g1, gi;
+    f256u_zero(g1);
+    f256u_zero(gi);
+    comb_apply(g1, wl, eh, s1);
+    comb_apply(gi, wl, eh, si);
+    const f128 r1 = f256u_reduce(g1), ri = f256u_reduce(gi);
+    __syncthreads();  // every wave is done with the comb workspace
+    grid_xor_pair_in(r1, ri, partials, ro, reinterpret_cast<PairShared*>(ws));
+}
+
+// out[j * 256 + t] = eq(r[j .. j+8), t) for j < count: one workgroup per window.
+__global__ __launch_bounds__(256) void k_eq_windows(const f128* __restrict__ r, uint32_t count,
+                                                     f128* __restrict__ out) {
+    __shared__ f128 tabs[8][16];
+    __shared__ uint32_t his[8];
+    const uint32_t j = blockIdx.x;
+    if (threadIdx.x < 8) {
+        tabs[threadIdx.x][0] = f128_zero();
+        his[threadIdx.x] = comb_build<1>(tabs[threadIdx.x], r[j + threadIdx.x]);
+    }
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    f128 acc = f128_one();
+#pragma unroll 1
+    for (int i = 0; i < 8; i++) {
+        f256u p;
+        f256u_zero(p);
+        comb_apply<1>(p, tabs[i], his[i], acc);
+        const f128 ar = f256u_reduce(p);  // acc * r_i
+        acc = ((t >> i) & 1) ? ar : f128_add(acc, ar);
+    }
+    out[(uint64_t)j * 256 + t] = acc;
+}
+
+// Levels 1..levels-1 of an eq table over nv variables: level j is the table without its j
+// lowest variables, entry y = XOR of e0[y * 2^j .. (y + 1) * 2^j) (the lowest variables'
+// eq factors sum to one). Level j starts at out + sum_{1 <= i < j} 2^(nv - i).
+__global__ __launch_bounds__(256) void k_eq_levels(const f128* __restrict__ e0, uint32_t nv,
+                                                    uint32_t levels, f128* __restrict__ out) {
+    uint64_t total = 0;
+    for (uint32_t j = 1; j < levels; j++) total += 1ull << (nv - j);
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t j = 1;
+        uint64_t off = 0;
+        while (g >= off + (1ull << (nv - j))) {
+            off += 1ull << (nv - j);
+            j++;
+        }
+        const uint64_t y = g - off;
+        f128 v = f128_zero();
+        for (uint64_t z = 0; z < (1ull << j); z++) v = f128_add(v, e0[(y << j) + z]);
+        out[g] = v;
+    }
+}
+
 // eq(r[0..nv), x) for every x as a product of nv factors (nv <= kEqDirectMax). Every factor
 // multiply is by a fixed r_i: acc * (1 + r_i) = acc + acc * r_i, with acc * r_i through a
 // 16-entry comb table of r_i shared by the workgroup, so no thread runs a chain of generic
@@ -231,6 +396,15 @@ static hipError_t init_attrs() {
     return st;
 }
 
+static hipError_t split_attrs() {
+    static hipError_t st = [] {
+        hipError_t e = set_ws_attr((const void*)k_zc_round_split);
+        if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_fold_round_split);
+        return e;
+    }();
+    return st;
+}
+
 static inline unsigned red_grid(uint64_t work) {
     uint64_t g = (work + 255) / 256;
     if (g < 1) g = 1;
@@ -271,6 +445,48 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
     const unsigned g = red_grid(n / 4);
     k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, rin, ao, bo, e, e_next, partials,
                                                  ro);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_round_split(const uint4* a, const uint4* b, uint64_t n, const uint4* e_lo,
+                                 const uint4* e_hi, uint4* partials, RoundOut ro, hipStream_t s) {
+    if (n < 512 || (n & 511)) return hipErrorInvalidValue;
+    if (hipError_t ae = split_attrs(); ae != hipSuccess) return ae;
+    // one workgroup per x_hi step, at most kMaxReduceBlocks (all resident)
+    const uint64_t steps = (n / 2) >> 8;
+    const unsigned g = (unsigned)(steps < (uint64_t)kMaxReduceBlocks ? steps : kMaxReduceBlocks);
+    k_zc_round_split<<<g, 256, kCombWsBytes, s>>>(a, b, n / 2, e_lo, e_hi, partials, ro);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_fold_round_split(const uint4* a, const uint4* b, uint64_t n, uint4 rho,
+                                      uint4* ao, uint4* bo, const uint4* e_lo, const uint4* e_hi,
+                                      uint4* partials, RoundOut ro, hipStream_t s,
+                                      const uint4* rin) {
+    if (n < 1024 || (n & 1023)) return hipErrorInvalidValue;
+    if (hipError_t ae = split_attrs(); ae != hipSuccess) return ae;
+    const uint64_t steps = (n / 4) >> 8;
+    const unsigned g = (unsigned)(steps < (uint64_t)kMaxReduceBlocks ? steps : kMaxReduceBlocks);
+    k_zc_fold_round_split<<<g, 256, kCombWsBytes, s>>>(a, b, n / 4, rho, rin, ao, bo, e_lo, e_hi,
+                                                       partials, ro);
+    return hipGetLastError();
+}
+
+hipError_t launch_eq_windows(const uint4* r, uint32_t count, uint4* out, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    k_eq_windows<<<count, 256, 0, s>>>(r, count, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_eq_levels(const uint4* e0, uint32_t nv, uint32_t levels, uint4* out,
+                            hipStream_t s) {
+    if (levels < 2) return hipSuccess;
+    if (levels > nv + 1) return hipErrorInvalidValue;
+    uint64_t total = 0;
+    for (uint32_t j = 1; j < levels; j++) total += 1ull << (nv - j);
+    uint64_t g = (total + 255) / 256;
+    if (g > 4096) g = 4096;
+    k_eq_levels<<<(unsigned)g, 256, 0, s>>>(e0, nv, levels, out);
     return hipGetLastError();
 }
 

@@ -152,12 +152,19 @@ __device__ __forceinline__ bool fs_pow_ok(const uint32_t d[8], uint64_t nonce, u
     return bits >= 32 ? h[0] == 0u : (h[0] >> (32 - bits)) == 0u;
 }
 
+// LDS the round step works in: the transcript state, the PoW digest and the best nonce.
+struct FsShared {
+    FsLocal st;
+    uint32_t digest[8];
+    unsigned long long found;
+};
+
 // The round step; called by every thread of one workgroup with the message pair valid in
-// thread 0.
-__device__ __noinline__ void fs_round_step(DevTranscript* t, uint4 m0, uint4 m1) {
-    __shared__ FsLocal st;
-    __shared__ uint32_t digest[8];
-    __shared__ unsigned long long found;
+// thread 0, working in the workgroup's LDS `sh`.
+__device__ __noinline__ void fs_round_step(DevTranscript* t, uint4 m0, uint4 m1, FsShared* sh) {
+    FsLocal& st = sh->st;
+    uint32_t* digest = sh->digest;
+    unsigned long long& found = sh->found;
     const uint32_t bits = t->grind_bits;
     if (threadIdx.x == 0) {
         FsLocal s;
