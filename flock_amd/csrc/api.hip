@@ -1397,7 +1397,12 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     // n/4), eq challenges and build_eq scratch.
     const size_t h = n / 2 > 0 ? n / 2 : 1, q = n / 4 > 0 ? n / 4 : 1;
     const size_t eq_scratch = 64 + ((size_t)1 << (log_n / 2 + 1)) + ((size_t)1 << (log_n / 2 + 1));
-    const size_t words = 2 * h + 2 * q + h + q + eq_scratch;
+    // zerocheck rounds j < j0 (eq over r[1+j..log_n) with >= 16 variables) run split, as in
+    // flock_hip_zerocheck_prove: e_lo windows EL, e_hi = level j of EH0 = eq over r[9..log_n)
+    const uint32_t nv0 = log_n - 1;
+    const uint32_t j0 = nv0 >= kZcSplitMinVars ? nv0 - kZcSplitMinVars + 1 : 0;
+    const size_t eh0 = j0 ? (size_t)1 << (nv0 - 8) : 0;
+    const size_t words = 2 * h + 2 * q + h + q + eq_scratch + 2 * eh0 + 256 * (size_t)j0;
     if (int rc = c->ensure_scratch(words * 16)) return rc;
     uint4* base = reinterpret_cast<uint4*>(c->scratch);
     uint4* P0f = base;
@@ -1408,10 +1413,18 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     uint4* E1 = E0 + h;
     uint4* R = E1 + q;  // up to 64 challenges
     uint4* EQS = R + 64;
+    uint4* EH0 = EQS + (eq_scratch - 64);
+    uint4* EHL = EH0 + eh0;
+    uint4* EL = EHL + eh0;
     uint4* bufs[2][2] = {{P0f, P0b}, {P1f, P1b}};
     uint4* eqb[2] = {E0, E1};
+    auto e_hi = [&](uint32_t j) -> const uint4* {
+        if (j == 0) return EH0;
+        size_t off = 0;
+        for (uint32_t i = 1; i < j; i++) off += eh0 >> i;
+        return EHL + off;
+    };
     int mi = 0;
-    flock_f128 m2[2];
 
     // ---- product sumcheck: round j < log_n folds at challenge j-1 (j > 0) and emits message j;
     // launch log_n is the closing fold ----
@@ -1456,13 +1469,25 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     flock_fs_sample_f128_vec(fs, r.data(), log_n);
     if (log_n > 1) HIP_TRY(hipMemcpyAsync(R, r.data() + 1, (log_n - 1) * 16, hipMemcpyHostToDevice,
                                           c->stream));
-    LAUNCH(flock::launch_build_eq(R, (int)log_n - 1, E0, EQS, c->stream));
+    if (j0) {
+        // split tables of rounds 0..j0-1, and the full table of round j0 (eq over r[1+j0..))
+        LAUNCH(flock::launch_build_eq(R + 8, (int)(nv0 - 8), EH0, EQS, c->stream));
+        LAUNCH(flock::launch_eq_levels(EH0, nv0 - 8, j0, EHL, c->stream));
+        LAUNCH(flock::launch_eq_windows(R, j0, EL, c->stream));
+        LAUNCH(flock::launch_build_eq(R + j0, (int)(nv0 - j0), eqb[j0 & 1], EQS, c->stream));
+    } else {
+        LAUNCH(flock::launch_build_eq(R, (int)log_n - 1, E0, EQS, c->stream));
+    }
     RoundLoop zc{c, fs, log_n, 0u,
                  [&](uint32_t j, flock::RoundOut ro, const uint4* rin, uint4 rho) -> int {
                      if (j == 0) {
-                         LAUNCH(flock::launch_zc_round(U(d_a), U(d_b), n, E0,
-                                                       n >= 4 ? E1 : nullptr, c->partials, ro,
-                                                       c->stream));
+                         if (j0)
+                             LAUNCH(flock::launch_zc_round_split(U(d_a), U(d_b), n, EL, e_hi(0),
+                                                                 c->partials, ro, c->stream));
+                         else
+                             LAUNCH(flock::launch_zc_round(U(d_a), U(d_b), n, E0,
+                                                           n >= 4 ? E1 : nullptr, c->partials,
+                                                           ro, c->stream));
                          return 0;
                      }
                      const uint32_t i = j - 1;
@@ -1471,7 +1496,12 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
                      const size_t len = n >> i;
                      uint4* nf = bufs[i & 1][0];
                      uint4* nb = bufs[i & 1][1];
-                     if (j < log_n) {
+                     if (j < j0) {
+                         LAUNCH(flock::launch_zc_fold_round_split(cf, cb, len, rho, nf, nb,
+                                                                  EL + 256ull * j, e_hi(j),
+                                                                  c->partials, ro, c->stream,
+                                                                  rin));
+                     } else if (j < log_n) {
                          const int ep = 1 ^ (int)(i & 1);  // eq table of the next message
                          LAUNCH(flock::launch_zc_fold_round(
                              cf, cb, len, rho, nf, nb, eqb[ep], len >= 8 ? eqb[ep ^ 1] : nullptr,

@@ -273,7 +273,7 @@ def test_zerocheck_rounds(ctx, log_n):
             assert np.array_equal(nxt.download_f128(n // 8), O.build_eq(r[3:]))
 
 
-@pytest.mark.parametrize("log_n", [1, 2, 3, 6, 11, 16])
+@pytest.mark.parametrize("log_n", [1, 2, 3, 6, 11, 16, 17, 18])
 def test_prove_sumcheck_transcript(ctx, log_n):
     say(f"case: prove_sumcheck log_n={log_n}")
     n = 1 << log_n
