@@ -375,6 +375,31 @@ void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n) {
 void flock_fs_digest(flock_fs* fs, uint8_t out[32]) { fs->st.digest(out); }
 // PoW (challenger.rs:884-958, 1096-1135): smallest nonce with SHA256(digest || nonce_le)
 // having `bits` leading zero bits (MSB-first), then observe_bytes(nonce_le).
+// Challenger::verify_pow (challenger.rs:960-981): bits = 0 accepts only nonce 0; otherwise
+// SHA256(digest || nonce_le) must start with `bits` zero bits. The nonce bytes are absorbed
+// either way.
+int flock_fs_verify_pow(flock_fs* fs, uint64_t nonce, uint32_t bits) {
+    uint8_t digest[32];
+    fs->st.digest(digest);
+    bool ok = nonce == 0;
+    if (bits) {
+        uint8_t pre[40], h[32];
+        memcpy(pre, digest, 32);
+        for (int i = 0; i < 8; i++) pre[32 + i] = (uint8_t)(nonce >> (8 * i));
+        Sha256 sh;
+        sh.update(pre, 40);
+        sh.digest(h);
+        const uint32_t full = bits / 8, extra = bits % 8;
+        ok = full <= 32;
+        for (uint32_t i = 0; i < full && ok; i++) ok = h[i] == 0;
+        if (ok && extra && full < 32) ok = (h[full] >> (8 - extra)) == 0;
+    }
+    uint8_t nb[8];
+    for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
+    flock_fs_observe_bytes(fs, nb, 8);
+    return ok ? 1 : 0;
+}
+
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
     uint8_t digest[32];
     fs->st.digest(digest);
@@ -1206,6 +1231,20 @@ static uint32_t bits_for(uint64_t n) {  // floor(log2 n) + 1, zerocheck.rs:91-94
 
 static const uint8_t kLabelZcDriver[] = "flock-zerocheck-v0";
 
+// The 7 fixed inner challenges r[6..13) of the zerocheck (zerocheck.rs:644-669): phi8 of
+// F7, 53, B5 (univariate_skip_optimized.rs:81), then gamma^(2^i) / (1 + gamma^(2^i)) for
+// gamma = x (:109-139).
+static void zc_inner_constants(flock_f128* out) {
+    const F* phi = phi8_table();
+    const uint8_t small[3] = {0xF7, 0x53, 0xB5};
+    for (int i = 0; i < 3; i++) out[i] = flock_f128{phi[small[i]].lo, phi[small[i]].hi};
+    for (int i = 0; i < 4; i++) {
+        const F g{1ull << (1 << i), 0};
+        const F v = fmul(g, finv(fadd(F{1, 0}, g)));
+        out[3 + i] = flock_f128{v.lo, v.hi};
+    }
+}
+
 int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a,
                               const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
                               flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
@@ -1263,16 +1302,7 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, bits_for(m));
     flock_fs_sample_f128_vec(fs, r.data(), ks);
     flock_fs_sample_f128_vec(fs, r.data() + ks + n_inner, m - ks - n_inner);
-    {
-        const F* phi = phi8_table();
-        const uint8_t small[3] = {0xF7, 0x53, 0xB5};  // univariate_skip_optimized.rs:81
-        for (int i = 0; i < 3; i++) r[ks + i] = flock_f128{phi[small[i]].lo, phi[small[i]].hi};
-        for (int i = 0; i < 4; i++) {  // gamma^(2^i) / (1 + gamma^(2^i)), :109-139
-            const F g{1ull << (1 << i), 0};
-            const F v = fmul(g, finv(fadd(F{1, 0}, g)));
-            r[ks + 3 + i] = flock_f128{v.lo, v.hi};
-        }
-    }
+    zc_inner_constants(r.data() + ks);
     // eq over r[7..m) (2^(m-7) words: round 2 and shat) and over r[9..m) (round 1's groups)
     HIP_TRY(hipMemcpyAsync(R, r.data() + ks, n_mlv * 16, hipMemcpyHostToDevice, c->stream));
     LAUNCH(flock::launch_build_eq(R + 1, (int)n_mlv - 1, EB, EQS, c->stream));
@@ -1379,6 +1409,96 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     flock_fs_observe_f128(fs, h_finals[1]);
     h_claim[0] = z;
     for (uint32_t k = 0; k < n_mlv; k++) h_claim[1 + n_mlv + k] = r[ks + k];
+    return 0;
+}
+
+// ---- zerocheck verifier (zerocheck.rs:1032-1225) ----
+
+namespace {
+bool feq(F a, F b) { return a.lo == b.lo && a.hi == b.hi; }
+F fo(flock_f128 v) { return F{v.lo, v.hi}; }
+flock_f128 to(F v) { return flock_f128{v.lo, v.hi}; }
+
+int reject(const std::string& what) {
+    return fail(FLOCK_HIP_E_REJECT, "proof rejected: " + what);
+}
+
+// interpolate_at_z_combined (zerocheck/multilinear.rs:238-265): the polynomial of degree < 128
+// that is zero on S = phi8(0..63) and takes `lambda_vals` on Lambda = phi8(64..127), at z
+// (plain Lagrange over the 128 nodes).
+F interpolate_combined(const flock_f128* lambda_vals, F z) {
+    const F* phi = phi8_table();
+    F acc{0, 0};
+    for (int i = 64; i < 128; i++) {
+        F num{1, 0}, den{1, 0};
+        for (int j = 0; j < 128; j++) {
+            if (j == i) continue;
+            num = fmul(num, fadd(z, phi[j]));
+            den = fmul(den, fadd(phi[i], phi[j]));
+        }
+        acc = fadd(acc, fmul(fmul(num, finv(den)), fo(lambda_vals[i - 64])));
+    }
+    return acc;
+}
+}  // namespace
+
+int flock_zerocheck_verify(flock_fs* fs, uint32_t m, int grinding, const flock_f128* h_round1,
+                           const flock_f128* h_ml, const flock_f128 h_finals[3],
+                           const uint64_t* h_nonces, flock_f128* h_claim) {
+    const int ks = 6, n_inner = 7;
+    if (!fs || !h_round1 || !h_ml || !h_finals || (grinding && !h_nonces) ||
+        m < (uint32_t)(ks + n_inner) || m > 40)
+        return fail(FLOCK_HIP_E_INVALID, "bad zerocheck verify arguments (need 13 <= m <= 40)");
+    const uint32_t n_mlv = m - ks;
+    int ni = 0;
+    // r: r_skip | inner constants | r_outer, in the prover's order
+    flock_fs_observe_label(fs, kLabelZcDriver, sizeof(kLabelZcDriver) - 1);
+    if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], bits_for(m)))
+        return reject("initial proof of work");
+    std::vector<flock_f128> r(m);
+    flock_fs_sample_f128_vec(fs, r.data(), ks);
+    flock_fs_sample_f128_vec(fs, r.data() + ks + n_inner, m - ks - n_inner);
+    zc_inner_constants(r.data() + ks);
+    // round 1 messages, z
+    flock_fs_observe_f128_slice(fs, h_round1, 64);
+    flock_fs_observe_f128_slice(fs, h_round1 + 64, 64);
+    if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], bits_for((1u << (ks + 1)) - 1)))
+        return reject("skip proof of work");
+    flock_f128 zv;
+    flock_fs_sample_f128(fs, &zv);
+    const F z = fo(zv);
+    // c(z, r_rest) from round1_c on Lambda (interpolate_at_z_on_lambda, multilinear.rs:215)
+    F w[64];
+    lagrange_phi8(64, z, w);
+    F c_eval{0, 0};
+    for (int i = 0; i < 64; i++) c_eval = fadd(c_eval, fmul(w[i], fo(h_round1[64 + i])));
+    if (!feq(c_eval, fo(h_finals[2]))) return reject("round-1 c evaluation");
+    // the AB running claim from the S-zero assumption
+    std::vector<flock_f128> comb(64);
+    for (int i = 0; i < 64; i++) comb[i] = to(fadd(fo(h_round1[i]), fo(h_round1[64 + i])));
+    F run = fadd(interpolate_combined(comb.data(), z), c_eval);
+    // multilinear chain: G(0) from (1 + r_eq) G(0) + r_eq G(1) = claim, then claim <- G(rho)
+    std::vector<flock_f128> rho(n_mlv);
+    for (uint32_t i = 0; i < n_mlv; i++) {
+        const F r_eq = fo(r[ks + i]);
+        const F g1 = fo(h_ml[2 * i]), gi = fo(h_ml[2 * i + 1]);
+        const F g0 = fmul(fadd(run, fmul(r_eq, g1)), finv(fadd(F{1, 0}, r_eq)));
+        flock_fs_observe_f128(fs, h_ml[2 * i]);
+        flock_fs_observe_f128(fs, h_ml[2 * i + 1]);
+        if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], bits_for(2)))
+            return reject("multilinear round proof of work");
+        flock_fs_sample_f128(fs, &rho[i]);
+        const F x = fo(rho[i]), x1 = fadd(F{1, 0}, x);
+        run = fadd(fadd(fmul(g0, x1), fmul(g1, x)), fmul(gi, fmul(x, x1)));
+    }
+    if (!feq(run, fmul(fo(h_finals[0]), fo(h_finals[1])))) return reject("final a*b claim");
+    flock_fs_observe_f128(fs, h_finals[0]);
+    flock_fs_observe_f128(fs, h_finals[1]);
+    if (h_claim) {
+        h_claim[0] = zv;
+        for (uint32_t k = 0; k < n_mlv; k++) h_claim[1 + k] = rho[k];
+        for (uint32_t k = 0; k < n_mlv; k++) h_claim[1 + n_mlv + k] = r[ks + k];
+    }
     return 0;
 }
 

@@ -30,6 +30,7 @@ E_NODEVICE = 3
 E_HIP = 100
 E_M_TOO_SMALL = 101
 E_INTERNAL = 102
+E_REJECT = 103
 
 
 class FlockHipError(RuntimeError):
@@ -225,6 +226,8 @@ def load():
         "flock_fs_sample_f128": (None, [P, ctypes.POINTER(F128)]),
         "flock_fs_sample_f128_vec": (None, [P, P, S]),
         "flock_fs_grind_pow": (U64, [P, U32]),
+        "flock_fs_verify_pow": (I, [P, U64, U32]),
+        "flock_zerocheck_verify": (I, [P, U32, I, P, P, P, P, P]),
         "flock_fs_digest": (None, [P, ctypes.c_char_p]),
         "flock_hip_prover_create": (I, [I, ctypes.POINTER(P)]),
         "flock_hip_prover_destroy": (None, [P]),
@@ -627,6 +630,22 @@ def registry_digest(types, nu: int) -> bytes:
     return out.raw
 
 
+def zerocheck_verify(fs: "FsChallenger", m: int, grinding: bool, round1, multilinear, finals,
+                     nonces=None) -> np.ndarray:
+    """zerocheck::verify_with_grinding (zerocheck.rs:1032-1225) on the host transcript `fs`;
+    the arrays are the prover's outputs ((n, 2) uint64). Returns the claim, or raises
+    FlockHipError with E_REJECT."""
+    n_mlv = m - 6
+    claim = np.zeros((1 + 2 * n_mlv, 2), dtype=np.uint64)
+    r1 = np.ascontiguousarray(round1, dtype=np.uint64)
+    ml = np.ascontiguousarray(multilinear, dtype=np.uint64)
+    fin = np.ascontiguousarray(finals, dtype=np.uint64)
+    nc = np.ascontiguousarray(nonces if nonces is not None else np.zeros(1), dtype=np.uint64)
+    _check(load().flock_zerocheck_verify(fs.h, m, int(bool(grinding)), _np_ptr(r1), _np_ptr(ml),
+                                         _np_ptr(fin), nc.ctypes.data, _np_ptr(claim)))
+    return claim
+
+
 def registry_new(types, nu: int) -> Registry:
     """Registry::new (schedule.rs:337-495); types = [(k_log, useful_bits, is_element)]."""
     arr = (TableType * max(len(types), 1))(*[TableType(k, int(e), u) for k, u, e in types])
@@ -698,6 +717,9 @@ class FsChallenger:
 
     def grind_pow(self, bits: int) -> int:
         return int(load().flock_fs_grind_pow(self.h, bits))
+
+    def verify_pow(self, nonce: int, bits: int) -> bool:
+        return bool(load().flock_fs_verify_pow(self.h, nonce, bits))
 
     def digest(self) -> bytes:
         """SHA-256 of everything the transcript has absorbed."""

@@ -36,6 +36,7 @@ extern "C" {
 #define FLOCK_HIP_E_HIP 100         /* HIP runtime error (prove_ffi.cu:44-46 uses 100 for CUDA) */
 #define FLOCK_HIP_E_M_TOO_SMALL 101 /* m below the commit minimum (prove_ffi.cu:429) */
 #define FLOCK_HIP_E_INTERNAL 102    /* a device result failed its host-side check */
+#define FLOCK_HIP_E_REJECT 103      /* a verifier rejected the proof */
 
 typedef struct {
     uint64_t lo, hi;
@@ -217,6 +218,16 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* ctx, flock_fs* fs, const uint8_t* d
                               const uint8_t* d_b, const uint8_t* d_c, uint32_t m, int grinding,
                               flock_f128* h_round1, flock_f128* h_ml, flock_f128 h_finals[3],
                               uint64_t* h_nonces, flock_f128* h_claim, flock_f128* h_s_hat_v_c);
+/* zerocheck::verify_with_grinding (zerocheck.rs:1032-1225): replays a zerocheck proof (the
+ * outputs of flock_hip_zerocheck_prove, same layouts) on the verifier's transcript `fs`. It
+ * re-derives r and z, checks round 1's c evaluation on Lambda, rebuilds the AB claim from the
+ * zero-on-S assumption, runs the multilinear chain and checks the final a*b claim. Every PoW
+ * nonce is checked when grinding != 0. Returns 0 and, when h_claim is not NULL, the
+ * ZerocheckClaim (the prover's h_claim layout), or FLOCK_HIP_E_REJECT naming the failed check.
+ * Host only. */
+int flock_zerocheck_verify(flock_fs* fs, uint32_t m, int grinding, const flock_f128* h_round1,
+                           const flock_f128* h_ml, const flock_f128 h_finals[3],
+                           const uint64_t* h_nonces, flock_f128* h_claim);
 /* fold_1b_rows (crates/flock-core/src/pcs/ring_switch.rs:182-230, :410-470): out[j] =
  * sum_y d_t[y] * bit j of the 128-bit row d_rows[y], j < 128 (d_rows and d_t n_rows words). */
 int flock_hip_fold_1b_rows(flock_hip_ctx* ctx, const flock_f128* d_rows, size_t n_rows,
@@ -407,6 +418,9 @@ void flock_fs_observe_bytes(flock_fs* fs, const uint8_t* b, size_t n);
 void flock_fs_sample_f128(flock_fs* fs, flock_f128* out);
 void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n);
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits);
+/* Challenger::verify_pow (challenger.rs:960-981): 1 if nonce passes (bits = 0: only nonce 0),
+ * else 0; the nonce bytes are absorbed either way. */
+int flock_fs_verify_pow(flock_fs* fs, uint64_t nonce, uint32_t bits);
 
 /* The transcript's running state digest: SHA-256 of everything absorbed so far (the value
  * grind_pow hashes with its nonce, challenger.rs:884-958). Two transcripts that absorbed the

@@ -100,3 +100,25 @@ def test_fold_1b_rows(ctx, n):
     L.or_fold_1b_rows(O._p(rows), O._p(t), n, O._p(exp))
     got = ctx.fold_1b_rows(ctx.upload(rows), n, ctx.upload(t))
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("m,grinding", [(17, True), (24, False)])
+def test_gpu_zerocheck_proof_verifies(ctx, m, grinding):
+    # the GPU prover's proof through the host verifier (zerocheck.rs:1032-1225): accepted, the
+    # verifier's claim is the prover's, both transcripts end in the same state
+    say(f"case: zerocheck verify m={m} grinding={grinding}")
+    a, b, c = _witness(m, 800 + m, satisfying=True)
+    dom = b"flock-zc-gpu-verify"
+    fs = F.FsChallenger(dom)
+    r1, ml, fin, nonces, claim = ctx.zerocheck_prove(fs, ctx.upload(a), ctx.upload(b),
+                                                     ctx.upload(c), m, grinding)
+    vfs = F.FsChallenger(dom)
+    got = F.zerocheck_verify(vfs, m, grinding, r1, ml, fin, nonces if grinding else None)
+    assert np.array_equal(got, claim)
+    assert vfs.digest() == fs.digest()
+    bad = ml.copy()
+    bad[3, 1] ^= np.uint64(1)
+    with pytest.raises(F.FlockHipError) as e:
+        F.zerocheck_verify(F.FsChallenger(dom), m, grinding, r1, bad, fin,
+                           nonces if grinding else None)
+    assert e.value.code == F.E_REJECT
