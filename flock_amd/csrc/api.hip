@@ -1984,6 +1984,177 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     return 0;
 }
 
+// ---- lincheck verifier (lincheck/union.rs:662-960 with MatrixAssertion, :467-628) ----
+
+namespace {
+// eq(x, j) for every j < 2^n, coordinate k at bit k of j (build_eq_table)
+std::vector<F> eq_table(const flock_f128* x, uint32_t n) {
+    std::vector<F> t((size_t)1 << n, F{0, 0});
+    t[0] = F{1, 0};
+    for (uint32_t k = 0; k < n; k++) {
+        const F xk = fo(x[k]), xk1 = fadd(F{1, 0}, xk);
+        for (size_t j = (size_t)1 << k; j-- > 0;) {
+            t[j | ((size_t)1 << k)] = fmul(t[j], xk);
+            t[j] = fmul(t[j], xk1);
+        }
+    }
+    return t;
+}
+
+// eq_prefix_sum (union.rs:107-130): sum of eq(p, i) over i < n, in O(d). The indices below n
+// split by the highest bit where i drops below n; each such block contributes the eq factors
+// of n's bits above it times eq(p_j, 0), its free low bits summing to one.
+F eq_prefix_sum(const flock_f128* p, uint32_t d, uint64_t n) {
+    if (n == 1ull << d) return F{1, 0};
+    F acc{0, 0}, high{1, 0};
+    for (uint32_t j = d; j-- > 0;) {
+        const F x = fo(p[j]), x1 = fadd(F{1, 0}, x);
+        if ((n >> j) & 1) {
+            acc = fadd(acc, fmul(high, x1));
+            high = fmul(high, x);
+        } else {
+            high = fmul(high, x1);
+        }
+    }
+    return acc;
+}
+
+// CSC column sums of eq (CscCircuit::fold_split, lincheck.rs:463-497) on host arrays, with the
+// structure checked as it is read.
+bool host_csc_fold(uint64_t K, const uint32_t* ptr, const uint32_t* rows, uint64_t nnz,
+                   const std::vector<F>& eq, std::vector<F>& out) {
+    if (ptr[0] != 0 || ptr[K] != nnz) return false;
+    for (uint64_t col = 0; col < K; col++) {
+        if (ptr[col] > ptr[col + 1]) return false;
+        F acc{0, 0};
+        for (uint32_t k = ptr[col]; k < ptr[col + 1]; k++) {
+            if (rows[k] >= K) return false;
+            acc = fadd(acc, eq[rows[k]]);
+        }
+        out[col] = acc;
+    }
+    return true;
+}
+}  // namespace
+
+int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_log,
+                          int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                          const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                          flock_f128 v_a, flock_f128 v_b, int grinding,
+                          const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                          const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
+                          flock_f128* h_claim) {
+    const int ks = 6;
+    if (!fs || k_log < (uint32_t)ks || k_log > 24 || nu > 40 || n_rows > (1ull << nu) ||
+        const_pin >= ((int64_t)1 << k_log) || const_pin < -1)
+        return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (need 6 <= k_log <= 24, "
+                                         "nu <= 40, n_rows <= 2^nu, const_pin < 2^k_log)");
+    const uint32_t L = k_log - ks;
+    if ((L && (!h_x_inner_rest || !h_rounds)) || (nu && !h_x_outer) || !h_z_partial ||
+        (!h_circuit && !h_matrix_evals) || (grinding && !h_nonces))
+        return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (null buffer)");
+    if (h_circuit) {
+        if (int rc = csc_check(h_circuit)) return rc;
+        if (h_circuit->k_log != k_log || h_circuit->const_pin != const_pin)
+            return fail(FLOCK_HIP_E_INVALID, "circuit k_log / const_pin differ from the type's");
+    }
+    int ni = 0;
+    flock_fs_observe_label(fs, kLabelLincheck, sizeof(kLabelLincheck) - 1);
+    // 1. alpha (union.rs:795-804)
+    if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], 1))
+        return reject("alpha proof of work");
+    flock_f128 alpha_v;
+    flock_fs_sample_f128(fs, &alpha_v);
+    const F alpha = fo(alpha_v);
+    // 2b. const-wire pin: beta after alpha; the target gains beta times the count-derived
+    // prefix sum of eq over the declared rows (union.rs:815-846)
+    F target = fadd(fmul(alpha, fo(v_a)), fo(v_b));
+    F beta{0, 0};
+    if (const_pin >= 0) {
+        if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], 1))
+            return reject("beta proof of work");
+        flock_f128 b;
+        flock_fs_sample_f128(fs, &b);
+        beta = fo(b);
+        target = fadd(target, fmul(beta, eq_prefix_sum(h_x_outer, nu, n_rows)));
+    }
+    // 3. the product-sumcheck rounds: q(0) = claim + q(1), q(X) = einf X^2 + c1 X + e0
+    // (union.rs:851-873)
+    F run = target;
+    std::vector<flock_f128> rounds_r(L);
+    for (uint32_t t = 0; t < L; t++) {
+        const F e1 = fo(h_rounds[2 * t]), einf = fo(h_rounds[2 * t + 1]);
+        flock_fs_observe_f128(fs, h_rounds[2 * t]);
+        flock_fs_observe_f128(fs, h_rounds[2 * t + 1]);
+        if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], 2))
+            return reject("sumcheck round proof of work");
+        flock_fs_sample_f128(fs, &rounds_r[t]);
+        const F r = fo(rounds_r[t]);
+        const F e0 = fadd(run, e1);
+        const F c1 = fadd(fadd(e0, e1), einf);
+        run = fadd(fadd(fmul(einf, fmul(r, r)), fmul(c1, r)), e0);
+    }
+    // 4. z_partial after the rounds; rr = the challenges LSB-first (union.rs:876-883)
+    flock_fs_observe_f128_slice(fs, h_z_partial, 64);
+    std::vector<flock_f128> rr(L);
+    for (uint32_t j = 0; j < L; j++) rr[j] = rounds_r[L - 1 - j];
+    // 6-7. fresh skip challenge and w = <lambda(r_skip), z_partial> (union.rs:903-946)
+    if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], (uint32_t)ks))
+        return reject("inner-skip proof of work");
+    flock_f128 r_skip;
+    flock_fs_sample_f128(fs, &r_skip);
+    F w{0, 0};
+    {
+        F lam[64];
+        lagrange_phi8(0, fo(r_skip), lam);
+        for (int i = 0; i < 64; i++) w = fadd(w, fmul(lam[i], fo(h_z_partial[i])));
+    }
+    // the matrix assertion; one slot filling the boolean region has prefix weights 1
+    const std::vector<F> err = eq_table(rr.data(), L);
+    if (!h_circuit) {
+        // check_reported (union.rs:538-576): alpha <W,A_0> + <W,B_0> + beta W_col[pin] = claim,
+        // W_col = z_partial (x) eq(rr)
+        F acc = fadd(fmul(alpha, fo(h_matrix_evals[0])), fo(h_matrix_evals[1]));
+        if (const_pin >= 0)
+            acc = fadd(acc, fmul(beta, fmul(fo(h_z_partial[const_pin & 63]), err[const_pin >> 6])));
+        if (!feq(acc, run)) return reject("matrix evals");
+    } else {
+        // check (union.rs:579-628): comb = alpha A_0^T eq + B_0^T eq at the quirky eq of
+        // (z_skip, x_inner_rest), + beta at the pin; its collapse at rr against z_partial
+        const uint64_t K = 1ull << k_log;
+        std::vector<F> eqi(K), sa(K), sb(K);
+        {
+            F lam[64];
+            lagrange_phi8(0, fo(z_skip), lam);
+            const std::vector<F> ex = eq_table(h_x_inner_rest, L);
+            for (uint64_t j = 0; j < ((uint64_t)1 << L); j++)
+                for (int i = 0; i < 64; i++) eqi[j * 64 + i] = fmul(lam[i], ex[j]);
+        }
+        if (!host_csc_fold(K, h_circuit->a_col_ptr, h_circuit->a_rows, h_circuit->nnz_a, eqi,
+                           sa) ||
+            !host_csc_fold(K, h_circuit->b_col_ptr, h_circuit->b_rows, h_circuit->nnz_b, eqi, sb))
+            return fail(FLOCK_HIP_E_INVALID, "CSC circuit has a row index >= n_cols or a "
+                                             "col_ptr entry out of order or > nnz");
+        F part[64];
+        for (int i = 0; i < 64; i++) part[i] = F{0, 0};
+        for (uint64_t j = 0; j < ((uint64_t)1 << L); j++)
+            for (int i = 0; i < 64; i++) {
+                F cmb = fadd(fmul(alpha, sa[j * 64 + i]), sb[j * 64 + i]);
+                if ((int64_t)(j * 64 + i) == const_pin) cmb = fadd(cmb, beta);
+                part[i] = fadd(part[i], fmul(err[j], cmb));
+            }
+        F acc{0, 0};
+        for (int i = 0; i < 64; i++) acc = fadd(acc, fmul(part[i], fo(h_z_partial[i])));
+        if (!feq(acc, run)) return reject("sumcheck final consistency");
+    }
+    if (h_claim) {
+        h_claim[0] = r_skip;
+        h_claim[1] = to(w);
+        for (uint32_t j = 0; j < L; j++) h_claim[2 + j] = rr[j];
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Union instance host glue (crates/flock-core/src/{schedule,union}.rs)
 // ---------------------------------------------------------------------------

@@ -228,6 +228,8 @@ def load():
         "flock_fs_grind_pow": (U64, [P, U32]),
         "flock_fs_verify_pow": (I, [P, U64, U32]),
         "flock_zerocheck_verify": (I, [P, U32, I, P, P, P, P, P]),
+        "flock_lincheck_verify": (I, [P, P, U32, ctypes.c_int64, U32, S, F128, P, P, F128, F128,
+                                      I, P, P, P, P, P]),
         "flock_fs_digest": (None, [P, ctypes.c_char_p]),
         "flock_hip_prover_create": (I, [I, ctypes.POINTER(P)]),
         "flock_hip_prover_destroy": (None, [P]),
@@ -643,6 +645,36 @@ def zerocheck_verify(fs: "FsChallenger", m: int, grinding: bool, round1, multili
     nc = np.ascontiguousarray(nonces if nonces is not None else np.zeros(1), dtype=np.uint64)
     _check(load().flock_zerocheck_verify(fs.h, m, int(bool(grinding)), _np_ptr(r1), _np_ptr(ml),
                                          _np_ptr(fin), nc.ctypes.data, _np_ptr(claim)))
+    return claim
+
+
+def lincheck_verify(fs: "FsChallenger", k_log: int, const_pin: int, nu: int, n_rows: int,
+                    z_skip: int, x_inner_rest, x_outer, v_a: int, v_b: int, grinding: bool,
+                    proof: dict, circuit=None) -> np.ndarray:
+    """verify_union_with_grinding (lincheck/union.rs:662-960) for one boolean type, on the host
+    transcript `fs`. proof = the prover's output dict (rounds, z_partial, matrix_evals, nonces).
+    circuit = (a_ptr, a_rows, b_ptr, b_rows) host CSC arrays for MatrixAssertion::check, or
+    None for the deferred form checked against the reported matrix evals. Returns the claim,
+    or raises FlockHipError with E_REJECT."""
+    L = k_log - 6
+    claim = np.zeros((2 + L, 2), dtype=np.uint64)
+    keep = []
+    cp = None
+    if circuit is not None:
+        keep = [np.ascontiguousarray(x, dtype=np.uint32) if len(x) else np.zeros(1, np.uint32)
+                for x in circuit]
+        c = Csc(k_log, 0, *[x.ctypes.data for x in keep], len(circuit[1]), len(circuit[3]),
+                const_pin)
+        cp = ctypes.byref(c)
+    arr = {k: np.ascontiguousarray(proof[k], dtype=np.uint64) if len(proof[k]) else
+           np.zeros((1, 2), dtype=np.uint64) for k in ("rounds", "z_partial", "matrix_evals")}
+    nc = np.ascontiguousarray(proof.get("nonces", np.zeros(1)), dtype=np.uint64)
+    xr, xo = _ints_arr(x_inner_rest), _ints_arr(x_outer)
+    _check(load().flock_lincheck_verify(
+        fs.h, cp, k_log, const_pin, nu, n_rows, f128(z_skip), _np_ptr(xr), _np_ptr(xo),
+        f128(v_a), f128(v_b), int(bool(grinding)), _np_ptr(arr["rounds"]),
+        _np_ptr(arr["z_partial"]), _np_ptr(arr["matrix_evals"]), nc.ctypes.data, _np_ptr(claim)))
+    del keep
     return claim
 
 

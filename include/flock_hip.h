@@ -294,6 +294,27 @@ int flock_hip_lincheck_prove_keccak(flock_hip_ctx* ctx, flock_fs* fs, const uint
                                     int grinding, flock_f128* h_rounds,
                                     flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
                                     uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
+/* verify_union_with_grinding (lincheck/union.rs:662-960) for a registry with one boolean type
+ * filling the boolean region, the prover of flock_hip_lincheck_prove: replays the proof (same
+ * layouts) on the verifier's transcript `fs` — alpha, beta when const_pin >= 0 (the target
+ * gains beta * sum_{row < n_rows} eq(x_outer, row)), the rounds' chain from
+ * alpha v_a + v_b, z_partial, the fresh skip challenge — checking every PoW nonce when
+ * grinding != 0 (h_nonces[k_log-6 + 2 + (const_pin >= 0)]). Then the matrix assertion:
+ *  h_circuit != NULL: MatrixAssertion::check (:579-628), the O(nnz) comb collapse at the bound
+ *    point against z_partial; h_circuit is the CSC on the HOST (host pointers) and must carry
+ *    k_log and const_pin; h_matrix_evals is not read (as in the reference);
+ *  h_circuit == NULL: the deferred form with MatrixAssertion::check_reported (:538-576),
+ *    alpha <W,A_0> + <W,B_0> + beta W_col[pin] against the claim from h_matrix_evals.
+ * (v_a, v_b) is the lincheck's input claim at x_ab = (z_skip, x_inner_rest, x_outer). Returns
+ * 0 and, when h_claim is not NULL, the LincheckClaim in the prover's layout, or
+ * FLOCK_HIP_E_REJECT naming the failed check. Host only. */
+int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_log,
+                          int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                          const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                          flock_f128 v_a, flock_f128 v_b, int grinding,
+                          const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                          const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
+                          flock_f128* h_claim);
 
 /* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ------------ */
 

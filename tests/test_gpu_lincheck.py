@@ -9,7 +9,8 @@ import pytest
 
 import oracle_lib as O
 from flock_amd import lib as F
-from test_oracle_lincheck import csc_fold, lincheck_prove, partial_fold, random_csc, stripe
+from test_oracle_lincheck import (csc_fold, lincheck_prove, partial_fold, quirky_eq, random_csc,
+                                  stripe)
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +25,28 @@ def ctx():
 def say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
+
+
+def input_claim(csc_a, csc_b, z_skip, x_rest, z_vec):
+    """(v_a, v_b) = (<A_0^T eq, z_vec>, <B_0^T eq, z_vec>): the claim a lincheck proof answers."""
+    sa, sb = csc_fold(*csc_a, *csc_b, quirky_eq(z_skip, x_rest))
+
+    def dot(s):
+        r = np.bitwise_xor.reduce(O.mul_batch(s, z_vec), axis=0)
+        return int(r[0]) | int(r[1]) << 64
+
+    return dot(sa), dot(sb)
+
+
+def verify_gpu_proof(dom, got, csc_a, csc_b, k_log, pin, nu, nr, z_skip, x_rest, x_outer,
+                     grinding, z_vec):
+    """The GPU proof passes flock_lincheck_verify in both forms with the prover's claim."""
+    va, vb = input_claim(csc_a, csc_b, z_skip, x_rest, z_vec)
+    for circuit in ((*csc_a, *csc_b), None):
+        fs = F.FsChallenger(dom)
+        claim = F.lincheck_verify(fs, k_log, pin, nu, nr, z_skip, x_rest, x_outer, va, vb,
+                                  grinding, got, circuit)
+        assert np.array_equal(claim, got["claim"])
 
 
 @pytest.mark.parametrize("k_log,nnz", [(6, 2), (8, 3), (15, 20)])
@@ -87,7 +110,7 @@ def test_lincheck_prove(ctx, k_log, nu, pin, grinding, ub, nr):
     nr = (1 << nu) if nr is None else nr
     csc_a = random_csc(k, 4, 400 + k_log)
     csc_b = random_csc(k, 3, 500 + k_log)
-    z = stripe(k_log, nu, 600 + k_log, ub, nr)
+    z = stripe(k_log, nu, 600 + k_log, ub, nr, pin)
     pts = O.arr_to_ints(O.splitmix(0x800 + k_log, 1 + L + nu))
     z_skip, x_rest, x_outer = pts[0], pts[1:1 + L], pts[1 + L:]
     dom = b"flock-lincheck-gpu-test"
@@ -102,6 +125,8 @@ def test_lincheck_prove(ctx, k_log, nu, pin, grinding, ub, nr):
     if grinding:
         assert np.array_equal(got["nonces"], exp["nonces"])
     assert np.array_equal(zv.download_f128(k), exp["z_vec"])
+    verify_gpu_proof(dom, got, csc_a, csc_b, k_log, pin, nu, nr, z_skip, x_rest, x_outer,
+                     grinding, exp["z_vec"])
 
 
 def test_lincheck_sha256_circuit_config2(ctx):
@@ -126,7 +151,13 @@ def test_lincheck_sha256_circuit_config2(ctx):
     exp = lincheck_prove(dom, k_log, csc_a, csc_b, pin, zs, nu, ub, n_decl, z_skip, x_rest,
                          x_outer, 1)
     circuit, keep = ctx.csc(k_log, *csc_a, *csc_b, const_pin=pin)
+    zv = ctx.alloc(16 << k_log)
     got = ctx.lincheck_prove(F.FsChallenger(dom), circuit, stripe, nu, ub, n_decl, z_skip,
-                             x_rest, x_outer, True)
+                             x_rest, x_outer, True, z_vec=zv)
     for name in ("rounds", "z_partial", "matrix_evals", "claim", "nonces"):
         assert np.array_equal(got[name], exp[name]), name
+    # the proof of the real circuit over the GPU witness verifies (the constant column is 1 on
+    # the 1000 declared rows)
+    say("case: lincheck sha256 verify")
+    verify_gpu_proof(dom, got, csc_a, csc_b, k_log, pin, nu, n_decl, z_skip, x_rest, x_outer,
+                     True, zv.download_f128(1 << k_log))

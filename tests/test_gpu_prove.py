@@ -68,8 +68,9 @@ def inputs_for(kind, n, seed):
     return np.ascontiguousarray(random_inputs(n, seed))
 
 
-def oracle_prove_core(kind, n_log, inputs, circuits):
-    """The oracle's functions in the order of flock_hip_prove_core, on one transcript."""
+def oracle_prove_core(kind, n_log, inputs, circuits, parts=None):
+    """The oracle's functions in the order of flock_hip_prove_core, on one transcript. When
+    `parts` is a dict it receives the statement and both proofs."""
     k_log, ub, pin = SHAPES[kind]
     n = 1 << n_log
     if kind == F.HASH_SHA256:
@@ -125,6 +126,10 @@ def oracle_prove_core(kind, n_log, inputs, circuits):
     assert rc == 0
     dig = ctypes.create_string_buffer(32)
     L.or_fs_digest(fs.h, dig)
+    if parts is not None:
+        parts.update(root=root, count=count, m=m, zc=(r1, ml, fin, nonces, claim),
+                     lc=dict(rounds=rounds, z_partial=zp, matrix_evals=mev, nonces=lnon,
+                             claim=lcl))
     return root, dig.raw
 
 

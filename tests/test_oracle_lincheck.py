@@ -20,6 +20,7 @@ L.or_lc_round_eval.argtypes = [P, P, S, P]
 L.or_lc_bind_top.argtypes = [P, S, O.F128]
 L.or_lincheck_prove.argtypes = [P, ctypes.c_int, P, P, P, P, ctypes.c_long, P, ctypes.c_int, S,
                                 S, O.F128, P, P, ctypes.c_int, P, P, P, P, P, P]
+L.or_fs_digest.argtypes = [P, P]
 L.or_phi8.restype = O.F128
 L.or_phi8.argtypes = [ctypes.c_uint8]
 
@@ -41,9 +42,11 @@ def random_csc(k: int, nnz_per_col: int, seed: int):
 
 
 def stripe(k_log: int, nu: int, seed: int, useful_bits: int | None = None,
-           n_rows: int | None = None) -> np.ndarray:
+           n_rows: int | None = None, pin: int = -1, pin_rows: int | None = None) -> np.ndarray:
     """Random lincheck z stripe (z[byte_idx * k + i_inner], bit r = outer row 8*byte_idx + r),
-    zero past useful_bits columns and n_rows outer rows when given (honest padding)."""
+    zero past useful_bits columns and n_rows outer rows when given (honest padding). With
+    pin >= 0, column pin is the constant wire: 1 on the first pin_rows (default n_rows) rows
+    and 0 below."""
     k = 1 << k_log
     rng = np.random.default_rng(seed)
     z = rng.integers(0, 256, (1 << nu) // 8 * k, dtype=np.uint8).reshape(-1, k)
@@ -54,6 +57,11 @@ def stripe(k_log: int, nu: int, seed: int, useful_bits: int | None = None,
             for r in range(8):
                 if 8 * s + r >= n_rows:
                     z[s] &= np.uint8(~(1 << r) & 0xFF)
+    if pin >= 0:
+        rows = ((1 << nu) if n_rows is None else n_rows) if pin_rows is None else pin_rows
+        z[:, pin] = 0
+        for row in range(rows):
+            z[row // 8, pin] |= np.uint8(1 << (row % 8))
     return np.ascontiguousarray(z.reshape(-1))
 
 
@@ -104,6 +112,9 @@ def lincheck_prove(dom, k_log, csc_a, csc_b, const_pin, z, nu, useful_bits, n_ro
                              O._p(out["claim"]), O._p(out["z_vec"]))
     assert rc == 0
     out["rounds"] = out["rounds"][:2 * Lr]
+    dig = ctypes.create_string_buffer(32)
+    L.or_fs_digest(fs.h, dig)
+    out["digest"] = dig.raw
     return out
 
 

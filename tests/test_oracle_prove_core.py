@@ -1,6 +1,7 @@
 """The committed prove-core golden vectors regenerate from their seeds: the BLAKE3 case (the
 cheapest) is recomputed with the oracle and compared with tests/golden/prove_core_vectors.json,
-so the GPU test's fixture cannot drift from the oracle or from the seeded inputs."""
+so the GPU test's fixture cannot drift from the oracle or from the seeded inputs; and that proof
+verifies (statement, zerocheck and lincheck verifiers chained)."""
 from __future__ import annotations
 
 import json
@@ -16,7 +17,13 @@ def test_blake3_golden_case_regenerates():
         gold = json.load(fh)
     case = next(c for c in gold["cases"] if c["kind"] == F.HASH_BLAKE3)
     _, _, a, b = B3.matrices()
+    circuits = {F.HASH_BLAKE3: (None, a, b)}
+    parts = {}
     root, dig = T.oracle_prove_core(case["kind"], case["n_log"],
                                     T.inputs_for(case["kind"], case["n_decl"], case["seed"]),
-                                    {F.HASH_BLAKE3: (None, a, b)})
+                                    circuits, parts)
     assert root.hex() == case["root"] and dig.hex() == case["transcript"]
+    # the golden proof (whose digest the GPU prove core reproduces at this size) verifies
+    from test_verify_prove_core import verify_core
+    assert verify_core(case["kind"], case["n_log"], parts, circuits, gold["domain"].encode(),
+                       bytes.fromhex(gold["digest"])) == dig
