@@ -240,9 +240,12 @@ __device__ __forceinline__ f128 f128_shl_mod(f128 a, int s) {
 
 // ---------------------------------------------------------------------------
 // Fixed-multiplier tables (Shoup, 4-bit windows) in LDS.
-// Layout: table[k * 16 + n] = (n(x) * x^(4k)) * t, k in [0,32), n in [0,16). 8 KiB.
+// Layout: table[k * 16 + n] = (n(x) * x^(4k)) * t, k in [0,32), n in [0,16). 8 KiB. A row of
+// 16 entries is 256 B, every LDS bank once, so random nibble lookups never conflict.
 // ---------------------------------------------------------------------------
 constexpr int SHOUP_ENTRIES = 512;
+
+__device__ __forceinline__ void shoup_put(f128* table, int idx, f128 v) { table[idx] = v; }
 
 // Build subtable k (16 entries) of the table for multiplier t.
 __device__ __forceinline__ void shoup_build_row(f128* table, f128 t, int k) {
@@ -250,28 +253,73 @@ __device__ __forceinline__ void shoup_build_row(f128* table, f128 t, int k) {
     const f128 e2 = f128_mul_x(e1);
     const f128 e4 = f128_mul_x(e2);
     const f128 e8 = f128_mul_x(e4);
-    f128* row = table + 16 * k;
+    // Rows are 256 B apart, so lanes writing the same entry of different rows at once would
+    // hit the same banks. Lane k writes entry g(s) ^ r at step s instead (g = Gray code,
+    // r = k mod 16): 16 lanes, 16 distinct entries, every bank once. Entries are linear in
+    // n, so entry(g(s) ^ r) = entry(g(s - 1) ^ r) + e_(bit flipped at step s).
+    const uint32_t r = (uint32_t)k & 15u;
+    const uint32_t m1 = 0u - (r & 1u), m2 = 0u - ((r >> 1) & 1u);
+    const uint32_t m4 = 0u - ((r >> 2) & 1u), m8 = 0u - (r >> 3);
+    f128 v;
+    v.x = (e1.x & m1) ^ (e2.x & m2) ^ (e4.x & m4) ^ (e8.x & m8);
+    v.y = (e1.y & m1) ^ (e2.y & m2) ^ (e4.y & m4) ^ (e8.y & m8);
+    v.z = (e1.z & m1) ^ (e2.z & m2) ^ (e4.z & m4) ^ (e8.z & m8);
+    v.w = (e1.w & m1) ^ (e2.w & m2) ^ (e4.w & m4) ^ (e8.w & m8);
 #pragma unroll
-    for (int n = 0; n < 16; n++) {
-        f128 v = f128_zero();
-        if (n & 1) v = f128_add(v, e1);
-        if (n & 2) v = f128_add(v, e2);
-        if (n & 4) v = f128_add(v, e4);
-        if (n & 8) v = f128_add(v, e8);
-        row[n] = v;
+    for (int s = 0; s < 16; s++) {
+        if (s) {
+            const int bit = __builtin_ctz(s);
+            v = f128_add(v, bit == 0 ? e1 : bit == 1 ? e2 : bit == 2 ? e4 : e8);
+        }
+        shoup_put(table, 16 * k + (int)(((uint32_t)(s ^ (s >> 1))) ^ r), v);
     }
 }
 
-// XOR of 32 lookups, two per three-input XOR (v_bitop3 0x96).
+// (x >> 8K) & 0xF0 in one VALU op: an SDWA and reading byte K (the compiler selects the SDWA
+// form by itself for byte 3 only).
+template <int K>
+__device__ __forceinline__ uint32_t byte_hi_nibble(uint32_t x) {
+    if constexpr (K == 0) {
+        return x & 0xF0u;
+    } else if constexpr (K == 1) {
+        uint32_t r;
+        asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1 "
+            "src1_sel:DWORD"
+            : "=v"(r)
+            : "v"(x), "v"(0xF0u));
+        return r;
+    } else if constexpr (K == 2) {
+        uint32_t r;
+        asm("v_and_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2 "
+            "src1_sel:DWORD"
+            : "=v"(r)
+            : "v"(x), "v"(0xF0u));
+        return r;
+    } else {
+        return (x >> 24) & 0xF0u;
+    }
+}
+
+// XOR of 32 lookups, two per three-input XOR (v_bitop3 0x96). The byte offset of entry n is
+// n << 4, so each lookup's offset is one masked byte: (w >> 8k) & 0xF0 is nibble 2k + 1 of w,
+// and the same of w << 4 is nibble 2k. Tables at a constant LDS address fold their row offset
+// into the read's immediate.
 __device__ __forceinline__ f128 shoup_mul(const f128* __restrict__ table, f128 a) {
+    const char* tb = reinterpret_cast<const char*>(table);
     const uint32_t aw[4] = {a.x, a.y, a.z, a.w};
     f128 acc = f128_zero();
 #pragma unroll
     for (int w = 0; w < 4; w++) {
+        const uint32_t odd = aw[w], even = aw[w] << 4;
+        const uint32_t o[8] = {byte_hi_nibble<0>(even), byte_hi_nibble<0>(odd),
+                               byte_hi_nibble<1>(even), byte_hi_nibble<1>(odd),
+                               byte_hi_nibble<2>(even), byte_hi_nibble<2>(odd),
+                               byte_hi_nibble<3>(even), byte_hi_nibble<3>(odd)};
 #pragma unroll
-        for (int j = 0; j < 8; j += 2) {
-            const f128 t0 = table[(w * 8 + j) * 16 + ((aw[w] >> (4 * j)) & 15u)];
-            const f128 t1 = table[(w * 8 + j + 1) * 16 + ((aw[w] >> (4 * j + 4)) & 15u)];
+        for (int k = 0; k < 4; k++) {
+            const f128 t0 = *reinterpret_cast<const f128*>(tb + (w * 8 + 2 * k) * 256 + o[2 * k]);
+            const f128 t1 =
+                *reinterpret_cast<const f128*>(tb + (w * 8 + 2 * k + 1) * 256 + o[2 * k + 1]);
             acc.x = __builtin_amdgcn_bitop3_b32(acc.x, t0.x, t1.x, 0x96);
             acc.y = __builtin_amdgcn_bitop3_b32(acc.y, t0.y, t1.y, 0x96);
             acc.z = __builtin_amdgcn_bitop3_b32(acc.z, t0.z, t1.z, 0x96);

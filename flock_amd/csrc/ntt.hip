@@ -91,8 +91,11 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict
                                                            uint64_t cols_per_hb,
                                                            uint32_t wgs_per_hb, int from_msg) {
     constexpr int NT = (1 << K) - 1;  // twiddles (and Shoup tables) per group
-    extern __shared__ f128 tables[];   // NT * SHOUP_ENTRIES
-    __shared__ f128 tw[NT];
+    // one LDS array, tables first: they sit at LDS address 0, so a lookup's address is its
+    // masked nibble plus an immediate
+    __shared__ f128 lds[NT * SHOUP_ENTRIES + NT];
+    f128* tables = lds;
+    f128* tw = lds + NT * SHOUP_ENTRIES;
 
     const uint64_t hb = blockIdx.x / wgs_per_hb;
     const uint32_t chunk = blockIdx.x - (uint32_t)(hb * wgs_per_hb);
@@ -218,11 +221,6 @@ template <int K>
 static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t live, int k,
                               int l0, const f128* evals, bool from_msg, hipStream_t s) {
     constexpr int NT = (1 << K) - 1;
-    const size_t lds = (size_t)NT * SHOUP_ENTRIES * sizeof(f128);
-    // set once per process (thread-safe static initialisation)
-    static const hipError_t attr = hipFuncSetAttribute(
-        (const void*)k_ntt_pass<K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (attr != hipSuccess) return attr;
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
     if (cols <= kDeepCols) {
@@ -244,7 +242,7 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
     if (wgs < 1) wgs = 1;
     const uint64_t grid = n_hb * wgs;
     if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    k_ntt_pass<K><<<(unsigned)grid, kNttThreads, lds, s>>>(src, data, t, live, k, l0, evals, cols,
+    k_ntt_pass<K><<<(unsigned)grid, kNttThreads, 0, s>>>(src, data, t, live, k, l0, evals, cols,
                                                            (uint32_t)wgs, from_msg ? 1 : 0);
     return hipGetLastError();
 }

@@ -194,7 +194,7 @@ __device__ __forceinline__ void shoup_build_block(f128* table, f128 t) {
         if (n & 2) v = f128_add(v, e2);
         if (n & 4) v = f128_add(v, e4);
         if (n & 8) v = f128_add(v, e8);
-        table[16 * row + n] = v;
+        shoup_put(table, 16 * row + n, v);
     }
 }
 
