@@ -329,6 +329,43 @@ __device__ __forceinline__ f128 shoup_mul(const f128* __restrict__ table, f128 a
     return acc;
 }
 
+// a * t with an 8-row table of t (rows 0..7 of the Shoup table above, 2 KiB): per 32-bit word
+// w of a the XOR of 8 lookups gives S_w = a_w * t, and a * t = sum of x^(32 w) S_w by Horner
+// from the top word. acc * x^32 moves words up by one and folds the top word h back as
+// h * (x^7 + x^2 + x + 1), which spans bits 0..38.
+__device__ __forceinline__ f128 shoup8_word(const char* tb, uint32_t x) {
+    const uint32_t even = x << 4;
+    const uint32_t o[8] = {byte_hi_nibble<0>(even), byte_hi_nibble<0>(x), byte_hi_nibble<1>(even),
+                           byte_hi_nibble<1>(x),    byte_hi_nibble<2>(even), byte_hi_nibble<2>(x),
+                           byte_hi_nibble<3>(even), byte_hi_nibble<3>(x)};
+    f128 acc = f128_zero();
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+        const f128 t0 = *reinterpret_cast<const f128*>(tb + k * 256 + o[k]);
+        const f128 t1 = *reinterpret_cast<const f128*>(tb + (k + 1) * 256 + o[k + 1]);
+        acc.x = __builtin_amdgcn_bitop3_b32(acc.x, t0.x, t1.x, 0x96);
+        acc.y = __builtin_amdgcn_bitop3_b32(acc.y, t0.y, t1.y, 0x96);
+        acc.z = __builtin_amdgcn_bitop3_b32(acc.z, t0.z, t1.z, 0x96);
+        acc.w = __builtin_amdgcn_bitop3_b32(acc.w, t0.w, t1.w, 0x96);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ f128 shoup8_mul(const f128* __restrict__ table, f128 a) {
+    const char* tb = reinterpret_cast<const char*>(table);
+    f128 acc = shoup8_word(tb, a.w);
+    const uint32_t aw[3] = {a.x, a.y, a.z};
+#pragma unroll
+    for (int w = 2; w >= 0; w--) {
+        const uint32_t h = acc.w;
+        const uint32_t lo = __builtin_amdgcn_bitop3_b32(h, h << 1, h << 2, 0x96) ^ (h << 7);
+        const uint32_t hi = __builtin_amdgcn_bitop3_b32(h >> 31, h >> 30, h >> 25, 0x96);
+        const f128 s = shoup8_word(tb, aw[w]);
+        acc = make_uint4(lo ^ s.x, acc.x ^ hi ^ s.y, acc.y ^ s.z, acc.z ^ s.w);
+    }
+    return acc;
+}
+
 // ---------------------------------------------------------------------------
 // Wave / block XOR reductions.
 // ---------------------------------------------------------------------------

@@ -145,13 +145,21 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict
     }
 }
 
-// Deep passes (at most 256 columns per high block): a 512-entry Shoup table per twiddle would
-// be used only a few hundred times and a 512-thread group would mostly idle. Here a 256-thread
-// group takes H = 256 / cols consecutive high blocks, builds one shared 16-entry comb table
-// per (block, twiddle) (256 B, every LDS bank once, so random nibble lookups never conflict),
-// and each thread runs the K layers of one (block, column) with those tables.
+// Deep passes (at most 256 columns per high block): a 32-row Shoup table per twiddle would be
+// used only a few hundred times and a 512-thread group would mostly idle. Here a 256-thread
+// group takes H = min(256 / cols, 4) consecutive high blocks and builds, per (block, twiddle),
+// the 8-row (2 KiB) table of shoup8_mul; each thread runs the K layers of one (block, column)
+// with those tables. The tables come first in the group's LDS (address 0), then the twiddle
+// rows and the twiddles.
 constexpr int kDeepThreads = 256;
-constexpr int kDeepMaxHb = 32;  // high blocks per group (tables: 32 x 7 x 256 B = 56 KiB)
+constexpr int kDeepMaxHb = 4;       // high blocks per group (tables: 4 x 7 x 2 KiB = 56 KiB)
+constexpr int kDeepTabEntries = 128;  // 8 rows of 16
+
+template <int K>
+__host__ __device__ constexpr size_t deep_lds_bytes(uint32_t hbs) {
+    return ((size_t)hbs * ((1 << K) - 1) * (kDeepTabEntries + 1) + (size_t)K * kMaxNttK) *
+           sizeof(f128);
+}
 
 template <int K>
 __global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
@@ -159,32 +167,35 @@ __global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
     int l0, const f128* __restrict__ evals, uint32_t cols_per_hb, uint32_t hbs_per_wg,
     uint64_t n_hb, int from_msg) {
     constexpr int NT = (1 << K) - 1;
-    __shared__ f128 rows[K][kMaxNttK];  // rows[j][bit] = evals[k - (l0+j) - 1][1 + bit]
-    extern __shared__ f128 tabs[];      // hbs_per_wg * NT tables of 16 entries, then b_hi words
-    uint32_t* his = reinterpret_cast<uint32_t*>(tabs + 16 * NT * hbs_per_wg);
+    extern __shared__ f128 lds[];
+    f128* tabs = lds;                                   // hbs_per_wg * NT tables
+    f128* rows = tabs + (size_t)hbs_per_wg * NT * kDeepTabEntries;  // [K][kMaxNttK]
+    f128* tws = rows + K * kMaxNttK;                    // hbs_per_wg * NT twiddles
     for (int i = threadIdx.x; i < K * kMaxNttK; i += blockDim.x) {
         const int j = i / kMaxNttK, bit = i - j * kMaxNttK;
         const int layer = l0 + j;
-        rows[j][bit] = bit < layer ? evals[(size_t)(k - layer - 1) * k + 1 + bit] : f128_zero();
+        rows[i] = bit < layer ? evals[(size_t)(k - layer - 1) * k + 1 + bit] : f128_zero();
     }
     __syncthreads();
     const uint64_t hb0 = (uint64_t)blockIdx.x * hbs_per_wg;
-    // table (h, tix): twiddle index tix = 2^j - 1 + u serves layer l0 + j, block (hb << j) | u
+    // twiddle (h, tix): tix = 2^j - 1 + u serves layer l0 + j, block (hb << j) | u
     for (uint32_t i = threadIdx.x; i < hbs_per_wg * NT; i += blockDim.x) {
         const uint32_t h = i / NT, tix = i - h * NT;
         const uint64_t hb = hb0 + h;
-        if (hb >= n_hb) continue;
-        const int j = 31 - __clz(tix + 1);
-        uint64_t blk = (hb << j) | (uint64_t)(tix + 1 - (1u << j));
         f128 tw = f128_zero();
-        while (blk) {
-            tw = f128_add(tw, rows[j][__builtin_ctzll(blk)]);
-            blk &= blk - 1;
+        if (hb < n_hb) {
+            const int j = 31 - __clz(tix + 1);
+            uint64_t blk = (hb << j) | (uint64_t)(tix + 1 - (1u << j));
+            while (blk) {
+                tw = f128_add(tw, rows[j * kMaxNttK + __builtin_ctzll(blk)]);
+                blk &= blk - 1;
+            }
         }
-        f128* tab = tabs + 16 * i;
-        tab[0] = f128_zero();
-        his[i] = comb_build<1>(tab, tw);
+        tws[i] = tw;
     }
+    __syncthreads();
+    for (uint32_t r = threadIdx.x; r < hbs_per_wg * NT * 8; r += blockDim.x)
+        shoup_build_row(tabs + (size_t)(r >> 3) * kDeepTabEntries, tws[r >> 3], (int)(r & 7));
     __syncthreads();
     const uint32_t h = threadIdx.x / cols_per_hb;
     const uint32_t c = threadIdx.x - h * cols_per_hb;
@@ -197,8 +208,7 @@ __global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
     f128 e[1 << K];
 #pragma unroll
     for (int i = 0; i < (1 << K); i++) e[i] = src[sbase + i * stride + c];
-    const f128* mytabs = tabs + 16 * NT * h;
-    const uint32_t* myhis = his + NT * h;
+    const f128* mytabs = tabs + (size_t)NT * kDeepTabEntries * h;
 #pragma unroll
     for (int j = 0; j < K; j++) {
         const int hh = 1 << (K - 1 - j);
@@ -206,10 +216,7 @@ __global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
         for (int i = 0; i < (1 << K); i++) {
             if (i & hh) continue;
             const int tix = (1 << j) - 1 + (i >> (K - j));
-            f256u acc;
-            f256u_zero(acc);
-            comb_apply<1>(acc, mytabs + 16 * tix, myhis[tix], e[i | hh]);
-            e[i] = f128_add(e[i], f256u_reduce(acc));
+            e[i] = f128_add(e[i], shoup8_mul(mytabs + tix * kDeepTabEntries, e[i | hh]));
             e[i | hh] = f128_add(e[i | hh], e[i]);
         }
     }
@@ -228,10 +235,10 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
         if (per > (uint32_t)kDeepMaxHb) per = kDeepMaxHb;
         const uint64_t grid = (n_hb + per - 1) / per;
         if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
-        const size_t dyn = (size_t)per * NT * (16 * sizeof(f128) + sizeof(uint32_t));
+        const size_t dyn = deep_lds_bytes<K>(per);
         static const hipError_t deep_attr = hipFuncSetAttribute(
             (const void*)k_ntt_pass_deep<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-            (int)((size_t)kDeepMaxHb * NT * (16 * sizeof(f128) + sizeof(uint32_t))));
+            (int)deep_lds_bytes<K>(kDeepMaxHb));
         if (deep_attr != hipSuccess) return deep_attr;
         k_ntt_pass_deep<K><<<(unsigned)grid, kDeepThreads, dyn, s>>>(
             src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
