@@ -146,13 +146,15 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict
 }
 
 // Deep passes (at most 256 columns per high block): a 32-row Shoup table per twiddle would be
-// used only a few hundred times and a 512-thread group would mostly idle. Here a 256-thread
-// group takes H = min(256 / cols, 4) consecutive high blocks and builds, per (block, twiddle),
+// used only a few hundred times and a 512-thread group would mostly idle. Here a group of
+// H * cols threads (rounded up to whole waves) takes H = min(256 / cols, 5) consecutive high
+// blocks (50 columns: 250 of 256 lanes busy; 74 KB of LDS for K = 3), and builds, per
+// (block, twiddle),
 // the 8-row (2 KiB) table of shoup8_mul; each thread runs the K layers of one (block, column)
 // with those tables. The tables come first in the group's LDS (address 0), then the twiddle
 // rows and the twiddles.
 constexpr int kDeepThreads = 256;
-constexpr int kDeepMaxHb = 4;       // high blocks per group (tables: 4 x 7 x 2 KiB = 56 KiB)
+constexpr int kDeepMaxHb = 5;       // high blocks per group (tables: 5 x 7 x 2 KiB = 70 KiB)
 constexpr int kDeepTabEntries = 128;  // 8 rows of 16
 
 template <int K>
@@ -240,7 +242,8 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
             (const void*)k_ntt_pass_deep<K>, hipFuncAttributeMaxDynamicSharedMemorySize,
             (int)deep_lds_bytes<K>(kDeepMaxHb));
         if (deep_attr != hipSuccess) return deep_attr;
-        k_ntt_pass_deep<K><<<(unsigned)grid, kDeepThreads, dyn, s>>>(
+        const unsigned threads = (unsigned)((per * cols + 63) / 64 * 64);
+        k_ntt_pass_deep<K><<<(unsigned)grid, threads, dyn, s>>>(
             src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
         return hipGetLastError();
     }
