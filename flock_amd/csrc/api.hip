@@ -1737,10 +1737,14 @@ static void kc_preimage(int s_out, int out[11]) {
     }
 }
 
-int flock_hip_ctx::ensure_kc() {
-    if (kc_inv) return 0;
+// The walker's tables: inv[s * 11 + i] = the 11 state bits whose preimage holds s (phi^T),
+// trow = the chi A / B row positions each of them feeds, rcm[r] = the round-constant vector
+// of state_r as 50 words.
+static int kc_build_tables(std::vector<uint16_t>& inv, std::vector<uint16_t>& trow,
+                           std::vector<uint32_t>& rcm) {
     constexpr int S = 1600;
-    std::vector<uint16_t> inv(S * 11), trow(2 * S * 11);
+    inv.assign(S * 11, 0);
+    trow.assign(2 * S * 11, 0);
     std::vector<int> fill(S, 0);
     std::vector<std::array<int, 11>> pre(S);
     for (int so = 0; so < S; so++) {
@@ -1762,7 +1766,7 @@ int flock_hip_ctx::ensure_kc() {
         }
     }
     // RC_0 = 0, RC_{r+1} = phi(RC_r) + iota's constant on lane (0, 0)
-    std::vector<uint32_t> rcm(25 * 50, 0);
+    rcm.assign(25 * 50, 0);
     std::vector<uint8_t> rc(S, 0), nrc(S);
     for (int r = 0;; r++) {
         for (int s0 = 0; s0 < S; s0++)
@@ -1777,6 +1781,14 @@ int flock_hip_ctx::ensure_kc() {
         for (int z = 0; z < 64; z++)
             if ((kKcRc[r] >> z) & 1) rc[kc_state(0, 0, z)] ^= 1;
     }
+    return 0;
+}
+
+int flock_hip_ctx::ensure_kc() {
+    if (kc_inv) return 0;
+    std::vector<uint16_t> inv, trow;
+    std::vector<uint32_t> rcm;
+    if (int rc = kc_build_tables(inv, trow, rcm)) return rc;
     if (hipMalloc(&kc_inv, inv.size() * 2) != hipSuccess ||
         hipMalloc(&kc_trow, trow.size() * 2) != hipSuccess ||
         hipMalloc(&kc_rcm, rcm.size() * 4) != hipSuccess) {
@@ -1787,6 +1799,59 @@ int flock_hip_ctx::ensure_kc() {
     HIP_TRY(hipMemcpy(kc_trow, trow.data(), trow.size() * 2, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(kc_rcm, rcm.data(), rcm.size() * 4, hipMemcpyHostToDevice));
     return 0;
+}
+
+// The walk of k_kc_walk on the host, for the verifier: sa = A_0^T eq and sb = B_0^T eq over the
+// 2^16 columns (KeccakLincheckCircuit::fold_split, keccak.rs:692-835). Per side, the
+// coefficient vector on state_r is carried from K_24 (the pin rows) down to K_0 through
+// phi^T plus the chi rows of round r - 1, and dropped on the t_(r-2) columns; the constant
+// parts collect on the constant column.
+static void kc_host_fold(const F* eq, F* sa, F* sb, const std::vector<uint16_t>& inv,
+                         const std::vector<uint16_t>& trow, const std::vector<uint32_t>& rcm) {
+    constexpr int S = 1600;
+    constexpr uint32_t kZc = 4096, kS24 = 2048, kT = 4160;
+    std::fill(sa, sa + (1u << 16), F{0, 0});
+    std::fill(sb, sb + (1u << 16), F{0, 0});
+    auto rc_bit = [&](int r, int st) { return (rcm[r * 50 + (st >> 5)] >> (st & 31)) & 1u; };
+    std::vector<F> kv(S), nk(S);
+    for (int side = 0; side < 2; side++) {
+        F* out = side ? sb : sa;
+        const uint16_t* tr = trow.data() + (size_t)side * S * 11;
+        F zc = eq[kZc];  // the constant row
+        for (int st = 0; st < S; st++) {
+            const uint32_t p = (uint32_t)kc_lane_pos(st);
+            const F pin = eq[kS24 + p];
+            if (side == 0) {
+                kv[st] = pin;                       // K_24
+                out[kT + 23u * 1600u + p] = pin;    // t_23 takes K_24
+                if (rc_bit(24, st)) zc = fadd(zc, pin);
+                for (int r = 0; r < 24; r++) zc = fadd(zc, eq[kT + 1600u * r + p]);  // NOT of A
+            } else {
+                kv[st] = F{0, 0};
+                zc = fadd(zc, fadd(pin, eq[p]));  // pin and state_0 rows: B = [1]
+            }
+        }
+        for (int r = 24; r >= 1; r--) {
+            const F* erow = eq + kT + 1600u * (r - 1);
+            for (int st = 0; st < S; st++) {
+                F v{0, 0}, c{0, 0};
+                for (int i = 0; i < 11; i++) {
+                    v = fadd(v, kv[inv[st * 11 + i]]);
+                    c = fadd(c, erow[tr[st * 11 + i]]);
+                }
+                if (rc_bit(r - 1, st)) zc = fadd(zc, c);
+                const F k = fadd(v, c);  // K_(r-1)
+                nk[st] = k;
+                const uint32_t p = (uint32_t)kc_lane_pos(st);
+                if (r >= 2)
+                    out[kT + 1600u * (r - 2) + p] = k;
+                else
+                    out[p] = side == 0 ? fadd(k, eq[p]) : k;  // state_0 self-loop on A
+            }
+            kv.swap(nk);
+        }
+        out[kZc] = zc;
+    }
 }
 
 int flock_hip_lc_keccak_fold(flock_hip_ctx* c, const flock_f128* eq, flock_f128 alpha,
@@ -2037,6 +2102,19 @@ bool host_csc_fold(uint64_t K, const uint32_t* ptr, const uint32_t* rows, uint64
 }
 }  // namespace
 
+// The circuit's column marginals on the host: (eq over the rows) -> (A_0^T eq, B_0^T eq); false
+// when the circuit is malformed.
+using HostFold = std::function<bool(const std::vector<F>& eq, std::vector<F>& sa,
+                                    std::vector<F>& sb)>;
+
+static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_log,
+                                int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                flock_f128 v_a, flock_f128 v_b, int grinding,
+                                const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
+                                flock_f128* h_claim);
+
 int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_log,
                           int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
                           const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
@@ -2044,6 +2122,49 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
                           const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
                           const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
                           flock_f128* h_claim) {
+    if (!h_circuit)
+        return lincheck_verify_impl(fs, nullptr, k_log, const_pin, nu, n_rows, z_skip,
+                                    h_x_inner_rest, h_x_outer, v_a, v_b, grinding, h_rounds,
+                                    h_z_partial, h_matrix_evals, h_nonces, h_claim);
+    if (int rc = csc_check(h_circuit)) return rc;
+    if (h_circuit->k_log != k_log || h_circuit->const_pin != const_pin)
+        return fail(FLOCK_HIP_E_INVALID, "circuit k_log / const_pin differ from the type's");
+    const flock_csc* k = h_circuit;
+    const HostFold fold = [k](const std::vector<F>& eq, std::vector<F>& sa, std::vector<F>& sb) {
+        const uint64_t K = 1ull << k->k_log;
+        return host_csc_fold(K, k->a_col_ptr, k->a_rows, k->nnz_a, eq, sa) &&
+               host_csc_fold(K, k->b_col_ptr, k->b_rows, k->nnz_b, eq, sb);
+    };
+    return lincheck_verify_impl(fs, &fold, k_log, const_pin, nu, n_rows, z_skip, h_x_inner_rest,
+                                h_x_outer, v_a, v_b, grinding, h_rounds, h_z_partial,
+                                h_matrix_evals, h_nonces, h_claim);
+}
+
+int flock_lincheck_verify_keccak(flock_fs* fs, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                 const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                 flock_f128 v_a, flock_f128 v_b, int grinding,
+                                 const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                 const uint64_t* h_nonces, flock_f128* h_claim) {
+    std::vector<uint16_t> inv, trow;
+    std::vector<uint32_t> rcm;
+    if (int rc = kc_build_tables(inv, trow, rcm)) return rc;
+    const HostFold fold = [&](const std::vector<F>& eq, std::vector<F>& sa, std::vector<F>& sb) {
+        kc_host_fold(eq.data(), sa.data(), sb.data(), inv, trow, rcm);
+        return true;
+    };
+    // K_LOG 16, const pin Z_CONST = 4096 (keccak.rs:436-451, 688-690)
+    return lincheck_verify_impl(fs, &fold, 16, 4096, nu, n_rows, z_skip, h_x_inner_rest,
+                                h_x_outer, v_a, v_b, grinding, h_rounds, h_z_partial, nullptr,
+                                h_nonces, h_claim);
+}
+
+static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_log,
+                                int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                flock_f128 v_a, flock_f128 v_b, int grinding,
+                                const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
+                                flock_f128* h_claim) {
     const int ks = 6;
     if (!fs || k_log < (uint32_t)ks || k_log > 24 || nu > 40 || n_rows > (1ull << nu) ||
         const_pin >= ((int64_t)1 << k_log) || const_pin < -1)
@@ -2051,13 +2172,8 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
                                          "nu <= 40, n_rows <= 2^nu, const_pin < 2^k_log)");
     const uint32_t L = k_log - ks;
     if ((L && (!h_x_inner_rest || !h_rounds)) || (nu && !h_x_outer) || !h_z_partial ||
-        (!h_circuit && !h_matrix_evals) || (grinding && !h_nonces))
+        (!fold && !h_matrix_evals) || (grinding && !h_nonces))
         return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (null buffer)");
-    if (h_circuit) {
-        if (int rc = csc_check(h_circuit)) return rc;
-        if (h_circuit->k_log != k_log || h_circuit->const_pin != const_pin)
-            return fail(FLOCK_HIP_E_INVALID, "circuit k_log / const_pin differ from the type's");
-    }
     int ni = 0;
     flock_fs_observe_label(fs, kLabelLincheck, sizeof(kLabelLincheck) - 1);
     // 1. alpha (union.rs:795-804)
@@ -2111,7 +2227,7 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
     }
     // the matrix assertion; one slot filling the boolean region has prefix weights 1
     const std::vector<F> err = eq_table(rr.data(), L);
-    if (!h_circuit) {
+    if (!fold) {
         // check_reported (union.rs:538-576): alpha <W,A_0> + <W,B_0> + beta W_col[pin] = claim,
         // W_col = z_partial (x) eq(rr)
         F acc = fadd(fmul(alpha, fo(h_matrix_evals[0])), fo(h_matrix_evals[1]));
@@ -2130,9 +2246,7 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
             for (uint64_t j = 0; j < ((uint64_t)1 << L); j++)
                 for (int i = 0; i < 64; i++) eqi[j * 64 + i] = fmul(lam[i], ex[j]);
         }
-        if (!host_csc_fold(K, h_circuit->a_col_ptr, h_circuit->a_rows, h_circuit->nnz_a, eqi,
-                           sa) ||
-            !host_csc_fold(K, h_circuit->b_col_ptr, h_circuit->b_rows, h_circuit->nnz_b, eqi, sb))
+        if (!(*fold)(eqi, sa, sb))
             return fail(FLOCK_HIP_E_INVALID, "CSC circuit has a row index >= n_cols or a "
                                              "col_ptr entry out of order or > nnz");
         F part[64];

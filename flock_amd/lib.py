@@ -230,6 +230,7 @@ def load():
         "flock_zerocheck_verify": (I, [P, U32, I, P, P, P, P, P]),
         "flock_lincheck_verify": (I, [P, P, U32, ctypes.c_int64, U32, S, F128, P, P, F128, F128,
                                       I, P, P, P, P, P]),
+        "flock_lincheck_verify_keccak": (I, [P, U32, S, F128, P, P, F128, F128, I, P, P, P, P]),
         "flock_fs_digest": (None, [P, ctypes.c_char_p]),
         "flock_hip_prover_create": (I, [I, ctypes.POINTER(P)]),
         "flock_hip_prover_destroy": (None, [P]),
@@ -675,6 +676,21 @@ def lincheck_verify(fs: "FsChallenger", k_log: int, const_pin: int, nu: int, n_r
         f128(v_a), f128(v_b), int(bool(grinding)), _np_ptr(arr["rounds"]),
         _np_ptr(arr["z_partial"]), _np_ptr(arr["matrix_evals"]), nc.ctypes.data, _np_ptr(claim)))
     del keep
+    return claim
+
+
+def lincheck_verify_keccak(fs: "FsChallenger", nu: int, n_rows: int, z_skip: int, x_inner_rest,
+                           x_outer, v_a: int, v_b: int, grinding: bool, proof: dict) -> np.ndarray:
+    """lincheck_verify for a Keccak-f[1600] slot with the full matrix check (host walk of
+    KeccakLincheckCircuit, keccak.rs:678-835). Returns the claim, or raises E_REJECT."""
+    claim = np.zeros((2 + 10, 2), dtype=np.uint64)
+    arr = {k: np.ascontiguousarray(proof[k], dtype=np.uint64) for k in ("rounds", "z_partial")}
+    nc = np.ascontiguousarray(proof.get("nonces", np.zeros(1)), dtype=np.uint64)
+    xr, xo = _ints_arr(x_inner_rest), _ints_arr(x_outer)
+    _check(load().flock_lincheck_verify_keccak(
+        fs.h, nu, n_rows, f128(z_skip), _np_ptr(xr), _np_ptr(xo), f128(v_a), f128(v_b),
+        int(bool(grinding)), _np_ptr(arr["rounds"]), _np_ptr(arr["z_partial"]), nc.ctypes.data,
+        _np_ptr(claim)))
     return claim
 
 

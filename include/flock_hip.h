@@ -315,6 +315,15 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
                           const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
                           const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
                           flock_f128* h_claim);
+/* flock_lincheck_verify for a Keccak-f[1600] slot (KeccakLincheckCircuit, keccak.rs:678-835;
+ * k_log 16, constant pin at column 4096), the proofs of flock_hip_lincheck_prove_keccak, with
+ * MatrixAssertion::check: the circuit's column marginals come from a host walk of the
+ * permutation's rows (no matrices are formed). Host only. */
+int flock_lincheck_verify_keccak(flock_fs* fs, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+                                 const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                 flock_f128 v_a, flock_f128 v_b, int grinding,
+                                 const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                 const uint64_t* h_nonces, flock_f128* h_claim);
 
 /* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ------------ */
 

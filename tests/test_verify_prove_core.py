@@ -27,10 +27,20 @@ def verify_core(kind, n_log, parts, circuits, domain, digest):
     zc = O.arr_to_ints(claim)
     x_rest, x_outer = F.x_ab_from_mlv(reg, zc[1:1 + n_mlv])
     va, vb = O.arr_to_ints(fin[:2])
-    # the Keccak circuit enters through its walker, not a CSC: the deferred form
-    circuit = None if kind == F.HASH_KECCAK else (*circuits[kind][1], *circuits[kind][2])
-    lclaim = F.lincheck_verify(fs, k_log, pin, n_log, parts["count"], zc[0], x_rest, x_outer, va,
-                               vb, True, parts["lc"], circuit)
+    if kind == F.HASH_KECCAK:
+        # the Keccak circuit enters through its walker: the host walk, and the deferred form
+        fs2 = F.FsChallenger(domain)
+        fs2.bind_statement(digest, [parts["count"]], parts["root"])
+        F.zerocheck_verify(fs2, parts["m"], True, r1, ml, fin, nonces)
+        dclaim = F.lincheck_verify(fs2, k_log, pin, n_log, parts["count"], zc[0], x_rest,
+                                   x_outer, va, vb, True, parts["lc"], None)
+        assert np.array_equal(dclaim, parts["lc"]["claim"])
+        lclaim = F.lincheck_verify_keccak(fs, n_log, parts["count"], zc[0], x_rest, x_outer, va,
+                                          vb, True, parts["lc"])
+    else:
+        lclaim = F.lincheck_verify(fs, k_log, pin, n_log, parts["count"], zc[0], x_rest,
+                                   x_outer, va, vb, True, parts["lc"],
+                                   (*circuits[kind][1], *circuits[kind][2]))
     assert np.array_equal(lclaim, parts["lc"]["claim"])
     return fs.digest()
 
@@ -54,6 +64,26 @@ def test_prove_core_proof_verifies(kind, n_log, n_decl):
     inputs = T.inputs_for(kind, n_decl, 100 + 10 * kind + n_log)
     _, dig = T.oracle_prove_core(kind, n_log, inputs, circuits, parts)
     assert verify_core(kind, n_log, parts, circuits, T.DOMAIN, T.DIGEST) == dig
+
+
+def test_keccak_walk_check_rejects_a_wrong_claim():
+    # an honest Keccak proof passes the full check only if the host walk reproduces the
+    # prover's marginals (test_prove_core_proof_verifies); a proof checked against another
+    # input claim is rejected
+    import test_gpu_prove as T
+    parts = {}
+    T.oracle_prove_core(F.HASH_KECCAK, 3, T.inputs_for(F.HASH_KECCAK, 5, 9), {}, parts)
+    fs = F.FsChallenger(T.DOMAIN)
+    fs.bind_statement(T.DIGEST, [parts["count"]], parts["root"])
+    r1, ml, fin, nonces, _ = parts["zc"]
+    zc = O.arr_to_ints(F.zerocheck_verify(fs, parts["m"], True, r1, ml, fin, nonces))
+    reg = F.registry_new([(16, 42_560, False)], 3)
+    x_rest, x_outer = F.x_ab_from_mlv(reg, zc[1:1 + parts["m"] - 6])
+    va, vb = O.arr_to_ints(fin[:2])
+    with pytest.raises(F.FlockHipError) as e:
+        F.lincheck_verify_keccak(fs, 3, parts["count"], zc[0], x_rest, x_outer, va ^ (1 << 77),
+                                 vb, True, parts["lc"])
+    assert e.value.code == F.E_REJECT
 
 
 def test_prove_core_with_another_statement_is_rejected():
