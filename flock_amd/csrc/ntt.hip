@@ -247,24 +247,13 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
             src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
         return hipGetLastError();
     }
-    // Up to 2 columns per thread, so the twiddle-table build is amortised; groups sized to the
-    // columns in whole waves. One column per thread when that idles fewer lanes (400 columns:
-    // 448 threads instead of 2 x 256).
-    auto shape = [cols](uint64_t cpt, uint64_t& wgs, uint64_t& threads) {
-        wgs = (cols + cpt * kNttThreads - 1) / (cpt * kNttThreads);
-        const uint64_t per = (cols + wgs * cpt - 1) / (wgs * cpt);
-        threads = (per + 63) / 64 * 64;
-        return wgs * threads * cpt;  // column slots
-    };
-    uint64_t wgs, threads, wgs1, threads1;
-    if (shape(1, wgs1, threads1) < shape(2, wgs, threads)) {
-        wgs = wgs1;
-        threads = threads1;
-    }
+    // up to 2 columns per thread so the twiddle-table build is amortised
+    uint64_t wgs = (cols + 2 * kNttThreads - 1) / (2 * kNttThreads);
+    if (wgs < 1) wgs = 1;
     const uint64_t grid = n_hb * wgs;
     if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
-    k_ntt_pass<K><<<(unsigned)grid, (unsigned)threads, 0, s>>>(
-        src, data, t, live, k, l0, evals, cols, (uint32_t)wgs, from_msg ? 1 : 0);
+    k_ntt_pass<K><<<(unsigned)grid, kNttThreads, 0, s>>>(src, data, t, live, k, l0, evals, cols,
+                                                           (uint32_t)wgs, from_msg ? 1 : 0);
     return hipGetLastError();
 }
 
