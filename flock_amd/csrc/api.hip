@@ -1014,9 +1014,13 @@ static int encode_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len
         if (src_len != ((size_t)1 << (p->m - 7)))
             return fail(FLOCK_HIP_E_INVALID, "lane-major message must be the full 2^(m-7) stack");
         const uint64_t d = src_len >> p->log_batch_size;
-        LAUNCH(flock::launch_fill_lane_major(U(src), U(cw), d, (uint32_t)t, reps, c->stream));
-        LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
-                                         (int)p->log_inv_rate, ev, nullptr, c->stream));
+        if (k > p->log_inv_rate) {
+            // replicate_lane_major_fill fused into the first NTT pass
+            LAUNCH(flock::launch_ntt_forward(U(cw), (uint32_t)t, (uint32_t)t, (int)k,
+                                             (int)p->log_inv_rate, ev, U(src), c->stream, d));
+        } else {
+            LAUNCH(flock::launch_fill_lane_major(U(src), U(cw), d, (uint32_t)t, reps, c->stream));
+        }
     } else {
         if (src_len != msg_len) return fail(FLOCK_HIP_E_INVALID, "message length mismatch");
         if (k > p->log_inv_rate) {

@@ -30,9 +30,11 @@ hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, 
 // transforming only lanes < live. evals: k x k basis table (device). If msg != nullptr the
 // first pass reads the 2^(k-start_layer) x t message instead of data (the state after
 // start_layer copy layers is the message replicated 2^start_layer times), so data needs
-// no prior fill.
+// no prior fill. lm_d != 0: the message is lane-major, lane l at msg[l * lm_d ..] (lm_d =
+// 2^(k-start_layer) positions), as commit_lane_major's stack.
 hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
-                              const uint4* evals, const uint4* msg, hipStream_t s);
+                              const uint4* evals, const uint4* msg, hipStream_t s,
+                              uint64_t lm_d = 0);
 
 // One fused pass of `layers` in {1,2,3} layers starting at l0, in place.
 hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0, int layers,

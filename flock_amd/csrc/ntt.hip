@@ -82,14 +82,17 @@ constexpr uint64_t kDeepCols = 256; // up to this many columns per high block: k
 // One fused pass over layers [l0, l0 + K). src == data for an in-place pass; for the
 // first pass of a commit src is the message itself: the state after the log_inv_rate
 // copy layers is the message replicated, so position P reads message position
-// P mod 2^(k-l0) and the group's source offset drops the high block (base) term.
+// P mod 2^(k-l0) and the group's source offset drops the high block (base) term. With
+// lm_d != 0 that message is lane-major (lane l at src[l * lm_d ..]), which fuses
+// replicate_lane_major_fill (pcs/commit.rs:516-536) into the pass.
 template <int K>
 __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict__ src,
                                                            f128* __restrict__ data, uint32_t t,
                                                            uint32_t live, int k, int l0,
                                                            const f128* __restrict__ evals,
                                                            uint64_t cols_per_hb,
-                                                           uint32_t wgs_per_hb, int from_msg) {
+                                                           uint32_t wgs_per_hb, int from_msg,
+                                                           uint64_t lm_d) {
     constexpr int NT = (1 << K) - 1;  // twiddles (and Shoup tables) per group
     // one LDS array, tables first: they sit at LDS address 0, so a lookup's address is its
     // masked nibble plus an immediate
@@ -127,8 +130,16 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_pass(const f128* __restrict
         const uint32_t lane = (uint32_t)(c % t);
         if (lane >= live) continue;  // dead lanes are zero and stay zero
         f128 e[1 << K];
+        if (lm_d) {
+            // lane-major source (the first pass of commit_lane_major): message position
+            // i * S + lowp of this lane is q[lane * D + i * S + lowp]
+            const f128* ql = src + (uint64_t)lane * lm_d + c / t;
 #pragma unroll
-        for (int i = 0; i < (1 << K); i++) e[i] = src[sbase + i * stride + c];
+            for (int i = 0; i < (1 << K); i++) e[i] = ql[i * S];
+        } else {
+#pragma unroll
+            for (int i = 0; i < (1 << K); i++) e[i] = src[sbase + i * stride + c];
+        }
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const int h = 1 << (K - 1 - j);
@@ -228,11 +239,13 @@ __global__ __launch_bounds__(kDeepThreads) void k_ntt_pass_deep(
 
 template <int K>
 static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t live, int k,
-                              int l0, const f128* evals, bool from_msg, hipStream_t s) {
+                              int l0, const f128* evals, bool from_msg, hipStream_t s,
+                              uint64_t lm_d = 0) {
     constexpr int NT = (1 << K) - 1;
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
     if (cols <= kDeepCols) {
+        if (lm_d) return hipErrorInvalidValue;  // the caller fills instead
         uint32_t per = (uint32_t)(kDeepThreads / cols);  // high blocks per group
         if (per > (uint32_t)kDeepMaxHb) per = kDeepMaxHb;
         const uint64_t grid = (n_hb + per - 1) / per;
@@ -253,7 +266,7 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
     const uint64_t grid = n_hb * wgs;
     if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
     k_ntt_pass<K><<<(unsigned)grid, kNttThreads, 0, s>>>(src, data, t, live, k, l0, evals, cols,
-                                                           (uint32_t)wgs, from_msg ? 1 : 0);
+                                                           (uint32_t)wgs, from_msg ? 1 : 0, lm_d);
     return hipGetLastError();
 }
 
@@ -289,20 +302,30 @@ hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0
 }
 
 hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int start_layer,
-                              const uint4* evals, const uint4* msg, hipStream_t s) {
+                              const uint4* evals, const uint4* msg, hipStream_t s,
+                              uint64_t lm_d) {
     int ks[64];
     const int n = plan_passes(k - start_layer, ks);
+    if (lm_d && n > 0 && (1ull << (k - start_layer - ks[0])) * t <= kDeepCols) {
+        // the first pass is a deep one: fill the replicated lane-major message, then transform
+        // in place
+        if (hipError_t e = launch_fill_lane_major(msg, data, lm_d, t, 1u << start_layer, s))
+            return e;
+        msg = nullptr;
+        lm_d = 0;
+    }
     int l = start_layer;
     for (int p = 0; p < n; p++) {
         const bool from_msg = (p == 0) && msg != nullptr;
         const f128* src = from_msg ? msg : data;
         // a message-sourced pass writes every lane (dead lanes copy their zeros)
         const uint32_t lv = from_msg ? t : live;
+        const uint64_t lmd = from_msg ? lm_d : 0;
         hipError_t e;
         switch (ks[p]) {
-            case 3: e = launch_pass<3>(src, data, t, lv, k, l, evals, from_msg, s); break;
-            case 2: e = launch_pass<2>(src, data, t, lv, k, l, evals, from_msg, s); break;
-            default: e = launch_pass<1>(src, data, t, lv, k, l, evals, from_msg, s); break;
+            case 3: e = launch_pass<3>(src, data, t, lv, k, l, evals, from_msg, s, lmd); break;
+            case 2: e = launch_pass<2>(src, data, t, lv, k, l, evals, from_msg, s, lmd); break;
+            default: e = launch_pass<1>(src, data, t, lv, k, l, evals, from_msg, s, lmd); break;
         }
         if (e != hipSuccess) return e;
         l += ks[p];
