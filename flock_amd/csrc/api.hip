@@ -20,6 +20,7 @@
 #if defined(__x86_64__)
 #include <cpuid.h>
 #include <emmintrin.h>
+#include <immintrin.h>
 #include <wmmintrin.h>
 #endif
 
@@ -225,6 +226,58 @@ const uint32_t K256[64] = {
     0x19a4c116, 0x1e376c08, 0x2748774c, 0x34b0bcb5, 0x391c0cb3, 0x4ed8aa4a, 0x5b9cca4f, 0x682e6ff3,
     0x748f82ee, 0x78a5636f, 0x84c87814, 0x8cc70208, 0x90befffa, 0xa4506ceb, 0xbef9a3f7, 0xc67178f2};
 
+#if defined(__x86_64__)
+// One block on the x86 SHA extensions. The state lives as (A,B,E,F) and (C,D,G,H) halves;
+// sha256rnds2 runs two rounds from them and the next two W+K words, msg1 / msg2 extend the
+// message schedule four words at a time (W[t] = W[t-16] + s0(W[t-15]) + W[t-7] + s1(W[t-2])).
+__attribute__((target("sha,sse4.1,ssse3"))) void sha256_block_ni(uint32_t h[8],
+                                                                 const uint8_t* p) {
+    const __m128i bswap = _mm_set_epi64x(0x0c0d0e0f08090a0bLL, 0x0405060700010203LL);
+    __m128i lo = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&h[0]), 0xB1);  // b a d c
+    __m128i hi = _mm_shuffle_epi32(_mm_loadu_si128((const __m128i*)&h[4]), 0x1B);  // h g f e
+    __m128i abef = _mm_alignr_epi8(lo, hi, 8);      // f e b a (dword 0 first)
+    __m128i cdgh = _mm_blend_epi16(hi, lo, 0xF0);   // h g d c
+    const __m128i abef0 = abef, cdgh0 = cdgh;
+    __m128i m[4];
+    for (int i = 0; i < 4; i++)
+        m[i] = _mm_shuffle_epi8(_mm_loadu_si128((const __m128i*)(p + 16 * i)), bswap);
+    for (int g = 0; g < 16; g++) {
+        __m128i w;
+        if (g < 4) {
+            w = m[g];
+        } else {
+            w = _mm_sha256msg2_epu32(
+                _mm_add_epi32(_mm_sha256msg1_epu32(m[0], m[1]), _mm_alignr_epi8(m[3], m[2], 4)),
+                m[3]);
+            m[0] = m[1];
+            m[1] = m[2];
+            m[2] = m[3];
+            m[3] = w;
+        }
+        __m128i wk = _mm_add_epi32(w, _mm_loadu_si128((const __m128i*)&K256[4 * g]));
+        cdgh = _mm_sha256rnds2_epu32(cdgh, abef, wk);  // (A,B,E,F) two rounds on
+        wk = _mm_shuffle_epi32(wk, 0x0E);
+        abef = _mm_sha256rnds2_epu32(abef, cdgh, wk);
+    }
+    abef = _mm_add_epi32(abef, abef0);
+    cdgh = _mm_add_epi32(cdgh, cdgh0);
+    lo = _mm_shuffle_epi32(abef, 0x1B);    // a b e f
+    hi = _mm_shuffle_epi32(cdgh, 0xB1);    // g h c d
+    _mm_storeu_si128((__m128i*)&h[0], _mm_blend_epi16(lo, hi, 0xF0));  // a b c d
+    _mm_storeu_si128((__m128i*)&h[4], _mm_alignr_epi8(hi, lo, 8));    // e f g h
+}
+bool have_sha_ni() {
+    static const bool v = [] {
+        unsigned a, b, c, d;
+        return __get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & (1u << 29)) &&
+               __get_cpuid(1, &a, &b, &c, &d) && (c & bit_SSE4_1) && (c & bit_SSSE3);
+    }();
+    return v;
+}
+#endif
+
+void sha256_block(uint32_t h[8], const uint8_t* p);
+
 struct Sha256 {
     uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
                      0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
@@ -233,7 +286,8 @@ struct Sha256 {
     size_t blen = 0;
 
     static uint32_t rr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
-    void block(const uint8_t* p) {
+    void block(const uint8_t* p) { sha256_block(h, p); }
+    static void block_portable(uint32_t h[8], const uint8_t* p) {
         uint32_t w[64];
         for (int i = 0; i < 16; i++)
             w[i] = (uint32_t)p[4 * i] << 24 | (uint32_t)p[4 * i + 1] << 16 |
@@ -259,18 +313,60 @@ struct Sha256 {
         }
     }
     void digest(uint8_t out[32]) const {
-        Sha256 c = *this;
-        uint64_t bits = c.total * 8;
-        uint8_t pad = 0x80, z = 0;
-        c.update(&pad, 1);
-        while (c.blen != 56) c.update(&z, 1);
-        uint8_t len[8];
-        for (int i = 0; i < 8; i++) len[i] = (uint8_t)(bits >> (56 - 8 * i));
-        c.update(len, 8);
+        uint32_t st[8];
+        memcpy(st, h, sizeof(st));
+        uint8_t b[64];
+        memcpy(b, buf, blen);
+        b[blen] = 0x80;
+        if (blen >= 56) {
+            memset(b + blen + 1, 0, 63 - blen);
+            sha256_block(st, b);
+            memset(b, 0, 56);
+        } else {
+            memset(b + blen + 1, 0, 55 - blen);
+        }
+        const uint64_t bits = total * 8;
+        for (int i = 0; i < 8; i++) b[56 + i] = (uint8_t)(bits >> (56 - 8 * i));
+        sha256_block(st, b);
         for (int i = 0; i < 8; i++)
-            for (int j = 0; j < 4; j++) out[4 * i + j] = (uint8_t)(c.h[i] >> (24 - 8 * j));
+            for (int j = 0; j < 4; j++) out[4 * i + j] = (uint8_t)(st[i] >> (24 - 8 * j));
     }
 };
+
+void sha256_block(uint32_t h[8], const uint8_t* p) {
+#if defined(__x86_64__)
+    if (have_sha_ni()) {
+        sha256_block_ni(h, p);
+        return;
+    }
+#endif
+    Sha256::block_portable(h, p);
+}
+
+// PoW predicate (challenger.rs:884-958): SHA256(digest || nonce_le) starts with `bits` zero
+// bits, MSB first. `blk` is the padded one-block message with the digest in bytes 0..31; the
+// nonce goes into bytes 32..39.
+bool pow_ok(uint8_t blk[64], uint64_t nonce, uint32_t bits) {
+    if (bits > 256) return false;
+    for (int i = 0; i < 8; i++) blk[32 + i] = (uint8_t)(nonce >> (8 * i));
+    uint32_t h[8] = {0x6a09e667, 0xbb67ae85, 0x3c6ef372, 0xa54ff53a,
+                     0x510e527f, 0x9b05688c, 0x1f83d9ab, 0x5be0cd19};
+    sha256_block(h, blk);
+    for (int i = 0; i < 8 && bits; i++) {
+        const uint32_t take = bits < 32 ? bits : 32;
+        if (take == 32 ? h[i] != 0 : (h[i] >> (32 - take)) != 0) return false;
+        bits -= take;
+    }
+    return true;
+}
+
+void pow_block(const uint8_t digest[32], uint8_t blk[64]) {
+    memcpy(blk, digest, 32);
+    memset(blk + 32, 0, 32);
+    blk[40] = 0x80;
+    blk[62] = 0x01;  // 320 bits, big-endian in bytes 56..63
+    blk[63] = 0x40;
+}
 
 }  // namespace
 
@@ -383,16 +479,9 @@ int flock_fs_verify_pow(flock_fs* fs, uint64_t nonce, uint32_t bits) {
     fs->st.digest(digest);
     bool ok = nonce == 0;
     if (bits) {
-        uint8_t pre[40], h[32];
-        memcpy(pre, digest, 32);
-        for (int i = 0; i < 8; i++) pre[32 + i] = (uint8_t)(nonce >> (8 * i));
-        Sha256 sh;
-        sh.update(pre, 40);
-        sh.digest(h);
-        const uint32_t full = bits / 8, extra = bits % 8;
-        ok = full <= 32;
-        for (uint32_t i = 0; i < full && ok; i++) ok = h[i] == 0;
-        if (ok && extra && full < 32) ok = (h[full] >> (8 - extra)) == 0;
+        uint8_t blk[64];
+        pow_block(digest, blk);
+        ok = pow_ok(blk, nonce, bits);
     }
     uint8_t nb[8];
     for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
@@ -405,19 +494,9 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
     fs->st.digest(digest);
     uint64_t nonce = 0;
     if (bits) {
-        for (;; nonce++) {
-            uint8_t pre[40], h[32];
-            memcpy(pre, digest, 32);
-            for (int i = 0; i < 8; i++) pre[32 + i] = (uint8_t)(nonce >> (8 * i));
-            Sha256 s;
-            s.update(pre, 40);
-            s.digest(h);
-            uint32_t full = bits / 8, extra = bits % 8;
-            bool ok = true;
-            for (uint32_t i = 0; i < full && ok; i++) ok = h[i] == 0;
-            if (ok && extra) ok = (h[full] >> (8 - extra)) == 0;
-            if (ok) break;
-        }
+        uint8_t blk[64];
+        pow_block(digest, blk);
+        while (!pow_ok(blk, nonce, bits)) nonce++;
     }
     uint8_t nb[8];
     for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
