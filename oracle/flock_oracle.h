@@ -260,6 +260,26 @@ int or_blake3_witness_batch_major(const uint32_t* inputs, size_t n_decl, int n_l
 /* build_matrices (blake3.rs:659-783) as CSR; a_ptr/b_ptr have 16385 entries */
 int or_blake3_matrices(uint64_t* a_ptr, uint32_t* a_cols, uint64_t* b_ptr, uint32_t* b_cols);
 
+/* ---- merged PCS opening (crates/flock-core/src/pcs.rs:1433-1822), opening.c ---- */
+/* initial_k of the embedded `fast` Ligerito config for m (configs/ligerito/m*_fast.toml), or -1 */
+int or_lig_initial_k(unsigned m);
+/* PcsParams::l0_cap_depth (pcs/commit.rs:266-277): the L0 schedule's cap depth, or -1 */
+int or_cap_depth_for(unsigned m);
+/* f̂_t(z_row, z_col, x) of the jagged layout with the given column heights (jagged.rs:243-262),
+ * through the branching program; n = row vars, m = dense vars */
+int or_jag_fhat(const uint64_t* heights, size_t nh, int n, int m, const or_f128* z_row,
+                const or_f128* z_col, const or_f128* x, or_f128* out);
+/* open_batch_merged for n_claims ring-switched claims with precomputed s_hat_v (128 each).
+ * q: the committed dense stack (2^(m-7) words); cw/tree: the L0 commitment (full tree);
+ * x_outers: n_claims points of x_len = 1 + n_log + log2(n_heights) words (quirky x_outer_full);
+ * heights: jagged column heights. On success *out_bytes (free with or_free_bytes) holds the
+ * bincode of MergedOpenProof (pcs.rs:1362-1377). */
+int or_open_merged(or_fs* fs, const or_f128* q, const or_pcs_params* p, const or_f128* cw,
+                   const uint8_t* tree, size_t n_claims, const or_f128* x_outers, size_t x_len,
+                   const or_f128* s_hat_v, const uint64_t* heights, size_t n_heights,
+                   unsigned n_log, uint8_t** out_bytes, size_t* out_len, or_f128* q_eval_out);
+void or_free_bytes(uint8_t* p);
+
 #ifdef __cplusplus
 }
 #endif
