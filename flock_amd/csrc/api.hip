@@ -866,6 +866,12 @@ int flock_hip_device_count(void) {
 
 const char* flock_hip_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+int flock::set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
+extern "C" {
+
 int flock_hip_ctx_set_round_mode(flock_hip_ctx* c, int mode) {
     CTX_CHECK(c);
     if (mode != FLOCK_ROUNDS_HOST && mode != FLOCK_ROUNDS_DEVICE)

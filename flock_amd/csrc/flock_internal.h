@@ -172,4 +172,7 @@ hipError_t launch_blake3_witness(const uint32_t* inputs, uint64_t n_decl, uint32
 hipError_t launch_sha2_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t n_log,
                                uint4* z, uint4* a, uint4* b, uint8_t* stripe, hipStream_t s);
 
+// Record an error for flock_hip_last_error (implemented in api.hip); returns code.
+int set_error(int code, const char* msg);
+
 }  // namespace flock

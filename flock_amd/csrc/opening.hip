@@ -1,0 +1,1860 @@
+// The merged PCS opening of prove_fast on the GPU (crates/flock-core/src/pcs.rs:1433-1822).
+//
+// Host orchestration over device-resident tables, in the reference's transcript order:
+//   1. ring switch of the ring-switched claims (pcs/ring_switch.rs:2723-2800) — host, 128 words;
+//   2. the jagged merged weight W[d] = sum_c Phi_c(eq_row_c[row(d)] eq_col_c[col(d)]) over the
+//      dense cube with its round-0 message fused (pcs/jagged.rs:1521-1650), then the dense merged
+//      sumcheck <q, W> (pcs.rs:1640-1700) on the product-sumcheck kernels;
+//   3. on a forked transcript, the multipoint twisted assist (pcs/jagged.rs:2884-3290): the
+//      128 dual values per claim by the jagged branching program, the two-product sumcheck
+//      sum_d abar_d g_d with g = L_gamma(eq(rho, .)), and the Frobenius anchor (host);
+//   4. on the parent transcript, the eq-basis Ligerito opening of q^(rho): the F256 ladder with
+//      code switches, level commits (NTT + Merkle), OOD samples, stratified queries and capped
+//      paths (pcs.rs:214-560, pcs/ligerito/extension.rs:1636-2290);
+//   5. the bincode 1.3 encoding of MergedOpenProof (pcs.rs:1362-1377).
+//
+// Every sum the protocol sends is an XOR reduction, so kernels reduce by block and combine
+// block results with atomicXor: the value does not depend on the order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#if defined(__x86_64__)
+#include <cpuid.h>
+#include <emmintrin.h>
+#include <wmmintrin.h>
+#endif
+
+#include "../../include/flock_hip.h"
+#include "f128_device.h"
+#include "flock_internal.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// host field arithmetic
+// ---------------------------------------------------------------------------
+using H = flock_f128;
+inline H hz() { return H{0, 0}; }
+inline H h1() { return H{1, 0}; }
+inline H ha(H a, H b) { return H{a.lo ^ b.lo, a.hi ^ b.hi}; }
+inline bool hzero(H a) { return !(a.lo | a.hi); }
+inline H hbit(int t) {
+    return H{t < 64 ? 1ull << t : 0, t >= 64 ? 1ull << (t - 64) : 0};
+}
+inline int hgetbit(H x, int t) { return (int)(((t < 64 ? x.lo : x.hi) >> (t & 63)) & 1); }
+
+inline H hreduce(uint64_t r0, uint64_t r1, uint64_t r2, uint64_t r3) {
+    uint64_t tl = r2 ^ (r2 << 1) ^ (r2 << 2) ^ (r2 << 7);
+    uint64_t th = r3 ^ ((r3 << 1) | (r2 >> 63)) ^ ((r3 << 2) | (r2 >> 62)) ^ ((r3 << 7) | (r2 >> 57));
+    uint64_t ov = (r3 >> 63) ^ (r3 >> 62) ^ (r3 >> 57);
+    uint64_t corr = ov ^ (ov << 1) ^ (ov << 2) ^ (ov << 7);
+    return H{r0 ^ tl ^ corr, r1 ^ th};
+}
+#if defined(__x86_64__)
+__attribute__((target("pclmul,sse2"))) H hmul_clmul(H a, H b) {
+    __m128i A = _mm_set_epi64x((long long)a.hi, (long long)a.lo);
+    __m128i B = _mm_set_epi64x((long long)b.hi, (long long)b.lo);
+    __m128i ll = _mm_clmulepi64_si128(A, B, 0x00), hh = _mm_clmulepi64_si128(A, B, 0x11);
+    __m128i cr = _mm_xor_si128(_mm_clmulepi64_si128(A, B, 0x10), _mm_clmulepi64_si128(A, B, 0x01));
+    uint64_t l[2], h[2], c[2];
+    _mm_storeu_si128((__m128i*)l, ll);
+    _mm_storeu_si128((__m128i*)h, hh);
+    _mm_storeu_si128((__m128i*)c, cr);
+    return hreduce(l[0], l[1] ^ c[0], h[0] ^ c[1], h[1]);
+}
+bool has_clmul() {
+    static const bool v = [] {
+        unsigned a, b, c, d;
+        return __get_cpuid(1, &a, &b, &c, &d) && (c & bit_PCLMUL);
+    }();
+    return v;
+}
+#endif
+void clmul64h(uint64_t a, uint64_t b, uint64_t* lo, uint64_t* hi) {
+    uint64_t l = 0, h = 0;
+    for (int i = 0; i < 64; i++)
+        if ((a >> i) & 1) {
+            l ^= b << i;
+            if (i) h ^= b >> (64 - i);
+        }
+    *lo = l;
+    *hi = h;
+}
+H hm(H a, H b) {
+#if defined(__x86_64__)
+    if (has_clmul()) return hmul_clmul(a, b);
+#endif
+    uint64_t ll, lh, hl, hh, t0, t1, t2, t3;
+    clmul64h(a.lo, b.lo, &ll, &t0);
+    clmul64h(a.lo, b.hi, &lh, &t1);
+    clmul64h(a.hi, b.lo, &hl, &t2);
+    clmul64h(a.hi, b.hi, &hh, &t3);
+    return hreduce(ll, t0 ^ lh ^ hl, hh ^ t1 ^ t2, t3);
+}
+H hinv(H x) {  // x^(2^128 - 2)
+    H r = h1(), cur = hm(x, x);
+    for (int i = 1; i < 128; i++) {
+        r = hm(r, cur);
+        cur = hm(cur, cur);
+    }
+    return r;
+}
+std::vector<H> heq(const H* r, size_t n) {  // build_eq, LSB-first
+    std::vector<H> t((size_t)1 << n);
+    t[0] = h1();
+    for (size_t i = 0; i < n; i++) {
+        const size_t len = (size_t)1 << i;
+        const H rp = ha(h1(), r[i]);
+        for (size_t x = 0; x < len; x++) {
+            t[x + len] = hm(t[x], r[i]);
+            t[x] = hm(t[x], rp);
+        }
+    }
+    return t;
+}
+H heq_at(const H* a, const H* b, size_t n) {
+    H acc = h1();
+    for (size_t i = 0; i < n; i++) acc = hm(acc, ha(hm(a[i], b[i]), hm(ha(h1(), a[i]), ha(h1(), b[i]))));
+    return acc;
+}
+
+// F256 = F128[u]/(u^2 + u + x^-1) (field/gf2_256.rs:1-160)
+struct E {
+    H c0, c1;
+};
+inline E ez() { return E{hz(), hz()}; }
+inline E e1() { return E{h1(), hz()}; }
+inline E ea(E a, E b) { return E{ha(a.c0, b.c0), ha(a.c1, b.c1)}; }
+inline H hmulxinv(H z) {
+    const uint64_t mask = 0 - (z.lo & 1);
+    return H{((z.lo >> 1) | (z.hi << 63)) ^ (0x43 & mask), (z.hi >> 1) ^ ((1ull << 63) & mask)};
+}
+inline E em(E a, E b) {
+    H p0 = hm(a.c0, b.c0), p1 = hm(a.c1, b.c1), p2 = hm(ha(a.c0, a.c1), ha(b.c0, b.c1));
+    return E{ha(p0, hmulxinv(p1)), ha(p2, p0)};
+}
+inline E emb(E a, H b) { return E{hm(a.c0, b), hm(a.c1, b)}; }
+std::vector<E> heq256(const E* r, size_t n) {
+    std::vector<E> t((size_t)1 << n);
+    t[0] = e1();
+    for (size_t j = 0; j < n; j++) {
+        const size_t len = (size_t)1 << j;
+        const E rp = ea(e1(), r[j]);
+        for (size_t i = 0; i < len; i++) {
+            const E v = t[i];
+            t[i + len] = em(v, r[j]);
+            t[i] = em(v, rp);
+        }
+    }
+    return t;
+}
+
+// x^(2^-1) = x^(2^127) through the square-root images of the bit basis (jagged.rs:2266-2300)
+const H* sqrt_basis() {
+    static const std::vector<H> t = [] {
+        std::vector<H> v(128);
+        for (int b = 0; b < 128; b++) {
+            H y = hbit(b);
+            for (int i = 0; i < 127; i++) y = hm(y, y);
+            v[b] = y;
+        }
+        return v;
+    }();
+    return t.data();
+}
+H frob_inv(H x) {
+    const H* t = sqrt_basis();
+    H acc = hz();
+    for (int b = 0; b < 128; b++)
+        if (hgetbit(x, b)) acc = ha(acc, t[b]);
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// bincode 1.3 (fixint, little-endian, u64 lengths)
+// ---------------------------------------------------------------------------
+struct Bin {
+    std::vector<uint8_t> b;
+    void u64(uint64_t v) {
+        for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+    }
+    void f(H v) {
+        u64(v.lo);
+        u64(v.hi);
+    }
+    void e(E v) {
+        f(v.c0);
+        f(v.c1);
+    }
+    void raw(const uint8_t* p, size_t n) { b.insert(b.end(), p, p + n); }
+    void vec_f(const H* v, size_t n) {
+        u64(n);
+        for (size_t i = 0; i < n; i++) f(v[i]);
+    }
+    void vec_u64(const uint64_t* v, size_t n) {
+        u64(n);
+        for (size_t i = 0; i < n; i++) u64(v[i]);
+    }
+    void vec_hash(const uint8_t* h, size_t n) {
+        u64(n);
+        raw(h, 32 * n);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Ligerito `fast` configurations (crates/flock-core/configs/ligerito/m{22..35}_fast.toml,
+// looked up by ligerito.rs:542-620): per level rate, log message columns, log lanes,
+// queries, query-PoW bits, claim-batch bits, consistency-batch bits, OOD samples.
+// ---------------------------------------------------------------------------
+struct Lvl {
+    uint8_t rate, cols, lanes;
+    uint16_t queries;
+    uint8_t grind, claim, cons, ood;
+};
+struct Lig {
+    uint8_t m, initial_k, n_levels;
+    Lvl lv[7];
+};
+const Lig kLigFast[] = {
+    {22, 6, 3, {{1, 9, 6, 279, 0, 6, 9, 1}, {2, 6, 4, 136, 0, 6, 9, 2}, {4, 3, 4, 68, 0, 7, 10, 2}}},
+    {23, 6, 3, {{1, 10, 6, 279, 0, 6, 9, 1}, {2, 7, 4, 136, 0, 6, 9, 2}, {3, 4, 4, 91, 0, 7, 9, 2}}},
+    {24, 6, 3, {{1, 11, 6, 279, 0, 6, 9, 1}, {2, 8, 4, 136, 0, 6, 9, 2}, {3, 5, 4, 91, 0, 7, 9, 2}}},
+    {25, 6, 4, {{1, 12, 6, 279, 0, 6, 9, 1}, {2, 9, 4, 136, 0, 6, 9, 2}, {3, 6, 4, 91, 0, 7, 9, 2},
+                {4, 3, 4, 68, 0, 7, 10, 2}}},
+    {26, 6, 4, {{1, 13, 6, 279, 0, 6, 9, 1}, {2, 10, 4, 136, 0, 6, 9, 2}, {3, 7, 4, 91, 0, 7, 9, 2},
+                {4, 4, 4, 68, 0, 7, 10, 2}}},
+    {27, 6, 4, {{1, 14, 6, 279, 0, 6, 9, 1}, {2, 11, 4, 136, 0, 6, 9, 2}, {3, 8, 4, 91, 0, 7, 9, 2},
+                {4, 5, 4, 68, 0, 7, 10, 2}}},
+    {28, 4, 5, {{1, 17, 4, 279, 0, 6, 9, 1}, {2, 14, 4, 136, 0, 6, 9, 2}, {3, 11, 4, 91, 0, 7, 9, 2},
+                {4, 8, 4, 68, 0, 7, 10, 2}, {5, 5, 4, 55, 0, 8, 10, 2}}},
+    {29, 5, 5, {{1, 17, 5, 279, 0, 6, 9, 1}, {2, 14, 4, 136, 0, 6, 9, 2}, {3, 11, 4, 91, 0, 7, 9, 2},
+                {4, 8, 4, 68, 0, 7, 10, 2}, {5, 5, 4, 55, 0, 8, 10, 2}}},
+    {30, 6, 5, {{1, 17, 6, 279, 0, 6, 9, 1}, {2, 14, 4, 136, 0, 6, 9, 2}, {3, 11, 4, 91, 0, 7, 9, 2},
+                {4, 8, 4, 68, 0, 7, 10, 2}, {5, 5, 4, 55, 0, 8, 10, 2}}},
+    {31, 6, 6, {{1, 18, 6, 279, 0, 6, 9, 1}, {2, 15, 4, 136, 0, 6, 9, 2}, {3, 12, 4, 91, 0, 7, 9, 2},
+                {4, 9, 4, 68, 0, 7, 10, 2}, {5, 6, 4, 55, 0, 8, 10, 2}, {6, 3, 4, 46, 0, 8, 11, 2}}},
+    {32, 6, 6, {{1, 19, 6, 279, 0, 6, 9, 1}, {2, 16, 4, 136, 0, 6, 9, 2}, {3, 13, 4, 91, 0, 7, 9, 2},
+                {4, 10, 4, 68, 0, 7, 10, 2}, {5, 7, 4, 55, 0, 8, 10, 2}, {6, 4, 4, 46, 0, 8, 11, 2}}},
+    {33, 6, 6, {{1, 20, 6, 279, 0, 6, 9, 1}, {2, 17, 4, 136, 0, 6, 9, 2}, {3, 14, 4, 91, 0, 7, 9, 2},
+                {4, 11, 4, 68, 0, 7, 10, 2}, {5, 8, 4, 55, 0, 8, 10, 2}, {6, 5, 4, 46, 0, 8, 11, 2}}},
+    {34, 6, 7, {{1, 21, 6, 279, 0, 6, 9, 1}, {2, 18, 4, 136, 0, 6, 9, 2}, {3, 15, 4, 91, 0, 7, 9, 2},
+                {4, 12, 4, 68, 0, 7, 10, 2}, {5, 9, 4, 55, 0, 8, 10, 2}, {6, 6, 4, 46, 0, 8, 11, 2},
+                {7, 3, 4, 40, 0, 9, 11, 2}}},
+    {35, 6, 7, {{1, 22, 6, 279, 0, 6, 9, 1}, {2, 19, 4, 136, 0, 6, 9, 2}, {3, 16, 4, 91, 0, 7, 9, 2},
+                {4, 13, 4, 68, 0, 7, 10, 2}, {5, 10, 4, 55, 0, 8, 10, 2}, {6, 7, 4, 46, 0, 8, 11, 2},
+                {7, 4, 4, 40, 0, 9, 11, 2}}},
+};
+const Lig* lig_for(uint32_t m) {
+    for (const Lig& c : kLigFast)
+        if (c.m == m) return &c;
+    return nullptr;
+}
+
+// stratified schedule (pcs/stratified.rs:73-130)
+struct Sched {
+    int d = 0;
+    std::vector<int> depth;
+    int cap() const { return depth.empty() ? 0 : depth[0]; }
+    size_t queries() const {
+        size_t q = 0;
+        for (int c : depth) q += (size_t)1 << c;
+        return q;
+    }
+};
+Sched sched_of(size_t q, int d) {
+    Sched s;
+    s.d = d;
+    while (q) {
+        int c = 63 - __builtin_clzll((unsigned long long)q);
+        if (c > d) c = d;
+        s.depth.push_back(c);
+        q -= (size_t)1 << c;
+    }
+    return s;
+}
+// queries_from_words (ligerito.rs:4530-4547)
+std::vector<uint64_t> queries_from(const Sched& s, const std::vector<H>& w) {
+    std::vector<uint64_t> out;
+    size_t k = 0;
+    for (int c : s.depth) {
+        const int lo = s.d - c;
+        for (uint64_t st = 0; st < (1ull << c); st++, k++)
+            out.push_back((st << lo) | (w[k].lo & ((1ull << lo) - 1)));
+    }
+    return out;
+}
+size_t ceil_log2(size_t n) {
+    if (n <= 1) return 0;
+    size_t b = 0;
+    for (n -= 1; n; n >>= 1) b++;
+    return b;
+}
+uint32_t bits_for_degree(size_t d) {
+    uint32_t b = 0;
+    for (; d; d >>= 1) b++;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// transcript helpers (challenger.rs:57-216 default methods over the host FsChallenger)
+// ---------------------------------------------------------------------------
+H sample(flock_fs* fs) {
+    H v;
+    flock_fs_sample_f128(fs, &v);
+    return v;
+}
+std::vector<H> sample_vec(flock_fs* fs, size_t n) {
+    std::vector<H> v(n ? n : 1);
+    flock_fs_sample_f128_vec(fs, v.data(), n);
+    v.resize(n);
+    return v;
+}
+E sample_e(flock_fs* fs) {
+    std::vector<H> w = sample_vec(fs, 2);
+    return E{w[0], w[1]};
+}
+void observe(flock_fs* fs, H v) { flock_fs_observe_f128(fs, v); }
+void observe_e(flock_fs* fs, E v) {
+    H w[2] = {v.c0, v.c1};
+    flock_fs_observe_f128_slice(fs, w, 2);
+}
+void label(flock_fs* fs, const char* s) {
+    flock_fs_observe_label(fs, reinterpret_cast<const uint8_t*>(s), std::strlen(s));
+}
+H grind_sample(flock_fs* fs, uint32_t bits, uint64_t* nonce) {
+    *nonce = flock_fs_grind_pow(fs, bits);
+    return sample(fs);
+}
+// fold_round_claim (jagged.rs:4592-4597): G(r) from (claim, G(1), G(inf))
+H round_claim(H claim, H g1, H gi, H r) {
+    const H g0 = ha(claim, g1);
+    return ha(ha(g0, hm(ha(ha(g1, g0), gi), r)), hm(gi, hm(r, r)));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// device kernels
+// ---------------------------------------------------------------------------
+namespace flock {
+namespace {
+
+struct e256d {
+    f128 c0, c1;
+};
+__device__ __forceinline__ e256d eadd(e256d a, e256d b) {
+    return e256d{f128_add(a.c0, b.c0), f128_add(a.c1, b.c1)};
+}
+__device__ __forceinline__ f128 mul_x_inv_d(f128 z) {
+    const uint32_t mask = 0u - (z.x & 1u);
+    return make_uint4(((z.x >> 1) | (z.y << 31)) ^ (0x43u & mask), (z.y >> 1) | (z.z << 31),
+                      (z.z >> 1) | (z.w << 31), (z.w >> 1) ^ (0x80000000u & mask));
+}
+__device__ __forceinline__ e256d emul(e256d a, e256d b) {
+    const f128 p0 = f128_mul(a.c0, b.c0), p1 = f128_mul(a.c1, b.c1);
+    const f128 p2 = f128_mul(f128_add(a.c0, a.c1), f128_add(b.c0, b.c1));
+    return e256d{f128_add(p0, mul_x_inv_d(p1)), f128_add(p2, p0)};
+}
+__device__ __forceinline__ e256d emulb(e256d a, f128 b) {
+    return e256d{f128_mul(a.c0, b), f128_mul(a.c1, b)};
+}
+
+constexpr int kThreads = 256;
+
+// XOR-reduce W words across the block and fold them into dst with atomicXor.
+template <int W>
+__device__ __forceinline__ void block_xor_atomic(uint32_t (&v)[W], uint32_t* dst) {
+    __shared__ uint32_t sm[kThreads / 64][W];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < W; k++) v[k] ^= __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < W; k++) sm[wid][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < W) {
+        uint32_t acc = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) acc ^= sm[w][threadIdx.x];
+        if (acc) atomicXor(dst + threadIdx.x, acc);
+    }
+}
+__device__ __forceinline__ void put4(uint32_t* v, f128 a) {
+    v[0] ^= a.x;
+    v[1] ^= a.y;
+    v[2] ^= a.z;
+    v[3] ^= a.w;
+}
+
+__global__ void k_zero32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// F2-linear map through 32 nibble tables: T[k * 16 + n] = Phi(n x^(4k)) (16 entries per row)
+__device__ __forceinline__ f128 lin_apply_d(const f128* T, f128 x) {
+    f128 acc = f128_zero();
+    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc = f128_add(acc, T[(q * 8 + j) * 16 + ((w[q] >> (4 * j)) & 15)]);
+    return acc;
+}
+
+constexpr int kMaxClaims = 4;
+struct JagClaims {
+    const f128* er[kMaxClaims];  // eq(z_row_c, .) over 2^n rows
+    const f128* ec[kMaxClaims];  // eq(z_col_c, .) (scaled for the scalar form) over the columns
+    const f128* tab[kMaxClaims]; // 512-word nibble tables of Phi_c (weight form only)
+    uint32_t n;
+};
+
+__device__ __forceinline__ uint32_t col_of(const uint64_t* ps, uint32_t n_cols, uint64_t d) {
+    // the column whose range [ps[c], ps[c+1]) holds d
+    uint32_t lo = 0, hi = n_cols;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ps[mid] <= d)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// W[d] = sum_c Phi_c(er_c[row(d)] ec_c[col(d)]) for d < area, zero past it; acc gets the
+// round-0 message (u0, u2) of <q, W> over LSB pairs (jagged.rs:1521-1650).
+__global__ void __launch_bounds__(kThreads) k_jag_weight(const f128* q, f128* W, uint64_t N,
+                                                         uint64_t area, const uint64_t* ps,
+                                                         uint32_t n_cols, JagClaims jc,
+                                                         uint32_t* acc) {
+    __shared__ f128 T[kMaxClaims * 512];
+    for (uint32_t i = threadIdx.x; i < jc.n * 512; i += blockDim.x) T[i] = jc.tab[i / 512][i % 512];
+    __syncthreads();
+    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t u = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < N / 2;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        f128 w2[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint64_t d = 2 * u + e;
+            f128 s = f128_zero();
+            if (d < area) {
+                const uint32_t c = col_of(ps, n_cols, d);
+                const uint64_t row = d - ps[c];
+                for (uint32_t k = 0; k < jc.n; k++)
+                    s = f128_add(s, lin_apply_d(T + k * 512, f128_mul(jc.er[k][row], jc.ec[k][c])));
+            }
+            w2[e] = s;
+            W[d] = s;
+        }
+        const f128 q0 = q[2 * u], q1 = q[2 * u + 1];
+        put4(v, f128_mul(q0, w2[0]));
+        put4(v + 4, f128_mul(f128_add(q0, q1), f128_add(w2[0], w2[1])));
+    }
+    block_xor_atomic<8>(v, acc);
+}
+
+// out[d] = sum_c er_c[row(d)] ec_c[col(d)] for d < area, zero past it (the combined jagged
+// weight of the multipoint two-product sumcheck, jagged.rs:2729-2783).
+__global__ void __launch_bounds__(kThreads) k_jag_scalar(f128* out, uint64_t N, uint64_t area,
+                                                         const uint64_t* ps, uint32_t n_cols,
+                                                         JagClaims jc) {
+    for (uint64_t d = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; d < N;
+         d += (uint64_t)gridDim.x * blockDim.x) {
+        f128 s = f128_zero();
+        if (d < area) {
+            const uint32_t c = col_of(ps, n_cols, d);
+            const uint64_t row = d - ps[c];
+            for (uint32_t k = 0; k < jc.n; k++) s = f128_add(s, f128_mul(jc.er[k][row], jc.ec[k][c]));
+        }
+        out[d] = s;
+    }
+}
+
+// v[i] = L(v[i]) for an F2-linear map L given by its nibble tables (512 words)
+__global__ void __launch_bounds__(kThreads) k_lin_apply(f128* v, uint64_t N, const f128* tab) {
+    __shared__ f128 T[512];
+    for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) T[i] = tab[i];
+    __syncthreads();
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = lin_apply_d(T, v[i]);
+}
+
+// The jagged branching program (jagged.rs:163-240, 690-720): one block per (claim, j) computes
+// sum_runs w_run * ghat(z_row, x_j, t_c, t_n), x_j = rho^(2^-j) (multipoint_values,
+// jagged.rs:2547-2585). eq4[l] = eq((z_row[l], x[l]), .) per layer l <= m; sparse[cd][s][a] =
+// (eq4 index, next state).
+struct BpArgs {
+    const uint64_t* bc;   // run boundaries t_c
+    const uint64_t* bn;   // run boundaries t_n
+    uint32_t n_runs;
+    const f128* w;        // per claim: n_runs weights
+    const f128* zrow;     // per claim: n row coords
+    const f128* xs;       // 128 points x_j, m coords each
+    uint32_t n, m;
+    int sparse[4][4][2][2];
+};
+__global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
+    const uint32_t claim = blockIdx.x / 128, j = blockIdx.x % 128;
+    __shared__ f128 eq4[64][4];
+    for (uint32_t l = threadIdx.x; l <= a.m; l += blockDim.x) {
+        const f128 r = l < a.n ? a.zrow[claim * a.n + l] : f128_zero();
+        const f128 x = l < a.m ? a.xs[(size_t)j * a.m + l] : f128_zero();
+        const f128 r1 = f128_add(f128_one(), r), x1 = f128_add(f128_one(), x);
+        eq4[l][0] = f128_mul(r1, x1);
+        eq4[l][1] = f128_mul(r, x1);
+        eq4[l][2] = f128_mul(r1, x);
+        eq4[l][3] = f128_mul(r, x);
+    }
+    __syncthreads();
+    uint32_t v[4] = {0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < a.n_runs; b += blockDim.x) {
+        f128 S[4] = {f128_zero(), f128_zero(), f128_one(), f128_zero()};  // e_SUCCESS = state 2
+        const uint64_t tc = a.bc[b], tn = a.bn[b];
+        for (int l = (int)a.m; l >= 0; l--) {
+            const int cd = (int)(((tc >> l) & 1) + 2 * ((tn >> l) & 1));
+            f128 T2[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                f128 acc = f128_zero();
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+                    acc = f128_add(acc, f128_mul(eq4[l][a.sparse[cd][s][q][0]], S[a.sparse[cd][s][q][1]]));
+                T2[s] = acc;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) S[s] = T2[s];
+        }
+        put4(v, f128_mul(a.w[(size_t)claim * a.n_runs + b], S[0]));  // read out e_INITIAL
+    }
+    block_xor_atomic<4>(v, reinterpret_cast<uint32_t*>(out + blockIdx.x));
+}
+
+__global__ void k_scale(f128* v, uint64_t N, f128 s) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = f128_mul(v[i], s);
+}
+// y += beta x
+__global__ void k_axpy(f128* y, const f128* x, uint64_t N, f128 beta) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        y[i] = f128_add(y[i], f128_mul(beta, x[i]));
+}
+__global__ void __launch_bounds__(kThreads) k_dot(const f128* a, const f128* b, uint64_t N,
+                                                  uint32_t* acc) {
+    uint32_t v[4] = {0, 0, 0, 0};
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        put4(v, f128_mul(a[i], b[i]));
+    block_xor_atomic<4>(v, acc);
+}
+// the blocked pair of output o: lo = 2 (o / d) d + o % d, hi = lo + d (extension.rs:145-224)
+__device__ __forceinline__ uint64_t pair_lo(uint64_t o, uint64_t d) {
+    return 2 * (o / d) * d + o % d;
+}
+// F128 round message (u0, u2) over blocked pairs
+__global__ void __launch_bounds__(kThreads) k_msg_blocked(const f128* f, const f128* b, uint64_t n,
+                                                          uint64_t d, uint32_t* acc) {
+    uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = pair_lo(o, d), hi = lo + d;
+        put4(v, f128_mul(f[lo], b[lo]));
+        put4(v + 4, f128_mul(f128_add(f[lo], f[hi]), f128_add(b[lo], b[hi])));
+    }
+    block_xor_atomic<8>(v, acc);
+}
+// first F256 fold of base tables: out[o] = v[lo] + r (v[hi] + v[lo])
+__global__ void k_fold_first(const f128* f, const f128* b, uint64_t n, uint64_t d, e256d r,
+                             e256d* nf, e256d* nb) {
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = pair_lo(o, d), hi = lo + d;
+        const f128 df = f128_add(f[hi], f[lo]), db = f128_add(b[hi], b[lo]);
+        nf[o] = e256d{f128_add(f[lo], f128_mul(r.c0, df)), f128_mul(r.c1, df)};
+        nb[o] = e256d{f128_add(b[lo], f128_mul(r.c0, db)), f128_mul(r.c1, db)};
+    }
+}
+__global__ void k_fold_e(const e256d* f, const e256d* b, uint64_t n, uint64_t d, e256d r,
+                         e256d* nf, e256d* nb) {
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = pair_lo(o, d), hi = lo + d;
+        nf[o] = eadd(f[lo], emul(r, eadd(f[hi], f[lo])));
+        nb[o] = eadd(b[lo], emul(r, eadd(b[hi], b[lo])));
+    }
+}
+__global__ void __launch_bounds__(kThreads) k_msg_e(const e256d* f, const e256d* b, uint64_t n,
+                                                    uint64_t d, uint32_t* acc) {
+    uint32_t v[16];
+    for (int i = 0; i < 16; i++) v[i] = 0;
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = pair_lo(o, d), hi = lo + d;
+        const e256d u0 = emul(f[lo], b[lo]);
+        const e256d u2 = emul(eadd(f[lo], f[hi]), eadd(b[lo], b[hi]));
+        put4(v, u0.c0);
+        put4(v + 4, u0.c1);
+        put4(v + 8, u2.c0);
+        put4(v + 12, u2.c1);
+    }
+    block_xor_atomic<16>(v, acc);
+}
+// the code switch (extension.rs:17-45, 1525-1533): f -> (c0, c1) pairs, b -> (b, u b) pairs;
+// u (b0 + b1 u) = x^-1 b1 + (b0 + b1) u
+__global__ void k_switch(const e256d* f, const e256d* b, uint64_t n, e256d* nf, e256d* nb) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const e256d x = f[i], y = b[i];
+        nf[2 * i] = e256d{x.c0, f128_zero()};
+        nf[2 * i + 1] = e256d{x.c1, f128_zero()};
+        nb[2 * i] = y;
+        nb[2 * i + 1] = e256d{mul_x_inv_d(y.c1), f128_add(y.c0, y.c1)};
+    }
+}
+__global__ void k_base_table(const e256d* f, uint64_t n, f128* out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = f[i].c0;
+}
+// introduce an F128 basis e (OOD eq table): y = sum f.c0 e; message over LSB pairs of (f, e)
+__global__ void __launch_bounds__(kThreads) k_intro_base(const e256d* f, const f128* e, uint64_t n,
+                                                         uint32_t* acc /* 4 y + 16 msg */) {
+    uint32_t v[20];
+    for (int i = 0; i < 20; i++) v[i] = 0;
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const e256d f0 = f[2 * o], f1 = f[2 * o + 1];
+        const f128 e0 = e[2 * o], e1 = e[2 * o + 1];
+        put4(v, f128_add(f128_mul(f0.c0, e0), f128_mul(f1.c0, e1)));
+        const e256d u0 = emulb(f0, e0), u2 = emulb(eadd(f0, f1), f128_add(e0, e1));
+        put4(v + 4, u0.c0);
+        put4(v + 8, u0.c1);
+        put4(v + 12, u2.c0);
+        put4(v + 16, u2.c1);
+    }
+    block_xor_atomic<20>(v, acc);
+}
+__global__ void k_glue_base(e256d* b, const f128* e, uint64_t n, f128 beta) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        b[i].c0 = f128_add(b[i].c0, f128_mul(e[i], beta));
+}
+// introduce a presplit basis B (2^k words) on the split table f (2^(k+1)): the basis pairs
+// are (B_x, u B_x) = ((B_x, 0), (0, B_x)) (extension.rs:1570-1580)
+__global__ void __launch_bounds__(kThreads) k_intro_presplit(const e256d* f, const f128* B,
+                                                             uint64_t nb, uint32_t* acc) {
+    uint32_t v[16];
+    for (int i = 0; i < 16; i++) v[i] = 0;
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nb;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const e256d f0 = f[2 * x], f1 = f[2 * x + 1];
+        const f128 bx = B[x];
+        const e256d u0 = emulb(f0, bx);
+        const e256d u2 = emul(eadd(f0, f1), e256d{bx, bx});
+        put4(v, u0.c0);
+        put4(v + 4, u0.c1);
+        put4(v + 8, u2.c0);
+        put4(v + 12, u2.c1);
+    }
+    block_xor_atomic<16>(v, acc);
+}
+__global__ void k_glue_presplit(e256d* b, const f128* B, uint64_t nb, f128 beta) {
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nb;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 t = f128_mul(B[x], beta);
+        b[2 * x].c0 = f128_add(b[2 * x].c0, t);
+        b[2 * x + 1].c1 = f128_add(b[2 * x + 1].c1, t);
+    }
+}
+// rows of a row-major matrix at the given positions, packed
+__global__ void k_gather_rows(const f128* mat, uint64_t row_len, const uint64_t* idx, uint32_t nq,
+                              f128* out) {
+    const uint64_t total = (uint64_t)nq * row_len;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = mat[idx[i / row_len] * row_len + i % row_len];
+}
+// 32-byte tree nodes at the given node indices
+__global__ void k_gather_nodes(const uint4* tree, const uint64_t* idx, uint32_t n, uint4* out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x)
+        out[i] = tree[2 * idx[i / 2] + (i & 1)];
+}
+// induced basis (ligerito.rs:2676-2770 with empty rows): out[jl + 2^h jh] =
+// sum_i lo[i][jl] hi[i][jh], lo/hi the per-query tensors over the low h and high bits
+__global__ void k_induced(const f128* lo, const f128* hi, uint32_t nq, uint32_t h, uint32_t dims,
+                          f128* out) {
+    const uint64_t n = 1ull << dims, nl = 1ull << h, nh = n >> h;
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t jl = j & (nl - 1), jh = j >> h;
+        f128 s = f128_zero();
+        for (uint32_t i = 0; i < nq; i++) s = f128_add(s, f128_mul(lo[i * nl + jl], hi[i * nh + jh]));
+        out[j] = s;
+    }
+}
+// s_hat_v[b] = sum_k eq[k] z_vec[128 k + b] (pcs/ring_switch.rs:1278-1330)
+__global__ void k_shv_zvec(const f128* zv, const f128* eq, uint64_t n_tail, f128* out) {
+    const uint32_t b = threadIdx.x;
+    f128 s = f128_zero();
+    for (uint64_t k = 0; k < n_tail; k++) s = f128_add(s, f128_mul(eq[k], zv[128 * k + b]));
+    out[b] = s;
+}
+
+}  // namespace
+}  // namespace flock
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace {
+using flock::e256d;
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct flock_open_ws {
+    std::vector<DBuf> bufs;
+    uint32_t* acc = nullptr;        // device accumulators (256 words)
+    uint32_t* hacc = nullptr;       // pinned mirror
+    void* get(size_t slot, size_t bytes) {
+        if (bufs.size() <= slot) bufs.resize(slot + 1);
+        DBuf& b = bufs[slot];
+        if (b.bytes >= bytes) return b.p;
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (hipMalloc(&b.p, bytes ? bytes : 16) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        b.bytes = bytes;
+        return b.p;
+    }
+};
+
+namespace {
+
+using flock::kThreads;
+
+struct Ctx {
+    flock_hip_ctx* c;
+    flock_open_ws* ws;
+    hipStream_t st;
+    int err = 0;
+    std::string what;
+
+    uint32_t grid(uint64_t n) const {
+        uint64_t g = (n + kThreads - 1) / kThreads;
+        if (g > 2048) g = 2048;
+        return (uint32_t)(g ? g : 1);
+    }
+    bool fail(int code, const std::string& m) {
+        if (!err) {
+            err = code;
+            what = m;
+        }
+        return false;
+    }
+    bool hip(hipError_t e, const char* where) {
+        if (e != hipSuccess) return fail(FLOCK_HIP_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+        return true;
+    }
+    template <class T>
+    T* buf(size_t slot, size_t count) {
+        void* p = ws->get(slot, count * sizeof(T));
+        if (!p) fail(FLOCK_HIP_E_NOMEM, "opening workspace");
+        return static_cast<T*>(p);
+    }
+    bool zero_acc(uint32_t words) {
+        hipLaunchKernelGGL(flock::k_zero32, dim3(1), dim3(256), 0, st, ws->acc, words);
+        return hip(hipGetLastError(), "zero");
+    }
+    // read back `words` accumulator words after the stream drains
+    bool read_acc(uint32_t words) {
+        if (!hip(hipMemcpyAsync(ws->hacc, ws->acc, 4 * words, hipMemcpyDeviceToHost, st), "acc"))
+            return false;
+        return hip(hipStreamSynchronize(st), "sync");
+    }
+    H acc_f(int k) const {
+        const uint32_t* v = ws->hacc + 4 * k;
+        return H{(uint64_t)v[1] << 32 | v[0], (uint64_t)v[3] << 32 | v[2]};
+    }
+    bool h2d(void* d, const void* h, size_t n) {
+        return hip(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, st), "h2d");
+    }
+    bool d2h(void* h, const void* d, size_t n) {
+        if (!hip(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, st), "d2h")) return false;
+        return hip(hipStreamSynchronize(st), "sync");
+    }
+    bool chk(const char* where) { return hip(hipGetLastError(), where); }
+    // device eq table of host coordinates
+    bool eq_dev(const H* r, size_t n, flock_f128* out) {
+        if (int rc = flock_hip_build_eq(c, r, n, out)) return fail(rc, flock_hip_last_error());
+        return true;
+    }
+};
+
+inline uint4 U4(H v) {
+    return make_uint4((uint32_t)v.lo, (uint32_t)(v.lo >> 32), (uint32_t)v.hi, (uint32_t)(v.hi >> 32));
+}
+inline e256d E4(E v) { return e256d{U4(v.c0), U4(v.c1)}; }
+
+// 512-word nibble tables of an F2-linear map with basis images img[128]
+std::vector<H> lin_tables(const H* img) {
+    std::vector<H> t(512);
+    for (int k = 0; k < 32; k++) {
+        t[k * 16] = hz();
+        for (int n = 1; n < 16; n++) {
+            const int low = __builtin_ctz((unsigned)n);
+            t[k * 16 + n] = ha(t[k * 16 + (n & (n - 1))], img[4 * k + low]);
+        }
+    }
+    return t;
+}
+
+// jagged layout (pcs/jagged.rs:87-140, 569-584)
+struct Jag {
+    int n = 0, k = 0, m = 0;
+    std::vector<uint64_t> ps;        // 2^k + 1
+    std::vector<uint64_t> bc, bn;    // deduped boundary runs
+    std::vector<uint32_t> brun;
+    uint64_t area() const { return ps.back(); }
+};
+Jag jag_make(const uint64_t* heights, size_t nh, int n, int m) {
+    Jag J;
+    J.n = n;
+    J.m = m;
+    J.k = __builtin_ctzll(nh);
+    J.ps.assign(nh + 1, 0);
+    for (size_t y = 0; y < nh; y++) J.ps[y + 1] = J.ps[y] + heights[y];
+    for (size_t y = 0; y < nh; y++) {
+        const uint64_t c = J.ps[y], d = J.ps[y + 1];
+        if (!J.bc.empty() && J.bc.back() == c && J.bn.back() == d)
+            J.brun.back()++;
+        else {
+            J.bc.push_back(c);
+            J.bn.push_back(d);
+            J.brun.push_back(1);
+        }
+    }
+    return J;
+}
+std::vector<H> run_weights(const Jag& J, const H* z_col) {
+    const std::vector<H> ec = heq(z_col, (size_t)J.k);
+    std::vector<H> w(J.bc.size());
+    size_t y = 0;
+    for (size_t b = 0; b < J.bc.size(); b++) {
+        H acc = hz();
+        for (uint32_t r = 0; r < J.brun[b]; r++) acc = ha(acc, ec[y++]);
+        w[b] = acc;
+    }
+    return w;
+}
+
+// branching-program transitions (jagged.rs:163-190, 700-716)
+int bp_transition(int row, int index, int curr, int next, int state) {
+    const int carry = state & 1, cmp = (state >> 1) & 1, sum = row + carry + curr;
+    if (index != (sum & 1)) return -1;
+    const int ncmp = (index == next) ? cmp : next;
+    return (sum >> 1) + (ncmp << 1);
+}
+struct Sparse {
+    int t[4][4][2][2];
+    Sparse() {
+        for (int cd = 0; cd < 4; cd++)
+            for (int s = 0; s < 4; s++)
+                for (int a = 0; a < 2; a++) {
+                    const int c = cd & 1, d = cd >> 1, b = (a + (s & 1) + c) & 1;
+                    t[cd][s][a][0] = a + 2 * b;
+                    t[cd][s][a][1] = bp_transition(a, b, c, d, s);
+                }
+    }
+};
+const Sparse& sparse_tr() {
+    static const Sparse s;
+    return s;
+}
+inline int col_cd(uint64_t tc, uint64_t tn, int l) { return (int)(((tc >> l) & 1) + 2 * ((tn >> l) & 1)); }
+void eq4h(H a, H b, H out[4]) {
+    const H a1 = ha(h1(), a), b1 = ha(h1(), b);
+    out[0] = hm(a1, b1);
+    out[1] = hm(a, b1);
+    out[2] = hm(a1, b);
+    out[3] = hm(a, b);
+}
+void bp_step_h(const H e[4], int cd, const H src[4], H dst[4]) {
+    const Sparse& S = sparse_tr();
+    for (int s = 0; s < 4; s++) {
+        H acc = hz();
+        for (int a = 0; a < 2; a++) acc = ha(acc, hm(e[S.t[cd][s][a][0]], src[S.t[cd][s][a][1]]));
+        dst[s] = acc;
+    }
+}
+
+// Frobenius assist (pcs/jagged.rs:1712-2025) on the host: statements (z_row, run weights),
+// sumcheck over the 2(m+1) boundary bits c_0, d_0, c_1, d_1, ...
+void frobenius_assist(flock_fs* fs, const Jag& J, const std::vector<std::vector<H>>& zr,
+                      const std::vector<std::vector<H>>& ww, const std::vector<H>& rho, uint32_t bits,
+                      Bin& out) {
+    const int m = J.m;
+    const size_t nb = J.bc.size(), ns = zr.size(), sl = (size_t)m + 2;
+    const Sparse& SP = sparse_tr();
+    std::vector<H> E((size_t)ns * (m + 1) * 4);
+    for (size_t s = 0; s < ns; s++)
+        for (int l = 0; l <= m; l++)
+            eq4h((size_t)l < (size_t)J.n ? zr[s][l] : hz(), l < m ? rho[l] : hz(),
+                 &E[(s * (m + 1) + l) * 4]);
+    std::vector<H> S(ns * nb * sl * 4);
+    for (size_t s = 0; s < ns; s++)
+        for (size_t b = 0; b < nb; b++) {
+            H* Sb = &S[(s * nb + b) * sl * 4];
+            for (int i = 0; i < 4; i++) Sb[(m + 1) * 4 + i] = hz();
+            Sb[(m + 1) * 4 + 2] = h1();
+            for (int l = m; l >= 0; l--)
+                bp_step_h(&E[(s * (m + 1) + l) * 4], col_cd(J.bc[b], J.bn[b], l), Sb + (l + 1) * 4,
+                          Sb + l * 4);
+        }
+    H v = hz();
+    for (size_t s = 0; s < ns; s++)
+        for (size_t b = 0; b < nb; b++) v = ha(v, hm(ww[s][b], S[(s * nb + b) * sl * 4 + 0]));
+    label(fs, "flock-frobenius-assist-v0");
+    observe(fs, v);
+    out.f(v);
+    std::vector<H> p(ns * nb), pre(ns * 4, hz()), rounds;
+    std::vector<uint64_t> nonces;
+    for (size_t s = 0; s < ns; s++) {
+        for (size_t b = 0; b < nb; b++) p[s * nb + b] = ww[s][b];
+        pre[s * 4 + 0] = h1();
+    }
+    H ch4[4];
+    std::vector<H> U(ns * 16), B(ns * 16), UD(ns * 8);
+    for (int l = 0; l <= m; l++) {
+        for (size_t s = 0; s < ns; s++) {
+            H* ps = &p[s * nb];
+            if (l > 0)
+                for (size_t b = 0; b < nb; b++) ps[b] = hm(ps[b], ch4[col_cd(J.bc[b], J.bn[b], l - 1)]);
+            H* Bs = &B[s * 16];
+            for (int i = 0; i < 16; i++) Bs[i] = hz();
+            for (size_t b = 0; b < nb; b++) {
+                const int cd = col_cd(J.bc[b], J.bn[b], l);
+                const H* sv = &S[((s * nb + b) * sl + l + 1) * 4];
+                for (int i = 0; i < 4; i++) Bs[cd * 4 + i] = ha(Bs[cd * 4 + i], hm(ps[b], sv[i]));
+            }
+            const H* e = &E[(s * (m + 1) + l) * 4];
+            H* Us = &U[s * 16];
+            for (int i = 0; i < 16; i++) Us[i] = hz();
+            for (int cd = 0; cd < 4; cd++)
+                for (int st = 0; st < 4; st++)
+                    for (int a = 0; a < 2; a++) {
+                        const int ix = SP.t[cd][st][a][0], o = SP.t[cd][st][a][1];
+                        Us[cd * 4 + o] = ha(Us[cd * 4 + o], hm(pre[s * 4 + st], e[ix]));
+                    }
+        }
+        H g1 = hz(), gi = hz();
+        for (size_t s = 0; s < ns; s++) {
+            const H* Us = &U[s * 16];
+            const H* Bs = &B[s * 16];
+            for (int i = 0; i < 4; i++) {
+                g1 = ha(g1, ha(hm(Us[4 + i], Bs[4 + i]), hm(Us[12 + i], Bs[12 + i])));
+                gi = ha(gi, ha(hm(ha(Us[i], Us[4 + i]), ha(Bs[i], Bs[4 + i])),
+                               hm(ha(Us[8 + i], Us[12 + i]), ha(Bs[8 + i], Bs[12 + i]))));
+            }
+        }
+        observe(fs, g1);
+        observe(fs, gi);
+        uint64_t nonce = 0;
+        const H rc = bits ? grind_sample(fs, bits, &nonce) : sample(fs);
+        if (bits) nonces.push_back(nonce);
+        rounds.push_back(g1);
+        rounds.push_back(gi);
+        const H rc1 = ha(h1(), rc);
+        g1 = hz();
+        gi = hz();
+        for (size_t s = 0; s < ns; s++) {
+            const H* Us = &U[s * 16];
+            const H* Bs = &B[s * 16];
+            for (int i = 0; i < 4; i++) {
+                const H ud0 = ha(hm(rc1, Us[i]), hm(rc, Us[4 + i]));
+                const H ud1 = ha(hm(rc1, Us[8 + i]), hm(rc, Us[12 + i]));
+                const H d0 = ha(hm(rc1, Bs[i]), hm(rc, Bs[4 + i]));
+                const H d1 = ha(hm(rc1, Bs[8 + i]), hm(rc, Bs[12 + i]));
+                g1 = ha(g1, hm(ud1, d1));
+                gi = ha(gi, hm(ha(ud0, ud1), ha(d0, d1)));
+                UD[s * 8 + i] = ud0;
+                UD[s * 8 + 4 + i] = ud1;
+            }
+        }
+        observe(fs, g1);
+        observe(fs, gi);
+        const H rd = bits ? grind_sample(fs, bits, &nonce) : sample(fs);
+        if (bits) nonces.push_back(nonce);
+        rounds.push_back(g1);
+        rounds.push_back(gi);
+        const H rd1 = ha(h1(), rd);
+        for (size_t s = 0; s < ns; s++)
+            for (int i = 0; i < 4; i++)
+                pre[s * 4 + i] = ha(hm(rd1, UD[s * 8 + i]), hm(rd, UD[s * 8 + 4 + i]));
+        ch4[0] = hm(rc1, rd1);
+        ch4[1] = hm(rc, rd1);
+        ch4[2] = hm(rc1, rd);
+        ch4[3] = hm(rc, rd);
+    }
+    out.u64(rounds.size() / 2);
+    for (const H& x : rounds) out.f(x);
+    out.vec_u64(nonces.data(), nonces.size());
+}
+
+// Product-sumcheck round on two device tables through the public kernels, re-expressed in the
+// jagged convention: the kernels give (u0, u2) = (G(0), G(inf)); G(1) = claim + G(0).
+struct PairSumcheck {
+    Ctx* X;
+    flock_f128* a;
+    flock_f128* b;
+    flock_f128* ha_;  // half buffers (ping-pong)
+    flock_f128* hb_;
+    flock_f128* qa_;
+    flock_f128* qb_;
+};
+
+// Runs n rounds of the (G(1), G(inf)) sumcheck of sum_x a(x) b(x) = claim over LSB pairs,
+// each round: observe the pair, grind `bits`, sample. The final fold is skipped when
+// `final_fold` is false; on return *fa/*fb hold the final folded values if folded.
+bool pair_sumcheck(Ctx& X, flock_f128* a, flock_f128* b, flock_f128* s1a, flock_f128* s1b,
+                   flock_f128* s2a, flock_f128* s2b, uint64_t N, uint32_t n_rounds, H claim,
+                   flock_fs* fs, uint32_t bits, bool final_fold, std::vector<H>& msgs,
+                   std::vector<uint64_t>& nonces, std::vector<H>& chal, H* fa) {
+    flock_f128 m2[2];
+    if (int rc = flock_hip_sumcheck_round_msg(X.c, a, b, N, m2)) return X.fail(rc, flock_hip_last_error());
+    const flock_f128* ca = a;
+    const flock_f128* cb = b;
+    uint64_t cur = N;
+    for (uint32_t r = 0; r < n_rounds; r++) {
+        const H g1 = ha(claim, m2[0]), gi = m2[1];
+        observe(fs, g1);
+        observe(fs, gi);
+        uint64_t nonce = 0;
+        const H ch = bits ? grind_sample(fs, bits, &nonce) : sample(fs);
+        if (bits) nonces.push_back(nonce);
+        msgs.push_back(g1);
+        msgs.push_back(gi);
+        chal.push_back(ch);
+        claim = round_claim(claim, g1, gi, ch);
+        if (r + 1 == n_rounds && !final_fold) break;
+        flock_f128* na = (r & 1) ? s2a : s1a;
+        flock_f128* nb = (r & 1) ? s2b : s1b;
+        if (int rc = flock_hip_sumcheck_fold_msg(X.c, ca, cb, cur, ch, na, nb, m2))
+            return X.fail(rc, flock_hip_last_error());
+        ca = na;
+        cb = nb;
+        cur /= 2;
+    }
+    if (final_fold && fa) {
+        flock_f128 v;
+        if (!X.d2h(&v, ca, 16)) return false;
+        *fa = v;
+    }
+    return true;
+}
+
+// ---- the eq-basis Ligerito opening (pcs.rs:214-560; extension.rs:1636-2290) ----
+struct LevelW {
+    flock_f128* cw = nullptr;  // block_len x lanes, row-major
+    uint8_t* tree = nullptr;
+    uint64_t block_len = 0, lanes = 0;
+};
+
+bool ligero_commit(Ctx& X, const flock_f128* d_poly, int log_cols, int log_lanes, int log_rate,
+                   uint32_t kind, size_t slot, LevelW* w, int cap, std::vector<uint8_t>* cap_out) {
+    flock_pcs_params p = {(uint32_t)(log_cols + log_lanes + 7), (uint32_t)log_rate,
+                          (uint32_t)log_lanes, 0, kind};
+    w->lanes = 1ull << log_lanes;
+    w->block_len = 1ull << (log_cols + log_rate);
+    w->cw = X.buf<flock_f128>(slot, w->block_len * w->lanes);
+    w->tree = X.buf<uint8_t>(slot + 1, 32 * (2 * w->block_len - 1));
+    if (X.err) return false;
+    cap_out->assign(32ull << cap, 0);
+    if (int rc = flock_hip_commit(X.c, d_poly, w->lanes << log_cols, &p, w->cw, w->tree, (uint32_t)cap,
+                                  cap_out->data()))
+        return X.fail(rc, flock_hip_last_error());
+    return true;
+}
+
+// rows at the queries and their capped paths (merkle.rs:663-682), appended to out as
+// RecursiveProof {opened_rows, merkle_proof}; the rows also come back in rows_h
+bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* tree,
+                uint64_t block_len, const std::vector<uint64_t>& qs, int cap, Bin& out,
+                std::vector<H>* rows_h, bool write_rows_first = true) {
+    const uint32_t nq = (uint32_t)qs.size();
+    uint64_t* d_idx = X.buf<uint64_t>(40, nq);
+    flock_f128* d_rows = X.buf<flock_f128>(41, (size_t)nq * row_len);
+    const int d = __builtin_ctzll(block_len);
+    const int depth = d - cap;
+    std::vector<uint64_t> nodes;
+    nodes.reserve((size_t)nq * depth);
+    for (uint64_t q : qs) {
+        uint64_t start = 0, len = block_len, idx = q;
+        while (len > (1ull << cap)) {
+            nodes.push_back(start + (idx ^ 1));
+            start += len;
+            len >>= 1;
+            idx >>= 1;
+        }
+    }
+    uint64_t* d_nidx = X.buf<uint64_t>(42, nodes.size() + 1);
+    uint8_t* d_nodes = X.buf<uint8_t>(43, 32 * (nodes.size() + 1));
+    if (X.err) return false;
+    if (!X.h2d(d_idx, qs.data(), 8 * nq)) return false;
+    hipLaunchKernelGGL(flock::k_gather_rows, dim3(X.grid((uint64_t)nq * row_len)), dim3(kThreads), 0,
+                       X.st, reinterpret_cast<const uint4*>(mat), row_len, d_idx, nq,
+                       reinterpret_cast<uint4*>(d_rows));
+    if (!X.chk("gather rows")) return false;
+    if (!nodes.empty()) {
+        if (!X.h2d(d_nidx, nodes.data(), 8 * nodes.size())) return false;
+        hipLaunchKernelGGL(flock::k_gather_nodes, dim3(X.grid(2 * nodes.size())), dim3(kThreads), 0,
+                           X.st, reinterpret_cast<const uint4*>(tree), d_nidx, (uint32_t)nodes.size(),
+                           reinterpret_cast<uint4*>(d_nodes));
+        if (!X.chk("gather nodes")) return false;
+    }
+    rows_h->assign((size_t)nq * row_len, hz());
+    if (!X.d2h(rows_h->data(), d_rows, 16 * (size_t)nq * row_len)) return false;
+    std::vector<uint8_t> nb(32 * nodes.size());
+    if (!nodes.empty() && !X.d2h(nb.data(), d_nodes, nb.size())) return false;
+    (void)write_rows_first;
+    out.u64(nq);
+    for (uint32_t i = 0; i < nq; i++) out.vec_f(rows_h->data() + (size_t)i * row_len, row_len);
+    out.vec_hash(nb.data(), nodes.size());
+    return true;
+}
+
+// induced basis over 2^dims coefficients (ligerito.rs:2676-2770 with empty rows)
+bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std::vector<H>& alpha,
+                   flock_f128* out) {
+    // eval_sk_at_vks (ligerito.rs:2402-2420)
+    std::vector<H> skv(dims + 1), layer(dims > 0 ? dims : 1);
+    skv[0] = h1();
+    for (int i = 1; i <= dims; i++) layer[i - 1] = hbit(i);
+    for (int i = 0, cur = dims; i < dims; i++, cur--)
+        for (int j = 0; j < cur; j++) {
+            const H s = layer[j], nx = ha(hm(s, s), hm(skv[i], s));
+            if (j == 0)
+                skv[i + 1] = nx;
+            else
+                layer[j - 1] = nx;
+        }
+    std::vector<H> inv(dims + 1);
+    for (int i = 0; i <= dims; i++) inv[i] = hzero(skv[i]) ? hz() : hinv(skv[i]);
+    const std::vector<H> aw = heq(alpha.data(), alpha.size());
+    const uint32_t nq = (uint32_t)qs.size();
+    const int h = dims / 2;
+    const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
+    std::vector<H> lo((size_t)nq * nl), hi((size_t)nq * nh), sk(dims > 0 ? dims : 1);
+    for (uint32_t i = 0; i < nq; i++) {
+        if (dims > 0) {
+            sk[0] = H{qs[i], 0};
+            for (int k = 1; k < dims; k++) sk[k] = ha(hm(sk[k - 1], sk[k - 1]), hm(skv[k - 1], sk[k - 1]));
+            for (int k = 0; k < dims; k++) sk[k] = hm(sk[k], inv[k]);
+        }
+        H* L = &lo[(size_t)i * nl];
+        L[0] = aw[i];
+        for (int k = 0; k < h; k++)
+            for (uint64_t j = 0; j < (1ull << k); j++) L[j + (1ull << k)] = hm(sk[k], L[j]);
+        H* Hh = &hi[(size_t)i * nh];
+        Hh[0] = h1();
+        for (int k = h; k < dims; k++)
+            for (uint64_t j = 0; j < (1ull << (k - h)); j++) Hh[j + (1ull << (k - h))] = hm(sk[k], Hh[j]);
+    }
+    flock_f128* d_lo = X.buf<flock_f128>(44, lo.size());
+    flock_f128* d_hi = X.buf<flock_f128>(45, hi.size());
+    if (X.err) return false;
+    if (!X.h2d(d_lo, lo.data(), 16 * lo.size()) || !X.h2d(d_hi, hi.data(), 16 * hi.size())) return false;
+    hipLaunchKernelGGL(flock::k_induced, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st,
+                       reinterpret_cast<const uint4*>(d_lo), reinterpret_cast<const uint4*>(d_hi), nq,
+                       (uint32_t)h, (uint32_t)dims, reinterpret_cast<uint4*>(out));
+    if (!X.chk("induced basis")) return false;
+    // the host copies must outlive the async uploads
+    return X.hip(hipStreamSynchronize(X.st), "sync");
+}
+
+// induce_enforced_sum (extension.rs:2893-2916)
+E enforced_sum(const std::vector<H>& rows, size_t nq, size_t row_len, const std::vector<E>& lane_ch,
+               const std::vector<H>& alpha) {
+    const std::vector<E> lw = heq256(lane_ch.data(), lane_ch.size());
+    const std::vector<H> rw = heq(alpha.data(), alpha.size());
+    E acc = ez();
+    for (size_t i = 0; i < nq; i++) {
+        E d = ez();
+        for (size_t l = 0; l < row_len; l++) d = ea(d, emb(lw[l], rows[i * row_len + l]));
+        acc = ea(acc, emb(d, rw[i]));
+    }
+    return acc;
+}
+
+struct Ladder {
+    Ctx& X;
+    flock_fs* fs;
+    e256d* f;   // current table
+    e256d* b;   // combined basis
+    e256d* f2;  // ping-pong
+    e256d* b2;
+    uint64_t n = 0;
+    Bin msgs;
+    size_t n_msgs = 0;
+
+    void push(E u0, E u2) {
+        msgs.e(u0);
+        msgs.e(u2);
+        n_msgs++;
+        observe_e(fs, u0);
+        observe_e(fs, u2);
+    }
+    void swap() {
+        std::swap(f, f2);
+        std::swap(b, b2);
+    }
+    bool msg(uint64_t d, E* u0, E* u2) {
+        if (!X.zero_acc(16)) return false;
+        const uint64_t dd = (d == 1 || n == d) ? 1 : d;
+        hipLaunchKernelGGL(flock::k_msg_e, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st, f, b, n, dd,
+                           X.ws->acc);
+        if (!X.chk("msg") || !X.read_acc(16)) return false;
+        *u0 = E{X.acc_f(0), X.acc_f(1)};
+        *u2 = E{X.acc_f(2), X.acc_f(3)};
+        return true;
+    }
+    bool fold(uint64_t d, E r) {
+        hipLaunchKernelGGL(flock::k_fold_e, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st, f, b, n, d,
+                           E4(r), f2, b2);
+        if (!X.chk("fold")) return false;
+        swap();
+        n /= 2;
+        return true;
+    }
+    bool sw(E* u0, E* u2) {  // code switch + its LSB message
+        hipLaunchKernelGGL(flock::k_switch, dim3(X.grid(n)), dim3(kThreads), 0, X.st, f, b, n, f2, b2);
+        if (!X.chk("switch")) return false;
+        swap();
+        n *= 2;
+        return msg(1, u0, u2);
+    }
+};
+
+bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_params* P,
+                const flock_f128* cw0, const uint8_t* tree0, const std::vector<H>& rho, H q_eval,
+                Bin& out) {
+    const Lig* C = lig_for(P->m);
+    if (!C || C->initial_k != P->log_batch_size)
+        return X.fail(FLOCK_HIP_E_INVALID, "no Ligerito config for this m / log_batch_size");
+    const int log_n = (int)P->m - 7, ik = C->initial_k, R = C->n_levels - 1;
+    const uint64_t N = 1ull << log_n;
+    const uint64_t t = P->num_lanes ? P->num_lanes : (1ull << ik);
+    const int n1 = log_n - ik;
+    const uint64_t D = t < (1ull << ik) ? (1ull << n1) : 1;
+    const uint64_t bl0 = 1ull << (n1 + C->lv[0].rate);
+    std::vector<Sched> sch;
+    for (int l = 0; l <= R; l++) sch.push_back(sched_of(C->lv[l].queries, C->lv[l].cols + C->lv[l].rate));
+    const uint32_t kind = P->merkle_hash;
+
+    // compute_combined_basis_and_target: one packed-direct EqPoint claim (pcs.rs:423-560)
+    label(fs, "flock-pcs-open-batch-v0");
+    label(fs, "flock-pcs-packed-direct-v0");
+    observe(fs, q_eval);
+    const H gamma = sample_vec(fs, 1)[0];
+    H target = hm(gamma, q_eval);
+    flock_f128* b0 = X.buf<flock_f128>(10, N);
+    flock_f128* ez_ = X.buf<flock_f128>(11, N);
+    if (X.err) return false;
+    if (!X.eq_dev(rho.data(), (size_t)log_n, b0)) return false;
+    hipLaunchKernelGGL(flock::k_scale, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0), N,
+                       U4(gamma));
+    if (!X.chk("scale")) return false;
+    label(fs, "flock-ligerito-basis-f256-split-v0");
+    observe(fs, target);
+    const int c0 = sch[0].cap();
+    std::vector<uint8_t> icap(32ull << c0);
+    if (!X.d2h(icap.data(), tree0 + 32 * (2 * bl0 - 2 * (1ull << c0)), icap.size())) return false;
+    flock_fs_observe_bytes(fs, icap.data(), icap.size());
+    std::vector<H> ood;
+    std::vector<uint64_t> claim_n, cons_n, grind_n;
+    for (int o = 0; o < C->lv[0].ood; o++) {  // L0 OOD samples
+        const std::vector<H> z = sample_vec(fs, (size_t)log_n);
+        if (!X.eq_dev(z.data(), (size_t)log_n, ez_) || !X.zero_acc(4)) return false;
+        hipLaunchKernelGGL(flock::k_dot, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(q),
+                           reinterpret_cast<const uint4*>(ez_), N, X.ws->acc);
+        if (!X.chk("ood dot") || !X.read_acc(4)) return false;
+        const H y = X.acc_f(0);
+        observe(fs, y);
+        ood.push_back(y);
+        uint64_t nonce;
+        const H beta = grind_sample(fs, C->lv[0].claim, &nonce);
+        claim_n.push_back(nonce);
+        target = ha(target, hm(beta, y));
+        hipLaunchKernelGGL(flock::k_axpy, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0),
+                           reinterpret_cast<const uint4*>(ez_), N, U4(beta));
+        if (!X.chk("axpy")) return false;
+    }
+    // first message over blocked pairs (F128)
+    if (!X.zero_acc(8)) return false;
+    hipLaunchKernelGGL(flock::k_msg_blocked, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
+                       reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D, X.ws->acc);
+    if (!X.chk("first msg") || !X.read_acc(8)) return false;
+    Ladder Lg{X, fs, X.buf<e256d>(12, N / 2), X.buf<e256d>(13, N / 2), X.buf<e256d>(14, N / 2),
+              X.buf<e256d>(15, N / 2)};
+    if (X.err) return false;
+    Lg.push(E{X.acc_f(0), hz()}, E{X.acc_f(1), hz()});
+    // the initial_k lane folds; the last one ends in the code switch
+    std::vector<E> lane_ch;
+    for (int j = 0; j < ik; j++) {
+        const E r = sample_e(fs);
+        lane_ch.push_back(r);
+        E u0, u2;
+        if (j == 0) {
+            hipLaunchKernelGGL(flock::k_fold_first, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
+                               reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D,
+                               E4(r), Lg.f, Lg.b);
+            if (!X.chk("fold first")) return false;
+            Lg.n = N / 2;
+        } else if (!Lg.fold(D, r)) {
+            return false;
+        }
+        if (j + 1 < ik) {
+            if (!Lg.msg(D, &u0, &u2)) return false;
+        } else if (!Lg.sw(&u0, &u2)) {
+            return false;
+        }
+        Lg.push(u0, u2);
+    }
+    // the split table's buffers must hold the doubled length: the first fold wrote N/2, the
+    // switch's output is 2 * 2^n1 <= N/2 for ik >= 2
+    int split_dim = n1 + 1;
+    flock_f128* base = X.buf<flock_f128>(16, 1ull << split_dim);
+    if (X.err) return false;
+    auto base_of = [&]() {
+        hipLaunchKernelGGL(flock::k_base_table, dim3(X.grid(Lg.n)), dim3(kThreads), 0, X.st, Lg.f, Lg.n,
+                           reinterpret_cast<uint4*>(base));
+        return X.chk("base table");
+    };
+    int cur_slot = 20;
+    LevelW prev;
+    std::vector<uint8_t> capb;
+    Bin caps;
+    size_t n_caps = 0;
+    if (!base_of() ||
+        !ligero_commit(X, base, split_dim - C->lv[1].lanes, C->lv[1].lanes, C->lv[1].rate, kind, cur_slot,
+                       &prev, sch[1].cap(), &capb))
+        return false;
+    flock_fs_observe_bytes(fs, capb.data(), capb.size());
+    caps.vec_hash(capb.data(), capb.size() / 32);
+    n_caps++;
+    flock_f128* ezl = X.buf<flock_f128>(17, 1ull << split_dim);
+    flock_f128* basis = X.buf<flock_f128>(18, 1ull << split_dim);
+    if (X.err) return false;
+    auto level_ood = [&](int level) -> bool {
+        for (int o = 0; o < C->lv[level].ood; o++) {
+            const std::vector<H> z = sample_vec(fs, (size_t)split_dim);
+            if (!X.eq_dev(z.data(), (size_t)split_dim, ezl) || !X.zero_acc(20)) return false;
+            hipLaunchKernelGGL(flock::k_intro_base, dim3(X.grid(Lg.n / 2)), dim3(kThreads), 0, X.st, Lg.f,
+                               reinterpret_cast<const uint4*>(ezl), Lg.n, X.ws->acc);
+            if (!X.chk("intro ood") || !X.read_acc(20)) return false;
+            const H y = X.acc_f(0);
+            const E u0{X.acc_f(1), X.acc_f(2)}, u2{X.acc_f(3), X.acc_f(4)};
+            Lg.msgs.e(u0);
+            Lg.msgs.e(u2);
+            Lg.n_msgs++;
+            observe(fs, y);
+            ood.push_back(y);
+            observe_e(fs, u0);
+            observe_e(fs, u2);
+            uint64_t nonce;
+            const H beta = grind_sample(fs, C->lv[level].claim, &nonce);
+            claim_n.push_back(nonce);
+            hipLaunchKernelGGL(flock::k_glue_base, dim3(X.grid(Lg.n)), dim3(kThreads), 0, X.st, Lg.b,
+                               reinterpret_cast<const uint4*>(ezl), Lg.n, U4(beta));
+            if (!X.chk("glue")) return false;
+        }
+        return true;
+    };
+    auto presplit = [&](int dims, int next_level) -> bool {
+        if (!X.zero_acc(16)) return false;
+        hipLaunchKernelGGL(flock::k_intro_presplit, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.f,
+                           reinterpret_cast<const uint4*>(basis), 1ull << dims, X.ws->acc);
+        if (!X.chk("intro presplit") || !X.read_acc(16)) return false;
+        const E u0{X.acc_f(0), X.acc_f(1)}, u2{X.acc_f(2), X.acc_f(3)};
+        Lg.push(u0, u2);
+        uint64_t nonce;
+        const H beta = grind_sample(fs, C->lv[next_level].claim, &nonce);
+        claim_n.push_back(nonce);
+        hipLaunchKernelGGL(flock::k_glue_presplit, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.b,
+                           reinterpret_cast<const uint4*>(basis), 1ull << dims, U4(beta));
+        return X.chk("glue presplit");
+    };
+    auto queries = [&](int level, uint64_t block_len, std::vector<uint64_t>* qs, std::vector<H>* alpha) {
+        const size_t nq = C->lv[level].queries;
+        grind_n.push_back(flock_fs_grind_pow(fs, C->lv[level].grind));
+        const std::vector<H> w = sample_vec(fs, nq);
+        (void)block_len;
+        *qs = queries_from(sch[level], w);
+        cons_n.push_back(flock_fs_grind_pow(fs, C->lv[level].cons));
+        *alpha = sample_vec(fs, ceil_log2(nq));
+    };
+    if (!level_ood(1)) return false;
+    Bin l0p;
+    {
+        std::vector<uint64_t> qs;
+        std::vector<H> alpha, rows;
+        queries(0, bl0, &qs, &alpha);
+        if (!open_level(X, cw0, t, tree0, bl0, qs, c0, l0p, &rows)) return false;
+        if (!induced_basis(X, n1, qs, alpha, basis)) return false;
+        (void)enforced_sum(rows, qs.size(), t, lane_ch, alpha);
+        if (!presplit(n1, 1)) return false;
+    }
+    Bin recp, finp;
+    size_t n_rec = 0;
+    for (int i = 0; i < R; i++) {
+        const int k = C->lv[i + 1].lanes;
+        std::vector<E> lch;
+        for (int j = 0; j < k; j++) {
+            const E r = sample_e(fs);
+            lch.push_back(r);
+            const bool switching = (j + 1 == k) && (i + 1 != R);
+            if (!Lg.fold(1, r)) return false;
+            E u0, u2;
+            if (switching) {
+                if (!Lg.sw(&u0, &u2)) return false;
+            } else if (!Lg.msg(1, &u0, &u2)) {
+                return false;
+            }
+            Lg.push(u0, u2);
+        }
+        const int ext_dim = split_dim - k, level = i + 1;
+        if (i + 1 == R) {
+            std::vector<e256d> fv(Lg.n);
+            if (!X.d2h(fv.data(), Lg.f, sizeof(e256d) * Lg.n)) return false;
+            std::vector<H> yr;
+            for (const e256d& v : fv) {
+                yr.push_back(H{(uint64_t)v.c0.y << 32 | v.c0.x, (uint64_t)v.c0.w << 32 | v.c0.z});
+                yr.push_back(H{(uint64_t)v.c1.y << 32 | v.c1.x, (uint64_t)v.c1.w << 32 | v.c1.z});
+            }
+            for (const H& y : yr) observe(fs, y);
+            std::vector<uint64_t> qs;
+            std::vector<H> alpha, rows;
+            queries(level, prev.block_len, &qs, &alpha);
+            uint64_t nonce;
+            (void)grind_sample(fs, C->lv[level].claim, &nonce);
+            claim_n.push_back(nonce);
+            finp.vec_f(yr.data(), yr.size());
+            if (!open_level(X, prev.cw, prev.lanes, prev.tree, prev.block_len, qs, sch[level].cap(), finp,
+                            &rows))
+                return false;
+            break;
+        }
+        split_dim = ext_dim + 1;
+        const int next_level = i + 2;
+        LevelW next;
+        cur_slot = cur_slot == 20 ? 22 : 20;
+        if (!base_of() ||
+            !ligero_commit(X, base, split_dim - C->lv[next_level].lanes, C->lv[next_level].lanes,
+                           C->lv[next_level].rate, kind, cur_slot, &next, sch[next_level].cap(), &capb))
+            return false;
+        flock_fs_observe_bytes(fs, capb.data(), capb.size());
+        caps.vec_hash(capb.data(), capb.size() / 32);
+        n_caps++;
+        if (!level_ood(next_level)) return false;
+        {
+            std::vector<uint64_t> qs;
+            std::vector<H> alpha, rows;
+            queries(level, prev.block_len, &qs, &alpha);
+            if (!open_level(X, prev.cw, prev.lanes, prev.tree, prev.block_len, qs, sch[level].cap(), recp,
+                            &rows))
+                return false;
+            n_rec++;
+            if (!induced_basis(X, ext_dim, qs, alpha, basis)) return false;
+            (void)enforced_sum(rows, qs.size(), prev.lanes, lch, alpha);
+            if (!presplit(ext_dim, next_level)) return false;
+        }
+        prev = next;
+    }
+    // BatchOpeningProofLigerito {ring_switches: [], batching_nonces: [], ligerito}
+    out.u64(0);
+    out.u64(0);
+    // LigeritoProof (ligerito.rs:2177-2218)
+    out.vec_hash(icap.data(), icap.size() / 32);
+    out.raw(l0p.b.data(), l0p.b.size());
+    out.u64(n_caps);
+    out.raw(caps.b.data(), caps.b.size());
+    out.u64(n_rec);
+    out.raw(recp.b.data(), recp.b.size());
+    out.raw(finp.b.data(), finp.b.size());
+    out.u64(0);  // sumcheck_transcript (F128)
+    out.u64(Lg.n_msgs);
+    out.raw(Lg.msgs.b.data(), Lg.msgs.b.size());
+    out.vec_u64(grind_n.data(), grind_n.size());
+    out.vec_f(ood.data(), ood.size());
+    out.u64(0);  // fold_grinding_nonces
+    out.vec_u64(claim_n.data(), claim_n.size());
+    out.vec_u64(cons_n.data(), cons_n.size());
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flock_open_ws_create(flock_open_ws** out) {
+    if (!out) return flock::set_error(FLOCK_HIP_E_INVALID, "null output");
+    flock_open_ws* w = new (std::nothrow) flock_open_ws();
+    if (!w) return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace");
+    if (hipMalloc(&w->acc, 4 * 256) != hipSuccess ||
+        hipHostMalloc((void**)&w->hacc, 4 * 256) != hipSuccess) {
+        (void)hipGetLastError();
+        if (w->acc) (void)hipFree(w->acc);
+        delete w;
+        return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace accumulators");
+    }
+    *out = w;
+    return 0;
+}
+
+void flock_open_ws_destroy(flock_open_ws* w) {
+    if (!w) return;
+    for (DBuf& b : w->bufs)
+        if (b.p) (void)hipFree(b.p);
+    if (w->acc) (void)hipFree(w->acc);
+    if (w->hacc) (void)hipHostFree(w->hacc);
+    delete w;
+}
+
+uint64_t flock_open_ws_bytes(const flock_open_ws* w) {
+    uint64_t s = 0;
+    if (w)
+        for (const DBuf& b : w->bufs) s += b.bytes;
+    return s;
+}
+
+int flock_lig_initial_k(uint32_t m) {
+    const Lig* c = lig_for(m);
+    return c ? c->initial_k : -1;
+}
+
+int flock_l0_cap_depth(uint32_t m) {
+    const Lig* c = lig_for(m);
+    if (!c) return -1;
+    return sched_of(c->lv[0].queries, c->lv[0].cols + c->lv[0].rate).cap();
+}
+
+int flock_hip_s_hat_v_from_z_vec(flock_hip_ctx* ctx, flock_open_ws* ws, const flock_f128* d_z_vec,
+                                 size_t z_vec_len, const flock_f128* h_tail, size_t tail_len,
+                                 flock_f128 h_out[128]) {
+    if (!ctx || !ws || !d_z_vec || !h_out || z_vec_len != (128ull << tail_len))
+        return flock::set_error(FLOCK_HIP_E_INVALID, "s_hat_v_from_z_vec: bad arguments");
+    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
+    flock_f128* eq = X.buf<flock_f128>(46, 1ull << tail_len);
+    flock_f128* out = X.buf<flock_f128>(47, 128);
+    if (X.err) return flock::set_error(X.err, X.what.c_str());
+    if (!X.eq_dev(h_tail, tail_len, eq)) return flock::set_error(X.err, X.what.c_str());
+    hipLaunchKernelGGL(flock::k_shv_zvec, dim3(1), dim3(128), 0, X.st, reinterpret_cast<const uint4*>(d_z_vec),
+                       reinterpret_cast<const uint4*>(eq), 1ull << tail_len, reinterpret_cast<uint4*>(out));
+    if (!X.chk("s_hat_v") || !X.d2h(h_out, out, 16 * 128)) return flock::set_error(X.err, X.what.c_str());
+    return 0;
+}
+
+int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, const flock_f128* d_q,
+                          const flock_pcs_params* P, const flock_f128* d_codeword, const uint8_t* d_tree,
+                          size_t n_claims, const flock_f128* h_x_outers, size_t x_len,
+                          const flock_f128* h_s_hat_v, const uint64_t* h_heights, size_t n_heights,
+                          uint32_t n_log, uint8_t** out_bytes, size_t* out_len, flock_f128* h_q_eval) {
+    if (!ctx || !ws || !fs || !d_q || !P || !d_codeword || !d_tree || !h_x_outers || !h_s_hat_v ||
+        !h_heights || !out_bytes || !out_len || n_claims < 1 || n_claims > flock::kMaxClaims ||
+        n_heights == 0 || (n_heights & (n_heights - 1)) || P->m < 14 || P->m > 40)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged: bad arguments");
+    *out_bytes = nullptr;
+    *out_len = 0;
+    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
+    const int dense_log = (int)P->m - 7;
+    const uint64_t N = 1ull << dense_log;
+    const Jag J = jag_make(h_heights, n_heights, (int)n_log, dense_log);
+    if (x_len != 1 + n_log + (size_t)J.k || J.area() > N)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged: point / heights mismatch");
+    const uint64_t area = J.area();
+    const uint32_t n_cols = (uint32_t)n_heights;
+    Bin out;
+    label(fs, "flock-merged-open-v1");
+    // ring switch per claim (ring_switch.rs:2723-2760), 3 PoW bits each
+    std::vector<std::vector<H>> eqr(n_claims);
+    std::vector<H> rs_claim(n_claims);
+    std::vector<uint64_t> rs_nonce(n_claims);
+    for (size_t c = 0; c < n_claims; c++) {
+        const H* sv = h_s_hat_v + 128 * c;
+        label(fs, "flock-ring-switch-v0");
+        flock_fs_observe_f128_slice(fs, sv, 128);
+        rs_nonce[c] = flock_fs_grind_pow(fs, 3);
+        const std::vector<H> rdp = sample_vec(fs, 7);
+        eqr[c] = heq(rdp.data(), 7);
+        H acc = hz();
+        for (int bt = 0; bt < 128; bt++) {
+            H u = hz();
+            for (int i = 0; i < 128; i++)
+                if (hgetbit(sv[i], bt)) {
+                    if (i < 64)
+                        u.lo |= 1ull << i;
+                    else
+                        u.hi |= 1ull << (i - 64);
+                }
+            acc = ha(acc, hm(u, eqr[c][bt]));
+        }
+        rs_claim[c] = acc;
+    }
+    const uint32_t bb = n_claims > 1 ? 6 : 0;
+    const uint64_t batch_nonce = flock_fs_grind_pow(fs, bb);
+    const std::vector<H> gam = sample_vec(fs, n_claims);
+    H target = hz();
+    std::vector<std::vector<H>> tabs(n_claims);
+    for (size_t c = 0; c < n_claims; c++) {
+        target = ha(target, hm(gam[c], rs_claim[c]));
+        H ge[128];
+        for (int bt = 0; bt < 128; bt++) ge[bt] = hm(gam[c], eqr[c][bt]);
+        tabs[c] = lin_tables(ge);
+    }
+    // device tables: prefix sums, per-claim eq_row / eq_col / Phi tables
+    uint64_t* d_ps = X.buf<uint64_t>(0, J.ps.size());
+    flock_f128* d_tab = X.buf<flock_f128>(1, 512 * n_claims);
+    flock_f128* d_er = X.buf<flock_f128>(2, (n_claims << n_log));
+    flock_f128* d_ec = X.buf<flock_f128>(3, n_claims * (size_t)n_cols);
+    flock_f128* W = X.buf<flock_f128>(4, N);
+    flock_f128* A = X.buf<flock_f128>(5, N);
+    flock_f128* s1a = X.buf<flock_f128>(6, N / 2);
+    flock_f128* s1b = X.buf<flock_f128>(7, N / 2);
+    flock_f128* s2a = X.buf<flock_f128>(8, N / 4 + 1);
+    flock_f128* s2b = X.buf<flock_f128>(9, N / 4 + 1);
+    if (X.err) return flock::set_error(X.err, X.what.c_str());
+    std::vector<H> ec_all(n_claims * (size_t)n_cols), tab_all(512 * n_claims);
+    flock::JagClaims jc{};
+    jc.n = (uint32_t)n_claims;
+    for (size_t c = 0; c < n_claims; c++) {
+        const H* x = h_x_outers + c * x_len;
+        if (!X.eq_dev(x + 1, n_log, d_er + (c << n_log))) return flock::set_error(X.err, X.what.c_str());
+        const std::vector<H> ec = heq(x + 1 + n_log, (size_t)J.k);
+        std::copy(ec.begin(), ec.end(), ec_all.begin() + c * n_cols);
+        std::copy(tabs[c].begin(), tabs[c].end(), tab_all.begin() + 512 * c);
+        jc.er[c] = reinterpret_cast<const uint4*>(d_er + (c << n_log));
+        jc.ec[c] = reinterpret_cast<const uint4*>(d_ec + c * n_cols);
+        jc.tab[c] = reinterpret_cast<const uint4*>(d_tab + 512 * c);
+    }
+    if (!X.h2d(d_ps, J.ps.data(), 8 * J.ps.size()) || !X.h2d(d_ec, ec_all.data(), 16 * ec_all.size()) ||
+        !X.h2d(d_tab, tab_all.data(), 16 * tab_all.size()) || !X.zero_acc(8))
+        return flock::set_error(X.err, X.what.c_str());
+    hipLaunchKernelGGL(flock::k_jag_weight, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
+                       reinterpret_cast<const uint4*>(d_q), reinterpret_cast<uint4*>(W), N, area, d_ps, n_cols,
+                       jc, X.ws->acc);
+    if (!X.chk("jagged weight") || !X.read_acc(8)) return flock::set_error(X.err, X.what.c_str());
+    // the merged sumcheck <q, W> = target (pcs.rs:1640-1700), merged_round_bits = 2
+    if (!X.hip(hipMemcpyAsync(A, d_q, 16 * N, hipMemcpyDeviceToDevice, X.st), "copy q"))
+        return flock::set_error(X.err, X.what.c_str());
+    std::vector<H> mrounds, rho;
+    std::vector<uint64_t> mnon;
+    H q_eval = hz();
+    {
+        // round 0's message is the fused prime; later rounds come from the fold kernels
+        flock_f128 m2[2] = {X.acc_f(0), X.acc_f(1)};
+        const flock_f128* ca = A;
+        const flock_f128* cb = W;
+        uint64_t cur = N;
+        H claim = target;
+        for (int r = 0; r < dense_log; r++) {
+            const H g1 = ha(claim, m2[0]), gi = m2[1];
+            observe(fs, g1);
+            observe(fs, gi);
+            uint64_t nonce;
+            const H ch = grind_sample(fs, 2, &nonce);
+            mnon.push_back(nonce);
+            mrounds.push_back(g1);
+            mrounds.push_back(gi);
+            rho.push_back(ch);
+            claim = round_claim(claim, g1, gi, ch);
+            flock_f128* na = (r & 1) ? s2a : s1a;
+            flock_f128* nb = (r & 1) ? s2b : s1b;
+            if (int rc = flock_hip_sumcheck_fold_msg(ctx, ca, cb, cur, ch, na, nb, m2))
+                return rc;
+            ca = na;
+            cb = nb;
+            cur /= 2;
+        }
+        if (!X.d2h(&q_eval, ca, 16)) return flock::set_error(X.err, X.what.c_str());
+    }
+    if (h_q_eval) *h_q_eval = q_eval;
+    // fork the assist's transcript (challenger.rs:167-191)
+    const H seed0 = sample(fs), seed1 = sample(fs);
+    static const uint8_t kFork[] = "flock-par-assist-v1";
+    flock_fs* ch = flock_fs_new(kFork, sizeof(kFork) - 1);
+    if (!ch) return flock::set_error(FLOCK_HIP_E_NOMEM, "transcript");
+    observe(ch, seed0);
+    observe(ch, seed1);
+    Bin mp;  // MultipointTwistedProof
+    const int m = dense_log;
+    bool ok = true;
+    {
+        // rho^(2^-j), j < 128
+        std::vector<H> rp(128 * (size_t)m);
+        for (int i = 0; i < m; i++) rp[i] = rho[i];
+        for (int j = 1; j < 128; j++)
+            for (int i = 0; i < m; i++) rp[(size_t)j * m + i] = frob_inv(rp[(size_t)(j - 1) * m + i]);
+        const size_t nr = J.bc.size();
+        std::vector<H> wall(n_claims * nr), zrow(n_claims * (size_t)n_log + 1);
+        for (size_t c = 0; c < n_claims; c++) {
+            const H* x = h_x_outers + c * x_len;
+            const std::vector<H> w = run_weights(J, x + 1 + n_log);
+            std::copy(w.begin(), w.end(), wall.begin() + c * nr);
+            std::copy(x + 1, x + 1 + n_log, zrow.begin() + c * n_log);
+        }
+        uint64_t* d_bc = X.buf<uint64_t>(30, nr);
+        uint64_t* d_bn = X.buf<uint64_t>(31, nr);
+        flock_f128* d_w = X.buf<flock_f128>(32, wall.size());
+        flock_f128* d_zr = X.buf<flock_f128>(33, zrow.size());
+        flock_f128* d_xs = X.buf<flock_f128>(34, rp.size());
+        flock_f128* d_vals = X.buf<flock_f128>(35, 128 * n_claims);
+        std::vector<H> vals(128 * n_claims);
+        if (X.err || !X.h2d(d_bc, J.bc.data(), 8 * nr) || !X.h2d(d_bn, J.bn.data(), 8 * nr) ||
+            !X.h2d(d_w, wall.data(), 16 * wall.size()) || !X.h2d(d_zr, zrow.data(), 16 * zrow.size()) ||
+            !X.h2d(d_xs, rp.data(), 16 * rp.size()) ||
+            !X.hip(hipMemsetAsync(d_vals, 0, 16 * 128 * n_claims, X.st), "memset"))
+            ok = false;
+        if (ok) {
+            flock::BpArgs ba{};
+            ba.bc = d_bc;
+            ba.bn = d_bn;
+            ba.n_runs = (uint32_t)nr;
+            ba.w = reinterpret_cast<const uint4*>(d_w);
+            ba.zrow = reinterpret_cast<const uint4*>(d_zr);
+            ba.xs = reinterpret_cast<const uint4*>(d_xs);
+            ba.n = n_log;
+            ba.m = (uint32_t)m;
+            std::memcpy(ba.sparse, sparse_tr().t, sizeof(ba.sparse));
+            hipLaunchKernelGGL(flock::k_bp_values, dim3(128 * (uint32_t)n_claims), dim3(kThreads), 0, X.st, ba,
+                               reinterpret_cast<uint4*>(d_vals));
+            ok = X.chk("bp values") && X.d2h(vals.data(), d_vals, 16 * vals.size());
+        }
+        if (ok) {
+            label(ch, "flock-multipoint-twisted-v1");
+            for (const H& v : vals) observe(ch, v);
+            uint32_t gbits = bits_for_degree(128 * n_claims - 1);
+            if (gbits < 1) gbits = 1;
+            uint64_t gnonce;
+            const H gmm = grind_sample(ch, gbits, &gnonce);
+            std::vector<H> gpow(128 * n_claims);
+            gpow[0] = h1();
+            for (size_t i = 1; i < gpow.size(); i++) gpow[i] = hm(gpow[i - 1], gmm);
+            // the claim sum_{c,j} gamma^(128c+j) A_cj = sum_d abar_d g_d
+            H claim2 = hz();
+            for (size_t i = 0; i < vals.size(); i++) claim2 = ha(claim2, hm(gpow[i], vals[i]));
+            // g = L(eq(rho, .)), L(x) = sum_{j<128} gamma^j x^(2^-j)
+            H img[128];
+            for (int b = 0; b < 128; b++) {
+                H acc = hz(), y = hbit(b);
+                for (int j = 0; j < 128; j++) {
+                    acc = ha(acc, hm(gpow[j], y));
+                    y = frob_inv(y);
+                }
+                img[b] = acc;
+            }
+            const std::vector<H> lt = lin_tables(img);
+            flock_f128* d_lt = X.buf<flock_f128>(36, 512);
+            if (X.err || !X.h2d(d_lt, lt.data(), 16 * 512) || !X.eq_dev(rho.data(), (size_t)m, A)) ok = false;
+            if (ok) {
+                hipLaunchKernelGGL(flock::k_lin_apply, dim3(X.grid(N)), dim3(kThreads), 0, X.st,
+                                   reinterpret_cast<uint4*>(A), N, reinterpret_cast<const uint4*>(d_lt));
+                ok = X.chk("lin apply");
+            }
+            // abar = sum_c gamma^(128 c) eq_row_c (x) eq_col_c on the jagged area
+            std::vector<H> ecs(n_claims * (size_t)n_cols);
+            for (size_t c = 0; c < n_claims && ok; c++)
+                for (uint32_t y = 0; y < n_cols; y++) ecs[c * n_cols + y] = hm(gpow[128 * c], ec_all[c * n_cols + y]);
+            if (ok && X.h2d(d_ec, ecs.data(), 16 * ecs.size())) {
+                hipLaunchKernelGGL(flock::k_jag_scalar, dim3(X.grid(N)), dim3(kThreads), 0, X.st,
+                                   reinterpret_cast<uint4*>(W), N, area, d_ps, n_cols, jc);
+                ok = X.chk("jagged scalar");
+            } else {
+                ok = false;
+            }
+            std::vector<H> trounds, point;
+            std::vector<uint64_t> tnon;
+            if (ok)
+                ok = pair_sumcheck(X, W, A, s1a, s1b, s2a, s2b, N, (uint32_t)m, claim2, ch, 2, false,
+                                   trounds, tnon, point, nullptr);
+            // anchor: the untwisted assist at the endpoint (jagged.rs:3240-3285)
+            if (ok) {
+                H g_at = hz();
+                for (int j = 0; j < 128; j++) g_at = ha(g_at, hm(gpow[j], heq_at(&rp[(size_t)j * m], point.data(), m)));
+                std::vector<std::vector<H>> zr, ww;
+                for (size_t c = 0; c < n_claims; c++) {
+                    const H* x = h_x_outers + c * x_len;
+                    const H co = hm(gpow[128 * c], g_at);
+                    if (hzero(co)) continue;
+                    std::vector<H> w = run_weights(J, x + 1 + n_log);
+                    for (H& v : w) v = hm(v, co);
+                    std::vector<H> z(x + 1, x + 1 + n_log);
+                    size_t s = 0;
+                    for (; s < zr.size(); s++)
+                        if (std::memcmp(zr[s].data(), z.data(), 16 * n_log) == 0) break;
+                    if (s < zr.size()) {
+                        for (size_t b = 0; b < w.size(); b++) ww[s][b] = ha(ww[s][b], w[b]);
+                    } else {
+                        zr.push_back(z);
+                        ww.push_back(w);
+                    }
+                }
+                mp.u64(n_claims);
+                for (size_t c = 0; c < n_claims; c++) mp.vec_f(&vals[128 * c], 128);
+                mp.u64(0);  // group_values
+                mp.u64(gnonce);
+                mp.u64(trounds.size() / 2);
+                for (const H& x : trounds) mp.f(x);
+                mp.vec_u64(tnon.data(), tnon.size());
+                frobenius_assist(ch, J, zr, ww, point, 2, mp);
+            }
+        }
+    }
+    Bin inner;
+    if (ok) ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);
+    // merge the child (challenger.rs:193-203)
+    const H d0 = sample(ch), d1 = sample(ch);
+    observe(fs, d0);
+    observe(fs, d1);
+    flock_fs_free(ch);
+    if (!ok) return flock::set_error(X.err ? X.err : FLOCK_HIP_E_INTERNAL, X.what.c_str());
+    // MergedOpenProof (pcs.rs:1362-1377)
+    out.u64(n_claims);
+    for (size_t c = 0; c < n_claims; c++) {
+        out.vec_f(h_s_hat_v + 128 * c, 128);
+        out.u64(rs_nonce[c]);
+    }
+    out.vec_u64(&batch_nonce, bb ? 1 : 0);
+    out.u64(mrounds.size() / 2);
+    for (const H& x : mrounds) out.f(x);
+    out.vec_u64(mnon.data(), mnon.size());
+    out.f(q_eval);
+    out.raw(mp.b.data(), mp.b.size());
+    out.raw(inner.b.data(), inner.b.size());
+    uint8_t* p = static_cast<uint8_t*>(std::malloc(out.b.size()));
+    if (!p) return flock::set_error(FLOCK_HIP_E_NOMEM, "proof bytes");
+    std::memcpy(p, out.b.data(), out.b.size());
+    *out_bytes = p;
+    *out_len = out.b.size();
+    return 0;
+}
+
+}  // extern "C"

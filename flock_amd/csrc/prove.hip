@@ -65,7 +65,7 @@ struct DevBuf {
     }
 };
 
-enum { B_IN, B_Z, B_A, B_B, B_STRIPE, B_Q, B_CW, B_TREE, B_COUNT };
+enum { B_IN, B_Z, B_A, B_B, B_STRIPE, B_Q, B_CW, B_TREE, B_ZVEC, B_COUNT };
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
@@ -78,6 +78,8 @@ struct Plan {
     flock_registry reg;
     flock_union_sizes sz;
     flock_pcs_params pcs;
+    int cap_depth;
+    bool has_config;
     size_t cw_len, n_leaves;
     uint64_t count, n_rows;
     uint32_t m_zc;
@@ -95,7 +97,20 @@ int plan_request(const flock_prove_request* r, Plan* p) {
     p->count = r->kind == FLOCK_HASH_KECCAK ? (1ull << r->n_log) : r->n_decl;
     p->n_rows = p->count;
     if (int rc = flock_union_sizes_compute(&p->reg, &p->count, 0, 6, &p->sz)) return rc;
-    p->pcs = {p->sz.dense_m, 1, 6, (uint32_t)p->sz.commit_lanes, 0};
+    // log_batch_size is the Ligerito config's initial_k (ligerito.rs:564-574: 6, but 4 at
+    // m = 28 and 5 at m = 29), and commit_lanes depends on it (union.rs:386-390)
+    int lbs = flock_lig_initial_k(p->sz.dense_m);
+    p->cap_depth = flock_l0_cap_depth(p->sz.dense_m);
+    // below the m22 configs there is no opening (the reference's prover_config_for fails,
+    // ligerito.rs:590-618); the prove core still runs there and binds the root
+    p->has_config = lbs >= 0 && p->cap_depth >= 0;
+    if (!p->has_config) {
+        lbs = 6;
+        p->cap_depth = 0;
+    }
+    if (lbs != 6)
+        if (int rc = flock_union_sizes_compute(&p->reg, &p->count, 0, (uint32_t)lbs, &p->sz)) return rc;
+    p->pcs = {p->sz.dense_m, 1, (uint32_t)lbs, (uint32_t)p->sz.commit_lanes, 0};
     uint32_t k_code = 0;
     if (int rc = flock_hip_pcs_sizes(&p->pcs, nullptr, &p->cw_len, &p->n_leaves, &k_code, nullptr))
         return rc;
@@ -103,13 +118,17 @@ int plan_request(const flock_prove_request* r, Plan* p) {
     return 0;
 }
 
-// Device bytes of one request: witness, stripe, dense stack, codeword, tree, and the
-// zerocheck scratch (about 3.9 x 2^(m-6) words), the largest driver scratch.
+// Device bytes of one request: witness, stripe, dense stack, codeword, tree, the zerocheck
+// driver's scratch (2N + 2h + 2q + h + q + N/8 = 4.375 N words for N = 2^(m-6) rows,
+// api.hip flock_hip_zerocheck_prove), the lincheck's z_vec, and the opening's workspace
+// (about 9.5 words per committed word: W, q copy, the ping-pong halves, the eq bases, and
+// the F256 ladder's four half-size tables of 32 B words).
 uint64_t plan_footprint(const Plan& p, uint32_t n_log) {
     const uint64_t words = 1ull << (n_log + p.hs.k_log - 7);  // F128 words of z (a, b alike)
+    const uint64_t zc_rows = 1ull << (p.m_zc - 6);
     return 3 * 16 * words + 16 * words /* stripe */ + 16 * p.sz.committed_words +
-           16 * (uint64_t)p.cw_len + 32 * (2 * (uint64_t)p.n_leaves) +
-           (uint64_t)(3.9 * 16.0 * (double)(1ull << (p.m_zc - 6)));
+           16 * (uint64_t)p.cw_len + 32 * (2 * (uint64_t)p.n_leaves) + 16 * zc_rows * 35 / 8 +
+           16 * (1ull << p.hs.k_log) + 16 * p.sz.committed_words * 19 / 2;
 }
 
 }  // namespace
@@ -117,6 +136,7 @@ uint64_t plan_footprint(const Plan& p, uint32_t n_log) {
 struct flock_hip_prover {
     flock_hip_ctx* ctx = nullptr;
     DevBuf buf[B_COUNT];
+    flock_open_ws* ws = nullptr;  // the opening's workspace, created on first use
 };
 
 extern "C" {
@@ -139,6 +159,7 @@ void flock_hip_prover_destroy(flock_hip_prover* p) {
     if (p->ctx) {
         flock_hip_sync(p->ctx);
         for (auto& b : p->buf) b.release(p->ctx);
+        flock_open_ws_destroy(p->ws);
         flock_hip_ctx_destroy(p->ctx);
     }
     delete p;
@@ -154,12 +175,48 @@ int flock_hip_prove_footprint(const flock_prove_request* req, uint64_t* bytes) {
     return 0;
 }
 
-int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
-                         flock_prove_result* res) {
+}  // extern "C"
+
+namespace {
+
+struct BinW {
+    std::vector<uint8_t> b;
+    void u64(uint64_t v) {
+        for (int i = 0; i < 8; i++) b.push_back((uint8_t)(v >> (8 * i)));
+    }
+    void f(flock_f128 v) {
+        u64(v.lo);
+        u64(v.hi);
+    }
+    void vec_f(const flock_f128* v, size_t n) {
+        u64(n);
+        for (size_t i = 0; i < n; i++) f(v[i]);
+    }
+    void vec_pairs(const flock_f128* v, size_t n_pairs) {
+        u64(n_pairs);
+        for (size_t i = 0; i < 2 * n_pairs; i++) f(v[i]);
+    }
+    void vec_u64(const uint64_t* v, size_t n) {
+        u64(n);
+        for (size_t i = 0; i < n; i++) u64(v[i]);
+    }
+};
+
+uint8_t* dup_bytes(const uint8_t* p, size_t n) {
+    uint8_t* o = static_cast<uint8_t*>(std::malloc(n ? n : 1));
+    if (o && n) std::memcpy(o, p, n);
+    return o;
+}
+
+// The shared body: the prove core, and when `out` is set the opening and the wire bytes.
+int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove_result* res,
+               flock_proof_bytes* out) {
     if (!pr || !pr->ctx || !res) return FLOCK_HIP_E_INVALID;
     std::memset(res, 0, sizeof(*res));
+    if (out) std::memset(out, 0, sizeof(*out));
     Plan p;
     if (int rc = plan_request(req, &p)) return res->status = rc;
+    if (out && !p.has_config) return res->status = FLOCK_HIP_E_INVALID;
     flock_hip_ctx* c = pr->ctx;
     const uint32_t nl = req->n_log, kl = p.hs.k_log;
     const uint64_t n = 1ull << nl;
@@ -173,8 +230,10 @@ int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
                                   (size_t)stripe_bytes,
                                   (size_t)(16 * p.sz.committed_words),
                                   (size_t)(16 * p.cw_len),
-                                  (size_t)(32 * (2 * p.n_leaves - 1))};
+                                  (size_t)(32 * (2 * p.n_leaves - 1)),
+                                  (size_t)(out ? 16ull << kl : 16)};
     for (int i = 0; i < B_COUNT && !rc; i++) rc = pr->buf[i].ensure(c, need[i]);
+    if (!rc && out && !pr->ws) rc = flock_open_ws_create(&pr->ws);
     if (rc) return res->status = rc;
     void* d_in = pr->buf[B_IN].p;
     auto* z = static_cast<flock_f128*>(pr->buf[B_Z].p);
@@ -184,6 +243,7 @@ int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
     auto* q = static_cast<flock_f128*>(pr->buf[B_Q].p);
     auto* cw = static_cast<flock_f128*>(pr->buf[B_CW].p);
     auto* tree = static_cast<uint8_t*>(pr->buf[B_TREE].p);
+    auto* zvec = out ? static_cast<flock_f128*>(pr->buf[B_ZVEC].p) : nullptr;
     hipStream_t st = static_cast<hipStream_t>(flock_hip_ctx_stream(c));
     if (hipMemcpyAsync(d_in, req->h_inputs, need[B_IN], hipMemcpyHostToDevice, st) !=
         hipSuccess)
@@ -202,24 +262,37 @@ int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
             rc = flock_hip_witness_keccak(c, static_cast<const uint64_t*>(d_in), req->n_decl, nl,
                                           z, a, b, stripe);
     }
-    // 2. dense stack and 3. commit (prover.rs:979-983)
+    // 2. dense stack and 3. commit (prover.rs:979-983): lane-major for an integer lane count,
+    // the interleaved commit at full width; the cap at the L0 cap depth
+    std::vector<uint8_t> cap(32ull << p.cap_depth);
     if (!rc) rc = flock_hip_union_compact(c, &p.reg, &p.count, 0, z, q);
-    if (!rc) rc = flock_hip_commit_lane_major(c, q, p.sz.committed_words, &p.pcs, cw, tree, 0,
-                                              res->root);
+    if (!rc) {
+        if (p.pcs.num_lanes)
+            rc = flock_hip_commit_lane_major(c, q, p.sz.committed_words, &p.pcs, cw, tree,
+                                             (uint32_t)p.cap_depth, cap.data());
+        else
+            rc = flock_hip_commit(c, q, p.sz.committed_words, &p.pcs, cw, tree,
+                                  (uint32_t)p.cap_depth, cap.data());
+    }
+    if (!rc && hipMemcpyAsync(res->root, tree + 32 * (2 * p.n_leaves - 2), 32, hipMemcpyDeviceToHost,
+                              st) != hipSuccess)
+        rc = FLOCK_HIP_E_HIP;
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = FLOCK_HIP_E_HIP;
     if (rc) return res->status = rc;
-    // 4. statement binding, zerocheck with s_hat_v_c capture (prover.rs:1094-1103)
+    // 4. statement binding (union.rs:619-628), zerocheck with s_hat_v_c (prover.rs:1094-1103)
     flock_fs* fs = flock_fs_new(req->domain, req->domain_len);
     if (!fs) return res->status = FLOCK_HIP_E_NOMEM;
     const uint32_t n_mlv = p.m_zc - 6;
-    std::vector<flock_f128> r1(128), ml(2 * n_mlv), claim(1 + 2 * n_mlv), shv(128);
+    std::vector<flock_f128> r1(128), ml(2 * n_mlv), claim(1 + 2 * n_mlv), shv(2 * 128);
     std::vector<uint64_t> nonces(2 + n_mlv);
     flock_f128 finals[3];
-    rc = flock_union_bind_statement(fs, req->digest, &p.count, 1, res->root, 32);
+    rc = flock_union_bind_statement(fs, req->digest, &p.count, 1, cap.data(), cap.size());
     if (!rc)
         rc = flock_hip_zerocheck_prove(c, fs, reinterpret_cast<const uint8_t*>(a),
                                        reinterpret_cast<const uint8_t*>(b),
                                        reinterpret_cast<const uint8_t*>(z), p.m_zc, 1, r1.data(),
-                                       ml.data(), finals, nonces.data(), claim.data(), shv.data());
+                                       ml.data(), finals, nonces.data(), claim.data(),
+                                       shv.data() + 128);
     // 5. x_ab and the union lincheck (prover.rs:1140-1160)
     const uint32_t L = kl - 6;
     std::vector<flock_f128> x_rest(L ? L : 1), x_outer(nl), rounds(2 * (L ? L : 1)),
@@ -232,12 +305,59 @@ int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
         if (req->kind == FLOCK_HASH_KECCAK)
             rc = flock_hip_lincheck_prove_keccak(c, fs, stripe, nl, p.n_rows, claim[0],
                                                  x_rest.data(), x_outer.data(), 1, rounds.data(),
-                                                 zp, mev, lnonces.data(), lclaim.data(), nullptr);
+                                                 zp, mev, lnonces.data(), lclaim.data(), zvec);
         else
             rc = flock_hip_lincheck_prove(c, fs, req->circuit, stripe, nl, p.hs.useful_bits,
                                           p.n_rows, claim[0], x_rest.data(), x_outer.data(), 1,
                                           rounds.data(), zp, mev, lnonces.data(), lclaim.data(),
-                                          nullptr);
+                                          zvec);
+    }
+    if (!rc && out) {
+        // 6. s_hat_v_ab from the lincheck's z_vec (prover.rs:1186-1200), the claim points
+        // (union.rs:564-600 as quirky x_outer_full, prover.rs:43-48) and the jagged heights
+        // (union.rs:293-310); then open_batch_merged (prover.rs:1412-1426)
+        const flock_f128* r_inner_rest = lclaim.data() + 2;
+        rc = flock_hip_s_hat_v_from_z_vec(c, pr->ws, zvec, 1ull << kl, r_inner_rest + 1, L - 1,
+                                          shv.data());
+        const size_t x_len = p.m_zc - 6;
+        std::vector<flock_f128> xs(2 * x_len);
+        xs[0] = r_inner_rest[0];
+        for (uint32_t i = 0; i < nl; i++) xs[1 + i] = x_outer[i];
+        for (uint32_t i = 1; i < L; i++) xs[nl + i] = r_inner_rest[i];
+        for (size_t i = 0; i < x_len; i++) xs[x_len + i] = claim[1 + n_mlv + i];
+        const uint32_t col_log = kl - 7;
+        const uint64_t used = std::min<uint64_t>((p.hs.useful_bits + 127) / 128, 1ull << col_log);
+        std::vector<uint64_t> heights(1ull << col_log, 0);
+        for (uint64_t y = 0; y < used; y++) heights[y] = p.count;
+        uint8_t* ob = nullptr;
+        size_t on = 0;
+        if (!rc)
+            rc = flock_hip_open_merged(c, pr->ws, fs, q, &p.pcs, cw, tree, 2, xs.data(), x_len,
+                                       shv.data(), heights.data(), heights.size(), nl, &ob, &on,
+                                       nullptr);
+        if (!rc) {
+            // R1csProofMergedLigerito (proof.rs:42-47): ZerocheckProof (zerocheck.rs:427-447),
+            // LincheckProof (lincheck.rs:520-549), MergedOpenProof
+            BinW w;
+            w.vec_f(r1.data(), 64);
+            w.vec_f(r1.data() + 64, 64);
+            w.vec_pairs(ml.data(), n_mlv);
+            for (int i = 0; i < 3; i++) w.f(finals[i]);
+            w.vec_u64(nonces.data(), nonces.size());
+            w.vec_pairs(rounds.data(), L);
+            w.vec_f(zp, 64);
+            w.vec_pairs(mev, 1);
+            w.vec_u64(lnonces.data(), lnonces.size());
+            w.b.insert(w.b.end(), ob, ob + on);
+            out->proof = dup_bytes(w.b.data(), w.b.size());
+            out->proof_len = w.b.size();
+            out->cap = dup_bytes(cap.data(), cap.size());
+            out->cap_len = cap.size();
+            out->claim_ab = lclaim[1];
+            out->claim_c = finals[2];
+            if (!out->proof || !out->cap) rc = FLOCK_HIP_E_NOMEM;
+        }
+        flock_hip_free(ob);
     }
     if (!rc) {
         flock_fs_digest(fs, res->transcript);
@@ -247,6 +367,21 @@ int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
     }
     flock_fs_free(fs);
     return res->status = rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
+                         flock_prove_result* res) {
+    return prove_impl(pr, req, res, nullptr);
+}
+
+int flock_hip_prove_fast(flock_hip_prover* pr, const flock_prove_request* req,
+                         flock_prove_result* res, flock_proof_bytes* out) {
+    if (!out) return FLOCK_HIP_E_INVALID;
+    return prove_impl(pr, req, res, out);
 }
 
 int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,

@@ -141,6 +141,14 @@ class ProveResult(ctypes.Structure):
                 ("reserved", ctypes.c_int)]
 
 
+class ProofBytes(ctypes.Structure):
+    """flock_proof_bytes: the wire bytes of one prove_fast."""
+
+    _fields_ = [("proof", ctypes.POINTER(ctypes.c_uint8)), ("proof_len", ctypes.c_size_t),
+                ("cap", ctypes.POINTER(ctypes.c_uint8)), ("cap_len", ctypes.c_size_t),
+                ("claim_ab", F128), ("claim_c", F128)]
+
+
 _lib = None
 
 
@@ -239,6 +247,17 @@ def load():
         "flock_hip_prove_core": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult)]),
         "flock_hip_serve": (I, [ctypes.POINTER(P), U32, ctypes.POINTER(ProveRequest), S, U64,
                                 ctypes.POINTER(ProveResult), ctypes.POINTER(ctypes.c_double)]),
+        "flock_open_ws_create": (I, [ctypes.POINTER(P)]),
+        "flock_open_ws_destroy": (None, [P]),
+        "flock_open_ws_bytes": (U64, [P]),
+        "flock_lig_initial_k": (I, [U32]),
+        "flock_l0_cap_depth": (I, [U32]),
+        "flock_hip_s_hat_v_from_z_vec": (I, [P, P, P, S, P, S, P]),
+        "flock_hip_open_merged": (I, [P, P, P, P, PP, P, P, S, P, S, P, P, S, U32,
+                                      ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                      ctypes.POINTER(S), P]),
+        "flock_hip_prove_fast": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult),
+                                     ctypes.POINTER(ProofBytes)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -829,6 +848,27 @@ def prove_core(prover: Prover, req: ProveRequest) -> ProveResult:
     res = ProveResult()
     _check(load().flock_hip_prove_core(prover.h, ctypes.byref(req), ctypes.byref(res)))
     return res
+
+
+def prove_fast(prover: Prover, req: ProveRequest):
+    """flock_hip_prove_fast: returns (result, proof bytes, cap bytes, claim_ab, claim_c)."""
+    res = ProveResult()
+    out = ProofBytes()
+    _check(load().flock_hip_prove_fast(prover.h, ctypes.byref(req), ctypes.byref(res),
+                                       ctypes.byref(out)))
+    proof = ctypes.string_at(out.proof, out.proof_len)
+    cap = ctypes.string_at(out.cap, out.cap_len)
+    load().flock_hip_free(ctypes.cast(out.proof, ctypes.c_void_p))
+    load().flock_hip_free(ctypes.cast(out.cap, ctypes.c_void_p))
+    return res, proof, cap, to_int(out.claim_ab), to_int(out.claim_c)
+
+
+def lig_initial_k(m: int) -> int:
+    return int(load().flock_lig_initial_k(m))
+
+
+def l0_cap_depth(m: int) -> int:
+    return int(load().flock_l0_cap_depth(m))
 
 
 def prove_footprint(req: ProveRequest) -> int:

@@ -473,6 +473,47 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* ctx, flock_fs* fs, const flock_f128*
                              const flock_f128* d_b, uint32_t log_n, flock_f128* h_msgs,
                              flock_f128 h_finals[4]);
 
+/* ---- merged PCS opening (pcs.rs:1433-1822) ---------------------------------------- */
+
+/* Device workspace of the opening: grow-only buffers reused across calls on one context.
+ * No counterpart in the reference (its opening allocates through scratch.rs pools). */
+typedef struct flock_open_ws flock_open_ws;
+int flock_open_ws_create(flock_open_ws** out);
+void flock_open_ws_destroy(flock_open_ws* ws);
+/* device bytes the workspace holds */
+uint64_t flock_open_ws_bytes(const flock_open_ws* ws);
+
+/* initial_k of the embedded `fast` Ligerito config for m (ligerito::embedded_initial_k,
+ * ligerito.rs:564-574; configs/ligerito/m{22..35}_fast.toml), or -1 without one. The commit's
+ * log_batch_size must equal it. */
+int flock_lig_initial_k(uint32_t m);
+/* PcsParams::l0_cap_depth (pcs/commit.rs:266-277): the cap depth of the L0 commitment for
+ * dense m under the `fast` profile (the top set bit of the L0 query count), or -1. */
+int flock_l0_cap_depth(uint32_t m);
+
+/* s_hat_v_from_z_vec (pcs/ring_switch.rs:1278-1330): h_out[b] = sum_k eq(tail, k) z_vec[128 k + b]
+ * over the lincheck's captured z_vec (z_vec_len = 128 << tail_len words, device). */
+int flock_hip_s_hat_v_from_z_vec(flock_hip_ctx* ctx, flock_open_ws* ws, const flock_f128* d_z_vec,
+                                 size_t z_vec_len, const flock_f128* h_tail, size_t tail_len,
+                                 flock_f128 h_out[128]);
+
+/* pcs::open_batch_merged (pcs.rs:1433-1822) for n_claims (1..4) ring-switched claims with
+ * precomputed s_hat_v, on transcript fs: ring switch, jagged merged weight and dense merged
+ * sumcheck, the multipoint twisted assist with its Frobenius anchor on a forked transcript,
+ * and the eq-basis Ligerito opening (F256 ladder, stratified queries, capped paths) under the
+ * embedded `fast` config for p->m (OpeningGrinding::per_challenge_128, pcs.rs:155-205).
+ * d_q: the committed dense stack (2^(m-7) words); d_codeword / d_tree: its commitment (full
+ * tree); h_x_outers: n_claims points of x_len = 1 + n_log + log2(n_heights) words (the quirky
+ * point's x_inner_rest ++ x_outer, prover.rs:43-48); h_s_hat_v: 128 words per claim;
+ * h_heights: the jagged column heights (union.rs:293-310). On success *out_bytes (free with
+ * flock_hip_free) holds the bincode of MergedOpenProof (pcs.rs:1362-1377) and h_q_eval, when
+ * not NULL, q^(rho). */
+int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, const flock_f128* d_q,
+                          const flock_pcs_params* p, const flock_f128* d_codeword, const uint8_t* d_tree,
+                          size_t n_claims, const flock_f128* h_x_outers, size_t x_len,
+                          const flock_f128* h_s_hat_v, const uint64_t* h_heights, size_t n_heights,
+                          uint32_t n_log, uint8_t** out_bytes, size_t* out_len, flock_f128* h_q_eval);
+
 /* ---- prove core and serving (host orchestration over the entry points above) ----------- */
 
 /* The three hash circuits of the batched and served workloads (r1cs_hashes/{sha2,blake3,
@@ -526,6 +567,23 @@ int flock_hip_prove_footprint(const flock_prove_request* req, uint64_t* bytes);
  * Stops before the PCS opening (pcs.rs:1433), so it is not a complete proof. */
 int flock_hip_prove_core(flock_hip_prover* p, const flock_prove_request* req,
                          flock_prove_result* res);
+/* The whole prove_fast for a registry with one boolean hash type: prove_union_with_binding_zc
+ * (prover.rs:831-1468) through prove_fast_ligerito_union (prover.rs:587-624) — the prove core
+ * above, binding the commitment CAP at flock_l0_cap_depth, then s_hat_v_ab and the merged
+ * opening. On success out->proof (free with flock_hip_free) is bincode(R1csProofMergedLigerito)
+ * (proof.rs:42-47): ZerocheckProof, LincheckProof (one matrix-eval pair), MergedOpenProof;
+ * out->cap (free with flock_hip_free) the commitment cap; claim_ab / claim_c the R1csClaim
+ * values. The SHA-256 of proof || cap || claim values is the digest the reference's
+ * byte-identity anchors pin (union_m6_fixtures.rs:156-172). */
+typedef struct {
+    uint8_t* proof;
+    size_t proof_len;
+    uint8_t* cap;
+    size_t cap_len;
+    flock_f128 claim_ab, claim_c;
+} flock_proof_bytes;
+int flock_hip_prove_fast(flock_hip_prover* p, const flock_prove_request* req, flock_prove_result* res,
+                         flock_proof_bytes* out);
 /* Serving loop: `workers` host threads, one prover each, take the requests in arrival order;
  * a request starts once its arrival time has passed and its footprint fits hbm_budget beside
  * the requests in flight (always, when none is). Results and timings go to res[i]; wall_ms

@@ -1,5 +1,6 @@
 """Write prove_core_vectors.json: the oracle's prove core (witness, dense stack, commit,
-bind_statement, zerocheck, lincheck on one transcript) at the config-2 batch size of 1024
+bind_statement of the registry digest and the commitment cap, zerocheck, lincheck on one
+transcript) at the config-2 batch size of 1024
 instances, for SHA-256, BLAKE3 and Keccak-f[1600] inputs drawn from a seed. The GPU test
 regenerates the same inputs from the seed and compares the commitment root and the transcript
 digest. Run from the repository root: python tests/golden/make_prove_core_vectors.py"""
@@ -33,11 +34,12 @@ def main():
     for kind, n_log, n_decl, seed in CASES:
         t = time.time()
         root, dig = T.oracle_prove_core(kind, n_log, T.inputs_for(kind, n_decl, seed), circ)
-        out.append(dict(kind=kind, n_log=n_log, n_decl=n_decl, seed=seed, root=root.hex(),
+        out.append(dict(kind=kind, n_log=n_log, n_decl=n_decl, seed=seed,
+                        digest=T.digest_for(kind, n_log).hex(), root=root.hex(),
                         transcript=dig.hex()))
         print(kind, n_log, f"{time.time() - t:.1f} s", flush=True)
     with open(os.path.join(HERE, "prove_core_vectors.json"), "w") as fh:
-        json.dump(dict(domain=T.DOMAIN.decode(), digest=T.DIGEST.hex(), cases=out), fh, indent=1)
+        json.dump(dict(domain=T.DOMAIN.decode(), cases=out), fh, indent=1)
 
 
 if __name__ == "__main__":

@@ -87,8 +87,9 @@ class _Bin:
 
 
 def min_n_log(n: int) -> int:
-    """min_n_blocks_log: the smallest nu with 2^nu >= n (r1cs_hashes/common.rs)."""
-    nu = 0
+    """min_n_blocks_log (blake3.rs:1225-1229, keccak3.rs:144-149): the smallest nu >= 3 with
+    2^nu >= n."""
+    nu = 3
     while (1 << nu) < n:
         nu += 1
     return nu
@@ -97,16 +98,23 @@ def min_n_log(n: int) -> int:
 def hash_types(kind):
     """The registry's TableType for the hash (schedule.rs:150-186) as an oracle TypeStmt,
     plus the CSC lincheck circuit."""
-    if kind == HASH_SHA256:
-        import test_oracle_sha2_r1cs as M
-    elif kind == HASH_BLAKE3:
-        import test_oracle_blake3_r1cs as M
-    else:
-        raise ValueError("keccak registries take the walker circuit")
-    (ap, ac), (bp, bc), csc_a, csc_b = M.matrices()
     k_log, ub, pin = SHAPES[kind]
     k = 1 << k_log
     keep = []
+    if kind == HASH_KECCAK:
+        # KeccakSetup's registry type (keccak.rs:880-888): the empty-stub R1CS
+        # (common.rs:222-260: k x k matrices without entries, C_0 = I) with the walker's
+        # constant pin copied onto the type; the lincheck takes the walker circuit.
+        zeros = np.zeros(k + 1, dtype=np.uint64)
+        none = np.zeros(1, dtype=np.uint32)
+        ty = O.TypeStmt(k_log, 1, ub, pin, O.sparse_csr(zeros, none, k, keep),
+                        O.sparse_csr(zeros, none, k, keep), O.sparse(num_rows=k, identity=True))
+        return ty, None, keep
+    if kind == HASH_SHA256:
+        import test_oracle_sha2_r1cs as M
+    else:
+        import test_oracle_blake3_r1cs as M
+    (ap, ac), (bp, bc), csc_a, csc_b = M.matrices()
     ty = O.TypeStmt(k_log, 1, ub, pin, O.sparse_csr(ap, ac, k, keep), O.sparse_csr(bp, bc, k, keep),
                     O.sparse(num_rows=k, identity=True))
     return ty, (csc_a, csc_b), keep
@@ -155,7 +163,8 @@ def prove_fast(kind, inputs, domain: bytes, n_log: int | None = None, parts: dic
         n_log = min_n_log(n_decl)
     k_log, ub, pin = SHAPES[kind]
     z, a, b, stripe = witness(kind, inputs, n_log)
-    count = n_decl
+    # Keccak batches count the full capacity: padding rows run keccak_f(0) (keccak.rs:922-939)
+    count = (1 << n_log) if kind == HASH_KECCAK else n_decl
     U = Union([(k_log, ub, False)], n_log)
     sz0 = U.sizes([count], 0, 0)
     dense_m = sz0["dense_m"]

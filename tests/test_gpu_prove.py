@@ -25,7 +25,14 @@ L.or_x_ab_from_mlv.argtypes = [P, ctypes.c_size_t, ctypes.c_uint, P, P]
 L.or_zerocheck_prove.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]
 
 DOMAIN = b"flock-readme-bench-v0"
-DIGEST = bytes(range(32))
+
+
+def digest_for(kind, n_log):
+    """Registry::digest (schedule.rs:542-580) of the one-type registry of the hash at nu = n_log,
+    through the library (flock_registry_digest) — the value bind_statement absorbs."""
+    from test_oracle_blake3_hash import lib_type_statement
+    keep = []
+    return F.registry_digest([lib_type_statement(kind, keep)], n_log)
 # kind: (k_log, useful_bits, const pin)
 SHAPES = {F.HASH_SHA256: (15, 25_470, 25_469), F.HASH_BLAKE3: (14, 11_707, 11_706),
           F.HASH_KECCAK: (16, 42_560, 4096)}
@@ -82,13 +89,24 @@ def oracle_prove_core(kind, n_log, inputs, circuits, parts=None):
     z, a, b, stripe = batch_major(inputs, n_log)
     count = n if kind == F.HASH_KECCAK else len(inputs)
     U = Union([(k_log, ub, False)], n_log)
-    sz = U.sizes([count], 0, 6)
+    dense_m = U.sizes([count], 0, 6)["dense_m"]
+    # log_batch_size = the Ligerito config's initial_k and the cap at the L0 cap depth
+    # (pcs/commit.rs:266-277); shapes below the m22 configs keep 6 and the root
+    lbs = L.or_lig_initial_k(dense_m)
+    cd = max(L.or_cap_depth_for(dense_m), 0)
+    lbs = lbs if lbs > 0 else 6
+    sz = U.sizes([count], 0, lbs)
     q = U.compact([count], z)
-    _, tree = O.commit_lane_major(q, sz["dense_m"], 1, 6, sz["commit_lanes"] or 0)
+    if sz["commit_lanes"]:
+        _, tree = O.commit_lane_major(q, sz["dense_m"], 1, lbs, sz["commit_lanes"])
+    else:
+        _, tree = O.commit(q, sz["dense_m"], 1, lbs, 0)
     root = tree[-32:]
+    n_leaves = (len(tree) // 32 + 1) // 2
+    cap = tree[32 * (2 * n_leaves - (2 << cd)):][:32 << cd]
     fs = O.Fs(DOMAIN)
     counts = np.array([count], dtype=np.uint64)
-    L.or_bind_statement(fs.h, DIGEST, counts.ctypes.data, 1, root, 32)
+    L.or_bind_statement(fs.h, digest_for(kind, n_log), counts.ctypes.data, 1, cap, len(cap))
     m = n_log + k_log
     n_mlv = m - 6
     r1 = np.zeros((128, 2), dtype=np.uint64)
@@ -127,7 +145,7 @@ def oracle_prove_core(kind, n_log, inputs, circuits, parts=None):
     dig = ctypes.create_string_buffer(32)
     L.or_fs_digest(fs.h, dig)
     if parts is not None:
-        parts.update(root=root, count=count, m=m, zc=(r1, ml, fin, nonces, claim),
+        parts.update(root=root, cap=cap, count=count, m=m, zc=(r1, ml, fin, nonces, claim),
                      lc=dict(rounds=rounds, z_partial=zp, matrix_evals=mev, nonces=lnon,
                              claim=lcl))
     return root, dig.raw
@@ -143,7 +161,7 @@ def spec(kind, n_log, n_decl, seed, circuits, arrival=0.0):
 def test_prove_core_matches_oracle(circuits, kind, n_log, n_decl):
     say(f"case: prove core kind={kind} n_log={n_log} n_decl={n_decl}")
     sp = spec(kind, n_log, n_decl, 100 + 10 * kind + n_log, circuits)
-    reqs, keep = F.make_requests([sp], digest=DIGEST, domain=DOMAIN)
+    reqs, keep = F.make_requests([sp], digest=digest_for(kind, n_log), domain=DOMAIN)
     pr = F.Prover(0)
     res = F.prove_core(pr, reqs[0])
     pr.close()
@@ -151,6 +169,16 @@ def test_prove_core_matches_oracle(circuits, kind, n_log, n_decl):
     assert bytes(res.root) == root
     assert bytes(res.transcript) == dig
     assert res.hashes == n_decl
+
+
+def make_requests(specs):
+    """Requests whose statement binds each hash's real registry digest (per kind and nu)."""
+    arr, keep = F.make_requests(specs, digest=bytes(32), domain=DOMAIN)
+    for i, sp in enumerate(specs):
+        d = digest_for(sp[0], sp[1])
+        keep.append(d)
+        arr[i].digest = d
+    return arr, keep
 
 
 def _mix(circuits):
@@ -167,7 +195,7 @@ def _mix(circuits):
 def test_serve_results_are_the_same_alone_and_in_the_mix(circuits):
     say("case: serve mix of 12 requests on 3 workers")
     specs = _mix(circuits)
-    reqs, keep = F.make_requests(specs, digest=DIGEST, domain=DOMAIN)
+    reqs, keep = make_requests(specs)
     provers = [F.Prover(0) for _ in range(3)]
     res, wall = F.serve(provers, reqs, 1 << 40)
     alone = F.Prover(0)
@@ -189,7 +217,7 @@ def test_serve_results_are_the_same_alone_and_in_the_mix(circuits):
 def test_serve_admission_serialises_under_a_small_budget(circuits):
     say("case: serve with a one-request budget")
     specs = _mix(circuits)[:6]
-    reqs, keep = F.make_requests(specs, digest=DIGEST, domain=DOMAIN)
+    reqs, keep = make_requests(specs)
     provers = [F.Prover(0) for _ in range(3)]
     res, _ = F.serve(provers, reqs, 1)
     spans = sorted((r.done_s - r.run_ms * 1e-3, r.done_s) for r in res)
@@ -216,12 +244,13 @@ def test_prove_core_at_config2_size_matches_golden_vectors(circuits):
     path = os.path.join(os.path.dirname(__file__), "golden", "prove_core_vectors.json")
     with open(path) as fh:
         gold = json.load(fh)
-    assert gold["domain"].encode() == DOMAIN and bytes.fromhex(gold["digest"]) == DIGEST
+    assert gold["domain"].encode() == DOMAIN
     pr = F.Prover(0)
     for case in gold["cases"]:
         say(f"case: golden prove core kind={case['kind']} n_log={case['n_log']}")
         sp = spec(case["kind"], case["n_log"], case["n_decl"], case["seed"], circuits)
-        reqs, keep = F.make_requests([sp], digest=DIGEST, domain=DOMAIN)
+        assert digest_for(case["kind"], case["n_log"]).hex() == case["digest"]
+        reqs, keep = make_requests([sp])
         res = F.prove_core(pr, reqs[0])
         assert bytes(res.root).hex() == case["root"]
         assert bytes(res.transcript).hex() == case["transcript"]

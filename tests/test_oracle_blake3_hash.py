@@ -141,6 +141,34 @@ def _hash_types():
     return out
 
 
+_LIB_TYPES = {}
+
+
+def lib_type_statement(kind, keep):
+    """The library TypeStatement of a hash's registry type: the SHA-256 / BLAKE3 circuits'
+    (A_0, B_0, I) with their constant pins (r1cs_hashes/{sha2,blake3}.rs), and Keccak-f[1600]'s
+    empty-stub matrices with the walker's pin at 4096 (keccak.rs:880-888, common.rs:222-260)."""
+    if kind not in _LIB_TYPES:
+        kp = []
+        if kind == F.HASH_KECCAK:
+            k = 1 << 16
+            zeros = np.zeros(k + 1, dtype=np.uint64)
+            none = np.zeros(1, dtype=np.uint32)
+            ty = F.TypeStatement(16, 1, 42_560, 4096, F.sparse_matrix(zeros, none, num_cols=k, keep=kp),
+                                 F.sparse_matrix(zeros, none, num_cols=k, keep=kp),
+                                 F.sparse_matrix(num_rows=k, identity=True))
+        else:
+            name = "sha256" if kind == F.HASH_SHA256 else "blake3"
+            k_log, useful, pin, (ap, ac), (bp, bc) = _hash_types()[name]
+            k = 1 << k_log
+            ty = F.TypeStatement(k_log, 1, useful, pin, F.sparse_matrix(ap, ac, num_cols=k, keep=kp),
+                                 F.sparse_matrix(bp, bc, num_cols=k, keep=kp),
+                                 F.sparse_matrix(num_rows=k, identity=True))
+        _LIB_TYPES[kind] = (ty, kp)
+    keep.append(_LIB_TYPES[kind][1])
+    return _LIB_TYPES[kind][0]
+
+
 def test_registry_digest_of_the_hash_circuits():
     types = _hash_types()
     keep = []

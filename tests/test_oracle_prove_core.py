@@ -26,4 +26,4 @@ def test_blake3_golden_case_regenerates():
     # the golden proof (whose digest the GPU prove core reproduces at this size) verifies
     from test_verify_prove_core import verify_core
     assert verify_core(case["kind"], case["n_log"], parts, circuits, gold["domain"].encode(),
-                       bytes.fromhex(gold["digest"])) == dig
+                       bytes.fromhex(case["digest"])) == dig

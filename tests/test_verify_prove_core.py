@@ -18,7 +18,7 @@ def verify_core(kind, n_log, parts, circuits, domain, digest):
     from test_gpu_prove import SHAPES
     k_log, ub, pin = SHAPES[kind]
     fs = F.FsChallenger(domain)
-    fs.bind_statement(digest, [parts["count"]], parts["root"])
+    fs.bind_statement(digest, [parts["count"]], parts["cap"])
     r1, ml, fin, nonces, zclaim = parts["zc"]
     claim = F.zerocheck_verify(fs, parts["m"], True, r1, ml, fin, nonces)
     assert np.array_equal(claim, zclaim)
@@ -30,7 +30,7 @@ def verify_core(kind, n_log, parts, circuits, domain, digest):
     if kind == F.HASH_KECCAK:
         # the Keccak circuit enters through its walker: the host walk, and the deferred form
         fs2 = F.FsChallenger(domain)
-        fs2.bind_statement(digest, [parts["count"]], parts["root"])
+        fs2.bind_statement(digest, [parts["count"]], parts["cap"])
         F.zerocheck_verify(fs2, parts["m"], True, r1, ml, fin, nonces)
         dclaim = F.lincheck_verify(fs2, k_log, pin, n_log, parts["count"], zc[0], x_rest,
                                    x_outer, va, vb, True, parts["lc"], None)
@@ -63,7 +63,7 @@ def test_prove_core_proof_verifies(kind, n_log, n_decl):
     parts = {}
     inputs = T.inputs_for(kind, n_decl, 100 + 10 * kind + n_log)
     _, dig = T.oracle_prove_core(kind, n_log, inputs, circuits, parts)
-    assert verify_core(kind, n_log, parts, circuits, T.DOMAIN, T.DIGEST) == dig
+    assert verify_core(kind, n_log, parts, circuits, T.DOMAIN, T.digest_for(kind, n_log)) == dig
 
 
 def test_keccak_walk_check_rejects_a_wrong_claim():
@@ -74,7 +74,7 @@ def test_keccak_walk_check_rejects_a_wrong_claim():
     parts = {}
     T.oracle_prove_core(F.HASH_KECCAK, 3, T.inputs_for(F.HASH_KECCAK, 5, 9), {}, parts)
     fs = F.FsChallenger(T.DOMAIN)
-    fs.bind_statement(T.DIGEST, [parts["count"]], parts["root"])
+    fs.bind_statement(T.digest_for(F.HASH_KECCAK, 3), [parts["count"]], parts["cap"])
     r1, ml, fin, nonces, _ = parts["zc"]
     zc = O.arr_to_ints(F.zerocheck_verify(fs, parts["m"], True, r1, ml, fin, nonces))
     reg = F.registry_new([(16, 42_560, False)], 3)
@@ -92,7 +92,7 @@ def test_prove_core_with_another_statement_is_rejected():
     circuits = _host_circuits()
     parts = {}
     T.oracle_prove_core(F.HASH_SHA256, 3, T.inputs_for(F.HASH_SHA256, 8, 5), circuits, parts)
-    parts["root"] = bytes(32)
+    parts["cap"] = bytes(len(parts["cap"]))
     with pytest.raises(F.FlockHipError) as e:
-        verify_core(F.HASH_SHA256, 3, parts, circuits, T.DOMAIN, T.DIGEST)
+        verify_core(F.HASH_SHA256, 3, parts, circuits, T.DOMAIN, T.digest_for(F.HASH_SHA256, 3))
     assert e.value.code == F.E_REJECT
