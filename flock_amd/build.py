@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libflock_hip.so")
 SOURCES = ["api.hip", "field.hip", "ntt.hip", "merkle.hip", "sumcheck.hip", "zerocheck.hip",
-           "lincheck.hip", "witness.hip", "prove.hip", "opening.hip"]
+           "lincheck.hip", "witness.hip", "prove.hip", "opening.hip", "circuits.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -870,6 +870,16 @@ const char* flock_hip_last_error(void) { return g_err.c_str(); }
 
 int flock::set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
 
+uint64_t flock::ctx_scratch_bytes(const flock_hip_ctx* c) { return c ? c->scratch_bytes : 0; }
+
+void flock::ctx_release_scratch(flock_hip_ctx* c) {
+    if (!c || !c->scratch) return;
+    hipStreamSynchronize(c->stream);
+    hipFree(c->scratch);
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+}
+
 extern "C" {
 
 int flock_hip_ctx_set_round_mode(flock_hip_ctx* c, int mode) {

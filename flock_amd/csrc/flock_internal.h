@@ -7,6 +7,8 @@
 
 #include "transcript_device.h"
 
+struct flock_hip_ctx;
+
 namespace flock {
 
 // Two F128 per block partial (e.g. u0, u2) — 32 bytes.
@@ -174,5 +176,8 @@ hipError_t launch_sha2_witness(const uint32_t* inputs, uint64_t n_decl, uint32_t
 
 // Record an error for flock_hip_last_error (implemented in api.hip); returns code.
 int set_error(int code, const char* msg);
+// Bytes of the context's grow-only driver scratch, and its release (after a stream sync).
+uint64_t ctx_scratch_bytes(const flock_hip_ctx* c);
+void ctx_release_scratch(flock_hip_ctx* c);
 
 }  // namespace flock

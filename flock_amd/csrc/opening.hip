@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -732,6 +733,13 @@ struct flock_open_ws {
     std::vector<DBuf> bufs;
     uint32_t* acc = nullptr;        // device accumulators (256 words)
     uint32_t* hacc = nullptr;       // pinned mirror
+    double stage_ms[6] = {};        // host wall time of the last opening's stages
+    std::chrono::steady_clock::time_point mark;
+    void stamp(int i) {
+        const auto t = std::chrono::steady_clock::now();
+        if (i >= 0) stage_ms[i] = std::chrono::duration<double, std::milli>(t - mark).count();
+        mark = t;
+    }
     void* get(size_t slot, size_t bytes) {
         if (bufs.size() <= slot) bufs.resize(slot + 1);
         DBuf& b = bufs[slot];
@@ -1593,6 +1601,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     const uint64_t area = J.area();
     const uint32_t n_cols = (uint32_t)n_heights;
     Bin out;
+    ws->stamp(-1);
     label(fs, "flock-merged-open-v1");
     // ring switch per claim (ring_switch.rs:2723-2760), 3 PoW bits each
     std::vector<std::vector<H>> eqr(n_claims);
@@ -1662,6 +1671,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
                        reinterpret_cast<const uint4*>(d_q), reinterpret_cast<uint4*>(W), N, area, d_ps, n_cols,
                        jc, X.ws->acc);
     if (!X.chk("jagged weight") || !X.read_acc(8)) return flock::set_error(X.err, X.what.c_str());
+    ws->stamp(0);
     // the merged sumcheck <q, W> = target (pcs.rs:1640-1700), merged_round_bits = 2
     if (!X.hip(hipMemcpyAsync(A, d_q, 16 * N, hipMemcpyDeviceToDevice, X.st), "copy q"))
         return flock::set_error(X.err, X.what.c_str());
@@ -1697,6 +1707,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         if (!X.d2h(&q_eval, ca, 16)) return flock::set_error(X.err, X.what.c_str());
     }
     if (h_q_eval) *h_q_eval = q_eval;
+    ws->stamp(1);
     // fork the assist's transcript (challenger.rs:167-191)
     const H seed0 = sample(fs), seed1 = sample(fs);
     static const uint8_t kFork[] = "flock-par-assist-v1";
@@ -1824,12 +1835,15 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
                 mp.u64(trounds.size() / 2);
                 for (const H& x : trounds) mp.f(x);
                 mp.vec_u64(tnon.data(), tnon.size());
+                ws->stamp(2);
                 frobenius_assist(ch, J, zr, ww, point, 2, mp);
+                ws->stamp(3);
             }
         }
     }
     Bin inner;
     if (ok) ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);
+    ws->stamp(4);
     // merge the child (challenger.rs:193-203)
     const H d0 = sample(ch), d1 = sample(ch);
     observe(fs, d0);
@@ -1854,6 +1868,13 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     std::memcpy(p, out.b.data(), out.b.size());
     *out_bytes = p;
     *out_len = out.b.size();
+    ws->stamp(5);
+    return 0;
+}
+
+int flock_open_ws_stage_ms(const flock_open_ws* ws, double* out, size_t n) {
+    if (!ws || !out) return FLOCK_HIP_E_INVALID;
+    for (size_t i = 0; i < n && i < 6; i++) out[i] = ws->stage_ms[i];
     return 0;
 }
 

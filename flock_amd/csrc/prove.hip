@@ -1,15 +1,14 @@
-// Host orchestration above the phase-level ABI: one prove core per call, and a serving loop
-// that runs many such requests concurrently on one GPU.
+// Host orchestration above the phase-level ABI: one whole prove_fast (or its prove core) per
+// call, and a serving loop that runs many such requests concurrently on one GPU.
 //
-// The prove core is the GPU part of prove_fast for a registry with one boolean hash type
-// (crates/flock-prover/src/prover.rs:926-1160): batch-major witness, dense stack, lane-major
-// commit, bind_statement, zerocheck (capturing s_hat_v_c), x_ab split and the union lincheck.
-// It stops before the PCS opening, so the output is the commitment root and the transcript
-// state (which has absorbed every message of the two sumcheck phases), not a proof.
+// prove_fast for a registry with one boolean hash type (crates/flock-prover/src/prover.rs:
+// 831-1468): batch-major witness, dense stack, commit, bind_statement of the cap, zerocheck
+// (capturing s_hat_v_c), x_ab split, the union lincheck, then s_hat_v_ab and the merged PCS
+// opening, assembled into the bincode proof. The prove core stops before the opening.
 //
 // Serving: each worker owns a prover (context = HIP stream + grow-only device buffers) on its
 // own host thread. Requests are taken in arrival order; one is admitted when its arrival time
-// has passed and its device footprint fits the budget beside the requests in flight (a
+// has passed and its device footprint fits the budget beside what the other workers hold (a
 // request is always admitted when nothing is in flight). Workers never share a stream, so
 // one request's latency-bound rounds (host transcript steps between kernels) overlap the
 // others' kernels. Every request's transcript is independent, so a result does not depend
@@ -26,6 +25,7 @@
 #include <vector>
 
 #include "../../include/flock_hip.h"
+#include "flock_internal.h"
 
 namespace {
 
@@ -87,8 +87,7 @@ struct Plan {
 
 int plan_request(const flock_prove_request* r, Plan* p) {
     if (!r || !shape_of(r->kind, &p->hs) || r->n_log < 3 || r->n_log > 24 || r->n_decl == 0 ||
-        r->n_decl > (1ull << r->n_log) || !r->h_inputs || !r->digest ||
-        (r->kind != FLOCK_HASH_KECCAK && !r->circuit) || (r->domain_len && !r->domain))
+        r->n_decl > (1ull << r->n_log) || !r->h_inputs || (r->domain_len && !r->domain))
         return FLOCK_HIP_E_INVALID;
     const flock_table_type t = {p->hs.k_log, 0, p->hs.useful_bits};
     if (int rc = flock_registry_new(&t, 1, r->n_log, &p->reg)) return rc;
@@ -137,7 +136,32 @@ struct flock_hip_prover {
     flock_hip_ctx* ctx = nullptr;
     DevBuf buf[B_COUNT];
     flock_open_ws* ws = nullptr;  // the opening's workspace, created on first use
+    flock_csc circ[2] = {};       // built-in SHA-256 / BLAKE3 circuits on the device, on first use
 };
+
+namespace {
+
+// Device bytes a prover holds between requests: its request buffers, the opening's workspace,
+// the context's driver scratch and the built-in circuits it uploaded.
+uint64_t held_bytes(const flock_hip_prover* p) {
+    uint64_t s = 0;
+    for (const auto& b : p->buf) s += b.bytes;
+    s += flock_open_ws_bytes(p->ws) + flock::ctx_scratch_bytes(p->ctx);
+    for (const auto& c : p->circ)
+        if (c.a_col_ptr) s += 4 * (2 * ((1ull << c.k_log) + 1) + c.nnz_a + c.nnz_b);
+    return s;
+}
+
+// Give back everything but the circuits (the next request grows what it needs again).
+void release_buffers(flock_hip_prover* p) {
+    flock_hip_sync(p->ctx);
+    for (auto& b : p->buf) b.release(p->ctx);
+    flock_open_ws_destroy(p->ws);
+    p->ws = nullptr;
+    flock::ctx_release_scratch(p->ctx);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -159,6 +183,8 @@ void flock_hip_prover_destroy(flock_hip_prover* p) {
     if (p->ctx) {
         flock_hip_sync(p->ctx);
         for (auto& b : p->buf) b.release(p->ctx);
+        for (auto& c : p->circ)
+            if (c.a_col_ptr) flock_hip_circuit_release(p->ctx, &c);
         flock_open_ws_destroy(p->ws);
         flock_hip_ctx_destroy(p->ctx);
     }
@@ -214,6 +240,12 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     if (!pr || !pr->ctx || !res) return FLOCK_HIP_E_INVALID;
     std::memset(res, 0, sizeof(*res));
     if (out) std::memset(out, 0, sizeof(*out));
+    double t_mark = now_s();
+    auto stage = [&](int i) {
+        const double t = now_s();
+        res->stage_ms[i] = (t - t_mark) * 1e3;
+        t_mark = t;
+    };
     Plan p;
     if (int rc = plan_request(req, &p)) return res->status = rc;
     if (out && !p.has_config) return res->status = FLOCK_HIP_E_INVALID;
@@ -223,7 +255,7 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     const uint64_t zwords = n << (kl - 7);      // F128 words of z (and of a, b)
     const uint64_t stripe_bytes = n << (kl - 3);
     int rc = 0;
-    const size_t need[B_COUNT] = {(size_t)(req->n_decl * p.hs.input_bytes),
+    const size_t need[B_COUNT] = {req->inputs_on_device ? 0 : (size_t)(req->n_decl * p.hs.input_bytes),
                                   (size_t)(16 * zwords),
                                   (size_t)(16 * zwords),
                                   (size_t)(16 * zwords),
@@ -234,8 +266,22 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
                                   (size_t)(out ? 16ull << kl : 16)};
     for (int i = 0; i < B_COUNT && !rc; i++) rc = pr->buf[i].ensure(c, need[i]);
     if (!rc && out && !pr->ws) rc = flock_open_ws_create(&pr->ws);
+    // the request's circuit, else the built-in one; its digest, else the built-in registry's
+    const flock_csc* circuit = req->circuit;
+    if (!rc && !circuit && req->kind != FLOCK_HASH_KECCAK) {
+        flock_csc& bc = pr->circ[req->kind];
+        if (!bc.a_col_ptr) rc = flock_hip_circuit_upload(c, req->kind, &bc);
+        circuit = &bc;
+    }
+    uint8_t digest[32];
+    if (!rc) {
+        if (req->digest)
+            std::memcpy(digest, req->digest, 32);
+        else
+            rc = flock_circuit_registry_digest(req->kind, nl, digest);
+    }
     if (rc) return res->status = rc;
-    void* d_in = pr->buf[B_IN].p;
+    void* d_in = req->inputs_on_device ? const_cast<void*>(req->h_inputs) : pr->buf[B_IN].p;
     auto* z = static_cast<flock_f128*>(pr->buf[B_Z].p);
     auto* a = static_cast<flock_f128*>(pr->buf[B_A].p);
     auto* b = static_cast<flock_f128*>(pr->buf[B_B].p);
@@ -245,8 +291,8 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     auto* tree = static_cast<uint8_t*>(pr->buf[B_TREE].p);
     auto* zvec = out ? static_cast<flock_f128*>(pr->buf[B_ZVEC].p) : nullptr;
     hipStream_t st = static_cast<hipStream_t>(flock_hip_ctx_stream(c));
-    if (hipMemcpyAsync(d_in, req->h_inputs, need[B_IN], hipMemcpyHostToDevice, st) !=
-        hipSuccess)
+    if (!req->inputs_on_device &&
+        hipMemcpyAsync(d_in, req->h_inputs, need[B_IN], hipMemcpyHostToDevice, st) != hipSuccess)
         return res->status = FLOCK_HIP_E_HIP;
     // 1. witness (generate_witness_batch_major{,_partial})
     switch (req->kind) {
@@ -279,6 +325,7 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
         rc = FLOCK_HIP_E_HIP;
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = FLOCK_HIP_E_HIP;
     if (rc) return res->status = rc;
+    stage(0);
     // 4. statement binding (union.rs:619-628), zerocheck with s_hat_v_c (prover.rs:1094-1103)
     flock_fs* fs = flock_fs_new(req->domain, req->domain_len);
     if (!fs) return res->status = FLOCK_HIP_E_NOMEM;
@@ -286,13 +333,14 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     std::vector<flock_f128> r1(128), ml(2 * n_mlv), claim(1 + 2 * n_mlv), shv(2 * 128);
     std::vector<uint64_t> nonces(2 + n_mlv);
     flock_f128 finals[3];
-    rc = flock_union_bind_statement(fs, req->digest, &p.count, 1, cap.data(), cap.size());
+    rc = flock_union_bind_statement(fs, digest, &p.count, 1, cap.data(), cap.size());
     if (!rc)
         rc = flock_hip_zerocheck_prove(c, fs, reinterpret_cast<const uint8_t*>(a),
                                        reinterpret_cast<const uint8_t*>(b),
                                        reinterpret_cast<const uint8_t*>(z), p.m_zc, 1, r1.data(),
                                        ml.data(), finals, nonces.data(), claim.data(),
                                        shv.data() + 128);
+    stage(1);
     // 5. x_ab and the union lincheck (prover.rs:1140-1160)
     const uint32_t L = kl - 6;
     std::vector<flock_f128> x_rest(L ? L : 1), x_outer(nl), rounds(2 * (L ? L : 1)),
@@ -307,11 +355,12 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
                                                  x_rest.data(), x_outer.data(), 1, rounds.data(),
                                                  zp, mev, lnonces.data(), lclaim.data(), zvec);
         else
-            rc = flock_hip_lincheck_prove(c, fs, req->circuit, stripe, nl, p.hs.useful_bits,
+            rc = flock_hip_lincheck_prove(c, fs, circuit, stripe, nl, p.hs.useful_bits,
                                           p.n_rows, claim[0], x_rest.data(), x_outer.data(), 1,
                                           rounds.data(), zp, mev, lnonces.data(), lclaim.data(),
                                           zvec);
     }
+    stage(2);
     if (!rc && out) {
         // 6. s_hat_v_ab from the lincheck's z_vec (prover.rs:1186-1200), the claim points
         // (union.rs:564-600 as quirky x_outer_full, prover.rs:43-48) and the jagged heights
@@ -358,6 +407,7 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
             if (!out->proof || !out->cap) rc = FLOCK_HIP_E_NOMEM;
         }
         flock_hip_free(ob);
+        stage(3);
     }
     if (!rc) {
         flock_fs_digest(fs, res->transcript);
@@ -373,6 +423,14 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
 
 extern "C" {
 
+flock_open_ws* flock_hip_prover_open_ws(flock_hip_prover* pr) { return pr ? pr->ws : nullptr; }
+
+uint64_t flock_hip_prover_held_bytes(const flock_hip_prover* pr) { return pr ? held_bytes(pr) : 0; }
+
+void flock_hip_prover_release(flock_hip_prover* pr) {
+    if (pr && pr->ctx) release_buffers(pr);
+}
+
 int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
                          flock_prove_result* res) {
     return prove_impl(pr, req, res, nullptr);
@@ -386,7 +444,7 @@ int flock_hip_prove_fast(flock_hip_prover* pr, const flock_prove_request* req,
 
 int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
                     const flock_prove_request* reqs, size_t n, uint64_t hbm_budget,
-                    flock_prove_result* res, double* wall_ms) {
+                    flock_prove_result* res, flock_proof_bytes* proofs, double* wall_ms) {
     if (!provers || workers == 0 || (n && (!reqs || !res))) return FLOCK_HIP_E_INVALID;
     for (uint32_t w = 0; w < workers; w++)
         if (!provers[w] || !provers[w]->ctx) return FLOCK_HIP_E_INVALID;
@@ -397,14 +455,20 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
         if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s) return FLOCK_HIP_E_INVALID;
         fp[i] = plan_footprint(p, reqs[i].n_log);
     }
+    // Admission counts what every worker holds: a running worker reserves max(held, its
+    // request's footprint), an idle one what its grow-only buffers keep. A request that does
+    // not fit makes its worker give back its own buffers when that suffices; with nothing
+    // running it frees the idle workers' buffers and runs alone.
+    std::vector<uint64_t> reserved(workers);
+    for (uint32_t w = 0; w < workers; w++) reserved[w] = held_bytes(provers[w]);
     std::mutex mu;
     std::condition_variable cv;
     size_t next = 0;
-    uint64_t inflight = 0;
     uint32_t running = 0;
     int first_err = 0;
     const double t0 = now_s();
-    auto worker = [&](flock_hip_prover* pr) {
+    auto worker = [&](uint32_t w) {
+        flock_hip_prover* pr = provers[w];
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             if (next >= n || first_err) return;
@@ -415,22 +479,39 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
                 cv.wait_for(lk, std::chrono::duration<double>(due - t));
                 continue;
             }
-            if (running && inflight + fp[i] > hbm_budget) {
-                cv.wait(lk);
-                continue;
+            uint64_t others = 0;
+            for (uint32_t v = 0; v < workers; v++)
+                if (v != w) others += reserved[v];
+            uint64_t mine = held_bytes(pr);
+            if (others + std::max(mine, fp[i]) > hbm_budget) {
+                if (running == 0) {
+                    for (uint32_t v = 0; v < workers; v++)
+                        if (v != w && reserved[v]) {
+                            release_buffers(provers[v]);
+                            reserved[v] = held_bytes(provers[v]);
+                        }
+                    if (mine > fp[i]) release_buffers(pr);
+                } else if (others + fp[i] <= hbm_budget && mine > fp[i]) {
+                    release_buffers(pr);
+                } else {
+                    cv.wait(lk);
+                    continue;
+                }
+                mine = held_bytes(pr);
             }
             next++;
-            inflight += fp[i];
+            reserved[w] = std::max(mine, fp[i]);
             running++;
             lk.unlock();
             const double start = now_s();
-            const int rc = flock_hip_prove_core(pr, &reqs[i], &res[i]);
+            const int rc = proofs ? flock_hip_prove_fast(pr, &reqs[i], &res[i], &proofs[i])
+                                  : flock_hip_prove_core(pr, &reqs[i], &res[i]);
             const double end = now_s();
             res[i].queued_ms = (start - due) * 1e3;
             res[i].run_ms = (end - start) * 1e3;
             res[i].done_s = end - t0;
             lk.lock();
-            inflight -= fp[i];
+            reserved[w] = held_bytes(pr);
             running--;
             if (rc && !first_err) first_err = rc;
             cv.notify_all();
@@ -438,7 +519,7 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
     };
     std::vector<std::thread> th;
     th.reserve(workers);
-    for (uint32_t w = 0; w < workers; w++) th.emplace_back(worker, provers[w]);
+    for (uint32_t w = 0; w < workers; w++) th.emplace_back(worker, w);
     for (auto& x : th) x.join();
     if (wall_ms) *wall_ms = (now_s() - t0) * 1e3;
     return first_err;

@@ -128,7 +128,8 @@ class ProveRequest(ctypes.Structure):
                 ("n_decl", ctypes.c_uint64), ("h_inputs", ctypes.c_void_p),
                 ("circuit", ctypes.c_void_p), ("digest", ctypes.c_char_p),
                 ("domain", ctypes.c_char_p), ("domain_len", ctypes.c_size_t),
-                ("arrival_s", ctypes.c_double)]
+                ("arrival_s", ctypes.c_double), ("inputs_on_device", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class ProveResult(ctypes.Structure):
@@ -137,7 +138,8 @@ class ProveResult(ctypes.Structure):
     _fields_ = [("root", ctypes.c_uint8 * 32), ("transcript", ctypes.c_uint8 * 32),
                 ("zc_z", F128), ("lc_w", F128), ("hashes", ctypes.c_uint64),
                 ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
-                ("done_s", ctypes.c_double), ("status", ctypes.c_int),
+                ("done_s", ctypes.c_double), ("stage_ms", ctypes.c_double * 4),
+                ("status", ctypes.c_int),
                 ("reserved", ctypes.c_int)]
 
 
@@ -195,6 +197,15 @@ def load():
         "flock_hip_merkle_tree": (I, [P, P, S, S, P]),
         "flock_hip_merkle_tree_kind": (I, [P, P, S, S, U32, P]),
         "flock_registry_digest": (I, [P, S, U32, P]),
+        "flock_circuit_get": (I, [U32, ctypes.POINTER(P)]),
+        "flock_circuit_info": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(U64),
+                                   ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64)]),
+        "flock_circuit_csr": (I, [P, U32, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "flock_circuit_csc": (I, [P, U32, ctypes.POINTER(P), ctypes.POINTER(P)]),
+        "flock_circuit_type_statement": (I, [P, P]),
+        "flock_circuit_registry_digest": (I, [U32, U32, P]),
+        "flock_hip_circuit_upload": (I, [P, U32, P]),
+        "flock_hip_circuit_release": (None, [P, P]),
         "flock_blake3_hash": (None, [P, S, P]),
         "flock_hip_encode": (I, [P, P, S, PP, P]),
         "flock_hip_encode_lane_major": (I, [P, P, S, PP, P]),
@@ -246,10 +257,15 @@ def load():
         "flock_hip_prove_footprint": (I, [ctypes.POINTER(ProveRequest), ctypes.POINTER(U64)]),
         "flock_hip_prove_core": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult)]),
         "flock_hip_serve": (I, [ctypes.POINTER(P), U32, ctypes.POINTER(ProveRequest), S, U64,
-                                ctypes.POINTER(ProveResult), ctypes.POINTER(ctypes.c_double)]),
+                                ctypes.POINTER(ProveResult), ctypes.POINTER(ProofBytes),
+                                ctypes.POINTER(ctypes.c_double)]),
         "flock_open_ws_create": (I, [ctypes.POINTER(P)]),
         "flock_open_ws_destroy": (None, [P]),
         "flock_open_ws_bytes": (U64, [P]),
+        "flock_open_ws_stage_ms": (I, [P, ctypes.POINTER(ctypes.c_double), S]),
+        "flock_hip_prover_open_ws": (P, [P]),
+        "flock_hip_prover_held_bytes": (U64, [P]),
+        "flock_hip_prover_release": (None, [P]),
         "flock_lig_initial_k": (I, [U32]),
         "flock_l0_cap_depth": (I, [U32]),
         "flock_hip_s_hat_v_from_z_vec": (I, [P, P, P, S, P, S, P]),
@@ -543,6 +559,16 @@ class Context:
                 len(b_rows), const_pin)
         return c, bufs
 
+    def circuit_csc(self, kind: int) -> "Csc":
+        """Upload a built-in circuit's CSC (flock_hip_circuit_upload); release with
+        release_circuit."""
+        c = Csc()
+        _check(load().flock_hip_circuit_upload(self.h, kind, ctypes.byref(c)))
+        return c
+
+    def release_circuit(self, c: "Csc"):
+        load().flock_hip_circuit_release(self.h, ctypes.byref(c))
+
     def lc_csc_fold(self, circuit: "Csc", eq: DeviceBuffer, alpha: int, sa=None, sb=None,
                     comb=None):
         _check(load().flock_hip_lc_csc_fold(self.h, ctypes.byref(circuit), eq.ptr, f128(alpha),
@@ -642,6 +668,50 @@ def sparse_matrix(row_ptr=None, cols=None, num_cols: int = 0, num_rows=None,
     keep.extend([ptr, c])
     return SparseMatrix(len(ptr) - 1 if num_rows is None else num_rows, num_cols,
                         ptr.ctypes.data, c.ctypes.data, 0, 0)
+
+
+class Circuit:
+    """A built-in hash circuit (flock_circuit_get): (A_0, B_0) as CSR and CSC numpy copies."""
+
+    def __init__(self, kind: int):
+        h = ctypes.c_void_p()
+        _check(load().flock_circuit_get(kind, ctypes.byref(h)))
+        self.h, self.kind = h, kind
+        U64 = ctypes.c_uint64
+        k_log, useful, pin, na, nb = ctypes.c_uint32(), U64(), U64(), U64(), U64()
+        _check(load().flock_circuit_info(h, ctypes.byref(k_log), ctypes.byref(useful),
+                                         ctypes.byref(pin), ctypes.byref(na), ctypes.byref(nb)))
+        self.k_log, self.useful_bits, self.const_pin = k_log.value, useful.value, pin.value
+        self.nnz = (na.value, nb.value)
+
+    def _view(self, fn, which, ptr_ty, n_ptr):
+        p, c = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(fn(self.h, which, ctypes.byref(p), ctypes.byref(c)))
+        ptr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ptr_ty)), (n_ptr,)).copy()
+        n = int(ptr[-1])
+        cols = (np.ctypeslib.as_array(ctypes.cast(c, ctypes.POINTER(ctypes.c_uint32)), (n,)).copy()
+                if n else np.zeros(0, dtype=np.uint32))
+        return ptr, cols
+
+    def csr(self, which: int):
+        """(row_ptr uint64, cols uint32) of A_0 (which=0) or B_0 (which=1)."""
+        return self._view(load().flock_circuit_csr, which, ctypes.c_uint64, (1 << self.k_log) + 1)
+
+    def csc(self, which: int):
+        """(col_ptr uint32, rows uint32) of A_0 (which=0) or B_0 (which=1)."""
+        return self._view(load().flock_circuit_csc, which, ctypes.c_uint32, (1 << self.k_log) + 1)
+
+    def type_statement(self) -> "TypeStatement":
+        t = TypeStatement()
+        _check(load().flock_circuit_type_statement(self.h, ctypes.byref(t)))
+        return t
+
+
+def circuit_registry_digest(kind: int, nu: int) -> bytes:
+    """Registry::digest of the one-type registry of a built-in circuit at nu."""
+    out = ctypes.create_string_buffer(32)
+    _check(load().flock_circuit_registry_digest(kind, nu, out))
+    return out.raw
 
 
 def registry_digest(types, nu: int) -> bytes:
@@ -814,6 +884,20 @@ class Prover:
         ctx = load().flock_hip_prover_ctx(self.h)
         _check(load().flock_hip_ctx_set_round_mode(ctx, mode))
 
+    def held_bytes(self) -> int:
+        return int(load().flock_hip_prover_held_bytes(self.h))
+
+    def release(self):
+        load().flock_hip_prover_release(self.h)
+
+    def open_stage_ms(self) -> list[float]:
+        """Host wall times of the last prove_fast's opening stages (flock_open_ws_stage_ms)."""
+        ws = load().flock_hip_prover_open_ws(self.h)
+        out = (ctypes.c_double * 6)()
+        if ws:
+            _check(load().flock_open_ws_stage_ms(ws, out, 6))
+        return list(out)
+
     def close(self):
         if self.h:
             load().flock_hip_prover_destroy(self.h)
@@ -826,9 +910,10 @@ class Prover:
             pass
 
 
-def make_requests(specs, digest: bytes = bytes(32), domain: bytes = b"flock-readme-bench-v0"):
-    """specs: (kind, n_log, inputs ndarray, circuit Csc or None, arrival_s). Returns the ctypes
-    request array and the objects it points into (keep them alive while it is used)."""
+def make_requests(specs, digest: bytes | None = None, domain: bytes = b"flock-readme-bench-v0"):
+    """specs: (kind, n_log, inputs ndarray, circuit Csc or None, arrival_s). circuit None: the
+    built-in circuit; digest None: the built-in registry's digest. Returns the ctypes request
+    array and the objects it points into (keep them alive while it is used)."""
     arr = (ProveRequest * len(specs))()
     keep = [digest, domain]
     for i, (kind, n_log, inputs, circuit, arrival) in enumerate(specs):
@@ -842,6 +927,18 @@ def make_requests(specs, digest: bytes = bytes(32), domain: bytes = b"flock-read
         arr[i].digest, arr[i].domain, arr[i].domain_len = digest, domain, len(domain)
         arr[i].arrival_s = arrival
     return arr, keep
+
+
+def device_request(kind: int, n_log: int, n_decl: int, d_inputs: "DeviceBuffer",
+                   domain: bytes = b"flock-readme-bench-v0", digest: bytes | None = None):
+    """A request whose inputs are already in HBM (inputs_on_device), with the built-in circuit.
+    Returns (request, keep)."""
+    req = ProveRequest()
+    req.kind, req.n_log, req.n_decl = kind, n_log, n_decl
+    req.h_inputs = d_inputs.ptr
+    req.inputs_on_device = 1
+    req.digest, req.domain, req.domain_len = digest, domain, len(domain)
+    return req, [domain, digest, d_inputs]
 
 
 def prove_core(prover: Prover, req: ProveRequest) -> ProveResult:
@@ -877,11 +974,26 @@ def prove_footprint(req: ProveRequest) -> int:
     return int(b.value)
 
 
-def serve(provers, reqs, hbm_budget: int):
-    """flock_hip_serve over the given provers (one worker each). Returns (results, wall_ms)."""
+def serve(provers, reqs, hbm_budget: int, proofs: bool = False):
+    """flock_hip_serve over the given provers (one worker each). Returns (results, wall_ms), or
+    with proofs=True (whole prove_fast per request) (results, wall_ms, [(proof, cap, claim_ab,
+    claim_c)])."""
     hs = (ctypes.c_void_p * len(provers))(*[p.h.value for p in provers])
     res = (ProveResult * len(reqs))()
     wall = ctypes.c_double()
-    _check(load().flock_hip_serve(hs, len(provers), reqs, len(reqs), hbm_budget, res,
-                                  ctypes.byref(wall)))
-    return res, wall.value
+    pb = (ProofBytes * len(reqs))() if proofs else None
+    try:
+        _check(load().flock_hip_serve(hs, len(provers), reqs, len(reqs), hbm_budget, res, pb,
+                                      ctypes.byref(wall)))
+        if not proofs:
+            return res, wall.value
+        out = []
+        for b in pb:
+            out.append((ctypes.string_at(b.proof, b.proof_len), ctypes.string_at(b.cap, b.cap_len),
+                        to_int(b.claim_ab), to_int(b.claim_c)))
+        return res, wall.value, out
+    finally:
+        if pb is not None:
+            for b in pb:
+                load().flock_hip_free(ctypes.cast(b.proof, ctypes.c_void_p))
+                load().flock_hip_free(ctypes.cast(b.cap, ctypes.c_void_p))

@@ -393,6 +393,36 @@ int flock_registry_digest(const flock_type_statement* types, size_t n_types, uin
 /* blake3::hash (the 32-byte root hash, any length), the hash behind the registry digest.
  * Host only. */
 void flock_blake3_hash(const void* data, size_t len, uint8_t out[32]);
+/* ---- built-in hash circuits (crates/flock-prover/src/r1cs_hashes) ------------------ */
+
+/* (A_0, B_0) of one compression's boolean R1CS, C_0 = I, built on the host on first use and
+ * kept for the process (read-only, shared, never freed):
+ *   FLOCK_HASH_SHA256  sha2.rs:688-701 build_matrices (k_log 15, pin 25,469, 47.4 M nnz)
+ *   FLOCK_HASH_BLAKE3  blake3.rs:659-783 build_matrices (k_log 14, pin 11,706)
+ *   FLOCK_HASH_KECCAK  the registry's empty-stub pair (keccak.rs:880-888; k_log 16, pin 4096),
+ *                      its lincheck runs through the walker.
+ * The Rust seams: Sha2LincheckCircuit / Blake3LincheckCircuit (sha2.rs:837, blake3.rs:937) and
+ * the TableType of build_block_r1cs (schedule.rs:175-186). Host only. */
+typedef struct flock_circuit flock_circuit;
+int flock_circuit_get(uint32_t kind, const flock_circuit** out);
+int flock_circuit_info(const flock_circuit* c, uint32_t* k_log, uint64_t* useful_bits,
+                       uint64_t* const_pin, uint64_t* nnz_a, uint64_t* nnz_b);
+/* CSR rows (which = 0: A_0, 1: B_0), 2^k_log + 1 row pointers, ascending columns per row. */
+int flock_circuit_csr(const flock_circuit* c, uint32_t which, const uint64_t** row_ptr,
+                      const uint32_t** cols);
+/* The lincheck's CSC form (csc_from_rows, lincheck.rs:367-388): ascending rows per column. */
+int flock_circuit_csc(const flock_circuit* c, uint32_t which, const uint32_t** col_ptr,
+                      const uint32_t** rows);
+/* The registry TableType of the circuit (views into c). */
+int flock_circuit_type_statement(const flock_circuit* c, flock_type_statement* out);
+/* Registry::digest of the one-type registry of the built-in circuit at nu (what
+ * bind_statement absorbs for a single-hash prove_fast). */
+int flock_circuit_registry_digest(uint32_t kind, uint32_t nu, uint8_t out[32]);
+/* Upload the circuit's CSC to the device (flock_hip_malloc on ctx); release with
+ * flock_hip_circuit_release. */
+int flock_hip_circuit_upload(flock_hip_ctx* ctx, uint32_t kind, flock_csc* out);
+void flock_hip_circuit_release(flock_hip_ctx* ctx, flock_csc* c);
+
 /* bind_statement (union.rs:619-628): label "flock-mixed-v1", the 32-byte Registry::digest
  * (flock_registry_digest), the counts as u64 LE, the cap bytes. */
 int flock_union_bind_statement(flock_fs* fs, const uint8_t registry_digest[32],
@@ -482,6 +512,11 @@ int flock_open_ws_create(flock_open_ws** out);
 void flock_open_ws_destroy(flock_open_ws* ws);
 /* device bytes the workspace holds */
 uint64_t flock_open_ws_bytes(const flock_open_ws* ws);
+/* host wall time (ms) of the last flock_hip_open_merged's stages, n <= 6 of them: [0] ring
+ * switch and the jagged weight W, [1] the merged sumcheck, [2] the multipoint values and the
+ * twisted sumcheck, [3] the Frobenius assist, [4] the inner Ligerito opening, [5] the wire
+ * assembly. No counterpart in the reference (a profiling aid). */
+int flock_open_ws_stage_ms(const flock_open_ws* ws, double* out, size_t n);
 
 /* initial_k of the embedded `fast` Ligerito config for m (ligerito::embedded_initial_k,
  * ligerito.rs:564-574; configs/ligerito/m{22..35}_fast.toml), or -1 without one. The commit's
@@ -524,9 +559,12 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
 
 /* One proving request. h_inputs: n_decl instances of 24 u32 (SHA-256: h_in[8], m[16]), 28 u32
  * (BLAKE3: cv[8], m[16], counter_lo, counter_hi, block_len, flags) or 25 u64 (Keccak state
- * lanes). circuit: the device CSC circuit (SHA-256, BLAKE3); NULL for Keccak (walker).
- * digest: the 32-byte Registry::digest bound into the transcript. arrival_s: arrival offset
- * for flock_hip_serve (non-decreasing across a request array). 3 <= n_log <= 24. */
+ * lanes). circuit: the device CSC circuit (SHA-256, BLAKE3), or NULL for the built-in one
+ * (flock_hip_circuit_upload, once per prover); unused for Keccak (walker). digest: the
+ * 32-byte Registry::digest bound into the transcript, or NULL for the built-in one-type
+ * registry's (flock_circuit_registry_digest(kind, n_log)). arrival_s: arrival offset
+ * for flock_hip_serve (non-decreasing across a request array). inputs_on_device: the inputs
+ * are already in HBM (no host-to-device copy). 3 <= n_log <= 24. */
 typedef struct {
     uint32_t kind;
     uint32_t n_log;
@@ -537,19 +575,24 @@ typedef struct {
     const uint8_t* domain;
     size_t domain_len;
     double arrival_s;
+    uint32_t inputs_on_device; /* 1: h_inputs is a device pointer on the prover's device */
+    uint32_t reserved;
 } flock_prove_request;
 
 /* root: the commitment root (cap depth 0); transcript: flock_fs_digest after the lincheck,
  * which has absorbed the statement, every zerocheck and lincheck message and nonce;
  * zc_z / lc_w: the zerocheck skip point and the lincheck claim value; hashes = n_decl.
  * queued_ms, run_ms, done_s: set by flock_hip_serve (wait after arrival, run time, completion
- * time from the start of the call). */
+ * time from the start of the call). stage_ms: host wall time of the stages, each ending at a
+ * stream synchronisation — [0] upload, witness, dense stack and commit, [1] zerocheck,
+ * [2] lincheck, [3] the merged opening and the wire assembly (0 for the prove core). */
 typedef struct {
     uint8_t root[32];
     uint8_t transcript[32];
     flock_f128 zc_z, lc_w;
     uint64_t hashes;
     double queued_ms, run_ms, done_s;
+    double stage_ms[4];
     int status;
     int reserved;
 } flock_prove_result;
@@ -559,12 +602,21 @@ typedef struct flock_hip_prover flock_hip_prover;
 int flock_hip_prover_create(int device, flock_hip_prover** out);
 void flock_hip_prover_destroy(flock_hip_prover* p);
 flock_hip_ctx* flock_hip_prover_ctx(flock_hip_prover* p);
+/* The prover's opening workspace (null before its first prove_fast). */
+flock_open_ws* flock_hip_prover_open_ws(flock_hip_prover* p);
+/* Device bytes the prover holds between requests (grow-only request buffers, the opening's
+ * workspace, the context's driver scratch, uploaded built-in circuits), and their release
+ * (all but the circuits; the next request allocates again). */
+uint64_t flock_hip_prover_held_bytes(const flock_hip_prover* p);
+void flock_hip_prover_release(flock_hip_prover* p);
 /* Device bytes one request needs (the admission size of flock_hip_serve). */
 int flock_hip_prove_footprint(const flock_prove_request* req, uint64_t* bytes);
 /* The GPU prove core of prove_fast for a registry with one boolean hash type
- * (prover.rs:926-1160): witness, dense stack, lane-major commit (rate 1/2, log_batch_size 6),
- * bind_statement, zerocheck with s_hat_v_c, x_ab split and the union lincheck, with grinding.
- * Stops before the PCS opening (pcs.rs:1433), so it is not a complete proof. */
+ * (prover.rs:926-1160): witness, dense stack, commit (rate 1/2, log_batch_size = the
+ * Ligerito config's initial_k; lane-major for an integer lane count), bind_statement of the
+ * cap at flock_l0_cap_depth, zerocheck with s_hat_v_c, x_ab split and the union lincheck,
+ * with grinding. Stops before the PCS opening (pcs.rs:1433), so it is not a complete proof;
+ * below dense m 22 (no Ligerito config) it commits at log_batch_size 6 and binds the root. */
 int flock_hip_prove_core(flock_hip_prover* p, const flock_prove_request* req,
                          flock_prove_result* res);
 /* The whole prove_fast for a registry with one boolean hash type: prove_union_with_binding_zc
@@ -586,12 +638,16 @@ int flock_hip_prove_fast(flock_hip_prover* p, const flock_prove_request* req, fl
                          flock_proof_bytes* out);
 /* Serving loop: `workers` host threads, one prover each, take the requests in arrival order;
  * a request starts once its arrival time has passed and its footprint fits hbm_budget beside
- * the requests in flight (always, when none is). Results and timings go to res[i]; wall_ms
- * gets the time from the call's start to the last completion. No counterpart in the reference
- * Rust (its CUDA FFI serialises calls behind one mutex, prove_ffi.cu:421-422). */
+ * what the other workers hold (running ones: max(held, footprint)); a worker gives back its
+ * own buffers when that makes the request fit, and with nothing running the idle workers'
+ * buffers are freed and the request runs alone. Results and timings go to res[i]; with
+ * proofs != NULL each request is a whole prove_fast and proofs[i] gets its bytes (free each
+ * with flock_hip_free), else the prove core runs. wall_ms gets the time from the call's start
+ * to the last completion. No counterpart in the reference Rust (its CUDA FFI serialises calls
+ * behind one mutex, prove_ffi.cu:421-422). */
 int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
                     const flock_prove_request* reqs, size_t n, uint64_t hbm_budget,
-                    flock_prove_result* res, double* wall_ms);
+                    flock_prove_result* res, flock_proof_bytes* proofs, double* wall_ms);
 
 #ifdef __cplusplus
 }

@@ -3,7 +3,8 @@ the oracle:
 
 - the reference's byte-identity anchors (crates/flock-prover/tests/union_m6_fixtures.rs:289-323):
   BLAKE3 x 256 and SHA-256 x 128 over the fixtures' SplitMix64 inputs in the FS domain
-  `flock-m6-fixture-v0` give the pinned SHA-256 of proof || cap || claim values;
+  `flock-m6-fixture-v0` give the pinned SHA-256 of proof || cap || claim values, here through
+  the library's built-in circuits and registry digest;
 - the oracle's digests at larger and partial shapes (tests/golden/prove_fast_vectors.json: the
   1024-compression SHA-256 batch, BLAKE3 1000 of 1024 rows, SHA-256 300 of 512 rows);
 - byte-for-byte equality with the oracle's proof for Keccak-f[1600] (walker lincheck), whose
@@ -30,11 +31,13 @@ def say(*a):
     sys.stdout.flush()
 
 
-def gpu_prove_fast(kind, inputs, domain, circuits, n_log=None):
+def gpu_prove_fast(kind, inputs, domain, circuits, n_log=None, builtin=False):
+    """circuits / digest_for: the oracle's matrices and their registry digest; builtin: the
+    library's own circuit and digest (request circuit and digest NULL)."""
     n_log = n_log if n_log is not None else OP.min_n_log(len(inputs))
-    circ = circuits[kind][0] if kind != F.HASH_KECCAK else None
-    reqs, keep = F.make_requests([(kind, n_log, inputs, circ, 0.0)], digest=digest_for(kind, n_log),
-                                 domain=domain)
+    circ = circuits[kind][0] if kind != F.HASH_KECCAK and not builtin else None
+    dig = None if builtin else digest_for(kind, n_log)
+    reqs, keep = F.make_requests([(kind, n_log, inputs, circ, 0.0)], digest=dig, domain=domain)
     pr = F.Prover(0)
     try:
         res, proof, cap, ab, c = F.prove_fast(pr, reqs[0])
@@ -55,7 +58,8 @@ def test_gpu_reproduces_the_reference_anchor(circuits, kind, n, seed, expected):
     rng = OP.SplitMix32(seed)
     inputs = OP.fixture_blake3_inputs(rng, n) if kind == F.HASH_BLAKE3 else \
         OP.fixture_sha2_inputs(rng, n)
-    proof, cap, ab, c = gpu_prove_fast(kind, inputs, FIXTURE_DOMAIN, circuits)
+    # the library's built-in circuit and registry digest: nothing from the oracle
+    proof, cap, ab, c = gpu_prove_fast(kind, inputs, FIXTURE_DOMAIN, circuits, builtin=True)
     assert OP.anchor_digest(proof, cap, ab, c) == expected
 
 
