@@ -17,6 +17,10 @@
 // The eq table of round i+1 is the pairwise XOR of round i's: eq(r', (0,x)) +
 // eq(r', (1,x)) = ((1+r0) + r0) eq(r'', x) = eq(r'', x), so each message kernel also
 // emits the next round's table (n/4 -> n/8 entries) instead of rebuilding it.
+#include <cstring>
+#include <mutex>
+
+#include "../../include/flock_hip.h"
 #include "f128_device.h"
 #include "flock_internal.h"
 
@@ -519,3 +523,73 @@ hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, h
 }
 
 }  // namespace flock
+
+// ---- fold_and_msg_lsb without allocation ---------------------------------------------------
+//
+// The grid reduction's partial pairs and its ticket counter live in `scratch` below, a
+// file-scope device array of kMaxReduceBlocks * 2 + 1 words (16,400 bytes) that each device
+// holds for the life of the process. Host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (kept per device); the array's name is a host
+// stand-in, never a device address. Every call on a device shares that device's array, so
+// calls that overlap on one device race.
+
+static __device__ __align__(256) uint4 scratch[flock::kMaxReduceBlocks * 2 + 1];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint4* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(uint4** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return flock::set_error(FLOCK_HIP_E_NODEVICE, "fold_msg_static: no current device");
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return flock::set_error(FLOCK_HIP_E_HIP, "fold_msg_static: hipGetSymbolAddress failed");
+        }
+        g_scratch_ptr[dev] = static_cast<uint4*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// the ticket counter must read zero when a launch starts
+__global__ void k_clear_ticket(uint4* partials) {
+    partials[2 * flock::kMaxReduceBlocks] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+}  // namespace
+
+extern "C" int flock_hip_sumcheck_fold_msg_static(void* stream, const flock_f128* d_f,
+                                                  const flock_f128* d_b, size_t n, flock_f128 r,
+                                                  flock_f128* d_nf, flock_f128* d_nb,
+                                                  flock_f128* d_msg) {
+    if (n < 2 || (n & (n - 1)) || n > (1ull << 40))
+        return flock::set_error(FLOCK_HIP_E_INVALID,
+                                "fold_msg_static: n must be a power of two in [2, 2^40]");
+    if (!d_f || !d_b || !d_nf || !d_nb || !d_msg)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "fold_msg_static: null buffer");
+    uint4* partials = nullptr;
+    if (int rc = scratch_on_current_device(&partials)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    k_clear_ticket<<<1, 1, 0, s>>>(partials);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        uint4 rr;
+        std::memcpy(&rr, &r, sizeof(rr));
+        e = flock::launch_sc_fold_msg(reinterpret_cast<const uint4*>(d_f),
+                                      reinterpret_cast<const uint4*>(d_b), n, rr,
+                                      reinterpret_cast<uint4*>(d_nf), reinterpret_cast<uint4*>(d_nb),
+                                      partials, flock::round_out(reinterpret_cast<uint4*>(d_msg)), s);
+    }
+    if (e != hipSuccess) return flock::set_error(FLOCK_HIP_E_HIP, hipGetErrorString(e));
+    return 0;
+}

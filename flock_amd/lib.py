@@ -211,6 +211,7 @@ def load():
         "flock_hip_encode_lane_major": (I, [P, P, S, PP, P]),
         "flock_hip_sumcheck_round_msg": (I, [P, P, P, S, P]),
         "flock_hip_sumcheck_fold_msg": (I, [P, P, P, S, F128, P, P, P]),
+        "flock_hip_sumcheck_fold_msg_static": (I, [P, P, P, S, F128, P, P, P]),
         "flock_hip_build_eq": (I, [P, P, S, P]),
         "flock_hip_zc_round_pair": (I, [P, P, P, S, P, F128, P, P]),
         "flock_hip_zc_fold_round_pair": (I, [P, P, P, S, F128, P, P, P, F128, P, P]),

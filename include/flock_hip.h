@@ -167,6 +167,16 @@ int flock_hip_sumcheck_round_msg(flock_hip_ctx* ctx, const flock_f128* d_f,
 int flock_hip_sumcheck_fold_msg(flock_hip_ctx* ctx, const flock_f128* d_f,
                                 const flock_f128* d_b, size_t n, flock_f128 r, flock_f128* d_nf,
                                 flock_f128* d_nb, flock_f128 h_msg[2]);
+/* The same fold and message without any allocation: no context, no hipMalloc / hipFree, no
+ * host synchronisation. No counterpart in the reference; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops). The grid reduction's scratch
+ * is a fixed 16,400-byte device array per GPU, held until the process ends, so the size
+ * limit is only the kernel's own (n a power of two in [2, 2^40]). Enqueued on `stream` (a
+ * hipStream_t of the current device, NULL for the null stream); d_msg (2 device words) gets
+ * {u_0, u_2} when the stream reaches it. Calls that overlap on one GPU race on the scratch. */
+int flock_hip_sumcheck_fold_msg_static(void* stream, const flock_f128* d_f,
+                                       const flock_f128* d_b, size_t n, flock_f128 r,
+                                       flock_f128* d_nf, flock_f128* d_nb, flock_f128* d_msg);
 /* build_eq (zerocheck/univariate_skip.rs:31-47): d_out[x] = prod_i (bit_i(x) ? r_i : 1+r_i)
  * for x < 2^nv. */
 int flock_hip_build_eq(flock_hip_ctx* ctx, const flock_f128* h_r, size_t nv, flock_f128* d_out);
