@@ -169,11 +169,27 @@ const H* sqrt_basis() {
     }();
     return t.data();
 }
+// the same F2-linear map through 16 byte tables: T[k][v] = frob_inv(v x^(8k))
+const H* frob_inv_tables() {
+    static const std::vector<H> t = [] {
+        const H* sb = sqrt_basis();
+        std::vector<H> v(16 * 256);
+        for (int k = 0; k < 16; k++)
+            for (int n = 1; n < 256; n++) {
+                const int low = __builtin_ctz((unsigned)n);
+                v[k * 256 + n] = ha(v[k * 256 + (n & (n - 1))], sb[8 * k + low]);
+            }
+        return v;
+    }();
+    return t.data();
+}
 H frob_inv(H x) {
-    const H* t = sqrt_basis();
+    const H* t = frob_inv_tables();
     H acc = hz();
-    for (int b = 0; b < 128; b++)
-        if (hgetbit(x, b)) acc = ha(acc, t[b]);
+    for (int k = 0; k < 8; k++) {
+        acc = ha(acc, t[k * 256 + ((x.lo >> (8 * k)) & 255)]);
+        acc = ha(acc, t[(k + 8) * 256 + ((x.hi >> (8 * k)) & 255)]);
+    }
     return acc;
 }
 
@@ -706,11 +722,274 @@ __global__ void k_induced(const f128* lo, const f128* hi, uint32_t nq, uint32_t 
     }
 }
 // s_hat_v[b] = sum_k eq[k] z_vec[128 k + b] (pcs/ring_switch.rs:1278-1330)
-__global__ void k_shv_zvec(const f128* zv, const f128* eq, uint64_t n_tail, f128* out) {
-    const uint32_t b = threadIdx.x;
+// split over the k range: each block takes kShvK consecutive k for all 128 b and XORs its
+// partial sums into out (zeroed beforehand)
+constexpr uint32_t kShvK = 4;
+__global__ void __launch_bounds__(kThreads) k_shv_zvec(const f128* zv, const f128* eq,
+                                                       uint64_t n_tail, f128* out) {
+    const uint32_t b = threadIdx.x & 127, half = threadIdx.x >> 7;
     f128 s = f128_zero();
-    for (uint64_t k = 0; k < n_tail; k++) s = f128_add(s, f128_mul(eq[k], zv[128 * k + b]));
-    out[b] = s;
+    const uint64_t k0 = (uint64_t)blockIdx.x * kShvK;
+    for (uint64_t k = k0 + half; k < k0 + kShvK && k < n_tail; k += 2)
+        s = f128_add(s, f128_mul(eq[k], zv[128 * k + b]));
+    __shared__ f128 part[128];
+    if (half) part[b] = s;
+    __syncthreads();
+    if (!half) {
+        s = f128_add(s, part[b]);
+        uint32_t* o = reinterpret_cast<uint32_t*>(out + b);
+        if (s.x) atomicXor(o, s.x);
+        if (s.y) atomicXor(o + 1, s.y);
+        if (s.z) atomicXor(o + 2, s.z);
+        if (s.w) atomicXor(o + 3, s.w);
+    }
+}
+
+// ---- fixed multipliers through LDS Shoup tables ----
+
+// the block builds the Shoup tables of m[0..nt) at tab + i * SHOUP_ENTRIES (one row per thread)
+template <int NT>
+__device__ __forceinline__ void build_tables(f128* tab, const f128 (&m)[NT]) {
+    for (int r = threadIdx.x; r < 32 * NT; r += blockDim.x) {
+        const int t = r >> 5;
+        f128 v = m[0];
+#pragma unroll
+        for (int i = 1; i < NT; i++)
+            if (t == i) v = m[i];
+        shoup_build_row(tab + t * SHOUP_ENTRIES, v, r & 31);
+    }
+    __syncthreads();
+}
+// x r for r in F256 with the tables of (r0, r1, r0 + r1) at tab: p0 = x0 r0, p1 = x1 r1,
+// p2 = (x0 + x1)(r0 + r1); x r = (p0 + x^-1 p1, p2 + p0)
+__device__ __forceinline__ e256d emul_tab(const f128* tab, e256d x) {
+    const f128 p0 = shoup_mul(tab, x.c0), p1 = shoup_mul(tab + SHOUP_ENTRIES, x.c1);
+    const f128 p2 = shoup_mul(tab + 2 * SHOUP_ENTRIES, f128_add(x.c0, x.c1));
+    return e256d{f128_add(p0, mul_x_inv_d(p1)), f128_add(p2, p0)};
+}
+
+__global__ void __launch_bounds__(kThreads) k_scale_tab(f128* v, uint64_t N, f128 s) {
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    const f128 m[1] = {s};
+    build_tables<1>(tab, m);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        v[i] = shoup_mul(tab, v[i]);
+}
+// y += beta x
+__global__ void __launch_bounds__(kThreads) k_axpy_tab(f128* y, const f128* x, uint64_t N, f128 beta) {
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    const f128 m[1] = {beta};
+    build_tables<1>(tab, m);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        y[i] = f128_add(y[i], shoup_mul(tab, x[i]));
+}
+__global__ void __launch_bounds__(kThreads) k_glue_base_tab(e256d* b, const f128* e, uint64_t n, f128 beta) {
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    const f128 m[1] = {beta};
+    build_tables<1>(tab, m);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        b[i].c0 = f128_add(b[i].c0, shoup_mul(tab, e[i]));
+}
+__global__ void __launch_bounds__(kThreads) k_glue_presplit_tab(e256d* b, const f128* B, uint64_t nb,
+                                                                f128 beta) {
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    const f128 m[1] = {beta};
+    build_tables<1>(tab, m);
+    for (uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; x < nb;
+         x += (uint64_t)gridDim.x * blockDim.x) {
+        const f128 t = shoup_mul(tab, B[x]);
+        b[2 * x].c0 = f128_add(b[2 * x].c0, t);
+        b[2 * x + 1].c1 = f128_add(b[2 * x + 1].c1, t);
+    }
+}
+
+// ---- products of two varying values: per-lane comb tables, unreduced accumulation ----
+
+// acc ^= a b, unreduced (the lane's comb column is rebuilt for b)
+__device__ __forceinline__ void comb_acc(f256u& acc, f128* wl, f128 a, f128 b) {
+    const uint32_t bh = comb_build(wl, b);
+    comb_apply(acc, wl, bh, a);
+}
+// the three Karatsuba sums of an F256 product a b: S0 += a0 b0, S1 += a1 b1,
+// S2 += (a0 + a1)(b0 + b1)
+__device__ __forceinline__ void emul_acc(f256u (&S)[3], f128* wl, e256d a, e256d b) {
+    comb_acc(S[0], wl, a.c0, b.c0);
+    comb_acc(S[1], wl, a.c1, b.c1);
+    comb_acc(S[2], wl, f128_add(a.c0, a.c1), f128_add(b.c0, b.c1));
+}
+// sum of the products from the three sums: (S0 + x^-1 S1, S2 + S0), each reduced once
+__device__ __forceinline__ e256d emul_sum(const f256u (&S)[3]) {
+    const f128 p0 = f256u_reduce(S[0]), p1 = f256u_reduce(S[1]), p2 = f256u_reduce(S[2]);
+    return e256d{f128_add(p0, mul_x_inv_d(p1)), f128_add(p2, p0)};
+}
+__device__ __forceinline__ void f256u_zero3(f256u (&S)[3]) {
+    f256u_zero(S[0]);
+    f256u_zero(S[1]);
+    f256u_zero(S[2]);
+}
+
+// one fold at r over blocked pairs at distance d (n -> n/2), then, with `msg`, the next
+// round's message over blocked pairs at distance d2 of the folded tables: acc[0..16) ^= (u0, u2)
+// with u0 = sum g_lo h_lo, u2 = sum (g_lo + g_hi)(h_lo + h_hi) (extension.rs:145-224). BASE:
+// the inputs are F128 tables (the first fold, x -> (x_lo + r0 dx, r1 dx), dx = x_hi + x_lo).
+template <bool BASE>
+__device__ __forceinline__ e256d fold_at(const void* src, uint64_t o, uint64_t d, const f128* tab) {
+    const uint64_t lo = pair_lo(o, d), hi = lo + d;
+    if constexpr (BASE) {
+        const f128* v = static_cast<const f128*>(src);
+        const f128 dx = f128_add(v[hi], v[lo]);
+        return e256d{f128_add(v[lo], shoup_mul(tab, dx)), shoup_mul(tab + SHOUP_ENTRIES, dx)};
+    } else {
+        const e256d* v = static_cast<const e256d*>(src);
+        return eadd(v[lo], emul_tab(tab, eadd(v[hi], v[lo])));
+    }
+}
+template <bool BASE>
+__global__ void __launch_bounds__(kThreads) k_fold_msg(const void* f, const void* b, uint64_t n,
+                                                       uint64_t d, e256d r, uint64_t d2, int msg,
+                                                       e256d* nf, e256d* nb, uint32_t* acc) {
+    __shared__ f128 tab[3 * SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    const f128 m[3] = {r.c0, r.c1, f128_add(r.c0, r.c1)};
+    build_tables<3>(tab, m);
+    const uint64_t half = n / 2;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    if (!msg || half < 2) {
+        for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < half; o += step) {
+            nf[o] = fold_at<BASE>(f, o, d, tab);
+            nb[o] = fold_at<BASE>(b, o, d, tab);
+        }
+        return;
+    }
+    f128* wl = comb_lane(ws);
+    f256u U0[3], U2[3];
+    f256u_zero3(U0);
+    f256u_zero3(U2);
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < half / 2; o += step) {
+        const uint64_t p0 = pair_lo(o, d2), p1 = p0 + d2;
+        const e256d g0 = fold_at<BASE>(f, p0, d, tab), g1 = fold_at<BASE>(f, p1, d, tab);
+        const e256d h0 = fold_at<BASE>(b, p0, d, tab), h1 = fold_at<BASE>(b, p1, d, tab);
+        nf[p0] = g0;
+        nf[p1] = g1;
+        nb[p0] = h0;
+        nb[p1] = h1;
+        emul_acc(U0, wl, g0, h0);
+        emul_acc(U2, wl, eadd(g0, g1), eadd(h0, h1));
+    }
+    const e256d u0 = emul_sum(U0), u2 = emul_sum(U2);
+    uint32_t v[16] = {0};
+    put4(v, u0.c0);
+    put4(v + 4, u0.c1);
+    put4(v + 8, u2.c0);
+    put4(v + 12, u2.c1);
+    block_xor_atomic<16>(v, acc);
+}
+
+// F128 message over blocked pairs (u0 = sum f_lo b_lo, u2 = sum (f_lo + f_hi)(b_lo + b_hi))
+__global__ void __launch_bounds__(kThreads) k_msg_blocked_comb(const f128* f, const f128* b, uint64_t n,
+                                                               uint64_t d, uint32_t* acc) {
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    f256u u0, u2;
+    f256u_zero(u0);
+    f256u_zero(u2);
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t lo = pair_lo(o, d), hi = lo + d;
+        const f128 f0 = f[lo], b0 = b[lo];
+        comb_acc(u0, wl, f0, b0);
+        comb_acc(u2, wl, f128_add(f0, f[hi]), f128_add(b0, b[hi]));
+    }
+    uint32_t v[8] = {0};
+    put4(v, f256u_reduce(u0));
+    put4(v + 4, f256u_reduce(u2));
+    block_xor_atomic<8>(v, acc);
+}
+// <a, b>
+__global__ void __launch_bounds__(kThreads) k_dot_comb(const f128* a, const f128* b, uint64_t N,
+                                                       uint32_t* acc) {
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    f256u s;
+    f256u_zero(s);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        comb_acc(s, wl, a[i], b[i]);
+    uint32_t v[4] = {0};
+    put4(v, f256u_reduce(s));
+    block_xor_atomic<4>(v, acc);
+}
+
+// Induced basis (ligerito.rs:2676-2770 with empty rows) as out = sum_i lo_i (x) hi_i:
+// lo_i[j] = aw_i prod_{k < h, bit k of j} s_ik over the low h coordinates, hi_i[j] =
+// prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i)
+__global__ void __launch_bounds__(kThreads) k_induced_tables(const f128* s, const f128* aw, uint32_t dims,
+                                                             uint32_t h, f128* lo, f128* hi) {
+    const uint32_t i = blockIdx.x;
+    const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
+    const f128* si = s + (size_t)i * dims;
+    for (uint64_t j = threadIdx.x; j < nl + nh; j += blockDim.x) {
+        if (j < nl) {
+            f128 v = aw[i];
+            for (uint32_t k = 0; k < h; k++)
+                if ((j >> k) & 1) v = f128_mul(v, si[k]);
+            lo[i * nl + j] = v;
+        } else {
+            const uint64_t jj = j - nl;
+            f128 v = f128_one();
+            for (uint32_t k = h; k < dims; k++)
+                if ((jj >> (k - h)) & 1) v = f128_mul(v, si[k]);
+            hi[i * nh + jj] = v;
+        }
+    }
+}
+// out[jl + 2^h jh] ^= sum_{i in this block's chunk} lo_i[jl] hi_i[jh]: kIndTL rows jl per block
+// (their Shoup tables in LDS, rebuilt per i), one column jh per thread; with several chunks the
+// blocks XOR into out atomically (out zeroed beforehand)
+constexpr int kIndTL = 4;
+__global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const f128* hi, uint32_t nq,
+                                                           uint32_t h, uint32_t dims, uint32_t chunk,
+                                                           int atomic, f128* out) {
+    __shared__ f128 tab[kIndTL * SHOUP_ENTRIES];
+    const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
+    const uint64_t jl0 = (uint64_t)blockIdx.x * kIndTL;
+    const uint64_t jh = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    const uint32_t i0 = blockIdx.z * chunk, i1 = min(nq, i0 + chunk);
+    f128 acc[kIndTL];
+#pragma unroll
+    for (int t = 0; t < kIndTL; t++) acc[t] = f128_zero();
+    for (uint32_t i = i0; i < i1; i++) {
+        __syncthreads();  // the previous tables are consumed
+        if (threadIdx.x < 32 * kIndTL) {
+            const int t = threadIdx.x >> 5;
+            const f128 m = jl0 + t < nl ? lo[i * nl + jl0 + t] : f128_zero();
+            shoup_build_row(tab + t * SHOUP_ENTRIES, m, threadIdx.x & 31);
+        }
+        __syncthreads();
+        if (jh < nh) {
+            const f128 x = hi[i * nh + jh];
+#pragma unroll
+            for (int t = 0; t < kIndTL; t++) acc[t] = f128_add(acc[t], shoup_mul(tab + t * SHOUP_ENTRIES, x));
+        }
+    }
+    if (jh >= nh) return;
+#pragma unroll
+    for (int t = 0; t < kIndTL; t++) {
+        if (jl0 + t >= nl) break;
+        f128* o = out + (jl0 + t) + nl * jh;
+        if (atomic) {
+            uint32_t* w = reinterpret_cast<uint32_t*>(o);
+            if (acc[t].x) atomicXor(w, acc[t].x);
+            if (acc[t].y) atomicXor(w + 1, acc[t].y);
+            if (acc[t].z) atomicXor(w + 2, acc[t].z);
+            if (acc[t].w) atomicXor(w + 3, acc[t].w);
+        } else {
+            *o = acc[t];
+        }
+    }
 }
 
 }  // namespace
@@ -759,6 +1038,23 @@ struct flock_open_ws {
 namespace {
 
 using flock::kThreads;
+
+// kernels whose dynamic LDS is the per-lane comb workspace
+hipError_t comb_attrs() {
+    static const hipError_t e = [] {
+        const void* fns[] = {reinterpret_cast<const void*>(&flock::k_fold_msg<true>),
+                             reinterpret_cast<const void*>(&flock::k_fold_msg<false>),
+                             reinterpret_cast<const void*>(&flock::k_msg_blocked_comb),
+                             reinterpret_cast<const void*>(&flock::k_dot_comb)};
+        for (const void* f : fns)
+            if (hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   kCombWsBytes);
+                r != hipSuccess)
+                return r;
+        return hipSuccess;
+    }();
+    return e;
+}
 
 struct Ctx {
     flock_hip_ctx* c;
@@ -1171,29 +1467,39 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     const uint32_t nq = (uint32_t)qs.size();
     const int h = dims / 2;
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
-    std::vector<H> lo((size_t)nq * nl), hi((size_t)nq * nh), sk(dims > 0 ? dims : 1);
-    for (uint32_t i = 0; i < nq; i++) {
-        if (dims > 0) {
-            sk[0] = H{qs[i], 0};
-            for (int k = 1; k < dims; k++) sk[k] = ha(hm(sk[k - 1], sk[k - 1]), hm(skv[k - 1], sk[k - 1]));
-            for (int k = 0; k < dims; k++) sk[k] = hm(sk[k], inv[k]);
-        }
-        H* L = &lo[(size_t)i * nl];
-        L[0] = aw[i];
-        for (int k = 0; k < h; k++)
-            for (uint64_t j = 0; j < (1ull << k); j++) L[j + (1ull << k)] = hm(sk[k], L[j]);
-        H* Hh = &hi[(size_t)i * nh];
-        Hh[0] = h1();
-        for (int k = h; k < dims; k++)
-            for (uint64_t j = 0; j < (1ull << (k - h)); j++) Hh[j + (1ull << (k - h))] = hm(sk[k], Hh[j]);
+    // the normalized subspace polynomials at each query, s_ik = s_k(q_i) / s_k(v_k)
+    std::vector<H> sk((size_t)nq * (dims > 0 ? dims : 1));
+    for (uint32_t i = 0; i < nq && dims > 0; i++) {
+        H* si = &sk[(size_t)i * dims];
+        si[0] = H{qs[i], 0};
+        for (int k = 1; k < dims; k++) si[k] = ha(hm(si[k - 1], si[k - 1]), hm(skv[k - 1], si[k - 1]));
+        for (int k = 0; k < dims; k++) si[k] = hm(si[k], inv[k]);
     }
-    flock_f128* d_lo = X.buf<flock_f128>(44, lo.size());
-    flock_f128* d_hi = X.buf<flock_f128>(45, hi.size());
+    flock_f128* d_s = X.buf<flock_f128>(48, sk.size());
+    flock_f128* d_aw = X.buf<flock_f128>(49, nq);
+    flock_f128* d_lo = X.buf<flock_f128>(44, (size_t)nq * nl);
+    flock_f128* d_hi = X.buf<flock_f128>(45, (size_t)nq * nh);
     if (X.err) return false;
-    if (!X.h2d(d_lo, lo.data(), 16 * lo.size()) || !X.h2d(d_hi, hi.data(), 16 * hi.size())) return false;
-    hipLaunchKernelGGL(flock::k_induced, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st,
-                       reinterpret_cast<const uint4*>(d_lo), reinterpret_cast<const uint4*>(d_hi), nq,
-                       (uint32_t)h, (uint32_t)dims, reinterpret_cast<uint4*>(out));
+    if (!X.h2d(d_s, sk.data(), 16 * sk.size()) || !X.h2d(d_aw, aw.data(), 16 * nq)) return false;
+    hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(d_s),
+                       reinterpret_cast<const uint4*>(d_aw), (uint32_t)dims, (uint32_t)h, reinterpret_cast<uint4*>(d_lo),
+                       reinterpret_cast<uint4*>(d_hi));
+    if (!X.chk("induced tables")) return false;
+    // split the query sum over blocks until the grid holds about 2048 workgroups
+    const uint32_t gx = (uint32_t)((nl + flock::kIndTL - 1) / flock::kIndTL);
+    const uint32_t gy = (uint32_t)((nh + kThreads - 1) / kThreads);
+    uint32_t gz = std::max<uint32_t>(1, std::min<uint32_t>(nq, 2048 / std::max<uint32_t>(1, gx * gy)));
+    const uint32_t chunk = (nq + gz - 1) / gz;
+    gz = (nq + chunk - 1) / chunk;
+    if (gz > 1) {
+        const uint64_t words = 4ull << dims;
+        hipLaunchKernelGGL(flock::k_zero32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, X.st,
+                           reinterpret_cast<uint32_t*>(out), (uint32_t)words);
+        if (!X.chk("zero")) return false;
+    }
+    hipLaunchKernelGGL(flock::k_induced_gemm, dim3(gx, gy, gz), dim3(kThreads), 0, X.st,
+                       reinterpret_cast<const uint4*>(d_lo), reinterpret_cast<const uint4*>(d_hi), nq, (uint32_t)h,
+                       (uint32_t)dims, chunk, gz > 1 ? 1 : 0, reinterpret_cast<uint4*>(out));
     if (!X.chk("induced basis")) return false;
     // the host copies must outlive the async uploads
     return X.hip(hipStreamSynchronize(X.st), "sync");
@@ -1246,11 +1552,28 @@ struct Ladder {
         return true;
     }
     bool fold(uint64_t d, E r) {
-        hipLaunchKernelGGL(flock::k_fold_e, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st, f, b, n, d,
-                           E4(r), f2, b2);
+        hipLaunchKernelGGL(flock::k_fold_msg<false>, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st,
+                           static_cast<const void*>(f), static_cast<const void*>(b), n, d, E4(r),
+                           (uint64_t)1, 0, f2, b2, X.ws->acc);
         if (!X.chk("fold")) return false;
         swap();
         n /= 2;
+        return true;
+    }
+    // the fold and the next round's message (over pairs at d, or LSB pairs once a single block
+    // of d is left) in one pass; BASE: the inputs are the F128 tables (fb, bb) of length n
+    template <bool BASE>
+    bool fold_msg(uint64_t d, E r, E* u0, E* u2, const void* fb = nullptr, const void* bb = nullptr) {
+        const uint64_t nn = n / 2, d2 = (d == 1 || nn == d) ? 1 : d;
+        if (!X.zero_acc(16)) return false;
+        hipLaunchKernelGGL(flock::k_fold_msg<BASE>, dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes,
+                           X.st, BASE ? fb : static_cast<const void*>(f), BASE ? bb : static_cast<const void*>(b),
+                           n, d, E4(r), d2, 1, BASE ? f : f2, BASE ? b : b2, X.ws->acc);
+        if (!X.chk("fold msg") || !X.read_acc(16)) return false;
+        if (!BASE) swap();
+        n = nn;
+        *u0 = E{X.acc_f(0), X.acc_f(1)};
+        *u2 = E{X.acc_f(2), X.acc_f(3)};
         return true;
     }
     bool sw(E* u0, E* u2) {  // code switch + its LSB message
@@ -1288,7 +1611,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     flock_f128* ez_ = X.buf<flock_f128>(11, N);
     if (X.err) return false;
     if (!X.eq_dev(rho.data(), (size_t)log_n, b0)) return false;
-    hipLaunchKernelGGL(flock::k_scale, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0), N,
+    hipLaunchKernelGGL(flock::k_scale_tab, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0), N,
                        U4(gamma));
     if (!X.chk("scale")) return false;
     label(fs, "flock-ligerito-basis-f256-split-v0");
@@ -1302,7 +1625,8 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     for (int o = 0; o < C->lv[0].ood; o++) {  // L0 OOD samples
         const std::vector<H> z = sample_vec(fs, (size_t)log_n);
         if (!X.eq_dev(z.data(), (size_t)log_n, ez_) || !X.zero_acc(4)) return false;
-        hipLaunchKernelGGL(flock::k_dot, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(q),
+        hipLaunchKernelGGL(flock::k_dot_comb, dim3(X.grid(N)), dim3(kThreads), kCombWsBytes, X.st,
+                           reinterpret_cast<const uint4*>(q),
                            reinterpret_cast<const uint4*>(ez_), N, X.ws->acc);
         if (!X.chk("ood dot") || !X.read_acc(4)) return false;
         const H y = X.acc_f(0);
@@ -1312,13 +1636,13 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         const H beta = grind_sample(fs, C->lv[0].claim, &nonce);
         claim_n.push_back(nonce);
         target = ha(target, hm(beta, y));
-        hipLaunchKernelGGL(flock::k_axpy, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0),
+        hipLaunchKernelGGL(flock::k_axpy_tab, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0),
                            reinterpret_cast<const uint4*>(ez_), N, U4(beta));
         if (!X.chk("axpy")) return false;
     }
     // first message over blocked pairs (F128)
     if (!X.zero_acc(8)) return false;
-    hipLaunchKernelGGL(flock::k_msg_blocked, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
+    hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
                        reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D, X.ws->acc);
     if (!X.chk("first msg") || !X.read_acc(8)) return false;
     Ladder Lg{X, fs, X.buf<e256d>(12, N / 2), X.buf<e256d>(13, N / 2), X.buf<e256d>(14, N / 2),
@@ -1331,18 +1655,12 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         const E r = sample_e(fs);
         lane_ch.push_back(r);
         E u0, u2;
-        if (j == 0) {
-            hipLaunchKernelGGL(flock::k_fold_first, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
-                               reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D,
-                               E4(r), Lg.f, Lg.b);
-            if (!X.chk("fold first")) return false;
-            Lg.n = N / 2;
-        } else if (!Lg.fold(D, r)) {
-            return false;
-        }
-        if (j + 1 < ik) {
-            if (!Lg.msg(D, &u0, &u2)) return false;
-        } else if (!Lg.sw(&u0, &u2)) {
+        if (j == 0) {  // initial_k >= 2: a message always follows the base fold
+            Lg.n = N;
+            if (!Lg.fold_msg<true>(D, r, &u0, &u2, q, b0)) return false;
+        } else if (j + 1 < ik) {
+            if (!Lg.fold_msg<false>(D, r, &u0, &u2)) return false;
+        } else if (!Lg.fold(D, r) || !Lg.sw(&u0, &u2)) {
             return false;
         }
         Lg.push(u0, u2);
@@ -1391,7 +1709,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
             uint64_t nonce;
             const H beta = grind_sample(fs, C->lv[level].claim, &nonce);
             claim_n.push_back(nonce);
-            hipLaunchKernelGGL(flock::k_glue_base, dim3(X.grid(Lg.n)), dim3(kThreads), 0, X.st, Lg.b,
+            hipLaunchKernelGGL(flock::k_glue_base_tab, dim3(X.grid(Lg.n)), dim3(kThreads), 0, X.st, Lg.b,
                                reinterpret_cast<const uint4*>(ezl), Lg.n, U4(beta));
             if (!X.chk("glue")) return false;
         }
@@ -1407,7 +1725,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         uint64_t nonce;
         const H beta = grind_sample(fs, C->lv[next_level].claim, &nonce);
         claim_n.push_back(nonce);
-        hipLaunchKernelGGL(flock::k_glue_presplit, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.b,
+        hipLaunchKernelGGL(flock::k_glue_presplit_tab, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.b,
                            reinterpret_cast<const uint4*>(basis), 1ull << dims, U4(beta));
         return X.chk("glue presplit");
     };
@@ -1440,11 +1758,10 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
             const E r = sample_e(fs);
             lch.push_back(r);
             const bool switching = (j + 1 == k) && (i + 1 != R);
-            if (!Lg.fold(1, r)) return false;
             E u0, u2;
             if (switching) {
-                if (!Lg.sw(&u0, &u2)) return false;
-            } else if (!Lg.msg(1, &u0, &u2)) {
+                if (!Lg.fold(1, r) || !Lg.sw(&u0, &u2)) return false;
+            } else if (!Lg.fold_msg<false>(1, r, &u0, &u2)) {
                 return false;
             }
             Lg.push(u0, u2);
@@ -1575,8 +1892,11 @@ int flock_hip_s_hat_v_from_z_vec(flock_hip_ctx* ctx, flock_open_ws* ws, const fl
     flock_f128* out = X.buf<flock_f128>(47, 128);
     if (X.err) return flock::set_error(X.err, X.what.c_str());
     if (!X.eq_dev(h_tail, tail_len, eq)) return flock::set_error(X.err, X.what.c_str());
-    hipLaunchKernelGGL(flock::k_shv_zvec, dim3(1), dim3(128), 0, X.st, reinterpret_cast<const uint4*>(d_z_vec),
-                       reinterpret_cast<const uint4*>(eq), 1ull << tail_len, reinterpret_cast<uint4*>(out));
+    hipLaunchKernelGGL(flock::k_zero32, dim3(2), dim3(256), 0, X.st, reinterpret_cast<uint32_t*>(out), 512u);
+    const uint64_t n_tail = 1ull << tail_len;
+    hipLaunchKernelGGL(flock::k_shv_zvec, dim3((uint32_t)((n_tail + flock::kShvK - 1) / flock::kShvK)),
+                       dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(d_z_vec),
+                       reinterpret_cast<const uint4*>(eq), n_tail, reinterpret_cast<uint4*>(out));
     if (!X.chk("s_hat_v") || !X.d2h(h_out, out, 16 * 128)) return flock::set_error(X.err, X.what.c_str());
     return 0;
 }
@@ -1593,6 +1913,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     *out_bytes = nullptr;
     *out_len = 0;
     Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
+    if (!X.hip(comb_attrs(), "kernel attributes")) return flock::set_error(X.err, X.what.c_str());
     const int dense_log = (int)P->m - 7;
     const uint64_t N = 1ull << dense_log;
     const Jag J = jag_make(h_heights, n_heights, (int)n_log, dense_log);
