@@ -497,6 +497,62 @@ __global__ void __launch_bounds__(kThreads) k_jag_scalar(f128* out, uint64_t N, 
     }
 }
 
+// Tiled forms of the two jagged tables: a tile is a run of one column's rows (col, d0, len), so
+// ec_c[col] is fixed inside it and x -> Phi_c(x ec_c[col]) (weight form) or x -> x ec_c[col]
+// (scalar form) is one F2-linear map per claim, held as 32 nibble tables in LDS. Entries past
+// the jagged area are written by k_zero_f128.
+struct JagTile {
+    uint32_t col;
+    uint32_t len;
+    uint64_t d0;
+};
+constexpr uint32_t kJagTile = 8192;
+__global__ void k_zero_f128(f128* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = f128_zero();
+}
+// WEIGHT: W[d] = sum_c Phi_c(er_c[row] ec_c[col]); else out[d] = sum_c er_c[row] ec_c[col]
+template <bool WEIGHT>
+__global__ void __launch_bounds__(kThreads) k_jag_tiles(f128* out, const JagTile* tiles,
+                                                        const uint64_t* ps, JagClaims jc) {
+    __shared__ f128 T[kMaxClaims * 512];
+    __shared__ f128 img[kMaxClaims * 128];
+    const JagTile tl = tiles[blockIdx.x];
+    if constexpr (WEIGHT) {
+        // basis images Phi_c(x^b ec) (Phi_c's tables staged in T first), then the nibble
+        // tables of the composed map over them
+        for (uint32_t i = threadIdx.x; i < jc.n * 512; i += blockDim.x) T[i] = jc.tab[i / 512][i % 512];
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < jc.n * 128; i += blockDim.x) {
+            const uint32_t c = i / 128, b = i % 128;
+            img[i] = lin_apply_d(T + c * 512, f128_shl_mod(jc.ec[c][tl.col], (int)b));
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < jc.n * 512; i += blockDim.x) {
+            const uint32_t c = i / 512, k = (i % 512) / 16, nib = i % 16;
+            f128 v = f128_zero();
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((nib >> j) & 1) v = f128_add(v, img[c * 128 + 4 * k + j]);
+            T[i] = v;
+        }
+    } else {
+        for (uint32_t r = threadIdx.x; r < 32 * jc.n; r += blockDim.x)
+            shoup_build_row(T + (r / 32) * SHOUP_ENTRIES, jc.ec[r / 32][tl.col], (int)(r % 32));
+    }
+    __syncthreads();
+    const uint64_t row0 = tl.d0 - ps[tl.col];
+    for (uint32_t i = threadIdx.x; i < tl.len; i += blockDim.x) {
+        f128 s = f128_zero();
+        for (uint32_t c = 0; c < jc.n; c++) {
+            const f128 e = jc.er[c][row0 + i];
+            s = f128_add(s, WEIGHT ? lin_apply_d(T + c * 512, e) : shoup_mul(T + c * SHOUP_ENTRIES, e));
+        }
+        out[tl.d0 + i] = s;
+    }
+}
+
 // v[i] = L(v[i]) for an F2-linear map L given by its nibble tables (512 words)
 __global__ void __launch_bounds__(kThreads) k_lin_apply(f128* v, uint64_t N, const f128* tab) {
     __shared__ f128 T[512];
@@ -534,25 +590,34 @@ __global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
         eq4[l][3] = f128_mul(r, x);
     }
     __syncthreads();
+    // eight lanes per run: lane (s, q) forms the q-th product of state s; the pair XORs its two
+    // products and every lane of the group gathers the four new states (all lanes run every
+    // layer, so the shuffles stay converged)
     uint32_t v[4] = {0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < a.n_runs; b += blockDim.x) {
+    const int lane = (int)(threadIdx.x & 63), gbase = lane & ~7, st = (lane & 7) >> 1, q = lane & 1;
+    for (uint32_t base = 0; base < a.n_runs; base += blockDim.x / 8) {
+        const uint32_t b = base + threadIdx.x / 8;
+        const bool live = b < a.n_runs;
         f128 S[4] = {f128_zero(), f128_zero(), f128_one(), f128_zero()};  // e_SUCCESS = state 2
-        const uint64_t tc = a.bc[b], tn = a.bn[b];
+        const uint64_t tc = live ? a.bc[b] : 0, tn = live ? a.bn[b] : 0;
         for (int l = (int)a.m; l >= 0; l--) {
             const int cd = (int)(((tc >> l) & 1) + 2 * ((tn >> l) & 1));
-            f128 T2[4];
+            const int ei = a.sparse[cd][st][q][0], si = a.sparse[cd][st][q][1];
+            const f128 sv = si == 0 ? S[0] : si == 1 ? S[1] : si == 2 ? S[2] : S[3];
+            f128 p = f128_mul(eq4[l][ei], sv);
+            p.x ^= __shfl_xor(p.x, 1, 64);
+            p.y ^= __shfl_xor(p.y, 1, 64);
+            p.z ^= __shfl_xor(p.z, 1, 64);
+            p.w ^= __shfl_xor(p.w, 1, 64);
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                f128 acc = f128_zero();
-#pragma unroll
-                for (int q = 0; q < 2; q++)
-                    acc = f128_add(acc, f128_mul(eq4[l][a.sparse[cd][s][q][0]], S[a.sparse[cd][s][q][1]]));
-                T2[s] = acc;
+            for (int t = 0; t < 4; t++) {
+                const int src = gbase + 2 * t;
+                S[t] = make_uint4(__shfl(p.x, src, 64), __shfl(p.y, src, 64), __shfl(p.z, src, 64),
+                                  __shfl(p.w, src, 64));
             }
-#pragma unroll
-            for (int s = 0; s < 4; s++) S[s] = T2[s];
         }
-        put4(v, f128_mul(a.w[(size_t)claim * a.n_runs + b], S[0]));  // read out e_INITIAL
+        if (live && (lane & 7) == 0)
+            put4(v, f128_mul(a.w[(size_t)claim * a.n_runs + b], S[0]));  // read out e_INITIAL
     }
     block_xor_atomic<4>(v, reinterpret_cast<uint32_t*>(out + blockIdx.x));
 }
@@ -1988,9 +2053,34 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     if (!X.h2d(d_ps, J.ps.data(), 8 * J.ps.size()) || !X.h2d(d_ec, ec_all.data(), 16 * ec_all.size()) ||
         !X.h2d(d_tab, tab_all.data(), 16 * tab_all.size()) || !X.zero_acc(8))
         return flock::set_error(X.err, X.what.c_str());
-    hipLaunchKernelGGL(flock::k_jag_weight, dim3(X.grid(N / 2)), dim3(kThreads), 0, X.st,
-                       reinterpret_cast<const uint4*>(d_q), reinterpret_cast<uint4*>(W), N, area, d_ps, n_cols,
-                       jc, X.ws->acc);
+    // tiles of at most kJagTile rows of one column; W past the area is zero
+    std::vector<flock::JagTile> tiles;
+    for (uint32_t c = 0; c < n_cols; c++)
+        for (uint64_t off = 0; off < J.ps[c + 1] - J.ps[c]; off += flock::kJagTile)
+            tiles.push_back({c, (uint32_t)std::min<uint64_t>(flock::kJagTile, J.ps[c + 1] - J.ps[c] - off),
+                             J.ps[c] + off});
+    flock::JagTile* d_tiles = X.buf<flock::JagTile>(50, tiles.size() + 1);
+    if (X.err || (!tiles.empty() && !X.h2d(d_tiles, tiles.data(), sizeof(flock::JagTile) * tiles.size())))
+        return flock::set_error(X.err, X.what.c_str());
+    auto jag_table = [&](bool weight, flock_f128* dst) -> bool {
+        if (area < N)
+            hipLaunchKernelGGL(flock::k_zero_f128, dim3(X.grid(N - area)), dim3(kThreads), 0, X.st,
+                               reinterpret_cast<uint4*>(dst + area), N - area);
+        if (!tiles.empty()) {
+            if (weight)
+                hipLaunchKernelGGL(flock::k_jag_tiles<true>, dim3((uint32_t)tiles.size()), dim3(kThreads), 0, X.st,
+                                   reinterpret_cast<uint4*>(dst), d_tiles, d_ps, jc);
+            else
+                hipLaunchKernelGGL(flock::k_jag_tiles<false>, dim3((uint32_t)tiles.size()), dim3(kThreads), 0,
+                                   X.st, reinterpret_cast<uint4*>(dst), d_tiles, d_ps, jc);
+        }
+        return X.chk("jagged table");
+    };
+    // W and the round-0 message of <q, W> over LSB pairs
+    if (!jag_table(true, W)) return flock::set_error(X.err, X.what.c_str());
+    hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
+                       reinterpret_cast<const uint4*>(d_q), reinterpret_cast<const uint4*>(W), N, (uint64_t)1,
+                       X.ws->acc);
     if (!X.chk("jagged weight") || !X.read_acc(8)) return flock::set_error(X.err, X.what.c_str());
     ws->stamp(0);
     // the merged sumcheck <q, W> = target (pcs.rs:1640-1700), merged_round_bits = 2
@@ -2116,9 +2206,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
             for (size_t c = 0; c < n_claims && ok; c++)
                 for (uint32_t y = 0; y < n_cols; y++) ecs[c * n_cols + y] = hm(gpow[128 * c], ec_all[c * n_cols + y]);
             if (ok && X.h2d(d_ec, ecs.data(), 16 * ecs.size())) {
-                hipLaunchKernelGGL(flock::k_jag_scalar, dim3(X.grid(N)), dim3(kThreads), 0, X.st,
-                                   reinterpret_cast<uint4*>(W), N, area, d_ps, n_cols, jc);
-                ok = X.chk("jagged scalar");
+                ok = jag_table(false, W);
             } else {
                 ok = false;
             }
