@@ -579,6 +579,7 @@ struct BpArgs {
 };
 __global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
     const uint32_t claim = blockIdx.x / 128, j = blockIdx.x % 128;
+    // blockIdx.y picks this block's chunk of blockDim / 8 runs
     __shared__ f128 eq4[64][4];
     for (uint32_t l = threadIdx.x; l <= a.m; l += blockDim.x) {
         const f128 r = l < a.n ? a.zrow[claim * a.n + l] : f128_zero();
@@ -595,8 +596,8 @@ __global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
     // layer, so the shuffles stay converged)
     uint32_t v[4] = {0, 0, 0, 0};
     const int lane = (int)(threadIdx.x & 63), gbase = lane & ~7, st = (lane & 7) >> 1, q = lane & 1;
-    for (uint32_t base = 0; base < a.n_runs; base += blockDim.x / 8) {
-        const uint32_t b = base + threadIdx.x / 8;
+    {
+        const uint32_t b = blockIdx.y * (blockDim.x / 8) + threadIdx.x / 8;
         const bool live = b < a.n_runs;
         f128 S[4] = {f128_zero(), f128_zero(), f128_one(), f128_zero()};  // e_SUCCESS = state 2
         const uint64_t tc = live ? a.bc[b] : 0, tn = live ? a.bn[b] : 0;
@@ -1570,20 +1571,6 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     return X.hip(hipStreamSynchronize(X.st), "sync");
 }
 
-// induce_enforced_sum (extension.rs:2893-2916)
-E enforced_sum(const std::vector<H>& rows, size_t nq, size_t row_len, const std::vector<E>& lane_ch,
-               const std::vector<H>& alpha) {
-    const std::vector<E> lw = heq256(lane_ch.data(), lane_ch.size());
-    const std::vector<H> rw = heq(alpha.data(), alpha.size());
-    E acc = ez();
-    for (size_t i = 0; i < nq; i++) {
-        E d = ez();
-        for (size_t l = 0; l < row_len; l++) d = ea(d, emb(lw[l], rows[i * row_len + l]));
-        acc = ea(acc, emb(d, rw[i]));
-    }
-    return acc;
-}
-
 struct Ladder {
     Ctx& X;
     flock_fs* fs;
@@ -1715,10 +1702,8 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     if (X.err) return false;
     Lg.push(E{X.acc_f(0), hz()}, E{X.acc_f(1), hz()});
     // the initial_k lane folds; the last one ends in the code switch
-    std::vector<E> lane_ch;
     for (int j = 0; j < ik; j++) {
         const E r = sample_e(fs);
-        lane_ch.push_back(r);
         E u0, u2;
         if (j == 0) {  // initial_k >= 2: a message always follows the base fold
             Lg.n = N;
@@ -1811,17 +1796,14 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         queries(0, bl0, &qs, &alpha);
         if (!open_level(X, cw0, t, tree0, bl0, qs, c0, l0p, &rows)) return false;
         if (!induced_basis(X, n1, qs, alpha, basis)) return false;
-        (void)enforced_sum(rows, qs.size(), t, lane_ch, alpha);
         if (!presplit(n1, 1)) return false;
     }
     Bin recp, finp;
     size_t n_rec = 0;
     for (int i = 0; i < R; i++) {
         const int k = C->lv[i + 1].lanes;
-        std::vector<E> lch;
         for (int j = 0; j < k; j++) {
             const E r = sample_e(fs);
-            lch.push_back(r);
             const bool switching = (j + 1 == k) && (i + 1 != R);
             E u0, u2;
             if (switching) {
@@ -1874,7 +1856,6 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
                 return false;
             n_rec++;
             if (!induced_basis(X, ext_dim, qs, alpha, basis)) return false;
-            (void)enforced_sum(rows, qs.size(), prev.lanes, lch, alpha);
             if (!presplit(ext_dim, next_level)) return false;
         }
         prev = next;
@@ -2084,15 +2065,13 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     if (!X.chk("jagged weight") || !X.read_acc(8)) return flock::set_error(X.err, X.what.c_str());
     ws->stamp(0);
     // the merged sumcheck <q, W> = target (pcs.rs:1640-1700), merged_round_bits = 2
-    if (!X.hip(hipMemcpyAsync(A, d_q, 16 * N, hipMemcpyDeviceToDevice, X.st), "copy q"))
-        return flock::set_error(X.err, X.what.c_str());
     std::vector<H> mrounds, rho;
     std::vector<uint64_t> mnon;
     H q_eval = hz();
     {
         // round 0's message is the fused prime; later rounds come from the fold kernels
         flock_f128 m2[2] = {X.acc_f(0), X.acc_f(1)};
-        const flock_f128* ca = A;
+        const flock_f128* ca = d_q;  // the first fold reads q in place
         const flock_f128* cb = W;
         uint64_t cur = N;
         H claim = target;
@@ -2166,7 +2145,8 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
             ba.n = n_log;
             ba.m = (uint32_t)m;
             std::memcpy(ba.sparse, sparse_tr().t, sizeof(ba.sparse));
-            hipLaunchKernelGGL(flock::k_bp_values, dim3(128 * (uint32_t)n_claims), dim3(kThreads), 0, X.st, ba,
+            hipLaunchKernelGGL(flock::k_bp_values, dim3(128 * (uint32_t)n_claims, (uint32_t)((nr + 31) / 32)),
+                               dim3(kThreads), 0, X.st, ba,
                                reinterpret_cast<uint4*>(d_vals));
             ok = X.chk("bp values") && X.d2h(vals.data(), d_vals, 16 * vals.size());
         }
