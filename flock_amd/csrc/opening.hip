@@ -714,19 +714,35 @@ __global__ void k_base_table(const e256d* f, uint64_t n, f128* out) {
 // introduce an F128 basis e (OOD eq table): y = sum f.c0 e; message over LSB pairs of (f, e)
 __global__ void __launch_bounds__(kThreads) k_intro_base(const e256d* f, const f128* e, uint64_t n,
                                                          uint32_t* acc /* 4 y + 16 msg */) {
-    uint32_t v[20];
-    for (int i = 0; i < 20; i++) v[i] = 0;
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    // y = sum f.c0 e; u0 = sum f0 e0 (per component), u2 = sum (f0 + f1)(e0 + e1)
+    f256u Y, A0, A1, B0, B1;
+    f256u_zero(Y);
+    f256u_zero(A0);
+    f256u_zero(A1);
+    f256u_zero(B0);
+    f256u_zero(B1);
     for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
          o += (uint64_t)gridDim.x * blockDim.x) {
         const e256d f0 = f[2 * o], f1 = f[2 * o + 1];
-        const f128 e0 = e[2 * o], e1 = e[2 * o + 1];
-        put4(v, f128_add(f128_mul(f0.c0, e0), f128_mul(f1.c0, e1)));
-        const e256d u0 = emulb(f0, e0), u2 = emulb(eadd(f0, f1), f128_add(e0, e1));
-        put4(v + 4, u0.c0);
-        put4(v + 8, u0.c1);
-        put4(v + 12, u2.c0);
-        put4(v + 16, u2.c1);
+        const f128 e0 = e[2 * o], e1 = e[2 * o + 1], es = f128_add(e0, e1);
+        uint32_t bh = comb_build(wl, e0);
+        comb_apply(Y, wl, bh, f0.c0);
+        comb_apply(A0, wl, bh, f0.c0);
+        comb_apply(A1, wl, bh, f0.c1);
+        bh = comb_build(wl, e1);
+        comb_apply(Y, wl, bh, f1.c0);
+        bh = comb_build(wl, es);
+        comb_apply(B0, wl, bh, f128_add(f0.c0, f1.c0));
+        comb_apply(B1, wl, bh, f128_add(f0.c1, f1.c1));
     }
+    uint32_t v[20] = {0};
+    put4(v, f256u_reduce(Y));
+    put4(v + 4, f256u_reduce(A0));
+    put4(v + 8, f256u_reduce(A1));
+    put4(v + 12, f256u_reduce(B0));
+    put4(v + 16, f256u_reduce(B1));
     block_xor_atomic<20>(v, acc);
 }
 __global__ void k_glue_base(e256d* b, const f128* e, uint64_t n, f128 beta) {
@@ -913,9 +929,10 @@ __device__ __forceinline__ e256d fold_at(const void* src, uint64_t o, uint64_t d
         return eadd(v[lo], emul_tab(tab, eadd(v[hi], v[lo])));
     }
 }
-template <bool BASE>
+// MSG = false: the fold alone (a separate kernel, so its register and LDS footprint stays small)
+template <bool BASE, bool MSG>
 __global__ void __launch_bounds__(kThreads) k_fold_msg(const void* f, const void* b, uint64_t n,
-                                                       uint64_t d, e256d r, uint64_t d2, int msg,
+                                                       uint64_t d, e256d r, uint64_t d2,
                                                        e256d* nf, e256d* nb, uint32_t* acc) {
     __shared__ f128 tab[3 * SHOUP_ENTRIES];
     extern __shared__ f128 ws[];
@@ -923,7 +940,7 @@ __global__ void __launch_bounds__(kThreads) k_fold_msg(const void* f, const void
     build_tables<3>(tab, m);
     const uint64_t half = n / 2;
     const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    if (!msg || half < 2) {
+    if (!MSG || half < 2) {
         for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < half; o += step) {
             nf[o] = fold_at<BASE>(f, o, d, tab);
             nb[o] = fold_at<BASE>(b, o, d, tab);
@@ -942,6 +959,30 @@ __global__ void __launch_bounds__(kThreads) k_fold_msg(const void* f, const void
         nf[p1] = g1;
         nb[p0] = h0;
         nb[p1] = h1;
+        emul_acc(U0, wl, g0, h0);
+        emul_acc(U2, wl, eadd(g0, g1), eadd(h0, h1));
+    }
+    const e256d u0 = emul_sum(U0), u2 = emul_sum(U2);
+    uint32_t v[16] = {0};
+    put4(v, u0.c0);
+    put4(v + 4, u0.c1);
+    put4(v + 8, u2.c0);
+    put4(v + 12, u2.c1);
+    block_xor_atomic<16>(v, acc);
+}
+
+// F256 message over blocked pairs at d2 (the unfused form for large tables)
+__global__ void __launch_bounds__(kThreads) k_msg_e_comb(const e256d* f, const e256d* b, uint64_t n,
+                                                         uint64_t d2, uint32_t* acc) {
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    f256u U0[3], U2[3];
+    f256u_zero3(U0);
+    f256u_zero3(U2);
+    for (uint64_t o = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; o < n / 2;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t p0 = pair_lo(o, d2), p1 = p0 + d2;
+        const e256d g0 = f[p0], g1 = f[p1], h0 = b[p0], h1 = b[p1];
         emul_acc(U0, wl, g0, h0);
         emul_acc(U2, wl, eadd(g0, g1), eadd(h0, h1));
     }
@@ -1108,8 +1149,10 @@ using flock::kThreads;
 // kernels whose dynamic LDS is the per-lane comb workspace
 hipError_t comb_attrs() {
     static const hipError_t e = [] {
-        const void* fns[] = {reinterpret_cast<const void*>(&flock::k_fold_msg<true>),
-                             reinterpret_cast<const void*>(&flock::k_fold_msg<false>),
+        const void* fns[] = {reinterpret_cast<const void*>(&flock::k_fold_msg<true, true>),
+                             reinterpret_cast<const void*>(&flock::k_fold_msg<false, true>),
+                             reinterpret_cast<const void*>(&flock::k_msg_e_comb),
+                             reinterpret_cast<const void*>(&flock::k_intro_base),
                              reinterpret_cast<const void*>(&flock::k_msg_blocked_comb),
                              reinterpret_cast<const void*>(&flock::k_dot_comb)};
         for (const void* f : fns)
@@ -1571,6 +1614,7 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     return X.hip(hipStreamSynchronize(X.st), "sync");
 }
 
+constexpr uint64_t kSplitFoldMsg = 1ull << 16;  // folded length from which fold and message split
 struct Ladder {
     Ctx& X;
     flock_fs* fs;
@@ -1596,31 +1640,43 @@ struct Ladder {
     bool msg(uint64_t d, E* u0, E* u2) {
         if (!X.zero_acc(16)) return false;
         const uint64_t dd = (d == 1 || n == d) ? 1 : d;
-        hipLaunchKernelGGL(flock::k_msg_e, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st, f, b, n, dd,
-                           X.ws->acc);
+        hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(n / 2)), dim3(kThreads), kCombWsBytes, X.st, f, b, n,
+                           dd, X.ws->acc);
         if (!X.chk("msg") || !X.read_acc(16)) return false;
         *u0 = E{X.acc_f(0), X.acc_f(1)};
         *u2 = E{X.acc_f(2), X.acc_f(3)};
         return true;
     }
     bool fold(uint64_t d, E r) {
-        hipLaunchKernelGGL(flock::k_fold_msg<false>, dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st,
+        hipLaunchKernelGGL((flock::k_fold_msg<false, false>), dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st,
                            static_cast<const void*>(f), static_cast<const void*>(b), n, d, E4(r),
-                           (uint64_t)1, 0, f2, b2, X.ws->acc);
+                           (uint64_t)1, f2, b2, X.ws->acc);
         if (!X.chk("fold")) return false;
         swap();
         n /= 2;
         return true;
     }
     // the fold and the next round's message (over pairs at d, or LSB pairs once a single block
-    // of d is left) in one pass; BASE: the inputs are the F128 tables (fb, bb) of length n
+    // of d is left); BASE: the inputs are the F128 tables (fb, bb) of length n. Small tables take
+    // one fused pass; large ones a fold pass and a message pass, each at a higher occupancy
+    // than the fused kernel's.
     template <bool BASE>
     bool fold_msg(uint64_t d, E r, E* u0, E* u2, const void* fb = nullptr, const void* bb = nullptr) {
         const uint64_t nn = n / 2, d2 = (d == 1 || nn == d) ? 1 : d;
+        const void* src_f = BASE ? fb : static_cast<const void*>(f);
+        const void* src_b = BASE ? bb : static_cast<const void*>(b);
+        e256d* dst_f = BASE ? f : f2;
+        e256d* dst_b = BASE ? b : b2;
         if (!X.zero_acc(16)) return false;
-        hipLaunchKernelGGL(flock::k_fold_msg<BASE>, dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes,
-                           X.st, BASE ? fb : static_cast<const void*>(f), BASE ? bb : static_cast<const void*>(b),
-                           n, d, E4(r), d2, 1, BASE ? f : f2, BASE ? b : b2, X.ws->acc);
+        if (nn >= kSplitFoldMsg) {
+            hipLaunchKernelGGL((flock::k_fold_msg<BASE, false>), dim3(X.grid(nn)), dim3(kThreads), 0, X.st, src_f,
+                               src_b, n, d, E4(r), d2, dst_f, dst_b, X.ws->acc);
+            hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes, X.st,
+                               dst_f, dst_b, nn, d2, X.ws->acc);
+        } else {
+            hipLaunchKernelGGL((flock::k_fold_msg<BASE, true>), dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes,
+                               X.st, src_f, src_b, n, d, E4(r), d2, dst_f, dst_b, X.ws->acc);
+        }
         if (!X.chk("fold msg") || !X.read_acc(16)) return false;
         if (!BASE) swap();
         n = nn;
@@ -1744,7 +1800,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         for (int o = 0; o < C->lv[level].ood; o++) {
             const std::vector<H> z = sample_vec(fs, (size_t)split_dim);
             if (!X.eq_dev(z.data(), (size_t)split_dim, ezl) || !X.zero_acc(20)) return false;
-            hipLaunchKernelGGL(flock::k_intro_base, dim3(X.grid(Lg.n / 2)), dim3(kThreads), 0, X.st, Lg.f,
+            hipLaunchKernelGGL(flock::k_intro_base, dim3(X.grid(Lg.n / 2)), dim3(kThreads), kCombWsBytes, X.st, Lg.f,
                                reinterpret_cast<const uint4*>(ezl), Lg.n, X.ws->acc);
             if (!X.chk("intro ood") || !X.read_acc(20)) return false;
             const H y = X.acc_f(0);
