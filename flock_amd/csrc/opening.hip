@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -1030,6 +1031,42 @@ __global__ void __launch_bounds__(kThreads) k_dot_comb(const f128* a, const f128
     block_xor_atomic<4>(v, acc);
 }
 
+// s[i][k] = s_k(q_i) inv_k with s_0(x) = x, s_k(x) = s_{k-1}(x)^2 + s_{k-1}(v_{k-1}) s_{k-1}(x)
+// (eval_sk_at_vks, ligerito.rs:2402-2420); c = [s_k(v_k), k <= dims | inv_k, k <= dims]
+// squaring through its nibble tables (x -> x^2 is F2-linear), the two per-step constants
+// through Shoup tables rebuilt per step
+__global__ void __launch_bounds__(kThreads) k_induced_sk(const uint64_t* qs, uint32_t nq, const f128* c,
+                                                         uint32_t dims, f128* s) {
+    __shared__ f128 sq[512];
+    __shared__ f128 tab[2 * SHOUP_ENTRIES];
+    __shared__ f128 img[128];
+    // (x^b)^2 = x^(2b); shifts stay below 128
+    for (uint32_t b = threadIdx.x; b < 128; b += blockDim.x)
+        img[b] = 2 * b < 128 ? f128_shl_mod(f128_one(), (int)(2 * b))
+                             : f128_shl_mod(f128_shl_mod(f128_one(), 127), (int)(2 * b - 127));
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < 512; e += blockDim.x) {
+        const uint32_t k = e / 16, nib = e % 16;
+        f128 v = f128_zero();
+        for (int j = 0; j < 4; j++)
+            if ((nib >> j) & 1) v = f128_add(v, img[4 * k + j]);
+        sq[e] = v;
+    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < nq;
+    f128 cur = live ? make_uint4((uint32_t)qs[i], (uint32_t)(qs[i] >> 32), 0u, 0u) : f128_zero();
+    f128* si = s + (size_t)(live ? i : 0) * dims;
+    for (uint32_t k = 0; k < dims; k++) {
+        __syncthreads();  // the previous step's tables are consumed
+        if (threadIdx.x < 64)
+            shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES,
+                            threadIdx.x < 32 ? (k ? c[k - 1] : f128_zero()) : c[dims + 1 + k], threadIdx.x & 31);
+        __syncthreads();
+        if (k) cur = f128_add(lin_apply_d(sq, cur), shoup_mul(tab, cur));
+        if (live) si[k] = shoup_mul(tab + SHOUP_ENTRIES, cur);
+    }
+}
+
 // Induced basis (ligerito.rs:2676-2770 with empty rows) as out = sum_i lo_i (x) hi_i:
 // lo_i[j] = aw_i prod_{k < h, bit k of j} s_ik over the low h coordinates, hi_i[j] =
 // prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i)
@@ -1570,26 +1607,35 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
             else
                 layer[j - 1] = nx;
         }
-    std::vector<H> inv(dims + 1);
-    for (int i = 0; i <= dims; i++) inv[i] = hzero(skv[i]) ? hz() : hinv(skv[i]);
+    // the inverses of the nonzero s_k(v_k) with one field inversion (batched: prefix products)
+    std::vector<H> inv(dims + 1, hz()), pre(dims + 2, h1());
+    for (int i = 0; i <= dims; i++) pre[i + 1] = hzero(skv[i]) ? pre[i] : hm(pre[i], skv[i]);
+    H acc = hinv(pre[dims + 1]);
+    for (int i = dims; i >= 0; i--) {
+        if (hzero(skv[i])) continue;
+        inv[i] = hm(acc, pre[i]);
+        acc = hm(acc, skv[i]);
+    }
     const std::vector<H> aw = heq(alpha.data(), alpha.size());
     const uint32_t nq = (uint32_t)qs.size();
     const int h = dims / 2;
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
-    // the normalized subspace polynomials at each query, s_ik = s_k(q_i) / s_k(v_k)
-    std::vector<H> sk((size_t)nq * (dims > 0 ? dims : 1));
-    for (uint32_t i = 0; i < nq && dims > 0; i++) {
-        H* si = &sk[(size_t)i * dims];
-        si[0] = H{qs[i], 0};
-        for (int k = 1; k < dims; k++) si[k] = ha(hm(si[k - 1], si[k - 1]), hm(skv[k - 1], si[k - 1]));
-        for (int k = 0; k < dims; k++) si[k] = hm(si[k], inv[k]);
-    }
-    flock_f128* d_s = X.buf<flock_f128>(48, sk.size());
+    // per-query subspace polynomials (device): s_ik = s_k(q_i) / s_k(v_k)
+    std::vector<H> consts(2 * (size_t)(dims + 1));
+    std::copy(skv.begin(), skv.end(), consts.begin());
+    std::copy(inv.begin(), inv.end(), consts.begin() + dims + 1);
+    flock_f128* d_s = X.buf<flock_f128>(48, (size_t)nq * (dims > 0 ? dims : 1));
     flock_f128* d_aw = X.buf<flock_f128>(49, nq);
     flock_f128* d_lo = X.buf<flock_f128>(44, (size_t)nq * nl);
     flock_f128* d_hi = X.buf<flock_f128>(45, (size_t)nq * nh);
+    uint64_t* d_q = X.buf<uint64_t>(51, nq);
+    flock_f128* d_c = X.buf<flock_f128>(52, consts.size());
     if (X.err) return false;
-    if (!X.h2d(d_s, sk.data(), 16 * sk.size()) || !X.h2d(d_aw, aw.data(), 16 * nq)) return false;
+    if (!X.h2d(d_q, qs.data(), 8 * nq) || !X.h2d(d_c, consts.data(), 16 * consts.size()) ||
+        !X.h2d(d_aw, aw.data(), 16 * nq))
+        return false;
+    hipLaunchKernelGGL(flock::k_induced_sk, dim3((nq + kThreads - 1) / kThreads), dim3(kThreads), 0, X.st, d_q, nq,
+                       reinterpret_cast<const uint4*>(d_c), (uint32_t)dims, reinterpret_cast<uint4*>(d_s));
     hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(d_s),
                        reinterpret_cast<const uint4*>(d_aw), (uint32_t)dims, (uint32_t)h, reinterpret_cast<uint4*>(d_lo),
                        reinterpret_cast<uint4*>(d_hi));
@@ -1693,6 +1739,29 @@ struct Ladder {
     }
 };
 
+// FLOCK_OPEN_TRACE=1: wall time of the inner opening's steps, summed per step name, on stderr
+struct StepTrace {
+    bool on = std::getenv("FLOCK_OPEN_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    std::vector<std::pair<const char*, double>> acc;
+    void mark(const char* name) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+        for (auto& a : acc)
+            if (a.first == name) {
+                a.second += ms;
+                return;
+            }
+        acc.emplace_back(name, ms);
+    }
+    ~StepTrace() {
+        if (!on) return;
+        for (auto& a : acc) std::fprintf(stderr, "open-trace %-14s %8.3f ms\n", a.first, a.second);
+    }
+};
+
 bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_params* P,
                 const flock_f128* cw0, const uint8_t* tree0, const std::vector<H>& rho, H q_eval,
                 Bin& out) {
@@ -1708,6 +1777,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     std::vector<Sched> sch;
     for (int l = 0; l <= R; l++) sch.push_back(sched_of(C->lv[l].queries, C->lv[l].cols + C->lv[l].rate));
     const uint32_t kind = P->merkle_hash;
+    StepTrace T;
 
     // compute_combined_basis_and_target: one packed-direct EqPoint claim (pcs.rs:423-560)
     label(fs, "flock-pcs-open-batch-v0");
@@ -1748,6 +1818,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
                            reinterpret_cast<const uint4*>(ez_), N, U4(beta));
         if (!X.chk("axpy")) return false;
     }
+    T.mark("ood0");
     // first message over blocked pairs (F128)
     if (!X.zero_acc(8)) return false;
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
@@ -1773,6 +1844,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     }
     // the split table's buffers must hold the doubled length: the first fold wrote N/2, the
     // switch's output is 2 * 2^n1 <= N/2 for ik >= 2
+    T.mark("lane folds");
     int split_dim = n1 + 1;
     flock_f128* base = X.buf<flock_f128>(16, 1ull << split_dim);
     if (X.err) return false;
@@ -1793,6 +1865,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     flock_fs_observe_bytes(fs, capb.data(), capb.size());
     caps.vec_hash(capb.data(), capb.size() / 32);
     n_caps++;
+    T.mark("commit");
     flock_f128* ezl = X.buf<flock_f128>(17, 1ull << split_dim);
     flock_f128* basis = X.buf<flock_f128>(18, 1ull << split_dim);
     if (X.err) return false;
@@ -1845,14 +1918,19 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         *alpha = sample_vec(fs, ceil_log2(nq));
     };
     if (!level_ood(1)) return false;
+    T.mark("ood");
     Bin l0p;
     {
         std::vector<uint64_t> qs;
         std::vector<H> alpha, rows;
         queries(0, bl0, &qs, &alpha);
+        T.mark("queries");
         if (!open_level(X, cw0, t, tree0, bl0, qs, c0, l0p, &rows)) return false;
+        T.mark("open level");
         if (!induced_basis(X, n1, qs, alpha, basis)) return false;
+        T.mark("induced");
         if (!presplit(n1, 1)) return false;
+        T.mark("presplit");
     }
     Bin recp, finp;
     size_t n_rec = 0;
@@ -1869,6 +1947,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
             }
             Lg.push(u0, u2);
         }
+        T.mark("level folds");
         const int ext_dim = split_dim - k, level = i + 1;
         if (i + 1 == R) {
             std::vector<e256d> fv(Lg.n);
@@ -1902,17 +1981,23 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         flock_fs_observe_bytes(fs, capb.data(), capb.size());
         caps.vec_hash(capb.data(), capb.size() / 32);
         n_caps++;
+        T.mark("commit");
         if (!level_ood(next_level)) return false;
+        T.mark("ood");
         {
             std::vector<uint64_t> qs;
             std::vector<H> alpha, rows;
             queries(level, prev.block_len, &qs, &alpha);
+            T.mark("queries");
             if (!open_level(X, prev.cw, prev.lanes, prev.tree, prev.block_len, qs, sch[level].cap(), recp,
                             &rows))
                 return false;
             n_rec++;
+            T.mark("open level");
             if (!induced_basis(X, ext_dim, qs, alpha, basis)) return false;
+            T.mark("induced");
             if (!presplit(ext_dim, next_level)) return false;
+            T.mark("presplit");
         }
         prev = next;
     }
