@@ -266,8 +266,12 @@ __attribute__((target("sha,sse4.1,ssse3"))) void sha256_block_ni(uint32_t h[8],
     _mm_storeu_si128((__m128i*)&h[0], _mm_blend_epi16(lo, hi, 0xF0));  // a b c d
     _mm_storeu_si128((__m128i*)&h[4], _mm_alignr_epi8(hi, lo, 8));    // e f g h
 }
+// FLOCK_HIP_PORTABLE_SHA256=1 forces the portable block function (to exercise it on hosts
+// with the SHA extensions)
 bool have_sha_ni() {
     static const bool v = [] {
+        const char* force = std::getenv("FLOCK_HIP_PORTABLE_SHA256");
+        if (force && force[0] && force[0] != '0') return false;
         unsigned a, b, c, d;
         return __get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & (1u << 29)) &&
                __get_cpuid(1, &a, &b, &c, &d) && (c & bit_SSE4_1) && (c & bit_SSSE3);
@@ -346,6 +350,7 @@ void sha256_block(uint32_t h[8], const uint8_t* p) {
 // PoW predicate (challenger.rs:884-958): SHA256(digest || nonce_le) starts with `bits` zero
 // bits, MSB first. `blk` is the padded one-block message with the digest in bytes 0..31; the
 // nonce goes into bytes 32..39.
+constexpr uint32_t kMaxPowBits = 64;
 bool pow_ok(uint8_t blk[64], uint64_t nonce, uint32_t bits) {
     if (bits > 256) return false;
     for (int i = 0; i < 8; i++) blk[32 + i] = (uint8_t)(nonce >> (8 * i));
@@ -490,6 +495,10 @@ int flock_fs_verify_pow(flock_fs* fs, uint64_t nonce, uint32_t bits) {
 }
 
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
+    if (bits > kMaxPowBits) {
+        fail(FLOCK_HIP_E_INVALID, "grind_pow: more than 64 bits");
+        return UINT64_MAX;  // nothing absorbed
+    }
     uint8_t digest[32];
     fs->st.digest(digest);
     uint64_t nonce = 0;

@@ -487,6 +487,11 @@ void flock_fs_observe_f128_slice(flock_fs* fs, const flock_f128* v, size_t n);
 void flock_fs_observe_bytes(flock_fs* fs, const uint8_t* b, size_t n);
 void flock_fs_sample_f128(flock_fs* fs, flock_f128* out);
 void flock_fs_sample_f128_vec(flock_fs* fs, flock_f128* out, size_t n);
+/* Challenger::grind_pow (challenger.rs:884-958): the smallest nonce whose SHA-256 with the
+ * state digest starts with `bits` zero bits (bits = 0: nonce 0), absorbed as 8 LE bytes. bits
+ * above 64 are refused: UINT64_MAX, FLOCK_HIP_E_INVALID in flock_hip_last_error, nothing
+ * absorbed. FLOCK_HIP_PORTABLE_SHA256=1 in the environment forces the portable SHA-256 block
+ * function over the SHA extensions for every host transcript. */
 uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits);
 /* Challenger::verify_pow (challenger.rs:960-981): 1 if nonce passes (bits = 0: only nonce 0),
  * else 0; the nonce bytes are absorbed either way. */

@@ -349,3 +349,55 @@ def test_oracle_fs_against_reference_challenger(tmp_path):
                   + struct.pack("<I", nops) + body)
     out = subprocess.run([REF_FS, str(p)], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout
+
+
+REF_B3W = os.path.join(O.ORACLE_DIR, "_ref", "test_blake3_witness_host")
+need_ref_b3w = pytest.mark.skipif(
+    not (os.path.exists("/root/reference") and os.path.exists(REF_B3W)),
+    reason="reference checkout / oracle/_ref/test_blake3_witness_host not present")
+
+
+def write_b3wt(path, n_blocks_log, inputs, z, a, b, stripe):
+    """The B3WT vector file cuda-ghash/test_blake3_witness_host.cpp reads: magic, n_blocks_log,
+    n_blocks, k_log; per block cv[8], m[16], counter u64, block_len, flags; then the per-block
+    packed z, a, b (256 u64 each) and the lincheck stripe."""
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<IIII", 0x42335754, n_blocks_log, len(inputs), 14))
+        for row in inputs:
+            fh.write(struct.pack("<8I", *[int(x) for x in row[:8]]))
+            fh.write(struct.pack("<16I", *[int(x) for x in row[8:24]]))
+            fh.write(struct.pack("<QII", int(row[24]) | (int(row[25]) << 32), int(row[26]),
+                                 int(row[27])))
+        for t in (z, a, b):
+            fh.write(np.ascontiguousarray(t, dtype=np.uint64).tobytes())
+        fh.write(np.ascontiguousarray(stripe, dtype=np.uint8).tobytes())
+
+
+@need_ref_b3w
+def test_oracle_blake3_witness_against_reference_host_check(tmp_path):
+    # the oracle's per-block (z, a, b) traces and its batch-major lincheck stripe, checked by
+    # the reference's own host replica of the BLAKE3 witness builder (blake3_witness.cuh)
+    import test_oracle_blake3_r1cs as B3
+    n_log = 4
+    inputs = B3.random_inputs(1 << n_log, 0xB3B3)
+    inputs[3, 24:26] = (0xFFFFFFFF, 7)  # a counter with both halves set
+    traces = [B3.instance(row) for row in inputs]
+    z = np.stack([t[0] for t in traces])
+    a = np.stack([t[1] for t in traces])
+    b = np.stack([t[2] for t in traces])
+    _, _, _, stripe = B3.batch_major(inputs, n_log)
+    p = tmp_path / "b3.bin"
+    write_b3wt(p, n_log, inputs, z, a, b, stripe)
+    out = subprocess.run([REF_B3W, str(p)], capture_output=True, text=True)
+    assert out.returncode == 0 and "BLAKE3 WITNESS HOST CHECK OK" in out.stdout, out.stdout
+    # planted mistakes: one bit of one trace, then one stripe bit, must each be caught
+    a2 = a.copy()
+    a2[5, 77] ^= 1 << 13
+    write_b3wt(p, n_log, inputs, z, a2, b, stripe)
+    out = subprocess.run([REF_B3W, str(p)], capture_output=True, text=True)
+    assert out.returncode != 0 and "a FAIL" in out.stdout
+    s2 = stripe.copy()
+    s2[9000] ^= 4
+    write_b3wt(p, n_log, inputs, z, a, b, s2)
+    out = subprocess.run([REF_B3W, str(p)], capture_output=True, text=True)
+    assert out.returncode != 0 and "z_lincheck FAIL" in out.stdout

@@ -2,6 +2,8 @@
 fold-at-z, the full prover) against pure mathematics."""
 from __future__ import annotations
 
+import hashlib
+
 import ctypes
 import random
 
@@ -204,8 +206,64 @@ def test_zerocheck_prove_final_claims_are_mle_evaluations():
     inner = np.zeros((7, 2), dtype=np.uint64)
     L.or_zc_inner_constants(O._p(inner))
     assert r_rest[:7] == O.arr_to_ints(inner)
-    # grinding: nonce bits per zerocheck.rs:96-139 (initial m -> 4 bits, skip 7, rounds 2)
-    assert len([n for n in nonces]) == 2 + n_mlv
+    # grinding: replay the transcript and check every nonce (below)
+    _check_nonce_schedule(b"zc-test", m, r1, ml, nonces)
+
+
+def _pow_zero_bits(digest: bytes, nonce: int) -> int:
+    h = hashlib.sha256(digest + nonce.to_bytes(8, "little")).digest()
+    bits = 0
+    for byte in h:
+        if byte:
+            return bits + 8 - byte.bit_length()
+        bits += 8
+    return bits
+
+
+def _check_nonce_schedule(dom, m, r1, ml, nonces):
+    """Replay the zerocheck transcript on the library's FsChallenger (the order of
+    zerocheck.rs:641-728) and check each nonce against the grinding bits of zerocheck.rs:96-139
+    — bits_for(m) at the initial point, bits_for(2^7 - 1) = 7 at the skip point, bits_for(2) = 2
+    per multilinear round — with hashlib: SHA-256(state digest || nonce LE) has that many
+    leading zero bits and no smaller nonce does."""
+    from flock_amd import lib as F
+    n_mlv = m - 6
+    schedule = [m.bit_length(), 7] + [2] * n_mlv
+    fs = F.FsChallenger(dom)
+    fs.observe_label(b"flock-zerocheck-v0")
+
+    def ground(i):
+        bits, nonce = schedule[i], int(nonces[i])
+        d = fs.digest()
+        assert _pow_zero_bits(d, nonce) >= bits, (i, bits, nonce)
+        assert all(_pow_zero_bits(d, k) < bits for k in range(nonce)), (i, "not the smallest")
+        assert fs.verify_pow(nonce, bits)
+
+    ground(0)
+    fs.sample_f128_vec(6)
+    fs.sample_f128_vec(m - 6 - 7)  # the 7 inner coordinates are constants
+    fs.observe_f128_slice(O.arr_to_ints(r1[:64]))
+    fs.observe_f128_slice(O.arr_to_ints(r1[64:]))
+    ground(1)
+    fs.sample_f128()
+    mlv = O.arr_to_ints(ml)
+    for j in range(n_mlv):
+        fs.observe_f128(mlv[2 * j])
+        fs.observe_f128(mlv[2 * j + 1])
+        ground(2 + j)
+        fs.sample_f128()
+    assert len(nonces) == len(schedule)
+
+
+def test_nonce_schedule_rejects_a_planted_nonce():
+    m = 14
+    a, b, c = _witness(m, 13)
+    r1, ml, fin, nonces, claim = zerocheck_prove(b"zc-test", a, b, c, m, 1)
+    _check_nonce_schedule(b"zc-test", m, r1, ml, nonces)
+    bad = nonces.copy()
+    bad[3] += 1
+    with pytest.raises(AssertionError):
+        _check_nonce_schedule(b"zc-test", m, r1, ml, bad)
 
 
 L.or_zc_s_hat_v_c.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
