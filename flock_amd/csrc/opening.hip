@@ -582,6 +582,7 @@ __global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
     const uint32_t claim = blockIdx.x / 128, j = blockIdx.x % 128;
     // blockIdx.y picks this block's chunk of blockDim / 8 runs
     __shared__ f128 eq4[64][4];
+    __shared__ f128 tab[4 * SHOUP_ENTRIES];
     for (uint32_t l = threadIdx.x; l <= a.m; l += blockDim.x) {
         const f128 r = l < a.n ? a.zrow[claim * a.n + l] : f128_zero();
         const f128 x = l < a.m ? a.xs[(size_t)j * a.m + l] : f128_zero();
@@ -603,10 +604,16 @@ __global__ void __launch_bounds__(kThreads) k_bp_values(BpArgs a, f128* out) {
         f128 S[4] = {f128_zero(), f128_zero(), f128_one(), f128_zero()};  // e_SUCCESS = state 2
         const uint64_t tc = live ? a.bc[b] : 0, tn = live ? a.bn[b] : 0;
         for (int l = (int)a.m; l >= 0; l--) {
+            // Shoup tables of the layer's four eq4 values (shared by every run of the block)
+            __syncthreads();
+            if (threadIdx.x < 128)
+                shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES, eq4[l][threadIdx.x >> 5],
+                                (int)(threadIdx.x & 31));
+            __syncthreads();
             const int cd = (int)(((tc >> l) & 1) + 2 * ((tn >> l) & 1));
             const int ei = a.sparse[cd][st][q][0], si = a.sparse[cd][st][q][1];
             const f128 sv = si == 0 ? S[0] : si == 1 ? S[1] : si == 2 ? S[2] : S[3];
-            f128 p = f128_mul(eq4[l][ei], sv);
+            f128 p = shoup_mul(tab + ei * SHOUP_ENTRIES, sv);
             p.x ^= __shfl_xor(p.x, 1, 64);
             p.y ^= __shfl_xor(p.y, 1, 64);
             p.z ^= __shfl_xor(p.z, 1, 64);
@@ -1093,18 +1100,21 @@ __global__ void __launch_bounds__(kThreads) k_induced_tables(const f128* s, cons
 // out[jl + 2^h jh] ^= sum_{i in this block's chunk} lo_i[jl] hi_i[jh]: kIndTL rows jl per block
 // (their Shoup tables in LDS, rebuilt per i), one column jh per thread; with several chunks the
 // blocks XOR into out atomically (out zeroed beforehand)
-constexpr int kIndTL = 4;
+constexpr int kIndTL = 4;  // jl rows per block
+constexpr int kIndTH = 2;  // jh columns per thread (blockDim * kIndTH per block)
 __global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const f128* hi, uint32_t nq,
                                                            uint32_t h, uint32_t dims, uint32_t chunk,
                                                            int atomic, f128* out) {
     __shared__ f128 tab[kIndTL * SHOUP_ENTRIES];
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
     const uint64_t jl0 = (uint64_t)blockIdx.x * kIndTL;
-    const uint64_t jh = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x;
+    const uint64_t jh0 = (uint64_t)blockIdx.y * blockDim.x * kIndTH + threadIdx.x;
     const uint32_t i0 = blockIdx.z * chunk, i1 = min(nq, i0 + chunk);
-    f128 acc[kIndTL];
+    f128 acc[kIndTH][kIndTL];
 #pragma unroll
-    for (int t = 0; t < kIndTL; t++) acc[t] = f128_zero();
+    for (int u = 0; u < kIndTH; u++)
+#pragma unroll
+        for (int t = 0; t < kIndTL; t++) acc[u][t] = f128_zero();
     for (uint32_t i = i0; i < i1; i++) {
         __syncthreads();  // the previous tables are consumed
         if (threadIdx.x < 32 * kIndTL) {
@@ -1113,25 +1123,34 @@ __global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const
             shoup_build_row(tab + t * SHOUP_ENTRIES, m, threadIdx.x & 31);
         }
         __syncthreads();
-        if (jh < nh) {
-            const f128 x = hi[i * nh + jh];
 #pragma unroll
-            for (int t = 0; t < kIndTL; t++) acc[t] = f128_add(acc[t], shoup_mul(tab + t * SHOUP_ENTRIES, x));
+        for (int u = 0; u < kIndTH; u++) {
+            const uint64_t jh = jh0 + (uint64_t)u * blockDim.x;
+            if (jh < nh) {
+                const f128 x = hi[i * nh + jh];
+#pragma unroll
+                for (int t = 0; t < kIndTL; t++)
+                    acc[u][t] = f128_add(acc[u][t], shoup_mul(tab + t * SHOUP_ENTRIES, x));
+            }
         }
     }
-    if (jh >= nh) return;
 #pragma unroll
-    for (int t = 0; t < kIndTL; t++) {
-        if (jl0 + t >= nl) break;
-        f128* o = out + (jl0 + t) + nl * jh;
-        if (atomic) {
-            uint32_t* w = reinterpret_cast<uint32_t*>(o);
-            if (acc[t].x) atomicXor(w, acc[t].x);
-            if (acc[t].y) atomicXor(w + 1, acc[t].y);
-            if (acc[t].z) atomicXor(w + 2, acc[t].z);
-            if (acc[t].w) atomicXor(w + 3, acc[t].w);
-        } else {
-            *o = acc[t];
+    for (int u = 0; u < kIndTH; u++) {
+        const uint64_t jh = jh0 + (uint64_t)u * blockDim.x;
+        if (jh >= nh) break;
+#pragma unroll
+        for (int t = 0; t < kIndTL; t++) {
+            if (jl0 + t >= nl) break;
+            f128* o = out + (jl0 + t) + nl * jh;
+            if (atomic) {
+                uint32_t* w = reinterpret_cast<uint32_t*>(o);
+                if (acc[u][t].x) atomicXor(w, acc[u][t].x);
+                if (acc[u][t].y) atomicXor(w + 1, acc[u][t].y);
+                if (acc[u][t].z) atomicXor(w + 2, acc[u][t].z);
+                if (acc[u][t].w) atomicXor(w + 3, acc[u][t].w);
+            } else {
+                *o = acc[u][t];
+            }
         }
     }
 }
@@ -1642,8 +1661,9 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     if (!X.chk("induced tables")) return false;
     // split the query sum over blocks until the grid holds about 2048 workgroups
     const uint32_t gx = (uint32_t)((nl + flock::kIndTL - 1) / flock::kIndTL);
-    const uint32_t gy = (uint32_t)((nh + kThreads - 1) / kThreads);
-    uint32_t gz = std::max<uint32_t>(1, std::min<uint32_t>(nq, 2048 / std::max<uint32_t>(1, gx * gy)));
+    const uint32_t gy = (uint32_t)((nh + kThreads * flock::kIndTH - 1) / (kThreads * flock::kIndTH));
+    // at least 16 queries per block so the per-query table builds amortize
+    uint32_t gz = std::max<uint32_t>(1, std::min<uint32_t>((nq + 15) / 16, 2048 / std::max<uint32_t>(1, gx * gy)));
     const uint32_t chunk = (nq + gz - 1) / gz;
     gz = (nq + chunk - 1) / chunk;
     if (gz > 1) {
