@@ -1247,6 +1247,125 @@ static int lincheck_prove_with(fold_split_fn fold, const void* circuit, or_fs* f
     return 0;
 }
 
+/* prove_union_capture_z_vec_with_grinding over several boolean slots (lincheck/union.rs:
+ * 203-430): per slot the alpha-batched comb through the slot's own quirky table (inner
+ * coordinates x_inner_rest[..k_t - 6]) scaled by eq_prefix_weight(x_inner_rest[k_t - 6..],
+ * prefix_t) (:90-101) at column offset prefix_t << k_t; then each slot's constant-wire beta in
+ * slot order; the slots' partial folds against one eq(x_outer) table at the same offsets; the
+ * column sumcheck over the 2^col_vars union columns; and per slot the unscaled matrix evals
+ * with W_col = z_partial (x) eq(r_inner_rest[..k_t - 6]). matrix_evals holds two per slot. */
+int or_lincheck_prove_union(or_fs* fs, const or_lc_slot* slots, size_t n_slots, int col_vars,
+                            int nu, or_f128 z_skip, const or_f128* x_inner_rest,
+                            const or_f128* x_outer, int grinding, or_f128* rounds,
+                            or_f128 z_partial[64], or_f128* matrix_evals, uint64_t* nonces,
+                            or_f128* claim, or_f128* z_vec_out) {
+    const int ks = 6;
+    if (col_vars < ks || nu < 3 || n_slots == 0) return 1;
+    const size_t K = (size_t)1 << col_vars;
+    const int L = col_vars - ks;
+    int ni = 0;
+    static const uint8_t label[] = "flock-lincheck-v0";
+    or_fs_observe_label(fs, label, sizeof(label) - 1);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 1);
+    const or_f128 alpha = or_fs_sample_f128(fs);
+    or_f128* comb = (or_f128*)calloc(K, sizeof(or_f128));
+    or_f128* zv = (or_f128*)calloc(K, sizeof(or_f128));
+    or_f128** sa = (or_f128**)calloc(n_slots, sizeof(or_f128*));
+    or_f128** sb = (or_f128**)calloc(n_slots, sizeof(or_f128*));
+    for (size_t t = 0; t < n_slots; t++) {
+        const or_lc_slot* sl = &slots[t];
+        const size_t Kt = (size_t)1 << sl->k_log;
+        const int inner = sl->k_log - ks;
+        if (sl->k_log < ks || sl->k_log > col_vars) return 1;
+        or_f128* eq_inner = (or_f128*)malloc(sizeof(or_f128) * Kt);
+        sa[t] = (or_f128*)malloc(sizeof(or_f128) * Kt);
+        sb[t] = (or_f128*)malloc(sizeof(or_f128) * Kt);
+        or_quirky_eq(z_skip, x_inner_rest, (size_t)inner, eq_inner);
+        if (sl->a_ptr) {
+            const csc_ref k = {sl->a_ptr, sl->a_rows, sl->b_ptr, sl->b_rows};
+            csc_fold_cb(&k, Kt, eq_inner, sa[t], sb[t]);
+        } else {
+            keccak_fold_cb(NULL, Kt, eq_inner, sa[t], sb[t]);
+        }
+        or_f128 w = {1, 0};
+        for (int j = 0; j < col_vars - sl->k_log; j++) {
+            const or_f128 x = x_inner_rest[inner + j];
+            w = or_f128_mul(w, ((sl->prefix >> j) & 1) ? x : add(x, (or_f128){1, 0}));
+        }
+        const size_t off = (size_t)sl->prefix << sl->k_log;
+        for (size_t c = 0; c < Kt; c++)
+            comb[off + c] = or_f128_mul(w, add(or_f128_mul(alpha, sa[t][c]), sb[t][c]));
+        free(eq_inner);
+    }
+    for (size_t t = 0; t < n_slots; t++)
+        if (slots[t].const_pin >= 0) {
+            if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 1);
+            const or_f128 beta = or_fs_sample_f128(fs);
+            const size_t off = ((size_t)slots[t].prefix << slots[t].k_log) + (size_t)slots[t].const_pin;
+            comb[off] = add(comb[off], beta);
+        }
+    const size_t n_outer = (size_t)1 << nu;
+    or_f128* eq_outer = (or_f128*)malloc(sizeof(or_f128) * n_outer);
+    or_build_eq(x_outer, (size_t)nu, eq_outer);
+    for (size_t t = 0; t < n_slots; t++) {
+        const size_t off = (size_t)slots[t].prefix << slots[t].k_log;
+        or_partial_fold(slots[t].z, slots[t].k_log, nu, slots[t].useful_bits, slots[t].n_rows, eq_outer,
+                        zv + off);
+    }
+    if (z_vec_out) memcpy(z_vec_out, zv, sizeof(or_f128) * K);
+    or_f128* rr = (or_f128*)malloc(sizeof(or_f128) * (L > 0 ? L : 1));
+    size_t len = K;
+    for (int t = 0; t < L; t++) {
+        or_f128 msg[2];
+        or_lc_round_eval(comb, zv, len, msg);
+        or_fs_observe_f128(fs, msg[0]);
+        or_fs_observe_f128(fs, msg[1]);
+        if (grinding) nonces[ni++] = or_fs_grind_pow(fs, 2);
+        const or_f128 r = or_fs_sample_f128(fs);
+        rounds[2 * t] = msg[0];
+        rounds[2 * t + 1] = msg[1];
+        rr[t] = r;
+        or_lc_bind_top(comb, len, r);
+        or_lc_bind_top(zv, len, r);
+        len /= 2;
+    }
+    for (int i = 0; i < 64; i++) z_partial[i] = zv[i];
+    or_fs_observe_f128_slice(fs, z_partial, 64);
+    if (grinding) nonces[ni++] = or_fs_grind_pow(fs, (uint32_t)ks);
+    const or_f128 r_skip = or_fs_sample_f128(fs);
+    or_f128 lam[64];
+    or_lagrange_phi8(0, r_skip, lam);
+    or_f128 wv = ZERO;
+    for (int i = 0; i < 64; i++) wv = add(wv, or_f128_mul(lam[i], z_partial[i]));
+    claim[0] = r_skip;
+    claim[1] = wv;
+    for (int j = 0; j < L; j++) claim[2 + j] = rr[L - 1 - j];
+    for (size_t t = 0; t < n_slots; t++) {
+        const int inner = slots[t].k_log - ks;
+        or_f128* e = (or_f128*)malloc(sizeof(or_f128) * ((size_t)1 << inner));
+        or_build_eq(claim + 2, (size_t)inner, e);
+        or_f128 ma = ZERO, mb = ZERO;
+        for (size_t j = 0; j < ((size_t)1 << inner); j++)
+            for (int i = 0; i < 64; i++) {
+                const or_f128 wc = or_f128_mul(e[j], z_partial[i]);
+                ma = add(ma, or_f128_mul(wc, sa[t][j * 64 + i]));
+                mb = add(mb, or_f128_mul(wc, sb[t][j * 64 + i]));
+            }
+        matrix_evals[2 * t] = ma;
+        matrix_evals[2 * t + 1] = mb;
+        free(e);
+        free(sa[t]);
+        free(sb[t]);
+    }
+    free(sa);
+    free(sb);
+    free(rr);
+    free(eq_outer);
+    free(zv);
+    free(comb);
+    return 0;
+}
+
 /* ======================================================================== */
 /* Union instance host glue (crates/flock-core/src/{schedule,union}.rs)     */
 /* ======================================================================== */

@@ -185,6 +185,21 @@ int or_lincheck_prove_keccak(or_fs* fs, const uint8_t* z, int nu, size_t n_rows,
                              or_f128 matrix_evals[2], uint64_t* nonces, or_f128* claim,
                              or_f128* z_vec_out);
 
+/* one boolean slot of the union lincheck: a_ptr = NULL selects the Keccak-f[1600] walker */
+typedef struct {
+    int k_log;
+    uint64_t prefix;
+    size_t useful_bits, n_rows;
+    long const_pin;
+    const uint8_t* z; /* the slot's lincheck stripe */
+    const uint32_t *a_ptr, *a_rows, *b_ptr, *b_rows;
+} or_lc_slot;
+int or_lincheck_prove_union(or_fs* fs, const or_lc_slot* slots, size_t n_slots, int col_vars,
+                            int nu, or_f128 z_skip, const or_f128* x_inner_rest,
+                            const or_f128* x_outer, int grinding, or_f128* rounds,
+                            or_f128 z_partial[64], or_f128* matrix_evals, uint64_t* nonces,
+                            or_f128* claim, or_f128* z_vec_out);
+
 /* ---- union instance host glue (crates/flock-core/src/{schedule,union}.rs) ---- */
 #define OR_MAX_TYPES 16
 typedef struct {

@@ -274,6 +274,183 @@ def prove_fast(kind, inputs, domain: bytes, n_log: int | None = None, parts: dic
     return proof, cap, claim_ab, claim_c
 
 
+class OrLcSlot(ctypes.Structure):
+    _fields_ = [("k_log", ctypes.c_int), ("prefix", ctypes.c_uint64),
+                ("useful_bits", ctypes.c_size_t), ("n_rows", ctypes.c_size_t),
+                ("const_pin", ctypes.c_long), ("z", P), ("a_ptr", P), ("a_rows", P),
+                ("b_ptr", P), ("b_rows", P)]
+
+
+L.or_lincheck_prove_union.argtypes = [P, ctypes.POINTER(OrLcSlot), S, ctypes.c_int, ctypes.c_int,
+                                      O.F128, P, P, ctypes.c_int, P, P, P, P, P, P]
+
+
+def prove_fast_union(slots, nu: int, domain: bytes, parts: dict | None = None):
+    """prove_union_with_binding_zc (prover.rs:831-1468) for a registry of several boolean hash
+    types. slots: [(kind, inputs)] in any order; the registry sorts them (Registry::new, k_log
+    descending, stable). Returns (proof_bytes, cap_bytes, claim_ab, claim_c, info) with info =
+    dict(counts in slot order, dense_m, log_batch_size, lanes)."""
+    kinds = [k for k, _ in slots]
+    U = Union([(SHAPES[k][0], SHAPES[k][1], False) for k in kinds], nu)
+    n_slots = len(slots)
+    order = [U.reg.slots[s].type_index for s in range(n_slots)]  # caller index of slot s
+    m_bool = U.reg.m_bool
+    packed_words = 1 << (m_bool - 7)
+    Z, A, B = (np.zeros((packed_words, 2), dtype=np.uint64) for _ in range(3))
+    counts, stripes = [], []
+    for s, ci in enumerate(order):
+        kind, inputs = slots[ci]
+        inputs = np.ascontiguousarray(inputs)
+        z, a, b, stripe = witness(kind, inputs, nu)
+        off = U.reg.slots[s].offset >> 7
+        Z[off:off + len(z)] = z
+        A[off:off + len(a)] = a
+        B[off:off + len(b)] = b
+        stripes.append(stripe)
+        counts.append((1 << nu) if kind == HASH_KECCAK else len(inputs))
+    sz0 = U.sizes(counts, 0, 0)
+    dense_m = sz0["dense_m"]
+    lbs = L.or_lig_initial_k(dense_m)
+    assert lbs > 0, f"no Ligerito config for m={dense_m}"
+    sz = U.sizes(counts, 0, lbs)
+    q = U.compact(counts, Z)
+    t = sz["commit_lanes"] or 0
+    if t:
+        cw, tree = O.commit_lane_major(q, dense_m, 1, lbs, t)
+    else:
+        cw, tree = O.commit(q, dense_m, 1, lbs, 0)
+    n_leaves = (len(tree) // 32 + 1) // 2
+    c0 = L.or_cap_depth_for(dense_m)
+    cap = tree[32 * (2 * n_leaves - 2 * (1 << c0)):][:32 << c0]
+    fs = O.Fs(domain)
+    digest = O.registry_digest([circuit(k)[0] for k in kinds], nu)
+    cnt = np.array(counts, dtype=np.uint64)
+    L.or_bind_statement(fs.h, digest, cnt.ctypes.data, n_slots, cap, len(cap))
+    m = m_bool
+    n_mlv = m - 6
+    r1 = np.zeros((128, 2), dtype=np.uint64)
+    ml = np.zeros((2 * n_mlv, 2), dtype=np.uint64)
+    fin = np.zeros((3, 2), dtype=np.uint64)
+    zc_non = np.zeros(2 + n_mlv, dtype=np.uint64)
+    claim = np.zeros((1 + 2 * n_mlv, 2), dtype=np.uint64)
+    zb, ab, bb = (x.view(np.uint8) for x in (Z, A, B))
+    assert L.or_zerocheck_prove(fs.h, ab.ctypes.data, bb.ctypes.data, zb.ctypes.data, m, 1,
+                                O._p(r1), O._p(ml), O._p(fin), zc_non.ctypes.data,
+                                O._p(claim)) == 0
+    r_rest = np.ascontiguousarray(claim[1 + n_mlv:1 + 2 * n_mlv])
+    rr = np.zeros((m, 2), dtype=np.uint64)
+    rr[6:] = r_rest
+    shv_c = np.zeros((128, 2), dtype=np.uint64)
+    L.or_zc_s_hat_v_c(zb.ctypes.data, m, O._p(rr), O._p(shv_c))
+    col_vars = m - nu
+    Lr = col_vars - 6
+    xr = np.zeros((max(Lr, 1), 2), dtype=np.uint64)
+    xo = np.zeros((nu, 2), dtype=np.uint64)
+    mlv = np.ascontiguousarray(claim[1:1 + n_mlv])
+    L.or_x_ab_from_mlv(O._p(mlv), n_mlv, nu, O._p(xr), O._p(xo))
+    lc_slots = (OrLcSlot * n_slots)()
+    keep = []
+    for s, ci in enumerate(order):
+        kind = kinds[ci]
+        k_log, ub, pin = SHAPES[kind]
+        sl = lc_slots[s]
+        sl.k_log, sl.prefix, sl.useful_bits, sl.n_rows, sl.const_pin = \
+            k_log, U.reg.slots[s].prefix, ub, counts[s], pin
+        sl.z = stripes[s].ctypes.data
+        if kind != HASH_KECCAK:
+            _, (csc_a, csc_b), _ = circuit(kind)
+            sl.a_ptr, sl.a_rows = csc_a[0].ctypes.data, csc_a[1].ctypes.data
+            sl.b_ptr, sl.b_rows = csc_b[0].ctypes.data, csc_b[1].ctypes.data
+            keep.append((csc_a, csc_b))
+    n_pins = n_slots
+    rounds = np.zeros((max(2 * Lr, 1), 2), dtype=np.uint64)
+    zp = np.zeros((64, 2), dtype=np.uint64)
+    mev = np.zeros((2 * n_slots, 2), dtype=np.uint64)
+    lnon = np.zeros(Lr + 2 + n_pins, dtype=np.uint64)
+    lcl = np.zeros((2 + Lr, 2), dtype=np.uint64)
+    z_vec = np.zeros((1 << col_vars, 2), dtype=np.uint64)
+    z_skip = O.f(O.arr_to_ints(claim[:1])[0])
+    assert L.or_lincheck_prove_union(fs.h, lc_slots, n_slots, col_vars, nu, z_skip, O._p(xr),
+                                     O._p(xo), 1, O._p(rounds), O._p(zp), O._p(mev),
+                                     lnon.ctypes.data, O._p(lcl), O._p(z_vec)) == 0
+    r_inner_rest = O.arr_to_ints(lcl[2:2 + Lr])
+    shv_ab = s_hat_v_from_z_vec(z_vec, r_inner_rest[1:])
+    x_ab = [r_inner_rest[0]] + O.arr_to_ints(xo) + r_inner_rest[1:]
+    x_c = O.arr_to_ints(r_rest)
+    assert len(x_ab) == len(x_c) == m - 6
+    # jagged heights (union.rs:293-310): each slot's used columns at its count
+    col_log = m - nu - 7
+    heights = np.zeros(1 << col_log, dtype=np.uint64)
+    for s, ci in enumerate(order):
+        k_log, ub, _ = SHAPES[kinds[ci]]
+        used = min((ub + 127) // 128, 1 << (k_log - 7))
+        c_off = U.reg.slots[s].offset >> (7 + nu)
+        heights[c_off:c_off + used] = counts[s]
+    xs = O.ints_to_arr(x_ab + x_c)
+    sv = np.ascontiguousarray(np.concatenate([O.ints_to_arr(shv_ab), shv_c]))
+    pp = O.PcsParams(dense_m, 1, lbs, t, 0)
+    out_p = ctypes.POINTER(ctypes.c_uint8)()
+    out_n = ctypes.c_size_t()
+    qev = np.zeros((1, 2), dtype=np.uint64)
+    assert L.or_open_merged(fs.h, O._p(q), ctypes.byref(pp), O._p(cw), tree, 2, O._p(xs), m - 6,
+                            O._p(sv), heights.ctypes.data, len(heights), nu, ctypes.byref(out_p),
+                            ctypes.byref(out_n), O._p(qev)) == 0
+    open_bytes = ctypes.string_at(out_p, out_n.value)
+    L.or_free_bytes(out_p)
+    w = _Bin()
+    r1i = O.arr_to_ints(r1)
+    w.vec_f128(r1i[:64])
+    w.vec_f128(r1i[64:])
+    w.vec_pairs(O.arr_to_ints(ml))
+    for v in O.arr_to_ints(fin):
+        w.f128(v)
+    w.vec_u64(zc_non)
+    # LincheckProof (lincheck.rs:520-549): one matrix-eval pair per boolean type
+    w.vec_pairs(O.arr_to_ints(rounds[:2 * Lr]))
+    w.vec_f128(O.arr_to_ints(zp))
+    w.vec_pairs(O.arr_to_ints(mev))
+    w.vec_u64(lnon)
+    w.raw(open_bytes)
+    proof = w.bytes()
+    claim_ab = O.arr_to_ints(lcl[1:2])[0]
+    claim_c = O.arr_to_ints(fin[2:3])[0]
+    info = dict(counts=counts, dense_m=dense_m, log_batch_size=lbs, lanes=t)
+    if parts is not None:
+        parts.update(info)
+    return proof, cap, claim_ab, claim_c, info
+
+
+def mixed_bundle_bytes(registry_code: int, counts, cap: bytes, dense_m: int, lbs: int,
+                       lanes: int, proof: bytes) -> bytes:
+    """MixedProofBundleLigerito::to_bytes (proof_io.rs:267-282): "FLOCK", version 21, flavor 4,
+    then bincode of {registry_id (u8 code, mixed.rs:91-98), counts, Commitment {cap, PcsParams
+    {m, log_inv_rate, log_batch_size, profile Fast (variant 0), num_lanes, merkle_hash SHA-256
+    (variant 0)}}, proof}."""
+    w = _Bin()
+    w.raw(b"FLOCK" + bytes([21, 4, registry_code]))
+    w.vec_u64(counts)
+    w.u64(len(cap) // 32)
+    w.raw(cap)
+    w.u64(dense_m)
+    w.u64(1)
+    w.u64(lbs)
+    w.raw(struct.pack("<I", 0))
+    w.raw(b"\x01" + struct.pack("<Q", lanes) if lanes else b"\x00")
+    w.raw(struct.pack("<I", 0))
+    w.raw(proof)
+    return w.bytes()
+
+
+def mixed_anchor_digest(bundle: bytes, claim_ab: int, claim_c: int) -> str:
+    """m6_merged_union_proof_bytes_pinned's digest (union_m6_fixtures.rs:265-272): SHA-256 of
+    the bundle bytes and the two claim values."""
+    h = hashlib.sha256()
+    h.update(bundle)
+    for v in (claim_ab, claim_c):
+        h.update(struct.pack("<QQ", v & (2**64 - 1), v >> 64))
+    return h.hexdigest()
+
+
 def anchor_digest(proof: bytes, cap: bytes, claim_ab: int, claim_c: int) -> str:
     """merged_bundle_digest (union_m6_fixtures.rs:156-172): SHA-256 over bincode(proof),
     the commitment cap, and the two claim values (lo, hi little-endian)."""
