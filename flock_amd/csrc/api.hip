@@ -1980,13 +1980,23 @@ static const uint8_t kLabelLincheck[] = "flock-lincheck-v0";
 using LcFold = std::function<int(const uint4* eq, uint4 alpha, uint4* sa, uint4* sb, uint4* comb,
                                  int* d_err)>;
 
-static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fold,
-                               uint32_t k_log, int64_t const_pin, const uint8_t* d_z, uint32_t nu,
-                               size_t useful_bits, size_t n_rows, flock_f128 z_skip,
-                               const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
-                               int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
-                               flock_f128 h_matrix_evals[2], uint64_t* h_nonces,
-                               flock_f128* h_claim, flock_f128* d_z_vec);
+// One boolean slot of the union lincheck: its circuit fold, k_log, column prefix (column
+// offset prefix << k_log), constant-wire pin (-1: none) and lincheck stripe.
+struct LcSlotRun {
+    LcFold fold;
+    uint32_t k_log;
+    uint64_t prefix;
+    int64_t const_pin;
+    const uint8_t* d_z;
+    size_t useful_bits, n_rows;
+};
+
+static int lincheck_prove_slots(flock_hip_ctx* c, flock_fs* fs, const std::vector<LcSlotRun>& slots,
+                                uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
+                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
+                                flock_f128* h_matrix_evals, uint64_t* h_nonces,
+                                flock_f128* h_claim, flock_f128* d_z_vec);
 
 int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
                              const uint8_t* d_z, uint32_t nu, size_t useful_bits, size_t n_rows,
@@ -2000,9 +2010,10 @@ int flock_hip_lincheck_prove(flock_hip_ctx* c, flock_fs* fs, const flock_csc* k,
                                int* d_err) {
         return csc_fold_impl(c, k, eq, alpha, sa, sb, comb, d_err);
     };
-    return lincheck_prove_impl(c, fs, fold, k->k_log, k->const_pin, d_z, nu, useful_bits, n_rows,
-                               z_skip, h_x_inner_rest, h_x_outer, grinding, h_rounds, h_z_partial,
-                               h_matrix_evals, h_nonces, h_claim, d_z_vec);
+    const std::vector<LcSlotRun> one{{fold, k->k_log, 0, k->const_pin, d_z, useful_bits, n_rows}};
+    return lincheck_prove_slots(c, fs, one, k->k_log, nu, z_skip, h_x_inner_rest, h_x_outer,
+                                grinding, h_rounds, h_z_partial, h_matrix_evals, h_nonces, h_claim,
+                                d_z_vec);
 }
 
 int flock_hip_lincheck_prove_keccak(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_z,
@@ -2020,31 +2031,49 @@ int flock_hip_lincheck_prove_keccak(flock_hip_ctx* c, flock_fs* fs, const uint8_
         return 0;
     };
     // K_LOG 16, USEFUL_BITS 42,560, const pin Z_CONST = 4096 (keccak.rs:436-451, 688-690)
-    return lincheck_prove_impl(c, fs, fold, 16, 4096, d_z, nu, 42560, n_rows, z_skip,
-                               h_x_inner_rest, h_x_outer, grinding, h_rounds, h_z_partial,
-                               h_matrix_evals, h_nonces, h_claim, d_z_vec);
+    const std::vector<LcSlotRun> one{{fold, 16, 0, 4096, d_z, 42560, n_rows}};
+    return lincheck_prove_slots(c, fs, one, 16, nu, z_skip, h_x_inner_rest, h_x_outer, grinding,
+                                h_rounds, h_z_partial, h_matrix_evals, h_nonces, h_claim, d_z_vec);
 }
 
-static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fold,
-                               uint32_t k_log, int64_t const_pin, const uint8_t* d_z, uint32_t nu,
-                               size_t useful_bits, size_t n_rows, flock_f128 z_skip,
-                               const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
-                               int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
-                               flock_f128 h_matrix_evals[2], uint64_t* h_nonces,
-                               flock_f128* h_claim, flock_f128* d_z_vec) {
+static int lincheck_prove_slots(flock_hip_ctx* c, flock_fs* fs, const std::vector<LcSlotRun>& slots,
+                                uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
+                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                int grinding, flock_f128* h_rounds, flock_f128 h_z_partial[64],
+                                flock_f128* h_matrix_evals, uint64_t* h_nonces,
+                                flock_f128* h_claim, flock_f128* d_z_vec) {
     const int ks = 6;
-    if (int rc = lc_fold_args(d_z, k_log, nu, useful_bits, n_rows)) return rc;
-    const uint32_t L = k_log - ks;
+    if (slots.empty() || slots.size() > FLOCK_MAX_TYPES || col_vars < 6 || col_vars > 24)
+        return fail(FLOCK_HIP_E_INVALID, "bad lincheck arguments (need 1..16 slots, 6 <= "
+                                         "col_vars <= 24)");
+    // every slot inside the 2^col_vars columns, the slots disjoint
+    std::vector<std::pair<uint64_t, uint64_t>> spans;
+    uint64_t pf_words = 1;
+    for (const LcSlotRun& sl : slots) {
+        if (int rc = lc_fold_args(sl.d_z, sl.k_log, nu, sl.useful_bits, sl.n_rows)) return rc;
+        if (sl.k_log > col_vars || sl.prefix >= (1ull << (col_vars - sl.k_log)) ||
+            sl.const_pin >= ((int64_t)1 << sl.k_log))
+            return fail(FLOCK_HIP_E_INVALID, "lincheck slot outside the union columns (need "
+                                             "k_log <= col_vars, prefix < 2^(col_vars - k_log), "
+                                             "const_pin < 2^k_log)");
+        spans.emplace_back(sl.prefix << sl.k_log, (sl.prefix + 1) << sl.k_log);
+        pf_words = std::max<uint64_t>(pf_words, lc_fold_partial_words(1ull << sl.k_log));
+    }
+    std::sort(spans.begin(), spans.end());
+    for (size_t i = 1; i < spans.size(); i++)
+        if (spans[i].first < spans[i - 1].second)
+            return fail(FLOCK_HIP_E_INVALID, "lincheck slots overlap");
+    const uint32_t L = col_vars - ks;
     if (!fs || (L && (!h_x_inner_rest || !h_rounds)) || !h_x_outer || !h_z_partial ||
         !h_matrix_evals || !h_claim || (grinding && !h_nonces))
         return fail(FLOCK_HIP_E_INVALID, "bad lincheck arguments");
-    const uint64_t K = 1ull << k_log;
+    const uint64_t K = 1ull << col_vars;
     const uint64_t n_outer = 1ull << nu;
-    // scratch (words): EQI | SA | SB | COMB | ZV (K each) | EO (2^nu) | ER (2^L) |
-    // EQS (build_eq) | PF (partial-fold rows) | RV (64 + L + nu) | ERR (1)
+    // scratch (words): EQI | SA | SB | COMB | ZV (K each; slot t's sa, sb at its column
+    // offset) | EO (2^nu) | ER (2^L) | EQS, EQS2 (build_eq) | PF (partial-fold rows of the
+    // largest slot) | RV (64 + L + nu) | ERR (1)
     const uint32_t vmax = nu > L ? nu : L;
     const uint64_t eqs_words = 2ull << ((vmax + 1) / 2);
-    const uint64_t pf_words = lc_fold_partial_words(K);
     const uint64_t words =
         5 * K + n_outer + (1ull << L) + 2 * eqs_words + pf_words + 64 + L + nu + 1;
     if (int rc = c->ensure_scratch(words * 16)) return rc;
@@ -2060,6 +2089,7 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     uint4* PF = EQS2 + eqs_words;
     uint4* RV = PF + pf_words;
     int* ERR = reinterpret_cast<int*>(RV + 64 + L + nu);
+    const bool whole = slots.size() == 1 && slots[0].k_log == col_vars;  // one slot, all columns
 
     int ni = 0;
     flock_fs_observe_label(fs, kLabelLincheck, sizeof(kLabelLincheck) - 1);
@@ -2067,7 +2097,9 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
     flock_f128 alpha;
     flock_fs_sample_f128(fs, &alpha);
-    // 2. quirky eq over the column index, then the CSC comb (union.rs:270-300)
+    // 2. per slot, the quirky eq over its inner coordinates x_inner_rest[..k_t - 6] and the
+    // circuit comb at its column offset, scaled by eq_prefix_weight(x_inner_rest[k_t - 6..],
+    // prefix_t) (union.rs:90-101, 270-300)
     std::vector<flock_f128> rv(64 + L + nu);
     {
         F w[64];
@@ -2077,25 +2109,42 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
         for (uint32_t j = 0; j < nu; j++) rv[64 + L + j] = h_x_outer[j];
     }
     HIP_TRY(hipMemcpyAsync(RV, rv.data(), rv.size() * 16, hipMemcpyHostToDevice, c->stream));
-    // 3. (on the aux stream, beside the comb) partial fold of the stripe at x_outer
-    // (union.rs:354-384); 3b. capture z_vec
+    // 3. (on the aux stream, beside the combs) the slots' partial folds of their stripes at
+    // x_outer into z_vec at their offsets (union.rs:354-384); 3b. capture z_vec
     if (int rc = c->fork()) return rc;
     LAUNCH(flock::launch_build_eq(RV + 64 + L, (int)nu, EO, EQS2, c->aux));
-    LAUNCH(flock::launch_lc_partial_fold(d_z, K, useful_bits, (n_rows + 7) / 8, EO, PF, ZV,
-                                         c->aux));
+    if (!whole) LAUNCH(flock::launch_zero_words(ZV, K, c->aux));
+    for (const LcSlotRun& sl : slots)
+        LAUNCH(flock::launch_lc_partial_fold(sl.d_z, 1ull << sl.k_log, sl.useful_bits,
+                                             (sl.n_rows + 7) / 8, EO, PF,
+                                             ZV + (sl.prefix << sl.k_log), c->aux));
     if (d_z_vec) HIP_TRY(hipMemcpyAsync(d_z_vec, ZV, K * 16, hipMemcpyDeviceToDevice, c->aux));
-    LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
-    LAUNCH(flock::launch_outer_product(RV, 6, ER, (int)L, EQI, c->stream));
-    if (int rc = fold(EQI, U(alpha), SA, SB, COMB, ERR)) {
-        hipStreamSynchronize(c->aux);
-        return rc;
+    if (!whole) LAUNCH(flock::launch_zero_words(COMB, K, c->stream));
+    for (const LcSlotRun& sl : slots) {
+        const uint32_t inner = sl.k_log - ks;
+        const uint64_t off = sl.prefix << sl.k_log;
+        LAUNCH(flock::launch_build_eq(RV + 64, (int)inner, ER, EQS, c->stream));
+        LAUNCH(flock::launch_outer_product(RV, 6, ER, (int)inner, EQI, c->stream));
+        if (int rc = sl.fold(EQI, U(alpha), SA + off, SB + off, COMB + off, ERR)) {
+            hipStreamSynchronize(c->aux);
+            return rc;
+        }
+        F wt{1, 0};
+        for (uint32_t j = 0; j < col_vars - sl.k_log; j++) {
+            const F x = fo(h_x_inner_rest[inner + j]);
+            wt = fmul(wt, ((sl.prefix >> j) & 1) ? x : fadd(x, F{1, 0}));
+        }
+        if (!feq(wt, F{1, 0}))
+            LAUNCH(flock::launch_lc_scale(COMB + off, 1ull << sl.k_log, U(to(wt)), c->stream));
     }
-    // 2b. constant-wire pin (union.rs:318-335)
-    if (const_pin >= 0) {
+    // 2b. each slot's constant-wire pin, in slot order (union.rs:318-335)
+    for (const LcSlotRun& sl : slots) {
+        if (sl.const_pin < 0) continue;
         if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, 1);
         flock_f128 beta;
         flock_fs_sample_f128(fs, &beta);
-        LAUNCH(flock::launch_lc_add_at(COMB, (uint64_t)const_pin, U(beta), c->stream));
+        LAUNCH(flock::launch_lc_add_at(COMB, (sl.prefix << sl.k_log) + (uint64_t)sl.const_pin,
+                                       U(beta), c->stream));
     }
     if (int rc = c->join()) return rc;
     // 4-5. product sumcheck over the top L bits (column_sumcheck_prove, lincheck.rs:1907-1980)
@@ -2124,15 +2173,11 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
             rr[t] = loop.rhos[t];
             if (grinding) h_nonces[ni++] = loop.nonces[t];
         }
-        // z_partial = the first 64 words of the bound z (lincheck.rs:1983-1990)
-        HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
     }
-    // 6-8. z_partial (read back above when L > 0), fresh skip point, w (lincheck.rs:1983-2005)
-    if (!L) {
-        HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-    }
+    // 6-8. z_partial = the first 64 words of the bound z, fresh skip point, w
+    // (lincheck.rs:1983-2005)
+    HIP_TRY(hipMemcpyAsync(h_z_partial, ZV, 64 * 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     flock_fs_observe_f128_slice(fs, h_z_partial, 64);
     if (grinding) h_nonces[ni++] = flock_fs_grind_pow(fs, (uint32_t)ks);
     flock_f128 r_skip;
@@ -2148,13 +2193,56 @@ static int lincheck_prove_impl(flock_hip_ctx* c, flock_fs* fs, const LcFold& fol
     }
     // 9. r_inner_rest = reversed round challenges (LSB-first)
     for (uint32_t j = 0; j < L; j++) h_claim[2 + j] = rr[L - 1 - j];
-    // 10. matrix evals with W_col = z_partial (x) eq(r_inner_rest) (union.rs:404-430)
+    // 10. per slot, the unscaled matrix evals with W_col = z_partial (x)
+    // eq(r_inner_rest[..k_t - 6]) (union.rs:404-430)
     HIP_TRY(hipMemcpyAsync(RV + 64, h_claim + 2, L * 16, hipMemcpyHostToDevice, c->stream));
-    LAUNCH(flock::launch_build_eq(RV + 64, (int)L, ER, EQS, c->stream));
-    LAUNCH(flock::launch_outer_product(ZV, 6, ER, (int)L, EQI, c->stream));
-    LAUNCH(flock::launch_dot2(EQI, SA, SB, K, c->partials, c->dout, c->stream));
-    if (int rc = c->fetch_pair(h_matrix_evals)) return rc;
+    for (size_t t = 0; t < slots.size(); t++) {
+        const uint32_t inner = slots[t].k_log - ks;
+        const uint64_t off = slots[t].prefix << slots[t].k_log;
+        LAUNCH(flock::launch_build_eq(RV + 64, (int)inner, ER, EQS, c->stream));
+        LAUNCH(flock::launch_outer_product(ZV, 6, ER, (int)inner, EQI, c->stream));
+        LAUNCH(flock::launch_dot2(EQI, SA + off, SB + off, 1ull << slots[t].k_log, c->partials,
+                                  c->dout, c->stream));
+        if (int rc = c->fetch_pair(h_matrix_evals + 2 * t)) return rc;
+    }
     return 0;
+}
+
+int flock_hip_lincheck_prove_union(flock_hip_ctx* c, flock_fs* fs, const flock_lc_slot* slots,
+                                   size_t n_slots, uint32_t col_vars, uint32_t nu,
+                                   flock_f128 z_skip, const flock_f128* h_x_inner_rest,
+                                   const flock_f128* h_x_outer, int grinding, flock_f128* h_rounds,
+                                   flock_f128 h_z_partial[64], flock_f128* h_matrix_evals,
+                                   uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec) {
+    CTX_CHECK(c);
+    if (!slots || n_slots == 0 || n_slots > FLOCK_MAX_TYPES)
+        return fail(FLOCK_HIP_E_INVALID, "bad union lincheck slots (need 1..16)");
+    std::vector<LcSlotRun> run;
+    for (size_t t = 0; t < n_slots; t++) {
+        const flock_lc_slot& s = slots[t];
+        if (s.circuit) {
+            const flock_csc* k = s.circuit;
+            if (int rc = csc_check(k)) return rc;
+            const LcFold fold = [c, k](const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,
+                                       uint4* comb, int* d_err) {
+                return csc_fold_impl(c, k, eq, alpha, sa, sb, comb, d_err);
+            };
+            run.push_back({fold, k->k_log, s.prefix, k->const_pin, s.d_z, (size_t)s.useful_bits,
+                           (size_t)s.n_rows});
+        } else {
+            if (int rc = c->ensure_kc()) return rc;
+            const LcFold fold = [c](const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,
+                                    uint4* comb, int*) {
+                LAUNCH(flock::launch_kc_fold(eq, c->kc_inv, c->kc_trow, c->kc_rcm, alpha, sa, sb,
+                                             comb, c->stream));
+                return 0;
+            };
+            run.push_back({fold, 16, s.prefix, 4096, s.d_z, 42560, (size_t)s.n_rows});
+        }
+    }
+    return lincheck_prove_slots(c, fs, run, col_vars, nu, z_skip, h_x_inner_rest, h_x_outer,
+                                grinding, h_rounds, h_z_partial, h_matrix_evals, h_nonces, h_claim,
+                                d_z_vec);
 }
 
 // ---- lincheck verifier (lincheck/union.rs:662-960 with MatrixAssertion, :467-628) ----

@@ -17,6 +17,7 @@
 // per kind for the life of the process; device copies per context on request.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <cstring>
@@ -410,7 +411,8 @@ flock_circuit* make(uint32_t kind) {
 
 std::mutex g_mu;
 flock_circuit* g_circ[3] = {nullptr, nullptr, nullptr};
-std::map<uint32_t, std::array<uint8_t, 32>> g_digest;  // kind * 64 + nu -> Registry::digest
+// (sorted kinds, nu) -> Registry::digest
+std::map<std::pair<std::vector<uint32_t>, uint32_t>, std::array<uint8_t, 32>> g_digest;
 
 }  // namespace
 
@@ -472,25 +474,36 @@ int flock_circuit_type_statement(const flock_circuit* c, flock_type_statement* o
     return 0;
 }
 
-int flock_circuit_registry_digest(uint32_t kind, uint32_t nu, uint8_t out[32]) {
-    const flock_circuit* c = nullptr;
-    if (int rc = flock_circuit_get(kind, &c)) return rc;
-    if (!out || nu > 63) return FLOCK_HIP_E_INVALID;
+int flock_circuit_union_digest(const uint32_t* kinds, size_t n_kinds, uint32_t nu,
+                               uint8_t out[32]) {
+    if (!kinds || !out || n_kinds == 0 || n_kinds > FLOCK_MAX_TYPES || nu > 63)
+        return FLOCK_HIP_E_INVALID;
+    std::vector<flock_type_statement> t(n_kinds);
+    for (size_t i = 0; i < n_kinds; i++) {
+        const flock_circuit* c = nullptr;
+        if (int rc = flock_circuit_get(kinds[i], &c)) return rc;
+        flock_circuit_type_statement(c, &t[i]);
+    }
+    // types of equal k_log are equal built-in types, so the digest depends on the multiset
+    std::vector<uint32_t> key(kinds, kinds + n_kinds);
+    std::sort(key.begin(), key.end());
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_digest.find(kind * 64 + nu);
+        auto it = g_digest.find({key, nu});
         if (it != g_digest.end()) {
             std::memcpy(out, it->second.data(), 32);
             return 0;
         }
     }
-    flock_type_statement t;
-    flock_circuit_type_statement(c, &t);
-    if (int rc = flock_registry_digest(&t, 1, nu, out)) return rc;
+    if (int rc = flock_registry_digest(t.data(), n_kinds, nu, out)) return rc;
     std::lock_guard<std::mutex> lk(g_mu);
-    std::array<uint8_t, 32>& d = g_digest[kind * 64 + nu];
+    std::array<uint8_t, 32>& d = g_digest[{key, nu}];
     std::memcpy(d.data(), out, 32);
     return 0;
+}
+
+int flock_circuit_registry_digest(uint32_t kind, uint32_t nu, uint8_t out[32]) {
+    return flock_circuit_union_digest(&kind, 1, nu, out);
 }
 
 int flock_hip_circuit_upload(flock_hip_ctx* ctx, uint32_t kind, flock_csc* out) {

@@ -140,6 +140,8 @@ hipError_t launch_lc_csc_fold(uint64_t n_cols, const uint32_t* a_ptr, const uint
                               uint64_t nnz_b, const uint4* eq, uint4 alpha, uint4* sa, uint4* sb,
                               uint4* comb, int* err, hipStream_t s);
 hipError_t launch_lc_add_at(uint4* v, uint64_t idx, uint4 beta, hipStream_t s);
+// v[0..n) *= w
+hipError_t launch_lc_scale(uint4* v, uint64_t n, uint4 w, hipStream_t s);
 // upper bound of the partial rows (gy) for any stripe count
 uint32_t lc_partial_fold_rows(uint64_t K);
 void lc_partial_fold_plan(uint64_t K, uint64_t n_stripes, uint32_t* spg, uint64_t* n_groups,

@@ -93,6 +93,17 @@ __global__ void k_lc_add_at(f128* __restrict__ v, uint64_t idx, f128 beta) {
     if (threadIdx.x == 0 && blockIdx.x == 0) v[idx] = f128_add(v[idx], beta);
 }
 
+// v[c] = w * v[c]: a union slot's comb scaled by its prefix weight (eq_prefix_weight,
+// lincheck/union.rs:90-101).
+__global__ __launch_bounds__(256) void k_lc_scale(f128* __restrict__ v, f128 w, uint64_t n) {
+    __shared__ f128 table[SHOUP_ENTRIES];
+    if (threadIdx.x < 32) shoup_build_row(table, w, threadIdx.x);
+    __syncthreads();
+    for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (uint64_t)gridDim.x * blockDim.x)
+        v[c] = shoup_mul(table, v[c]);
+}
+
 __global__ __launch_bounds__(256) void k_fill_i32_zero(int* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
@@ -400,6 +411,11 @@ hipError_t launch_kc_fold(const uint4* eq, const uint16_t* inv, const uint16_t* 
 
 hipError_t launch_lc_add_at(uint4* v, uint64_t idx, uint4 beta, hipStream_t s) {
     k_lc_add_at<<<1, 64, 0, s>>>(v, idx, beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_lc_scale(uint4* v, uint64_t n, uint4 w, hipStream_t s) {
+    k_lc_scale<<<grid_for(n, 1024), 256, 0, s>>>(v, w, n);
     return hipGetLastError();
 }
 

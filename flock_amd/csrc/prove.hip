@@ -72,30 +72,83 @@ double now_s() {
         .count();
 }
 
-// Sizes derived for one request; false for an unsupported shape.
+// One boolean hash type of a request and the request as a registry of such types (a single
+// prove_fast request is a registry of one).
+struct Part {
+    uint32_t kind;
+    uint64_t n_decl;
+    const void* inputs;
+    const flock_csc* circuit;
+};
+struct Job {
+    uint32_t nu;
+    std::vector<Part> parts;
+    const uint8_t* digest;
+    const uint8_t* domain;
+    size_t domain_len;
+    bool on_device;
+    uint64_t min_decl;  // 1 for a single-type request, 0 for a part of a union
+};
+
+Job job_of(const flock_prove_request* r) {
+    return Job{r->n_log, {{r->kind, r->n_decl, r->h_inputs, r->circuit}}, r->digest, r->domain,
+               r->domain_len, r->inputs_on_device != 0, 1};
+}
+
+Job job_of(const flock_union_request* r) {
+    Job j{r->n_log, {}, r->digest, r->domain, r->domain_len, r->inputs_on_device != 0, 0};
+    for (uint32_t i = 0; i < r->n_parts; i++)
+        j.parts.push_back({r->parts[i].kind, r->parts[i].n_decl, r->parts[i].inputs,
+                           r->parts[i].circuit});
+    return j;
+}
+
+// Sizes derived for one request; an error for an unsupported shape. Slot s holds part
+// reg.slots[s].type_index.
 struct Plan {
-    HashShape hs;
+    std::vector<HashShape> hs;  // per part
     flock_registry reg;
+    std::vector<uint64_t> counts;  // statement counts, slot order
     flock_union_sizes sz;
     flock_pcs_params pcs;
     int cap_depth;
     bool has_config;
     size_t cw_len, n_leaves;
-    uint64_t count, n_rows;
-    uint32_t m_zc;
+    uint32_t m_zc;                       // M_bool
+    uint64_t in_bytes, stripe_bytes;     // staged inputs, every slot's stripe
+    std::vector<uint64_t> in_off, stripe_off;  // per slot
 };
 
-int plan_request(const flock_prove_request* r, Plan* p) {
-    if (!r || !shape_of(r->kind, &p->hs) || r->n_log < 3 || r->n_log > 24 || r->n_decl == 0 ||
-        r->n_decl > (1ull << r->n_log) || !r->h_inputs || (r->domain_len && !r->domain))
+int plan_job(const Job& j, Plan* p) {
+    const size_t n = j.parts.size();
+    if (n == 0 || n > FLOCK_MAX_TYPES || j.nu < 3 || j.nu > 24 || (j.domain_len && !j.domain))
         return FLOCK_HIP_E_INVALID;
-    const flock_table_type t = {p->hs.k_log, 0, p->hs.useful_bits};
-    if (int rc = flock_registry_new(&t, 1, r->n_log, &p->reg)) return rc;
+    std::vector<flock_table_type> types(n);
+    p->hs.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        const Part& t = j.parts[i];
+        if (!shape_of(t.kind, &p->hs[i]) || t.n_decl < j.min_decl || t.n_decl > (1ull << j.nu) ||
+            (t.n_decl && !t.inputs))
+            return FLOCK_HIP_E_INVALID;
+        types[i] = {p->hs[i].k_log, 0, p->hs[i].useful_bits};
+    }
+    if (int rc = flock_registry_new(types.data(), n, j.nu, &p->reg)) return rc;
     // Keccak batches always count the full capacity: padding rows run keccak_f(0)
     // (keccak.rs KeccakSetup); the compression circuits count the declared instances.
-    p->count = r->kind == FLOCK_HASH_KECCAK ? (1ull << r->n_log) : r->n_decl;
-    p->n_rows = p->count;
-    if (int rc = flock_union_sizes_compute(&p->reg, &p->count, 0, 6, &p->sz)) return rc;
+    p->counts.resize(n);
+    p->in_off.resize(n);
+    p->stripe_off.resize(n);
+    p->in_bytes = p->stripe_bytes = 0;
+    for (size_t s = 0; s < n; s++) {
+        const uint32_t ti = p->reg.slots[s].type_index;
+        const Part& t = j.parts[ti];
+        p->counts[s] = t.kind == FLOCK_HASH_KECCAK ? (1ull << j.nu) : t.n_decl;
+        p->in_off[s] = p->in_bytes;
+        if (!j.on_device) p->in_bytes += (t.n_decl * p->hs[ti].input_bytes + 255) & ~255ull;
+        p->stripe_off[s] = p->stripe_bytes;
+        p->stripe_bytes += 1ull << (j.nu + p->hs[ti].k_log - 3);
+    }
+    if (int rc = flock_union_sizes_compute(&p->reg, p->counts.data(), 0, 6, &p->sz)) return rc;
     // log_batch_size is the Ligerito config's initial_k (ligerito.rs:564-574: 6, but 4 at
     // m = 28 and 5 at m = 29), and commit_lanes depends on it (union.rs:386-390)
     int lbs = flock_lig_initial_k(p->sz.dense_m);
@@ -108,26 +161,34 @@ int plan_request(const flock_prove_request* r, Plan* p) {
         p->cap_depth = 0;
     }
     if (lbs != 6)
-        if (int rc = flock_union_sizes_compute(&p->reg, &p->count, 0, (uint32_t)lbs, &p->sz)) return rc;
+        if (int rc = flock_union_sizes_compute(&p->reg, p->counts.data(), 0, (uint32_t)lbs,
+                                               &p->sz))
+            return rc;
     p->pcs = {p->sz.dense_m, 1, (uint32_t)lbs, (uint32_t)p->sz.commit_lanes, 0};
     uint32_t k_code = 0;
     if (int rc = flock_hip_pcs_sizes(&p->pcs, nullptr, &p->cw_len, &p->n_leaves, &k_code, nullptr))
         return rc;
-    p->m_zc = r->n_log + p->hs.k_log;
+    p->m_zc = p->reg.m_bool;
     return 0;
 }
 
-// Device bytes of one request: witness, stripe, dense stack, codeword, tree, the zerocheck
-// driver's scratch (2N + 2h + 2q + h + q + N/8 = 4.375 N words for N = 2^(m-6) rows,
-// api.hip flock_hip_zerocheck_prove), the lincheck's z_vec, and the opening's workspace
-// (about 9.5 words per committed word: W, q copy, the ping-pong halves, the eq bases, and
-// the F256 ladder's four half-size tables of 32 B words).
-uint64_t plan_footprint(const Plan& p, uint32_t n_log) {
-    const uint64_t words = 1ull << (n_log + p.hs.k_log - 7);  // F128 words of z (a, b alike)
+int plan_request(const flock_prove_request* r, Plan* p) {
+    if (!r) return FLOCK_HIP_E_INVALID;
+    return plan_job(job_of(r), p);
+}
+
+// Device bytes of one request: witness (z, a, b over the 2^M_bool-bit boolean region), the
+// slots' stripes, dense stack, codeword, tree, the zerocheck driver's scratch (2N + 2h + 2q +
+// h + q + N/8 = 4.375 N words for N = 2^(m-6) rows, api.hip flock_hip_zerocheck_prove), the
+// lincheck's z_vec, and the opening's workspace (about 9.5 words per committed word: W, q
+// copy, the ping-pong halves, the eq bases, and the F256 ladder's four half-size tables of
+// 32 B words).
+uint64_t plan_footprint(const Plan& p) {
+    const uint64_t words = 1ull << (p.m_zc - 7);  // F128 words of z (a, b alike)
     const uint64_t zc_rows = 1ull << (p.m_zc - 6);
-    return 3 * 16 * words + 16 * words /* stripe */ + 16 * p.sz.committed_words +
+    return 3 * 16 * words + p.in_bytes + p.stripe_bytes + 16 * p.sz.committed_words +
            16 * (uint64_t)p.cw_len + 32 * (2 * (uint64_t)p.n_leaves) + 16 * zc_rows * 35 / 8 +
-           16 * (1ull << p.hs.k_log) + 16 * p.sz.committed_words * 19 / 2;
+           16 * (1ull << (p.m_zc - p.reg.nu)) + 16 * p.sz.committed_words * 19 / 2;
 }
 
 }  // namespace
@@ -197,7 +258,7 @@ int flock_hip_prove_footprint(const flock_prove_request* req, uint64_t* bytes) {
     Plan p;
     if (!bytes) return FLOCK_HIP_E_INVALID;
     if (int rc = plan_request(req, &p)) return rc;
-    *bytes = plan_footprint(p, req->n_log);
+    *bytes = plan_footprint(p);
     return 0;
 }
 
@@ -235,7 +296,7 @@ uint8_t* dup_bytes(const uint8_t* p, size_t n) {
 }
 
 // The shared body: the prove core, and when `out` is set the opening and the wire bytes.
-int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove_result* res,
+int prove_impl(flock_hip_prover* pr, const Job& job, flock_prove_result* res,
                flock_proof_bytes* out) {
     if (!pr || !pr->ctx || !res) return FLOCK_HIP_E_INVALID;
     std::memset(res, 0, sizeof(*res));
@@ -247,41 +308,48 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
         t_mark = t;
     };
     Plan p;
-    if (int rc = plan_request(req, &p)) return res->status = rc;
+    if (int rc = plan_job(job, &p)) return res->status = rc;
     if (out && !p.has_config) return res->status = FLOCK_HIP_E_INVALID;
     flock_hip_ctx* c = pr->ctx;
-    const uint32_t nl = req->n_log, kl = p.hs.k_log;
-    const uint64_t n = 1ull << nl;
-    const uint64_t zwords = n << (kl - 7);      // F128 words of z (and of a, b)
-    const uint64_t stripe_bytes = n << (kl - 3);
+    const uint32_t nl = job.nu, n_slots = (uint32_t)job.parts.size();
+    const uint32_t col_vars = p.m_zc - nl;
+    const uint64_t zwords = 1ull << (p.m_zc - 7);  // F128 words of z (and of a, b)
     int rc = 0;
-    const size_t need[B_COUNT] = {req->inputs_on_device ? 0 : (size_t)(req->n_decl * p.hs.input_bytes),
+    const size_t need[B_COUNT] = {(size_t)(p.in_bytes ? p.in_bytes : 16),
                                   (size_t)(16 * zwords),
                                   (size_t)(16 * zwords),
                                   (size_t)(16 * zwords),
-                                  (size_t)stripe_bytes,
+                                  (size_t)p.stripe_bytes,
                                   (size_t)(16 * p.sz.committed_words),
                                   (size_t)(16 * p.cw_len),
                                   (size_t)(32 * (2 * p.n_leaves - 1)),
-                                  (size_t)(out ? 16ull << kl : 16)};
+                                  (size_t)(out ? 16ull << col_vars : 16)};
     for (int i = 0; i < B_COUNT && !rc; i++) rc = pr->buf[i].ensure(c, need[i]);
     if (!rc && out && !pr->ws) rc = flock_open_ws_create(&pr->ws);
-    // the request's circuit, else the built-in one; its digest, else the built-in registry's
-    const flock_csc* circuit = req->circuit;
-    if (!rc && !circuit && req->kind != FLOCK_HASH_KECCAK) {
-        flock_csc& bc = pr->circ[req->kind];
-        if (!bc.a_col_ptr) rc = flock_hip_circuit_upload(c, req->kind, &bc);
-        circuit = &bc;
+    // each slot's circuit: the request's, else the built-in one
+    std::vector<const flock_csc*> circuit(n_slots, nullptr);
+    for (uint32_t s = 0; s < n_slots && !rc; s++) {
+        const Part& t = job.parts[p.reg.slots[s].type_index];
+        circuit[s] = t.circuit;
+        if (!circuit[s] && t.kind != FLOCK_HASH_KECCAK) {
+            flock_csc& bc = pr->circ[t.kind];
+            if (!bc.a_col_ptr) rc = flock_hip_circuit_upload(c, t.kind, &bc);
+            circuit[s] = &bc;
+        }
     }
+    // the request's registry digest, else the built-in types' Registry::digest at nu
     uint8_t digest[32];
     if (!rc) {
-        if (req->digest)
-            std::memcpy(digest, req->digest, 32);
-        else
-            rc = flock_circuit_registry_digest(req->kind, nl, digest);
+        if (job.digest) {
+            std::memcpy(digest, job.digest, 32);
+        } else {
+            std::vector<uint32_t> kinds(n_slots);
+            for (uint32_t i = 0; i < n_slots; i++) kinds[i] = job.parts[i].kind;
+            rc = flock_circuit_union_digest(kinds.data(), n_slots, nl, digest);
+        }
     }
     if (rc) return res->status = rc;
-    void* d_in = req->inputs_on_device ? const_cast<void*>(req->h_inputs) : pr->buf[B_IN].p;
+    auto* d_in = static_cast<uint8_t*>(pr->buf[B_IN].p);
     auto* z = static_cast<flock_f128*>(pr->buf[B_Z].p);
     auto* a = static_cast<flock_f128*>(pr->buf[B_A].p);
     auto* b = static_cast<flock_f128*>(pr->buf[B_B].p);
@@ -291,27 +359,58 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     auto* tree = static_cast<uint8_t*>(pr->buf[B_TREE].p);
     auto* zvec = out ? static_cast<flock_f128*>(pr->buf[B_ZVEC].p) : nullptr;
     hipStream_t st = static_cast<hipStream_t>(flock_hip_ctx_stream(c));
-    if (!req->inputs_on_device &&
-        hipMemcpyAsync(d_in, req->h_inputs, need[B_IN], hipMemcpyHostToDevice, st) != hipSuccess)
-        return res->status = FLOCK_HIP_E_HIP;
-    // 1. witness (generate_witness_batch_major{,_partial})
-    switch (req->kind) {
-        case FLOCK_HASH_SHA256:
-            rc = flock_hip_witness_sha256(c, static_cast<const uint32_t*>(d_in), req->n_decl, nl,
-                                          z, a, b, stripe);
-            break;
-        case FLOCK_HASH_BLAKE3:
-            rc = flock_hip_witness_blake3(c, static_cast<const uint32_t*>(d_in), req->n_decl, nl,
-                                          z, a, b, stripe);
-            break;
-        default:
-            rc = flock_hip_witness_keccak(c, static_cast<const uint64_t*>(d_in), req->n_decl, nl,
-                                          z, a, b, stripe);
+    // 1. witness (generate_witness_batch_major{,_partial}) of each slot at its offset in the
+    // packed boolean region; the columns no slot covers are zero
+    {
+        std::vector<std::pair<uint64_t, uint64_t>> spans;  // in F128 words
+        for (uint32_t s = 0; s < n_slots; s++) {
+            const uint64_t w0 = p.reg.slots[s].offset >> 7;
+            spans.emplace_back(w0, w0 + (1ull << (p.reg.slots[s].m_slot - 7)));
+        }
+        std::sort(spans.begin(), spans.end());
+        uint64_t at = 0;
+        for (size_t i = 0; i <= spans.size() && !rc; i++) {
+            const uint64_t end = i < spans.size() ? spans[i].first : zwords;
+            if (end > at)
+                for (flock_f128* v : {z, a, b})
+                    if (flock::launch_zero_words(reinterpret_cast<uint4*>(v + at), end - at, st) !=
+                        hipSuccess)
+                        rc = FLOCK_HIP_E_HIP;
+            if (i < spans.size()) at = std::max(at, spans[i].second);
+        }
+    }
+    for (uint32_t s = 0; s < n_slots && !rc; s++) {
+        const Part& t = job.parts[p.reg.slots[s].type_index];
+        const HashShape& hs = p.hs[p.reg.slots[s].type_index];
+        const void* in = t.inputs;
+        if (!job.on_device && t.n_decl) {
+            in = d_in + p.in_off[s];
+            if (hipMemcpyAsync(d_in + p.in_off[s], t.inputs, t.n_decl * hs.input_bytes,
+                               hipMemcpyHostToDevice, st) != hipSuccess) {
+                rc = FLOCK_HIP_E_HIP;
+                break;
+            }
+        }
+        const uint64_t w0 = p.reg.slots[s].offset >> 7;
+        uint8_t* sp = stripe + p.stripe_off[s];
+        switch (t.kind) {
+            case FLOCK_HASH_SHA256:
+                rc = flock_hip_witness_sha256(c, static_cast<const uint32_t*>(in), t.n_decl, nl,
+                                              z + w0, a + w0, b + w0, sp);
+                break;
+            case FLOCK_HASH_BLAKE3:
+                rc = flock_hip_witness_blake3(c, static_cast<const uint32_t*>(in), t.n_decl, nl,
+                                              z + w0, a + w0, b + w0, sp);
+                break;
+            default:
+                rc = flock_hip_witness_keccak(c, static_cast<const uint64_t*>(in), t.n_decl, nl,
+                                              z + w0, a + w0, b + w0, sp);
+        }
     }
     // 2. dense stack and 3. commit (prover.rs:979-983): lane-major for an integer lane count,
     // the interleaved commit at full width; the cap at the L0 cap depth
     std::vector<uint8_t> cap(32ull << p.cap_depth);
-    if (!rc) rc = flock_hip_union_compact(c, &p.reg, &p.count, 0, z, q);
+    if (!rc) rc = flock_hip_union_compact(c, &p.reg, p.counts.data(), 0, z, q);
     if (!rc) {
         if (p.pcs.num_lanes)
             rc = flock_hip_commit_lane_major(c, q, p.sz.committed_words, &p.pcs, cw, tree,
@@ -327,13 +426,13 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
     if (rc) return res->status = rc;
     stage(0);
     // 4. statement binding (union.rs:619-628), zerocheck with s_hat_v_c (prover.rs:1094-1103)
-    flock_fs* fs = flock_fs_new(req->domain, req->domain_len);
+    flock_fs* fs = flock_fs_new(job.domain, job.domain_len);
     if (!fs) return res->status = FLOCK_HIP_E_NOMEM;
     const uint32_t n_mlv = p.m_zc - 6;
     std::vector<flock_f128> r1(128), ml(2 * n_mlv), claim(1 + 2 * n_mlv), shv(2 * 128);
     std::vector<uint64_t> nonces(2 + n_mlv);
     flock_f128 finals[3];
-    rc = flock_union_bind_statement(fs, digest, &p.count, 1, cap.data(), cap.size());
+    rc = flock_union_bind_statement(fs, digest, p.counts.data(), n_slots, cap.data(), cap.size());
     if (!rc)
         rc = flock_hip_zerocheck_prove(c, fs, reinterpret_cast<const uint8_t*>(a),
                                        reinterpret_cast<const uint8_t*>(b),
@@ -341,43 +440,49 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
                                        ml.data(), finals, nonces.data(), claim.data(),
                                        shv.data() + 128);
     stage(1);
-    // 5. x_ab and the union lincheck (prover.rs:1140-1160)
-    const uint32_t L = kl - 6;
+    // 5. x_ab and the union lincheck over the slots (prover.rs:1140-1160)
+    const uint32_t L = col_vars - 6;
+    std::vector<flock_lc_slot> lc(n_slots);
+    uint32_t n_pins = 0;
+    for (uint32_t s = 0; s < n_slots; s++) {
+        const HashShape& hs = p.hs[p.reg.slots[s].type_index];
+        lc[s] = {circuit[s], stripe + p.stripe_off[s], p.reg.slots[s].prefix, hs.useful_bits,
+                 p.counts[s]};
+        n_pins += circuit[s] ? (circuit[s]->const_pin >= 0) : 1;
+    }
     std::vector<flock_f128> x_rest(L ? L : 1), x_outer(nl), rounds(2 * (L ? L : 1)),
-        lclaim(2 + L);
-    std::vector<uint64_t> lnonces(L + 3);
-    flock_f128 zp[64], mev[2];
+        lclaim(2 + L), mev(2 * n_slots);
+    std::vector<uint64_t> lnonces(L + 2 + n_pins);
+    flock_f128 zp[64];
     if (!rc) rc = flock_union_x_ab_from_mlv(&p.reg, claim.data() + 1, n_mlv, x_rest.data(),
                                             x_outer.data());
-    if (!rc) {
-        if (req->kind == FLOCK_HASH_KECCAK)
-            rc = flock_hip_lincheck_prove_keccak(c, fs, stripe, nl, p.n_rows, claim[0],
-                                                 x_rest.data(), x_outer.data(), 1, rounds.data(),
-                                                 zp, mev, lnonces.data(), lclaim.data(), zvec);
-        else
-            rc = flock_hip_lincheck_prove(c, fs, circuit, stripe, nl, p.hs.useful_bits,
-                                          p.n_rows, claim[0], x_rest.data(), x_outer.data(), 1,
-                                          rounds.data(), zp, mev, lnonces.data(), lclaim.data(),
-                                          zvec);
-    }
+    if (!rc)
+        rc = flock_hip_lincheck_prove_union(c, fs, lc.data(), n_slots, col_vars, nl, claim[0],
+                                            x_rest.data(), x_outer.data(), 1, rounds.data(), zp,
+                                            mev.data(), lnonces.data(), lclaim.data(), zvec);
     stage(2);
     if (!rc && out) {
         // 6. s_hat_v_ab from the lincheck's z_vec (prover.rs:1186-1200), the claim points
         // (union.rs:564-600 as quirky x_outer_full, prover.rs:43-48) and the jagged heights
-        // (union.rs:293-310); then open_batch_merged (prover.rs:1412-1426)
+        // (union.rs:293-310: each slot's used columns at its count); then open_batch_merged
+        // (prover.rs:1412-1426)
         const flock_f128* r_inner_rest = lclaim.data() + 2;
-        rc = flock_hip_s_hat_v_from_z_vec(c, pr->ws, zvec, 1ull << kl, r_inner_rest + 1, L - 1,
-                                          shv.data());
+        rc = flock_hip_s_hat_v_from_z_vec(c, pr->ws, zvec, 1ull << col_vars, r_inner_rest + 1,
+                                          L - 1, shv.data());
         const size_t x_len = p.m_zc - 6;
         std::vector<flock_f128> xs(2 * x_len);
         xs[0] = r_inner_rest[0];
         for (uint32_t i = 0; i < nl; i++) xs[1 + i] = x_outer[i];
         for (uint32_t i = 1; i < L; i++) xs[nl + i] = r_inner_rest[i];
         for (size_t i = 0; i < x_len; i++) xs[x_len + i] = claim[1 + n_mlv + i];
-        const uint32_t col_log = kl - 7;
-        const uint64_t used = std::min<uint64_t>((p.hs.useful_bits + 127) / 128, 1ull << col_log);
-        std::vector<uint64_t> heights(1ull << col_log, 0);
-        for (uint64_t y = 0; y < used; y++) heights[y] = p.count;
+        std::vector<uint64_t> heights(1ull << (col_vars - 7), 0);
+        for (uint32_t s = 0; s < n_slots; s++) {
+            const HashShape& hs = p.hs[p.reg.slots[s].type_index];
+            const uint64_t used =
+                std::min<uint64_t>((hs.useful_bits + 127) / 128, 1ull << (hs.k_log - 7));
+            const uint64_t c_off = p.reg.slots[s].offset >> (7 + nl);
+            for (uint64_t y = 0; y < used; y++) heights[c_off + y] = p.counts[s];
+        }
         uint8_t* ob = nullptr;
         size_t on = 0;
         if (!rc)
@@ -386,7 +491,8 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
                                        nullptr);
         if (!rc) {
             // R1csProofMergedLigerito (proof.rs:42-47): ZerocheckProof (zerocheck.rs:427-447),
-            // LincheckProof (lincheck.rs:520-549), MergedOpenProof
+            // LincheckProof (lincheck.rs:520-549, one matrix-eval pair per boolean type),
+            // MergedOpenProof
             BinW w;
             w.vec_f(r1.data(), 64);
             w.vec_f(r1.data() + 64, 64);
@@ -395,7 +501,7 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
             w.vec_u64(nonces.data(), nonces.size());
             w.vec_pairs(rounds.data(), L);
             w.vec_f(zp, 64);
-            w.vec_pairs(mev, 1);
+            w.vec_pairs(mev.data(), n_slots);
             w.vec_u64(lnonces.data(), lnonces.size());
             w.b.insert(w.b.end(), ob, ob + on);
             out->proof = dup_bytes(w.b.data(), w.b.size());
@@ -413,7 +519,7 @@ int prove_impl(flock_hip_prover* pr, const flock_prove_request* req, flock_prove
         flock_fs_digest(fs, res->transcript);
         res->zc_z = claim[0];
         res->lc_w = lclaim[1];
-        res->hashes = req->n_decl;
+        for (const Part& t : job.parts) res->hashes += t.n_decl;
     }
     flock_fs_free(fs);
     return res->status = rc;
@@ -433,13 +539,42 @@ void flock_hip_prover_release(flock_hip_prover* pr) {
 
 int flock_hip_prove_core(flock_hip_prover* pr, const flock_prove_request* req,
                          flock_prove_result* res) {
-    return prove_impl(pr, req, res, nullptr);
+    if (!req) return FLOCK_HIP_E_INVALID;
+    return prove_impl(pr, job_of(req), res, nullptr);
 }
 
 int flock_hip_prove_fast(flock_hip_prover* pr, const flock_prove_request* req,
                          flock_prove_result* res, flock_proof_bytes* out) {
-    if (!out) return FLOCK_HIP_E_INVALID;
-    return prove_impl(pr, req, res, out);
+    if (!out || !req) return FLOCK_HIP_E_INVALID;
+    return prove_impl(pr, job_of(req), res, out);
+}
+
+int flock_union_request_info(const flock_union_request* req, flock_union_info* out) {
+    if (!req || !out || !req->parts || req->n_parts == 0 || req->n_parts > FLOCK_MAX_TYPES)
+        return FLOCK_HIP_E_INVALID;
+    const Job job = job_of(req);
+    Plan p;
+    if (int rc = plan_job(job, &p)) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->n_slots = req->n_parts;
+    out->m_bool = p.m_zc;
+    out->dense_m = p.sz.dense_m;
+    out->log_batch_size = p.pcs.log_batch_size;
+    out->num_lanes = p.pcs.num_lanes;
+    out->has_opening = p.has_config ? 1 : 0;
+    out->device_bytes = plan_footprint(p);
+    for (uint32_t s = 0; s < req->n_parts; s++) {
+        out->slot_part[s] = p.reg.slots[s].type_index;
+        out->counts[s] = p.counts[s];
+    }
+    return 0;
+}
+
+int flock_hip_prove_fast_union(flock_hip_prover* pr, const flock_union_request* req,
+                               flock_prove_result* res, flock_proof_bytes* out) {
+    if (!out || !req || !req->parts || req->n_parts == 0 || req->n_parts > FLOCK_MAX_TYPES)
+        return FLOCK_HIP_E_INVALID;
+    return prove_impl(pr, job_of(req), res, out);
 }
 
 int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
@@ -453,7 +588,7 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
         Plan p;
         if (int rc = plan_request(&reqs[i], &p)) return rc;
         if (i && reqs[i].arrival_s < reqs[i - 1].arrival_s) return FLOCK_HIP_E_INVALID;
-        fp[i] = plan_footprint(p, reqs[i].n_log);
+        fp[i] = plan_footprint(p);
     }
     // Admission counts what every worker holds: a running worker reserves max(held, its
     // request's footprint), an idle one what its grow-only buffers keep. A request that does

@@ -143,6 +143,41 @@ class ProveResult(ctypes.Structure):
                 ("reserved", ctypes.c_int)]
 
 
+class UnionPart(ctypes.Structure):
+    """flock_union_part: one hash type of a mixed request."""
+
+    _fields_ = [("kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("n_decl", ctypes.c_uint64), ("inputs", ctypes.c_void_p),
+                ("circuit", ctypes.c_void_p)]
+
+
+class UnionRequest(ctypes.Structure):
+    """flock_union_request: prove_fast over a registry of several boolean hash types."""
+
+    _fields_ = [("n_log", ctypes.c_uint32), ("n_parts", ctypes.c_uint32),
+                ("parts", ctypes.POINTER(UnionPart)), ("digest", ctypes.c_char_p),
+                ("domain", ctypes.c_char_p), ("domain_len", ctypes.c_size_t),
+                ("inputs_on_device", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class UnionInfo(ctypes.Structure):
+    """flock_union_info: slot order, counts and PCS parameters of a mixed request."""
+
+    _fields_ = [("n_slots", ctypes.c_uint32), ("m_bool", ctypes.c_uint32),
+                ("dense_m", ctypes.c_uint32), ("log_batch_size", ctypes.c_uint32),
+                ("num_lanes", ctypes.c_uint64), ("has_opening", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64),
+                ("slot_part", ctypes.c_uint32 * 16), ("counts", ctypes.c_uint64 * 16)]
+
+
+class LcSlot(ctypes.Structure):
+    """flock_lc_slot: one boolean slot of the union lincheck."""
+
+    _fields_ = [("circuit", ctypes.c_void_p), ("d_z", ctypes.c_void_p),
+                ("prefix", ctypes.c_uint64), ("useful_bits", ctypes.c_uint64),
+                ("n_rows", ctypes.c_uint64)]
+
+
 class ProofBytes(ctypes.Structure):
     """flock_proof_bytes: the wire bytes of one prove_fast."""
 
@@ -204,6 +239,7 @@ def load():
         "flock_circuit_csc": (I, [P, U32, ctypes.POINTER(P), ctypes.POINTER(P)]),
         "flock_circuit_type_statement": (I, [P, P]),
         "flock_circuit_registry_digest": (I, [U32, U32, P]),
+        "flock_circuit_union_digest": (I, [P, S, U32, P]),
         "flock_hip_circuit_upload": (I, [P, U32, P]),
         "flock_hip_circuit_release": (None, [P, P]),
         "flock_blake3_hash": (None, [P, S, P]),
@@ -237,6 +273,8 @@ def load():
         "flock_hip_lc_keccak_fold": (I, [P, P, F128, P, P, P]),
         "flock_hip_lincheck_prove_keccak": (I, [P, P, P, U32, S, F128, P, P, I, P, P, P, P, P,
                                                 P]),
+        "flock_hip_lincheck_prove_union": (I, [P, P, P, S, U32, U32, F128, P, P, I, P, P, P, P,
+                                               P, P]),
         "flock_fs_new": (P, [ctypes.c_char_p, S]),
         "flock_fs_free": (None, [P]),
         "flock_fs_observe_label": (None, [P, ctypes.c_char_p, S]),
@@ -275,6 +313,9 @@ def load():
                                       ctypes.POINTER(S), P]),
         "flock_hip_prove_fast": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult),
                                      ctypes.POINTER(ProofBytes)]),
+        "flock_union_request_info": (I, [ctypes.POINTER(UnionRequest), ctypes.POINTER(UnionInfo)]),
+        "flock_hip_prove_fast_union": (I, [P, ctypes.POINTER(UnionRequest),
+                                           ctypes.POINTER(ProveResult), ctypes.POINTER(ProofBytes)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -627,6 +668,34 @@ class Context:
         out["rounds"] = out["rounds"][:2 * 10]
         return out
 
+    def lincheck_prove_union(self, fs: "FsChallenger", slots, col_vars: int, nu: int,
+                             z_skip: int, x_inner_rest, x_outer, grinding: bool,
+                             z_vec: DeviceBuffer | None = None) -> dict:
+        """The union lincheck over several boolean slots (lincheck/union.rs:203-430). slots:
+        [(circuit Csc or None for the Keccak walker, stripe DeviceBuffer, prefix, useful_bits,
+        n_rows)] in slot order; matrix_evals holds one pair per slot."""
+        arr = (LcSlot * len(slots))()
+        pins = 0
+        for i, (circ, z, prefix, ub, n_rows) in enumerate(slots):
+            arr[i].circuit = ctypes.addressof(circ) if circ is not None else None
+            arr[i].d_z, arr[i].prefix, arr[i].useful_bits, arr[i].n_rows = z.ptr, prefix, ub, n_rows
+            pins += 1 if circ is None or circ.const_pin >= 0 else 0
+        L = col_vars - 6
+        out = dict(rounds=np.zeros((max(2 * L, 1), 2), dtype=np.uint64),
+                   z_partial=np.zeros((64, 2), dtype=np.uint64),
+                   matrix_evals=np.zeros((2 * len(slots), 2), dtype=np.uint64),
+                   nonces=np.zeros(L + 2 + pins, dtype=np.uint64),
+                   claim=np.zeros((2 + L, 2), dtype=np.uint64))
+        xr = _ints_arr(x_inner_rest)
+        xo = _ints_arr(x_outer)
+        _check(load().flock_hip_lincheck_prove_union(
+            self.h, fs.h, arr, len(slots), col_vars, nu, f128(z_skip), _np_ptr(xr), _np_ptr(xo),
+            1 if grinding else 0, _np_ptr(out["rounds"]), _np_ptr(out["z_partial"]),
+            _np_ptr(out["matrix_evals"]), _np_ptr(out["nonces"]), _np_ptr(out["claim"]),
+            z_vec.ptr if z_vec else None))
+        out["rounds"] = out["rounds"][:2 * L]
+        return out
+
     def fold_1b_rows(self, rows: DeviceBuffer, n_rows: int, t: DeviceBuffer) -> np.ndarray:
         """fold_1b_rows (pcs/ring_switch.rs:182-230): out[j] = sum_y t[y] bit_j(rows[y])."""
         out = np.zeros((128, 2), dtype=np.uint64)
@@ -712,6 +781,14 @@ def circuit_registry_digest(kind: int, nu: int) -> bytes:
     """Registry::digest of the one-type registry of a built-in circuit at nu."""
     out = ctypes.create_string_buffer(32)
     _check(load().flock_circuit_registry_digest(kind, nu, out))
+    return out.raw
+
+
+def circuit_union_digest(kinds, nu: int) -> bytes:
+    """Registry::digest of the registry of several built-in types at nu."""
+    arr = (ctypes.c_uint32 * len(kinds))(*kinds)
+    out = ctypes.create_string_buffer(32)
+    _check(load().flock_circuit_union_digest(arr, len(kinds), nu, out))
     return out.raw
 
 
@@ -954,6 +1031,51 @@ def prove_fast(prover: Prover, req: ProveRequest):
     out = ProofBytes()
     _check(load().flock_hip_prove_fast(prover.h, ctypes.byref(req), ctypes.byref(res),
                                        ctypes.byref(out)))
+    proof = ctypes.string_at(out.proof, out.proof_len)
+    cap = ctypes.string_at(out.cap, out.cap_len)
+    load().flock_hip_free(ctypes.cast(out.proof, ctypes.c_void_p))
+    load().flock_hip_free(ctypes.cast(out.cap, ctypes.c_void_p))
+    return res, proof, cap, to_int(out.claim_ab), to_int(out.claim_c)
+
+
+def union_request(parts, n_log: int, domain: bytes = b"flock-readme-bench-v0",
+                  digest: bytes | None = None):
+    """A mixed request from [(kind, inputs ndarray, n_decl or None)] on the host. Returns
+    (request, keep)."""
+    arr = (UnionPart * len(parts))()
+    keep = [domain, digest, arr]
+    for i, part in enumerate(parts):
+        kind, x = part[0], part[1]
+        n = part[2] if len(part) > 2 and part[2] is not None else len(x)
+        x = np.ascontiguousarray(x)
+        keep.append(x)
+        arr[i].kind, arr[i].n_decl = kind, n
+        arr[i].inputs = x.ctypes.data if n else None
+    req = UnionRequest()
+    req.n_log, req.n_parts, req.parts = n_log, len(parts), arr
+    req.digest, req.domain, req.domain_len = digest, domain, len(domain)
+    return req, keep
+
+
+def union_info(req: UnionRequest) -> dict:
+    """flock_union_request_info: slot order, counts (slot order), dense_m, log_batch_size,
+    num_lanes, m_bool, has_opening, device_bytes."""
+    info = UnionInfo()
+    _check(load().flock_union_request_info(ctypes.byref(req), ctypes.byref(info)))
+    n = info.n_slots
+    return dict(slot_part=list(info.slot_part[:n]), counts=list(info.counts[:n]),
+                dense_m=info.dense_m, log_batch_size=info.log_batch_size,
+                lanes=int(info.num_lanes), m_bool=info.m_bool,
+                has_opening=bool(info.has_opening), device_bytes=int(info.device_bytes))
+
+
+def prove_fast_union(prover: Prover, req: UnionRequest):
+    """flock_hip_prove_fast_union: returns (result, proof bytes, cap bytes, claim_ab,
+    claim_c)."""
+    res = ProveResult()
+    out = ProofBytes()
+    _check(load().flock_hip_prove_fast_union(prover.h, ctypes.byref(req), ctypes.byref(res),
+                                             ctypes.byref(out)))
     proof = ctypes.string_at(out.proof, out.proof_len)
     cap = ctypes.string_at(out.cap, out.cap_len)
     load().flock_hip_free(ctypes.cast(out.proof, ctypes.c_void_p))

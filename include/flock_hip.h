@@ -304,6 +304,37 @@ int flock_hip_lincheck_prove_keccak(flock_hip_ctx* ctx, flock_fs* fs, const uint
                                     int grinding, flock_f128* h_rounds,
                                     flock_f128 h_z_partial[64], flock_f128 h_matrix_evals[2],
                                     uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
+/* One boolean slot of the union lincheck. circuit: the slot type's device CSC circuit (its
+ * k_log and const_pin are used), or NULL for the Keccak-f[1600] walker (k_log 16, pin 4096,
+ * useful_bits 42,560). prefix: Slot::prefix (schedule.rs:247-268); the slot's columns are
+ * [prefix << k_log, (prefix + 1) << k_log) of the 2^col_vars union columns. d_z: the slot's
+ * lincheck stripe (as in flock_hip_lc_partial_fold) with n_rows rows; useful_bits: the
+ * type's USEFUL_BITS (ignored for the walker). */
+typedef struct {
+    const flock_csc* circuit;
+    const uint8_t* d_z;
+    uint64_t prefix;
+    uint64_t useful_bits;
+    uint64_t n_rows;
+} flock_lc_slot;
+/* The union lincheck over several boolean slots: prove_union_capture_z_vec_with_grinding
+ * (lincheck/union.rs:203-430) with column_sumcheck_prove (lincheck.rs:1907-2028), the call
+ * site of prove_union_with_binding_zc for a registry of several boolean types (prover.rs:
+ * 1140-1160). col_vars = M_bool - nu (at least every slot's k_log); per slot the alpha-batched
+ * comb through its quirky table over x_inner_rest[..k_t-6], scaled by eq_prefix_weight(
+ * x_inner_rest[k_t-6..], prefix_t) (union.rs:90-101); the pins' betas in slot order; one
+ * eq(x_outer) table for every slot's partial fold; the column sumcheck over 2^col_vars; then
+ * per slot the unscaled (<W_t, A_0>, <W_t, B_0>) with W_t = z_partial (x)
+ * eq(r_inner_rest[..k_t-6]). Outputs as flock_hip_lincheck_prove with L = col_vars - 6:
+ * h_rounds[2L], h_z_partial[64], h_matrix_evals[2 n_slots] (slot order), h_nonces[L + 2 +
+ * pins] when grinding, h_claim[2 + L], d_z_vec (2^col_vars words, may be NULL). One slot
+ * with k_log = col_vars is exactly flock_hip_lincheck_prove{,_keccak}. */
+int flock_hip_lincheck_prove_union(flock_hip_ctx* ctx, flock_fs* fs, const flock_lc_slot* slots,
+                                   size_t n_slots, uint32_t col_vars, uint32_t nu,
+                                   flock_f128 z_skip, const flock_f128* h_x_inner_rest,
+                                   const flock_f128* h_x_outer, int grinding, flock_f128* h_rounds,
+                                   flock_f128 h_z_partial[64], flock_f128* h_matrix_evals,
+                                   uint64_t* h_nonces, flock_f128* h_claim, flock_f128* d_z_vec);
 /* verify_union_with_grinding (lincheck/union.rs:662-960) for a registry with one boolean type
  * filling the boolean region, the prover of flock_hip_lincheck_prove: replays the proof (same
  * layouts) on the verifier's transcript `fs` — alpha, beta when const_pin >= 0 (the target
@@ -428,6 +459,12 @@ int flock_circuit_type_statement(const flock_circuit* c, flock_type_statement* o
 /* Registry::digest of the one-type registry of the built-in circuit at nu (what
  * bind_statement absorbs for a single-hash prove_fast). */
 int flock_circuit_registry_digest(uint32_t kind, uint32_t nu, uint8_t out[32]);
+/* Registry::digest of the registry of several built-in types at nu (kinds in any order;
+ * the digest takes them in Registry::new's order), e.g. BLAKE3 + SHA-256 at nu 10 for the
+ * mixed Blake3Sha2Nu10 registry (flock-prover/src/mixed.rs:59, 91-98). Cached per
+ * (kinds, nu). */
+int flock_circuit_union_digest(const uint32_t* kinds, size_t n_kinds, uint32_t nu,
+                               uint8_t out[32]);
 /* Upload the circuit's CSC to the device (flock_hip_malloc on ctx); release with
  * flock_hip_circuit_release. */
 int flock_hip_circuit_upload(flock_hip_ctx* ctx, uint32_t kind, flock_csc* out);
@@ -651,6 +688,52 @@ typedef struct {
 } flock_proof_bytes;
 int flock_hip_prove_fast(flock_hip_prover* p, const flock_prove_request* req, flock_prove_result* res,
                          flock_proof_bytes* out);
+/* One boolean hash type of a mixed request: kind, n_decl instances (0 allowed; Keccak
+ * counts 2^n_log whatever n_decl), inputs as in flock_prove_request (host, or device with
+ * inputs_on_device), circuit NULL for the built-in one. */
+typedef struct {
+    uint32_t kind;
+    uint32_t reserved;
+    uint64_t n_decl;
+    const void* inputs;
+    const flock_csc* circuit;
+} flock_union_part;
+/* A prove_fast request over a registry of several boolean hash types at one batch capacity
+ * 2^n_log (3 <= n_log <= 24): Registry::new sorts the parts into slots (k_log descending,
+ * stable, schedule.rs:337-495). digest NULL: flock_circuit_union_digest of the parts' kinds. */
+typedef struct {
+    uint32_t n_log;
+    uint32_t n_parts; /* 1 .. FLOCK_MAX_TYPES */
+    const flock_union_part* parts;
+    const uint8_t* digest;
+    const uint8_t* domain;
+    size_t domain_len;
+    uint32_t inputs_on_device;
+    uint32_t reserved;
+} flock_union_request;
+/* What a mixed bundle records beside the proof (MixedProofBundleLigerito, proof_io.rs:
+ * 230-282): slot_part[s] = the part in slot s, counts[s] its statement count (slot order),
+ * the PCS parameters (dense_m, log_batch_size, num_lanes 0 = None), M_bool, whether a
+ * Ligerito config exists (dense m >= 22), and the request's device footprint. */
+typedef struct {
+    uint32_t n_slots, m_bool, dense_m, log_batch_size;
+    uint64_t num_lanes;
+    uint32_t has_opening, reserved;
+    uint64_t device_bytes;
+    uint32_t slot_part[FLOCK_MAX_TYPES];
+    uint64_t counts[FLOCK_MAX_TYPES];
+} flock_union_info;
+int flock_union_request_info(const flock_union_request* req, flock_union_info* out);
+/* The whole prove_fast for a registry of several boolean hash types: prove_fast_ligerito_union
+ * (prover.rs:587-624) over prove_union_with_binding_zc (prover.rs:831-1468) with one
+ * UnionSlotProverInput per slot, as tests/union_m6_fixtures.rs:236-270 drives it: each slot's
+ * witness at its offset in the packed boolean region (the rest zero), the dense stack over
+ * the slots' counts, commit, bind_statement with the counts in slot order, zerocheck over
+ * M_bool, the union lincheck (flock_hip_lincheck_prove_union), s_hat_v_ab, the jagged
+ * heights of every slot, the merged opening; out->proof = bincode(R1csProofMergedLigerito)
+ * with one matrix-eval pair per slot. A request of one part equals flock_hip_prove_fast. */
+int flock_hip_prove_fast_union(flock_hip_prover* p, const flock_union_request* req,
+                               flock_prove_result* res, flock_proof_bytes* out);
 /* Serving loop: `workers` host threads, one prover each, take the requests in arrival order;
  * a request starts once its arrival time has passed and its footprint fits hbm_budget beside
  * what the other workers hold (running ones: max(held, footprint)); a worker gives back its
