@@ -881,6 +881,12 @@ int flock::set_error(int code, const char* msg) { return fail(code, msg ? msg : 
 
 uint64_t flock::ctx_scratch_bytes(const flock_hip_ctx* c) { return c ? c->scratch_bytes : 0; }
 
+void flock::sha256_host(const void* data, size_t n, uint8_t out[32]) {
+    Sha256 h;
+    h.update(static_cast<const uint8_t*>(data), n);
+    h.digest(out);
+}
+
 void flock::ctx_release_scratch(flock_hip_ctx* c) {
     if (!c || !c->scratch) return;
     hipStreamSynchronize(c->stream);
@@ -1549,6 +1555,16 @@ F interpolate_combined(const flock_f128* lambda_vals, F z) {
     return acc;
 }
 }  // namespace
+
+}  // extern "C"
+
+void flock::lagrange_phi8_host(int offset, flock_f128 z, flock_f128 w[64]) {
+    F fw[64];
+    lagrange_phi8(offset, fo(z), fw);
+    for (int i = 0; i < 64; i++) w[i] = to(fw[i]);
+}
+
+extern "C" {
 
 int flock_zerocheck_verify(flock_fs* fs, uint32_t m, int grinding, const flock_f128* h_round1,
                            const flock_f128* h_ml, const flock_f128 h_finals[3],
@@ -2303,13 +2319,30 @@ bool host_csc_fold(uint64_t K, const uint32_t* ptr, const uint32_t* rows, uint64
 using HostFold = std::function<bool(const std::vector<F>& eq, std::vector<F>& sa,
                                     std::vector<F>& sb)>;
 
-static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_log,
-                                int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
-                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
-                                flock_f128 v_a, flock_f128 v_b, int grinding,
-                                const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
-                                const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
-                                flock_f128* h_claim);
+// One slot of the union lincheck verifier: its host fold (null: the reported matrix evals are
+// checked instead), k_log, constant-wire pin (-1: none), Slot::prefix and statement count.
+struct LcVSlot {
+    const HostFold* fold;
+    uint32_t k_log;
+    int64_t const_pin;
+    uint64_t prefix, n_rows;
+};
+
+// eq_prefix_weight (union.rs:90-101): prod_j eq(x_j, bit j of prefix).
+static F eq_prefix_weight(const flock_f128* x, uint32_t n, uint64_t prefix) {
+    F w{1, 0};
+    for (uint32_t j = 0; j < n; j++)
+        w = fmul(w, ((prefix >> j) & 1) ? fo(x[j]) : fadd(F{1, 0}, fo(x[j])));
+    return w;
+}
+
+static int lincheck_verify_slots(flock_fs* fs, const std::vector<LcVSlot>& slots,
+                                 uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
+                                 const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                 flock_f128 v_a, flock_f128 v_b, int grinding,
+                                 const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                 const flock_f128* h_matrix_evals, const uint64_t* h_nonces,
+                                 flock_f128* h_claim);
 
 int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_log,
                           int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
@@ -2318,10 +2351,12 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
                           const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
                           const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
                           flock_f128* h_claim) {
-    if (!h_circuit)
-        return lincheck_verify_impl(fs, nullptr, k_log, const_pin, nu, n_rows, z_skip,
-                                    h_x_inner_rest, h_x_outer, v_a, v_b, grinding, h_rounds,
-                                    h_z_partial, h_matrix_evals, h_nonces, h_claim);
+    if (!h_circuit) {
+        const std::vector<LcVSlot> one{{nullptr, k_log, const_pin, 0, n_rows}};
+        return lincheck_verify_slots(fs, one, k_log, nu, z_skip, h_x_inner_rest, h_x_outer, v_a,
+                                     v_b, grinding, h_rounds, h_z_partial, h_matrix_evals,
+                                     h_nonces, h_claim);
+    }
     if (int rc = csc_check(h_circuit)) return rc;
     if (h_circuit->k_log != k_log || h_circuit->const_pin != const_pin)
         return fail(FLOCK_HIP_E_INVALID, "circuit k_log / const_pin differ from the type's");
@@ -2331,9 +2366,24 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
         return host_csc_fold(K, k->a_col_ptr, k->a_rows, k->nnz_a, eq, sa) &&
                host_csc_fold(K, k->b_col_ptr, k->b_rows, k->nnz_b, eq, sb);
     };
-    return lincheck_verify_impl(fs, &fold, k_log, const_pin, nu, n_rows, z_skip, h_x_inner_rest,
-                                h_x_outer, v_a, v_b, grinding, h_rounds, h_z_partial,
-                                h_matrix_evals, h_nonces, h_claim);
+    const std::vector<LcVSlot> one{{&fold, k_log, const_pin, 0, n_rows}};
+    return lincheck_verify_slots(fs, one, k_log, nu, z_skip, h_x_inner_rest, h_x_outer, v_a, v_b,
+                                 grinding, h_rounds, h_z_partial, h_matrix_evals, h_nonces,
+                                 h_claim);
+}
+
+// The Keccak-f[1600] walk on the host (k_log 16), tables built once per process.
+static const HostFold& keccak_host_fold() {
+    static std::vector<uint16_t> inv, trow;
+    static std::vector<uint32_t> rcm;
+    static const bool built = kc_build_tables(inv, trow, rcm) == 0;
+    static const HostFold fold = [](const std::vector<F>& eq, std::vector<F>& sa,
+                                    std::vector<F>& sb) {
+        if (!built) return false;
+        kc_host_fold(eq.data(), sa.data(), sb.data(), inv, trow, rcm);
+        return true;
+    };
+    return fold;
 }
 
 int flock_lincheck_verify_keccak(flock_fs* fs, uint32_t nu, size_t n_rows, flock_f128 z_skip,
@@ -2341,34 +2391,68 @@ int flock_lincheck_verify_keccak(flock_fs* fs, uint32_t nu, size_t n_rows, flock
                                  flock_f128 v_a, flock_f128 v_b, int grinding,
                                  const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
                                  const uint64_t* h_nonces, flock_f128* h_claim) {
-    std::vector<uint16_t> inv, trow;
-    std::vector<uint32_t> rcm;
-    if (int rc = kc_build_tables(inv, trow, rcm)) return rc;
-    const HostFold fold = [&](const std::vector<F>& eq, std::vector<F>& sa, std::vector<F>& sb) {
-        kc_host_fold(eq.data(), sa.data(), sb.data(), inv, trow, rcm);
-        return true;
-    };
     // K_LOG 16, const pin Z_CONST = 4096 (keccak.rs:436-451, 688-690)
-    return lincheck_verify_impl(fs, &fold, 16, 4096, nu, n_rows, z_skip, h_x_inner_rest,
-                                h_x_outer, v_a, v_b, grinding, h_rounds, h_z_partial, nullptr,
-                                h_nonces, h_claim);
+    const std::vector<LcVSlot> one{{&keccak_host_fold(), 16, 4096, 0, n_rows}};
+    return lincheck_verify_slots(fs, one, 16, nu, z_skip, h_x_inner_rest, h_x_outer, v_a, v_b,
+                                 grinding, h_rounds, h_z_partial, nullptr, h_nonces, h_claim);
 }
 
-static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_log,
-                                int64_t const_pin, uint32_t nu, size_t n_rows, flock_f128 z_skip,
+int flock_lincheck_verify_union(flock_fs* fs, const flock_lc_vslot* slots, size_t n_slots,
+                                uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
                                 const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
                                 flock_f128 v_a, flock_f128 v_b, int grinding,
                                 const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
-                                const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
+                                const flock_f128* h_matrix_evals, const uint64_t* h_nonces,
                                 flock_f128* h_claim) {
+    if (!slots || n_slots == 0 || n_slots > FLOCK_MAX_TYPES)
+        return fail(FLOCK_HIP_E_INVALID, "bad union lincheck verify slots (need 1..16)");
+    std::vector<HostFold> folds(n_slots);
+    std::vector<LcVSlot> run(n_slots);
+    for (size_t t = 0; t < n_slots; t++) {
+        const flock_lc_vslot& s = slots[t];
+        if (s.h_circuit) {
+            const flock_csc* k = s.h_circuit;
+            if (int rc = csc_check(k)) return rc;
+            folds[t] = [k](const std::vector<F>& eq, std::vector<F>& sa, std::vector<F>& sb) {
+                const uint64_t K = 1ull << k->k_log;
+                return host_csc_fold(K, k->a_col_ptr, k->a_rows, k->nnz_a, eq, sa) &&
+                       host_csc_fold(K, k->b_col_ptr, k->b_rows, k->nnz_b, eq, sb);
+            };
+            run[t] = {&folds[t], k->k_log, k->const_pin, s.prefix, s.n_rows};
+        } else if (s.walker) {
+            run[t] = {&keccak_host_fold(), 16, 4096, s.prefix, s.n_rows};
+        } else {
+            run[t] = {nullptr, s.k_log, s.const_pin, s.prefix, s.n_rows};
+        }
+    }
+    return lincheck_verify_slots(fs, run, col_vars, nu, z_skip, h_x_inner_rest, h_x_outer, v_a,
+                                 v_b, grinding, h_rounds, h_z_partial, h_matrix_evals, h_nonces,
+                                 h_claim);
+}
+
+static int lincheck_verify_slots(flock_fs* fs, const std::vector<LcVSlot>& slots,
+                                 uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
+                                 const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                 flock_f128 v_a, flock_f128 v_b, int grinding,
+                                 const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                 const flock_f128* h_matrix_evals, const uint64_t* h_nonces,
+                                 flock_f128* h_claim) {
     const int ks = 6;
-    if (!fs || k_log < (uint32_t)ks || k_log > 24 || nu > 40 || n_rows > (1ull << nu) ||
-        const_pin >= ((int64_t)1 << k_log) || const_pin < -1)
-        return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (need 6 <= k_log <= 24, "
-                                         "nu <= 40, n_rows <= 2^nu, const_pin < 2^k_log)");
-    const uint32_t L = k_log - ks;
+    if (!fs || col_vars < (uint32_t)ks || col_vars > 24 || nu > 40 || slots.empty())
+        return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (need 6 <= col_vars <= "
+                                         "24, nu <= 40, at least one slot)");
+    const bool folded = slots[0].fold != nullptr;
+    for (const LcVSlot& s : slots)
+        if (s.k_log < (uint32_t)ks || s.k_log > col_vars || s.n_rows > (1ull << nu) ||
+            s.const_pin >= ((int64_t)1 << s.k_log) || s.const_pin < -1 ||
+            s.prefix >= (1ull << (col_vars - s.k_log)) || (s.fold != nullptr) != folded)
+            return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify slot (need 6 <= k_log <= "
+                                             "col_vars, n_rows <= 2^nu, const_pin < 2^k_log, "
+                                             "prefix < 2^(col_vars - k_log), every slot with "
+                                             "a circuit or none)");
+    const uint32_t L = col_vars - ks;
     if ((L && (!h_x_inner_rest || !h_rounds)) || (nu && !h_x_outer) || !h_z_partial ||
-        (!fold && !h_matrix_evals) || (grinding && !h_nonces))
+        (!folded && !h_matrix_evals) || (grinding && !h_nonces))
         return fail(FLOCK_HIP_E_INVALID, "bad lincheck verify arguments (null buffer)");
     int ni = 0;
     flock_fs_observe_label(fs, kLabelLincheck, sizeof(kLabelLincheck) - 1);
@@ -2378,17 +2462,19 @@ static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_l
     flock_f128 alpha_v;
     flock_fs_sample_f128(fs, &alpha_v);
     const F alpha = fo(alpha_v);
-    // 2b. const-wire pin: beta after alpha; the target gains beta times the count-derived
-    // prefix sum of eq over the declared rows (union.rs:815-846)
+    // 2b. each slot's const-wire pin in slot order: beta_t after alpha; the target gains
+    // beta_t times the count-derived prefix sum of eq over the slot's declared rows
+    // (union.rs:815-846)
     F target = fadd(fmul(alpha, fo(v_a)), fo(v_b));
-    F beta{0, 0};
-    if (const_pin >= 0) {
+    std::vector<F> beta(slots.size(), F{0, 0});
+    for (size_t t = 0; t < slots.size(); t++) {
+        if (slots[t].const_pin < 0) continue;
         if (grinding && !flock_fs_verify_pow(fs, h_nonces[ni++], 1))
             return reject("beta proof of work");
         flock_f128 b;
         flock_fs_sample_f128(fs, &b);
-        beta = fo(b);
-        target = fadd(target, fmul(beta, eq_prefix_sum(h_x_outer, nu, n_rows)));
+        beta[t] = fo(b);
+        target = fadd(target, fmul(beta[t], eq_prefix_sum(h_x_outer, nu, slots[t].n_rows)));
     }
     // 3. the product-sumcheck rounds: q(0) = claim + q(1), q(X) = einf X^2 + c1 X + e0
     // (union.rs:851-873)
@@ -2421,39 +2507,57 @@ static int lincheck_verify_impl(flock_fs* fs, const HostFold* fold, uint32_t k_l
         lagrange_phi8(0, fo(r_skip), lam);
         for (int i = 0; i < 64; i++) w = fadd(w, fmul(lam[i], fo(h_z_partial[i])));
     }
-    // the matrix assertion; one slot filling the boolean region has prefix weights 1
-    const std::vector<F> err = eq_table(rr.data(), L);
-    if (!fold) {
-        // check_reported (union.rs:538-576): alpha <W,A_0> + <W,B_0> + beta W_col[pin] = claim,
-        // W_col = z_partial (x) eq(rr)
-        F acc = fadd(fmul(alpha, fo(h_matrix_evals[0])), fo(h_matrix_evals[1]));
-        if (const_pin >= 0)
-            acc = fadd(acc, fmul(beta, fmul(fo(h_z_partial[const_pin & 63]), err[const_pin >> 6])));
+    // the matrix assertion (union.rs:467-628), per slot with the row-side prefix weight w_t
+    // (at x_inner_rest) and the column-side p_t (at rr)
+    F acc{0, 0};
+    if (!folded) {
+        // check_reported: sum_t p_t w_t (alpha <W_t,A_0> + <W_t,B_0>) + p_t beta_t W_col_t[pin],
+        // W_col_t = z_partial (x) eq(rr[..k_t - 6])
+        for (size_t t = 0; t < slots.size(); t++) {
+            const uint32_t inner = slots[t].k_log - ks, hi = col_vars - slots[t].k_log;
+            const F wt = eq_prefix_weight(h_x_inner_rest + inner, hi, slots[t].prefix);
+            const F pt = eq_prefix_weight(rr.data() + inner, hi, slots[t].prefix);
+            acc = fadd(acc, fmul(fmul(pt, wt), fadd(fmul(alpha, fo(h_matrix_evals[2 * t])),
+                                                    fo(h_matrix_evals[2 * t + 1]))));
+            if (slots[t].const_pin >= 0) {
+                const int64_t pin = slots[t].const_pin;
+                F e{1, 0};
+                for (uint32_t j = 0; j < inner; j++)
+                    e = fmul(e, ((pin >> (6 + j)) & 1) ? fo(rr[j]) : fadd(F{1, 0}, fo(rr[j])));
+                acc = fadd(acc, fmul(fmul(pt, beta[t]), fmul(fo(h_z_partial[pin & 63]), e)));
+            }
+        }
         if (!feq(acc, run)) return reject("matrix evals");
     } else {
-        // check (union.rs:579-628): comb = alpha A_0^T eq + B_0^T eq at the quirky eq of
-        // (z_skip, x_inner_rest), + beta at the pin; its collapse at rr against z_partial
-        const uint64_t K = 1ull << k_log;
-        std::vector<F> eqi(K), sa(K), sb(K);
-        {
-            F lam[64];
-            lagrange_phi8(0, fo(z_skip), lam);
-            const std::vector<F> ex = eq_table(h_x_inner_rest, L);
-            for (uint64_t j = 0; j < ((uint64_t)1 << L); j++)
-                for (int i = 0; i < 64; i++) eqi[j * 64 + i] = fmul(lam[i], ex[j]);
-        }
-        if (!(*fold)(eqi, sa, sb))
-            return fail(FLOCK_HIP_E_INVALID, "CSC circuit has a row index >= n_cols or a "
-                                             "col_ptr entry out of order or > nnz");
+        // check: per slot comb_t = w_t (alpha A_0^T eq + B_0^T eq) at the quirky eq of
+        // (z_skip, x_inner_rest[..k_t - 6]), + beta_t at the pin; collapsed at rr and p_t
+        // against z_partial
         F part[64];
         for (int i = 0; i < 64; i++) part[i] = F{0, 0};
-        for (uint64_t j = 0; j < ((uint64_t)1 << L); j++)
-            for (int i = 0; i < 64; i++) {
-                F cmb = fadd(fmul(alpha, sa[j * 64 + i]), sb[j * 64 + i]);
-                if ((int64_t)(j * 64 + i) == const_pin) cmb = fadd(cmb, beta);
-                part[i] = fadd(part[i], fmul(err[j], cmb));
+        F lam[64];
+        lagrange_phi8(0, fo(z_skip), lam);
+        for (size_t t = 0; t < slots.size(); t++) {
+            const uint32_t inner = slots[t].k_log - ks, hi = col_vars - slots[t].k_log;
+            const uint64_t K = 1ull << slots[t].k_log;
+            std::vector<F> eqi(K), sa(K), sb(K);
+            const std::vector<F> ex = eq_table(h_x_inner_rest, inner);
+            for (uint64_t j = 0; j < ((uint64_t)1 << inner); j++)
+                for (int i = 0; i < 64; i++) eqi[j * 64 + i] = fmul(lam[i], ex[j]);
+            if (!(*slots[t].fold)(eqi, sa, sb))
+                return fail(FLOCK_HIP_E_INVALID, "CSC circuit has a row index >= n_cols or a "
+                                                 "col_ptr entry out of order or > nnz");
+            const F wt = eq_prefix_weight(h_x_inner_rest + inner, hi, slots[t].prefix);
+            const F pt = eq_prefix_weight(rr.data() + inner, hi, slots[t].prefix);
+            const std::vector<F> err = eq_table(rr.data(), inner);
+            for (uint64_t j = 0; j < ((uint64_t)1 << inner); j++) {
+                const F ej = fmul(pt, err[j]);
+                for (int i = 0; i < 64; i++) {
+                    F cmb = fmul(wt, fadd(fmul(alpha, sa[j * 64 + i]), sb[j * 64 + i]));
+                    if ((int64_t)(j * 64 + i) == slots[t].const_pin) cmb = fadd(cmb, beta[t]);
+                    part[i] = fadd(part[i], fmul(ej, cmb));
+                }
             }
-        F acc{0, 0};
+        }
         for (int i = 0; i < 64; i++) acc = fadd(acc, fmul(part[i], fo(h_z_partial[i])));
         if (!feq(acc, run)) return reject("sumcheck final consistency");
     }

@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/flock_hip.h"
 #include "transcript_device.h"
 
 struct flock_hip_ctx;
@@ -181,5 +182,9 @@ int set_error(int code, const char* msg);
 // Bytes of the context's grow-only driver scratch, and its release (after a stream sync).
 uint64_t ctx_scratch_bytes(const flock_hip_ctx* c);
 void ctx_release_scratch(flock_hip_ctx* c);
+// Host SHA-256 of n bytes (the x86 SHA extensions where present).
+void sha256_host(const void* data, size_t n, uint8_t out[32]);
+// Host Lagrange basis on the 64 nodes phi8(offset + i) at z (offset 0: S, 64: Lambda).
+void lagrange_phi8_host(int offset, flock_f128 z, flock_f128 w[64]);
 
 }  // namespace flock

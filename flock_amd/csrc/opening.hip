@@ -2157,7 +2157,8 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         rs_claim[c] = acc;
     }
     const uint32_t bb = n_claims > 1 ? 6 : 0;
-    const uint64_t batch_nonce = flock_fs_grind_pow(fs, bb);
+    // claim_batch_bits_for (pcs.rs:193-199): no PoW, and nothing absorbed, for one claim
+    const uint64_t batch_nonce = bb ? flock_fs_grind_pow(fs, bb) : 0;
     const std::vector<H> gam = sample_vec(fs, n_claims);
     H target = hz();
     std::vector<std::vector<H>> tabs(n_claims);
@@ -2419,6 +2420,796 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     *out_bytes = p;
     *out_len = out.b.size();
     ws->stamp(5);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The merged opening's verifier (pcs.rs:1971-2300): ring switch, merged sumcheck replay,
+// multipoint twisted evaluation with its Frobenius anchor (pcs/jagged.rs:2034-2230,
+// 4335-4585) on the forked transcript, and the succinct Ligerito verifier of q^(rho)
+// (pcs.rs:1129-1320, pcs/ligerito/extension.rs:2409-2880). Host only.
+// ---------------------------------------------------------------------------
+namespace {
+
+// bincode 1.3 reader; an overrun or a length larger than the bytes left marks it bad
+struct Rd {
+    const uint8_t* p;
+    size_t n, off = 0;
+    bool ok = true;
+    bool need(size_t k) {
+        if (!ok || k > n - off) ok = false;
+        return ok;
+    }
+    uint64_t u64() {
+        if (!need(8)) return 0;
+        uint64_t v = 0;
+        for (int i = 0; i < 8; i++) v |= (uint64_t)p[off + i] << (8 * i);
+        off += 8;
+        return v;
+    }
+    H f() {
+        H v;
+        v.lo = u64();
+        v.hi = u64();
+        return v;
+    }
+    E e() {
+        E v;
+        v.c0 = f();
+        v.c1 = f();
+        return v;
+    }
+    size_t len(size_t elem) {
+        const uint64_t k = u64();
+        if (ok && k > (n - off) / (elem ? elem : 1)) ok = false;
+        return ok ? (size_t)k : 0;
+    }
+    std::vector<H> vec_f() {
+        std::vector<H> v(len(16));
+        for (H& x : v) x = f();
+        return v;
+    }
+    std::vector<uint64_t> vec_u64() {
+        std::vector<uint64_t> v(len(8));
+        for (uint64_t& x : v) x = u64();
+        return v;
+    }
+    std::vector<uint8_t> vec_hash() {
+        const size_t k = len(32);
+        std::vector<uint8_t> v;
+        if (need(32 * k)) {
+            v.assign(p + off, p + off + 32 * k);
+            off += 32 * k;
+        }
+        return v;
+    }
+    std::vector<H> pairs() {  // Vec<(F128, F128)> flattened
+        std::vector<H> v(2 * len(32));
+        for (H& x : v) x = f();
+        return v;
+    }
+};
+
+// RecursiveProof {opened_rows: Vec<Vec<F128>>, merkle_proof: Vec<Hash>} (ligerito.rs:2160-2176)
+struct Opened {
+    std::vector<std::vector<H>> rows;
+    std::vector<uint8_t> paths;
+};
+Opened rd_opened(Rd& r) {
+    Opened o;
+    o.rows.resize(r.len(8));
+    for (auto& row : o.rows) row = r.vec_f();
+    o.paths = r.vec_hash();
+    return o;
+}
+
+// LigeritoProof (ligerito.rs:2177-2218) of the basis / F256 route
+struct LigProof {
+    std::vector<uint8_t> initial_cap;
+    Opened initial;
+    std::vector<std::vector<uint8_t>> caps;
+    std::vector<Opened> recursive;
+    std::vector<H> yr;
+    Opened fin;
+    size_t n_sc128 = 0;
+    std::vector<E> msgs;  // (u0, u2) per message
+    std::vector<uint64_t> grind_n, fold_n, claim_n, cons_n;
+    std::vector<H> ood;
+};
+LigProof rd_lig(Rd& r) {
+    LigProof L;
+    L.initial_cap = r.vec_hash();
+    L.initial = rd_opened(r);
+    L.caps.resize(r.len(8));
+    for (auto& c : L.caps) c = r.vec_hash();
+    L.recursive.resize(r.len(16));
+    for (auto& o : L.recursive) o = rd_opened(r);
+    L.yr = r.vec_f();
+    L.fin = rd_opened(r);
+    L.n_sc128 = r.len(32);
+    for (size_t i = 0; i < L.n_sc128; i++) r.f(), r.f();
+    L.msgs.resize(2 * r.len(64));
+    for (E& x : L.msgs) x = r.e();
+    L.grind_n = r.vec_u64();
+    L.ood = r.vec_f();
+    L.fold_n = r.vec_u64();
+    L.claim_n = r.vec_u64();
+    L.cons_n = r.vec_u64();
+    return L;
+}
+
+// RoundQuad256 (extension.rs:71-97): q(X) = c + b X + a X^2 from (u0, u2) and the claim
+struct Quad {
+    E c, b, a;
+    static Quad from(E u0, E u2, E claim) { return Quad{u0, ea(claim, u2), u2}; }
+    E eval(E r) const { return ea(ea(c, em(r, b)), em(em(r, r), a)); }
+    Quad fold(const Quad& o, H alpha) const {
+        return Quad{ea(c, emb(o.c, alpha)), ea(b, emb(o.b, alpha)), ea(a, emb(o.a, alpha))};
+    }
+};
+inline E eh(H x) { return E{x, hz()}; }
+inline bool eeq(E a, E b) { return std::memcmp(&a, &b, sizeof(E)) == 0; }
+inline bool heqv(H a, H b) { return a.lo == b.lo && a.hi == b.hi; }
+
+// verify_pow then sample (challenger.rs verify_pow_and_sample_f128{,_vec})
+bool vps(flock_fs* fs, uint64_t nonce, uint32_t bits, size_t n, std::vector<H>* out) {
+    if (!flock_fs_verify_pow(fs, nonce, bits)) return false;
+    *out = sample_vec(fs, n);
+    return true;
+}
+bool vps1(flock_fs* fs, uint64_t nonce, uint32_t bits, H* out) {
+    if (!flock_fs_verify_pow(fs, nonce, bits)) return false;
+    *out = sample(fs);
+    return true;
+}
+
+// verify_level_opens (ligerito.rs:5152-5220) with verify_merkle_proof_capped (merkle.rs:
+// 698-732): SHA-256 leaves over the rows' bytes, capped paths in sample order
+bool level_opens_ok(const std::vector<uint8_t>& cap, uint64_t block_len,
+                    const std::vector<uint64_t>& qs, const Opened& o, size_t lanes, const Sched& s) {
+    const int d = __builtin_ctzll(block_len);
+    if (qs.size() != o.rows.size() || s.d != d || s.queries() != qs.size()) return false;
+    const int c1 = s.cap();
+    if (cap.size() != (32ull << c1) || o.paths.size() != 32 * qs.size() * (size_t)(d - c1))
+        return false;
+    size_t off = 0;
+    for (size_t i = 0; i < qs.size(); i++) {
+        if (o.rows[i].size() != lanes || qs[i] >= block_len) return false;
+        uint8_t acc[32], pair[64];
+        flock::sha256_host(o.rows[i].data(), 16 * lanes, acc);
+        uint64_t idx = qs[i];
+        for (int l = 0; l < d - c1; l++, off += 32) {
+            if (idx & 1) {
+                std::memcpy(pair, &o.paths[off], 32);
+                std::memcpy(pair + 32, acc, 32);
+            } else {
+                std::memcpy(pair, acc, 32);
+                std::memcpy(pair + 32, &o.paths[off], 32);
+            }
+            flock::sha256_host(pair, 64, acc);
+            idx >>= 1;
+        }
+        if (std::memcmp(acc, &cap[32 * idx], 32) != 0) return false;
+    }
+    return true;
+}
+
+// induce_enforced_sum (extension.rs:2885-2910)
+E enforced_sum(const std::vector<std::vector<H>>& rows, const std::vector<E>& lane_ch,
+               const std::vector<H>& alpha, bool* ok) {
+    const std::vector<E> lw = heq256(lane_ch.data(), lane_ch.size());
+    const std::vector<H> rw = heq(alpha.data(), alpha.size());
+    E acc = ez();
+    for (size_t q = 0; q < rows.size(); q++) {
+        if (rows[q].size() > lw.size() || q >= rw.size()) {
+            *ok = false;
+            return acc;
+        }
+        E s = ez();
+        for (size_t i = 0; i < rows[q].size(); i++) s = ea(s, emb(lw[i], rows[q][i]));
+        acc = ea(acc, emb(s, rw[q]));
+    }
+    return acc;
+}
+
+const E kU{hz(), h1()};  // u, the F256 generator over F128
+// coordinate_fold_factor (extension.rs:2291-2293): 1 + c (1 + u)
+inline E coord_factor(E c) { return ea(e1(), em(c, E{h1(), h1()})); }
+
+// eval_sk_at_vks (ligerito.rs:2402-2420)
+std::vector<H> sk_at_vk(int log_n) {
+    std::vector<H> skv(log_n + 1, hz()), layer(log_n > 0 ? log_n : 1);
+    skv[0] = h1();
+    for (int i = 1; i <= log_n; i++) layer[i - 1] = H{1ull << i, 0};
+    for (int i = 0, cur = log_n; i < log_n; i++, cur--)
+        for (int j = 0; j < cur; j++) {
+            const H nx = ha(hm(layer[j], layer[j]), hm(skv[i], layer[j]));
+            if (j == 0)
+                skv[i + 1] = nx;
+            else
+                layer[j - 1] = nx;
+        }
+    return skv;
+}
+
+// induced_basis_at_residual (extension.rs:2295-2343)
+std::vector<E> induced_at_residual(int log_cols, const std::vector<uint64_t>& qs,
+                                   const std::vector<H>& alpha, const std::vector<E>& fixed,
+                                   int res_log) {
+    const std::vector<H> skv = sk_at_vk(log_cols);
+    std::vector<H> inv(skv.size());
+    for (size_t i = 0; i < skv.size(); i++) inv[i] = hzero(skv[i]) ? hz() : hinv(skv[i]);
+    const std::vector<H> wts = heq(alpha.data(), alpha.size());
+    std::vector<E> out((size_t)1 << res_log, ez());
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        std::vector<H> w(log_cols, hz());
+        if (log_cols > 0) {
+            w[0] = H{qs[qi], 0};
+            for (int k = 1; k < log_cols; k++) w[k] = ha(hm(w[k - 1], w[k - 1]), hm(skv[k - 1], w[k - 1]));
+            for (int k = 0; k < log_cols; k++) w[k] = hm(w[k], inv[k]);
+        }
+        E pre = e1();
+        for (size_t j = 0; j < fixed.size(); j++) pre = em(pre, ea(e1(), emb(fixed[j], ha(h1(), w[j]))));
+        const E pw = emb(pre, wts[qi]);
+        for (size_t y = 0; y < out.size(); y++) {
+            E t = pw;
+            for (int j = 0; j < res_log; j++)
+                if ((y >> j) & 1) t = emb(t, w[fixed.size() + j]);
+            out[y] = ea(out[y], t);
+        }
+    }
+    return out;
+}
+
+// eq_residual (extension.rs:2386-2404)
+std::vector<E> eq_residual(const std::vector<H>& pt, const std::vector<E>& fixed, int res_log, E scale) {
+    E pre = scale;
+    for (size_t j = 0; j < fixed.size(); j++) pre = em(pre, ea(ea(e1(), eh(pt[j])), fixed[j]));
+    std::vector<E> out((size_t)1 << res_log);
+    for (size_t y = 0; y < out.size(); y++) {
+        E t = pre;
+        for (int j = 0; j < res_log; j++) {
+            const H z = pt[fixed.size() + j];
+            t = emb(t, ((y >> j) & 1) ? z : ha(h1(), z));
+        }
+        out[y] = t;
+    }
+    return out;
+}
+
+// the original challenges from start_level on (residual_original_challenges, :2360-2378)
+std::vector<E> originals(const std::vector<E>& initial, const std::vector<std::vector<E>>& levels,
+                         size_t start) {
+    std::vector<E> out;
+    if (start == 0) out = initial;
+    for (size_t l = start; l < levels.size(); l++) out.insert(out.end(), levels[l].begin() + 1, levels[l].end());
+    return out;
+}
+E coord_product(const std::vector<std::vector<E>>& levels, size_t start) {
+    E acc = e1();
+    for (size_t l = start; l < levels.size(); l++) acc = em(acc, coord_factor(levels[l][0]));
+    return acc;
+}
+
+// recursive_verifier_with_basis_succinct (extension.rs:2409-2880) for one packed-direct
+// claim (point rho, batched by gamma), the shape verify_opening_batch_ligerito_mixed_with_
+// grinding drives for the merged opening's inner claim (pcs.rs:1150-1320)
+bool lig_verify(flock_fs* fs, const Lig* C, const LigProof& P, int log_n, H target,
+                const std::vector<uint8_t>& cap, size_t l0_lanes, const std::vector<H>& rho,
+                H gamma, std::string* why) {
+    const int ik = C->initial_k, R = C->n_levels - 1;
+    auto fail = [&](const char* w) {
+        *why = w;
+        return false;
+    };
+    if (R < 1 || P.initial_cap != cap || P.n_sc128 != 0 || P.msgs.empty() || !P.fold_n.empty())
+        return fail("Ligerito proof shape");
+    for (int i = 0; i < R; i++)
+        if (C->lv[i + 1].lanes < 2) return fail("Ligerito config");
+    std::vector<Sched> sch;
+    for (int l = 0; l <= R; l++) sch.push_back(sched_of(C->lv[l].queries, C->lv[l].cols + C->lv[l].rate));
+    label(fs, "flock-ligerito-basis-f256-split-v0");
+    observe(fs, target);
+    flock_fs_observe_bytes(fs, P.initial_cap.data(), P.initial_cap.size());
+    const int log_cols_0 = log_n - ik;
+    const uint64_t block_len_0 = 1ull << (log_cols_0 + C->lv[0].rate);
+    size_t tx = 0, oi = 0, cn = 0, kn = 0, qn = 0;
+    E claim = eh(target);
+    struct Ood {
+        std::vector<H> point;
+        H beta;
+        int split;  // -1: L0; s: consumed by recursive level s
+    };
+    struct Cons {
+        int log_cols;
+        std::vector<uint64_t> qs;
+        std::vector<H> alpha;
+        H beta;
+        size_t start;
+    };
+    std::vector<Ood> oods;
+    std::vector<Cons> cons;
+    const bool lane_major = l0_lanes < (1ull << ik);
+    auto next_msg = [&](E* u0, E* u2) {
+        if (2 * tx + 1 >= P.msgs.size()) return false;
+        *u0 = P.msgs[2 * tx];
+        *u2 = P.msgs[2 * tx + 1];
+        tx++;
+        observe_e(fs, *u0);
+        observe_e(fs, *u2);
+        return true;
+    };
+    for (int o = 0; o < C->lv[0].ood; o++) {
+        std::vector<H> pt = sample_vec(fs, (size_t)log_n);
+        if (lane_major) std::rotate(pt.begin(), pt.begin() + (log_n - ik), pt.end());
+        if (oi >= P.ood.size() || cn >= P.claim_n.size()) return fail("OOD values");
+        const H y = P.ood[oi++];
+        observe(fs, y);
+        H beta;
+        if (!vps1(fs, P.claim_n[cn++], C->lv[0].claim, &beta)) return fail("claim-batch proof of work");
+        claim = ea(claim, eh(hm(beta, y)));
+        oods.push_back({pt, beta, -1});
+    }
+    E u0, u2;
+    if (!next_msg(&u0, &u2)) return fail("sumcheck messages");
+    Quad quad = Quad::from(u0, u2, claim);
+    std::vector<E> initial;
+    for (int j = 0; j < ik; j++) {
+        const E ch = sample_e(fs);
+        claim = quad.eval(ch);
+        initial.push_back(ch);
+        if (!next_msg(&u0, &u2)) return fail("sumcheck messages");
+        quad = Quad::from(u0, u2, claim);
+    }
+    int split_dim = log_n - ik + 1;
+    if (P.caps.empty()) return fail("recursive caps");
+    flock_fs_observe_bytes(fs, P.caps[0].data(), P.caps[0].size());
+    auto level_ood = [&](int lvl, int split) -> bool {
+        for (int o = 0; o < C->lv[lvl].ood; o++) {
+            std::vector<H> pt = sample_vec(fs, (size_t)split_dim);
+            if (oi >= P.ood.size() || cn >= P.claim_n.size()) return fail("OOD values");
+            const H y = P.ood[oi++];
+            observe(fs, y);
+            E a0, a2;
+            if (!next_msg(&a0, &a2)) return fail("sumcheck messages");
+            const Quad intro = Quad::from(a0, a2, eh(y));
+            H beta;
+            if (!vps1(fs, P.claim_n[cn++], C->lv[lvl].claim, &beta))
+                return fail("claim-batch proof of work");
+            quad = quad.fold(intro, beta);
+            claim = ea(claim, eh(hm(beta, y)));
+            oods.push_back({pt, beta, split});
+        }
+        return true;
+    };
+    auto queries = [&](int lvl, uint64_t block_len, std::vector<uint64_t>* qs, std::vector<H>* alpha) {
+        const size_t nq = C->lv[lvl].queries;
+        std::vector<H> w;
+        if (qn >= P.grind_n.size() || !vps(fs, P.grind_n[qn++], C->lv[lvl].grind, nq, &w))
+            return fail("query proof of work");
+        (void)block_len;
+        *qs = queries_from(sch[lvl], w);
+        if (kn >= P.cons_n.size() || !vps(fs, P.cons_n[kn++], C->lv[lvl].cons, ceil_log2(nq), alpha))
+            return fail("consistency proof of work");
+        return true;
+    };
+    if (!level_ood(1, 0)) return false;
+    std::vector<uint64_t> qs;
+    std::vector<H> alpha;
+    if (!queries(0, block_len_0, &qs, &alpha)) return false;
+    if (!level_opens_ok(P.initial_cap, block_len_0, qs, P.initial, l0_lanes, sch[0]))
+        return fail("L0 Merkle openings");
+    bool ok = true;
+    E enforced = enforced_sum(P.initial.rows, initial, alpha, &ok);
+    if (!ok) return fail("L0 opened rows");
+    if (!next_msg(&u0, &u2)) return fail("sumcheck messages");
+    {
+        const Quad intro = Quad::from(u0, u2, enforced);
+        H beta;
+        if (cn >= P.claim_n.size() || !vps1(fs, P.claim_n[cn++], C->lv[1].claim, &beta))
+            return fail("claim-batch proof of work");
+        quad = quad.fold(intro, beta);
+        claim = ea(claim, emb(enforced, beta));
+        cons.push_back({log_n - ik, qs, alpha, beta, 0});
+    }
+    std::vector<std::vector<E>> levels;
+    const std::vector<uint8_t>* prev_cap = &P.caps[0];
+    int prev_lanes = C->lv[1].lanes;
+    if (split_dim < prev_lanes) return fail("Ligerito config");
+    int prev_cols = split_dim - prev_lanes, prev_rate = C->lv[1].rate;
+    size_t cap_i = 1, rec_i = 0;
+    for (int i = 0; i < R; i++) {
+        const int k = C->lv[i + 1].lanes;
+        if (split_dim < k) return fail("Ligerito config");
+        std::vector<E> chs;
+        for (int j = 0; j < k; j++) {
+            const E ch = sample_e(fs);
+            claim = quad.eval(ch);
+            chs.push_back(ch);
+            if (!next_msg(&u0, &u2)) return fail("sumcheck messages");
+            quad = Quad::from(u0, u2, claim);
+        }
+        levels.push_back(chs);
+        const int ext = split_dim - k, level = i + 1;
+        const uint64_t block_len = 1ull << (prev_cols + prev_rate);
+        if (i + 1 == R) {
+            if (tx != P.msgs.size() / 2 || P.yr.size() != (2ull << ext)) return fail("final shape");
+            for (const H& y : P.yr) observe(fs, y);
+            if (!queries(level, block_len, &qs, &alpha)) return false;
+            if (!level_opens_ok(*prev_cap, block_len, qs, P.fin, 1ull << prev_lanes, sch[level]))
+                return fail("final Merkle openings");
+            enforced = enforced_sum(P.fin.rows, levels[i], alpha, &ok);
+            if (!ok) return fail("final opened rows");
+            H beta;
+            if (cn >= P.claim_n.size() || !vps1(fs, P.claim_n[cn++], C->lv[level].claim, &beta))
+                return fail("claim-batch proof of work");
+            claim = ea(claim, emb(enforced, beta));
+            cons.push_back({ext, qs, alpha, beta, (size_t)R});
+            const std::vector<E> orig = originals(initial, levels, 0);
+            if ((int)orig.size() + ext != log_n) return fail("residual dimension");
+            // the basis at the residual: the packed-direct claim's eq (pcs.rs:1214-1304), the
+            // OOD eqs, the consistency bases, each with its coordinate fold factors
+            std::vector<H> pt(rho);
+            if (lane_major) std::rotate(pt.begin(), pt.begin() + (pt.size() - ik), pt.end());
+            std::vector<E> residual = eq_residual(pt, orig, ext, emb(e1(), gamma));
+            const E s0 = coord_product(levels, 0);
+            for (E& v : residual) v = em(v, s0);
+            for (const Ood& o : oods) {
+                std::vector<E> fixed;
+                E cs;
+                if (o.split < 0) {
+                    fixed = orig;
+                    cs = coord_product(levels, 0);
+                } else {
+                    fixed = levels[o.split];
+                    for (size_t l = o.split + 1; l < levels.size(); l++)
+                        fixed.insert(fixed.end(), levels[l].begin() + 1, levels[l].end());
+                    cs = coord_product(levels, (size_t)o.split + 1);
+                }
+                if (fixed.size() + ext != o.point.size()) return fail("OOD point dimension");
+                const std::vector<E> v = eq_residual(o.point, fixed, ext, emb(cs, o.beta));
+                for (size_t y = 0; y < residual.size(); y++) residual[y] = ea(residual[y], v[y]);
+            }
+            for (const Cons& c : cons) {
+                const std::vector<E> fixed = originals({}, levels, c.start);
+                if ((int)fixed.size() + ext != c.log_cols) return fail("consistency dimension");
+                const std::vector<E> v = induced_at_residual(c.log_cols, c.qs, c.alpha, fixed, ext);
+                const E sc = emb(coord_product(levels, c.start), c.beta);
+                for (size_t y = 0; y < residual.size(); y++) residual[y] = ea(residual[y], em(v[y], sc));
+            }
+            // split_basis + split_inner_product against yr (extension.rs:32-55)
+            E rc = ez();
+            for (size_t y = 0; y < residual.size(); y++) {
+                rc = ea(rc, emb(residual[y], P.yr[2 * y]));
+                rc = ea(rc, emb(em(kU, residual[y]), P.yr[2 * y + 1]));
+            }
+            if (!eeq(rc, claim)) return fail("residual claim");
+            if (oi != P.ood.size() || qn != P.grind_n.size() || cn != P.claim_n.size() ||
+                kn != P.cons_n.size() || cap_i != P.caps.size() || rec_i != P.recursive.size())
+                return fail("unused proof entries");
+            return true;
+        }
+        split_dim = ext + 1;
+        if (cap_i >= P.caps.size()) return fail("recursive caps");
+        const std::vector<uint8_t>* cap_n = &P.caps[cap_i++];
+        flock_fs_observe_bytes(fs, cap_n->data(), cap_n->size());
+        if (!level_ood(level + 1, i + 1)) return false;
+        if (!queries(level, block_len, &qs, &alpha)) return false;
+        if (rec_i >= P.recursive.size()) return fail("recursive openings");
+        const Opened& op = P.recursive[rec_i++];
+        if (!level_opens_ok(*prev_cap, block_len, qs, op, 1ull << prev_lanes, sch[level]))
+            return fail("recursive Merkle openings");
+        enforced = enforced_sum(op.rows, levels[i], alpha, &ok);
+        if (!ok) return fail("recursive opened rows");
+        if (!next_msg(&u0, &u2)) return fail("sumcheck messages");
+        const Quad intro = Quad::from(u0, u2, enforced);
+        H beta;
+        if (cn >= P.claim_n.size() || !vps1(fs, P.claim_n[cn++], C->lv[level + 1].claim, &beta))
+            return fail("claim-batch proof of work");
+        quad = quad.fold(intro, beta);
+        claim = ea(claim, emb(enforced, beta));
+        cons.push_back({ext, qs, alpha, beta, (size_t)i + 1});
+        prev_cap = cap_n;
+        prev_lanes = C->lv[i + 2].lanes;
+        if (split_dim < prev_lanes) return fail("Ligerito config");
+        prev_cols = split_dim - prev_lanes;
+        prev_rate = C->lv[i + 2].rate;
+    }
+    return fail("Ligerito rounds");
+}
+
+// the trace-dual basis of the bit basis, raised to every 2^j: D[j][b] = delta_b^(2^j), with
+// bit_b(x) = Tr(delta_b x); then an F2-linear map with basis images g has the linearized
+// coefficients c_j = sum_b g[b] D[j][b] (ring_switch linearized_coefficients)
+const std::vector<H>& dual_powers() {
+    static const std::vector<H> D = [] {
+        // Tr(x^e) for e < 255
+        std::vector<int> tr(255);
+        for (int e = 0; e < 255; e++) {
+            H y = h1();
+            for (int i = 0; i < e; i++) y = hm(y, H{2, 0});
+            H acc = hz(), s = y;
+            for (int k = 0; k < 128; k++) {
+                acc = ha(acc, s);
+                s = hm(s, s);
+            }
+            tr[e] = (int)(acc.lo & 1);
+        }
+        // invert T[a][b] = Tr(x^(a+b)) over F2 (rows as 128-bit masks)
+        std::vector<H> T(128), I(128);
+        for (int a = 0; a < 128; a++) {
+            T[a] = hz();
+            for (int b = 0; b < 128; b++)
+                if (tr[a + b]) T[a] = ha(T[a], hbit(b));
+            I[a] = hbit(a);
+        }
+        for (int col = 0; col < 128; col++) {
+            int piv = col;
+            while (piv < 128 && !hgetbit(T[piv], col)) piv++;
+            if (piv == 128) return std::vector<H>();  // singular (cannot happen: Tr is nondegenerate)
+            std::swap(T[col], T[piv]);
+            std::swap(I[col], I[piv]);
+            for (int r = 0; r < 128; r++)
+                if (r != col && hgetbit(T[r], col)) {
+                    T[r] = ha(T[r], T[col]);
+                    I[r] = ha(I[r], I[col]);
+                }
+        }
+        // delta_a = sum_b Tinv[a][b] x^b = I[a] as an element; then its 2^j powers
+        std::vector<H> out(128 * 128);
+        for (int b = 0; b < 128; b++) {
+            H v = I[b];
+            for (int j = 0; j < 128; j++) {
+                out[(size_t)j * 128 + b] = v;
+                v = hm(v, v);
+            }
+        }
+        return out;
+    }();
+    return D;
+}
+
+}  // namespace
+
+int flock_open_merged_verify(flock_fs* fs, const flock_pcs_params* P, const uint8_t* cap, size_t cap_len,
+                             size_t n_claims, const flock_f128* h_values, const flock_f128* h_z_skips,
+                             const flock_f128* h_x_outers, size_t x_len, const uint64_t* h_heights,
+                             size_t n_heights, uint32_t n_log, const uint8_t* proof, size_t proof_len,
+                             size_t* consumed) {
+    if (!fs || !P || !cap || !h_values || !h_z_skips || !h_x_outers || !h_heights || !proof || n_claims < 1 ||
+        n_claims > flock::kMaxClaims || n_heights == 0 || (n_heights & (n_heights - 1)) || P->m < 14 ||
+        P->m > 40 || x_len < 1)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged_verify: bad arguments");
+    if (P->merkle_hash != 0)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged_verify: SHA-256 Merkle commitments only");
+    const Lig* C = lig_for(P->m);
+    if (!C || C->initial_k != P->log_batch_size || P->log_inv_rate != 1)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged_verify: no Ligerito config for this m");
+    const int dense_log = (int)P->m - 7;
+    const Jag J = jag_make(h_heights, n_heights, (int)n_log, dense_log);
+    if (x_len != 1 + n_log + (size_t)J.k || J.area() > (1ull << dense_log))
+        return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged_verify: point / heights mismatch");
+    const size_t lanes0 = P->num_lanes ? P->num_lanes : (1ull << P->log_batch_size);
+    auto reject = [](const std::string& w) {
+        return flock::set_error(FLOCK_HIP_E_REJECT, ("proof rejected: " + w).c_str());
+    };
+    // MergedOpenProof (pcs.rs:1362-1377)
+    Rd r{proof, proof_len};
+    const size_t n_rs = r.len(16 * 128 + 16);
+    std::vector<std::vector<H>> shv(n_rs);
+    std::vector<uint64_t> rs_nonce(n_rs);
+    for (size_t c = 0; c < n_rs; c++) {
+        shv[c] = r.vec_f();
+        rs_nonce[c] = r.u64();
+    }
+    const std::vector<uint64_t> batch_n = r.vec_u64();
+    const std::vector<H> mrounds = r.pairs();
+    const std::vector<uint64_t> mnon = r.vec_u64();
+    const H q_eval = r.f();
+    std::vector<std::vector<H>> vals(r.len(8));
+    for (auto& v : vals) v = r.vec_f();
+    const std::vector<H> gvals = r.vec_f();
+    const uint64_t gnonce = r.u64();
+    const std::vector<H> trounds = r.pairs();
+    const std::vector<uint64_t> tnon = r.vec_u64();
+    const H av = r.f();
+    const std::vector<H> arounds = r.pairs();
+    const std::vector<uint64_t> anon = r.vec_u64();
+    const size_t inner_rs = r.len(16), inner_bn = r.len(8);
+    LigProof L = rd_lig(r);
+    if (!r.ok) return reject("malformed merged opening bytes");
+    const int m = dense_log;
+    if (n_rs != n_claims || inner_rs != 0 || inner_bn != 0 || mrounds.size() != 2ull * m ||
+        mnon.size() != (size_t)m || vals.size() != n_claims || !gvals.empty() ||
+        trounds.size() != 2ull * m || tnon.size() != (size_t)m || arounds.size() != 4ull * (m + 1) ||
+        anon.size() != 2ull * (m + 1))
+        return reject("merged opening shape");
+    for (size_t c = 0; c < n_claims; c++)
+        if (shv[c].size() != 128 || vals[c].size() != 128) return reject("merged opening shape");
+    const uint32_t bb = n_claims > 1 ? 6 : 0;
+    if (batch_n.size() != (bb ? 1u : 0u)) return reject("batching nonces");
+    label(fs, "flock-merged-open-v1");
+    // 1. ring switch per claim (ring_switch.rs:2966-3005), 3 PoW bits
+    std::vector<std::vector<H>> eqr(n_claims);
+    std::vector<H> sc_claim(n_claims);
+    for (size_t c = 0; c < n_claims; c++) {
+        const H* x = h_x_outers + c * x_len;
+        label(fs, "flock-ring-switch-v0");
+        flock_fs_observe_f128_slice(fs, shv[c].data(), 128);
+        H lam[64];
+        flock::lagrange_phi8_host(0, h_z_skips[c], lam);
+        const H e0 = ha(h1(), x[0]);
+        H chk = hz();
+        for (int i = 0; i < 128; i++) chk = ha(chk, hm(hm(lam[i & 63], (i >> 6) ? x[0] : e0), shv[c][i]));
+        if (!heqv(chk, h_values[c])) return reject("ring-switch claim");
+        std::vector<H> rdp;
+        if (!vps(fs, rs_nonce[c], 3, 7, &rdp)) return reject("ring-switch proof of work");
+        eqr[c] = heq(rdp.data(), 7);
+        H acc = hz();
+        for (int bt = 0; bt < 128; bt++) {
+            H u = hz();
+            for (int i = 0; i < 128; i++)
+                if (hgetbit(shv[c][i], bt)) u = ha(u, hbit(i));
+            acc = ha(acc, hm(u, eqr[c][bt]));
+        }
+        sc_claim[c] = acc;
+    }
+    // 2. batching coefficients (no PoW, nothing absorbed, for one claim)
+    std::vector<H> gam;
+    if (bb) {
+        if (!vps(fs, batch_n[0], bb, n_claims, &gam)) return reject("batching proof of work");
+    } else {
+        gam = sample_vec(fs, n_claims);
+    }
+    H running = hz();
+    for (size_t c = 0; c < n_claims; c++) running = ha(running, hm(gam[c], sc_claim[c]));
+    // 3. the dense merged sumcheck replay, 2 PoW bits a round
+    std::vector<H> rho;
+    for (int i = 0; i < m; i++) {
+        observe(fs, mrounds[2 * i]);
+        observe(fs, mrounds[2 * i + 1]);
+        H ch;
+        if (!vps1(fs, mnon[i], 2, &ch)) return reject("merged sumcheck proof of work");
+        running = round_claim(running, mrounds[2 * i], mrounds[2 * i + 1], ch);
+        rho.push_back(ch);
+    }
+    // 4. the multipoint twisted evaluation on the forked transcript (challenger.rs:167-191)
+    const H seed0 = sample(fs), seed1 = sample(fs);
+    static const uint8_t kFork[] = "flock-par-assist-v1";
+    flock_fs* ch = flock_fs_new(kFork, sizeof(kFork) - 1);
+    if (!ch) return flock::set_error(FLOCK_HIP_E_NOMEM, "transcript");
+    observe(ch, seed0);
+    observe(ch, seed1);
+    std::string why;
+    H V = hz();
+    {
+        label(ch, "flock-multipoint-twisted-v1");
+        for (const auto& v : vals)
+            for (const H& x : v) observe(ch, x);
+        uint32_t gbits = bits_for_degree(128 * n_claims - 1);
+        if (gbits < 1) gbits = 1;
+        H gmm;
+        std::vector<H> point;
+        H mrun = hz(), g_at = hz();
+        std::vector<H> gpow(128 * n_claims);
+        if (!vps1(ch, gnonce, gbits, &gmm)) {
+            why = "multipoint gamma proof of work";
+        } else {
+            gpow[0] = h1();
+            for (size_t i = 1; i < gpow.size(); i++) gpow[i] = hm(gpow[i - 1], gmm);
+            for (size_t c = 0; c < n_claims; c++)
+                for (int j = 0; j < 128; j++) mrun = ha(mrun, hm(gpow[128 * c + j], vals[c][j]));
+            for (int i = 0; i < m && why.empty(); i++) {
+                observe(ch, trounds[2 * i]);
+                observe(ch, trounds[2 * i + 1]);
+                H rr;
+                if (!vps1(ch, tnon[i], 2, &rr)) why = "multipoint round proof of work";
+                mrun = round_claim(mrun, trounds[2 * i], trounds[2 * i + 1], rr);
+                point.push_back(rr);
+            }
+        }
+        if (why.empty()) {
+            // g_at = sum_j gamma^j eq(rho^(2^-j), point) (twisted_eq_at)
+            std::vector<H> rj(rho);
+            for (int j = 0; j < 128; j++) {
+                g_at = ha(g_at, hm(gpow[j], heq_at(rj.data(), point.data(), m)));
+                for (H& x : rj) x = frob_inv(x);
+            }
+            // the anchor: the Frobenius assist (untwisted) at the endpoint
+            label(ch, "flock-frobenius-assist-v0");
+            observe(ch, av);
+            H aclaim = av;
+            std::vector<H> sigma;
+            for (int i = 0; i < 2 * (m + 1) && why.empty(); i++) {
+                observe(ch, arounds[2 * i]);
+                observe(ch, arounds[2 * i + 1]);
+                H rr;
+                if (!vps1(ch, anon[i], 2, &rr)) why = "assist round proof of work";
+                aclaim = round_claim(aclaim, arounds[2 * i], arounds[2 * i + 1], rr);
+                sigma.push_back(rr);
+            }
+            if (why.empty()) {
+                // per layer l, e_l[cd] = eq((c, d), (rc_l, rd_l))
+                std::vector<H> el(4 * (size_t)(m + 1));
+                for (int l = 0; l <= m; l++) {
+                    const H rc = sigma[2 * l], rd = sigma[2 * l + 1];
+                    const H rc1 = ha(h1(), rc), rd1 = ha(h1(), rd);
+                    el[4 * l + 0] = hm(rc1, rd1);
+                    el[4 * l + 1] = hm(rc, rd1);
+                    el[4 * l + 2] = hm(rc1, rd);
+                    el[4 * l + 3] = hm(rc, rd);
+                }
+                // eq over each boundary run's (c, d) bits
+                std::vector<H> ecd(J.bc.size());
+                for (size_t b = 0; b < J.bc.size(); b++) {
+                    H acc = h1();
+                    for (int l = 0; l <= m; l++) acc = hm(acc, el[4 * l + col_cd(J.bc[b], J.bn[b], l)]);
+                    ecd[b] = acc;
+                }
+                H expect = hz();
+                for (size_t c = 0; c < n_claims; c++) {
+                    const H co = hm(gpow[128 * c], g_at);
+                    if (hzero(co)) continue;
+                    const H* x = h_x_outers + c * x_len;
+                    const std::vector<H> w = run_weights(J, x + 1 + n_log);
+                    H wsum = hz();
+                    for (size_t b = 0; b < w.size(); b++) wsum = ha(wsum, hm(w[b], ecd[b]));
+                    // the branching program at sigma, from the success state down (jagged.rs:2160-2190)
+                    H g[4] = {hz(), hz(), h1(), hz()};
+                    for (int l = m; l >= 0; l--) {
+                        H e4[4];
+                        eq4h((uint32_t)l < n_log ? x[1 + l] : hz(), l < m ? point[l] : hz(), e4);
+                        H prev[4] = {hz(), hz(), hz(), hz()};
+                        for (int cd = 0; cd < 4; cd++) {
+                            H t[4];
+                            bp_step_h(e4, cd, g, t);
+                            for (int s = 0; s < 4; s++) prev[s] = ha(prev[s], hm(el[4 * l + cd], t[s]));
+                        }
+                        for (int s = 0; s < 4; s++) g[s] = prev[s];
+                    }
+                    expect = ha(expect, hm(hm(co, wsum), g[0]));
+                }
+                if (!heqv(aclaim, expect)) why = "Frobenius assist";
+                else if (!heqv(mrun, av)) why = "multipoint endpoint";
+            }
+        }
+        if (why.empty()) {
+            // V = sum_{c,j} coeff_{c,j} A_{c,j}^(2^j), coeff = the linearized coefficients of
+            // Phi_c: bit b -> gamma_c eq(r''_c)[b]
+            const std::vector<H>& D = dual_powers();
+            if (D.empty()) why = "trace-dual basis";
+            for (size_t c = 0; c < n_claims && why.empty(); c++) {
+                H ge[128];
+                for (int b = 0; b < 128; b++) ge[b] = hm(gam[c], eqr[c][b]);
+                for (int j = 0; j < 128; j++) {
+                    H cj = hz();
+                    for (int b = 0; b < 128; b++) cj = ha(cj, hm(ge[b], D[(size_t)j * 128 + b]));
+                    H x = vals[c][j];
+                    for (int s = 0; s < j; s++) x = hm(x, x);
+                    V = ha(V, hm(cj, x));
+                }
+            }
+        }
+    }
+    if (!why.empty()) {
+        flock_fs_free(ch);
+        return reject(why);
+    }
+    // 5. the inner eq-basis Ligerito opening of q^(rho) on the parent transcript
+    label(fs, "flock-pcs-open-batch-v0");
+    label(fs, "flock-pcs-packed-direct-v0");
+    observe(fs, q_eval);
+    const H gamma = sample_vec(fs, 1)[0];
+    const std::vector<uint8_t> capv(cap, cap + cap_len);
+    const bool lig_ok = lig_verify(fs, C, L, dense_log, hm(gamma, q_eval), capv, lanes0, rho, gamma, &why);
+    // merge the child (challenger.rs:193-203)
+    const H d0 = sample(ch), d1 = sample(ch);
+    flock_fs_free(ch);
+    if (!lig_ok) return reject(why);
+    observe(fs, d0);
+    observe(fs, d1);
+    if (!heqv(running, hm(q_eval, V))) return reject("merged sumcheck endpoint");
+    if (consumed) *consumed = r.off;
     return 0;
 }
 

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -88,6 +89,7 @@ struct Job {
     size_t domain_len;
     bool on_device;
     uint64_t min_decl;  // 1 for a single-type request, 0 for a part of a union
+    bool need_inputs = true;  // false for a verifier, which reads no inputs
 };
 
 Job job_of(const flock_prove_request* r) {
@@ -128,7 +130,7 @@ int plan_job(const Job& j, Plan* p) {
     for (size_t i = 0; i < n; i++) {
         const Part& t = j.parts[i];
         if (!shape_of(t.kind, &p->hs[i]) || t.n_decl < j.min_decl || t.n_decl > (1ull << j.nu) ||
-            (t.n_decl && !t.inputs))
+            (t.n_decl && !t.inputs && j.need_inputs))
             return FLOCK_HIP_E_INVALID;
         types[i] = {p->hs[i].k_log, 0, p->hs[i].useful_bits};
     }
@@ -658,6 +660,185 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
     for (auto& x : th) x.join();
     if (wall_ms) *wall_ms = (now_s() - t0) * 1e3;
     return first_err;
+}
+
+}  // extern "C"
+
+namespace {
+
+// bincode 1.3 reader over the proof bytes; an overrun or an impossible length marks it bad
+struct BinR {
+    const uint8_t* p;
+    size_t n, off = 0;
+    bool ok = true;
+    uint64_t u64() {
+        if (!ok || n - off < 8) {
+            ok = false;
+            return 0;
+        }
+        uint64_t v = 0;
+        for (int i = 0; i < 8; i++) v |= (uint64_t)p[off + i] << (8 * i);
+        off += 8;
+        return v;
+    }
+    flock_f128 f() {
+        flock_f128 v;
+        v.lo = u64();
+        v.hi = u64();
+        return v;
+    }
+    // a vector of `count` F128 words, its length prefix checked against `expect`
+    bool vec_f(std::vector<flock_f128>& v, size_t words_per, size_t expect) {
+        const uint64_t k = u64();
+        if (!ok || k != expect) return ok = false;
+        v.resize(k * words_per);
+        for (auto& x : v) x = f();
+        return ok;
+    }
+    bool vec_u64(std::vector<uint64_t>& v, size_t expect) {
+        const uint64_t k = u64();
+        if (!ok || k != expect) return ok = false;
+        v.resize(k);
+        for (auto& x : v) x = u64();
+        return ok;
+    }
+};
+
+int verify_impl(const Job& job, const uint8_t* cap, size_t cap_len, const uint8_t* proof,
+                size_t proof_len, flock_f128* claim_ab, flock_f128* claim_c) {
+    auto reject = [](const char* w) {
+        return flock::set_error(FLOCK_HIP_E_REJECT, (std::string("proof rejected: ") + w).c_str());
+    };
+    Plan p;
+    if (!cap || !proof) return flock::set_error(FLOCK_HIP_E_INVALID, "verify: null cap or proof");
+    for (const Part& t : job.parts)
+        if (t.circuit)
+            return flock::set_error(FLOCK_HIP_E_INVALID, "verify: the built-in circuits only (circuit "
+                                                         "must be NULL)");
+    if (int rc = plan_job(job, &p)) return flock::set_error(rc, "verify: bad request shape");
+    if (!p.has_config) return flock::set_error(FLOCK_HIP_E_INVALID, "verify: no Ligerito config");
+    if (cap_len != (32ull << p.cap_depth)) return reject("commitment cap size");
+    const uint32_t nl = job.nu, n_slots = (uint32_t)job.parts.size();
+    const uint32_t m = p.m_zc, n_mlv = m - 6, col_vars = m - nl, L = col_vars - 6;
+    // host circuits of the slots
+    std::vector<flock_csc> host(n_slots);
+    std::vector<flock_lc_vslot> vs(n_slots);
+    uint32_t n_pins = 0;
+    for (uint32_t s = 0; s < n_slots; s++) {
+        const Part& t = job.parts[p.reg.slots[s].type_index];
+        vs[s] = {nullptr, p.hs[p.reg.slots[s].type_index].k_log, 0, -1, p.reg.slots[s].prefix,
+                 p.counts[s]};
+        if (t.kind == FLOCK_HASH_KECCAK) {
+            vs[s].walker = 1;
+        } else {
+            const flock_circuit* c = nullptr;
+            uint32_t k_log = 0;
+            uint64_t ub = 0, pin = 0;
+            flock_csc& h = host[s];
+            std::memset(&h, 0, sizeof(h));
+            if (int rc = flock_circuit_get(t.kind, &c)) return rc;
+            if (int rc = flock_circuit_info(c, &k_log, &ub, &pin, &h.nnz_a, &h.nnz_b)) return rc;
+            if (int rc = flock_circuit_csc(c, 0, &h.a_col_ptr, &h.a_rows)) return rc;
+            if (int rc = flock_circuit_csc(c, 1, &h.b_col_ptr, &h.b_rows)) return rc;
+            h.k_log = k_log;
+            h.const_pin = (int64_t)pin;
+            vs[s].h_circuit = &h;
+        }
+        n_pins++;  // every built-in type carries a constant-wire pin
+    }
+    // R1csProofMergedLigerito (proof.rs:42-47): ZerocheckProof, LincheckProof, MergedOpenProof
+    BinR r{proof, proof_len};
+    std::vector<flock_f128> r1a, r1c, ml, rounds, zp, mev;
+    std::vector<uint64_t> zn, ln;
+    flock_f128 fin[3];
+    r.vec_f(r1a, 1, 64);
+    r.vec_f(r1c, 1, 64);
+    r.vec_f(ml, 2, n_mlv);
+    for (auto& x : fin) x = r.f();
+    r.vec_u64(zn, 2 + n_mlv);
+    r.vec_f(rounds, 2, L);
+    r.vec_f(zp, 1, 64);
+    r.vec_f(mev, 2, n_slots);
+    r.vec_u64(ln, L + 2 + n_pins);
+    if (!r.ok) return reject("malformed zerocheck / lincheck bytes");
+    uint8_t digest[32];
+    if (job.digest) {
+        std::memcpy(digest, job.digest, 32);
+    } else {
+        std::vector<uint32_t> kinds(n_slots);
+        for (uint32_t i = 0; i < n_slots; i++) kinds[i] = job.parts[i].kind;
+        if (int rc = flock_circuit_union_digest(kinds.data(), n_slots, nl, digest)) return rc;
+    }
+    flock_fs* fs = flock_fs_new(job.domain, job.domain_len);
+    if (!fs) return flock::set_error(FLOCK_HIP_E_NOMEM, "transcript");
+    struct FsGuard {
+        flock_fs* f;
+        ~FsGuard() { flock_fs_free(f); }
+    } guard{fs};
+    // statement binding, zerocheck over M_bool (verify_union_piops, verifier.rs:806-870)
+    if (int rc = flock_union_bind_statement(fs, digest, p.counts.data(), n_slots, cap, cap_len)) return rc;
+    std::vector<flock_f128> r1(128), claim(1 + 2 * n_mlv);
+    std::copy(r1a.begin(), r1a.end(), r1.begin());
+    std::copy(r1c.begin(), r1c.end(), r1.begin() + 64);
+    if (int rc = flock_zerocheck_verify(fs, m, 1, r1.data(), ml.data(), fin, zn.data(), claim.data()))
+        return rc;
+    // x_ab and the union lincheck with the O(nnz) matrix check (MatrixAssertion::check)
+    std::vector<flock_f128> x_rest(L ? L : 1), x_outer(nl), lclaim(2 + L);
+    if (int rc = flock_union_x_ab_from_mlv(&p.reg, claim.data() + 1, n_mlv, x_rest.data(), x_outer.data()))
+        return rc;
+    if (int rc = flock_lincheck_verify_union(fs, vs.data(), n_slots, col_vars, nl, claim[0], x_rest.data(),
+                                             x_outer.data(), fin[0], fin[1], 1, rounds.data(), zp.data(),
+                                             mev.data(), ln.data(), lclaim.data()))
+        return rc;
+    // the merged opening of (ab, c) at their claim points (verifier.rs:604-650)
+    const flock_f128* rir = lclaim.data() + 2;
+    const size_t x_len = m - 6;
+    std::vector<flock_f128> xs(2 * x_len), values = {lclaim[1], fin[2]}, skips = {lclaim[0], claim[0]};
+    xs[0] = rir[0];
+    for (uint32_t i = 0; i < nl; i++) xs[1 + i] = x_outer[i];
+    for (uint32_t i = 1; i < L; i++) xs[nl + i] = rir[i];
+    for (size_t i = 0; i < x_len; i++) xs[x_len + i] = claim[1 + n_mlv + i];
+    std::vector<uint64_t> heights(1ull << (col_vars - 7), 0);
+    for (uint32_t s = 0; s < n_slots; s++) {
+        const HashShape& hs = p.hs[p.reg.slots[s].type_index];
+        const uint64_t used = std::min<uint64_t>((hs.useful_bits + 127) / 128, 1ull << (hs.k_log - 7));
+        const uint64_t c_off = p.reg.slots[s].offset >> (7 + nl);
+        for (uint64_t y = 0; y < used; y++) heights[c_off + y] = p.counts[s];
+    }
+    size_t used = 0;
+    if (int rc = flock_open_merged_verify(fs, &p.pcs, cap, cap_len, 2, values.data(), skips.data(), xs.data(),
+                                          x_len, heights.data(), heights.size(), nl, proof + r.off,
+                                          proof_len - r.off, &used))
+        return rc;
+    if (r.off + used != proof_len) return reject("trailing bytes after the opening");
+    if (claim_ab) *claim_ab = lclaim[1];
+    if (claim_c) *claim_c = fin[2];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flock_verify_fast(const flock_prove_request* req, const uint8_t* cap, size_t cap_len,
+                      const uint8_t* proof, size_t proof_len, flock_f128* claim_ab, flock_f128* claim_c) {
+    if (!req || req->n_decl == 0) return flock::set_error(FLOCK_HIP_E_INVALID, "verify: bad request");
+    Job j = job_of(req);
+    j.need_inputs = false;
+    if (req->circuit)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "verify: the built-in circuits only (circuit "
+                                                     "must be NULL)");
+    return verify_impl(j, cap, cap_len, proof, proof_len, claim_ab, claim_c);
+}
+
+int flock_verify_fast_union(const flock_union_request* req, const uint8_t* cap, size_t cap_len,
+                            const uint8_t* proof, size_t proof_len, flock_f128* claim_ab,
+                            flock_f128* claim_c) {
+    if (!req || !req->parts || req->n_parts == 0 || req->n_parts > FLOCK_MAX_TYPES)
+        return flock::set_error(FLOCK_HIP_E_INVALID, "verify: bad request");
+    Job j = job_of(req);
+    j.need_inputs = false;
+    return verify_impl(j, cap, cap_len, proof, proof_len, claim_ab, claim_c);
 }
 
 }  // extern "C"

@@ -314,6 +314,11 @@ def load():
         "flock_hip_prove_fast": (I, [P, ctypes.POINTER(ProveRequest), ctypes.POINTER(ProveResult),
                                      ctypes.POINTER(ProofBytes)]),
         "flock_union_request_info": (I, [ctypes.POINTER(UnionRequest), ctypes.POINTER(UnionInfo)]),
+        "flock_verify_fast": (I, [ctypes.POINTER(ProveRequest), ctypes.c_char_p, S, ctypes.c_char_p, S,
+                                  ctypes.POINTER(F128), ctypes.POINTER(F128)]),
+        "flock_verify_fast_union": (I, [ctypes.POINTER(UnionRequest), ctypes.c_char_p, S,
+                                        ctypes.c_char_p, S, ctypes.POINTER(F128),
+                                        ctypes.POINTER(F128)]),
         "flock_hip_prove_fast_union": (I, [P, ctypes.POINTER(UnionRequest),
                                            ctypes.POINTER(ProveResult), ctypes.POINTER(ProofBytes)]),
     }
@@ -1081,6 +1086,35 @@ def prove_fast_union(prover: Prover, req: UnionRequest):
     load().flock_hip_free(ctypes.cast(out.proof, ctypes.c_void_p))
     load().flock_hip_free(ctypes.cast(out.cap, ctypes.c_void_p))
     return res, proof, cap, to_int(out.claim_ab), to_int(out.claim_c)
+
+
+def verify_fast(kind: int, n_log: int, n_decl: int, cap: bytes, proof: bytes,
+                domain: bytes = b"flock-readme-bench-v0", digest: bytes | None = None):
+    """flock_verify_fast (host only) for a built-in-circuit request: returns (claim_ab,
+    claim_c); a rejected proof raises FlockHipError with code E_REJECT."""
+    req = ProveRequest()
+    req.kind, req.n_log, req.n_decl = kind, n_log, n_decl
+    req.digest, req.domain, req.domain_len = digest, domain, len(domain)
+    ab, c = F128(), F128()
+    _check(load().flock_verify_fast(ctypes.byref(req), cap, len(cap), proof, len(proof),
+                                    ctypes.byref(ab), ctypes.byref(c)))
+    return to_int(ab), to_int(c)
+
+
+def verify_fast_union(parts, n_log: int, cap: bytes, proof: bytes,
+                      domain: bytes = b"flock-readme-bench-v0", digest: bytes | None = None):
+    """flock_verify_fast_union (host only): parts = [(kind, n_decl)] in any order. Returns
+    (claim_ab, claim_c); a rejected proof raises FlockHipError with code E_REJECT."""
+    arr = (UnionPart * len(parts))()
+    for i, (kind, n) in enumerate(parts):
+        arr[i].kind, arr[i].n_decl = kind, n
+    req = UnionRequest()
+    req.n_log, req.n_parts, req.parts = n_log, len(parts), arr
+    req.digest, req.domain, req.domain_len = digest, domain, len(domain)
+    ab, c = F128(), F128()
+    _check(load().flock_verify_fast_union(ctypes.byref(req), cap, len(cap), proof, len(proof),
+                                          ctypes.byref(ab), ctypes.byref(c)))
+    return to_int(ab), to_int(c)
 
 
 def lig_initial_k(m: int) -> int:

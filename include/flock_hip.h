@@ -356,6 +356,31 @@ int flock_lincheck_verify(flock_fs* fs, const flock_csc* h_circuit, uint32_t k_l
                           const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
                           const flock_f128 h_matrix_evals[2], const uint64_t* h_nonces,
                           flock_f128* h_claim);
+/* One boolean slot of the union lincheck verifier (slot order): h_circuit the type's CSC on
+ * the HOST; else walker != 0 for the Keccak-f[1600] host walk (k_log 16, pin 4096); else the
+ * slot is checked from the reported matrix evals with the given k_log and const_pin (-1: none).
+ * prefix: Slot::prefix; n_rows: the slot's statement count. */
+typedef struct {
+    const flock_csc* h_circuit;
+    uint32_t k_log, walker;
+    int64_t const_pin;
+    uint64_t prefix, n_rows;
+} flock_lc_vslot;
+/* verify_union_with_grinding (lincheck/union.rs:662-960) over several boolean slots, the
+ * verifier of flock_hip_lincheck_prove_union: alpha, one beta per pinned slot in slot order
+ * (the target gains beta_t * sum_{row < n_t} eq(x_outer, row)), the rounds' chain, z_partial,
+ * the fresh skip challenge; then MatrixAssertion::check (:579-628) — every slot's comb at its
+ * row-side prefix weight, plus beta_t at its pin, collapsed at the column-side prefix weight
+ * against z_partial — when every slot has a circuit or the walker, else check_reported
+ * (:538-576) from h_matrix_evals (2 per slot). col_vars = M_bool - nu; layouts as the
+ * prover's. Host only. */
+int flock_lincheck_verify_union(flock_fs* fs, const flock_lc_vslot* slots, size_t n_slots,
+                                uint32_t col_vars, uint32_t nu, flock_f128 z_skip,
+                                const flock_f128* h_x_inner_rest, const flock_f128* h_x_outer,
+                                flock_f128 v_a, flock_f128 v_b, int grinding,
+                                const flock_f128* h_rounds, const flock_f128 h_z_partial[64],
+                                const flock_f128* h_matrix_evals, const uint64_t* h_nonces,
+                                flock_f128* h_claim);
 /* flock_lincheck_verify for a Keccak-f[1600] slot (KeccakLincheckCircuit, keccak.rs:678-835;
  * k_log 16, constant pin at column 4096), the proofs of flock_hip_lincheck_prove_keccak, with
  * MatrixAssertion::check: the circuit's column marginals come from a host walk of the
@@ -601,6 +626,26 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
                           const flock_f128* h_s_hat_v, const uint64_t* h_heights, size_t n_heights,
                           uint32_t n_log, uint8_t** out_bytes, size_t* out_len, flock_f128* h_q_eval);
 
+/* The merged opening's verifier: verify_batch_merged (pcs.rs:1971-2300) for ring-switched
+ * claims — per claim the ring switch (ring_switch.rs:2966-3005: the claim against the
+ * skip (x) eq(x[0]) weights of s_hat_v, 3 PoW bits), the batching gammas, the dense merged
+ * sumcheck replay (2 PoW bits a round); on the forked transcript the multipoint twisted
+ * evaluation and its Frobenius anchor (pcs/jagged.rs:2034-2230, 4335-4585: the anchor's
+ * boundary-bit sumcheck checked against the branching program at its endpoint, the 128 dual
+ * values recombined through the linearized coefficients of each claim's fold map); on the
+ * parent transcript the succinct Ligerito verifier of q^(rho) (pcs.rs:1129-1320,
+ * pcs/ligerito/extension.rs:2409-2880: OOD samples, stratified queries, capped SHA-256
+ * Merkle paths, induced consistency sums, the residual basis against yr); then the merged
+ * endpoint q^(rho) V. Inputs as flock_hip_open_merged's (values = the claims' values,
+ * h_z_skips = their skip points), proof = the MergedOpenProof bytes; *consumed gets the bytes
+ * read. 0 or FLOCK_HIP_E_REJECT naming the failed check. SHA-256 Merkle commitments only.
+ * Host only. */
+int flock_open_merged_verify(flock_fs* fs, const flock_pcs_params* p, const uint8_t* cap, size_t cap_len,
+                             size_t n_claims, const flock_f128* h_values, const flock_f128* h_z_skips,
+                             const flock_f128* h_x_outers, size_t x_len, const uint64_t* h_heights,
+                             size_t n_heights, uint32_t n_log, const uint8_t* proof, size_t proof_len,
+                             size_t* consumed);
+
 /* ---- prove core and serving (host orchestration over the entry points above) ----------- */
 
 /* The three hash circuits of the batched and served workloads (r1cs_hashes/{sha2,blake3,
@@ -734,6 +779,19 @@ int flock_union_request_info(const flock_union_request* req, flock_union_info* o
  * with one matrix-eval pair per slot. A request of one part equals flock_hip_prove_fast. */
 int flock_hip_prove_fast_union(flock_hip_prover* p, const flock_union_request* req,
                                flock_prove_result* res, flock_proof_bytes* out);
+/* The whole-proof verifier: verify_ligerito_union (verifier.rs:156-190, as the mixed
+ * bundle's MixedSetup::verify drives it, mixed.rs:281-301) for a request's registry of
+ * built-in types (circuit NULL; inputs are not read): the PCS parameters re-derived from the
+ * counts, bind_statement of the cap, the zerocheck verifier over M_bool, x_ab, the union
+ * lincheck verifier with the O(nnz) matrix check on the host circuits, and the merged
+ * opening's verifier at the (ab, c) claim points; every byte of the proof must be read.
+ * Returns 0 with the R1csClaim values, or FLOCK_HIP_E_REJECT naming the failed check.
+ * Host only (no GPU). */
+int flock_verify_fast(const flock_prove_request* req, const uint8_t* cap, size_t cap_len,
+                      const uint8_t* proof, size_t proof_len, flock_f128* claim_ab, flock_f128* claim_c);
+int flock_verify_fast_union(const flock_union_request* req, const uint8_t* cap, size_t cap_len,
+                            const uint8_t* proof, size_t proof_len, flock_f128* claim_ab,
+                            flock_f128* claim_c);
 /* Serving loop: `workers` host threads, one prover each, take the requests in arrival order;
  * a request starts once its arrival time has passed and its footprint fits hbm_budget beside
  * what the other workers hold (running ones: max(held, footprint)); a worker gives back its

@@ -1291,7 +1291,7 @@ int or_open_merged(or_fs* fs, const or_f128* q, const or_pcs_params* P, const or
     or_f128 gam[8];
     uint64_t batch_nonce = 0;
     const uint32_t bb = n_claims > 1 ? 6 : 0;
-    batch_nonce = or_fs_grind_pow(fs, bb);
+    batch_nonce = bb ? or_fs_grind_pow(fs, bb) : 0; /* pcs.rs:193-199, 1514-1521 */
     or_fs_sample_f128_vec(fs, gam, n_claims);
     or_f128 target = Z0;
     lin_t* phi = (lin_t*)malloc(sizeof(lin_t) * n_claims);

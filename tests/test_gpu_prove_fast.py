@@ -61,6 +61,8 @@ def test_gpu_reproduces_the_reference_anchor(circuits, kind, n, seed, expected):
     # the library's built-in circuit and registry digest: nothing from the oracle
     proof, cap, ab, c = gpu_prove_fast(kind, inputs, FIXTURE_DOMAIN, circuits, builtin=True)
     assert OP.anchor_digest(proof, cap, ab, c) == expected
+    n_log = OP.min_n_log(n)
+    assert F.verify_fast(kind, n_log, n, cap, proof, domain=FIXTURE_DOMAIN) == (ab, c)
 
 
 def test_gpu_matches_the_oracle_golden_vectors(circuits):

@@ -7,6 +7,7 @@
   two claim values, here with the library's built-in circuits and registry digest. These pin
   the GPU union lincheck (per-slot combs, prefix weights, pins, one matrix-eval pair per type),
   the witness placement of several slots, the jagged heights and the counts in slot order;
+  the library's whole-proof verifier accepts each;
 - a union of one part equals flock_hip_prove_fast (the single-slot SHA-256 x 128 anchor);
 - Keccak-f[1600] + SHA-256 + BLAKE3 at nu = 7 (three slots, the Keccak walker among them, a
   gap-free and a padded region) equals the oracle's prove_fast_union byte for byte.
@@ -65,6 +66,9 @@ def test_gpu_reproduces_the_mixed_union_anchor(prover, counts):
     bundle = OP.mixed_bundle_bytes(2, info["counts"], cap, info["dense_m"],
                                    info["log_batch_size"], info["lanes"], proof)
     assert OP.mixed_anchor_digest(bundle, ab, c) == MIXED[counts]
+    # and the library's verifier accepts it (verify_ligerito_union)
+    assert F.verify_fast_union([(F.HASH_BLAKE3, n_blake3), (F.HASH_SHA256, n_sha2)], 10, cap,
+                               proof, domain=FIXTURE_DOMAIN) == (ab, c)
 
 
 def test_gpu_union_of_one_part_is_prove_fast(prover):
@@ -94,3 +98,5 @@ def test_gpu_three_type_union_equals_the_oracle(prover):
     assert cap == o_cap
     assert (ab, c) == (o_ab, o_c)
     assert proof == o_proof
+    assert F.verify_fast_union([(F.HASH_KECCAK, 40), (F.HASH_SHA256, 100), (F.HASH_BLAKE3, 50)],
+                               7, cap, proof, domain=domain) == (ab, c)

@@ -3,7 +3,8 @@
 - Batched (64 uneven requests at once): hash kind uniform over SHA-256 / BLAKE3 /
   Keccak-f[1600], batch log2 uniform in [10, 16], seeded inputs, eight workers through
   flock_hip_serve. Every proof equals, byte for byte, the same request proved alone; the
-  smallest SHA-256 and BLAKE3 requests also equal the oracle's prove_fast.
+  smallest SHA-256 and BLAKE3 requests also equal the oracle's prove_fast, and the library's
+  verifier accepts every proof.
 - Served (Poisson arrivals): 24 requests, batch log2 in [10, 18] with a Keccak-f[1600] 2^18
   request in the middle, admitted by HBM footprint against 288 GB; every proof equals its
   single-request run.
@@ -84,6 +85,10 @@ def test_batched_64_uneven_requests_equal_their_single_runs_and_the_oracle():
         say(f"case: oracle prove_fast of request {i} (kind {kind}, 2^{sp[i][1]})")
         o = OP.prove_fast(kind, sp[i][2], DOMAIN, n_log=sp[i][1])
         assert proofs[i] == o, f"request {i} differs from the oracle"
+    say("verifying every batched proof")
+    for i, (proof, cap, ab, c) in enumerate(proofs):
+        kind, nl = sp[i][0], sp[i][1]
+        assert F.verify_fast(kind, nl, 1 << nl, cap, proof, domain=DOMAIN) == (ab, c), i
 
 
 def test_served_poisson_with_keccak_2e18_equals_single_runs():
