@@ -109,6 +109,17 @@ or_f128 or_f128_mul(or_f128 a, or_f128 b) {
     return or_f128_mul_portable(a, b);
 }
 
+/* OpenMP thread count for the parallel loops (n <= 0: the runtime's default) */
+void or_set_threads(int n) {
+#if defined(_OPENMP)
+    extern void omp_set_num_threads(int);
+    extern int omp_get_num_procs(void);
+    omp_set_num_threads(n > 0 ? n : omp_get_num_procs());
+#else
+    (void)n;
+#endif
+}
+
 void or_f128_mul_batch(const or_f128* a, const or_f128* b, or_f128* out, size_t n) {
     for (size_t i = 0; i < n; i++) out[i] = or_f128_mul(a[i], b[i]);
 }
@@ -630,10 +641,22 @@ void or_round_pair(const or_f128* a, const or_f128* b, size_t n, const or_f128* 
     or_f128* eq = (or_f128*)malloc(sizeof(or_f128) * half);
     or_build_eq(r + 1, log_n - 1, eq);
     or_f128 g1 = ZERO, gi = ZERO;
-    for (size_t x = 0; x < half; x++) {
-        or_f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
-        g1 = add(g1, or_f128_mul(or_f128_mul(eq[x], a1), b1));
-        gi = add(gi, or_f128_mul(or_f128_mul(eq[x], add(a0, a1)), add(b0, b1)));
+#pragma omp parallel
+    {
+        or_f128 l1 = ZERO, li = ZERO;
+        long long xx;
+#pragma omp for schedule(static)
+        for (xx = 0; xx < (long long)half; xx++) {
+            const size_t x = (size_t)xx;
+            or_f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
+            l1 = add(l1, or_f128_mul(or_f128_mul(eq[x], a1), b1));
+            li = add(li, or_f128_mul(or_f128_mul(eq[x], add(a0, a1)), add(b0, b1)));
+        }
+#pragma omp critical
+        {
+            g1 = add(g1, l1);
+            gi = add(gi, li);
+        }
     }
     free(eq);
     out[0] = or_f128_mul(r[0], g1);
@@ -643,7 +666,10 @@ void or_round_pair(const or_f128* a, const or_f128* b, size_t n, const or_f128* 
 /* fold_in_place_pair, multilinear.rs:1148-1163 */
 void or_fold_pair(const or_f128* a, const or_f128* b, size_t n, or_f128 c, or_f128* ao,
                   or_f128* bo) {
-    for (size_t x = 0; x < n / 2; x++) {
+    long long xx;
+#pragma omp parallel for schedule(static)
+    for (xx = 0; xx < (long long)(n / 2); xx++) {
+        const size_t x = (size_t)xx;
         or_f128 a0 = a[2 * x], a1 = a[2 * x + 1], b0 = b[2 * x], b1 = b[2 * x + 1];
         ao[x] = add(a0, or_f128_mul(c, add(a1, a0)));
         bo[x] = add(b0, or_f128_mul(c, add(b1, b0)));
@@ -842,8 +868,9 @@ void or_f8_ntt_inverse(uint8_t* v, int k, uint8_t beta) {
 /* ------------------------------------------------------------------------- */
 static int bit_at(const uint8_t* p, size_t i) { return (p[i >> 3] >> (i & 7)) & 1; }
 
-/* round1_naive, univariate_skip.rs:68-124 */
-void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
+/* round1_naive, univariate_skip.rs:68-124: per row the F8 NTT pair extends the 64 bits on S to
+ * Lambda, and every product is weighted by eq directly. */
+void or_zc_round1_naive(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
                   or_f128 p_ab[64], or_f128 p_c[64]) {
     const int ks = 6, ell = 64;
     size_t nx = (size_t)1 << (m - ks);
@@ -873,6 +900,93 @@ void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, c
     free(eq);
 }
 
+/* The same round 1 as sums over buckets. The S -> Lambda extension of a row's 64 bits is
+ * F2-linear in the bits, so ext(row) is the XOR of ext(e_s) over the set bits s, read through
+ * 16 nibble tables of 64-byte vectors; and p_ab[i] = sum_x eq(x) phi(ab_i(x)) =
+ * sum_v phi(v) B_i[v] with B_i[v] the XOR of eq(x) over the rows where ab_i(x) = v. Rows are
+ * split over OpenMP threads, each with its own buckets (XOR sums: the order does not matter). */
+static uint64_t g_ext[16][16][8];
+static uint8_t g_mul8[256][256];
+static void round1_tables(void) {
+    static int ready = 0;
+    if (ready) return;
+    uint8_t unit[64][64];
+    for (int s = 0; s < 64; s++) {
+        uint8_t v[64];
+        memset(v, 0, sizeof(v));
+        v[s] = 1;
+        or_f8_ntt_inverse(v, 6, 0);
+        or_f8_ntt_forward(v, 6, 64);
+        memcpy(unit[s], v, 64);
+    }
+    for (int k = 0; k < 16; k++)
+        for (int nib = 0; nib < 16; nib++) {
+            uint8_t acc[64];
+            memset(acc, 0, sizeof(acc));
+            for (int bt = 0; bt < 4; bt++)
+                if ((nib >> bt) & 1)
+                    for (int i = 0; i < 64; i++) acc[i] ^= unit[4 * k + bt][i];
+            memcpy(g_ext[k][nib], acc, 64);
+        }
+    for (int x = 0; x < 256; x++)
+        for (int y = 0; y < 256; y++) g_mul8[x][y] = or_f8_mul((uint8_t)x, (uint8_t)y);
+    ready = 1;
+}
+static void ext_row(const uint8_t* p, uint8_t out[64]) {
+    uint64_t bits, acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    memcpy(&bits, p, 8);
+    for (int k = 0; k < 16; k++) {
+        const uint64_t* t = g_ext[k][(bits >> (4 * k)) & 15];
+        for (int i = 0; i < 8; i++) acc[i] ^= t[i];
+    }
+    memcpy(out, acc, 64);
+}
+
+void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
+                  or_f128 p_ab[64], or_f128 p_c[64]) {
+    const int ks = 6;
+    const size_t nx = (size_t)1 << (m - ks);
+    or_f128* eq = (or_f128*)malloc(sizeof(or_f128) * nx);
+    or_build_eq(r + ks, (size_t)(m - ks), eq);
+    round1_tables();
+    or_f128* tot = (or_f128*)calloc(2 * 64 * 256, sizeof(or_f128));
+#pragma omp parallel
+    {
+        or_f128* bk = (or_f128*)calloc(2 * 64 * 256, sizeof(or_f128));
+        uint8_t ac[64], bc[64], cc[64];
+        long long xx;
+#pragma omp for schedule(static)
+        for (xx = 0; xx < (long long)nx; xx++) {
+            const size_t x = (size_t)xx;
+            ext_row(a + 8 * x, ac);
+            ext_row(b + 8 * x, bc);
+            ext_row(c + 8 * x, cc);
+            for (int i = 0; i < 64; i++) {
+                or_f128* u = &bk[i * 256 + g_mul8[ac[i]][bc[i]]];
+                or_f128* w = &bk[(64 + i) * 256 + cc[i]];
+                *u = add(*u, eq[x]);
+                *w = add(*w, eq[x]);
+            }
+        }
+#pragma omp critical
+        for (size_t j = 0; j < 2 * 64 * 256; j++) tot[j] = add(tot[j], bk[j]);
+        free(bk);
+    }
+    or_f128 phi[256];
+    for (int v = 0; v < 256; v++) phi[v] = or_phi8((uint8_t)v);
+    for (int i = 0; i < 64; i++) {
+        or_f128 sab = ZERO, sc = ZERO;
+        for (int v = 1; v < 256; v++) {
+            sab = add(sab, or_f128_mul(phi[v], tot[i * 256 + v]));
+            sc = add(sc, or_f128_mul(phi[v], tot[(64 + i) * 256 + v]));
+        }
+        p_ab[i] = sab;
+        p_c[i] = sc;
+    }
+    free(tot);
+    free(eq);
+}
+
 /* round1_extract_c_packed_with_s_hat_v's c side (univariate_skip.rs:384-489): two banks
  * keyed by bit 0 of the row index x, each weighted by eq(r[6..m), x), then each divided by
  * eq(r[6], bank). out[lane | bank << 6]. */
@@ -883,9 +997,23 @@ void or_zc_s_hat_v_c(const uint8_t* c, int m, const or_f128* r, or_f128 out[128]
     or_build_eq(r + ks, (size_t)(m - ks), eq);
     or_f128 bank[2][64];
     for (int i = 0; i < ell; i++) bank[0][i] = bank[1][i] = ZERO;
-    for (size_t x = 0; x < nx; x++)
-        for (int s = 0; s < ell; s++)
-            if (bit_at(c, x * ell + s)) bank[x & 1][s] = add(bank[x & 1][s], eq[x]);
+#pragma omp parallel
+    {
+        or_f128 lb[2][64];
+        for (int i = 0; i < ell; i++) lb[0][i] = lb[1][i] = ZERO;
+        long long xx;
+#pragma omp for schedule(static)
+        for (xx = 0; xx < (long long)nx; xx++) {
+            const size_t x = (size_t)xx;
+            for (int s = 0; s < ell; s++)
+                if (bit_at(c, x * ell + s)) lb[x & 1][s] = add(lb[x & 1][s], eq[x]);
+        }
+#pragma omp critical
+        for (int i = 0; i < ell; i++) {
+            bank[0][i] = add(bank[0][i], lb[0][i]);
+            bank[1][i] = add(bank[1][i], lb[1][i]);
+        }
+    }
     const or_f128 inv0 = or_f128_inv(add(ONE, r[ks])), inv1 = or_f128_inv(r[ks]);
     for (int i = 0; i < ell; i++) {
         out[i] = or_f128_mul(bank[0][i], inv0);
@@ -925,7 +1053,10 @@ void or_fold_at_z(const uint8_t* bits, int m, or_f128 z, or_f128* out) {
     or_f128 w[64];
     or_lagrange_phi8(0, z, w);
     size_t nx = (size_t)1 << (m - 6);
-    for (size_t x = 0; x < nx; x++) {
+    long long xx;
+#pragma omp parallel for schedule(static)
+    for (xx = 0; xx < (long long)nx; xx++) {
+        const size_t x = (size_t)xx;
         or_f128 acc = ZERO;
         for (int s = 0; s < 64; s++)
             if (bit_at(bits, x * 64 + s)) acc = add(acc, w[s]);

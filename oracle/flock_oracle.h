@@ -139,6 +139,10 @@ void or_f8_ntt_inverse(uint8_t* v, int k, uint8_t beta);
  * r has m entries (only r[6..] used). p_ab, p_c: 64 values on Lambda. */
 void or_zc_round1(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
                   or_f128 p_ab[64], or_f128 p_c[64]);
+void or_zc_round1_naive(const uint8_t* a, const uint8_t* b, const uint8_t* c, int m, const or_f128* r,
+                  or_f128 p_ab[64], or_f128 p_c[64]);
+/* OpenMP threads of the parallel loops (n <= 0: the runtime default) */
+void or_set_threads(int n);
 /* s_hat_v_c of round1_extract_c_packed_with_s_hat_v (univariate_skip.rs:384-489), 128 values */
 void or_zc_s_hat_v_c(const uint8_t* c, int m, const or_f128* r, or_f128 out[128]);
 /* fold_1b_rows (pcs/ring_switch.rs:182-230, 410-470) on n 128-bit rows */

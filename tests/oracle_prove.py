@@ -129,9 +129,15 @@ def circuit(kind):
     return _CIRCUITS[kind]
 
 
+_DIGESTS = {}
+
+
 def registry_digest(kind, nu):
-    ty, _, _ = circuit(kind)
-    return O.registry_digest([ty], nu)
+    """Registry::digest of the one-type registry (setup work, cached per (kind, nu))."""
+    if (kind, nu) not in _DIGESTS:
+        ty, _, _ = circuit(kind)
+        _DIGESTS[(kind, nu)] = O.registry_digest([ty], nu)
+    return _DIGESTS[(kind, nu)]
 
 
 def witness(kind, inputs, n_log):

@@ -101,6 +101,23 @@ def _round1(a, b, c, m, r):
     return O.arr_to_ints(pab), O.arr_to_ints(pc)
 
 
+def test_bucketed_round1_equals_the_naive_one():
+    # or_zc_round1 (nibble-table extension, eq sums per value, parallel rows) against
+    # round1_naive's per-row F8 NTTs and direct eq products, on satisfying and random witnesses
+    L.or_zc_round1_naive.argtypes = L.or_zc_round1.argtypes
+    for m, sat, seed in ((6, True, 1), (9, False, 2), (11, True, 3), (12, False, 4)):
+        a, b, c = _witness(m, seed, satisfying=sat)
+        r = [int.from_bytes(np.random.default_rng(seed).bytes(16), "little") for _ in range(m)]
+        ra = O.ints_to_arr(r)
+        outs = []
+        for fn in (L.or_zc_round1, L.or_zc_round1_naive):
+            pab = np.zeros((64, 2), dtype=np.uint64)
+            pc = np.zeros((64, 2), dtype=np.uint64)
+            fn(a.ctypes.data, b.ctypes.data, c.ctypes.data, m, O._p(ra), O._p(pab), O._p(pc))
+            outs.append((pab.tobytes(), pc.tobytes()))
+        assert outs[0] == outs[1], (m, sat)
+
+
 def _fold(bits, m, z):
     out = np.zeros((1 << (m - 6), 2), dtype=np.uint64)
     L.or_fold_at_z(bits.ctypes.data, m, O.f(z), O._p(out))
