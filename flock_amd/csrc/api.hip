@@ -1221,6 +1221,21 @@ int flock_hip_sumcheck_fold_msg(flock_hip_ctx* c, const flock_f128* f, const flo
     return c->fetch_pair(msg);
 }
 
+extern "C++" {
+namespace flock {
+int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* out) {
+    CTX_CHECK(c);
+    if (nv > (size_t)kEqArgMax || !out || (nv && !h_r)) return fail(FLOCK_HIP_E_INVALID, "bad eq arguments");
+    const size_t scratch_words = nv + ((size_t)1 << (nv / 2)) + ((size_t)1 << (nv - nv / 2));
+    if (int rc = c->ensure_scratch(scratch_words * 16)) return rc;
+    uint4* dr = reinterpret_cast<uint4*>(c->scratch);
+    LAUNCH(flock::launch_eq_coords(U(h_r), (uint32_t)nv, dr, c->stream));
+    LAUNCH(flock::launch_build_eq(dr, (int)nv, U(out), dr + nv, c->stream));
+    return 0;
+}
+}  // namespace flock
+}
+
 int flock_hip_build_eq(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* out) {
     CTX_CHECK(c);
     if (nv > 40 || !out || (nv && !h_r)) return fail(FLOCK_HIP_E_INVALID, "bad eq arguments");

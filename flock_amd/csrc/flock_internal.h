@@ -95,6 +95,13 @@ hipError_t launch_fold_pair(const uint4* a, const uint4* b, uint64_t n, uint4 c,
 // eq table build_eq(r[0..nv)) into out (2^nv entries). r is a device array; scratch holds
 // at least 2^(nv/2) + 2^(nv - nv/2) entries.
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s);
+// copies n <= kEqArgMax host coordinates to dst through the launch's arguments
+constexpr int kEqArgMax = 40;
+hipError_t launch_eq_coords(const uint4* h_r, uint32_t n, uint4* dst, hipStream_t s);
+// build_eq of host coordinates queued on the context's stream without a wait: the coordinates
+// travel in a launch's arguments, so h_r may be released on return; d_out is ready when the
+// stream reaches it
+int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* d_out);
 // out[x_lo + 2^nlo x_hi] = lo[x_lo] * hi[x_hi] (2^(nlo+nhi) words).
 hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int nhi, uint4* out,
                                hipStream_t s);

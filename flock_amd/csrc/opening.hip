@@ -416,6 +416,19 @@ __global__ void k_zero32(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0;
 }
 
+// Hand `words` (<= 256) accumulator words to the host's coherent pinned mirror `pub`, clear them
+// for the next accumulation, and announce them by storing `seq` into `flag` at system scope. One
+// wave: its system-scope fence orders every lane's stores before the flag.
+__global__ void __launch_bounds__(64) k_acc_publish(uint32_t* acc, uint32_t words, uint32_t* pub,
+                                                    uint32_t* flag, uint32_t seq) {
+    for (uint32_t i = threadIdx.x; i < words; i += 64) {
+        pub[i] = acc[i];
+        acc[i] = 0u;
+    }
+    __threadfence_system();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // F2-linear map through 32 nibble tables: T[k * 16 + n] = Phi(n x^(4k)) (16 entries per row)
 __device__ __forceinline__ f128 lin_apply_d(const f128* T, f128 x) {
     f128 acc = f128_zero();
@@ -1173,8 +1186,10 @@ struct DBuf {
 
 struct flock_open_ws {
     std::vector<DBuf> bufs;
-    uint32_t* acc = nullptr;        // device accumulators (256 words)
-    uint32_t* hacc = nullptr;       // pinned mirror
+    uint32_t* acc = nullptr;        // device accumulators (256 words), zero between uses
+    uint32_t* hacc = nullptr;       // coherent pinned mirror: 256 words, then the flag word
+    uint32_t* dacc = nullptr;       // its device view
+    uint32_t acc_seq = 0;           // last sequence number announced through the flag
     double stage_ms[6] = {};        // host wall time of the last opening's stages
     std::chrono::steady_clock::time_point mark;
     void stamp(int i) {
@@ -1250,15 +1265,37 @@ struct Ctx {
         if (!p) fail(FLOCK_HIP_E_NOMEM, "opening workspace");
         return static_cast<T*>(p);
     }
-    bool zero_acc(uint32_t words) {
-        hipLaunchKernelGGL(flock::k_zero32, dim3(1), dim3(256), 0, st, ws->acc, words);
+    // Every accumulation kernel XORs into ws->acc, which read_acc leaves zero again; an entry
+    // point clears all of it once before its first accumulation (an earlier call may have
+    // stopped between a kernel and its read).
+    bool clear_acc() {
+        hipLaunchKernelGGL(flock::k_zero32, dim3(1), dim3(256), 0, st, ws->acc, 256u);
         return hip(hipGetLastError(), "zero");
     }
-    // read back `words` accumulator words after the stream drains
+    // Read back `words` accumulator words (and clear them): a one-wave kernel queued behind the
+    // accumulation copies them into the pinned mirror and then raises the flag word, which the
+    // host polls instead of synchronising the stream. Bounded: past 10 s, or once the stream
+    // reports an error, the wait ends with an error.
     bool read_acc(uint32_t words) {
-        if (!hip(hipMemcpyAsync(ws->hacc, ws->acc, 4 * words, hipMemcpyDeviceToHost, st), "acc"))
-            return false;
-        return hip(hipStreamSynchronize(st), "sync");
+        if (++ws->acc_seq == 0) ++ws->acc_seq;
+        const uint32_t seq = ws->acc_seq;
+        hipLaunchKernelGGL(flock::k_acc_publish, dim3(1), dim3(64), 0, st, ws->acc, words, ws->dacc,
+                           ws->dacc + 256, seq);
+        if (!hip(hipGetLastError(), "acc publish")) return false;
+        const uint32_t* flag = ws->hacc + 256;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint64_t it = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; it++) {
+            if ((it & 4095) == 0) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady) return hip(q, "accumulation kernel");
+                if (q == hipSuccess && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+                    return fail(FLOCK_HIP_E_INTERNAL, "accumulators not published");
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10))
+                    return fail(FLOCK_HIP_E_INTERNAL, "accumulators: no result within 10 s");
+            }
+            _mm_pause();
+        }
+        return true;
     }
     H acc_f(int k) const {
         const uint32_t* v = ws->hacc + 4 * k;
@@ -1274,7 +1311,7 @@ struct Ctx {
     bool chk(const char* where) { return hip(hipGetLastError(), where); }
     // device eq table of host coordinates
     bool eq_dev(const H* r, size_t n, flock_f128* out) {
-        if (int rc = flock_hip_build_eq(c, r, n, out)) return fail(rc, flock_hip_last_error());
+        if (int rc = flock::build_eq_async(c, r, n, out)) return fail(rc, flock_hip_last_error());
         return true;
     }
 };
@@ -1704,7 +1741,6 @@ struct Ladder {
         std::swap(b, b2);
     }
     bool msg(uint64_t d, E* u0, E* u2) {
-        if (!X.zero_acc(16)) return false;
         const uint64_t dd = (d == 1 || n == d) ? 1 : d;
         hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(n / 2)), dim3(kThreads), kCombWsBytes, X.st, f, b, n,
                            dd, X.ws->acc);
@@ -1733,7 +1769,6 @@ struct Ladder {
         const void* src_b = BASE ? bb : static_cast<const void*>(b);
         e256d* dst_f = BASE ? f : f2;
         e256d* dst_b = BASE ? b : b2;
-        if (!X.zero_acc(16)) return false;
         if (nn >= kSplitFoldMsg) {
             hipLaunchKernelGGL((flock::k_fold_msg<BASE, false>), dim3(X.grid(nn)), dim3(kThreads), 0, X.st, src_f,
                                src_b, n, d, E4(r), d2, dst_f, dst_b, X.ws->acc);
@@ -1822,7 +1857,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     std::vector<uint64_t> claim_n, cons_n, grind_n;
     for (int o = 0; o < C->lv[0].ood; o++) {  // L0 OOD samples
         const std::vector<H> z = sample_vec(fs, (size_t)log_n);
-        if (!X.eq_dev(z.data(), (size_t)log_n, ez_) || !X.zero_acc(4)) return false;
+        if (!X.eq_dev(z.data(), (size_t)log_n, ez_)) return false;
         hipLaunchKernelGGL(flock::k_dot_comb, dim3(X.grid(N)), dim3(kThreads), kCombWsBytes, X.st,
                            reinterpret_cast<const uint4*>(q),
                            reinterpret_cast<const uint4*>(ez_), N, X.ws->acc);
@@ -1840,7 +1875,6 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     }
     T.mark("ood0");
     // first message over blocked pairs (F128)
-    if (!X.zero_acc(8)) return false;
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
                        reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D, X.ws->acc);
     if (!X.chk("first msg") || !X.read_acc(8)) return false;
@@ -1892,7 +1926,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     auto level_ood = [&](int level) -> bool {
         for (int o = 0; o < C->lv[level].ood; o++) {
             const std::vector<H> z = sample_vec(fs, (size_t)split_dim);
-            if (!X.eq_dev(z.data(), (size_t)split_dim, ezl) || !X.zero_acc(20)) return false;
+            if (!X.eq_dev(z.data(), (size_t)split_dim, ezl)) return false;
             hipLaunchKernelGGL(flock::k_intro_base, dim3(X.grid(Lg.n / 2)), dim3(kThreads), kCombWsBytes, X.st, Lg.f,
                                reinterpret_cast<const uint4*>(ezl), Lg.n, X.ws->acc);
             if (!X.chk("intro ood") || !X.read_acc(20)) return false;
@@ -1915,7 +1949,6 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         return true;
     };
     auto presplit = [&](int dims, int next_level) -> bool {
-        if (!X.zero_acc(16)) return false;
         hipLaunchKernelGGL(flock::k_intro_presplit, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.f,
                            reinterpret_cast<const uint4*>(basis), 1ull << dims, X.ws->acc);
         if (!X.chk("intro presplit") || !X.read_acc(16)) return false;
@@ -2052,12 +2085,16 @@ int flock_open_ws_create(flock_open_ws** out) {
     flock_open_ws* w = new (std::nothrow) flock_open_ws();
     if (!w) return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace");
     if (hipMalloc(&w->acc, 4 * 256) != hipSuccess ||
-        hipHostMalloc((void**)&w->hacc, 4 * 256) != hipSuccess) {
+        hipHostMalloc((void**)&w->hacc, 4 * 260, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&w->dacc, w->hacc, 0) != hipSuccess) {
         (void)hipGetLastError();
         if (w->acc) (void)hipFree(w->acc);
+        if (w->hacc) (void)hipHostFree(w->hacc);
         delete w;
         return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace accumulators");
     }
+    // the flag word starts below every sequence number
+    std::memset(w->hacc, 0, 4 * 260);
     *out = w;
     return 0;
 }
@@ -2120,7 +2157,8 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     *out_bytes = nullptr;
     *out_len = 0;
     Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
-    if (!X.hip(comb_attrs(), "kernel attributes")) return flock::set_error(X.err, X.what.c_str());
+    if (!X.hip(comb_attrs(), "kernel attributes") || !X.clear_acc())
+        return flock::set_error(X.err, X.what.c_str());
     const int dense_log = (int)P->m - 7;
     const uint64_t N = 1ull << dense_log;
     const Jag J = jag_make(h_heights, n_heights, (int)n_log, dense_log);
@@ -2194,7 +2232,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         jc.tab[c] = reinterpret_cast<const uint4*>(d_tab + 512 * c);
     }
     if (!X.h2d(d_ps, J.ps.data(), 8 * J.ps.size()) || !X.h2d(d_ec, ec_all.data(), 16 * ec_all.size()) ||
-        !X.h2d(d_tab, tab_all.data(), 16 * tab_all.size()) || !X.zero_acc(8))
+        !X.h2d(d_tab, tab_all.data(), 16 * tab_all.size()))
         return flock::set_error(X.err, X.what.c_str());
     // tiles of at most kJagTile rows of one column; W past the area is zero
     std::vector<flock::JagTile> tiles;

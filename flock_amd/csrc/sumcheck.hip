@@ -509,6 +509,24 @@ hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int n
     return hipGetLastError();
 }
 
+// Coordinates carried in the launch's arguments (640 B of the 4 KiB argument space), so that a
+// host array reaches the device with no copy and no wait for the stream.
+struct EqCoords {
+    uint4 r[kEqArgMax];
+};
+__global__ void k_eq_coords(EqCoords c, uint32_t n, uint4* __restrict__ dst) {
+    if (threadIdx.x < n) dst[threadIdx.x] = c.r[threadIdx.x];
+}
+
+hipError_t launch_eq_coords(const uint4* h_r, uint32_t n, uint4* dst, hipStream_t s) {
+    if (n > (uint32_t)kEqArgMax) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    EqCoords c{};
+    for (uint32_t i = 0; i < n; i++) c.r[i] = h_r[i];
+    k_eq_coords<<<1, 64, 0, s>>>(c, n, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s) {
     // k_eq_direct with zero variables writes the single entry 1.
     const int nlo = nv / 2, nhi = nv - nlo;
