@@ -1223,6 +1223,8 @@ int flock_hip_sumcheck_fold_msg(flock_hip_ctx* c, const flock_f128* f, const flo
 
 extern "C++" {
 namespace flock {
+int ctx_device(const flock_hip_ctx* c) { return c ? c->device : -1; }
+
 int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* out) {
     CTX_CHECK(c);
     if (nv > (size_t)kEqArgMax || !out || (nv && !h_r)) return fail(FLOCK_HIP_E_INVALID, "bad eq arguments");

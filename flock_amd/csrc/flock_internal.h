@@ -102,6 +102,8 @@ hipError_t launch_eq_coords(const uint4* h_r, uint32_t n, uint4* dst, hipStream_
 // travel in a launch's arguments, so h_r may be released on return; d_out is ready when the
 // stream reaches it
 int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* d_out);
+// the device a context was made on (-1 for none)
+int ctx_device(const flock_hip_ctx* c);
 // out[x_lo + 2^nlo x_hi] = lo[x_lo] * hi[x_hi] (2^(nlo+nhi) words).
 hipError_t launch_outer_product(const uint4* lo, int nlo, const uint4* hi, int nhi, uint4* out,
                                hipStream_t s);

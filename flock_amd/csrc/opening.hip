@@ -24,6 +24,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #if defined(__x86_64__)
@@ -1197,8 +1198,24 @@ struct flock_open_ws {
         if (i >= 0) stage_ms[i] = std::chrono::duration<double, std::milli>(t - mark).count();
         mark = t;
     }
+    // the second context the multipoint block runs on beside the inner opening (lazily made on
+    // the prover's device)
+    flock_hip_ctx* aux = nullptr;
+    int aux_dev = -1;
+    flock_hip_ctx* aux_ctx(int device) {
+        if (aux && aux_dev == device) return aux;
+        if (aux) flock_hip_ctx_destroy(aux);
+        aux = nullptr;
+        if (device < 0 || flock_hip_ctx_create(device, &aux) != 0) return nullptr;
+        aux_dev = device;
+        return aux;
+    }
+    // Slots are fixed per use and bufs never grows past kSlots, so the two threads of an opening
+    // (which use disjoint slots) never touch the same entry or resize the vector.
+    static constexpr size_t kSlots = 64;
     void* get(size_t slot, size_t bytes) {
-        if (bufs.size() <= slot) bufs.resize(slot + 1);
+        if (slot >= kSlots) return nullptr;
+        if (bufs.size() < kSlots) bufs.resize(kSlots);
         DBuf& b = bufs[slot];
         if (b.bytes >= bytes) return b.p;
         if (b.p) (void)hipFree(b.p);
@@ -2095,12 +2112,14 @@ int flock_open_ws_create(flock_open_ws** out) {
     }
     // the flag word starts below every sequence number
     std::memset(w->hacc, 0, 4 * 260);
+    w->bufs.resize(flock_open_ws::kSlots);
     *out = w;
     return 0;
 }
 
 void flock_open_ws_destroy(flock_open_ws* w) {
     if (!w) return;
+    if (w->aux) flock_hip_ctx_destroy(w->aux);
     for (DBuf& b : w->bufs)
         if (b.p) (void)hipFree(b.p);
     if (w->acc) (void)hipFree(w->acc);
@@ -2243,22 +2262,22 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     flock::JagTile* d_tiles = X.buf<flock::JagTile>(50, tiles.size() + 1);
     if (X.err || (!tiles.empty() && !X.h2d(d_tiles, tiles.data(), sizeof(flock::JagTile) * tiles.size())))
         return flock::set_error(X.err, X.what.c_str());
-    auto jag_table = [&](bool weight, flock_f128* dst) -> bool {
+    auto jag_table = [&](Ctx& Y, bool weight, flock_f128* dst) -> bool {
         if (area < N)
-            hipLaunchKernelGGL(flock::k_zero_f128, dim3(X.grid(N - area)), dim3(kThreads), 0, X.st,
+            hipLaunchKernelGGL(flock::k_zero_f128, dim3(Y.grid(N - area)), dim3(kThreads), 0, Y.st,
                                reinterpret_cast<uint4*>(dst + area), N - area);
         if (!tiles.empty()) {
             if (weight)
-                hipLaunchKernelGGL(flock::k_jag_tiles<true>, dim3((uint32_t)tiles.size()), dim3(kThreads), 0, X.st,
+                hipLaunchKernelGGL(flock::k_jag_tiles<true>, dim3((uint32_t)tiles.size()), dim3(kThreads), 0, Y.st,
                                    reinterpret_cast<uint4*>(dst), d_tiles, d_ps, jc);
             else
                 hipLaunchKernelGGL(flock::k_jag_tiles<false>, dim3((uint32_t)tiles.size()), dim3(kThreads), 0,
-                                   X.st, reinterpret_cast<uint4*>(dst), d_tiles, d_ps, jc);
+                                   Y.st, reinterpret_cast<uint4*>(dst), d_tiles, d_ps, jc);
         }
-        return X.chk("jagged table");
+        return Y.chk("jagged table");
     };
     // W and the round-0 message of <q, W> over LSB pairs
-    if (!jag_table(true, W)) return flock::set_error(X.err, X.what.c_str());
+    if (!jag_table(X, true, W)) return flock::set_error(X.err, X.what.c_str());
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
                        reinterpret_cast<const uint4*>(d_q), reinterpret_cast<const uint4*>(W), N, (uint64_t)1,
                        X.ws->acc);
@@ -2307,8 +2326,14 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     observe(ch, seed1);
     Bin mp;  // MultipointTwistedProof
     const int m = dense_log;
-    bool ok = true;
-    {
+    // The multipoint evaluation, its twisted sumcheck and the Frobenius assist draw only on the
+    // forked transcript ch, and the inner Ligerito opening only on fs (ch is merged back after
+    // both), so the two run at once: this block on a second host thread with a context (stream,
+    // reduction scratch) of its own, the opening on this one. Their workspace slots are disjoint.
+    double t_mp = 0.0, t_fa = 0.0;
+    auto multipoint = [&](Ctx& X) -> bool {
+        bool ok = true;
+        const auto t0 = std::chrono::steady_clock::now();
         // rho^(2^-j), j < 128
         std::vector<H> rp(128 * (size_t)m);
         for (int i = 0; i < m; i++) rp[i] = rho[i];
@@ -2386,7 +2411,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
             for (size_t c = 0; c < n_claims && ok; c++)
                 for (uint32_t y = 0; y < n_cols; y++) ecs[c * n_cols + y] = hm(gpow[128 * c], ec_all[c * n_cols + y]);
             if (ok && X.h2d(d_ec, ecs.data(), 16 * ecs.size())) {
-                ok = jag_table(false, W);
+                ok = jag_table(X, false, W);
             } else {
                 ok = false;
             }
@@ -2424,15 +2449,42 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
                 mp.u64(trounds.size() / 2);
                 for (const H& x : trounds) mp.f(x);
                 mp.vec_u64(tnon.data(), tnon.size());
-                ws->stamp(2);
+                const auto t1 = std::chrono::steady_clock::now();
+                t_mp = std::chrono::duration<double, std::milli>(t1 - t0).count();
                 frobenius_assist(ch, J, zr, ww, point, 2, mp);
-                ws->stamp(3);
+                t_fa = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
             }
         }
+        // the stream is idle when the thread ends
+        return X.hip(hipStreamSynchronize(X.st), "assist sync") && ok;
+    };
+    bool ok = true;
+    flock_hip_ctx* actx = ws->aux_ctx(flock::ctx_device(ctx));
+    if (!actx) {
+        flock_fs_free(ch);
+        return flock::set_error(FLOCK_HIP_E_HIP, "open_merged: second context");
+    }
+    Ctx X2{actx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(actx))};
+    bool ok_mp = false;
+    std::thread th;
+    try {
+        th = std::thread([&] {
+            if (X2.hip(hipSetDevice(flock::ctx_device(actx)), "assist device")) ok_mp = multipoint(X2);
+        });
+    } catch (...) {
+        ok_mp = multipoint(X2);  // no thread: one after the other
     }
     Bin inner;
-    if (ok) ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);
+    ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);
     ws->stamp(4);
+    if (th.joinable()) th.join();
+    ws->stage_ms[2] = t_mp;
+    ws->stage_ms[3] = t_fa;
+    if (ok && !ok_mp) {
+        X.err = X2.err ? X2.err : FLOCK_HIP_E_INTERNAL;
+        X.what = X2.what;
+        ok = false;
+    }
     // merge the child (challenger.rs:193-203)
     const H d0 = sample(ch), d1 = sample(ch);
     observe(fs, d0);
