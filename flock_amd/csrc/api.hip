@@ -536,6 +536,17 @@ struct flock_hip_ctx {
         HIP_TRY(hipStreamWaitEvent(stream, ev_join, 0));
         return 0;
     }
+    // two-round (lookahead) reductions: 32 device words and a ticket counter (zero between
+    // launches), published to the coherent pinned hla (32 words, then the flag word)
+    uint32_t* la_acc = nullptr;
+    uint32_t* hla = nullptr;
+    uint32_t* dla = nullptr;
+    uint32_t la_seq = 0;
+    flock::LaOut la_out() {
+        if (++la_seq == 0) ++la_seq;
+        return flock::LaOut{la_acc, dla, dla + 32, la_seq};
+    }
+    int await_la(flock_f128 out[8]);
     uint4* partials = nullptr;  // kMaxReduceBlocks * 2 words, then the grid-reduce ticket counter
     uint4* dout = nullptr;      // device view of hout: round kernels write their message there
     flock_f128* hout = nullptr; // pinned, coherent: the pair, then a flag word
@@ -778,6 +789,28 @@ int flock_hip_ctx::await_pair(flock_f128 out[2]) {
     return 0;
 }
 
+// Poll hla's flag word for the eight sums published with la_seq (bounded like await_pair).
+int flock_hip_ctx::await_la(flock_f128 out[8]) {
+    const uint32_t* flag = hla + 32;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t it = 1;; it++) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == la_seq) break;
+        if ((it & 4095) == 0) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(FLOCK_HIP_E_HIP, std::string("round kernel: ") + hipGetErrorString(q));
+            if (q == hipSuccess && __atomic_load_n(flag, __ATOMIC_ACQUIRE) != la_seq)
+                return fail(FLOCK_HIP_E_INTERNAL, "round kernel finished without its sums");
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(10))
+                return fail(FLOCK_HIP_E_INTERNAL, "round kernel: no sums within 10 s");
+        }
+        _mm_pause();
+    }
+    for (int k = 0; k < 8; k++)
+        out[k] = flock_f128{(uint64_t)hla[4 * k + 1] << 32 | hla[4 * k], (uint64_t)hla[4 * k + 3] << 32 | hla[4 * k + 2]};
+    return 0;
+}
+
 namespace {
 // Advance the host transcript through one round the device transcript ran, checking that
 // both drew the same nonce and challenge.
@@ -812,6 +845,9 @@ inline flock_f128 from_u4(uint4 v) {
 // j-1, from the device pointer rin when that is set, else the value rho; launch(n, ...) is
 // the closing fold. Each message is observed on `fs`, ground (bits > 0) and answered with a
 // challenge, in the context's round mode (flock_hip_ctx_set_round_mode).
+// zerocheck tail rounds on tables of at most this many words run two rounds per launch
+constexpr uint64_t kZcLaMax = 1ull << 17;
+
 struct RoundLoop {
     using Launch = std::function<int(uint32_t, flock::RoundOut, const uint4*, uint4)>;
     flock_hip_ctx* c;
@@ -822,6 +858,12 @@ struct RoundLoop {
     std::vector<flock_f128> rhos;  // n
     std::vector<uint64_t> nonces;  // n, when bits > 0
     int mode = FLOCK_ROUNDS_HOST;
+    // Two rounds per launch from round la_from on, in the host round mode: la(j, nfold, r1, r2,
+    // out) folds the tables by the nfold (1 or 2) pending challenges and, for j < n, returns
+    // round j's pair and round j+1's as quadratics in its challenge (launch_zc_fold_la's eight
+    // sums); at j == n it is the closing fold. Round j+1's pair then needs no launch.
+    std::function<int(uint32_t, int, uint4, uint4, flock_f128*)> la;
+    uint32_t la_from = UINT32_MAX;
 
     void host_round(uint32_t k) {
         flock_fs_observe_f128(fs, msgs[2 * k]);
@@ -850,12 +892,40 @@ struct RoundLoop {
             return 0;
         }
         if (int rc = launch(0, c->polled_out(), nullptr, zero)) return rc;
+        flock_f128 s8[8];
+        bool from_la = false, look = false;
+        uint4 pend[2] = {zero, zero};
+        int np = 0;
         for (uint32_t k = 0; k < n; k++) {
-            if (int rc = c->await_pair(&msgs[2 * k])) return rc;
-            host_round(k);
-            if (int rc = launch(k + 1, k + 1 < n ? c->polled_out() : flock::RoundOut{}, nullptr,
-                                U(rhos[k])))
+            if (look) {  // a quadratic from its values at 0 and 1 and its leading coefficient
+                const F r{rhos[k - 1].lo, rhos[k - 1].hi};
+                auto quad = [&](const flock_f128* p) {
+                    const F p0{p[0].lo, p[0].hi}, p1{p[1].lo, p[1].hi}, pi{p[2].lo, p[2].hi};
+                    const F v = fadd(fadd(p0, fmul(fadd(fadd(p0, p1), pi), r)), fmul(pi, fmul(r, r)));
+                    return flock_f128{v.lo, v.hi};
+                };
+                msgs[2 * k] = quad(s8 + 2);
+                msgs[2 * k + 1] = quad(s8 + 5);
+                look = false;
+            } else if (from_la) {
+                msgs[2 * k] = s8[0];
+                msgs[2 * k + 1] = s8[1];
+                from_la = false;
+                look = k + 1 < n;
+            } else if (int rc = c->await_pair(&msgs[2 * k])) {
                 return rc;
+            }
+            host_round(k);
+            pend[np++] = U(rhos[k]);
+            if (look) continue;  // round k+1's pair is already known
+            const uint32_t j = k + 1;
+            if (la && j >= la_from) {
+                if (int rc = la(j, np, pend[0], pend[1], s8)) return rc;
+                from_la = j < n;
+            } else if (int rc = launch(j, j < n ? c->polled_out() : flock::RoundOut{}, nullptr, pend[0])) {
+                return rc;
+            }
+            np = 0;
         }
         return 0;
     }
@@ -926,10 +996,18 @@ int flock_hip_ctx_create(int device, flock_hip_ctx** out) {
         hipMalloc(&c->partials, sizeof(uint4) * (flock::kMaxReduceBlocks * 2 + 1)) != hipSuccess ||
         hipHostMalloc((void**)&c->hout, sizeof(flock_f128) * 3,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&c->dout, c->hout, 0) != hipSuccess) {
+        hipHostGetDevicePointer((void**)&c->dout, c->hout, 0) != hipSuccess ||
+        hipMalloc(&c->la_acc, 4 * 64) != hipSuccess ||
+        hipHostMalloc((void**)&c->hla, 4 * 40, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->dla, c->hla, 0) != hipSuccess) {
         hipError_t e = hipGetLastError();
         flock_hip_ctx_destroy(c);
         return fail(FLOCK_HIP_E_HIP, std::string("context setup: ") + hipGetErrorString(e));
+    }
+    std::memset(c->hla, 0, 4 * 40);
+    if (hipMemset(c->la_acc, 0, 4 * 64) != hipSuccess) {
+        flock_hip_ctx_destroy(c);
+        return fail(FLOCK_HIP_E_HIP, "context setup: lookahead accumulators");
     }
     // the polled flag word after the pair starts at zero, below every sequence number
     std::memset(c->hout, 0, sizeof(flock_f128) * 3);
@@ -965,6 +1043,8 @@ void flock_hip_ctx_destroy(flock_hip_ctx* c) {
     if (c->scratch) hipFree(c->scratch);
     if (c->partials) hipFree(c->partials);
     if (c->hout) hipHostFree(c->hout);
+    if (c->la_acc) hipFree(c->la_acc);
+    if (c->hla) hipHostFree(c->hla);
     if (c->aux) hipStreamSynchronize(c->aux);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -1524,6 +1604,38 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
                        }
                        return 0;
                    }};
+    // the unsplit rounds on tables of at most kZcLaMax words: two rounds per launch
+    // (launch_zc_fold_la), tracking the tables and round eq tables the launches hand on
+    const uint4* la_a = nullptr;
+    const uint4* la_b = nullptr;
+    uint4* la_e[2] = {nullptr, nullptr};  // [current round's eq, the free one]
+    loop.la_from = std::max<uint32_t>(std::max<uint32_t>(j0, 1u), n_mlv);
+    for (uint32_t j = std::max<uint32_t>(j0, 1u); j < n_mlv; j++)
+        if ((N >> (j - 1)) <= kZcLaMax) {
+            loop.la_from = j;
+            break;
+        }
+    loop.la = [&](uint32_t j, int nf, uint4 r1, uint4 r2, flock_f128* out) -> int {
+        if (!la_a) {  // the first: the tables and eq table the round-j launch would read
+            const uint32_t i = j - 1;
+            la_a = i == 0 ? Az : bufs[(i - 1) & 1][0];
+            la_b = i == 0 ? Bz : bufs[(i - 1) & 1][1];
+            la_e[0] = eqb[j & 1];
+            la_e[1] = eqb[(j & 1) ^ 1];
+        }
+        const uint64_t len = N >> (j - (uint32_t)nf);  // current table length
+        uint4* oa = la_a == bufs[0][0] ? bufs[1][0] : bufs[0][0];
+        uint4* ob = la_a == bufs[0][0] ? bufs[1][1] : bufs[0][1];
+        LAUNCH(flock::launch_zc_fold_la(la_a, la_b, len, nf, r1, r2, oa, ob, la_e[0], la_e[1],
+                                        c->la_out(), c->stream));
+        if (int rc = c->await_la(out)) return rc;
+        la_a = oa;
+        la_b = ob;
+        std::swap(la_e[0], la_e[1]);  // the launch wrote round j+2's eq into the free one
+        last_f = oa;
+        last_b = ob;
+        return 0;
+    };
     if (int rc = loop.run()) return rc;
     HIP_TRY(hipMemcpyAsync(c->hout, last_f, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(c->hout + 1, last_b, 16, hipMemcpyDeviceToHost, c->stream));

@@ -73,6 +73,23 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
                                 uint4* partials, RoundOut ro, hipStream_t s,
                                 const uint4* rin = nullptr);
+// Where a grid reduction of several words goes: the workgroups fold into acc (the words, then
+// a ticket counter; zero between launches), and the last one moves the sums to out (host-
+// visible), clears acc and stores seq into flag (system scope).
+struct LaOut {
+    uint32_t* acc;
+    uint32_t* out;
+    uint32_t* flag;
+    uint32_t seq;
+};
+// Two zerocheck tail rounds per launch: fold (a, b) of size n by nfold (1 or 2) challenges
+// into (ao, bo), then round j's pair with eq table e ((n >> nfold) / 2 entries) and round
+// j+1's pair as quadratics in its challenge (values at 0, 1 and leading coefficients); e2
+// gets round j+2's eq table ((n >> nfold) / 8 entries, written when that is >= 1). Eight F128
+// go to lo.out: [G(1), G(inf), next G(1) at 0, 1, inf, next G(inf) at 0, 1, inf].
+hipError_t launch_zc_fold_la(const uint4* a, const uint4* b, uint64_t n, int nfold, uint4 r1,
+                             uint4 r2, uint4* ao, uint4* bo, const uint4* e, uint4* e2, LaOut lo,
+                             hipStream_t s);
 // Split-eq message for rounds whose eq table has >= 16 variables: eq(x) = e_lo[x & 255] *
 // e_hi[x >> 8]; n (the table length) is a multiple of 512.
 hipError_t launch_zc_round_split(const uint4* a, const uint4* b, uint64_t n, const uint4* e_lo,

@@ -1169,6 +1169,106 @@ __global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const
     }
 }
 
+// Two rounds per pass of the (G(0), G(inf)) product sumcheck of <f, b> over LSB pairs (the
+// reference's lookahead, cuda-ghash/sumcheck_ab.cuh:125-239). The tables are folded by the
+// `nfold` (0..2) pending challenges r1, r2 (v' = v0 + r (v0 + v1)) into nf, nb of length
+// L = n >> nfold; then the kernel sums round r's pair on the folded table and, when L >= 4,
+// round r+1's pair as quadratics in its own challenge rho, by their values at rho = 0, 1 and
+// their leading coefficients. For a four-entry group g0..g3 (c0..c3 of b):
+//   G0 += g0 c0 + g2 c2, Ginf += (g0+g1)(c0+c1) + (g2+g3)(c2+c3);
+//   next G0:   at 0 g0 c0, at 1 g1 c1, leading (g0+g1)(c0+c1);
+//   next Ginf: at 0 (g0+g2)(c0+c2), at 1 (g1+g3)(c1+c3), leading (g0+..+g3)(c0+..+c3).
+// acc words (32): [g0c0, g2c2, (g0+g1)(c0+c1), (g2+g3)(c2+c3), g1c1, M0, M1, Minf].
+__device__ __forceinline__ f128 shfl_f128(f128 v, uint32_t src) {
+    return make_uint4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+}
+__device__ __forceinline__ f128 fold_by(const f128* T, f128 v0, f128 v1) {
+    return f128_add(v0, shoup_mul(T, f128_add(v0, v1)));
+}
+// One thread per folded entry (the groups' four lanes exchange their entries by shuffles and
+// share the eight products), so the small tables this runs on see short dependency chains.
+__global__ void __launch_bounds__(kThreads) k_sc_fold_la(const f128* f, const f128* b, uint64_t n,
+                                                         int nfold, f128 r1, f128 r2, f128* nf,
+                                                         f128* nb, uint32_t* acc) {
+    __shared__ f128 tab[2 * SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    if (threadIdx.x < 64)
+        shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES, threadIdx.x < 32 ? r1 : r2,
+                        threadIdx.x & 31);
+    __syncthreads();
+    const f128* T1 = tab;
+    const f128* T2 = tab + SHOUP_ENTRIES;
+    const uint64_t L = n >> nfold;
+    const int per = L >= 4 ? 4 : (int)L;  // entries per group
+    const int t = threadIdx.x & 3;        // entry in the group (groups never straddle a wave)
+    f128 s0 = f128_zero(), s1 = f128_zero();  // this lane's two product sums
+    // L is a power of two, so every lane of a group iterates alike
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~3u); base < 4 * (L / per);
+         base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = (base / 4) * per + t;  // this lane's folded entry
+        f128 g = f128_zero(), c = f128_zero();
+        if (t < per) {
+            if (nfold == 0) {
+                g = f[k];
+                c = b[k];
+            } else if (nfold == 1) {
+                g = fold_by(T1, f[2 * k], f[2 * k + 1]);
+                c = fold_by(T1, b[2 * k], b[2 * k + 1]);
+            } else {
+                g = fold_by(T2, fold_by(T1, f[4 * k], f[4 * k + 1]), fold_by(T1, f[4 * k + 2], f[4 * k + 3]));
+                c = fold_by(T2, fold_by(T1, b[4 * k], b[4 * k + 1]), fold_by(T1, b[4 * k + 2], b[4 * k + 3]));
+            }
+            if (nfold) {
+                nf[k] = g;
+                nb[k] = c;
+            }
+        }
+        f128 G[4], C[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            G[u] = shfl_f128(g, (threadIdx.x & ~3u) + u);
+            C[u] = shfl_f128(c, (threadIdx.x & ~3u) + u);
+        }
+        if (per == 4) {
+            // lane 0: g0c0, (g0+g1)(c0+c1); lane 1: g2c2, (g2+g3)(c2+c3);
+            // lane 2: g1c1, (g0+g2)(c0+c2); lane 3: (g1+g3)(c1+c3), (g0+..+g3)(c0+..+c3)
+            f128 x0, y0, x1, y1;
+            if (t == 0) {
+                x0 = G[0]; y0 = C[0];
+                x1 = f128_add(G[0], G[1]); y1 = f128_add(C[0], C[1]);
+            } else if (t == 1) {
+                x0 = G[2]; y0 = C[2];
+                x1 = f128_add(G[2], G[3]); y1 = f128_add(C[2], C[3]);
+            } else if (t == 2) {
+                x0 = G[1]; y0 = C[1];
+                x1 = f128_add(G[0], G[2]); y1 = f128_add(C[0], C[2]);
+            } else {
+                x0 = f128_add(G[1], G[3]); y0 = f128_add(C[1], C[3]);
+                x1 = f128_add(f128_add(G[0], G[1]), f128_add(G[2], G[3]));
+                y1 = f128_add(f128_add(C[0], C[1]), f128_add(C[2], C[3]));
+            }
+            s0 = f128_add(s0, comb_mul(wl, x0, y0));
+            s1 = f128_add(s1, comb_mul(wl, x1, y1));
+        } else if (per == 2 && t == 0) {
+            s0 = f128_add(s0, comb_mul(wl, G[0], C[0]));
+            s1 = f128_add(s1, comb_mul(wl, f128_add(G[0], G[1]), f128_add(C[0], C[1])));
+        }
+    }
+    // slots: lane 0 -> (0, 2), lane 1 -> (1, 3), lane 2 -> (4, 5), lane 3 -> (6, 7)
+    uint32_t v[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) v[k] = 0u;
+    const int a0 = t == 0 ? 0 : t == 1 ? 1 : t == 2 ? 4 : 6;
+    const int a1 = t == 0 ? 2 : t == 1 ? 3 : t == 2 ? 5 : 7;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (k == a0) put4(v + 4 * k, s0);
+        if (k == a1) put4(v + 4 * k, s1);
+    }
+    block_xor_atomic<32>(v, acc);
+}
+
 }  // namespace
 }  // namespace flock
 
@@ -1185,12 +1285,18 @@ struct DBuf {
 
 }  // namespace
 
+// One set of reduction accumulators: the device words the kernels XOR into and the host's
+// pinned mirror they are published to.
+struct AccSet {
+    uint32_t* acc = nullptr;   // device accumulators (256 words), zero between uses
+    uint32_t* hacc = nullptr;  // coherent pinned mirror: 256 words, then the flag word
+    uint32_t* dacc = nullptr;  // its device view
+    uint32_t seq = 0;          // last sequence number announced through the flag
+};
+
 struct flock_open_ws {
     std::vector<DBuf> bufs;
-    uint32_t* acc = nullptr;        // device accumulators (256 words), zero between uses
-    uint32_t* hacc = nullptr;       // coherent pinned mirror: 256 words, then the flag word
-    uint32_t* dacc = nullptr;       // its device view
-    uint32_t acc_seq = 0;           // last sequence number announced through the flag
+    AccSet accs[2];  // one per host thread of the opening (the inner opening; the assist)
     double stage_ms[6] = {};        // host wall time of the last opening's stages
     std::chrono::steady_clock::time_point mark;
     void stamp(int i) {
@@ -1242,7 +1348,8 @@ hipError_t comb_attrs() {
                              reinterpret_cast<const void*>(&flock::k_msg_e_comb),
                              reinterpret_cast<const void*>(&flock::k_intro_base),
                              reinterpret_cast<const void*>(&flock::k_msg_blocked_comb),
-                             reinterpret_cast<const void*>(&flock::k_dot_comb)};
+                             reinterpret_cast<const void*>(&flock::k_dot_comb),
+                             reinterpret_cast<const void*>(&flock::k_sc_fold_la)};
         for (const void* f : fns)
             if (hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    kCombWsBytes);
@@ -1257,6 +1364,7 @@ struct Ctx {
     flock_hip_ctx* c;
     flock_open_ws* ws;
     hipStream_t st;
+    AccSet* A;  // this thread's accumulators
     int err = 0;
     std::string what;
 
@@ -1282,11 +1390,11 @@ struct Ctx {
         if (!p) fail(FLOCK_HIP_E_NOMEM, "opening workspace");
         return static_cast<T*>(p);
     }
-    // Every accumulation kernel XORs into ws->acc, which read_acc leaves zero again; an entry
+    // Every accumulation kernel XORs into A->acc, which read_acc leaves zero again; an entry
     // point clears all of it once before its first accumulation (an earlier call may have
     // stopped between a kernel and its read).
     bool clear_acc() {
-        hipLaunchKernelGGL(flock::k_zero32, dim3(1), dim3(256), 0, st, ws->acc, 256u);
+        hipLaunchKernelGGL(flock::k_zero32, dim3(1), dim3(256), 0, st, A->acc, 256u);
         return hip(hipGetLastError(), "zero");
     }
     // Read back `words` accumulator words (and clear them): a one-wave kernel queued behind the
@@ -1294,12 +1402,12 @@ struct Ctx {
     // host polls instead of synchronising the stream. Bounded: past 10 s, or once the stream
     // reports an error, the wait ends with an error.
     bool read_acc(uint32_t words) {
-        if (++ws->acc_seq == 0) ++ws->acc_seq;
-        const uint32_t seq = ws->acc_seq;
-        hipLaunchKernelGGL(flock::k_acc_publish, dim3(1), dim3(64), 0, st, ws->acc, words, ws->dacc,
-                           ws->dacc + 256, seq);
+        if (++A->seq == 0) ++A->seq;
+        const uint32_t seq = A->seq;
+        hipLaunchKernelGGL(flock::k_acc_publish, dim3(1), dim3(64), 0, st, A->acc, words, A->dacc,
+                           A->dacc + 256, seq);
         if (!hip(hipGetLastError(), "acc publish")) return false;
-        const uint32_t* flag = ws->hacc + 256;
+        const uint32_t* flag = A->hacc + 256;
         const auto t0 = std::chrono::steady_clock::now();
         for (uint64_t it = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; it++) {
             if ((it & 4095) == 0) {
@@ -1315,7 +1423,7 @@ struct Ctx {
         return true;
     }
     H acc_f(int k) const {
-        const uint32_t* v = ws->hacc + 4 * k;
+        const uint32_t* v = A->hacc + 4 * k;
         return H{(uint64_t)v[1] << 32 | v[0], (uint64_t)v[3] << 32 | v[2]};
     }
     bool h2d(void* d, const void* h, size_t n) {
@@ -1555,20 +1663,25 @@ struct PairSumcheck {
     flock_f128* qb_;
 };
 
-// Runs n rounds of the (G(1), G(inf)) sumcheck of sum_x a(x) b(x) = claim over LSB pairs,
-// each round: observe the pair, grind `bits`, sample. The final fold is skipped when
-// `final_fold` is false; on return *fa/*fb hold the final folded values if folded.
-bool pair_sumcheck(Ctx& X, flock_f128* a, flock_f128* b, flock_f128* s1a, flock_f128* s1b,
-                   flock_f128* s2a, flock_f128* s2b, uint64_t N, uint32_t n_rounds, H claim,
-                   flock_fs* fs, uint32_t bits, bool final_fold, std::vector<H>& msgs,
-                   std::vector<uint64_t>& nonces, std::vector<H>& chal, H* fa) {
-    flock_f128 m2[2];
-    if (int rc = flock_hip_sumcheck_round_msg(X.c, a, b, N, m2)) return X.fail(rc, flock_hip_last_error());
-    const flock_f128* ca = a;
-    const flock_f128* cb = b;
-    uint64_t cur = N;
-    for (uint32_t r = 0; r < n_rounds; r++) {
-        const H g1 = ha(claim, m2[0]), gi = m2[1];
+// Runs the rounds of the (G(1), G(inf)) product sumcheck of <a, b> = claim over LSB pairs
+// from the round whose pair (G(0), G(inf)) is m2, on tables (ca, cb) of length cur, to the
+// end. Each round observes (claim + G(0), G(inf)), grinds `bits` (none when 0), samples its
+// challenge and updates the claim (jagged.rs:4592-4597). Tables longer than kLaMax entries
+// fold one round per launch (flock_hip_sumcheck_fold_msg); shorter ones two rounds per launch
+// (k_sc_fold_la: the second round's pair comes from the lookahead at the first's challenge),
+// which halves the host round trips of the small rounds. With final_fold the last challenges
+// are folded in too and *fin holds a's folded value. s1/s2 are ping-pong tables of cur / 2.
+constexpr uint64_t kLaMax = 1ull << 18;
+bool prod_rounds(Ctx& X, flock_fs* fs, const flock_f128* ca, const flock_f128* cb, uint64_t cur,
+                 const flock_f128 m2[2], H& claim, uint32_t bits, bool final_fold, flock_f128* s1a,
+                 flock_f128* s1b, flock_f128* s2a, flock_f128* s2b, std::vector<H>& msgs,
+                 std::vector<uint64_t>& nonces, std::vector<H>& chal, H* fin) {
+    H pend[2] = {hz(), hz()};
+    int np = 0, flip = 0;
+    bool have_la = false;
+    H la[6];
+    auto round = [&](H u0, H u2) {
+        const H g1 = ha(claim, u0), gi = u2;
         observe(fs, g1);
         observe(fs, gi);
         uint64_t nonce = 0;
@@ -1578,21 +1691,80 @@ bool pair_sumcheck(Ctx& X, flock_f128* a, flock_f128* b, flock_f128* s1a, flock_
         msgs.push_back(gi);
         chal.push_back(ch);
         claim = round_claim(claim, g1, gi, ch);
-        if (r + 1 == n_rounds && !final_fold) break;
-        flock_f128* na = (r & 1) ? s2a : s1a;
-        flock_f128* nb = (r & 1) ? s2b : s1b;
-        if (int rc = flock_hip_sumcheck_fold_msg(X.c, ca, cb, cur, ch, na, nb, m2))
-            return X.fail(rc, flock_hip_last_error());
+        pend[np++] = ch;
+    };
+    // a quadratic from its values at 0 and 1 and its leading coefficient, at r
+    auto quad = [](const H* p, H r) { return ha(ha(p[0], hm(ha(ha(p[0], p[1]), p[2]), r)), hm(p[2], hm(r, r))); };
+    auto next = [&]() -> flock_f128* {
+        flock_f128* na = flip ? s2a : s1a;
+        flip ^= 1;
+        return na;
+    };
+    auto launch_la = [&](flock_f128* na, flock_f128* nb) {
+        hipLaunchKernelGGL(flock::k_sc_fold_la, dim3(X.grid(std::max<uint64_t>(cur >> np, 4))),
+                           dim3(kThreads), kCombWsBytes, X.st, reinterpret_cast<const uint4*>(ca),
+                           reinterpret_cast<const uint4*>(cb), cur, np, U4(pend[0]), U4(pend[1]),
+                           reinterpret_cast<uint4*>(na), reinterpret_cast<uint4*>(nb), X.A->acc);
+        return X.chk("fold la") && X.read_acc(32);
+    };
+    H u0 = m2[0], u2 = m2[1];
+    while (true) {
+        round(u0, u2);
+        const uint64_t after = cur >> np;  // length once the pending challenges are folded in
+        if (after == 1) break;             // that was the last round
+        if (have_la) {
+            u0 = quad(la, pend[0]);
+            u2 = quad(la + 3, pend[0]);
+            have_la = false;
+            continue;
+        }
+        flock_f128* na = next();
+        flock_f128* nb = na == s1a ? s1b : s2b;
+        if (cur > kLaMax) {  // one pending challenge here
+            flock_f128 m[2];
+            if (int rc = flock_hip_sumcheck_fold_msg(X.c, ca, cb, cur, pend[0], na, nb, m))
+                return X.fail(rc, flock_hip_last_error());
+            u0 = m[0];
+            u2 = m[1];
+        } else {
+            if (!launch_la(na, nb)) return false;
+            u0 = ha(X.acc_f(0), X.acc_f(1));
+            u2 = ha(X.acc_f(2), X.acc_f(3));
+            if (after >= 4) {
+                la[0] = X.acc_f(0);
+                la[1] = X.acc_f(4);
+                la[2] = X.acc_f(2);
+                la[3] = X.acc_f(5);
+                la[4] = X.acc_f(6);
+                la[5] = X.acc_f(7);
+                have_la = true;
+            }
+        }
         ca = na;
         cb = nb;
-        cur /= 2;
+        cur = after;
+        np = 0;
     }
-    if (final_fold && fa) {
-        flock_f128 v;
-        if (!X.d2h(&v, ca, 16)) return false;
-        *fa = v;
+    if (final_fold) {
+        flock_f128* na = next();
+        flock_f128* nb = na == s1a ? s1b : s2b;
+        if (!launch_la(na, nb)) return false;
+        if (fin && !X.d2h(fin, na, 16)) return false;
     }
     return true;
+}
+
+// Runs n rounds of the (G(1), G(inf)) sumcheck of sum_x a(x) b(x) = claim over LSB pairs,
+// each round: observe the pair, grind `bits`, sample. The final fold is skipped when
+// `final_fold` is false; on return *fa holds a's final folded value if folded.
+bool pair_sumcheck(Ctx& X, flock_f128* a, flock_f128* b, flock_f128* s1a, flock_f128* s1b,
+                   flock_f128* s2a, flock_f128* s2b, uint64_t N, uint32_t n_rounds, H claim,
+                   flock_fs* fs, uint32_t bits, bool final_fold, std::vector<H>& msgs,
+                   std::vector<uint64_t>& nonces, std::vector<H>& chal, H* fa) {
+    if (n_rounds == 0 || (1ull << n_rounds) != N) return X.fail(FLOCK_HIP_E_INVALID, "pair sumcheck: n != 2^rounds");
+    flock_f128 m2[2];
+    if (int rc = flock_hip_sumcheck_round_msg(X.c, a, b, N, m2)) return X.fail(rc, flock_hip_last_error());
+    return prod_rounds(X, fs, a, b, N, m2, claim, bits, final_fold, s1a, s1b, s2a, s2b, msgs, nonces, chal, fa);
 }
 
 // ---- the eq-basis Ligerito opening (pcs.rs:214-560; extension.rs:1636-2290) ----
@@ -1760,7 +1932,7 @@ struct Ladder {
     bool msg(uint64_t d, E* u0, E* u2) {
         const uint64_t dd = (d == 1 || n == d) ? 1 : d;
         hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(n / 2)), dim3(kThreads), kCombWsBytes, X.st, f, b, n,
-                           dd, X.ws->acc);
+                           dd, X.A->acc);
         if (!X.chk("msg") || !X.read_acc(16)) return false;
         *u0 = E{X.acc_f(0), X.acc_f(1)};
         *u2 = E{X.acc_f(2), X.acc_f(3)};
@@ -1769,7 +1941,7 @@ struct Ladder {
     bool fold(uint64_t d, E r) {
         hipLaunchKernelGGL((flock::k_fold_msg<false, false>), dim3(X.grid(n / 2)), dim3(kThreads), 0, X.st,
                            static_cast<const void*>(f), static_cast<const void*>(b), n, d, E4(r),
-                           (uint64_t)1, f2, b2, X.ws->acc);
+                           (uint64_t)1, f2, b2, X.A->acc);
         if (!X.chk("fold")) return false;
         swap();
         n /= 2;
@@ -1788,12 +1960,12 @@ struct Ladder {
         e256d* dst_b = BASE ? b : b2;
         if (nn >= kSplitFoldMsg) {
             hipLaunchKernelGGL((flock::k_fold_msg<BASE, false>), dim3(X.grid(nn)), dim3(kThreads), 0, X.st, src_f,
-                               src_b, n, d, E4(r), d2, dst_f, dst_b, X.ws->acc);
+                               src_b, n, d, E4(r), d2, dst_f, dst_b, X.A->acc);
             hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes, X.st,
-                               dst_f, dst_b, nn, d2, X.ws->acc);
+                               dst_f, dst_b, nn, d2, X.A->acc);
         } else {
             hipLaunchKernelGGL((flock::k_fold_msg<BASE, true>), dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes,
-                               X.st, src_f, src_b, n, d, E4(r), d2, dst_f, dst_b, X.ws->acc);
+                               X.st, src_f, src_b, n, d, E4(r), d2, dst_f, dst_b, X.A->acc);
         }
         if (!X.chk("fold msg") || !X.read_acc(16)) return false;
         if (!BASE) swap();
@@ -1877,7 +2049,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         if (!X.eq_dev(z.data(), (size_t)log_n, ez_)) return false;
         hipLaunchKernelGGL(flock::k_dot_comb, dim3(X.grid(N)), dim3(kThreads), kCombWsBytes, X.st,
                            reinterpret_cast<const uint4*>(q),
-                           reinterpret_cast<const uint4*>(ez_), N, X.ws->acc);
+                           reinterpret_cast<const uint4*>(ez_), N, X.A->acc);
         if (!X.chk("ood dot") || !X.read_acc(4)) return false;
         const H y = X.acc_f(0);
         observe(fs, y);
@@ -1893,7 +2065,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     T.mark("ood0");
     // first message over blocked pairs (F128)
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
-                       reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D, X.ws->acc);
+                       reinterpret_cast<const uint4*>(q), reinterpret_cast<const uint4*>(b0), N, D, X.A->acc);
     if (!X.chk("first msg") || !X.read_acc(8)) return false;
     Ladder Lg{X, fs, X.buf<e256d>(12, N / 2), X.buf<e256d>(13, N / 2), X.buf<e256d>(14, N / 2),
               X.buf<e256d>(15, N / 2)};
@@ -1945,7 +2117,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
             const std::vector<H> z = sample_vec(fs, (size_t)split_dim);
             if (!X.eq_dev(z.data(), (size_t)split_dim, ezl)) return false;
             hipLaunchKernelGGL(flock::k_intro_base, dim3(X.grid(Lg.n / 2)), dim3(kThreads), kCombWsBytes, X.st, Lg.f,
-                               reinterpret_cast<const uint4*>(ezl), Lg.n, X.ws->acc);
+                               reinterpret_cast<const uint4*>(ezl), Lg.n, X.A->acc);
             if (!X.chk("intro ood") || !X.read_acc(20)) return false;
             const H y = X.acc_f(0);
             const E u0{X.acc_f(1), X.acc_f(2)}, u2{X.acc_f(3), X.acc_f(4)};
@@ -1967,7 +2139,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     };
     auto presplit = [&](int dims, int next_level) -> bool {
         hipLaunchKernelGGL(flock::k_intro_presplit, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st, Lg.f,
-                           reinterpret_cast<const uint4*>(basis), 1ull << dims, X.ws->acc);
+                           reinterpret_cast<const uint4*>(basis), 1ull << dims, X.A->acc);
         if (!X.chk("intro presplit") || !X.read_acc(16)) return false;
         const E u0{X.acc_f(0), X.acc_f(1)}, u2{X.acc_f(2), X.acc_f(3)};
         Lg.push(u0, u2);
@@ -2101,17 +2273,21 @@ int flock_open_ws_create(flock_open_ws** out) {
     if (!out) return flock::set_error(FLOCK_HIP_E_INVALID, "null output");
     flock_open_ws* w = new (std::nothrow) flock_open_ws();
     if (!w) return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace");
-    if (hipMalloc(&w->acc, 4 * 256) != hipSuccess ||
-        hipHostMalloc((void**)&w->hacc, 4 * 260, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&w->dacc, w->hacc, 0) != hipSuccess) {
-        (void)hipGetLastError();
-        if (w->acc) (void)hipFree(w->acc);
-        if (w->hacc) (void)hipHostFree(w->hacc);
-        delete w;
-        return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace accumulators");
+    for (AccSet& a : w->accs) {
+        if (hipMalloc(&a.acc, 4 * 256) != hipSuccess ||
+            hipHostMalloc((void**)&a.hacc, 4 * 260, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer((void**)&a.dacc, a.hacc, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            for (AccSet& b : w->accs) {
+                if (b.acc) (void)hipFree(b.acc);
+                if (b.hacc) (void)hipHostFree(b.hacc);
+            }
+            delete w;
+            return flock::set_error(FLOCK_HIP_E_NOMEM, "workspace accumulators");
+        }
+        // the flag word starts below every sequence number
+        std::memset(a.hacc, 0, 4 * 260);
     }
-    // the flag word starts below every sequence number
-    std::memset(w->hacc, 0, 4 * 260);
     w->bufs.resize(flock_open_ws::kSlots);
     *out = w;
     return 0;
@@ -2122,8 +2298,10 @@ void flock_open_ws_destroy(flock_open_ws* w) {
     if (w->aux) flock_hip_ctx_destroy(w->aux);
     for (DBuf& b : w->bufs)
         if (b.p) (void)hipFree(b.p);
-    if (w->acc) (void)hipFree(w->acc);
-    if (w->hacc) (void)hipHostFree(w->hacc);
+    for (AccSet& a : w->accs) {
+        if (a.acc) (void)hipFree(a.acc);
+        if (a.hacc) (void)hipHostFree(a.hacc);
+    }
     delete w;
 }
 
@@ -2150,7 +2328,7 @@ int flock_hip_s_hat_v_from_z_vec(flock_hip_ctx* ctx, flock_open_ws* ws, const fl
                                  flock_f128 h_out[128]) {
     if (!ctx || !ws || !d_z_vec || !h_out || z_vec_len != (128ull << tail_len))
         return flock::set_error(FLOCK_HIP_E_INVALID, "s_hat_v_from_z_vec: bad arguments");
-    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
+    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx)), &ws->accs[0]};
     flock_f128* eq = X.buf<flock_f128>(46, 1ull << tail_len);
     flock_f128* out = X.buf<flock_f128>(47, 128);
     if (X.err) return flock::set_error(X.err, X.what.c_str());
@@ -2175,7 +2353,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         return flock::set_error(FLOCK_HIP_E_INVALID, "open_merged: bad arguments");
     *out_bytes = nullptr;
     *out_len = 0;
-    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx))};
+    Ctx X{ctx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(ctx)), &ws->accs[0]};
     if (!X.hip(comb_attrs(), "kernel attributes") || !X.clear_acc())
         return flock::set_error(X.err, X.what.c_str());
     const int dense_log = (int)P->m - 7;
@@ -2280,7 +2458,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     if (!jag_table(X, true, W)) return flock::set_error(X.err, X.what.c_str());
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,
                        reinterpret_cast<const uint4*>(d_q), reinterpret_cast<const uint4*>(W), N, (uint64_t)1,
-                       X.ws->acc);
+                       X.A->acc);
     if (!X.chk("jagged weight") || !X.read_acc(8)) return flock::set_error(X.err, X.what.c_str());
     ws->stamp(0);
     // the merged sumcheck <q, W> = target (pcs.rs:1640-1700), merged_round_bits = 2
@@ -2288,32 +2466,13 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     std::vector<uint64_t> mnon;
     H q_eval = hz();
     {
-        // round 0's message is the fused prime; later rounds come from the fold kernels
-        flock_f128 m2[2] = {X.acc_f(0), X.acc_f(1)};
-        const flock_f128* ca = d_q;  // the first fold reads q in place
-        const flock_f128* cb = W;
-        uint64_t cur = N;
+        // round 0's message is the fused prime; later rounds come from the fold kernels, the
+        // first of which reads q in place
+        const flock_f128 m2[2] = {X.acc_f(0), X.acc_f(1)};
         H claim = target;
-        for (int r = 0; r < dense_log; r++) {
-            const H g1 = ha(claim, m2[0]), gi = m2[1];
-            observe(fs, g1);
-            observe(fs, gi);
-            uint64_t nonce;
-            const H ch = grind_sample(fs, 2, &nonce);
-            mnon.push_back(nonce);
-            mrounds.push_back(g1);
-            mrounds.push_back(gi);
-            rho.push_back(ch);
-            claim = round_claim(claim, g1, gi, ch);
-            flock_f128* na = (r & 1) ? s2a : s1a;
-            flock_f128* nb = (r & 1) ? s2b : s1b;
-            if (int rc = flock_hip_sumcheck_fold_msg(ctx, ca, cb, cur, ch, na, nb, m2))
-                return rc;
-            ca = na;
-            cb = nb;
-            cur /= 2;
-        }
-        if (!X.d2h(&q_eval, ca, 16)) return flock::set_error(X.err, X.what.c_str());
+        if (!prod_rounds(X, fs, d_q, W, N, m2, claim, 2, true, s1a, s1b, s2a, s2b, mrounds, mnon, rho,
+                         &q_eval))
+            return flock::set_error(X.err, X.what.c_str());
     }
     if (h_q_eval) *h_q_eval = q_eval;
     ws->stamp(1);
@@ -2464,15 +2623,16 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         flock_fs_free(ch);
         return flock::set_error(FLOCK_HIP_E_HIP, "open_merged: second context");
     }
-    Ctx X2{actx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(actx))};
+    Ctx X2{actx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(actx)), &ws->accs[1]};
     bool ok_mp = false;
     std::thread th;
     try {
         th = std::thread([&] {
-            if (X2.hip(hipSetDevice(flock::ctx_device(actx)), "assist device")) ok_mp = multipoint(X2);
+            if (X2.hip(hipSetDevice(flock::ctx_device(actx)), "assist device"))
+                ok_mp = X2.clear_acc() && multipoint(X2);
         });
     } catch (...) {
-        ok_mp = multipoint(X2);  // no thread: one after the other
+        ok_mp = X2.clear_acc() && multipoint(X2);  // no thread: one after the other
     }
     Bin inner;
     ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);

@@ -385,6 +385,174 @@ __global__ __launch_bounds__(256) void k_eq_outer(const f128* __restrict__ lo, i
         out[(xh << nlo) + xl] = shoup_mul(table, lo[xl]);
 }
 
+// XOR-reduce W words over the grid and hand them to the host: each workgroup folds its words
+// into lo.acc with atomicXor and draws a ticket from lo.acc[W]; the workgroup that draws the
+// last one moves the sums to lo.out (coherent pinned memory), leaves lo.acc and the ticket at
+// zero, and stores lo.seq into lo.flag at system scope, which the host polls.
+template <int W>
+__device__ __forceinline__ void grid_publish(uint32_t (&v)[W], const LaOut& lo) {
+    __shared__ uint32_t sm[4][W];
+    __shared__ int last;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < W; k++) v[k] ^= __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < W; k++) sm[wid][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < W) {
+        uint32_t x = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) x ^= sm[w][threadIdx.x];
+        if (x) atomicXor(lo.acc + threadIdx.x, x);
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(lo.acc + W, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x < W) {
+        lo.out[threadIdx.x] = atomicExch(lo.acc + threadIdx.x, 0u);
+        __threadfence_system();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicExch(lo.acc + W, 0u);
+        __hip_atomic_store(lo.flag, lo.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Two rounds per pass of the eq-weighted zerocheck tail (the reference's lookahead,
+// cuda-ghash/zerocheck_tail.cuh:429-552; multilinear.rs:331-355 per round). The tables a, b
+// (length n) are folded by the nfold (1 or 2) pending challenges r1, r2 into ao, bo of length
+// L = n >> nfold. With L >= 2 the kernel sums round j's pair over the folded tables,
+//   G(1) = sum_x e[x] a1 b1,  G(inf) = sum_x e[x] (a0+a1)(b0+b1)   (e: L/2 entries),
+// and with L >= 4 round j+1's pair as quadratics in its challenge rho, by their values at
+// rho = 0, 1 and leading coefficients, weighted by e'[y] = e[2y] + e[2y+1] (round j+1's eq).
+// For a four-entry group g0..g3 (c0..c3 of b), e0 = e[2y], e1 = e[2y+1]:
+//   G(1) += e0 g1c1 + e1 g3c3,  G(inf) += e0 (g0+g1)(c0+c1) + e1 (g2+g3)(c2+c3);
+//   next G(1):   e' g2c2, e' g3c3, e' (g2+g3)(c2+c3)     (at 0, at 1, leading);
+//   next G(inf): e' (g0+g2)(c0+c2), e' (g1+g3)(c1+c3), e' (g0+..+g3)(c0+..+c3).
+// One thread per folded entry; the four lanes of a group exchange entries by shuffles and
+// share the products. With L >= 8 it also writes e2 (L/8 entries), round j+2's eq table:
+// e2[z] = e[4z] + e[4z+1] + e[4z+2] + e[4z+3]. Output words: [G(1), G(inf), next G(1) at
+// 0, 1, leading, next G(inf) at 0, 1, leading].
+__device__ __forceinline__ f128 shfl_f128(f128 v, uint32_t src) {
+    return make_uint4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64),
+                      __shfl(v.w, src, 64));
+}
+__device__ __forceinline__ f128 fold_by(const f128* T, f128 v0, f128 v1) {
+    return f128_add(v0, shoup_mul(T, f128_add(v0, v1)));
+}
+__global__ __launch_bounds__(256) void k_zc_fold_la(const f128* __restrict__ a,
+                                                     const f128* __restrict__ b, uint64_t n,
+                                                     int nfold, f128 r1, f128 r2,
+                                                     f128* __restrict__ ao, f128* __restrict__ bo,
+                                                     const f128* __restrict__ e,
+                                                     f128* __restrict__ e2, LaOut lo) {
+    __shared__ f128 tab[2 * SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    if (threadIdx.x < 64)
+        shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES, threadIdx.x < 32 ? r1 : r2,
+                        threadIdx.x & 31);
+    __syncthreads();
+    const f128* T1 = tab;
+    const f128* T2 = tab + SHOUP_ENTRIES;
+    const uint64_t L = n >> nfold;
+    const int per = L >= 4 ? 4 : (int)L;
+    const int t = threadIdx.x & 3;
+    f256u acc[4];  // lane 0: G1, Ginf; lane 1: G1, Ginf, P1, Pinf; lane 2: P0, Q0; lane 3: Q1, Qinf
+#pragma unroll
+    for (int k = 0; k < 4; k++) f256u_zero(acc[k]);
+    // L is a power of two, so the four lanes of a group always iterate together
+    for (uint64_t base = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~3u); base < 4 * (L / per);
+         base += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t grp = base / 4;
+        const uint64_t k = grp * per + t;
+        f128 g = f128_zero(), c = f128_zero();
+        if (t < per) {
+            if (nfold == 1) {
+                g = fold_by(T1, a[2 * k], a[2 * k + 1]);
+                c = fold_by(T1, b[2 * k], b[2 * k + 1]);
+            } else {
+                g = fold_by(T2, fold_by(T1, a[4 * k], a[4 * k + 1]), fold_by(T1, a[4 * k + 2], a[4 * k + 3]));
+                c = fold_by(T2, fold_by(T1, b[4 * k], b[4 * k + 1]), fold_by(T1, b[4 * k + 2], b[4 * k + 3]));
+            }
+            ao[k] = g;
+            bo[k] = c;
+        }
+        if (L >= 8 && t == 0 && (grp & 1) == 0)  // round j+2's eq: 4 entries of e per entry
+            e2[grp >> 1] = f128_add(f128_add(e[2 * grp], e[2 * grp + 1]), f128_add(e[2 * grp + 2], e[2 * grp + 3]));
+        if (per < 2) continue;
+        f128 G[4], C[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            G[u] = shfl_f128(g, (threadIdx.x & ~3u) + u);
+            C[u] = shfl_f128(c, (threadIdx.x & ~3u) + u);
+        }
+        if (per == 2) {
+            if (t == 0) {  // one pair: G(1) = e0 g1c1, G(inf) = e0 (g0+g1)(c0+c1)
+                const f128 x0 = comb_mul(wl, G[1], C[1]);
+                const f128 x1 = comb_mul(wl, f128_add(G[0], G[1]), f128_add(C[0], C[1]));
+                const uint32_t h = comb_build(wl, e[0]);
+                comb_apply(acc[0], wl, h, x0);
+                comb_apply(acc[1], wl, h, x1);
+            }
+            continue;
+        }
+        const f128 e0 = e[2 * grp], e1 = e[2 * grp + 1];
+        f128 x0, y0, x1, y1;
+        if (t == 0) {
+            x0 = G[1]; y0 = C[1];
+            x1 = f128_add(G[0], G[1]); y1 = f128_add(C[0], C[1]);
+        } else if (t == 1) {
+            x0 = G[3]; y0 = C[3];
+            x1 = f128_add(G[2], G[3]); y1 = f128_add(C[2], C[3]);
+        } else if (t == 2) {
+            x0 = G[2]; y0 = C[2];
+            x1 = f128_add(G[0], G[2]); y1 = f128_add(C[0], C[2]);
+        } else {
+            x0 = f128_add(G[1], G[3]); y0 = f128_add(C[1], C[3]);
+            x1 = f128_add(f128_add(G[0], G[1]), f128_add(G[2], G[3]));
+            y1 = f128_add(f128_add(C[0], C[1]), f128_add(C[2], C[3]));
+        }
+        const f128 p0 = comb_mul(wl, x0, y0), p1 = comb_mul(wl, x1, y1);
+        // weights: lane 0 by e0; lane 1 by e1 (this round) and e' (the next); lanes 2, 3 by e'
+        const f128 w0 = t == 0 ? e0 : t == 1 ? e1 : f128_add(e0, e1);
+        uint32_t h = comb_build(wl, w0);
+        comb_apply(acc[0], wl, h, p0);
+        comb_apply(acc[1], wl, h, p1);
+        if (t == 1) {
+            h = comb_build(wl, f128_add(e0, e1));
+            comb_apply(acc[2], wl, h, p0);
+            comb_apply(acc[3], wl, h, p1);
+        }
+    }
+    // output slots of each lane's sums
+    uint32_t v[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) v[k] = 0u;
+    const int sl[4][4] = {{0, 1, -1, -1}, {0, 1, 3, 4}, {2, 5, -1, -1}, {6, 7, -1, -1}};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const f128 r = f256u_reduce(acc[q]);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int want = t == 0 ? sl[0][q] : t == 1 ? sl[1][q] : t == 2 ? sl[2][q] : sl[3][q];
+            if (k == want) {
+                v[4 * k] ^= r.x;
+                v[4 * k + 1] ^= r.y;
+                v[4 * k + 2] ^= r.z;
+                v[4 * k + 3] ^= r.w;
+            }
+        }
+    }
+    grid_publish<32>(v, lo);
+}
+
 static hipError_t set_ws_attr(const void* fn) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kCombWsBytes);
 }
@@ -395,6 +563,7 @@ static hipError_t init_attrs() {
         if (e == hipSuccess) e = set_ws_attr((const void*)k_sc_fold_msg);
         if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_round);
         if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_fold_round);
+        if (e == hipSuccess) e = set_ws_attr((const void*)k_zc_fold_la);
         return e;
     }();
     return st;
@@ -449,6 +618,17 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
     const unsigned g = red_grid(n / 4);
     k_zc_fold_round<<<g, 256, kCombWsBytes, s>>>(a, b, n, rho, rin, ao, bo, e, e_next, partials,
                                                  ro);
+    return hipGetLastError();
+}
+
+hipError_t launch_zc_fold_la(const uint4* a, const uint4* b, uint64_t n, int nfold, uint4 r1,
+                             uint4 r2, uint4* ao, uint4* bo, const uint4* e, uint4* e2, LaOut lo,
+                             hipStream_t s) {
+    if (nfold < 1 || nfold > 2 || (n >> nfold) == 0) return hipErrorInvalidValue;
+    if (hipError_t ae = init_attrs(); ae != hipSuccess) return ae;
+    const uint64_t L = n >> nfold;
+    const unsigned g = red_grid(L < 4 ? 4 : L);
+    k_zc_fold_la<<<g, 256, kCombWsBytes, s>>>(a, b, n, nfold, r1, r2, ao, bo, e, e2, lo);
     return hipGetLastError();
 }
 
