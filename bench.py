@@ -188,6 +188,39 @@ def cpu_baseline(n: int) -> dict:
                       f"{t_all:.2f} s on {cores} threads, {t_one:.2f} s on 1"}
 
 
+def fixed_multiplier(F, ctx, ev, stream, n: int = 1 << 22, reps: int = 64) -> dict:
+    """The NTT's fixed-multiplier product (a * t, t fixed) on its two candidate units: the LDS
+    Shoup table (VALU + LDS) and the int8 matrix cores (v_mfma_i32_32x32x32_i8 over 0/1 bytes,
+    parity from bit 0). Each element takes `reps` dependent products in registers, so the
+    timed kernels are compute-bound; products/s from HIP events on the library's stream."""
+    rng = np.random.default_rng(0xF1CED)
+    a = rng.integers(0, 2**63, size=(n, 2), dtype=np.uint64)
+    da, do = ctx.upload(a), ctx.alloc(16 * n)
+    t = 0x0DCB364640A222FE6B8330483C2E9849
+    out = {"n": n, "reps": reps}
+    results = {}
+    for name, method in (("shoup", F.MUL_SHOUP), ("mfma_i8", F.MUL_MFMA_I8)):
+        ctx.f128_mul_const_pow_batch(da, t, reps, method, do, n)  # warm-up
+        e0, e1 = ev.new(), ev.new()
+        ts = []
+        for _ in range(3):
+            ev.record(e0, stream)
+            ctx.f128_mul_const_pow_batch(da, t, reps, method, do, n)
+            ev.record(e1, stream)
+            ts.append(ev.ms(e0, e1))
+        ms = min(ts)
+        results[name] = do.download_f128(n)
+        out[f"{name}_ms"] = round(ms, 4)
+        out[f"{name}_products_per_s"] = round(n * reps / (ms * 1e-3), 1)
+    assert np.array_equal(results["shoup"], results["mfma_i8"]), "the two product forms differ"
+    # 128 x 128 int8 MACs per product at the dense int8 peak (5.0 POP/s = 2.5e15 MAC/s)
+    out["mfma_i8_frac_of_int8_peak"] = round(
+        out["mfma_i8_products_per_s"] * 16384 * 2 / 5.0e15, 4)
+    da.free()
+    do.free()
+    return out
+
+
 def config1_commit(F, ctx, ev, stream, with_cpu: bool) -> dict:
     """The reference's CPU-runnable case: a 2^10 x 2^8 message of SplitMix64 words (seed
     0xC0FFEE, interleaved msg[p * 256 + lane]), rate 1/4, committed with the SHA-256 Merkle
@@ -430,6 +463,7 @@ def main():
             "bytes_per_launch": ck["pass_bytes"],
         }
         result["config1_commit"] = config1_commit(F, ctx, ev, stream, not args.no_cpu_baseline)
+        result["fixed_multiplier"] = fixed_multiplier(F, ctx, ev, stream)
 
     if args.copy_calibration:
         nbytes = 1 << 30

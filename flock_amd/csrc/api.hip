@@ -1111,6 +1111,16 @@ int flock_hip_f128_mul_const_batch(flock_hip_ctx* c, const flock_f128* a, flock_
     return 0;
 }
 
+int flock_hip_f128_mul_const_pow_batch(flock_hip_ctx* c, const flock_f128* a, flock_f128 t,
+                                       uint32_t reps, int method, flock_f128* out, size_t n) {
+    CTX_CHECK(c);
+    if (n && (!a || !out)) return fail(FLOCK_HIP_E_INVALID, "null buffer");
+    if (method != FLOCK_MUL_SHOUP && method != FLOCK_MUL_MFMA_I8)
+        return fail(FLOCK_HIP_E_INVALID, "unknown multiply method");
+    LAUNCH(flock::launch_f128_mul_const_pow(U(a), U(t), reps, method, U(out), n, c->stream));
+    return 0;
+}
+
 // ---- NTT ----
 int flock_hip_ntt_forward_interleaved(flock_hip_ctx* c, flock_f128* data, size_t num_ntts,
                                       size_t live, uint32_t log_d, uint32_t start_layer) {

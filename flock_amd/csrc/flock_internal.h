@@ -19,6 +19,9 @@ constexpr int kMaxReduceBlocks = 512;  // two 256-thread groups per CU: the whol
 // ---- field.hip ----
 hipError_t launch_f128_mul_batch(const uint4* a, const uint4* b, uint4* out, size_t n,
                                  hipStream_t s);
+// out[i] = a[i] * t^reps: method 0 the LDS Shoup table, 1 the int8 matrix cores.
+hipError_t launch_f128_mul_const_pow(const uint4* a, uint4 t, uint32_t reps, int method,
+                                     uint4* out, size_t n, hipStream_t s);
 hipError_t launch_f128_mul_const_batch(const uint4* a, uint4 t, uint4* out, size_t n,
                                        hipStream_t s);
 

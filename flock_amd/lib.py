@@ -103,6 +103,7 @@ class PcsParams(ctypes.Structure):
 HASH_SHA256, HASH_BLAKE3, HASH_KECCAK = 0, 1, 2
 MERKLE_SHA256, MERKLE_BLAKE3 = 0, 1
 ROUNDS_HOST, ROUNDS_DEVICE = 0, 2
+MUL_SHOUP, MUL_MFMA_I8 = 0, 1
 
 
 class SparseMatrix(ctypes.Structure):
@@ -224,6 +225,7 @@ def load():
         "flock_hip_free": (None, [P]),
         "flock_hip_f128_mul_batch": (I, [P, P, P, P, S]),
         "flock_hip_f128_mul_const_batch": (I, [P, P, F128, P, S]),
+        "flock_hip_f128_mul_const_pow_batch": (I, [P, P, F128, U32, I, P, S]),
         "flock_hip_ntt_forward_interleaved": (I, [P, P, S, S, U32, U32]),
         "flock_hip_ntt_pass": (I, [P, P, S, S, U32, U32, U32]),
         "flock_hip_pcs_sizes": (I, [PP, P, P, P, P, P]),
@@ -459,6 +461,13 @@ class Context:
 
     def f128_mul_const_batch(self, a: DeviceBuffer, t: int, out: DeviceBuffer, n: int):
         _check(load().flock_hip_f128_mul_const_batch(self.h, a.ptr, f128(t), out.ptr, n))
+
+    def f128_mul_const_pow_batch(self, a: DeviceBuffer, t: int, reps: int, method: int,
+                                 out: DeviceBuffer, n: int):
+        """flock_hip_f128_mul_const_pow_batch: out[i] = a[i] * t^reps (method MUL_SHOUP or
+        MUL_MFMA_I8)."""
+        _check(load().flock_hip_f128_mul_const_pow_batch(self.h, a.ptr, f128(t), reps, method,
+                                                          out.ptr, n))
 
     # ---- NTT / commit ----
     def ntt_forward_interleaved(self, data: DeviceBuffer, num_ntts: int, live: int, log_d: int,

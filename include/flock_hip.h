@@ -88,6 +88,15 @@ int flock_hip_f128_mul_batch(flock_hip_ctx* ctx, const flock_f128* d_a, const fl
 int flock_hip_f128_mul_const_batch(flock_hip_ctx* ctx, const flock_f128* d_a, flock_f128 t,
                                    flock_f128* d_out, size_t n);
 
+/* d_out[i] = d_a[i] * t^reps by `reps` fixed-multiplier products per element, on the LDS Shoup
+ * table (FLOCK_MUL_SHOUP, the NTT's and folds' product) or on the int8 matrix cores
+ * (FLOCK_MUL_MFMA_I8: the 128 x 128 GF(2) map of t as v_mfma_i32_32x32x32_i8 over 0/1 bytes,
+ * parity from bit 0). No counterpart in the reference: the measurement surface on which the
+ * two product forms are timed and profiled against each other. */
+enum { FLOCK_MUL_SHOUP = 0, FLOCK_MUL_MFMA_I8 = 1 };
+int flock_hip_f128_mul_const_pow_batch(flock_hip_ctx* ctx, const flock_f128* d_a, flock_f128 t,
+                                       uint32_t reps, int method, flock_f128* d_out, size_t n);
+
 /* ---- additive NTT ------------------------------------------------------------ */
 
 /* In-place forward interleaved NTT of AdditiveNttF128::standard(log_d) over layers

@@ -51,6 +51,24 @@ def test_f128_mul_batch(ctx):
     assert np.array_equal(do.download_f128(n), O.mul_batch(a, b))
 
 
+@pytest.mark.parametrize("method", [F.MUL_SHOUP, F.MUL_MFMA_I8])
+def test_f128_mul_const_pow_batch(ctx, method):
+    """out = a * t^reps on the Shoup table and on the int8 matrix cores, against the oracle's
+    multiply applied reps times (ragged n: the MFMA form's last 32-element tile is partial)."""
+    say(f"case: f128_mul_const_pow_batch method={method}")
+    n = (1 << 12) + 37
+    a = O.splitmix(5, n)
+    da, do = ctx.upload(a), ctx.alloc(16 * n)
+    for t in (0, 1, 2, 1 << 127, (1 << 128) - 1, 0x0DCB364640A222FE6B8330483C2E9849):
+        tt = O.ints_to_arr([t] * n)
+        for reps in (0, 1, 3):
+            exp = a.copy()
+            for _ in range(reps):
+                exp = O.mul_batch(exp, tt)
+            ctx.f128_mul_const_pow_batch(da, t, reps, method, do, n)
+            assert np.array_equal(do.download_f128(n), exp), (hex(t), reps)
+
+
 def test_f128_mul_const_batch(ctx):
     say("case: f128_mul_const_batch")
     n = 1 << 14
