@@ -24,6 +24,7 @@
 
 #if defined(__x86_64__)
 #include <cpuid.h>
+#include <immintrin.h>
 #include <wmmintrin.h>
 #include <emmintrin.h>
 #endif
@@ -88,6 +89,68 @@ __attribute__((target("pclmul,sse2"))) static or_f128 mul_pclmul(or_f128 a, or_f
 }
 #endif
 
+#if defined(__x86_64__)
+/* Four products by one fixed multiplier t in a 512-bit register (the 4-lane VPCLMULQDQ form of
+ * the reference's x86 multiply, gf2_128/x86_64.rs:244): per 128-bit lane the four 64x64
+ * carry-less products of the schoolbook, then the same GHASH reduction as ghash_reduce done
+ * by folding the top word r3 (r3 x^192 = (r3 * 0x87) x^64) and then r2 (r2 x^128 = r2 * 0x87)
+ * with carry-less products by 0x87 = x^7 + x^2 + x + 1. */
+__attribute__((target("avx512f,avx512bw,vpclmulqdq"))) static __m512i mul4_v(__m512i A, __m512i B) {
+    const __m512i K = _mm512_set1_epi64(0x87);
+    const __m512i ll = _mm512_clmulepi64_epi128(A, B, 0x00);
+    const __m512i hh = _mm512_clmulepi64_epi128(A, B, 0x11);
+    const __m512i mid = _mm512_xor_si512(_mm512_clmulepi64_epi128(A, B, 0x01),
+                                         _mm512_clmulepi64_epi128(A, B, 0x10));
+    __m512i lo = _mm512_xor_si512(ll, _mm512_bslli_epi128(mid, 8)); /* r0, r1 */
+    __m512i hi = _mm512_xor_si512(hh, _mm512_bsrli_epi128(mid, 8)); /* r2, r3 */
+    const __m512i t = _mm512_clmulepi64_epi128(hi, K, 0x01);        /* r3 * 0x87 */
+    lo = _mm512_xor_si512(lo, _mm512_bslli_epi128(t, 8));
+    hi = _mm512_xor_si512(hi, _mm512_bsrli_epi128(t, 8));            /* r2 += carry of t */
+    return _mm512_xor_si512(lo, _mm512_clmulepi64_epi128(hi, K, 0x00)); /* + r2 * 0x87 */
+}
+
+/* one NTT butterfly row over 4 consecutive lanes: u = top + v tw, bot = v + u */
+__attribute__((target("avx512f,avx512bw,vpclmulqdq"))) static void butterfly4(or_f128* top, or_f128* bot,
+                                                                           or_f128 tw) {
+    const __m512i T = _mm512_set_epi64((long long)tw.hi, (long long)tw.lo, (long long)tw.hi,
+                                       (long long)tw.lo, (long long)tw.hi, (long long)tw.lo,
+                                       (long long)tw.hi, (long long)tw.lo);
+    const __m512i v = _mm512_loadu_si512((const void*)bot);
+    const __m512i u = _mm512_xor_si512(_mm512_loadu_si512((const void*)top), mul4_v(v, T));
+    _mm512_storeu_si512((void*)top, u);
+    _mm512_storeu_si512((void*)bot, _mm512_xor_si512(v, u));
+}
+
+/* a[i] * b[i] for four i */
+__attribute__((target("avx512f,avx512bw,vpclmulqdq"))) static void mul4(const or_f128* a, const or_f128* b,
+                                                                     or_f128* out) {
+    _mm512_storeu_si512((void*)out, mul4_v(_mm512_loadu_si512((const void*)a),
+                                           _mm512_loadu_si512((const void*)b)));
+}
+#endif
+
+static int g_vpclmul = -1;
+
+/* AVX-512F/BW + VPCLMULQDQ with the ZMM state enabled by the OS */
+int or_has_vpclmul(void) {
+    if (g_vpclmul < 0) {
+        g_vpclmul = 0;
+#if defined(__x86_64__)
+        unsigned a, b, c, d;
+        if (__get_cpuid(1, &a, &b, &c, &d) && (c & bit_OSXSAVE) &&
+            __get_cpuid_count(7, 0, &a, &b, &c, &d) && (b & bit_AVX512F) && (b & bit_AVX512BW) &&
+            (c & bit_VPCLMULQDQ)) {
+            unsigned lo, hi;
+            __asm__ volatile("xgetbv" : "=a"(lo), "=d"(hi) : "c"(0));
+            g_vpclmul = (lo & 0xE6) == 0xE6;
+        }
+        const char* off = getenv("FLOCK_ORACLE_NO_VPCLMUL");
+        if (off && *off && *off != '0') g_vpclmul = 0;
+#endif
+    }
+    return g_vpclmul;
+}
+
 static int g_pclmul = -1;
 
 int or_has_pclmul(void) {
@@ -121,7 +184,12 @@ void or_set_threads(int n) {
 }
 
 void or_f128_mul_batch(const or_f128* a, const or_f128* b, or_f128* out, size_t n) {
-    for (size_t i = 0; i < n; i++) out[i] = or_f128_mul(a[i], b[i]);
+    size_t i = 0;
+#if defined(__x86_64__)
+    if (or_has_vpclmul())
+        for (; i + 4 <= n; i += 4) mul4(a + i, b + i, out + i);
+#endif
+    for (; i < n; i++) out[i] = or_f128_mul(a[i], b[i]);
 }
 
 static inline or_f128 add(or_f128 a, or_f128 b) {
@@ -221,20 +289,28 @@ int or_ntt_forward_interleaved(or_f128* data, size_t num_ntts, size_t live, int 
         size_t num_blocks = (size_t)1 << layer;
         size_t block_size = (size_t)1 << (log_d - layer);
         size_t half = block_size >> 1;
-        for (size_t block = 0; block < num_blocks; block++) {
+        const int vec = or_has_vpclmul();
+        /* every (block, row) butterfly row is independent: split them over the threads */
+#pragma omp parallel for schedule(static) if (num_blocks * half * live >= 4096)
+        for (size_t br = 0; br < num_blocks * half; br++) {
+            const size_t block = br / half, row = br % half;
             or_f128 tw = or_ntt_twiddle(ev, L, layer, block);
             size_t block_start = block * block_size * num_ntts;
-            for (size_t row = 0; row < half; row++) {
-                size_t off_top = block_start + row * num_ntts;
-                size_t off_bot = off_top + half * num_ntts;
-                for (size_t lane = 0; lane < live; lane++) {
-                    or_f128 v = data[off_bot + lane];
-                    or_f128 u = add(data[off_top + lane], or_f128_mul(v, tw));
-                    data[off_top + lane] = u;
-                    data[off_bot + lane] = add(v, u);
-                }
+            size_t off_top = block_start + row * num_ntts;
+            size_t off_bot = off_top + half * num_ntts;
+            size_t lane = 0;
+#if defined(__x86_64__)
+            if (vec)
+                for (; lane + 4 <= live; lane += 4) butterfly4(data + off_top + lane, data + off_bot + lane, tw);
+#endif
+            for (; lane < live; lane++) {
+                or_f128 v = data[off_bot + lane];
+                or_f128 u = add(data[off_top + lane], or_f128_mul(v, tw));
+                data[off_top + lane] = u;
+                data[off_bot + lane] = add(v, u);
             }
         }
+        (void)vec;
     }
     free(ev);
     return 0;
@@ -364,10 +440,12 @@ void or_sha256(const uint8_t* data, size_t len, uint8_t out[32]) {
 int or_merkle_tree(const uint8_t* data, size_t data_len, size_t num_leaves, uint8_t* tree) {
     if (num_leaves == 0 || (num_leaves & (num_leaves - 1)) || data_len % num_leaves) return -1;
     size_t leaf = data_len / num_leaves;
+#pragma omp parallel for schedule(static) if (num_leaves >= 256)
     for (size_t i = 0; i < num_leaves; i++) or_sha256(data + i * leaf, leaf, tree + 32 * i);
     size_t rs = 0, rl = num_leaves;
     while (rl > 1) {
         size_t nl = rl >> 1;
+#pragma omp parallel for schedule(static) if (nl >= 256)
         for (size_t i = 0; i < nl; i++)
             or_sha256(tree + 32 * (rs + 2 * i), 64, tree + 32 * (rs + rl + i));
         rs += rl;
@@ -626,8 +704,11 @@ void or_build_eq(const or_f128* r, size_t n, or_f128* t) {
     t[0] = ONE;
     for (size_t i = 0; i < n; i++) {
         or_f128 ri = r[i], omr = add(ONE, r[i]);
-        for (size_t x = ((size_t)1 << i); x-- > 0;) {
-            t[x | ((size_t)1 << i)] = or_f128_mul(t[x], ri);
+        const size_t len = (size_t)1 << i;
+        /* entry x only feeds x and x | len: the level's entries are independent */
+#pragma omp parallel for schedule(static) if (len >= 8192)
+        for (size_t x = 0; x < len; x++) {
+            t[x | len] = or_f128_mul(t[x], ri);
             t[x] = or_f128_mul(t[x], omr);
         }
     }
@@ -1187,6 +1268,7 @@ void or_quirky_eq(or_f128 z_skip, const or_f128* x_rest, size_t n_rest, or_f128*
 void or_csc_fold_split(size_t n_cols, const uint32_t* a_ptr, const uint32_t* a_rows,
                        const uint32_t* b_ptr, const uint32_t* b_rows, const or_f128* eq,
                        or_f128* sa, or_f128* sb) {
+#pragma omp parallel for schedule(dynamic, 256) if (n_cols >= 1024)
     for (size_t c = 0; c < n_cols; c++) {
         or_f128 x = ZERO, y = ZERO;
         for (uint32_t k = a_ptr[c]; k < a_ptr[c + 1]; k++) x = add(x, eq[a_rows[k]]);
@@ -1206,12 +1288,17 @@ void or_partial_fold(const uint8_t* z, int k_log, int nu, size_t useful_bits, si
     size_t n_stripes = (n_rows + 7) / 8;
     (void)nu;
     for (size_t i = 0; i < k; i++) out[i] = ZERO;
-    for (size_t s = 0; s < n_stripes; s++)
-        for (size_t i = 0; i < useful_bits; i++) {
+    /* each output column i is its own sum over the stripes */
+#pragma omp parallel for schedule(static) if (useful_bits * n_stripes >= 65536)
+    for (size_t i = 0; i < useful_bits; i++) {
+        or_f128 acc = ZERO;
+        for (size_t s = 0; s < n_stripes; s++) {
             uint8_t byte = z[s * k + i];
             for (int r = 0; r < 8; r++)
-                if ((byte >> r) & 1) out[i] = add(out[i], eq_outer[8 * s + r]);
+                if ((byte >> r) & 1) acc = add(acc, eq_outer[8 * s + r]);
         }
+        out[i] = acc;
+    }
 }
 
 /* sumcheck_round_eval_par (lincheck.rs:1479-1502): top-bit split. */

@@ -30,6 +30,8 @@ or_f128 or_f128_inv(or_f128 a);                       /* inv, :67-80 */
 or_f128 or_f128_mul_by_x(or_f128 a);                  /* mul_by_x, :213-220 */
 void or_f128_mul_batch(const or_f128* a, const or_f128* b, or_f128* out, size_t n);
 int or_has_pclmul(void);
+/* AVX-512 with VPCLMULQDQ (the 4-lane multiply of the batch products and the NTT rows) */
+int or_has_vpclmul(void);
 
 /* ---- SplitMix64 (crates/flock-prover/src/bin/dump_commit_vectors.rs:42-68) ---- */
 void or_splitmix_f128(uint64_t seed, or_f128* out, size_t n);

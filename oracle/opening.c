@@ -80,6 +80,9 @@ static inline f256 ef(or_f128 a) {
     f256 r = {a, {0, 0}};
     return r;
 }
+/* XOR sums split over OpenMP threads (exact: the field sum is associative and commutative) */
+#pragma omp declare reduction(fx : or_f128 : omp_out = fa(omp_out, omp_in)) initializer(omp_priv = (or_f128){0, 0})
+#pragma omp declare reduction(ex : f256 : omp_out = ea(omp_out, omp_in)) initializer(omp_priv = (f256){{0, 0}, {0, 0}})
 static const f256 E1 = {{1, 0}, {0, 0}};
 
 /* ------------------------------------------------------------------------- */
@@ -728,6 +731,7 @@ static inline or_f128 lin_apply(const lin_t* L, or_f128 x) {
 /* (G(1), G(∞)) over LSB pairs (jagged.rs:4605-4620) */
 static void msg_g1(const or_f128* a, const or_f128* b, size_t n, or_f128* g1, or_f128* gi) {
     or_f128 x = Z0, y = Z0;
+#pragma omp parallel for schedule(static) reduction(fx : x, y) if (n >= 16384)
     for (size_t i = 0; i < n / 2; i++) {
         x = fa(x, fm(a[2 * i + 1], b[2 * i + 1]));
         y = fa(y, fm(fa(a[2 * i], a[2 * i + 1]), fa(b[2 * i], b[2 * i + 1])));
@@ -737,7 +741,16 @@ static void msg_g1(const or_f128* a, const or_f128* b, size_t n, or_f128* g1, or
 }
 /* v'[i] = v[2i] + r (v[2i+1] + v[2i]) (jagged.rs:4693-4707) */
 static void fold_lsb(or_f128* v, size_t n, or_f128 r) {
-    for (size_t i = 0; i < n / 2; i++) v[i] = fa(v[2 * i], fm(r, fa(v[2 * i + 1], v[2 * i])));
+    if (n < 16384) {
+        for (size_t i = 0; i < n / 2; i++) v[i] = fa(v[2 * i], fm(r, fa(v[2 * i + 1], v[2 * i])));
+        return;
+    }
+    /* in place, entry i overwrites a slot a lower entry still reads: fold into a copy */
+    or_f128* t = (or_f128*)malloc(sizeof(or_f128) * (n / 2));
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n / 2; i++) t[i] = fa(v[2 * i], fm(r, fa(v[2 * i + 1], v[2 * i])));
+    memcpy(v, t, sizeof(or_f128) * (n / 2));
+    free(t);
 }
 
 /* F256 blocked pairing: output o = blk*d + w reads lo = 2 blk d + w, hi = lo + d
@@ -745,6 +758,7 @@ static void fold_lsb(or_f128* v, size_t n, or_f128 r) {
 static void e_msg_blocked(const f256* f, const f256* b, size_t n, size_t d, f256* u0, f256* u2) {
     f256 x = E0, y = E0;
     if (d == 1 || n == d) d = 1;
+#pragma omp parallel for schedule(static) reduction(ex : x, y) if (n >= 8192)
     for (size_t o = 0; o < n / 2; o++) {
         size_t lo = 2 * (o / d) * d + o % d, hi = lo + d;
         x = ea(x, em(f[lo], b[lo]));
@@ -755,6 +769,7 @@ static void e_msg_blocked(const f256* f, const f256* b, size_t n, size_t d, f256
 }
 static void e_fold_blocked(f256* v, size_t n, size_t d, f256 r) {
     f256* t = (f256*)malloc(sizeof(f256) * (n / 2));
+#pragma omp parallel for schedule(static) if (n >= 8192)
     for (size_t o = 0; o < n / 2; o++) {
         size_t lo = 2 * (o / d) * d + o % d, hi = lo + d;
         t[o] = ea(v[lo], em(r, ea(v[hi], v[lo])));
@@ -800,23 +815,32 @@ static or_f128* induced_basis(int log_cols, const size_t* queries, size_t nq,
     for (int i = 0; i <= log_cols; i++) inv[i] = fz(skv[i]) ? Z0 : or_f128_inv(skv[i]);
     or_f128* aw = eq_table(alpha, n_alpha);
     or_f128* basis = (or_f128*)calloc(n, sizeof(or_f128));
-    or_f128* loc = (or_f128*)malloc(sizeof(or_f128) * n);
-    or_f128 sk[64];
-    for (size_t i = 0; i < nq; i++) {
-        or_f128 x = {(uint64_t)queries[i], 0};
-        if (log_cols > 0) {
-            sk[0] = x;
-            for (int k = 1; k < log_cols; k++) sk[k] = fa(fm(sk[k - 1], sk[k - 1]), fm(skv[k - 1], sk[k - 1]));
-            for (int k = 0; k < log_cols; k++) sk[k] = fm(sk[k], inv[k]);
+    /* the queries' tensors are summed into per-thread bases, then into basis */
+#pragma omp parallel if (n * nq >= 65536)
+    {
+        or_f128* loc = (or_f128*)malloc(sizeof(or_f128) * n);
+        or_f128* part = (or_f128*)calloc(n, sizeof(or_f128));
+        or_f128 sk[64];
+#pragma omp for schedule(static)
+        for (size_t i = 0; i < nq; i++) {
+            or_f128 x = {(uint64_t)queries[i], 0};
+            if (log_cols > 0) {
+                sk[0] = x;
+                for (int k = 1; k < log_cols; k++) sk[k] = fa(fm(sk[k - 1], sk[k - 1]), fm(skv[k - 1], sk[k - 1]));
+                for (int k = 0; k < log_cols; k++) sk[k] = fm(sk[k], inv[k]);
+            }
+            loc[0] = aw[i];
+            for (int k = 0; k < log_cols; k++) {
+                size_t len = (size_t)1 << k;
+                for (size_t j = 0; j < len; j++) loc[j + len] = fm(sk[k], loc[j]);
+            }
+            for (size_t j = 0; j < n; j++) part[j] = fa(part[j], loc[j]);
         }
-        loc[0] = aw[i];
-        for (int k = 0; k < log_cols; k++) {
-            size_t len = (size_t)1 << k;
-            for (size_t j = 0; j < len; j++) loc[j + len] = fm(sk[k], loc[j]);
-        }
-        for (size_t j = 0; j < n; j++) basis[j] = fa(basis[j], loc[j]);
+#pragma omp critical
+        for (size_t j = 0; j < n; j++) basis[j] = fa(basis[j], part[j]);
+        free(part);
+        free(loc);
     }
-    free(loc);
     free(aw);
     return basis;
 }
@@ -889,6 +913,7 @@ static void sc_introduce(sc256_t* s, f256* basis, f256* u0, f256* u2) {
     s->pend = basis;
 }
 static void sc_glue(sc256_t* s, or_f128 beta) { /* glue, :1583-1593 */
+#pragma omp parallel for schedule(static) if (s->n >= 8192)
     for (size_t i = 0; i < s->n; i++) s->b[i] = ea(s->b[i], emb(s->pend[i], beta));
     free(s->pend);
     s->pend = NULL;
@@ -938,6 +963,7 @@ static int inner_open(or_fs* fs, const or_f128* q, const or_pcs_params* P, const
     or_fs_sample_f128_vec(fs, &gamma, 1); /* batch bits 0 for one claim */
     or_f128 target = fm(gamma, q_eval);
     or_f128* b0 = eq_table(rho, (size_t)log_n);
+#pragma omp parallel for schedule(static) if (N >= 8192)
     for (size_t i = 0; i < N; i++) b0[i] = fm(gamma, b0[i]);
 
     /* recursive_prover_with_basis_impl */
@@ -956,6 +982,7 @@ static int inner_open(or_fs* fs, const or_f128* q, const or_pcs_params* P, const
         or_fs_sample_f128_vec(fs, z, (size_t)log_n);
         or_f128* ez = eq_table(z, (size_t)log_n);
         or_f128 y = Z0;
+#pragma omp parallel for schedule(static) reduction(fx : y) if (N >= 8192)
         for (size_t i = 0; i < N; i++) y = fa(y, fm(q[i], ez[i]));
         or_fs_observe_f128(fs, y);
         f128v_push(&ood, y);
@@ -963,6 +990,7 @@ static int inner_open(or_fs* fs, const or_f128* q, const or_pcs_params* P, const
         or_f128 beta = grind_sample(fs, C->lv[0].claim, &nonce);
         u64v_push(&claim_n, nonce);
         target = fa(target, fm(beta, y));
+#pragma omp parallel for schedule(static) if (N >= 8192)
         for (size_t i = 0; i < N; i++) b0[i] = fa(b0[i], fm(beta, ez[i]));
         free(z);
         free(ez);
@@ -1024,6 +1052,7 @@ static int inner_open(or_fs* fs, const or_f128* q, const or_pcs_params* P, const
         or_f128* ez = eq_table(z, (size_t)split_dim);                                         \
         or_f128 y = Z0;                                                                       \
         f256* eb = (f256*)malloc(sizeof(f256) * s.n);                                          \
+        _Pragma("omp parallel for schedule(static) reduction(fx : y) if (s.n >= 8192)")       \
         for (size_t i = 0; i < s.n; i++) {                                                     \
             y = fa(y, fm(s.f[i].c0, ez[i]));                                                  \
             eb[i] = ef(ez[i]);                                                                \
@@ -1396,6 +1425,7 @@ int or_open_merged(or_fs* fs, const or_f128* q, const or_pcs_params* P, const or
     lin_t* Lg = (lin_t*)malloc(sizeof(lin_t));
     lin_make(Lg, images);
     or_f128* G = eq_table(rho, (size_t)m);
+#pragma omp parallel for schedule(static) if (N >= 8192)
     for (size_t i = 0; i < N; i++) G[i] = lin_apply(Lg, G[i]);
     free(Lg);
     /* ā = Σ_c γ^(128 c) eq_row_c ⊗ eq_col_c on the jagged area */

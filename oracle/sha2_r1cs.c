@@ -136,8 +136,11 @@ static uint32_t const_parts(uint32_t k, uint32_t y, uint32_t* l, uint32_t* r, ui
 }
 
 static void orbits(uint64_t* buf, unsigned pos, uint32_t v, unsigned n) {
-    for (unsigned i = 0; i < n; i++)
-        if ((v >> i) & 1) buf[(pos + i) >> 6] |= 1ull << ((pos + i) & 63);
+    /* bits [0, n) of v at bit position pos (word-wise: one or two words) */
+    const uint64_t w = n >= 32 ? (uint64_t)v : ((uint64_t)v & ((1ull << n) - 1));
+    const unsigned sh = pos & 63;
+    buf[pos >> 6] |= w << sh;
+    if (sh && sh + n > 64) buf[(pos >> 6) + 1] |= w >> (64 - sh);
 }
 
 static void put3(uint64_t* z, uint64_t* a, uint64_t* b, unsigned pos, const uint32_t tr[3],
@@ -257,27 +260,49 @@ int or_sha2_witness_batch_major(const uint32_t* inputs, size_t n_decl, int n_log
     memset(a, 0, n * 256 * sizeof(or_f128));
     memset(b, 0, n * 256 * sizeof(or_f128));
     memset(stripe, 0, n * (K / 8));
-    uint64_t* bz = (uint64_t*)malloc(words * 8);
-    uint64_t* ba = (uint64_t*)malloc(words * 8);
-    uint64_t* bb = (uint64_t*)malloc(words * 8);
-    for (size_t o = 0; o < n_decl; o++) {
-        or_sha2_instance(inputs + 24 * o, inputs + 24 * o + 8, bz, ba, bb);
-        for (size_t c = 0; c < 256; c++) {
-            const size_t idx = (c << n_log) + o;
-            z[idx].lo = bz[2 * c];
-            z[idx].hi = bz[2 * c + 1];
-            a[idx].lo = ba[2 * c];
-            a[idx].hi = ba[2 * c + 1];
-            b[idx].lo = bb[2 * c];
-            b[idx].hi = bb[2 * c + 1];
+    /* groups of 8 rows are independent (each owns its stripe bytes): one group per iteration */
+    const size_t groups = (n_decl + 7) / 8;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+    for (size_t g = 0; g < groups; g++) {
+        uint64_t* bz = (uint64_t*)malloc(8 * words * 8);
+        uint64_t* ba = (uint64_t*)malloc(words * 8);
+        uint64_t* bb = (uint64_t*)malloc(words * 8);
+        if (!bz || !ba || !bb) {
+            bad = 1;
+        } else {
+            const size_t o1 = (g + 1) * 8 < n_decl ? (g + 1) * 8 : n_decl;
+            memset(bz, 0, 8 * words * 8);
+            for (size_t o = g * 8; o < o1; o++) {
+                uint64_t* zr = bz + (o - g * 8) * words;
+                or_sha2_instance(inputs + 24 * o, inputs + 24 * o + 8, zr, ba, bb);
+                for (size_t c = 0; c < 256; c++) {
+                    const size_t idx = (c << n_log) + o;
+                    z[idx].lo = zr[2 * c];
+                    z[idx].hi = zr[2 * c + 1];
+                    a[idx].lo = ba[2 * c];
+                    a[idx].hi = ba[2 * c + 1];
+                    b[idx].lo = bb[2 * c];
+                    b[idx].hi = bb[2 * c + 1];
+                }
+            }
+            /* stripe byte i of the group = bit i of its 8 rows (row r at bit r) */
+            uint8_t* sg = stripe + g * K;
+            for (size_t w = 0; w < words; w++) {
+                uint64_t x[8];
+                for (int r = 0; r < 8; r++) x[r] = bz[r * words + w];
+                for (int i = 0; i < 64; i++) {
+                    uint8_t v = 0;
+                    for (int r = 0; r < 8; r++) v |= (uint8_t)(((x[r] >> i) & 1) << r);
+                    sg[w * 64 + i] = v;
+                }
+            }
         }
-        for (size_t i = 0; i < K; i++)
-            if ((bz[i >> 6] >> (i & 63)) & 1) stripe[(o / 8) * K + i] |= (uint8_t)(1u << (o % 8));
+        free(bz);
+        free(ba);
+        free(bb);
     }
-    free(bz);
-    free(ba);
-    free(bb);
-    return 0;
+    return bad;
 }
 
 /* ======================================================================== */
