@@ -1090,35 +1090,44 @@ __global__ void __launch_bounds__(kThreads) k_induced_sk(const uint64_t* qs, uin
 
 // Induced basis (ligerito.rs:2676-2770 with empty rows) as out = sum_i lo_i (x) hi_i:
 // lo_i[j] = aw_i prod_{k < h, bit k of j} s_ik over the low h coordinates, hi_i[j] =
-// prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i)
+// prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i). Both are built by
+// doubling in LDS, level k doubling the table by its factor s_ik (one product per entry).
 __global__ void __launch_bounds__(kThreads) k_induced_tables(const f128* s, const f128* aw, uint32_t dims,
                                                              uint32_t h, f128* lo, f128* hi) {
+    extern __shared__ f128 tt[];  // lo (2^h entries), then hi (2^(dims - h))
     const uint32_t i = blockIdx.x;
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
     const f128* si = s + (size_t)i * dims;
-    for (uint64_t j = threadIdx.x; j < nl + nh; j += blockDim.x) {
-        if (j < nl) {
-            f128 v = aw[i];
-            for (uint32_t k = 0; k < h; k++)
-                if ((j >> k) & 1) v = f128_mul(v, si[k]);
-            lo[i * nl + j] = v;
-        } else {
-            const uint64_t jj = j - nl;
-            f128 v = f128_one();
-            for (uint32_t k = h; k < dims; k++)
-                if ((jj >> (k - h)) & 1) v = f128_mul(v, si[k]);
-            hi[i * nh + jj] = v;
-        }
+    f128* tl = tt;
+    f128* th = tt + nl;
+    if (threadIdx.x == 0) {
+        tl[0] = aw[i];
+        th[0] = f128_one();
     }
+    __syncthreads();
+    const uint32_t levels = h > dims - h ? h : dims - h;
+    for (uint32_t k = 0; k < levels; k++) {
+        const uint64_t len = 1ull << k;
+        for (uint64_t j = threadIdx.x; j < 2 * len; j += blockDim.x) {
+            if (j < len) {
+                if (k < h) tl[len + j] = f128_mul(tl[j], si[k]);
+            } else if (k < dims - h) {
+                th[j] = f128_mul(th[j - len], si[h + k]);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint64_t j = threadIdx.x; j < nl; j += blockDim.x) lo[i * nl + j] = tl[j];
+    for (uint64_t j = threadIdx.x; j < nh; j += blockDim.x) hi[i * nh + j] = th[j];
 }
-// out[jl + 2^h jh] ^= sum_{i in this block's chunk} lo_i[jl] hi_i[jh]: kIndTL rows jl per block
-// (their Shoup tables in LDS, rebuilt per i), one column jh per thread; with several chunks the
-// blocks XOR into out atomically (out zeroed beforehand)
+// out[jl + 2^h jh] = sum_{i in this block's chunk} lo_i[jl] hi_i[jh]: kIndTL rows jl per block
+// (their Shoup tables in LDS, rebuilt per i), kIndTH columns jh per thread; chunk z of the
+// queries writes its partial sums at out + z * 2^dims (k_xor_chunks adds them up)
 constexpr int kIndTL = 4;  // jl rows per block
 constexpr int kIndTH = 2;  // jh columns per thread (blockDim * kIndTH per block)
 __global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const f128* hi, uint32_t nq,
                                                            uint32_t h, uint32_t dims, uint32_t chunk,
-                                                           int atomic, f128* out) {
+                                                           f128* out) {
     __shared__ f128 tab[kIndTL * SHOUP_ENTRIES];
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
     const uint64_t jl0 = (uint64_t)blockIdx.x * kIndTL;
@@ -1155,16 +1164,7 @@ __global__ void __launch_bounds__(kThreads) k_induced_gemm(const f128* lo, const
 #pragma unroll
         for (int t = 0; t < kIndTL; t++) {
             if (jl0 + t >= nl) break;
-            f128* o = out + (jl0 + t) + nl * jh;
-            if (atomic) {
-                uint32_t* w = reinterpret_cast<uint32_t*>(o);
-                if (acc[u][t].x) atomicXor(w, acc[u][t].x);
-                if (acc[u][t].y) atomicXor(w + 1, acc[u][t].y);
-                if (acc[u][t].z) atomicXor(w + 2, acc[u][t].z);
-                if (acc[u][t].w) atomicXor(w + 3, acc[u][t].w);
-            } else {
-                *o = acc[u][t];
-            }
+            out[(blockIdx.z << dims) + (jl0 + t) + nl * jh] = acc[u][t];
         }
     }
 }
@@ -1269,6 +1269,17 @@ __global__ void __launch_bounds__(kThreads) k_sc_fold_la(const f128* f, const f1
     block_xor_atomic<32>(v, acc);
 }
 
+// out[i] = sum_z part[z * n + i], z < gz
+__global__ void __launch_bounds__(kThreads) k_xor_chunks(const f128* part, uint32_t gz, uint64_t n,
+                                                         f128* out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        f128 v = part[i];
+        for (uint32_t z = 1; z < gz; z++) v = f128_add(v, part[z * n + i]);
+        out[i] = v;
+    }
+}
+
 }  // namespace
 }  // namespace flock
 
@@ -1349,7 +1360,8 @@ hipError_t comb_attrs() {
                              reinterpret_cast<const void*>(&flock::k_intro_base),
                              reinterpret_cast<const void*>(&flock::k_msg_blocked_comb),
                              reinterpret_cast<const void*>(&flock::k_dot_comb),
-                             reinterpret_cast<const void*>(&flock::k_sc_fold_la)};
+                             reinterpret_cast<const void*>(&flock::k_sc_fold_la),
+                             reinterpret_cast<const void*>(&flock::k_induced_tables)};
         for (const void* f : fns)
             if (hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    kCombWsBytes);
@@ -1881,26 +1893,27 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
         return false;
     hipLaunchKernelGGL(flock::k_induced_sk, dim3((nq + kThreads - 1) / kThreads), dim3(kThreads), 0, X.st, d_q, nq,
                        reinterpret_cast<const uint4*>(d_c), (uint32_t)dims, reinterpret_cast<uint4*>(d_s));
-    hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 0, X.st, reinterpret_cast<const uint4*>(d_s),
+    hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 16 * (nl + nh), X.st, reinterpret_cast<const uint4*>(d_s),
                        reinterpret_cast<const uint4*>(d_aw), (uint32_t)dims, (uint32_t)h, reinterpret_cast<uint4*>(d_lo),
                        reinterpret_cast<uint4*>(d_hi));
     if (!X.chk("induced tables")) return false;
     // split the query sum over blocks until the grid holds about 2048 workgroups
     const uint32_t gx = (uint32_t)((nl + flock::kIndTL - 1) / flock::kIndTL);
     const uint32_t gy = (uint32_t)((nh + kThreads * flock::kIndTH - 1) / (kThreads * flock::kIndTH));
-    // at least 16 queries per block so the per-query table builds amortize
-    uint32_t gz = std::max<uint32_t>(1, std::min<uint32_t>((nq + 15) / 16, 2048 / std::max<uint32_t>(1, gx * gy)));
+    // split the query sum into chunks until the grid holds about 1024 workgroups, keeping at
+    // least 16 queries per chunk so the per-query table builds amortize
+    uint32_t gz = std::max<uint32_t>(
+        1, std::min<uint32_t>((1024 + gx * gy - 1) / std::max<uint32_t>(1, gx * gy), nq / 16));
     const uint32_t chunk = (nq + gz - 1) / gz;
     gz = (nq + chunk - 1) / chunk;
-    if (gz > 1) {
-        const uint64_t words = 4ull << dims;
-        hipLaunchKernelGGL(flock::k_zero32, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, X.st,
-                           reinterpret_cast<uint32_t*>(out), (uint32_t)words);
-        if (!X.chk("zero")) return false;
-    }
+    flock_f128* part = gz > 1 ? X.buf<flock_f128>(53, (size_t)gz << dims) : out;
+    if (X.err) return false;
     hipLaunchKernelGGL(flock::k_induced_gemm, dim3(gx, gy, gz), dim3(kThreads), 0, X.st,
                        reinterpret_cast<const uint4*>(d_lo), reinterpret_cast<const uint4*>(d_hi), nq, (uint32_t)h,
-                       (uint32_t)dims, chunk, gz > 1 ? 1 : 0, reinterpret_cast<uint4*>(out));
+                       (uint32_t)dims, chunk, reinterpret_cast<uint4*>(part));
+    if (gz > 1)
+        hipLaunchKernelGGL(flock::k_xor_chunks, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st,
+                           reinterpret_cast<const uint4*>(part), gz, 1ull << dims, reinterpret_cast<uint4*>(out));
     if (!X.chk("induced basis")) return false;
     // the host copies must outlive the async uploads
     return X.hip(hipStreamSynchronize(X.st), "sync");
