@@ -1325,6 +1325,16 @@ int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f12
     LAUNCH(flock::launch_build_eq(dr, (int)nv, U(out), dr + nv, c->stream));
     return 0;
 }
+int build_eq_halves_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* lo,
+                          flock_f128* hi) {
+    CTX_CHECK(c);
+    if (nv > (size_t)kEqArgMax || !lo || !hi || (nv && !h_r)) return fail(FLOCK_HIP_E_INVALID, "bad eq arguments");
+    if (int rc = c->ensure_scratch(nv * 16 + 16)) return rc;
+    uint4* dr = reinterpret_cast<uint4*>(c->scratch);
+    LAUNCH(flock::launch_eq_coords(U(h_r), (uint32_t)nv, dr, c->stream));
+    LAUNCH(flock::launch_eq_halves(dr, (int)nv, U(lo), U(hi), c->stream));
+    return 0;
+}
 }  // namespace flock
 }
 

@@ -122,6 +122,12 @@ hipError_t launch_eq_coords(const uint4* h_r, uint32_t n, uint4* dst, hipStream_
 // travel in a launch's arguments, so h_r may be released on return; d_out is ready when the
 // stream reaches it
 int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* d_out);
+// The two halves of eq(r, .) over nv <= 40 host coordinates: lo = eq over r[0 .. nv/2) (2^(nv/2)
+// entries), hi = eq over the rest, so that eq(r, x) = lo[x mod 2^(nv/2)] * hi[x >> nv/2].
+int build_eq_halves_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* d_lo,
+                          flock_f128* d_hi);
+// eq halves of nv <= 40 device coordinates r (k_eq_direct alone)
+hipError_t launch_eq_halves(const uint4* r, int nv, uint4* lo, uint4* hi, hipStream_t s);
 // the device a context was made on (-1 for none)
 int ctx_device(const flock_hip_ctx* c);
 // out[x_lo + 2^nlo x_hi] = lo[x_lo] * hi[x_hi] (2^(nlo+nhi) words).

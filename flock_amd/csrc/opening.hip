@@ -1269,6 +1269,77 @@ __global__ void __launch_bounds__(kThreads) k_sc_fold_la(const f128* f, const f1
     block_xor_atomic<32>(v, acc);
 }
 
+// <q, eq(z)> and the eq-basis table of the opening without materialising an eq table: with
+// x = xl + 2^nlo xh, eq(z)[x] = lo[xl] hi[xh] (the two halves, k_eq_direct).
+// k_ood_dot: block (bl, bh) adds hi[xh] q[x] over its kOodXh rows xh (one Shoup table of
+// hi[xh] at a time) for each of its columns xl, multiplies each column's sum by lo[xl] (one
+// comb product per column and block) and XORs the block's total into acc.
+constexpr int kOodXh = 64;  // rows xh per block
+constexpr int kOodXl = 2;   // columns xl per thread
+__global__ void __launch_bounds__(kThreads) k_ood_dot(const f128* q, int nlo, int nhi, const f128* lo,
+                                                      const f128* hi, uint32_t* acc) {
+    __shared__ f128 tab[SHOUP_ENTRIES];
+    extern __shared__ f128 ws[];
+    f128* wl = comb_lane(ws);
+    const uint64_t nl = 1ull << nlo, nh = 1ull << nhi;
+    const uint64_t xl0 = (uint64_t)blockIdx.x * (kThreads * kOodXl);
+    const uint64_t xh0 = (uint64_t)blockIdx.y * kOodXh;
+    f128 col[kOodXl];
+#pragma unroll
+    for (int u = 0; u < kOodXl; u++) col[u] = f128_zero();
+    for (int r = 0; r < kOodXh && xh0 + r < nh; r++) {
+        const uint64_t xh = xh0 + r;
+        __syncthreads();  // the previous row's table is consumed
+        if (threadIdx.x < 32) shoup_build_row(tab, hi[xh], threadIdx.x);
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kOodXl; u++) {
+            const uint64_t xl = xl0 + (uint64_t)u * kThreads + threadIdx.x;
+            if (xl < nl) col[u] = f128_add(col[u], shoup_mul(tab, q[(xh << nlo) + xl]));
+        }
+    }
+    f256u s;
+    f256u_zero(s);
+#pragma unroll
+    for (int u = 0; u < kOodXl; u++) {
+        const uint64_t xl = xl0 + (uint64_t)u * kThreads + threadIdx.x;
+        if (xl < nl) comb_acc(s, wl, col[u], lo[xl]);
+    }
+    uint32_t v[4] = {0};
+    put4(v, f256u_reduce(s));
+    block_xor_atomic<4>(v, acc);
+}
+// b[x] = sum_t (c_t hi_t[xh]) lo_t[xl]: the sum of up to kB0Terms scaled eq tables given by
+// their halves, one block per row xh with one Shoup table per term
+constexpr int kB0Terms = 4;
+struct B0Terms {
+    const f128* lo[kB0Terms];
+    const f128* hi[kB0Terms];
+    f128 c[kB0Terms];
+    int n;
+};
+constexpr int kB0Rows = 1;  // rows xh per block
+__global__ void __launch_bounds__(kThreads) k_b0_build(B0Terms T, int nlo, int nhi, f128* b) {
+    extern __shared__ f128 tab[];  // T.n tables
+    __shared__ f128 mult[kB0Terms];
+    const uint64_t nl = 1ull << nlo, nh = 1ull << nhi;
+    for (int r = 0; r < kB0Rows; r++) {
+        const uint64_t xh = (uint64_t)blockIdx.x * kB0Rows + r;
+        if (xh >= nh) break;
+        __syncthreads();  // the previous row's tables are consumed
+        if (threadIdx.x < (uint32_t)T.n) mult[threadIdx.x] = f128_mul(T.c[threadIdx.x], T.hi[threadIdx.x][xh]);
+        __syncthreads();
+        if (threadIdx.x < 32u * T.n)
+            shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES, mult[threadIdx.x >> 5], threadIdx.x & 31);
+        __syncthreads();
+        for (uint64_t xl = threadIdx.x; xl < nl; xl += blockDim.x) {
+            f128 v = f128_zero();
+            for (int t = 0; t < T.n; t++) v = f128_add(v, shoup_mul(tab + t * SHOUP_ENTRIES, T.lo[t][xl]));
+            b[(xh << nlo) + xl] = v;
+        }
+    }
+}
+
 // out[i] = sum_z part[z * n + i], z < gz
 __global__ void __launch_bounds__(kThreads) k_xor_chunks(const f128* part, uint32_t gz, uint64_t n,
                                                          f128* out) {
@@ -1361,7 +1432,8 @@ hipError_t comb_attrs() {
                              reinterpret_cast<const void*>(&flock::k_msg_blocked_comb),
                              reinterpret_cast<const void*>(&flock::k_dot_comb),
                              reinterpret_cast<const void*>(&flock::k_sc_fold_la),
-                             reinterpret_cast<const void*>(&flock::k_induced_tables)};
+                             reinterpret_cast<const void*>(&flock::k_induced_tables),
+                             reinterpret_cast<const void*>(&flock::k_ood_dot)};
         for (const void* f : fns)
             if (hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    kCombWsBytes);
@@ -1446,6 +1518,11 @@ struct Ctx {
         return hip(hipStreamSynchronize(st), "sync");
     }
     bool chk(const char* where) { return hip(hipGetLastError(), where); }
+    // the two halves of the eq table of host coordinates (flock::build_eq_halves_async)
+    bool eq_halves(const H* r, size_t n, flock_f128* lo, flock_f128* hi) {
+        if (int rc = flock::build_eq_halves_async(c, r, n, lo, hi)) return fail(rc, flock_hip_last_error());
+        return true;
+    }
     // device eq table of host coordinates
     bool eq_dev(const H* r, size_t n, flock_f128* out) {
         if (int rc = flock::build_eq_async(c, r, n, out)) return fail(rc, flock_hip_last_error());
@@ -2042,13 +2119,26 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     observe(fs, q_eval);
     const H gamma = sample_vec(fs, 1)[0];
     H target = hm(gamma, q_eval);
+    // b0 = gamma eq(rho) + sum_o beta_o eq(z_o), written once after the OOD samples: each eq
+    // table stays as its two halves (eq(r)[x] = lo[xl] hi[xh]), <q, eq(z_o)> is summed from q
+    // and the halves, and k_b0_build writes b0 from all the halves at once
+    const int nlo = log_n / 2, nhi = log_n - nlo;
+    const uint64_t nl = 1ull << nlo, nh = 1ull << nhi;
+    const int n_ood = C->lv[0].ood;
+    if (n_ood + 1 > flock::kB0Terms) return X.fail(FLOCK_HIP_E_INVALID, "too many L0 OOD samples");
     flock_f128* b0 = X.buf<flock_f128>(10, N);
-    flock_f128* ez_ = X.buf<flock_f128>(11, N);
+    flock_f128* halves = X.buf<flock_f128>(11, (size_t)(1 + n_ood) * (nl + nh));
+    const uint32_t nbh = (uint32_t)((nh + flock::kOodXh - 1) / flock::kOodXh);
     if (X.err) return false;
-    if (!X.eq_dev(rho.data(), (size_t)log_n, b0)) return false;
-    hipLaunchKernelGGL(flock::k_scale_tab, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0), N,
-                       U4(gamma));
-    if (!X.chk("scale")) return false;
+    flock::B0Terms terms{};
+    auto term = [&](int t, const H* r, H coef) {
+        terms.lo[t] = reinterpret_cast<const uint4*>(halves + (size_t)t * (nl + nh));
+        terms.hi[t] = reinterpret_cast<const uint4*>(halves + (size_t)t * (nl + nh) + nl);
+        terms.c[t] = U4(coef);
+        terms.n = t + 1;
+        return X.eq_halves(r, (size_t)log_n, halves + (size_t)t * (nl + nh), halves + (size_t)t * (nl + nh) + nl);
+    };
+    if (!term(0, rho.data(), gamma)) return false;
     label(fs, "flock-ligerito-basis-f256-split-v0");
     observe(fs, target);
     const int c0 = sch[0].cap();
@@ -2057,12 +2147,13 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     flock_fs_observe_bytes(fs, icap.data(), icap.size());
     std::vector<H> ood;
     std::vector<uint64_t> claim_n, cons_n, grind_n;
-    for (int o = 0; o < C->lv[0].ood; o++) {  // L0 OOD samples
+    for (int o = 0; o < n_ood; o++) {  // L0 OOD samples
         const std::vector<H> z = sample_vec(fs, (size_t)log_n);
-        if (!X.eq_dev(z.data(), (size_t)log_n, ez_)) return false;
-        hipLaunchKernelGGL(flock::k_dot_comb, dim3(X.grid(N)), dim3(kThreads), kCombWsBytes, X.st,
-                           reinterpret_cast<const uint4*>(q),
-                           reinterpret_cast<const uint4*>(ez_), N, X.A->acc);
+        if (!term(1 + o, z.data(), hz())) return false;
+        hipLaunchKernelGGL(flock::k_ood_dot,
+                           dim3((uint32_t)((nl + kThreads * flock::kOodXl - 1) / (kThreads * flock::kOodXl)), nbh),
+                           dim3(kThreads), kCombWsBytes, X.st, reinterpret_cast<const uint4*>(q), nlo, nhi,
+                           terms.lo[1 + o], terms.hi[1 + o], X.A->acc);
         if (!X.chk("ood dot") || !X.read_acc(4)) return false;
         const H y = X.acc_f(0);
         observe(fs, y);
@@ -2071,10 +2162,12 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
         const H beta = grind_sample(fs, C->lv[0].claim, &nonce);
         claim_n.push_back(nonce);
         target = ha(target, hm(beta, y));
-        hipLaunchKernelGGL(flock::k_axpy_tab, dim3(X.grid(N)), dim3(kThreads), 0, X.st, reinterpret_cast<uint4*>(b0),
-                           reinterpret_cast<const uint4*>(ez_), N, U4(beta));
-        if (!X.chk("axpy")) return false;
+        terms.c[1 + o] = U4(beta);
     }
+    hipLaunchKernelGGL(flock::k_b0_build, dim3((uint32_t)((nh + flock::kB0Rows - 1) / flock::kB0Rows)),
+                       dim3(kThreads), (size_t)terms.n * SHOUP_ENTRIES * 16, X.st, terms, nlo, nhi,
+                       reinterpret_cast<uint4*>(b0));
+    if (!X.chk("basis table")) return false;
     T.mark("ood0");
     // first message over blocked pairs (F128)
     hipLaunchKernelGGL(flock::k_msg_blocked_comb, dim3(X.grid(N / 2)), dim3(kThreads), kCombWsBytes, X.st,

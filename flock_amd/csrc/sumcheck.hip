@@ -707,6 +707,15 @@ hipError_t launch_eq_coords(const uint4* h_r, uint32_t n, uint4* dst, hipStream_
     return hipGetLastError();
 }
 
+hipError_t launch_eq_halves(const uint4* r, int nv, uint4* lo, uint4* hi, hipStream_t s) {
+    const int nlo = nv / 2, nhi = nv - nlo;
+    if (nhi > kEqDirectMax) return hipErrorInvalidValue;
+    const uint32_t lo_blocks = (uint32_t)(((1ull << nlo) + 255) / 256);
+    const uint32_t hi_blocks = (uint32_t)(((1ull << nhi) + 255) / 256);
+    k_eq_direct<<<lo_blocks + hi_blocks, 256, 0, s>>>(r, nlo, nhi, lo_blocks, lo, hi);
+    return hipGetLastError();
+}
+
 hipError_t launch_build_eq(const uint4* r, int nv, uint4* out, uint4* scratch, hipStream_t s) {
     // k_eq_direct with zero variables writes the single entry 1.
     const int nlo = nv / 2, nhi = nv - nlo;
