@@ -133,6 +133,21 @@ def mixed_workloads(F, args) -> dict:
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "proofs_identical_to_one_at_a_time": f"{same}/{len(b_specs)}",
     }
+    # the same comparison on small requests only (batch log2 in [10, 12]), where one proof at a
+    # time leaves the GPU mostly idle between round trips
+    sm_specs = specs(64, 10, 12, 0.0, False)
+    reqs_s, ksm = F.make_requests(sm_specs, domain=DOMAIN)
+    hashes_s = sum(1 << sp[1] for sp in sm_specs)
+    F.serve(provers, reqs_s, budget, proofs=True)  # warm-up
+    _, w1s, p1s = F.serve(provers[:1], reqs_s, budget, proofs=True)
+    _, wns, pns = F.serve(provers, reqs_s, budget, proofs=True)
+    res["batched_small"] = {
+        "requests": len(sm_specs), "hashes": hashes_s, "workers": args.workers,
+        "one_at_a_time_khps": round(hashes_s / w1s, 2),
+        "batched_khps": round(hashes_s / wns, 2),
+        "speedup": round(w1s / wns, 3),
+        "proofs_identical_to_one_at_a_time": f"{sum(a == b for a, b in zip(p1s, pns))}/{len(sm_specs)}",
+    }
     # served: Poisson arrivals at args.serve_rate requests/s
     s_specs = specs(32, 10, 18, args.serve_rate, True)
     reqs, ks = F.make_requests(s_specs, domain=DOMAIN)
