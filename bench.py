@@ -367,6 +367,7 @@ def main():
 
     from flock_amd import lib as F
 
+    tuning = F.apply_tuning()  # the autotuner's launch shapes (flock_amd/tuning.json), if any
     ctx = F.Context(0)
     ev = HipEvents()
     stream = ctx.stream
@@ -442,6 +443,7 @@ def main():
                         f"zerocheck m {nl + 15}, lincheck K 2^15, merged Ligerito opening)",
             "batch": n, "domain": DOMAIN.decode(),
         },
+        "tuning": tuning or "library defaults",
         "proof_bytes": len(proof),
         "proof_digest": digest,
         "verified": True,

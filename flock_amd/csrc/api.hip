@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <cstdio>
@@ -845,8 +846,6 @@ inline flock_f128 from_u4(uint4 v) {
 // j-1, from the device pointer rin when that is set, else the value rho; launch(n, ...) is
 // the closing fold. Each message is observed on `fs`, ground (bits > 0) and answered with a
 // challenge, in the context's round mode (flock_hip_ctx_set_round_mode).
-// zerocheck tail rounds on tables of at most this many words run two rounds per launch
-constexpr uint64_t kZcLaMax = 1ull << 17;
 
 struct RoundLoop {
     using Launch = std::function<int(uint32_t, flock::RoundOut, const uint4*, uint4)>;
@@ -932,6 +931,27 @@ struct RoundLoop {
 };
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Launch tunables: name, default, allowed range, current value (process-wide)
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneEntry {
+    const char* name;
+    int64_t def, lo, hi;
+    std::atomic<int64_t> v;
+};
+TuneEntry g_tune[flock::kTuneCount] = {
+    {"ntt_cols_per_thread", 2, 1, 4, {2}},
+    {"ntt_deep_cols", 256, 32, 256, {256}},
+    {"zc_split_min_vars", 16, 10, 24, {16}},
+    {"zc_la_max_log", 17, 10, 22, {17}},
+    {"sc_la_max_log", 18, 10, 24, {18}},
+    {"open_split_fold_log", 16, 10, 22, {16}},
+};
+}  // namespace
+
+int64_t flock::tunable(flock::Tunable t) { return g_tune[t].v.load(std::memory_order_relaxed); }
+
 extern "C" {
 
 int flock_hip_device_count(void) {
@@ -966,6 +986,28 @@ void flock::ctx_release_scratch(flock_hip_ctx* c) {
 }
 
 extern "C" {
+
+int flock_hip_set_tunable(const char* name, int64_t value) {
+    if (!name) return fail(FLOCK_HIP_E_INVALID, "null tunable name");
+    for (TuneEntry& e : g_tune)
+        if (std::strcmp(e.name, name) == 0) {
+            if (value < e.lo || value > e.hi)
+                return fail(FLOCK_HIP_E_INVALID, std::string("tunable ") + name + " out of range");
+            e.v.store(value);
+            return 0;
+        }
+    return fail(FLOCK_HIP_E_INVALID, std::string("unknown tunable ") + name);
+}
+
+int flock_hip_get_tunable(const char* name, int64_t* value) {
+    if (!name || !value) return fail(FLOCK_HIP_E_INVALID, "null tunable argument");
+    for (TuneEntry& e : g_tune)
+        if (std::strcmp(e.name, name) == 0) {
+            *value = e.v.load();
+            return 0;
+        }
+    return fail(FLOCK_HIP_E_INVALID, std::string("unknown tunable ") + name);
+}
 
 int flock_hip_ctx_set_round_mode(flock_hip_ctx* c, int mode) {
     CTX_CHECK(c);
@@ -1449,7 +1491,6 @@ int flock_hip_fold_1b_rows(flock_hip_ctx* c, const flock_f128* rows, size_t n_ro
 }
 
 // Tail rounds whose eq table has at least this many variables use the split-eq kernel.
-constexpr uint32_t kZcSplitMinVars = 16;
 
 static uint32_t bits_for(uint64_t n) {  // floor(log2 n) + 1, zerocheck.rs:91-94
     uint32_t b = 0;
@@ -1496,7 +1537,8 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
     // Tail rounds j < j0, whose eq table (over r[7+j..m), nv = n_mlv-1-j variables) has
     // nv >= 16, run split: eq = e_lo (8 variables, EL) x e_hi (nv-8 variables, level j of EH0).
     const uint32_t nv0 = n_mlv - 1;
-    const uint32_t j0 = nv0 >= kZcSplitMinVars ? nv0 - kZcSplitMinVars + 1 : 0;
+    const uint32_t split_min = (uint32_t)flock::tunable(flock::kTuneZcSplitMinVars);
+    const uint32_t j0 = nv0 >= split_min ? nv0 - split_min + 1 : 0;
     const uint64_t eh0 = j0 ? 1ull << (nv0 - 8) : 0;
     const uint64_t words = 2 * N + 2 * h + 2 * q + h + q + e9n + 2048 + 64 + eqs + 2 * eh0 +
                            256ull * j0;
@@ -1624,14 +1666,14 @@ int flock_hip_zerocheck_prove(flock_hip_ctx* c, flock_fs* fs, const uint8_t* d_a
                        }
                        return 0;
                    }};
-    // the unsplit rounds on tables of at most kZcLaMax words: two rounds per launch
+    // the unsplit rounds on tables of at most 2^zc_la_max_log words: two rounds per launch
     // (launch_zc_fold_la), tracking the tables and round eq tables the launches hand on
     const uint4* la_a = nullptr;
     const uint4* la_b = nullptr;
     uint4* la_e[2] = {nullptr, nullptr};  // [current round's eq, the free one]
     loop.la_from = std::max<uint32_t>(std::max<uint32_t>(j0, 1u), n_mlv);
     for (uint32_t j = std::max<uint32_t>(j0, 1u); j < n_mlv; j++)
-        if ((N >> (j - 1)) <= kZcLaMax) {
+        if ((N >> (j - 1)) <= (1ull << flock::tunable(flock::kTuneZcLaMaxLog))) {
             loop.la_from = j;
             break;
         }
@@ -1793,7 +1835,8 @@ int flock_hip_prove_sumcheck(flock_hip_ctx* c, flock_fs* fs, const flock_f128* d
     // zerocheck rounds j < j0 (eq over r[1+j..log_n) with >= 16 variables) run split, as in
     // flock_hip_zerocheck_prove: e_lo windows EL, e_hi = level j of EH0 = eq over r[9..log_n)
     const uint32_t nv0 = log_n - 1;
-    const uint32_t j0 = nv0 >= kZcSplitMinVars ? nv0 - kZcSplitMinVars + 1 : 0;
+    const uint32_t split_min = (uint32_t)flock::tunable(flock::kTuneZcSplitMinVars);
+    const uint32_t j0 = nv0 >= split_min ? nv0 - split_min + 1 : 0;
     const size_t eh0 = j0 ? (size_t)1 << (nv0 - 8) : 0;
     const size_t words = 2 * h + 2 * q + h + q + eq_scratch + 2 * eh0 + 256 * (size_t)j0;
     if (int rc = c->ensure_scratch(words * 16)) return rc;

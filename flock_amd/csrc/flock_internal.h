@@ -76,6 +76,18 @@ hipError_t launch_zc_fold_round(const uint4* a, const uint4* b, uint64_t n, uint
                                 uint4* ao, uint4* bo, const uint4* e, uint4* e_next,
                                 uint4* partials, RoundOut ro, hipStream_t s,
                                 const uint4* rin = nullptr);
+// Process-wide launch tunables (set through flock_hip_set_tunable, read at each launch).
+enum Tunable : int {
+    kTuneNttColsPerThread = 0,  // columns per thread of the interior NTT pass (1..4)
+    kTuneNttDeepCols,           // a pass with at most this many columns per block runs deep (32..256)
+    kTuneZcSplitMinVars,        // zerocheck rounds whose eq has >= this many variables split it (10..24)
+    kTuneZcLaMaxLog,            // zerocheck tail: two rounds per launch from 2^this words (10..22)
+    kTuneScLaMaxLog,            // opening product sumchecks: two rounds per launch from 2^this (10..24)
+    kTuneOpenSplitFoldLog,      // F256 ladder: fold and message in two passes from 2^this (10..22)
+    kTuneCount
+};
+int64_t tunable(Tunable t);
+
 // Where a grid reduction of several words goes: the workgroups fold into acc (the words, then
 // a ticket counter; zero between launches), and the last one moves the sums to out (host-
 // visible), clears acc and stores seq into flag (system scope).

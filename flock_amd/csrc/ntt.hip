@@ -77,7 +77,6 @@ hipError_t launch_fill_replicate(const uint4* msg, uint4* cw, uint64_t msg_len, 
 
 constexpr int kNttThreads = 512;
 constexpr int kMaxNttK = 40;       // largest transform log-size the library accepts
-constexpr uint64_t kDeepCols = 256; // up to this many columns per high block: k_ntt_pass_deep
 
 // One fused pass over layers [l0, l0 + K). src == data for an in-place pass; for the
 // first pass of a commit src is the message itself: the state after the log_inv_rate
@@ -244,7 +243,7 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
     constexpr int NT = (1 << K) - 1;
     const uint64_t n_hb = 1ull << l0;
     const uint64_t cols = (1ull << (k - l0 - K)) * t;
-    if (cols <= kDeepCols) {
+    if (cols <= (uint64_t)tunable(kTuneNttDeepCols)) {
         if (lm_d) return hipErrorInvalidValue;  // the caller fills instead
         uint32_t per = (uint32_t)(kDeepThreads / cols);  // high blocks per group
         if (per > (uint32_t)kDeepMaxHb) per = kDeepMaxHb;
@@ -260,8 +259,9 @@ static hipError_t launch_pass(const f128* src, f128* data, uint32_t t, uint32_t 
             src, data, t, live, k, l0, evals, (uint32_t)cols, per, n_hb, from_msg ? 1 : 0);
         return hipGetLastError();
     }
-    // up to 2 columns per thread so the twiddle-table build is amortised
-    uint64_t wgs = (cols + 2 * kNttThreads - 1) / (2 * kNttThreads);
+    // up to ntt_cols_per_thread (2) columns per thread so the twiddle-table build is amortised
+    const uint64_t cpt = (uint64_t)tunable(kTuneNttColsPerThread);
+    uint64_t wgs = (cols + cpt * kNttThreads - 1) / (cpt * kNttThreads);
     if (wgs < 1) wgs = 1;
     const uint64_t grid = n_hb * wgs;
     if (grid > 0x7fffffffull) return hipErrorInvalidConfiguration;
@@ -306,7 +306,7 @@ hipError_t launch_ntt_forward(uint4* data, uint32_t t, uint32_t live, int k, int
                               uint64_t lm_d) {
     int ks[64];
     const int n = plan_passes(k - start_layer, ks);
-    if (lm_d && n > 0 && (1ull << (k - start_layer - ks[0])) * t <= kDeepCols) {
+    if (lm_d && n > 0 && (1ull << (k - start_layer - ks[0])) * t <= (uint64_t)tunable(kTuneNttDeepCols)) {
         // the first pass is a deep one: fill the replicated lane-major message, then transform
         // in place
         if (hipError_t e = launch_fill_lane_major(msg, data, lm_d, t, 1u << start_layer, s))

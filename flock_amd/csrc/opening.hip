@@ -1755,12 +1755,12 @@ struct PairSumcheck {
 // Runs the rounds of the (G(1), G(inf)) product sumcheck of <a, b> = claim over LSB pairs
 // from the round whose pair (G(0), G(inf)) is m2, on tables (ca, cb) of length cur, to the
 // end. Each round observes (claim + G(0), G(inf)), grinds `bits` (none when 0), samples its
-// challenge and updates the claim (jagged.rs:4592-4597). Tables longer than kLaMax entries
+// challenge and updates the claim (jagged.rs:4592-4597). Tables longer than 2^sc_la_max_log
+// (a launch tunable, 2^18 by default) entries
 // fold one round per launch (flock_hip_sumcheck_fold_msg); shorter ones two rounds per launch
 // (k_sc_fold_la: the second round's pair comes from the lookahead at the first's challenge),
 // which halves the host round trips of the small rounds. With final_fold the last challenges
 // are folded in too and *fin holds a's folded value. s1/s2 are ping-pong tables of cur / 2.
-constexpr uint64_t kLaMax = 1ull << 18;
 bool prod_rounds(Ctx& X, flock_fs* fs, const flock_f128* ca, const flock_f128* cb, uint64_t cur,
                  const flock_f128 m2[2], H& claim, uint32_t bits, bool final_fold, flock_f128* s1a,
                  flock_f128* s1b, flock_f128* s2a, flock_f128* s2b, std::vector<H>& msgs,
@@ -1809,7 +1809,7 @@ bool prod_rounds(Ctx& X, flock_fs* fs, const flock_f128* ca, const flock_f128* c
         }
         flock_f128* na = next();
         flock_f128* nb = na == s1a ? s1b : s2b;
-        if (cur > kLaMax) {  // one pending challenge here
+        if (cur > (1ull << flock::tunable(flock::kTuneScLaMaxLog))) {  // one pending challenge here
             flock_f128 m[2];
             if (int rc = flock_hip_sumcheck_fold_msg(X.c, ca, cb, cur, pend[0], na, nb, m))
                 return X.fail(rc, flock_hip_last_error());
@@ -1996,7 +1996,6 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     return X.hip(hipStreamSynchronize(X.st), "sync");
 }
 
-constexpr uint64_t kSplitFoldMsg = 1ull << 16;  // folded length from which fold and message split
 struct Ladder {
     Ctx& X;
     flock_fs* fs;
@@ -2048,7 +2047,7 @@ struct Ladder {
         const void* src_b = BASE ? bb : static_cast<const void*>(b);
         e256d* dst_f = BASE ? f : f2;
         e256d* dst_b = BASE ? b : b2;
-        if (nn >= kSplitFoldMsg) {
+        if (nn >= (1ull << flock::tunable(flock::kTuneOpenSplitFoldLog))) {  // fold and message split
             hipLaunchKernelGGL((flock::k_fold_msg<BASE, false>), dim3(X.grid(nn)), dim3(kThreads), 0, X.st, src_f,
                                src_b, n, d, E4(r), d2, dst_f, dst_b, X.A->acc);
             hipLaunchKernelGGL(flock::k_msg_e_comb, dim3(X.grid(nn / 2)), dim3(kThreads), kCombWsBytes, X.st,

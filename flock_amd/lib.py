@@ -226,6 +226,8 @@ def load():
         "flock_hip_f128_mul_batch": (I, [P, P, P, P, S]),
         "flock_hip_f128_mul_const_batch": (I, [P, P, F128, P, S]),
         "flock_hip_f128_mul_const_pow_batch": (I, [P, P, F128, U32, I, P, S]),
+        "flock_hip_set_tunable": (I, [ctypes.c_char_p, ctypes.c_int64]),
+        "flock_hip_get_tunable": (I, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
         "flock_hip_ntt_forward_interleaved": (I, [P, P, S, S, U32, U32]),
         "flock_hip_ntt_pass": (I, [P, P, S, S, U32, U32, U32]),
         "flock_hip_pcs_sizes": (I, [PP, P, P, P, P, P]),
@@ -1000,6 +1002,35 @@ class Prover:
             self.close()
         except Exception:
             pass
+
+
+TUNABLES = ("ntt_cols_per_thread", "ntt_deep_cols", "zc_split_min_vars", "zc_la_max_log",
+            "sc_la_max_log", "open_split_fold_log")
+
+
+def set_tunable(name: str, value: int):
+    """flock_hip_set_tunable (process-wide launch shape / threshold)."""
+    _check(load().flock_hip_set_tunable(name.encode(), int(value)))
+
+
+def get_tunable(name: str) -> int:
+    v = ctypes.c_int64()
+    _check(load().flock_hip_get_tunable(name.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def apply_tuning(path: str | None = None) -> dict:
+    """Apply the autotuner's choices (flock_amd/tuning.json by default) if the file exists;
+    returns the values applied ({} when there is no file)."""
+    import json
+    path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        vals = json.load(fh).get("values", {})
+    for k, v in vals.items():
+        set_tunable(k, v)
+    return vals
 
 
 def make_requests(specs, digest: bytes | None = None, domain: bytes = b"flock-readme-bench-v0"):

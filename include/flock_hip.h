@@ -72,6 +72,20 @@ int flock_hip_sync(flock_hip_ctx* ctx);
 #define FLOCK_ROUNDS_HOST 0
 #define FLOCK_ROUNDS_DEVICE 2
 int flock_hip_ctx_set_round_mode(flock_hip_ctx* ctx, int mode);
+
+/* Process-wide launch tunables: the tile shapes and thresholds `flock_amd/tune.py` (the
+ * autotuner) picks. Every value in range gives the same bytes; set them before proving (a
+ * launch reads them as it goes). Names, defaults and ranges:
+ *   ntt_cols_per_thread  2   [1, 4]   columns per thread of the interior NTT pass
+ *   ntt_deep_cols        256 [32, 256] passes with at most this many columns per block run deep
+ *   zc_split_min_vars    16  [10, 24] zerocheck rounds whose eq has this many variables split it
+ *   zc_la_max_log        17  [10, 22] zerocheck tail: two rounds per launch from 2^this words
+ *   sc_la_max_log        18  [10, 24] the opening's product sumchecks: the same, from 2^this
+ *   open_split_fold_log  16  [10, 22] F256 ladder: fold and message as two passes from 2^this
+ * No counterpart in the reference (its CUDA launch shapes are compile-time constants).
+ * Unknown names and values out of range are FLOCK_HIP_E_INVALID. */
+int flock_hip_set_tunable(const char* name, int64_t value);
+int flock_hip_get_tunable(const char* name, int64_t* value);
 int flock_hip_malloc(flock_hip_ctx* ctx, void** d_ptr, size_t bytes);
 int flock_hip_dfree(flock_hip_ctx* ctx, void* d_ptr);
 int flock_hip_memcpy_h2d(flock_hip_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);

@@ -9,6 +9,7 @@ import os
 import random
 
 import numpy as np
+import pytest
 
 import oracle_lib as O
 from flock_amd import lib as F
@@ -116,3 +117,22 @@ def test_f128_struct_layout():
     v = F.f128((1 << 127) | 5)
     assert F.to_int(v) == (1 << 127) | 5
     assert np.frombuffer(bytes(v), dtype=np.uint64).tolist() == [5, 1 << 63]
+
+
+def test_tunables_set_get_and_reject():
+    """flock_hip_set_tunable / _get_tunable: defaults, a value in range, and rejections (no GPU
+    needed: the registry is host state)."""
+    from flock_amd import lib as F
+
+    defaults = {k: F.get_tunable(k) for k in F.TUNABLES}
+    assert defaults == {"ntt_cols_per_thread": 2, "ntt_deep_cols": 256, "zc_split_min_vars": 16,
+                        "zc_la_max_log": 17, "sc_la_max_log": 18, "open_split_fold_log": 16}
+    try:
+        F.set_tunable("sc_la_max_log", 14)
+        assert F.get_tunable("sc_la_max_log") == 14
+        for name, bad in (("sc_la_max_log", 99), ("ntt_cols_per_thread", 0), ("no_such", 1)):
+            with pytest.raises(F.FlockHipError):
+                F.set_tunable(name, bad)
+    finally:
+        for k, v in defaults.items():
+            F.set_tunable(k, v)

@@ -97,3 +97,35 @@ def test_shapes_below_the_m22_configs_are_rejected(circuits):
     with pytest.raises(F.FlockHipError) as e:
         gpu_prove_fast(F.HASH_SHA256, inputs_for(F.HASH_SHA256, 16, 1), b"x", circuits)
     assert e.value.code == F.E_INVALID
+
+
+@pytest.mark.parametrize("setting", [
+    {"ntt_cols_per_thread": 1, "ntt_deep_cols": 32, "zc_split_min_vars": 10, "zc_la_max_log": 10,
+     "sc_la_max_log": 10, "open_split_fold_log": 10},
+    {"ntt_cols_per_thread": 4, "ntt_deep_cols": 64, "zc_split_min_vars": 24, "zc_la_max_log": 22,
+     "sc_la_max_log": 24, "open_split_fold_log": 22},
+])
+def test_tunables_keep_the_anchor_bytes(circuits, setting):
+    """The launch tunables (flock_hip_set_tunable) at the ends of their ranges change launch
+    shapes and thresholds only: the SHA-256 x 128 anchor and the oracle's 1024-compression
+    golden proof are unchanged."""
+    say(f"case: tunables {setting}")
+    defaults = {k: F.get_tunable(k) for k in F.TUNABLES}
+    try:
+        for k, v in setting.items():
+            F.set_tunable(k, v)
+        inputs = OP.fixture_sha2_inputs(OP.SplitMix32(0x4D36_5252), 128)
+        proof, cap, ab, c = gpu_prove_fast(F.HASH_SHA256, inputs, FIXTURE_DOMAIN, circuits,
+                                           builtin=True)
+        assert OP.anchor_digest(proof, cap, ab, c) == \
+            "d4686311e9887d778c00b7e1b93ddddabb3bf10db74ca9bfdac75f8db55e0b80"
+        path = os.path.join(os.path.dirname(__file__), "golden", "prove_fast_vectors.json")
+        with open(path) as fh:
+            gold = json.load(fh)
+        case = gold["cases"][0]  # SHA-256 x 1024 (dense m 25)
+        inputs = OP.fixture_sha2_inputs(OP.SplitMix32(case["seed"]), case["n_decl"])
+        proof, cap, ab, c = gpu_prove_fast(case["kind"], inputs, gold["domain"].encode(), circuits)
+        assert OP.anchor_digest(proof, cap, ab, c) == case["digest"]
+    finally:
+        for k, v in defaults.items():
+            F.set_tunable(k, v)
