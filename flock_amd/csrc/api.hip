@@ -14,6 +14,8 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <condition_variable>
+#include <thread>
 #include <new>
 #include <string>
 #include <vector>
@@ -374,6 +376,108 @@ void pow_block(const uint8_t digest[32], uint8_t blk[64]) {
     blk[63] = 0x40;
 }
 
+// Parallel search for the smallest passing nonce, for the grinds the first kPowSerialTries
+// nonces do not settle (waking the workers costs about as much as 500 tries). The caller and whichever pool workers join claim blocks of kPowBlock
+// nonces in increasing order and stop claiming once a block starts past the best nonce found;
+// every block below the best is searched up to its first hit, so the result is the smallest
+// passing nonce, the sequential search's. A worker joins a search only while it is open; the
+// caller closes it when its own share is done and waits for the workers that joined, never for
+// sleeping ones. One search uses the pool at a time (a caller that finds it busy searches
+// alone). Idle workers spin for a while after a search (grinds come in bursts), then sleep.
+constexpr uint64_t kPowBlock = 64;
+constexpr uint64_t kPowSerialTries = 512;  // a multiple of kPowBlock
+class PowPool {
+  public:
+    static PowPool& get() {
+        static PowPool p;
+        return p;
+    }
+    // false when the pool is busy or has no workers; *out = the smallest passing nonce >= start
+    // (a multiple of kPowBlock) otherwise
+    bool search(const uint8_t blk[64], uint32_t bits, uint64_t start, uint64_t* out) {
+        if (th_.empty() || !busy_.try_lock()) return false;
+        memcpy(blk_, blk, 64);
+        bits_ = bits;
+        next_.store(start / kPowBlock);
+        best_.store(UINT64_MAX);
+        const uint64_t g = gen_.load() + 1;
+        open_.store(g, std::memory_order_release);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            gen_.store(g, std::memory_order_release);
+        }
+        cv_.notify_all();
+        run();
+        open_.store(0, std::memory_order_release);  // no new joiners
+        while (inflight_.load(std::memory_order_acquire) != 0) _mm_pause();
+        *out = best_.load();
+        busy_.unlock();
+        return true;
+    }
+    ~PowPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_.store(true);
+        }
+        cv_.notify_all();
+        for (std::thread& t : th_) t.join();
+    }
+
+  private:
+    PowPool() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned n = hw > 1 ? std::min(hw - 1, 7u) : 0u;
+        try {
+            for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { worker(); });
+        } catch (...) {
+            // fewer (or no) workers: searches run with what started, or alone
+        }
+    }
+    void run() {
+        uint8_t b[64];
+        memcpy(b, blk_, 64);
+        for (;;) {
+            const uint64_t base = next_.fetch_add(1) * kPowBlock;
+            if (base >= best_.load()) break;
+            for (uint64_t n = base; n < base + kPowBlock; n++)
+                if (pow_ok(b, n, bits_)) {
+                    uint64_t cur = best_.load();
+                    while (n < cur && !best_.compare_exchange_weak(cur, n)) {
+                    }
+                    break;
+                }
+        }
+    }
+    void worker() {
+        uint64_t seen = 0;
+        for (;;) {
+            // spin about 500 us for the next search, then sleep
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load(std::memory_order_acquire) == seen && !stop_.load()) {
+                _mm_pause();
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(500)) {
+                    std::unique_lock<std::mutex> lk(m_);
+                    cv_.wait(lk, [&] { return gen_.load() != seen || stop_.load(); });
+                }
+            }
+            if (stop_.load()) return;
+            seen = gen_.load(std::memory_order_acquire);
+            inflight_.fetch_add(1, std::memory_order_acq_rel);
+            if (open_.load(std::memory_order_acquire) == seen) run();
+            inflight_.fetch_sub(1, std::memory_order_release);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex busy_, m_;
+    std::condition_variable cv_;
+    std::atomic<uint64_t> gen_{0}, open_{0};
+    std::atomic<int> inflight_{0};
+    std::atomic<bool> stop_{false};
+    uint8_t blk_[64] = {};
+    uint32_t bits_ = 0;
+    std::atomic<uint64_t> next_{0}, best_{UINT64_MAX};
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -506,7 +610,9 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
     if (bits) {
         uint8_t blk[64];
         pow_block(digest, blk);
-        while (!pow_ok(blk, nonce, bits)) nonce++;
+        while (nonce < kPowSerialTries && !pow_ok(blk, nonce, bits)) nonce++;
+        if (nonce == kPowSerialTries && !PowPool::get().search(blk, bits, nonce, &nonce))
+            while (!pow_ok(blk, nonce, bits)) nonce++;
     }
     uint8_t nb[8];
     for (int i = 0; i < 8; i++) nb[i] = (uint8_t)(nonce >> (8 * i));
