@@ -1386,6 +1386,23 @@ struct flock_open_ws {
         if (i >= 0) stage_ms[i] = std::chrono::duration<double, std::milli>(t - mark).count();
         mark = t;
     }
+    // pinned host staging of the inner opening's thread (grow-only): [0] the level openings'
+    // rows and paths coming back, [1] the induced basis's inputs going out
+    void* pin[2] = {nullptr, nullptr};
+    size_t pin_bytes[2] = {0, 0};
+    void* pinned(int i, size_t bytes) {
+        if (pin_bytes[i] >= bytes) return pin[i];
+        if (pin[i]) (void)hipHostFree(pin[i]);
+        pin[i] = nullptr;
+        pin_bytes[i] = 0;
+        if (hipHostMalloc(&pin[i], bytes, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            pin[i] = nullptr;
+            return nullptr;
+        }
+        pin_bytes[i] = bytes;
+        return pin[i];
+    }
     // the second context the multipoint block runs on beside the inner opening (lazily made on
     // the prover's device)
     flock_hip_ctx* aux = nullptr;
@@ -1886,7 +1903,6 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
                 std::vector<H>* rows_h, bool write_rows_first = true) {
     const uint32_t nq = (uint32_t)qs.size();
     uint64_t* d_idx = X.buf<uint64_t>(40, nq);
-    flock_f128* d_rows = X.buf<flock_f128>(41, (size_t)nq * row_len);
     const int d = __builtin_ctzll(block_len);
     const int depth = d - cap;
     std::vector<uint64_t> nodes;
@@ -1901,8 +1917,14 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
         }
     }
     uint64_t* d_nidx = X.buf<uint64_t>(42, nodes.size() + 1);
-    uint8_t* d_nodes = X.buf<uint8_t>(43, 32 * (nodes.size() + 1));
+    // rows, then the path nodes, in one buffer: one copy back
+    const size_t row_bytes = 16 * (size_t)nq * row_len, node_bytes = 32 * nodes.size();
+    uint8_t* d_out = X.buf<uint8_t>(43, row_bytes + node_bytes + 32);
+    uint8_t* h_out = static_cast<uint8_t*>(X.ws->pinned(0, row_bytes + node_bytes + 32));
     if (X.err) return false;
+    if (!h_out) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
+    flock_f128* d_rows = reinterpret_cast<flock_f128*>(d_out);
+    uint8_t* d_nodes = d_out + row_bytes;
     if (!X.h2d(d_idx, qs.data(), 8 * nq)) return false;
     hipLaunchKernelGGL(flock::k_gather_rows, dim3(X.grid((uint64_t)nq * row_len)), dim3(kThreads), 0,
                        X.st, reinterpret_cast<const uint4*>(mat), row_len, d_idx, nq,
@@ -1915,14 +1937,13 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
                            reinterpret_cast<uint4*>(d_nodes));
         if (!X.chk("gather nodes")) return false;
     }
-    rows_h->assign((size_t)nq * row_len, hz());
-    if (!X.d2h(rows_h->data(), d_rows, 16 * (size_t)nq * row_len)) return false;
-    std::vector<uint8_t> nb(32 * nodes.size());
-    if (!nodes.empty() && !X.d2h(nb.data(), d_nodes, nb.size())) return false;
+    if (!X.d2h(h_out, d_out, row_bytes + node_bytes)) return false;
+    rows_h->resize((size_t)nq * row_len);
+    std::memcpy(rows_h->data(), h_out, row_bytes);
     (void)write_rows_first;
     out.u64(nq);
     for (uint32_t i = 0; i < nq; i++) out.vec_f(rows_h->data() + (size_t)i * row_len, row_len);
-    out.vec_hash(nb.data(), nodes.size());
+    out.vec_hash(h_out + row_bytes, nodes.size());
     return true;
 }
 
@@ -1965,8 +1986,17 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     uint64_t* d_q = X.buf<uint64_t>(51, nq);
     flock_f128* d_c = X.buf<flock_f128>(52, consts.size());
     if (X.err) return false;
-    if (!X.h2d(d_q, qs.data(), 8 * nq) || !X.h2d(d_c, consts.data(), 16 * consts.size()) ||
-        !X.h2d(d_aw, aw.data(), 16 * nq))
+    // the inputs go out through pinned staging, so nothing waits for the copies: the staging is
+    // rewritten only by the next level's call, after that level's row copy-back has synchronised
+    // the stream
+    const size_t up_bytes = 8 * (size_t)nq + 16 * consts.size() + 16 * (size_t)nq;
+    uint8_t* up = static_cast<uint8_t*>(X.ws->pinned(1, up_bytes));
+    if (!up) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
+    std::memcpy(up, qs.data(), 8 * nq);
+    std::memcpy(up + 8 * nq, consts.data(), 16 * consts.size());
+    std::memcpy(up + 8 * nq + 16 * consts.size(), aw.data(), 16 * nq);
+    if (!X.h2d(d_q, up, 8 * nq) || !X.h2d(d_c, up + 8 * nq, 16 * consts.size()) ||
+        !X.h2d(d_aw, up + 8 * nq + 16 * consts.size(), 16 * nq))
         return false;
     hipLaunchKernelGGL(flock::k_induced_sk, dim3((nq + kThreads - 1) / kThreads), dim3(kThreads), 0, X.st, d_q, nq,
                        reinterpret_cast<const uint4*>(d_c), (uint32_t)dims, reinterpret_cast<uint4*>(d_s));
@@ -1991,9 +2021,7 @@ bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std:
     if (gz > 1)
         hipLaunchKernelGGL(flock::k_xor_chunks, dim3(X.grid(1ull << dims)), dim3(kThreads), 0, X.st,
                            reinterpret_cast<const uint4*>(part), gz, 1ull << dims, reinterpret_cast<uint4*>(out));
-    if (!X.chk("induced basis")) return false;
-    // the host copies must outlive the async uploads
-    return X.hip(hipStreamSynchronize(X.st), "sync");
+    return X.chk("induced basis");
 }
 
 struct Ladder {
@@ -2407,6 +2435,8 @@ void flock_open_ws_destroy(flock_open_ws* w) {
         if (a.acc) (void)hipFree(a.acc);
         if (a.hacc) (void)hipHostFree(a.hacc);
     }
+    for (void* p : w->pin)
+        if (p) (void)hipHostFree(p);
     delete w;
 }
 
