@@ -1387,9 +1387,10 @@ struct flock_open_ws {
         mark = t;
     }
     // pinned host staging of the inner opening's thread (grow-only): [0] the level openings'
-    // rows and paths coming back, [1] the induced basis's inputs going out
-    void* pin[2] = {nullptr, nullptr};
-    size_t pin_bytes[2] = {0, 0};
+    // rows and paths coming back, [1] the induced basis's inputs going out, [2] the openings'
+    // row and node indices going out
+    void* pin[3] = {nullptr, nullptr, nullptr};
+    size_t pin_bytes[3] = {0, 0, 0};
     void* pinned(int i, size_t bytes) {
         if (pin_bytes[i] >= bytes) return pin[i];
         if (pin[i]) (void)hipHostFree(pin[i]);
@@ -1921,17 +1922,22 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
     const size_t row_bytes = 16 * (size_t)nq * row_len, node_bytes = 32 * nodes.size();
     uint8_t* d_out = X.buf<uint8_t>(43, row_bytes + node_bytes + 32);
     uint8_t* h_out = static_cast<uint8_t*>(X.ws->pinned(0, row_bytes + node_bytes + 32));
+    // the indices go out through pinned staging (rewritten only by the next level's call, after
+    // this call's copy-back has synchronised the stream)
+    uint64_t* h_idx = static_cast<uint64_t*>(X.ws->pinned(2, 8 * (nq + nodes.size())));
     if (X.err) return false;
-    if (!h_out) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
+    if (!h_out || !h_idx) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
     flock_f128* d_rows = reinterpret_cast<flock_f128*>(d_out);
     uint8_t* d_nodes = d_out + row_bytes;
-    if (!X.h2d(d_idx, qs.data(), 8 * nq)) return false;
+    std::memcpy(h_idx, qs.data(), 8 * nq);
+    std::memcpy(h_idx + nq, nodes.data(), 8 * nodes.size());
+    if (!X.h2d(d_idx, h_idx, 8 * nq)) return false;
     hipLaunchKernelGGL(flock::k_gather_rows, dim3(X.grid((uint64_t)nq * row_len)), dim3(kThreads), 0,
                        X.st, reinterpret_cast<const uint4*>(mat), row_len, d_idx, nq,
                        reinterpret_cast<uint4*>(d_rows));
     if (!X.chk("gather rows")) return false;
     if (!nodes.empty()) {
-        if (!X.h2d(d_nidx, nodes.data(), 8 * nodes.size())) return false;
+        if (!X.h2d(d_nidx, h_idx + nq, 8 * nodes.size())) return false;
         hipLaunchKernelGGL(flock::k_gather_nodes, dim3(X.grid(2 * nodes.size())), dim3(kThreads), 0,
                            X.st, reinterpret_cast<const uint4*>(tree), d_nidx, (uint32_t)nodes.size(),
                            reinterpret_cast<uint4*>(d_nodes));
