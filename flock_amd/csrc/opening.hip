@@ -25,6 +25,7 @@
 #include <new>
 #include <string>
 #include <thread>
+#include <condition_variable>
 #include <vector>
 
 #if defined(__x86_64__)
@@ -2129,9 +2130,29 @@ struct StepTrace {
     }
 };
 
+// A one-shot gate between the opening's two host threads.
+struct Gate {
+    std::mutex m;
+    std::condition_variable cv;
+    bool open = false;
+    void release() {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            open = true;
+        }
+        cv.notify_all();
+    }
+    void wait() {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return open; });
+    }
+};
+
+// `lanes_done` (optional) is released once the lane folds (the opening's large, GPU-bound
+// passes) are enqueued and their last message has come back.
 bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_params* P,
                 const flock_f128* cw0, const uint8_t* tree0, const std::vector<H>& rho, H q_eval,
-                Bin& out) {
+                Bin& out, Gate* lanes_done = nullptr) {
     const Lig* C = lig_for(P->m);
     if (!C || C->initial_k != P->log_batch_size)
         return X.fail(FLOCK_HIP_E_INVALID, "no Ligerito config for this m / log_batch_size");
@@ -2227,6 +2248,7 @@ bool inner_open(Ctx& X, flock_fs* fs, const flock_f128* q, const flock_pcs_param
     // the split table's buffers must hold the doubled length: the first fold wrote N/2, the
     // switch's output is 2 * 2^n1 <= N/2 for ik >= 2
     T.mark("lane folds");
+    if (lanes_done) lanes_done->release();
     int split_dim = n1 + 1;
     flock_f128* base = X.buf<flock_f128>(16, 1ull << split_dim);
     if (X.err) return false;
@@ -2767,8 +2789,13 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     Ctx X2{actx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(actx)), &ws->accs[1]};
     bool ok_mp = false;
     std::thread th;
+    // The multipoint block starts once the inner opening's lane folds are through: those are
+    // large GPU-bound passes, while the rest of the inner opening (its levels) is mostly host
+    // round trips that leave the GPU idle, which the multipoint block's kernels then fill.
+    Gate lanes_done;
     try {
         th = std::thread([&] {
+            lanes_done.wait();
             if (X2.hip(hipSetDevice(flock::ctx_device(actx)), "assist device"))
                 ok_mp = X2.clear_acc() && multipoint(X2);
         });
@@ -2776,7 +2803,8 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
         ok_mp = X2.clear_acc() && multipoint(X2);  // no thread: one after the other
     }
     Bin inner;
-    ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner);
+    ok = inner_open(X, fs, d_q, P, d_codeword, d_tree, rho, q_eval, inner, th.joinable() ? &lanes_done : nullptr);
+    lanes_done.release();  // also when the inner opening stopped early
     ws->stamp(4);
     if (th.joinable()) th.join();
     ws->stage_ms[2] = t_mp;
