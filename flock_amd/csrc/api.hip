@@ -386,6 +386,9 @@ void pow_block(const uint8_t digest[32], uint8_t blk[64]) {
 // alone). Idle workers spin for a while after a search (grinds come in bursts), then sleep.
 constexpr uint64_t kPowBlock = 64;
 constexpr uint64_t kPowSerialTries = 512;  // a multiple of kPowBlock
+// Proofs in flight in this process (flock::ProofScope); with more than one, their own threads
+// keep the host cores busy and every grind searches alone.
+std::atomic<int> g_proofs_in_flight{0};
 class PowPool {
   public:
     static PowPool& get() {
@@ -611,7 +614,9 @@ uint64_t flock_fs_grind_pow(flock_fs* fs, uint32_t bits) {
         uint8_t blk[64];
         pow_block(digest, blk);
         while (nonce < kPowSerialTries && !pow_ok(blk, nonce, bits)) nonce++;
-        if (nonce == kPowSerialTries && !PowPool::get().search(blk, bits, nonce, &nonce))
+        if (nonce == kPowSerialTries &&
+            (g_proofs_in_flight.load(std::memory_order_relaxed) > 1 ||
+             !PowPool::get().search(blk, bits, nonce, &nonce)))
             while (!pow_ok(blk, nonce, bits)) nonce++;
     }
     uint8_t nb[8];
@@ -1057,6 +1062,9 @@ TuneEntry g_tune[flock::kTuneCount] = {
 }  // namespace
 
 int64_t flock::tunable(flock::Tunable t) { return g_tune[t].v.load(std::memory_order_relaxed); }
+
+flock::ProofScope::ProofScope() { g_proofs_in_flight.fetch_add(1); }
+flock::ProofScope::~ProofScope() { g_proofs_in_flight.fetch_sub(1); }
 
 extern "C" {
 

@@ -88,6 +88,15 @@ enum Tunable : int {
 };
 int64_t tunable(Tunable t);
 
+// Marks a proof in flight for its lifetime (the host grinds use their worker pool only while a
+// single proof runs in the process).
+struct ProofScope {
+    ProofScope();
+    ~ProofScope();
+    ProofScope(const ProofScope&) = delete;
+    ProofScope& operator=(const ProofScope&) = delete;
+};
+
 // Where a grid reduction of several words goes: the workgroups fold into acc (the words, then
 // a ticket counter; zero between launches), and the last one moves the sums to out (host-
 // visible), clears acc and stores seq into flag (system scope).

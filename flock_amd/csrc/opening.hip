@@ -2510,6 +2510,7 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
                           size_t n_claims, const flock_f128* h_x_outers, size_t x_len,
                           const flock_f128* h_s_hat_v, const uint64_t* h_heights, size_t n_heights,
                           uint32_t n_log, uint8_t** out_bytes, size_t* out_len, flock_f128* h_q_eval) {
+    const flock::ProofScope in_flight;
     if (!ctx || !ws || !fs || !d_q || !P || !d_codeword || !d_tree || !h_x_outers || !h_s_hat_v ||
         !h_heights || !out_bytes || !out_len || n_claims < 1 || n_claims > flock::kMaxClaims ||
         n_heights == 0 || (n_heights & (n_heights - 1)) || P->m < 14 || P->m > 40)
@@ -2789,10 +2790,12 @@ int flock_hip_open_merged(flock_hip_ctx* ctx, flock_open_ws* ws, flock_fs* fs, c
     Ctx X2{actx, ws, static_cast<hipStream_t>(flock_hip_ctx_stream(actx)), &ws->accs[1]};
     bool ok_mp = false;
     std::thread th;
-    // The multipoint block starts once the inner opening's lane folds are through: those are
-    // large GPU-bound passes, while the rest of the inner opening (its levels) is mostly host
-    // round trips that leave the GPU idle, which the multipoint block's kernels then fill.
+    // On large stacks the multipoint block starts once the inner opening's lane folds are
+    // through: those are large GPU-bound passes, while the rest of the inner opening (its levels)
+    // is mostly host round trips that leave the GPU idle, which the multipoint block's kernels
+    // then fill. Small stacks are latency-bound throughout, and both start at once.
     Gate lanes_done;
+    if (N < (1ull << 20)) lanes_done.release();
     try {
         th = std::thread([&] {
             lanes_done.wait();
