@@ -26,6 +26,7 @@
 #include <string>
 #include <thread>
 #include <condition_variable>
+#include <mutex>
 #include <vector>
 
 #if defined(__x86_64__)
@@ -1053,70 +1054,63 @@ __global__ void __launch_bounds__(kThreads) k_dot_comb(const f128* a, const f128
     block_xor_atomic<4>(v, acc);
 }
 
-// s[i][k] = s_k(q_i) inv_k with s_0(x) = x, s_k(x) = s_{k-1}(x)^2 + s_{k-1}(v_{k-1}) s_{k-1}(x)
-// (eval_sk_at_vks, ligerito.rs:2402-2420); c = [s_k(v_k), k <= dims | inv_k, k <= dims]
-// squaring through its nibble tables (x -> x^2 is F2-linear), the two per-step constants
-// through Shoup tables rebuilt per step
-__global__ void __launch_bounds__(kThreads) k_induced_sk(const uint64_t* qs, uint32_t nq, const f128* c,
-                                                         uint32_t dims, f128* s) {
-    __shared__ f128 sq[512];
-    __shared__ f128 tab[2 * SHOUP_ENTRIES];
-    __shared__ f128 img[128];
-    // (x^b)^2 = x^(2b); shifts stay below 128
-    for (uint32_t b = threadIdx.x; b < 128; b += blockDim.x)
-        img[b] = 2 * b < 128 ? f128_shl_mod(f128_one(), (int)(2 * b))
-                             : f128_shl_mod(f128_shl_mod(f128_one(), 127), (int)(2 * b - 127));
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < 512; e += blockDim.x) {
-        const uint32_t k = e / 16, nib = e % 16;
-        f128 v = f128_zero();
-        for (int j = 0; j < 4; j++)
-            if ((nib >> j) & 1) v = f128_add(v, img[4 * k + j]);
-        sq[e] = v;
-    }
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < nq;
-    f128 cur = live ? make_uint4((uint32_t)qs[i], (uint32_t)(qs[i] >> 32), 0u, 0u) : f128_zero();
-    f128* si = s + (size_t)(live ? i : 0) * dims;
-    for (uint32_t k = 0; k < dims; k++) {
-        __syncthreads();  // the previous step's tables are consumed
-        if (threadIdx.x < 64)
-            shoup_build_row(tab + (threadIdx.x >> 5) * SHOUP_ENTRIES,
-                            threadIdx.x < 32 ? (k ? c[k - 1] : f128_zero()) : c[dims + 1 + k], threadIdx.x & 31);
-        __syncthreads();
-        if (k) cur = f128_add(lin_apply_d(sq, cur), shoup_mul(tab, cur));
-        if (live) si[k] = shoup_mul(tab + SHOUP_ENTRIES, cur);
-    }
-}
-
 // Induced basis (ligerito.rs:2676-2770 with empty rows) as out = sum_i lo_i (x) hi_i:
 // lo_i[j] = aw_i prod_{k < h, bit k of j} s_ik over the low h coordinates, hi_i[j] =
-// prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i). Both are built by
-// doubling in LDS, level k doubling the table by its factor s_ik (one product per entry).
-__global__ void __launch_bounds__(kThreads) k_induced_tables(const f128* s, const f128* aw, uint32_t dims,
-                                                             uint32_t h, f128* lo, f128* hi) {
-    extern __shared__ f128 tt[];  // lo (2^h entries), then hi (2^(dims - h))
+// prod_{k >= h, bit k - h of j} s_ik over the rest (one block per query i), with
+// s_ik = s_k(q_i) / s_k(v_k) (eval_sk_at_vks, ligerito.rs:2402-2420; zero where s_k(v_k) = 0).
+// The subspace polynomials s_k are F2-linear, so s_ik is the XOR of row k of skb
+// (skb[k * 64 + b] = s_k(x^b) / s_k(v_k)) over the set bits b of q_i: eight lanes per k, one
+// byte of q_i each, joined by shuffles. Both tables are then built by doubling in LDS, level k
+// doubling the table by its factor s_ik (one product per entry, through the level's Shoup table:
+// every product of a level shares its multiplier).
+constexpr int kSkRows = 64;  // rows of skb: dims <= kSkRows
+__global__ void __launch_bounds__(kThreads) k_induced_tables(const uint64_t* qs, const f128* skb,
+                                                             const f128* aw, uint32_t dims, uint32_t h,
+                                                             f128* lo, f128* hi) {
+    __shared__ f128 tab[2 * SHOUP_ENTRIES];  // the level's lo and hi multipliers
+    __shared__ f128 si[kSkRows];
+    extern __shared__ f128 tt[];             // lo (2^h entries), then hi (2^(dims - h))
     const uint32_t i = blockIdx.x;
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
-    const f128* si = s + (size_t)i * dims;
+    const uint64_t q = qs[i];
+    // groups of eight lanes are whole (blockDim is a multiple of 8): all active or none
+    for (uint32_t e = threadIdx.x; e < 8 * dims; e += blockDim.x) {
+        const uint32_t k = e >> 3, p = e & 7, byte = (uint32_t)(q >> (8 * p)) & 0xffu;
+        const f128* row = skb + (size_t)k * 64 + 8 * p;
+        f128 v = f128_zero();
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((byte >> j) & 1u) v = f128_add(v, row[j]);
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1)
+            v = make_uint4(v.x ^ __shfl_xor(v.x, m, 8), v.y ^ __shfl_xor(v.y, m, 8),
+                           v.z ^ __shfl_xor(v.z, m, 8), v.w ^ __shfl_xor(v.w, m, 8));
+        if (p == 0) si[k] = v;
+    }
     f128* tl = tt;
     f128* th = tt + nl;
     if (threadIdx.x == 0) {
         tl[0] = aw[i];
         th[0] = f128_one();
     }
-    __syncthreads();
+    __syncthreads();  // si is written
     const uint32_t levels = h > dims - h ? h : dims - h;
     for (uint32_t k = 0; k < levels; k++) {
+        if (threadIdx.x < 64) {
+            const uint32_t t = threadIdx.x >> 5;
+            if (t ? k < dims - h : k < h)
+                shoup_build_row(tab + t * SHOUP_ENTRIES, si[t ? h + k : k], threadIdx.x & 31);
+        }
+        __syncthreads();  // the tables and the previous level's entries are written
         const uint64_t len = 1ull << k;
         for (uint64_t j = threadIdx.x; j < 2 * len; j += blockDim.x) {
             if (j < len) {
-                if (k < h) tl[len + j] = f128_mul(tl[j], si[k]);
+                if (k < h) tl[len + j] = shoup_mul(tab, tl[j]);
             } else if (k < dims - h) {
-                th[j] = f128_mul(th[j - len], si[h + k]);
+                th[j] = shoup_mul(tab + SHOUP_ENTRIES, th[j - len]);
             }
         }
-        __syncthreads();
+        __syncthreads();  // the tables are consumed
     }
     for (uint64_t j = threadIdx.x; j < nl; j += blockDim.x) lo[i * nl + j] = tl[j];
     for (uint64_t j = threadIdx.x; j < nh; j += blockDim.x) hi[i * nh + j] = th[j];
@@ -1347,6 +1341,7 @@ __global__ void __launch_bounds__(kThreads) k_xor_chunks(const f128* part, uint3
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         f128 v = part[i];
+#pragma unroll 8
         for (uint32_t z = 1; z < gz; z++) v = f128_add(v, part[z * n + i]);
         out[i] = v;
     }
@@ -1390,6 +1385,9 @@ struct flock_open_ws {
     // pinned host staging of the inner opening's thread (grow-only): [0] the level openings'
     // rows and paths coming back, [1] the induced basis's inputs going out, [2] the openings'
     // row and node indices going out
+    // the induced basis's table of subspace polynomials (slot 54), built once
+    std::mutex sk_mu;
+    bool sk_ready = false;
     void* pin[3] = {nullptr, nullptr, nullptr};
     size_t pin_bytes[3] = {0, 0, 0};
     void* pinned(int i, size_t bytes) {
@@ -1955,69 +1953,84 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
 }
 
 // induced basis over 2^dims coefficients (ligerito.rs:2676-2770 with empty rows)
-bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std::vector<H>& alpha,
-                   flock_f128* out) {
-    // eval_sk_at_vks (ligerito.rs:2402-2420)
-    std::vector<H> skv(dims + 1), layer(dims > 0 ? dims : 1);
+// skb[k * 64 + b] = s_k(x^b) / s_k(v_k) for k < kSkRows, b < 64 (zero where s_k(v_k) = 0), with
+// s_0(x) = x, s_k(x) = s_{k-1}(x)^2 + s_{k-1}(v_{k-1}) s_{k-1}(x), v_k = x^k (eval_sk_at_vks,
+// ligerito.rs:2402-2420). It depends on neither the level nor the proof: a workspace builds it
+// once (slot 54).
+const flock_f128* sk_basis(Ctx& X) {
+    std::lock_guard<std::mutex> lk(X.ws->sk_mu);
+    flock_f128* d = X.buf<flock_f128>(54, (size_t)flock::kSkRows * 64);
+    if (!d || X.ws->sk_ready) return d;
+    constexpr int K = flock::kSkRows;
+    // s_k(v_k) by the reference's layered recursion
+    std::vector<H> skv(K + 1), layer(K);
     skv[0] = h1();
-    for (int i = 1; i <= dims; i++) layer[i - 1] = hbit(i);
-    for (int i = 0, cur = dims; i < dims; i++, cur--)
+    for (int i = 1; i <= K; i++) layer[i - 1] = hbit(i);
+    for (int i = 0, cur = K; i < K; i++, cur--)
         for (int j = 0; j < cur; j++) {
-            const H s = layer[j], nx = ha(hm(s, s), hm(skv[i], s));
+            const H sv = layer[j], nx = ha(hm(sv, sv), hm(skv[i], sv));
             if (j == 0)
                 skv[i + 1] = nx;
             else
                 layer[j - 1] = nx;
         }
-    // the inverses of the nonzero s_k(v_k) with one field inversion (batched: prefix products)
-    std::vector<H> inv(dims + 1, hz()), pre(dims + 2, h1());
-    for (int i = 0; i <= dims; i++) pre[i + 1] = hzero(skv[i]) ? pre[i] : hm(pre[i], skv[i]);
-    H acc = hinv(pre[dims + 1]);
-    for (int i = dims; i >= 0; i--) {
+    // their inverses with one field inversion (prefix products)
+    std::vector<H> inv(K, hz()), pre(K + 1, h1());
+    for (int i = 0; i < K; i++) pre[i + 1] = hzero(skv[i]) ? pre[i] : hm(pre[i], skv[i]);
+    H acc = hinv(pre[K]);
+    for (int i = K - 1; i >= 0; i--) {
         if (hzero(skv[i])) continue;
         inv[i] = hm(acc, pre[i]);
         acc = hm(acc, skv[i]);
     }
+    std::vector<H> tab((size_t)K * 64);
+    for (int b = 0; b < 64; b++) {
+        H cur = hbit(b);
+        for (int k = 0; k < K; k++) {
+            tab[(size_t)k * 64 + b] = hm(cur, inv[k]);
+            cur = ha(hm(cur, cur), hm(skv[k], cur));
+        }
+    }
+    if (!X.h2d(d, tab.data(), 16 * tab.size()) || !X.hip(hipStreamSynchronize(X.st), "sync")) return nullptr;
+    X.ws->sk_ready = true;
+    return d;
+}
+
+bool induced_basis(Ctx& X, int dims, const std::vector<uint64_t>& qs, const std::vector<H>& alpha,
+                   flock_f128* out) {
+    if (dims < 0 || dims > flock::kSkRows) return X.fail(FLOCK_HIP_E_INVALID, "induced basis: dims");
+    const flock_f128* d_skb = sk_basis(X);
+    if (!d_skb) return X.err ? false : X.fail(FLOCK_HIP_E_NOMEM, "induced basis table");
     const std::vector<H> aw = heq(alpha.data(), alpha.size());
     const uint32_t nq = (uint32_t)qs.size();
     const int h = dims / 2;
     const uint64_t nl = 1ull << h, nh = 1ull << (dims - h);
-    // per-query subspace polynomials (device): s_ik = s_k(q_i) / s_k(v_k)
-    std::vector<H> consts(2 * (size_t)(dims + 1));
-    std::copy(skv.begin(), skv.end(), consts.begin());
-    std::copy(inv.begin(), inv.end(), consts.begin() + dims + 1);
-    flock_f128* d_s = X.buf<flock_f128>(48, (size_t)nq * (dims > 0 ? dims : 1));
     flock_f128* d_aw = X.buf<flock_f128>(49, nq);
     flock_f128* d_lo = X.buf<flock_f128>(44, (size_t)nq * nl);
     flock_f128* d_hi = X.buf<flock_f128>(45, (size_t)nq * nh);
     uint64_t* d_q = X.buf<uint64_t>(51, nq);
-    flock_f128* d_c = X.buf<flock_f128>(52, consts.size());
     if (X.err) return false;
     // the inputs go out through pinned staging, so nothing waits for the copies: the staging is
     // rewritten only by the next level's call, after that level's row copy-back has synchronised
     // the stream
-    const size_t up_bytes = 8 * (size_t)nq + 16 * consts.size() + 16 * (size_t)nq;
+    const size_t up_bytes = 8 * (size_t)nq + 16 * (size_t)nq;
     uint8_t* up = static_cast<uint8_t*>(X.ws->pinned(1, up_bytes));
     if (!up) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
     std::memcpy(up, qs.data(), 8 * nq);
-    std::memcpy(up + 8 * nq, consts.data(), 16 * consts.size());
-    std::memcpy(up + 8 * nq + 16 * consts.size(), aw.data(), 16 * nq);
-    if (!X.h2d(d_q, up, 8 * nq) || !X.h2d(d_c, up + 8 * nq, 16 * consts.size()) ||
-        !X.h2d(d_aw, up + 8 * nq + 16 * consts.size(), 16 * nq))
-        return false;
-    hipLaunchKernelGGL(flock::k_induced_sk, dim3((nq + kThreads - 1) / kThreads), dim3(kThreads), 0, X.st, d_q, nq,
-                       reinterpret_cast<const uint4*>(d_c), (uint32_t)dims, reinterpret_cast<uint4*>(d_s));
-    hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 16 * (nl + nh), X.st, reinterpret_cast<const uint4*>(d_s),
-                       reinterpret_cast<const uint4*>(d_aw), (uint32_t)dims, (uint32_t)h, reinterpret_cast<uint4*>(d_lo),
+    std::memcpy(up + 8 * nq, aw.data(), 16 * nq);
+    if (!X.h2d(d_q, up, 8 * nq) || !X.h2d(d_aw, up + 8 * nq, 16 * nq)) return false;
+    hipLaunchKernelGGL(flock::k_induced_tables, dim3(nq), dim3(kThreads), 16 * (nl + nh), X.st, d_q,
+                       reinterpret_cast<const uint4*>(d_skb), reinterpret_cast<const uint4*>(d_aw),
+                       (uint32_t)dims, (uint32_t)h, reinterpret_cast<uint4*>(d_lo),
                        reinterpret_cast<uint4*>(d_hi));
     if (!X.chk("induced tables")) return false;
-    // split the query sum over blocks until the grid holds about 2048 workgroups
+    // jl rows in blocks of kIndTL, jh columns in blocks of kThreads * kIndTH
     const uint32_t gx = (uint32_t)((nl + flock::kIndTL - 1) / flock::kIndTL);
     const uint32_t gy = (uint32_t)((nh + kThreads * flock::kIndTH - 1) / (kThreads * flock::kIndTH));
     // split the query sum into chunks until the grid holds about 1024 workgroups, keeping at
-    // least 16 queries per chunk so the per-query table builds amortize
+    // least 4 queries per chunk (the chunk partials cost a write and a read of 2^dims words each)
     uint32_t gz = std::max<uint32_t>(
-        1, std::min<uint32_t>((1024 + gx * gy - 1) / std::max<uint32_t>(1, gx * gy), nq / 16));
+        1, std::min<uint32_t>((1024 + gx * gy - 1) / std::max<uint32_t>(1, gx * gy), nq / 4));
     const uint32_t chunk = (nq + gz - 1) / gz;
     gz = (nq + chunk - 1) / chunk;
     flock_f128* part = gz > 1 ? X.buf<flock_f128>(53, (size_t)gz << dims) : out;
