@@ -1383,7 +1383,7 @@ struct flock_open_ws {
         mark = t;
     }
     // pinned host staging of the inner opening's thread (grow-only): [0] the level openings'
-    // rows and paths coming back, [1] the induced basis's inputs going out, [2] the openings'
+    // rows and paths, written there by the gather kernels, [1] the induced basis's inputs going out, [2] the openings'
     // row and node indices going out
     // the induced basis's table of subspace polynomials (slot 54), built once
     std::mutex sk_mu;
@@ -1395,7 +1395,8 @@ struct flock_open_ws {
         if (pin[i]) (void)hipHostFree(pin[i]);
         pin[i] = nullptr;
         pin_bytes[i] = 0;
-        if (hipHostMalloc(&pin[i], bytes, 0) != hipSuccess) {
+        // [0] is written by kernels directly (coherent: visible to the host once they complete)
+        if (hipHostMalloc(&pin[i], bytes, i == 0 ? hipHostMallocCoherent : 0) != hipSuccess) {
             (void)hipGetLastError();
             pin[i] = nullptr;
             return nullptr;
@@ -1917,17 +1918,19 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
         }
     }
     uint64_t* d_nidx = X.buf<uint64_t>(42, nodes.size() + 1);
-    // rows, then the path nodes, in one buffer: one copy back
+    // rows, then the path nodes, written by the gather kernels straight into coherent pinned host
+    // memory: the host waits for the kernels, not for a copy behind them
     const size_t row_bytes = 16 * (size_t)nq * row_len, node_bytes = 32 * nodes.size();
-    uint8_t* d_out = X.buf<uint8_t>(43, row_bytes + node_bytes + 32);
     uint8_t* h_out = static_cast<uint8_t*>(X.ws->pinned(0, row_bytes + node_bytes + 32));
     // the indices go out through pinned staging (rewritten only by the next level's call, after
-    // this call's copy-back has synchronised the stream)
+    // this call's gathers have synchronised the stream)
     uint64_t* h_idx = static_cast<uint64_t*>(X.ws->pinned(2, 8 * (nq + nodes.size())));
     if (X.err) return false;
     if (!h_out || !h_idx) return X.fail(FLOCK_HIP_E_NOMEM, "pinned staging");
-    flock_f128* d_rows = reinterpret_cast<flock_f128*>(d_out);
-    uint8_t* d_nodes = d_out + row_bytes;
+    void* d_out = nullptr;
+    if (!X.hip(hipHostGetDevicePointer(&d_out, h_out, 0), "pinned device pointer")) return false;
+    flock_f128* d_rows = static_cast<flock_f128*>(d_out);
+    uint8_t* d_nodes = static_cast<uint8_t*>(d_out) + row_bytes;
     std::memcpy(h_idx, qs.data(), 8 * nq);
     std::memcpy(h_idx + nq, nodes.data(), 8 * nodes.size());
     if (!X.h2d(d_idx, h_idx, 8 * nq)) return false;
@@ -1942,7 +1945,7 @@ bool open_level(Ctx& X, const flock_f128* mat, uint64_t row_len, const uint8_t* 
                            reinterpret_cast<uint4*>(d_nodes));
         if (!X.chk("gather nodes")) return false;
     }
-    if (!X.d2h(h_out, d_out, row_bytes + node_bytes)) return false;
+    if (!X.hip(hipStreamSynchronize(X.st), "sync")) return false;
     rows_h->resize((size_t)nq * row_len);
     std::memcpy(rows_h->data(), h_out, row_bytes);
     (void)write_rows_first;
