@@ -789,6 +789,8 @@ struct flock_hip_ctx {
     flock::DevTranscript* dfs = nullptr;
     uint8_t* hfs = nullptr;
     int round_mode = FLOCK_ROUNDS_HOST;
+    // commits build their trees up to the cap layer only (flock::ctx_set_tree_to_cap)
+    bool tree_to_cap = false;
     static constexpr size_t fs_bytes() {
         return sizeof(flock::DevTranscript) + kFsLogRounds * sizeof(flock::DevRound);
     }
@@ -1406,7 +1408,8 @@ static int commit_common(flock_hip_ctx* c, const flock_f128* src, size_t src_len
         (t * 16 + flock::kB3ChunkLen - 1) / flock::kB3ChunkLen > (1ull << flock::kMerkleB3MaxDepth))
         return fail(FLOCK_HIP_E_INVALID, "BLAKE3 leaf too long");
     LAUNCH(flock::launch_merkle_tree(reinterpret_cast<const uint8_t*>(cw), n_leaves, t * 16,
-                                     p->merkle_hash, tree, c->stream));
+                                     p->merkle_hash, tree, c->stream,
+                                     c->tree_to_cap ? 1ull << cap_depth : 1ull));
     if (h_cap) {
         const size_t l = (size_t)1 << cap_depth;  // cap_layer, merkle.rs:649-656
         HIP_TRY(hipMemcpyAsync(h_cap, tree + (2 * n_leaves - 2 * l) * 32, l * 32,
@@ -1481,6 +1484,10 @@ int build_eq_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f12
     LAUNCH(flock::launch_build_eq(dr, (int)nv, U(out), dr + nv, c->stream));
     return 0;
 }
+void ctx_set_tree_to_cap(flock_hip_ctx* c, bool on) {
+    if (c) c->tree_to_cap = on;
+}
+
 int build_eq_halves_async(flock_hip_ctx* c, const flock_f128* h_r, size_t nv, flock_f128* lo,
                           flock_f128* hi) {
     CTX_CHECK(c);

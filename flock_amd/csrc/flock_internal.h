@@ -52,8 +52,13 @@ hipError_t launch_ntt_pass(uint4* data, uint32_t t, uint32_t live, int k, int l0
 constexpr uint32_t kMerkleSha256 = 0, kMerkleBlake3 = 1;
 // BLAKE3 leaves of up to 2^16 chunks (64 MiB): the per-thread subtree stack's depth
 constexpr int kMerkleB3MaxDepth = 16;
+// top_nodes (a power of two): the build may stop once a layer has this many nodes (the cap
+// layer); the layers above it are then left unwritten. 1 builds the whole tree.
 hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
-                              uint32_t kind, uint8_t* tree, hipStream_t s);
+                              uint32_t kind, uint8_t* tree, hipStream_t s, uint64_t top_nodes = 1);
+// A context whose commits build their trees up to the cap layer only (the opening's level
+// commits, whose paths end at the cap). Off by default.
+void ctx_set_tree_to_cap(flock_hip_ctx* c, bool on);
 
 // ---- sumcheck.hip ----
 // round_msg_lsb: out[0] = sum f0 b0, out[1] = sum (f0+f1)(b0+b1). partials >= kMaxReduceBlocks*2 + 1 words (the last holds a zeroed ticket counter).

@@ -236,16 +236,18 @@ __global__ __launch_bounds__(256) void k_merkle_reduce(uint8_t* __restrict__ tre
 
 template <class H>
 static hipError_t merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
-                              uint8_t* tree, hipStream_t s) {
+                              uint8_t* tree, uint64_t top, hipStream_t s) {
     const uint64_t width = n_leaves < 256 ? n_leaves : 256;
     k_merkle_leaves<H><<<(unsigned)(n_leaves / width), (unsigned)width, 0, s>>>(data, n_leaves,
                                                                                leaf_bytes, tree);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     uint64_t level = n_leaves / width;  // nodes at the current top
-    while (level > 1) {
+    while (level > top) {
         const uint64_t parents = level / 2;
-        const uint64_t w = parents < 256 ? parents : 256;
+        // a group of w threads ends at parents / w nodes: stop at the top layer asked for
+        uint64_t w = parents < 256 ? parents : 256;
+        if (parents / w < top) w = parents / top;
         k_merkle_reduce<H><<<(unsigned)(parents / w), (unsigned)w, 0, s>>>(tree, n_leaves, level);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -255,17 +257,19 @@ static hipError_t merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t l
 }
 
 hipError_t launch_merkle_tree(const uint8_t* data, uint64_t n_leaves, uint64_t leaf_bytes,
-                              uint32_t kind, uint8_t* tree, hipStream_t s) {
-    if (n_leaves == 0 || (n_leaves & (n_leaves - 1)) || leaf_bytes % 16 || leaf_bytes == 0)
+                              uint32_t kind, uint8_t* tree, hipStream_t s, uint64_t top_nodes) {
+    if (n_leaves == 0 || (n_leaves & (n_leaves - 1)) || leaf_bytes % 16 || leaf_bytes == 0 ||
+        top_nodes == 0 || (top_nodes & (top_nodes - 1)) || top_nodes > n_leaves)
         return hipErrorInvalidValue;
-    if (kind == kMerkleSha256) return merkle_tree<Sha256Node>(data, n_leaves, leaf_bytes, tree, s);
+    const uint64_t top = top_nodes;
+    if (kind == kMerkleSha256) return merkle_tree<Sha256Node>(data, n_leaves, leaf_bytes, tree, top, s);
     if (kind != kMerkleBlake3) return hipErrorInvalidValue;
     if (leaf_bytes <= kB3ChunkLen)
-        return merkle_tree<Blake3Node<false>>(data, n_leaves, leaf_bytes, tree, s);
+        return merkle_tree<Blake3Node<false>>(data, n_leaves, leaf_bytes, tree, top, s);
     // the per-thread stack holds one chaining value per level of the chunk tree
     if ((leaf_bytes + kB3ChunkLen - 1) / kB3ChunkLen > (1ull << kMerkleB3MaxDepth))
         return hipErrorInvalidValue;
-    return merkle_tree<Blake3Node<true>>(data, n_leaves, leaf_bytes, tree, s);
+    return merkle_tree<Blake3Node<true>>(data, n_leaves, leaf_bytes, tree, top, s);
 }
 
 }  // namespace flock

@@ -1891,9 +1891,13 @@ bool ligero_commit(Ctx& X, const flock_f128* d_poly, int log_cols, int log_lanes
     w->tree = X.buf<uint8_t>(slot + 1, 32 * (2 * w->block_len - 1));
     if (X.err) return false;
     cap_out->assign(32ull << cap, 0);
-    if (int rc = flock_hip_commit(X.c, d_poly, w->lanes << log_cols, &p, w->cw, w->tree, (uint32_t)cap,
-                                  cap_out->data()))
-        return X.fail(rc, flock_hip_last_error());
+    // nothing reads a level tree above its cap (the openings' paths end there): the tree stops at
+    // the cap layer
+    flock::ctx_set_tree_to_cap(X.c, true);
+    const int rc = flock_hip_commit(X.c, d_poly, w->lanes << log_cols, &p, w->cw, w->tree, (uint32_t)cap,
+                                    cap_out->data());
+    flock::ctx_set_tree_to_cap(X.c, false);
+    if (rc) return X.fail(rc, flock_hip_last_error());
     return true;
 }
 
