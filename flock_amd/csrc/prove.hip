@@ -598,6 +598,7 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
     // running it frees the idle workers' buffers and runs alone.
     std::vector<uint64_t> reserved(workers);
     for (uint32_t w = 0; w < workers; w++) reserved[w] = held_bytes(provers[w]);
+    std::vector<char> busy(workers, 0);
     std::mutex mu;
     std::condition_variable cv;
     size_t next = 0;
@@ -616,10 +617,23 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
                 cv.wait_for(lk, std::chrono::duration<double>(due - t));
                 continue;
             }
+            uint64_t mine = held_bytes(pr);
+            // a due request goes to an idle worker whose buffers already hold it when there is
+            // one: growing buffers (hipMalloc, and hipFree's device-wide wait) can stall a large
+            // request for hundreds of milliseconds
+            if (mine < fp[i]) {
+                bool other_fits = false;
+                for (uint32_t v = 0; v < workers && !other_fits; v++)
+                    other_fits = v != w && !busy[v] && held_bytes(provers[v]) >= fp[i];
+                if (other_fits) {
+                    cv.notify_all();
+                    cv.wait(lk);
+                    continue;
+                }
+            }
             uint64_t others = 0;
             for (uint32_t v = 0; v < workers; v++)
                 if (v != w) others += reserved[v];
-            uint64_t mine = held_bytes(pr);
             if (others + std::max(mine, fp[i]) > hbm_budget) {
                 if (running == 0) {
                     for (uint32_t v = 0; v < workers; v++)
@@ -639,6 +653,8 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
             next++;
             reserved[w] = std::max(mine, fp[i]);
             running++;
+            busy[w] = 1;
+            cv.notify_all();  // the idle workers look at the next request
             lk.unlock();
             const double start = now_s();
             const int rc = proofs ? flock_hip_prove_fast(pr, &reqs[i], &res[i], &proofs[i])
@@ -650,6 +666,7 @@ int flock_hip_serve(flock_hip_prover* const* provers, uint32_t workers,
             lk.lock();
             reserved[w] = held_bytes(pr);
             running--;
+            busy[w] = 0;
             if (rc && !first_err) first_err = rc;
             cv.notify_all();
         }
