@@ -21,6 +21,27 @@
 
 namespace flock {
 
+// The second block of a 64-byte message is the fixed padding (0x80000000, zeros, bit length
+// 512). Its message schedule is written with plain shifts and fully unrolled, so the compiler
+// folds all 64 words to constants and only the rounds remain.
+__device__ __forceinline__ uint32_t rotr_plain(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+__device__ __forceinline__ void sha256_compress_pad64(uint32_t h[8]) {
+    uint32_t w[64];
+#pragma unroll
+    for (int i = 0; i < 16; i++) w[i] = i == 0 ? 0x80000000u : (i == 15 ? 512u : 0u);
+#pragma unroll
+    for (int i = 16; i < 64; i++) {
+        const uint32_t w15 = w[i - 15], w2 = w[i - 2];
+        const uint32_t s0 = rotr_plain(w15, 7) ^ rotr_plain(w15, 18) ^ (w15 >> 3);
+        const uint32_t s1 = rotr_plain(w2, 17) ^ rotr_plain(w2, 19) ^ (w2 >> 10);
+        w[i] = w[i - 16] + s0 + w[i - 7] + s1;
+    }
+    uint32_t a = h[0], b = h[1], c = h[2], d = h[3], e = h[4], f = h[5], g = h[6], hh = h[7];
+#pragma unroll
+    for (int i = 0; i < 64; i++) sha256_round(a, b, c, d, e, f, g, hh, kSha256K[i], w[i]);
+    h[0] += a; h[1] += b; h[2] += c; h[3] += d; h[4] += e; h[5] += f; h[6] += g; h[7] += hh;
+}
+
 // SHA-256 of a 64-byte message given as 16 big-endian words (a parent node).
 __device__ __forceinline__ void sha256_64(const uint32_t m[16], uint32_t out[8]) {
     sha256_iv(out);
@@ -28,12 +49,7 @@ __device__ __forceinline__ void sha256_64(const uint32_t m[16], uint32_t out[8])
 #pragma unroll
     for (int i = 0; i < 16; i++) w[i] = m[i];
     sha256_compress(out, w);
-    uint32_t p[16];
-    p[0] = 0x80000000u;
-#pragma unroll
-    for (int i = 1; i < 15; i++) p[i] = 0u;
-    p[15] = 512u;
-    sha256_compress(out, p);
+    sha256_compress_pad64(out);
 }
 
 // SHA-256 nodes: digests are the big-endian state words.
